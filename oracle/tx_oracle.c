@@ -1,0 +1,1413 @@
+/*
+ * ORACLE — test infrastructure only.
+ *
+ * A plain-C, serial restatement of the reference's tsvdx4 timeSVD++ path
+ * (TxModel in tx4 mode and the rayon variant Tsvdx4pModel), used by the
+ * test-suite, smoke() and bench.py's cpu_baseline leg as the CHECKER of the
+ * HIP implementation.  Nothing in the product (netflix_prize_amd/) links,
+ * loads or calls this file.
+ *
+ * Every function cites the reference file:line it follows.  Floating-point
+ * operations are written in the reference's exact association order and the
+ * file is compiled with -ffp-contract=off (Rust never contracts a*b+c), so
+ * that the serial trainer is bit-identical to the reference wherever the
+ * RNG stack below is too.
+ *
+ * Pins (see tests/test_oracle_kat.py):
+ *   - tsvdx4 tiny-fixture probe RMSE == 1.020340388903108 exactly
+ *     (tests/regressors_test.rs:75-93), which pins the RNG stack, init order,
+ *     user shuffle, non-low-memory day terms, SGD op order and calc_rmse;
+ *   - tsvdx4p tiny-fixture probe RMSE == 1.02037806773939 exactly
+ *     (tests/regressors_test.rs:157-175), which pins the user/item passes and
+ *     apply_nsvd_deltas;
+ *   - ChaCha20 all-zero key keystream (published test vector) pins the block
+ *     function; the ziggurat table is regenerated from the published
+ *     construction and checked against the published table prefix.
+ *   - low_memory=true (not covered by any reference test) is pinned by
+ *     invariants and special cases in tests/test_oracle_invariants.py.
+ *
+ * Third-party algorithms restated here (absent from the reference tree, pinned
+ * versions from Cargo.lock): rand 0.9.2 (StdRng, SliceRandom::shuffle via
+ * IncreasingUniform, UniformInt::sample_single_inclusive, StandardUniform /
+ * Open01 for f64), rand_chacha 0.9.0 (ChaCha12 block RNG, 4-block buffer),
+ * rand_core 0.9.3 (SeedableRng::seed_from_u64 via PCG32, BlockRng::next_u64),
+ * rand_distr 0.5.1 (Normal<f32> via f64 StandardNormal ziggurat).
+ */
+#include <math.h>
+#include <pthread.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define OR_API __attribute__((visibility("default")))
+
+/* ------------------------------------------------------------------------ */
+/* RNG: rand_core 0.9.3 seed_from_u64 + rand_chacha 0.9.0 ChaCha12          */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    uint32_t key[8];
+    uint64_t block;       /* next block counter */
+    uint32_t buf[64];     /* 4 blocks */
+    int idx;              /* next word in buf; 64 = empty */
+    int rounds;
+} or_rng;
+
+#define ROTL32(v, n) (((v) << (n)) | ((v) >> (32 - (n))))
+#define QR(a, b, c, d)                                   \
+    do {                                                 \
+        a += b; d ^= a; d = ROTL32(d, 16);               \
+        c += d; b ^= c; b = ROTL32(b, 12);               \
+        a += b; d ^= a; d = ROTL32(d, 8);                \
+        c += d; b ^= c; b = ROTL32(b, 7);                \
+    } while (0)
+
+/* One ChaCha block with a 64-bit block counter in words 12..13 and a 64-bit
+ * stream id (0) in words 14..15 (the djb layout rand_chacha uses). */
+static void chacha_block(const uint32_t key[8], uint64_t ctr, int rounds, uint32_t out[16]) {
+    uint32_t in[16] = {0x61707865u, 0x3320646eu, 0x79622d32u, 0x6b206574u,
+                       key[0], key[1], key[2], key[3], key[4], key[5], key[6], key[7],
+                       (uint32_t)ctr, (uint32_t)(ctr >> 32), 0u, 0u};
+    uint32_t x[16];
+    memcpy(x, in, sizeof x);
+    for (int r = 0; r < rounds; r += 2) {
+        QR(x[0], x[4], x[8], x[12]);
+        QR(x[1], x[5], x[9], x[13]);
+        QR(x[2], x[6], x[10], x[14]);
+        QR(x[3], x[7], x[11], x[15]);
+        QR(x[0], x[5], x[10], x[15]);
+        QR(x[1], x[6], x[11], x[12]);
+        QR(x[2], x[7], x[8], x[13]);
+        QR(x[3], x[4], x[9], x[14]);
+    }
+    for (int i = 0; i < 16; i++) out[i] = x[i] + in[i];
+}
+
+/* rand_chacha: refill generates 4 consecutive blocks into a 64-word buffer. */
+static void rng_refill(or_rng* r) {
+    for (int b = 0; b < 4; b++) chacha_block(r->key, r->block + (uint64_t)b, r->rounds, r->buf + 16 * b);
+    r->block += 4;
+    r->idx = 0;
+}
+
+static uint32_t rng_u32(or_rng* r) {
+    if (r->idx >= 64) rng_refill(r);
+    return r->buf[r->idx++];
+}
+
+/* rand_core BlockRng::next_u64: low word first, then high word; across a
+ * buffer boundary the low word is the last of the old buffer. */
+static uint64_t rng_u64(or_rng* r) {
+    uint64_t lo = rng_u32(r);
+    uint64_t hi = rng_u32(r);
+    return lo | (hi << 32);
+}
+
+/* rand_core SeedableRng::seed_from_u64: PCG32 output fills the 32-byte seed. */
+static void rng_seed_u64(or_rng* r, uint64_t state, int rounds) {
+    const uint64_t MUL = 6364136223846793005ull, INC = 11634580027462260723ull;
+    for (int w = 0; w < 8; w++) {
+        state = state * MUL + INC;
+        uint32_t xorshifted = (uint32_t)(((state >> 18) ^ state) >> 27);
+        uint32_t rot = (uint32_t)(state >> 59);
+        r->key[w] = (xorshifted >> rot) | (xorshifted << ((32 - rot) & 31));
+    }
+    r->block = 0;
+    r->idx = 64;
+    r->rounds = rounds;
+}
+
+/* rand 0.9 UniformInt<u32>::sample_single_inclusive(0, bound-1). */
+static uint32_t rng_below_u32(or_rng* r, uint32_t bound) {
+    uint32_t range = bound; /* (bound-1) - 0 + 1 */
+    uint64_t m = (uint64_t)rng_u32(r) * (uint64_t)range;
+    uint32_t result = (uint32_t)(m >> 32), lo = (uint32_t)m;
+    if (lo > (uint32_t)(0u - range)) {
+        uint64_t m2 = (uint64_t)rng_u32(r) * (uint64_t)range;
+        uint32_t new_hi = (uint32_t)(m2 >> 32);
+        if ((uint64_t)lo + (uint64_t)new_hi > 0xffffffffull) result += 1;
+    }
+    return result;
+}
+
+/* rand 0.9 seq::increasing_uniform::calculate_bound_u32. */
+static void calc_bound_u32(uint32_t m, uint32_t* bound, int* count) {
+    uint64_t product = m;
+    uint32_t current = m + 1;
+    for (;;) {
+        uint64_t p = product * (uint64_t)current;
+        if (p <= 0xffffffffull) {
+            product = p;
+            current += 1;
+        } else {
+            *bound = (uint32_t)product;
+            *count = (int)(current - m);
+            return;
+        }
+    }
+}
+
+/* rand 0.9 SliceRandom::shuffle = partial_shuffle(len) with IncreasingUniform
+ * (inside-out Durstenfeld, several indices per random draw). */
+static void rng_shuffle_i64(or_rng* r, int64_t* v, int64_t len) {
+    if (len <= 1) return;
+    uint32_t n = 0, chunk = 0;
+    int chunk_remaining = 1; /* IncreasingUniform::new(rng, 0) */
+    for (int64_t i = 0; i < len; i++) {
+        uint32_t next_n = n + 1;
+        int next_rem;
+        if (chunk_remaining > 0) {
+            next_rem = chunk_remaining - 1;
+        } else {
+            uint32_t bound; int count;
+            calc_bound_u32(next_n, &bound, &count);
+            chunk = rng_below_u32(r, bound);
+            next_rem = count - 1;
+        }
+        uint32_t index;
+        if (next_rem == 0) {
+            index = chunk;
+        } else {
+            index = chunk % next_n;
+            chunk /= next_n;
+        }
+        chunk_remaining = next_rem;
+        n = next_n;
+        int64_t tmp = v[i]; v[i] = v[index]; v[index] = tmp;
+    }
+}
+
+/* rand_distr 0.5.1 StandardNormal (f64) ziggurat, ZIGNOR tables regenerated
+ * from the published construction (ziggurat_tables.py: R, V, 256 layers,
+ * literals printed with 18 decimals). */
+static double ZX[257], ZF[257];
+static const double ZIG_NORM_R = 3.654152885361008796;
+static int zig_ready = 0;
+
+static double round18(double v) {
+    char buf[64];
+    snprintf(buf, sizeof buf, "%.18f", v);
+    return strtod(buf, NULL);
+}
+
+static void zig_init(void) {
+    if (zig_ready) return;
+    const double R = 3.6541528853610088, V = 0.00492867323399;
+    double x[257];
+    x[0] = V / exp(-R * R / 2.0);
+    x[1] = R;
+    for (int i = 2; i < 256; i++) {
+        double last = x[i - 1];
+        x[i] = sqrt(-2.0 * log(V / last + exp(-last * last / 2.0)));
+    }
+    x[256] = 0.0;
+    for (int i = 0; i < 257; i++) {
+        ZF[i] = round18(exp(-x[i] * x[i] / 2.0));
+        ZX[i] = round18(x[i]);
+    }
+    zig_ready = 1;
+}
+
+static double f64_from_bits(uint64_t b) { double d; memcpy(&d, &b, 8); return d; }
+
+static double rng_open01(or_rng* r) {
+    uint64_t frac = rng_u64(r) >> 12;
+    return f64_from_bits((1023ull << 52) | frac) - (1.0 - 2.220446049250313e-16 / 2.0);
+}
+
+static double rng_uniform01(or_rng* r) {
+    return (double)(rng_u64(r) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+static double rng_std_normal(or_rng* r) {
+    for (;;) {
+        uint64_t bits = rng_u64(r);
+        int i = (int)(bits & 0xff);
+        double u = f64_from_bits((1024ull << 52) | (bits >> 12)) - 3.0;
+        double x = u * ZX[i];
+        if (fabs(x) < ZX[i + 1]) return x;
+        if (i == 0) {
+            double xx = 1.0, yy = 0.0;
+            while (-2.0 * yy < xx * xx) {
+                double x_ = rng_open01(r), y_ = rng_open01(r);
+                xx = log(x_) / ZIG_NORM_R;
+                yy = log(y_);
+            }
+            return u < 0.0 ? xx - ZIG_NORM_R : ZIG_NORM_R - xx;
+        }
+        if (ZF[i + 1] + (ZF[i] - ZF[i + 1]) * rng_uniform01(r) < exp(-x * x / 2.0)) return x;
+    }
+}
+
+/* rand_array2 (src/lib.rs:64-67): Normal<f32>(0, sigma), row-major fill.
+ * Normal::sample = mean + std_dev * (StandardNormal f64 as f32). */
+static void rand_fill(or_rng* r, float* out, int64_t n, float sigma) {
+    for (int64_t k = 0; k < n; k++) {
+        float z = (float)rng_std_normal(r);
+        out[k] = 0.0f + sigma * z;
+    }
+}
+
+/* get_users (src/lib.rs:486-494). */
+static void get_users(int64_t n, int shuffle, uint64_t seed, int64_t epoch, int64_t* out) {
+    for (int64_t k = 0; k < n; k++) out[k] = k;
+    if (shuffle) {
+        or_rng r;
+        uint64_t seed2 = seed ^ ((uint64_t)epoch * 0x9E3779B97F4A7C15ull);
+        rng_seed_u64(&r, seed2, 12);
+        rng_shuffle_i64(&r, out, n);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Data and config                                                          */
+/* ------------------------------------------------------------------------ */
+
+/* Dataset (src/lib.rs:227-243); residuals/raw_ratings may be NULL for the
+ * masked held-out set (MaskedDataset, src/lib.rs:421-435). */
+typedef struct {
+    int64_t n_ratings;
+    int32_t n_users, n_items;
+    const int32_t* user_idxs;
+    const int32_t* item_idxs;
+    const int16_t* dates;
+    const float* residuals;
+    const int8_t* raw_ratings;
+    const int8_t* is_test;
+    const int32_t* user_cnts;
+} or_dataset;
+
+/* TxConfig (src/tx.rs:128-204) with OrdinalHeadConfig (src/lib.rs:502-507)
+ * and the Tsvdx4pConfig extras (src/tsvdx4p.rs:128-187). */
+typedef struct {
+    uint64_t seed;
+    int32_t n_feat, n_epochs, n_time_bins, n_freq_bins;
+    int32_t reset_u_epoch, max_neighbors;
+    int32_t shuffle_users, sum_err_bug, low_memory, full_su, save_ifeat, has_ordinal;
+    float beta;
+    float lr_u, lr_ud, lr_u2, lr_ub, lr_ubd, lr_i, lr_ib, lr_y, lr_yb, lr_yd;
+    float lr_tu, lr_ti, lr_ta, lr_ibf, lr_iqf, lr_cu;
+    float reg_iqf, reg_cu, reg_u, reg_u2, reg_ud, reg_i, reg_y, reg_yd;
+    float sigma_iqf, sigma_u, sigma_i, sigma_y, sigma_yd;
+    float lr_w, lr_c, reg_w, reg_c, lr_w_day, lr_c_day, reg_w_day, reg_c_day;
+    float w_bias, w_factor, w_nbr;
+    float lambda1, lambda2;
+    float th_init[4];
+    float th_gap, lr_t, reg_t;
+    float nsvd_norm_exp;
+    int32_t n_threads;
+    int32_t seq_epochs[5];
+} or_config;
+
+/* SparseUD (src/tx.rs:21-84). */
+typedef struct {
+    int64_t* starts;   /* n_users + 1 */
+    int16_t* dates;
+    uint32_t* day_cnts;
+    int64_t n_total;
+} or_sud;
+
+typedef struct {
+    or_config cfg;
+    int32_t n_users, n_items, f, F, B;
+    float gbias;
+    float *ubias, *ibias, *ufeat, *ufeat2, *ifeat, *yfeat, *ycache, *yfeat_day;
+    int32_t day_range;
+    float *tu_mean, *alpha_u, *but_bin, *bit_bin, *ybias, *ycache_bias;
+    or_sud ud;
+    float *ubias_day, *ufeat_day, *ycache_day, *ibias_freq, *ifeat_freq, *cu, *cut;
+    float thresholds[4];
+    int64_t* user_offsets;
+    /* probe items per user / per (user, day), CSR */
+    int64_t *pbu_start, *pbd_start;
+    uint32_t *pbu_items, *pbd_items;
+    /* tsvdx4p ByItemIndex (src/tsvdx4p.rs:29-57) */
+    int64_t* bi_starts;
+    uint32_t *bi_user, *bi_rating;
+    char err[256];
+} or_model;
+
+static int cmp_i16(const void* a, const void* b) {
+    int16_t x = *(const int16_t*)a, y = *(const int16_t*)b;
+    return (x > y) - (x < y);
+}
+
+static void sud_build(or_sud* s, const or_dataset* d1, const or_dataset* d2, int32_t n_users) {
+    int64_t* cnt = calloc((size_t)n_users + 1, sizeof(int64_t));
+    for (int64_t t = 0; t < d1->n_ratings; t++) cnt[d1->user_idxs[t] + 1]++;
+    for (int64_t t = 0; t < d2->n_ratings; t++) cnt[d2->user_idxs[t] + 1]++;
+    for (int32_t u = 0; u < n_users; u++) cnt[u + 1] += cnt[u];
+    int64_t tot = cnt[n_users];
+    int16_t* all = malloc(sizeof(int16_t) * (size_t)(tot ? tot : 1));
+    int64_t* pos = malloc(sizeof(int64_t) * ((size_t)n_users + 1));
+    memcpy(pos, cnt, sizeof(int64_t) * ((size_t)n_users + 1));
+    for (int64_t t = 0; t < d1->n_ratings; t++) all[pos[d1->user_idxs[t]]++] = d1->dates[t];
+    for (int64_t t = 0; t < d2->n_ratings; t++) all[pos[d2->user_idxs[t]]++] = d2->dates[t];
+    s->starts = malloc(sizeof(int64_t) * ((size_t)n_users + 1));
+    s->dates = malloc(sizeof(int16_t) * (size_t)(tot ? tot : 1));
+    s->day_cnts = malloc(sizeof(uint32_t) * (size_t)(tot ? tot : 1));
+    int64_t n = 0;
+    for (int32_t u = 0; u < n_users; u++) {
+        s->starts[u] = n;
+        int64_t a = cnt[u], b = cnt[u + 1];
+        if (b > a) {
+            qsort(all + a, (size_t)(b - a), sizeof(int16_t), cmp_i16);
+            int16_t prev = all[a];
+            uint32_t c = 1;
+            for (int64_t k = a + 1; k < b; k++) {
+                if (all[k] == prev) c++;
+                else { s->dates[n] = prev; s->day_cnts[n] = c; n++; prev = all[k]; c = 1; }
+            }
+            s->dates[n] = prev; s->day_cnts[n] = c; n++;
+        }
+    }
+    s->starts[n_users] = n;
+    s->n_total = n;
+    free(all); free(pos); free(cnt);
+}
+
+/* SparseUD::index: binary search, -1 when absent. */
+static int64_t sud_index(const or_sud* s, int32_t u, int16_t day) {
+    int64_t lo = s->starts[u], hi = s->starts[u + 1];
+    while (lo < hi) {
+        int64_t mid = lo + (hi - lo) / 2;
+        if (s->dates[mid] < day) lo = mid + 1;
+        else if (s->dates[mid] > day) hi = mid;
+        else return mid;
+    }
+    return -1;
+}
+
+/* freq_bin (src/tx.rs:87-90). */
+static int freq_bin(uint32_t day_cnt, int n_freq_bins) {
+    if (day_cnt <= 1) return 0;
+    float v = floorf(log2f((float)day_cnt));
+    float cap = (float)(n_freq_bins - 1);
+    return (int)(v < cap ? v : cap);
+}
+
+/* TxModel::time_bin (src/tx.rs:285-289). */
+static int time_bin(const or_model* m, int32_t day) {
+    int64_t num = (int64_t)day * (int64_t)m->B;
+    int64_t b = num / (int64_t)m->day_range;
+    return (int)(b < m->B - 1 ? b : m->B - 1);
+}
+
+/* TxModel::dev (src/tx.rs:292-300). */
+static float dev_of(float tu_mean, int32_t day, float beta) {
+    float dt = (float)day - tu_mean;
+    if (dt == 0.0f) return 0.0f;
+    float s = dt > 0.0f ? 1.0f : -1.0f;
+    return s * powf(fabsf(dt), beta);
+}
+
+static float sigmoidf_(float x) { return 1.0f / (1.0f + expf(-x)); }
+
+/* OrdinalHead::predict_probs (src/lib.rs:536-559). */
+static void ord_probs(const float th[4], float score, float p[5]) {
+    float f[6];
+    f[0] = 0.0f; f[5] = 1.0f;
+    for (int k = 1; k < 5; k++) f[k] = sigmoidf_(th[k - 1] - score);
+    for (int k = 0; k < 5; k++) p[k] = f[k + 1] - f[k];
+}
+
+/* OrdinalHead::grad (src/lib.rs:565-599). */
+static float ord_grad(const float th[4], float score, int y, float g_t[4]) {
+    float f[6], a[6];
+    f[0] = 0.0f; f[5] = 1.0f;
+    for (int k = 1; k < 5; k++) f[k] = sigmoidf_(th[k - 1] - score);
+    for (int k = 0; k < 6; k++) a[k] = f[k] * (1.0f - f[k]);
+    float p_y = f[y] - f[y - 1];
+    if (!(p_y >= 1e-6f)) p_y = 1e-6f;
+    float g_ds = (a[y] - a[y - 1]) / p_y;
+    for (int k = 0; k < 4; k++) g_t[k] = 0.0f;
+    if (y >= 2) g_t[y - 2] += a[y - 1] / p_y;
+    if (y <= 4) g_t[y - 1] -= a[y] / p_y;
+    return g_ds;
+}
+
+static int cmp_f32(const void* a, const void* b) {
+    float x = *(const float*)a, y = *(const float*)b;
+    return (x > y) - (x < y);
+}
+
+/* ------------------------------------------------------------------------ */
+/* TxModel::new (src/tx.rs:460-649)                                         */
+/* ------------------------------------------------------------------------ */
+
+static float* zalloc(int64_t n) { return calloc((size_t)(n > 0 ? n : 1), sizeof(float)); }
+
+OR_API or_model* or_new(const or_config* cfg, const or_dataset* tr, const or_dataset* pr) {
+    zig_init();
+    or_model* m = calloc(1, sizeof(or_model));
+    m->cfg = *cfg;
+    if (cfg->lr_w > 0.0f || cfg->lr_c > 0.0f || cfg->lr_w_day > 0.0f || cfg->lr_c_day > 0.0f) {
+        snprintf(m->err, sizeof m->err, "neighbourhood terms are not part of tx4 mode");
+        return m;
+    }
+    const int32_t U = tr->n_users, I = tr->n_items, f = cfg->n_feat, F = cfg->n_freq_bins;
+    m->n_users = U; m->n_items = I; m->f = f; m->F = F; m->B = cfg->n_time_bins;
+    or_rng rng;
+    rng_seed_u64(&rng, cfg->seed, 12);
+
+    int32_t day_range = 0;
+    m->tu_mean = zalloc(U);
+    for (int64_t t = 0; t < tr->n_ratings; t++) {
+        m->tu_mean[tr->user_idxs[t]] += (float)tr->dates[t];
+        int32_t d1 = (int32_t)tr->dates[t] + 1;
+        if (d1 > day_range) day_range = d1;
+    }
+    for (int32_t u = 0; u < U; u++) {
+        int32_t c = tr->user_cnts[u];
+        if (c > 0) m->tu_mean[u] /= (float)c;
+    }
+    m->day_range = day_range;
+    sud_build(&m->ud, tr, pr, U);
+    const int64_t n_ud = m->ud.n_total;
+
+    /* calc_user_offsets (src/lib.rs:469-483) */
+    for (int64_t t = 1; t < tr->n_ratings; t++) {
+        if (tr->user_idxs[t] < tr->user_idxs[t - 1]) {
+            snprintf(m->err, sizeof m->err, "dataset is not user-sorted at index %lld", (long long)t);
+            return m;
+        }
+    }
+    m->user_offsets = malloc(sizeof(int64_t) * ((size_t)U + 1));
+    m->user_offsets[0] = 0;
+    for (int32_t u = 0; u < U; u++) m->user_offsets[u + 1] = m->user_offsets[u] + tr->user_cnts[u];
+
+    /* calc_gbias (src/lib.rs:463-466) */
+    double rsum = 0.0;
+    for (int64_t t = 0; t < tr->n_ratings; t++) rsum += (double)tr->residuals[t];
+    m->gbias = (float)(rsum / (double)tr->n_ratings);
+
+    /* probe_items_by_user / probe_items_by_ud (src/tx.rs:587-598), CSR */
+    m->pbu_start = calloc((size_t)U + 1, sizeof(int64_t));
+    m->pbd_start = calloc((size_t)n_ud + 1, sizeof(int64_t));
+    int64_t np = cfg->full_su ? pr->n_ratings : 0;
+    m->pbu_items = malloc(sizeof(uint32_t) * (size_t)(np ? np : 1));
+    m->pbd_items = malloc(sizeof(uint32_t) * (size_t)(np ? np : 1));
+    if (cfg->full_su) {
+        int64_t* udi = malloc(sizeof(int64_t) * (size_t)(np ? np : 1));
+        for (int64_t t = 0; t < np; t++) {
+            m->pbu_start[pr->user_idxs[t] + 1]++;
+            udi[t] = sud_index(&m->ud, pr->user_idxs[t], pr->dates[t]);
+            if (udi[t] >= 0) m->pbd_start[udi[t] + 1]++;
+        }
+        for (int32_t u = 0; u < U; u++) m->pbu_start[u + 1] += m->pbu_start[u];
+        for (int64_t k = 0; k < n_ud; k++) m->pbd_start[k + 1] += m->pbd_start[k];
+        int64_t* pu = malloc(sizeof(int64_t) * ((size_t)U + 1));
+        int64_t* pd = malloc(sizeof(int64_t) * ((size_t)n_ud + 1));
+        memcpy(pu, m->pbu_start, sizeof(int64_t) * ((size_t)U + 1));
+        memcpy(pd, m->pbd_start, sizeof(int64_t) * ((size_t)n_ud + 1));
+        for (int64_t t = 0; t < np; t++) {
+            m->pbu_items[pu[pr->user_idxs[t]]++] = (uint32_t)pr->item_idxs[t];
+            if (udi[t] >= 0) m->pbd_items[pd[udi[t]]++] = (uint32_t)pr->item_idxs[t];
+        }
+        free(pu); free(pd); free(udi);
+    }
+
+    /* parameter tables in struct-literal order (RNG draws in this order) */
+    m->ubias = zalloc(U);
+    m->ibias = zalloc(I);
+    m->ufeat = zalloc((int64_t)U * f);
+    rand_fill(&rng, m->ufeat, (int64_t)U * f, cfg->sigma_u);
+    m->ufeat2 = zalloc((int64_t)U * f);
+    m->ifeat = zalloc((int64_t)I * f);
+    rand_fill(&rng, m->ifeat, (int64_t)I * f, cfg->sigma_i);
+    m->yfeat = zalloc((int64_t)I * f);
+    rand_fill(&rng, m->yfeat, (int64_t)I * f, cfg->sigma_y);
+    m->ycache = zalloc((int64_t)U * f);
+    if (!cfg->low_memory) {
+        m->yfeat_day = zalloc((int64_t)I * f);
+        rand_fill(&rng, m->yfeat_day, (int64_t)I * f, cfg->sigma_yd);
+    } else {
+        m->yfeat_day = zalloc(0);
+    }
+    m->alpha_u = zalloc(U);
+    m->but_bin = zalloc((int64_t)U * m->B);
+    m->bit_bin = zalloc((int64_t)I * m->B);
+    m->ybias = zalloc(I);
+    m->ycache_bias = zalloc(U);
+    m->ubias_day = zalloc(n_ud);
+    m->ufeat_day = zalloc(cfg->low_memory ? 0 : n_ud * f);
+    m->ycache_day = zalloc(cfg->low_memory ? 0 : n_ud * f);
+    m->ibias_freq = zalloc((int64_t)I * F);
+    m->ifeat_freq = zalloc((int64_t)I * F * f);
+    rand_fill(&rng, m->ifeat_freq, (int64_t)I * F * f, cfg->sigma_iqf);
+    m->cu = zalloc(U);
+    for (int32_t u = 0; u < U; u++) m->cu[u] = 1.0f;
+    m->cut = zalloc(n_ud);
+    for (int k = 0; k < 4; k++) m->thresholds[k] = cfg->th_init[k];
+    return m;
+}
+
+OR_API const char* or_error(const or_model* m) { return m->err; }
+
+OR_API void or_free(or_model* m) {
+    if (!m) return;
+    void* ps[] = {m->ubias, m->ibias, m->ufeat, m->ufeat2, m->ifeat, m->yfeat, m->ycache,
+                  m->yfeat_day, m->tu_mean, m->alpha_u, m->but_bin, m->bit_bin, m->ybias,
+                  m->ycache_bias, m->ud.starts, m->ud.dates, m->ud.day_cnts, m->ubias_day,
+                  m->ufeat_day, m->ycache_day, m->ibias_freq, m->ifeat_freq, m->cu, m->cut,
+                  m->user_offsets, m->pbu_start, m->pbd_start, m->pbu_items, m->pbd_items,
+                  m->bi_starts, m->bi_user, m->bi_rating};
+    for (size_t k = 0; k < sizeof ps / sizeof ps[0]; k++) free(ps[k]);
+    free(m);
+}
+
+/* ------------------------------------------------------------------------ */
+/* rebuild_ycache / rebuild_ycache_day (src/tx.rs:391-454)                  */
+/* ------------------------------------------------------------------------ */
+
+static void rebuild_ycache(or_model* m, const or_dataset* tr, const or_dataset* pr) {
+    const int f = m->f;
+    memset(m->ycache, 0, sizeof(float) * (size_t)m->n_users * f);
+    memset(m->ycache_bias, 0, sizeof(float) * (size_t)m->n_users);
+    for (int64_t t = 0; t < tr->n_ratings; t++) {
+        int32_t u = tr->user_idxs[t], i = tr->item_idxs[t];
+        float* su = m->ycache + (int64_t)u * f;
+        const float* y = m->yfeat + (int64_t)i * f;
+        for (int k = 0; k < f; k++) su[k] += y[k];
+        m->ycache_bias[u] += m->ybias[i];
+    }
+    for (int64_t t = 0; t < pr->n_ratings; t++) {
+        int32_t u = pr->user_idxs[t], i = pr->item_idxs[t];
+        float* su = m->ycache + (int64_t)u * f;
+        const float* y = m->yfeat + (int64_t)i * f;
+        for (int k = 0; k < f; k++) su[k] += y[k];
+        m->ycache_bias[u] += m->ybias[i];
+    }
+    for (int32_t u = 0; u < m->n_users; u++) {
+        int32_t cnt = tr->user_cnts[u] + pr->user_cnts[u];
+        if (cnt > 0) {
+            float* su = m->ycache + (int64_t)u * f;
+            float nrm = sqrtf((float)cnt);
+            for (int k = 0; k < f; k++) su[k] /= nrm;
+            m->ycache_bias[u] /= sqrtf((float)cnt);
+        }
+    }
+}
+
+static void rebuild_ycache_day(or_model* m, const or_dataset* tr, const or_dataset* pr) {
+    const int f = m->f;
+    const int64_t n_ud = m->ud.n_total;
+    memset(m->ycache_day, 0, sizeof(float) * (size_t)(n_ud * f));
+    float* cnts = zalloc(n_ud);
+    const or_dataset* sets[2] = {pr, tr};
+    for (int s = 0; s < 2; s++) {
+        const or_dataset* d = sets[s];
+        for (int64_t t = 0; t < d->n_ratings; t++) {
+            int64_t idx = sud_index(&m->ud, d->user_idxs[t], d->dates[t]);
+            if (idx >= 0) {
+                cnts[idx] += 1.0f;
+                float* row = m->ycache_day + idx * f;
+                const float* y = m->yfeat_day + (int64_t)d->item_idxs[t] * f;
+                for (int k = 0; k < f; k++) row[k] += y[k];
+            }
+        }
+    }
+    for (int64_t idx = 0; idx < n_ud; idx++) {
+        if (cnts[idx] > 0.0f) {
+            float nrm = sqrtf(cnts[idx]);
+            float* row = m->ycache_day + idx * f;
+            for (int k = 0; k < f; k++) row[k] /= nrm;
+        }
+    }
+    free(cnts);
+}
+
+OR_API void or_rebuild_ycache(or_model* m, const or_dataset* tr, const or_dataset* pr) {
+    rebuild_ycache(m, tr, pr);
+    if (!m->cfg.low_memory) rebuild_ycache_day(m, tr, pr);
+}
+
+/* day_ranges (src/tx.rs:93-105): runs of equal dates within [start, end). */
+static int64_t day_ranges(const int16_t* dates, int64_t start, int64_t end, int64_t* rs, int64_t* re) {
+    if (start == end) return 0;
+    int64_t n = 0, run = start;
+    for (int64_t idx = start + 1; idx < end; idx++) {
+        if (dates[idx] != dates[idx - 1]) { rs[n] = run; re[n] = idx - 1; n++; run = idx; }
+    }
+    rs[n] = run; re[n] = end - 1; n++;
+    return n;
+}
+
+/* ------------------------------------------------------------------------ */
+/* TxModel::fit_epoch (src/tx.rs:734-1115), tx4 mode (no neighbourhood)     */
+/* ------------------------------------------------------------------------ */
+
+static void fit_epoch_users(or_model* m, const or_dataset* tr, const or_dataset* pr, int64_t epoch,
+                            const int64_t* users) {
+    const or_config c = m->cfg;
+    const int f = m->f, F = m->F, B = m->B;
+    const int low_mem = c.low_memory, use_probe = c.full_su;
+    const int has_ordinal = c.has_ordinal;
+    (void)B;
+    if (epoch == c.reset_u_epoch) {
+        memset(m->ufeat, 0, sizeof(float) * (size_t)m->n_users * f);
+        memset(m->ufeat2, 0, sizeof(float) * (size_t)m->n_users * f);
+    }
+    float* su = zalloc(f);
+    float* sum_err_q = zalloc(f);
+    float* sum_err_q_day = zalloc(f);
+    float* su_day = zalloc(f);
+    float* su_freq = zalloc(f);
+    int64_t maxcnt = 1;
+    for (int32_t u = 0; u < m->n_users; u++) {
+        int64_t cc = m->user_offsets[u + 1] - m->user_offsets[u];
+        if (cc > maxcnt) maxcnt = cc;
+    }
+    int64_t* rs = malloc(sizeof(int64_t) * (size_t)maxcnt);
+    int64_t* re = malloc(sizeof(int64_t) * (size_t)maxcnt);
+
+    for (int32_t ui = 0; ui < m->n_users; ui++) {
+        const int32_t u = (int32_t)users[ui];
+        const int64_t start = m->user_offsets[u], end = m->user_offsets[u + 1];
+        const int64_t cnt = end - start;
+        if (cnt == 0) continue;
+        if (!low_mem) day_ranges(tr->dates, start, end, rs, re);
+
+        /* NSVD1 user-level aggregation (src/tx.rs:801-822) */
+        float su_bias = 0.0f;
+        for (int k = 0; k < f; k++) su[k] = 0.0f;
+        for (int64_t t = start; t < end; t++) {
+            int32_t j = tr->item_idxs[t];
+            const float* y = m->yfeat + (int64_t)j * f;
+            for (int k = 0; k < f; k++) su[k] += y[k];
+            su_bias += m->ybias[j];
+        }
+        int64_t total_cnt = cnt;
+        if (use_probe) {
+            for (int64_t q = m->pbu_start[u]; q < m->pbu_start[u + 1]; q++) {
+                uint32_t j = m->pbu_items[q];
+                const float* y = m->yfeat + (int64_t)j * f;
+                for (int k = 0; k < f; k++) su[k] += y[k];
+                su_bias += m->ybias[j];
+            }
+            total_cnt = cnt + (m->pbu_start[u + 1] - m->pbu_start[u]);
+        }
+        const float norm = sqrtf((float)total_cnt);
+        for (int k = 0; k < f; k++) su[k] /= norm;
+        memcpy(m->ycache + (int64_t)u * f, su, sizeof(float) * (size_t)f);
+        su_bias /= norm;
+        m->ycache_bias[u] = su_bias;
+
+        float sum_err = 0.0f;
+        for (int k = 0; k < f; k++) { sum_err_q[k] = 0.0f; sum_err_q_day[k] = 0.0f; su_day[k] = 0.0f; su_freq[k] = 0.0f; }
+        float norm_day = 0.0f;
+        int64_t dr_idx = 0;
+        int cur_freq_bin = 0;
+
+        for (int64_t t = start; t < end; t++) {
+            const int32_t i = tr->item_idxs[t];
+            const float r = tr->residuals[t];
+            const int32_t day = tr->dates[t];
+            const int16_t day16 = tr->dates[t];
+            const int b = time_bin(m, day);
+            const float dev = dev_of(m->tu_mean[u], day, c.beta);
+            int64_t day_start = 0, day_stop = 0;
+            if (!low_mem) { day_start = rs[dr_idx]; day_stop = re[dr_idx]; }
+
+            if (!low_mem && t == day_start) {
+                for (int k = 0; k < f; k++) { sum_err_q_day[k] = 0.0f; su_day[k] = 0.0f; su_freq[k] = 0.0f; }
+                int64_t udd = sud_index(&m->ud, u, day16);
+                cur_freq_bin = udd >= 0 ? freq_bin(m->ud.day_cnts[udd], F) : 0;
+                for (int64_t td = day_start; td <= day_stop; td++) {
+                    const float* y = m->yfeat_day + (int64_t)tr->item_idxs[td] * f;
+                    for (int k = 0; k < f; k++) su_day[k] += y[k];
+                }
+                float day_cnt = (float)(day_stop - day_start + 1);
+                if (use_probe && udd >= 0) {
+                    for (int64_t q = m->pbd_start[udd]; q < m->pbd_start[udd + 1]; q++) {
+                        const float* y = m->yfeat_day + (int64_t)m->pbd_items[q] * f;
+                        for (int k = 0; k < f; k++) su_day[k] += y[k];
+                        day_cnt += 1.0f;
+                    }
+                }
+                norm_day = sqrtf(day_cnt);
+                for (int k = 0; k < f; k++) su_day[k] /= norm_day;
+                for (int k = 0; k < f; k++) su_freq[k] /= norm_day;
+            }
+            const int fb = cur_freq_bin;
+
+            /* forward (src/tx.rs:878-901) */
+            const int64_t ud_idx = sud_index(&m->ud, u, day16);
+            const float bu_day = ud_idx >= 0 ? m->ubias_day[ud_idx] : 0.0f;
+            const float bu_t = m->ubias[u] + m->but_bin[(int64_t)u * m->B + b] + m->alpha_u[u] * dev + bu_day + m->ycache_bias[u];
+            const float bi_t = m->ibias[i] + m->bit_bin[(int64_t)i * m->B + b] + m->ibias_freq[(int64_t)i * F + fb];
+            const float cu_t = m->cu[u] + (ud_idx >= 0 ? m->cut[ud_idx] : 0.0f);
+            const float* pu = m->ufeat + (int64_t)u * f;
+            const float* pu2 = m->ufeat2 + (int64_t)u * f;
+            const float* su_row = m->ycache + (int64_t)u * f;
+            const float* qi = m->ifeat + (int64_t)i * f;
+            const float* qf = m->ifeat_freq + ((int64_t)i * F + fb) * f;
+            float dot = 0.0f;
+            for (int k = 0; k < f; k++) {
+                float qi_eff = qi[k] + qf[k];
+                float pu_eff = pu[k] + su_row[k] + dev * pu2[k];
+                if (!low_mem) {
+                    if (ud_idx >= 0) pu_eff += m->ufeat_day[ud_idx * f + k];
+                    pu_eff += su_day[k] + su_freq[k];
+                }
+                dot += pu_eff * qi_eff;
+            }
+            const float bias_part = m->gbias + bu_t + bi_t * cu_t;
+            const float score = bias_part + dot + 0.0f;
+
+            float err;
+            if (has_ordinal) {
+                float g_t[4];
+                err = ord_grad(m->thresholds, score, (int)tr->raw_ratings[t], g_t);
+                for (int k = 0; k < 4; k++) {
+                    float tk = m->thresholds[k];
+                    m->thresholds[k] -= c.lr_t * (g_t[k] + c.reg_t * tk);
+                }
+            } else {
+                err = score - r;
+            }
+            const float err_b = err * c.w_bias;
+            const float err_d = err * c.w_factor;
+
+            /* bias steps (src/tx.rs:981-990) */
+            m->ubias[u] -= c.lr_ub * err_b;
+            m->but_bin[(int64_t)u * m->B + b] -= c.lr_tu * err_b;
+            m->alpha_u[u] -= c.lr_ta * err_b * dev;
+            if (ud_idx >= 0) m->ubias_day[ud_idx] -= c.lr_ubd * err_b;
+            m->ibias[i] -= c.lr_ib * (err_b * cu_t);
+            m->bit_bin[(int64_t)i * m->B + b] -= c.lr_ti * (err_b * cu_t);
+            m->ibias_freq[(int64_t)i * F + fb] -= c.lr_ibf * (err_b * cu_t);
+            m->cu[u] -= c.lr_cu * (err_b * bi_t + c.reg_cu * (m->cu[u] - 1.0f));
+
+            /* factor steps (src/tx.rs:995-1023) */
+            const int has_ud = !low_mem && c.lr_ud != 0.0f;
+            if (!c.sum_err_bug) sum_err += err_b;
+            float* pum = m->ufeat + (int64_t)u * f;
+            float* pu2m = m->ufeat2 + (int64_t)u * f;
+            float* qim = m->ifeat + (int64_t)i * f;
+            float* qfm = m->ifeat_freq + ((int64_t)i * F + fb) * f;
+            for (int k = 0; k < f; k++) {
+                const float qi_k = qim[k];
+                const float qf_k = qfm[k];
+                const float qi_eff_k = qi_k + qf_k;
+                const float pu_k = pum[k];
+                const float pu2_k = pu2m[k];
+                sum_err_q[k] += err_d * qi_eff_k;
+                if (!low_mem) sum_err_q_day[k] += err_d * qi_eff_k;
+                if (c.sum_err_bug) sum_err += err_b;
+                pum[k] -= c.lr_u * (err_d * qi_eff_k + c.reg_u * pu_k);
+                pu2m[k] -= c.lr_u2 * (err_d * qi_eff_k * dev + c.reg_u2 * pu2_k);
+                float pud_k = 0.0f;
+                if (has_ud && ud_idx >= 0) {
+                    float pud = m->ufeat_day[ud_idx * f + k];
+                    m->ufeat_day[ud_idx * f + k] -= c.lr_ud * (err_d * qi_eff_k + c.reg_ud * pud);
+                    pud_k = pud;
+                }
+                const float pu_eff = pu_k + dev * pu2_k + pud_k + su[k] + (low_mem ? 0.0f : su_day[k] + su_freq[k]);
+                qim[k] -= c.lr_i * (err_d * pu_eff + c.reg_i * qi_k);
+                qfm[k] -= c.lr_iqf * (err_d * pu_eff + c.reg_iqf * qf_k);
+            }
+
+            /* end of day run (src/tx.rs:1025-1036) */
+            if (!low_mem && t == day_stop) {
+                for (int64_t td = day_start; td <= day_stop; td++) {
+                    float* y = m->yfeat_day + (int64_t)tr->item_idxs[td] * f;
+                    for (int k = 0; k < f; k++) {
+                        float yj = y[k];
+                        y[k] -= c.lr_yd * (sum_err_q_day[k] / norm_day + c.reg_yd * yj);
+                    }
+                }
+                dr_idx += 1;
+            }
+        }
+
+        /* end of user (src/tx.rs:1099-1106) */
+        for (int64_t t = start; t < end; t++) {
+            int32_t j = tr->item_idxs[t];
+            float* y = m->yfeat + (int64_t)j * f;
+            for (int k = 0; k < f; k++) {
+                float yj = y[k];
+                y[k] -= c.lr_y * (sum_err_q[k] / norm + c.reg_y * yj);
+            }
+            m->ybias[j] -= c.lr_yb * (sum_err / norm);
+        }
+    }
+
+    rebuild_ycache(m, tr, pr);
+    if (!low_mem) rebuild_ycache_day(m, tr, pr);
+    if (has_ordinal) {
+        /* OrdinalHead::enforce_sorted_with_gap (src/lib.rs:519-525) */
+        qsort(m->thresholds, 4, sizeof(float), cmp_f32);
+        for (int k = 1; k < 4; k++) {
+            float min_ok = m->thresholds[k - 1] + c.th_gap;
+            if (m->thresholds[k] < min_ok) m->thresholds[k] = min_ok;
+        }
+    }
+    free(su); free(sum_err_q); free(sum_err_q_day); free(su_day); free(su_freq); free(rs); free(re);
+}
+
+OR_API void or_fit_epoch(or_model* m, const or_dataset* tr, const or_dataset* pr, int64_t epoch) {
+    int64_t* users = malloc(sizeof(int64_t) * (size_t)(m->n_users ? m->n_users : 1));
+    get_users(m->n_users, m->cfg.shuffle_users, m->cfg.seed, epoch, users);
+    fit_epoch_users(m, tr, pr, epoch, users);
+    free(users);
+}
+
+/* Same epoch with a caller-given user order (for order-controlled checks). */
+OR_API void or_fit_epoch_order(or_model* m, const or_dataset* tr, const or_dataset* pr, int64_t epoch,
+                               const int64_t* users) {
+    fit_epoch_users(m, tr, pr, epoch, users);
+}
+
+/* ------------------------------------------------------------------------ */
+/* predict / predict_subscores (src/tx.rs:303-339, 663-724)                 */
+/* ------------------------------------------------------------------------ */
+
+static float predict_one(const or_model* m, int32_t u, int32_t i, int32_t day) {
+    const int f = m->f, F = m->F;
+    const int b = time_bin(m, day);
+    const float dev = dev_of(m->tu_mean[u], day, m->cfg.beta);
+    const int64_t ud_idx = sud_index(&m->ud, u, (int16_t)day);
+    const int fb = ud_idx >= 0 ? freq_bin(m->ud.day_cnts[ud_idx], F) : 0;
+    const float bu_day = ud_idx >= 0 ? m->ubias_day[ud_idx] : 0.0f;
+    const float bu_t = m->ubias[u] + m->but_bin[(int64_t)u * m->B + b] + m->alpha_u[u] * dev + bu_day + m->ycache_bias[u];
+    const float bi_t = m->ibias[i] + m->bit_bin[(int64_t)i * m->B + b] + m->ibias_freq[(int64_t)i * F + fb];
+    const float cu_t = m->cu[u] + (ud_idx >= 0 ? m->cut[ud_idx] : 0.0f);
+    const float* pu = m->ufeat + (int64_t)u * f;
+    const float* pu2 = m->ufeat2 + (int64_t)u * f;
+    const float* su = m->ycache + (int64_t)u * f;
+    const float* qi = m->ifeat + (int64_t)i * f;
+    const float* qf = m->ifeat_freq + ((int64_t)i * F + fb) * f;
+    float dot = 0.0f;
+    for (int k = 0; k < f; k++) {
+        float qi_eff = qi[k] + qf[k];
+        float pu_eff = pu[k] + su[k] + dev * pu2[k];
+        if (!m->cfg.low_memory && ud_idx >= 0) {
+            pu_eff += m->ufeat_day[ud_idx * f + k];
+            pu_eff += m->ycache_day[ud_idx * f + k];
+        }
+        dot += pu_eff * qi_eff;
+    }
+    const float bias_part = m->gbias + bu_t + bi_t * cu_t;
+    const float score = bias_part + dot + 0.0f;
+    if (m->cfg.has_ordinal) {
+        float p[5];
+        ord_probs(m->thresholds, score, p);
+        return 1.0f * p[0] + 2.0f * p[1] + 3.0f * p[2] + 4.0f * p[3] + 5.0f * p[4];
+    }
+    return score;
+}
+
+OR_API void or_predict(const or_model* m, int64_t n, const int32_t* u, const int32_t* i, const int16_t* d,
+                       float* out) {
+    for (int64_t k = 0; k < n; k++) out[k] = predict_one(m, u[k], i[k], d[k]);
+}
+
+OR_API void or_predict_subscores(const or_model* m, int64_t n, const int32_t* us, const int32_t* is,
+                                 const int16_t* ds, float* bias_o, float* mf_o, float* nsvd_o) {
+    const int f = m->f, F = m->F;
+    for (int64_t q = 0; q < n; q++) {
+        const int32_t u = us[q], i = is[q], day = ds[q];
+        const int b = time_bin(m, day);
+        const float dev = dev_of(m->tu_mean[u], day, m->cfg.beta);
+        const int64_t ud_idx = sud_index(&m->ud, u, (int16_t)day);
+        const int fb = ud_idx >= 0 ? freq_bin(m->ud.day_cnts[ud_idx], F) : 0;
+        const float bu_day = ud_idx >= 0 ? m->ubias_day[ud_idx] : 0.0f;
+        const float bi_t = m->ibias[i] + m->bit_bin[(int64_t)i * m->B + b] + m->ibias_freq[(int64_t)i * F + fb];
+        const float cu_t = m->cu[u] + (ud_idx >= 0 ? m->cut[ud_idx] : 0.0f);
+        const float bias = m->gbias + m->ubias[u] + m->ibias[i] + m->ibias_freq[(int64_t)i * F + fb]
+                         + m->but_bin[(int64_t)u * m->B + b] + m->alpha_u[u] * dev + bu_day
+                         + m->bit_bin[(int64_t)i * m->B + b] + bi_t * (cu_t - 1.0f) + m->ycache_bias[u];
+        const float* pu = m->ufeat + (int64_t)u * f;
+        const float* pu2 = m->ufeat2 + (int64_t)u * f;
+        const float* su = m->ycache + (int64_t)u * f;
+        const float* qi = m->ifeat + (int64_t)i * f;
+        const float* qf = m->ifeat_freq + ((int64_t)i * F + fb) * f;
+        float mf = 0.0f, nsvd1 = 0.0f;
+        for (int k = 0; k < f; k++) {
+            float qi_eff = qi[k] + qf[k];
+            float pu_eff = pu[k] + dev * pu2[k];
+            if (!m->cfg.low_memory && ud_idx >= 0) pu_eff += m->ufeat_day[ud_idx * f + k];
+            mf += pu_eff * qi_eff;
+            float su_eff = su[k];
+            if (!m->cfg.low_memory && ud_idx >= 0) su_eff += m->ycache_day[ud_idx * f + k];
+            nsvd1 += su_eff * qi_eff;
+        }
+        bias_o[q] = bias; mf_o[q] = mf; nsvd_o[q] = nsvd1;
+    }
+}
+
+/* FP64 predict oracle: the predict formula of src/tx.rs:303-339 evaluated in
+ * double from the FP32 parameters (dev via pow in double). */
+OR_API void or_predict_f64(const or_model* m, int64_t n, const int32_t* us, const int32_t* is,
+                           const int16_t* ds, double* out) {
+    const int f = m->f, F = m->F;
+    for (int64_t q = 0; q < n; q++) {
+        const int32_t u = us[q], i = is[q], day = ds[q];
+        const int b = time_bin(m, day);
+        double dt = (double)day - (double)m->tu_mean[u];
+        double dev = dt == 0.0 ? 0.0 : (dt > 0 ? 1.0 : -1.0) * pow(fabs(dt), (double)m->cfg.beta);
+        const int64_t ud_idx = sud_index(&m->ud, u, (int16_t)day);
+        const int fb = ud_idx >= 0 ? freq_bin(m->ud.day_cnts[ud_idx], F) : 0;
+        double bu_day = ud_idx >= 0 ? m->ubias_day[ud_idx] : 0.0;
+        double bu_t = (double)m->ubias[u] + m->but_bin[(int64_t)u * m->B + b] + (double)m->alpha_u[u] * dev + bu_day + m->ycache_bias[u];
+        double bi_t = (double)m->ibias[i] + m->bit_bin[(int64_t)i * m->B + b] + m->ibias_freq[(int64_t)i * F + fb];
+        double cu_t = (double)m->cu[u] + (ud_idx >= 0 ? m->cut[ud_idx] : 0.0);
+        double dot = 0.0;
+        for (int k = 0; k < f; k++) {
+            double qi_eff = (double)m->ifeat[(int64_t)i * f + k] + m->ifeat_freq[((int64_t)i * F + fb) * f + k];
+            double pu_eff = (double)m->ufeat[(int64_t)u * f + k] + m->ycache[(int64_t)u * f + k] + dev * m->ufeat2[(int64_t)u * f + k];
+            if (!m->cfg.low_memory && ud_idx >= 0) pu_eff += (double)m->ufeat_day[ud_idx * f + k] + m->ycache_day[ud_idx * f + k];
+            dot += pu_eff * qi_eff;
+        }
+        double score = (double)m->gbias + bu_t + bi_t * cu_t + dot;
+        if (m->cfg.has_ordinal) {
+            double fc[6] = {0, 0, 0, 0, 0, 1};
+            for (int k = 1; k < 5; k++) fc[k] = 1.0 / (1.0 + exp(-((double)m->thresholds[k - 1] - score)));
+            double e = 0;
+            for (int k = 0; k < 5; k++) e += (k + 1) * (fc[k + 1] - fc[k]);
+            score = e;
+        }
+        out[q] = score;
+    }
+}
+
+/* calc_rmse (src/lib.rs:659-673): FP64 SSE over rows with is_test == 0. */
+OR_API double or_calc_rmse(const or_model* m, const or_dataset* ds) {
+    double cnt = 0.0, sse = 0.0;
+    for (int64_t t = 0; t < ds->n_ratings; t++) {
+        if (ds->is_test && ds->is_test[t] != 0) continue;
+        float p = predict_one(m, ds->user_idxs[t], ds->item_idxs[t], ds->dates[t]);
+        double e = (double)(p - ds->residuals[t]);
+        cnt += 1.0;
+        sse += e * e;
+    }
+    return sqrt(sse / cnt);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Tsvdx4pModel (src/tsvdx4p.rs:257-806)                                    */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    float *d_yfeat, *d_ybias, *d_yfeat_day;
+    uint32_t* n_contrib;
+} nsvd_delta;
+
+typedef struct {
+    or_model* m;
+    const or_dataset* tr;
+    const int64_t* users;
+    int64_t lo, hi;       /* user-chunk or item-chunk range */
+    nsvd_delta d;
+} p_job;
+
+static void* p_user_chunk(void* arg) {
+    p_job* J = arg;
+    or_model* m = J->m;
+    const or_dataset* tr = J->tr;
+    const or_config c = m->cfg;
+    const int f = m->f, F = m->F, low_mem = c.low_memory, use_probe = c.full_su;
+    const int has_ud = !low_mem && c.lr_ud != 0.0f;
+    float* su = zalloc(f);
+    float* sum_err_q = zalloc(f);
+    float* sum_err_q_day = zalloc(f);
+    float* su_day = zalloc(f);
+    int64_t maxcnt = 1;
+    for (int64_t k = J->lo; k < J->hi; k++) {
+        int32_t u = (int32_t)J->users[k];
+        int64_t cc = m->user_offsets[u + 1] - m->user_offsets[u];
+        if (cc > maxcnt) maxcnt = cc;
+    }
+    int64_t* rs = malloc(sizeof(int64_t) * (size_t)maxcnt);
+    int64_t* re = malloc(sizeof(int64_t) * (size_t)maxcnt);
+    for (int64_t k = J->lo; k < J->hi; k++) {
+        const int32_t u = (int32_t)J->users[k];
+        const int64_t start = m->user_offsets[u], end = m->user_offsets[u + 1], cnt = end - start;
+        if (cnt == 0) continue;
+        if (!low_mem) day_ranges(tr->dates, start, end, rs, re);
+        for (int kk = 0; kk < f; kk++) su[kk] = 0.0f;
+        float su_bias = 0.0f;
+        for (int64_t t = start; t < end; t++) {
+            int32_t j = tr->item_idxs[t];
+            for (int kk = 0; kk < f; kk++) su[kk] += m->yfeat[(int64_t)j * f + kk];
+            su_bias += m->ybias[j];
+        }
+        int64_t total = cnt;
+        if (use_probe) {
+            for (int64_t q = m->pbu_start[u]; q < m->pbu_start[u + 1]; q++) {
+                uint32_t j = m->pbu_items[q];
+                for (int kk = 0; kk < f; kk++) su[kk] += m->yfeat[(int64_t)j * f + kk];
+                su_bias += m->ybias[j];
+            }
+            total = cnt + (m->pbu_start[u + 1] - m->pbu_start[u]);
+        }
+        const float norm = sqrtf((float)total);
+        for (int kk = 0; kk < f; kk++) su[kk] /= norm;
+        su_bias /= norm;
+        for (int kk = 0; kk < f; kk++) m->ycache[(int64_t)u * f + kk] = su[kk];
+        m->ycache_bias[u] = su_bias;
+
+        float sum_err = 0.0f, norm_day = 0.0f;
+        for (int kk = 0; kk < f; kk++) { sum_err_q[kk] = 0.0f; sum_err_q_day[kk] = 0.0f; su_day[kk] = 0.0f; }
+        int64_t dr_idx = 0;
+        int cur_fb = 0;
+        for (int64_t t = start; t < end; t++) {
+            const int32_t i = tr->item_idxs[t];
+            const float r = tr->residuals[t];
+            const int32_t day = tr->dates[t];
+            const int16_t day16 = tr->dates[t];
+            const int b = time_bin(m, day);
+            const float dev = dev_of(m->tu_mean[u], day, c.beta);
+            int64_t ds = 0, de = 0;
+            if (!low_mem) { ds = rs[dr_idx]; de = re[dr_idx]; }
+            if (!low_mem && t == ds) {
+                for (int kk = 0; kk < f; kk++) { sum_err_q_day[kk] = 0.0f; su_day[kk] = 0.0f; }
+                int64_t udd = sud_index(&m->ud, u, day16);
+                cur_fb = udd >= 0 ? freq_bin(m->ud.day_cnts[udd], F) : 0;
+                for (int64_t td = ds; td <= de; td++) {
+                    int32_t j = tr->item_idxs[td];
+                    for (int kk = 0; kk < f; kk++) su_day[kk] += m->yfeat_day[(int64_t)j * f + kk];
+                }
+                float day_cnt = (float)(de - ds + 1);
+                if (use_probe && udd >= 0) {
+                    for (int64_t q = m->pbd_start[udd]; q < m->pbd_start[udd + 1]; q++) {
+                        uint32_t j = m->pbd_items[q];
+                        for (int kk = 0; kk < f; kk++) su_day[kk] += m->yfeat_day[(int64_t)j * f + kk];
+                        day_cnt += 1.0f;
+                    }
+                }
+                norm_day = sqrtf(day_cnt);
+                for (int kk = 0; kk < f; kk++) su_day[kk] /= norm_day;
+            }
+            const int fb = cur_fb;
+            const int64_t ud_idx = sud_index(&m->ud, u, day16);
+            const float ubias_u = m->ubias[u];
+            const float but_ub = m->but_bin[(int64_t)u * m->B + b];
+            const float alpha = m->alpha_u[u];
+            const float bu_day = ud_idx >= 0 ? m->ubias_day[ud_idx] : 0.0f;
+            const float cu_u = m->cu[u];
+            const float cu_t = cu_u + (ud_idx >= 0 ? m->cut[ud_idx] : 0.0f);
+            const float bu_t = ubias_u + but_ub + alpha * dev + bu_day + su_bias;
+            const float bi_t = m->ibias[i] + m->bit_bin[(int64_t)i * m->B + b] + m->ibias_freq[(int64_t)i * F + fb];
+            float dot = 0.0f;
+            for (int kk = 0; kk < f; kk++) {
+                float qi_eff = m->ifeat[(int64_t)i * f + kk] + m->ifeat_freq[((int64_t)i * F + fb) * f + kk];
+                float pu_k = m->ufeat[(int64_t)u * f + kk], pu2_k = m->ufeat2[(int64_t)u * f + kk];
+                float pu_eff = pu_k + su[kk] + dev * pu2_k;
+                if (!low_mem) {
+                    if (has_ud && ud_idx >= 0) pu_eff += m->ufeat_day[ud_idx * f + kk];
+                    pu_eff += su_day[kk];
+                }
+                dot += pu_eff * qi_eff;
+            }
+            const float score = m->gbias + bu_t + bi_t * cu_t + dot;
+            const float err = score - r;
+            m->ubias[u] -= c.lr_ub * err;
+            m->but_bin[(int64_t)u * m->B + b] -= c.lr_tu * err;
+            m->alpha_u[u] -= c.lr_ta * err * dev;
+            if (ud_idx >= 0) m->ubias_day[ud_idx] -= c.lr_ubd * err;
+            m->cu[u] -= c.lr_cu * (err * bi_t + c.reg_cu * (cu_u - 1.0f));
+            sum_err += err;
+            for (int kk = 0; kk < f; kk++) {
+                float qi_eff = m->ifeat[(int64_t)i * f + kk] + m->ifeat_freq[((int64_t)i * F + fb) * f + kk];
+                float pu_k = m->ufeat[(int64_t)u * f + kk], pu2_k = m->ufeat2[(int64_t)u * f + kk];
+                sum_err_q[kk] += err * qi_eff;
+                if (!low_mem) sum_err_q_day[kk] += err * qi_eff;
+                m->ufeat[(int64_t)u * f + kk] -= c.lr_u * (err * qi_eff + c.reg_u * pu_k);
+                m->ufeat2[(int64_t)u * f + kk] -= c.lr_u2 * (err * qi_eff * dev + c.reg_u2 * pu2_k);
+                if (has_ud && ud_idx >= 0) {
+                    float pud = m->ufeat_day[ud_idx * f + kk];
+                    m->ufeat_day[ud_idx * f + kk] -= c.lr_ud * (err * qi_eff + c.reg_ud * pud);
+                }
+            }
+            if (!low_mem && t == de) {
+                for (int64_t td = ds; td <= de; td++) {
+                    int32_t j = tr->item_idxs[td];
+                    for (int kk = 0; kk < f; kk++) J->d.d_yfeat_day[(int64_t)j * f + kk] += sum_err_q_day[kk] / norm_day;
+                }
+                dr_idx += 1;
+            }
+        }
+        for (int64_t t = start; t < end; t++) {
+            int32_t j = tr->item_idxs[t];
+            for (int kk = 0; kk < f; kk++) J->d.d_yfeat[(int64_t)j * f + kk] += sum_err_q[kk] / norm;
+            J->d.d_ybias[j] += sum_err / norm;
+            J->d.n_contrib[j] += 1;
+        }
+    }
+    free(su); free(sum_err_q); free(sum_err_q_day); free(su_day); free(rs); free(re);
+    return NULL;
+}
+
+static void* p_item_chunk(void* arg) {
+    p_job* J = arg;
+    or_model* m = J->m;
+    const or_dataset* tr = J->tr;
+    const or_config c = m->cfg;
+    const int f = m->f, F = m->F, low_mem = c.low_memory;
+    float* pu_effs = zalloc(f);
+    for (int64_t i = J->lo; i < J->hi; i++) {
+        for (int64_t e = m->bi_starts[i]; e < m->bi_starts[i + 1]; e++) {
+            const int32_t u = (int32_t)m->bi_user[e];
+            const int64_t t = m->bi_rating[e];
+            const float r = tr->residuals[t];
+            const int32_t day = tr->dates[t];
+            const int b = time_bin(m, day);
+            const float dev = dev_of(m->tu_mean[u], day, c.beta);
+            const int64_t ud_idx = sud_index(&m->ud, u, tr->dates[t]);
+            const int fb = ud_idx >= 0 ? freq_bin(m->ud.day_cnts[ud_idx], F) : 0;
+            const float bu_day = ud_idx >= 0 ? m->ubias_day[ud_idx] : 0.0f;
+            const float bu_t = m->ubias[u] + m->but_bin[(int64_t)u * m->B + b] + m->alpha_u[u] * dev + bu_day + m->ycache_bias[u];
+            const float bi_t = m->ibias[i] + m->bit_bin[i * m->B + b] + m->ibias_freq[i * F + fb];
+            const float cu_t = m->cu[u] + (ud_idx >= 0 ? m->cut[ud_idx] : 0.0f);
+            float dot = 0.0f;
+            for (int k = 0; k < f; k++) {
+                float qi_k = m->ifeat[i * f + k], qf_k = m->ifeat_freq[(i * F + fb) * f + k];
+                float qi_eff = qi_k + qf_k;
+                float pu_eff = m->ufeat[(int64_t)u * f + k] + m->ycache[(int64_t)u * f + k] + dev * m->ufeat2[(int64_t)u * f + k];
+                if (!low_mem && ud_idx >= 0) {
+                    pu_eff += m->ufeat_day[ud_idx * f + k];
+                    pu_eff += m->ycache_day[ud_idx * f + k];
+                }
+                pu_effs[k] = pu_eff;
+                dot += pu_eff * qi_eff;
+            }
+            const float score = m->gbias + bu_t + bi_t * cu_t + dot;
+            const float err = score - r;
+            m->ibias[i] -= c.lr_ib * (err * cu_t);
+            m->bit_bin[i * m->B + b] -= c.lr_ti * (err * cu_t);
+            m->ibias_freq[i * F + fb] -= c.lr_ibf * (err * cu_t);
+            for (int k = 0; k < f; k++) {
+                float qi_k = m->ifeat[i * f + k], qf_k = m->ifeat_freq[(i * F + fb) * f + k];
+                m->ifeat[i * f + k] -= c.lr_i * (err * pu_effs[k] + c.reg_i * qi_k);
+                m->ifeat_freq[(i * F + fb) * f + k] -= c.lr_iqf * (err * pu_effs[k] + c.reg_iqf * qf_k);
+            }
+        }
+    }
+    free(pu_effs);
+    return NULL;
+}
+
+static void run_jobs(p_job* jobs, int n, void* (*fn)(void*), int max_threads) {
+    if (max_threads <= 1 || n <= 1) {
+        for (int k = 0; k < n; k++) fn(&jobs[k]);
+        return;
+    }
+    pthread_t* th = malloc(sizeof(pthread_t) * (size_t)n);
+    for (int base = 0; base < n; base += max_threads) {
+        int lim = base + max_threads < n ? base + max_threads : n;
+        for (int k = base; k < lim; k++) pthread_create(&th[k], NULL, fn, &jobs[k]);
+        for (int k = base; k < lim; k++) pthread_join(th[k], NULL);
+    }
+    free(th);
+}
+
+static void p_build_by_item(or_model* m, const or_dataset* tr) {
+    if (m->bi_starts) return;
+    const int32_t I = m->n_items;
+    m->bi_starts = calloc((size_t)I + 1, sizeof(int64_t));
+    for (int64_t t = 0; t < tr->n_ratings; t++) m->bi_starts[tr->item_idxs[t] + 1]++;
+    for (int32_t i = 0; i < I; i++) m->bi_starts[i + 1] += m->bi_starts[i];
+    int64_t n = tr->n_ratings ? tr->n_ratings : 1;
+    m->bi_user = malloc(sizeof(uint32_t) * (size_t)n);
+    m->bi_rating = malloc(sizeof(uint32_t) * (size_t)n);
+    int64_t* pos = malloc(sizeof(int64_t) * (size_t)I);
+    memcpy(pos, m->bi_starts, sizeof(int64_t) * (size_t)I);
+    for (int64_t t = 0; t < tr->n_ratings; t++) {
+        int32_t i = tr->item_idxs[t];
+        m->bi_user[pos[i]] = (uint32_t)tr->user_idxs[t];
+        m->bi_rating[pos[i]] = (uint32_t)t;
+        pos[i]++;
+    }
+    free(pos);
+}
+
+/* Tsvdx4pModel::fit_epoch (src/tsvdx4p.rs:798-805).  `workers` is the number
+ * of OS threads used to run the n_threads chunks (rayon's pool size). */
+OR_API void or4p_fit_epoch(or_model* m, const or_dataset* tr, const or_dataset* pr, int64_t epoch, int workers) {
+    const or_config c = m->cfg;
+    for (int k = 0; k < 5; k++) {
+        if (c.seq_epochs[k] == epoch) { or_fit_epoch(m, tr, pr, epoch); return; }
+    }
+    p_build_by_item(m, tr);
+    const int f = m->f, I = m->n_items;
+    const int nt = c.n_threads > 0 ? c.n_threads : (workers > 0 ? workers : 1);
+    if (workers <= 0) workers = nt;
+
+    /* fit_epoch_user (src/tsvdx4p.rs:390-639) */
+    if (epoch == c.reset_u_epoch) {
+        memset(m->ufeat, 0, sizeof(float) * (size_t)m->n_users * f);
+        memset(m->ufeat2, 0, sizeof(float) * (size_t)m->n_users * f);
+    }
+    int64_t* users = malloc(sizeof(int64_t) * (size_t)(m->n_users ? m->n_users : 1));
+    get_users(m->n_users, c.shuffle_users, c.seed, epoch, users);
+    int64_t chunk = ((int64_t)m->n_users + nt - 1) / nt;
+    if (chunk < 1) chunk = 1;
+    int nchunks = (int)(((int64_t)m->n_users + chunk - 1) / chunk);
+    p_job* jobs = calloc((size_t)(nchunks ? nchunks : 1), sizeof(p_job));
+    for (int k = 0; k < nchunks; k++) {
+        jobs[k].m = m; jobs[k].tr = tr; jobs[k].users = users;
+        jobs[k].lo = (int64_t)k * chunk;
+        jobs[k].hi = jobs[k].lo + chunk < m->n_users ? jobs[k].lo + chunk : m->n_users;
+        jobs[k].d.d_yfeat = zalloc((int64_t)I * f);
+        jobs[k].d.d_ybias = zalloc(I);
+        jobs[k].d.d_yfeat_day = c.low_memory ? NULL : zalloc((int64_t)I * f);
+        jobs[k].d.n_contrib = calloc((size_t)I, sizeof(uint32_t));
+    }
+    run_jobs(jobs, nchunks, p_user_chunk, workers);
+
+    /* apply_nsvd_deltas (src/tsvdx4p.rs:341-385) */
+    float* tdy = zalloc((int64_t)I * f);
+    float* tdyb = zalloc(I);
+    uint32_t* tnc = calloc((size_t)I, sizeof(uint32_t));
+    for (int k = 0; k < nchunks; k++) {
+        for (int64_t q = 0; q < (int64_t)I * f; q++) tdy[q] += jobs[k].d.d_yfeat[q];
+        for (int32_t j = 0; j < I; j++) { tdyb[j] += jobs[k].d.d_ybias[j]; tnc[j] += jobs[k].d.n_contrib[j]; }
+    }
+    const float ex = c.nsvd_norm_exp;
+    for (int32_t j = 0; j < I; j++) {
+        uint32_t nc = tnc[j];
+        if (nc == 0) continue;
+        float scale = ex == 0.0f ? 1.0f : powf((float)nc, ex);
+        for (int k = 0; k < f; k++) {
+            float yj = m->yfeat[(int64_t)j * f + k];
+            m->yfeat[(int64_t)j * f + k] -= c.lr_y * (tdy[(int64_t)j * f + k] / scale + c.reg_y * yj);
+        }
+        m->ybias[j] -= c.lr_yb * (tdyb[j] / scale);
+    }
+    if (!c.low_memory) {
+        float* tdyd = zalloc((int64_t)I * f);
+        for (int k = 0; k < nchunks; k++)
+            for (int64_t q = 0; q < (int64_t)I * f; q++) tdyd[q] += jobs[k].d.d_yfeat_day[q];
+        for (int32_t j = 0; j < I; j++) {
+            uint32_t nc = tnc[j];
+            if (nc == 0) continue;
+            float scale = ex == 0.0f ? 1.0f : powf((float)nc, ex);
+            for (int k = 0; k < f; k++) {
+                float yj = m->yfeat_day[(int64_t)j * f + k];
+                m->yfeat_day[(int64_t)j * f + k] -= c.lr_yd * (tdyd[(int64_t)j * f + k] / scale + c.reg_yd * yj);
+            }
+        }
+        free(tdyd);
+    }
+    for (int k = 0; k < nchunks; k++) {
+        free(jobs[k].d.d_yfeat); free(jobs[k].d.d_ybias); free(jobs[k].d.d_yfeat_day); free(jobs[k].d.n_contrib);
+    }
+    free(tdy); free(tdyb); free(tnc); free(jobs); free(users);
+    rebuild_ycache(m, tr, pr);
+    if (!c.low_memory) rebuild_ycache_day(m, tr, pr);
+
+    /* fit_epoch_item (src/tsvdx4p.rs:644-762) */
+    int64_t ichunk = ((int64_t)I + nt - 1) / nt;
+    if (ichunk < 1) ichunk = 1;
+    int nich = (int)(((int64_t)I + ichunk - 1) / ichunk);
+    p_job* ij = calloc((size_t)(nich ? nich : 1), sizeof(p_job));
+    for (int k = 0; k < nich; k++) {
+        ij[k].m = m; ij[k].tr = tr;
+        ij[k].lo = (int64_t)k * ichunk;
+        ij[k].hi = ij[k].lo + ichunk < I ? ij[k].lo + ichunk : I;
+    }
+    run_jobs(ij, nich, p_item_chunk, workers);
+    free(ij);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Parameter access and RNG hooks for the checks                            */
+/* ------------------------------------------------------------------------ */
+
+/* Parameter ids, shared with the product's C-ABI (include/tsvd_tx.h). */
+enum {
+    P_UBIAS = 0, P_IBIAS, P_UFEAT, P_UFEAT2, P_IFEAT, P_YFEAT, P_YCACHE, P_YFEAT_DAY,
+    P_TU_MEAN, P_ALPHA_U, P_BUT_BIN, P_BIT_BIN, P_YBIAS, P_YCACHE_BIAS, P_UBIAS_DAY,
+    P_UFEAT_DAY, P_YCACHE_DAY, P_IBIAS_FREQ, P_IFEAT_FREQ, P_CU, P_CUT, P_GBIAS, P_THRESHOLDS,
+    P_COUNT
+};
+
+static float* param_ptr(or_model* m, int id, int64_t* n) {
+    const int64_t U = m->n_users, I = m->n_items, f = m->f, nud = m->ud.n_total;
+    const int lm = m->cfg.low_memory;
+    switch (id) {
+    case P_UBIAS: *n = U; return m->ubias;
+    case P_IBIAS: *n = I; return m->ibias;
+    case P_UFEAT: *n = U * f; return m->ufeat;
+    case P_UFEAT2: *n = U * f; return m->ufeat2;
+    case P_IFEAT: *n = I * f; return m->ifeat;
+    case P_YFEAT: *n = I * f; return m->yfeat;
+    case P_YCACHE: *n = U * f; return m->ycache;
+    case P_YFEAT_DAY: *n = lm ? 0 : I * f; return m->yfeat_day;
+    case P_TU_MEAN: *n = U; return m->tu_mean;
+    case P_ALPHA_U: *n = U; return m->alpha_u;
+    case P_BUT_BIN: *n = U * m->B; return m->but_bin;
+    case P_BIT_BIN: *n = I * m->B; return m->bit_bin;
+    case P_YBIAS: *n = I; return m->ybias;
+    case P_YCACHE_BIAS: *n = U; return m->ycache_bias;
+    case P_UBIAS_DAY: *n = nud; return m->ubias_day;
+    case P_UFEAT_DAY: *n = lm ? 0 : nud * f; return m->ufeat_day;
+    case P_YCACHE_DAY: *n = lm ? 0 : nud * f; return m->ycache_day;
+    case P_IBIAS_FREQ: *n = I * m->F; return m->ibias_freq;
+    case P_IFEAT_FREQ: *n = I * m->F * f; return m->ifeat_freq;
+    case P_CU: *n = U; return m->cu;
+    case P_CUT: *n = nud; return m->cut;
+    case P_GBIAS: *n = 1; return &m->gbias;
+    case P_THRESHOLDS: *n = 4; return m->thresholds;
+    default: *n = -1; return NULL;
+    }
+}
+
+OR_API int64_t or_param_len(or_model* m, int id) { int64_t n; param_ptr(m, id, &n); return n; }
+
+OR_API int or_get_param(or_model* m, int id, float* out, int64_t n) {
+    int64_t len; float* p = param_ptr(m, id, &len);
+    if (!p || n != len) return -1;
+    memcpy(out, p, sizeof(float) * (size_t)n);
+    return 0;
+}
+
+OR_API int or_set_param(or_model* m, int id, const float* in, int64_t n) {
+    int64_t len; float* p = param_ptr(m, id, &len);
+    if (!p || n != len) return -1;
+    memcpy(p, in, sizeof(float) * (size_t)n);
+    return 0;
+}
+
+OR_API int32_t or_day_range(const or_model* m) { return m->day_range; }
+OR_API int64_t or_n_ud(const or_model* m) { return m->ud.n_total; }
+
+OR_API void or_sud_arrays(const or_model* m, int64_t* starts, int16_t* dates, uint32_t* cnts) {
+    memcpy(starts, m->ud.starts, sizeof(int64_t) * ((size_t)m->n_users + 1));
+    memcpy(dates, m->ud.dates, sizeof(int16_t) * (size_t)m->ud.n_total);
+    memcpy(cnts, m->ud.day_cnts, sizeof(uint32_t) * (size_t)m->ud.n_total);
+}
+
+OR_API void or_get_users(int64_t n, int shuffle, uint64_t seed, int64_t epoch, int64_t* out) {
+    get_users(n, shuffle, seed, epoch, out);
+}
+
+OR_API void or_normal_fill(uint64_t seed, float sigma, int64_t n, float* out) {
+    zig_init();
+    or_rng r;
+    rng_seed_u64(&r, seed, 12);
+    rand_fill(&r, out, n, sigma);
+}
+
+OR_API void or_rng_words(uint64_t seed, int rounds, int64_t n, uint32_t* out) {
+    or_rng r;
+    rng_seed_u64(&r, seed, rounds);
+    for (int64_t k = 0; k < n; k++) out[k] = rng_u32(&r);
+}
+
+/* Raw ChaCha block for the published test vectors (key words, counter). */
+OR_API void or_chacha_block(const uint32_t key[8], uint64_t ctr, int rounds, uint32_t out[16]) {
+    chacha_block(key, ctr, rounds, out);
+}
+
+OR_API void or_zig_tables(double* x, double* f) {
+    zig_init();
+    memcpy(x, ZX, sizeof ZX);
+    memcpy(f, ZF, sizeof ZF);
+}
+
+OR_API float or_dev(float tu_mean, int32_t day, float beta) { return dev_of(tu_mean, day, beta); }
+OR_API int or_freq_bin(uint32_t cnt, int nfb) { return freq_bin(cnt, nfb); }
