@@ -1,0 +1,195 @@
+/*
+ * tsvd_tx.h — C ABI of the MI355X timeSVD++ (tsvdx4 / TxModel) trainer and
+ * predictor.
+ *
+ * The library libtsvd_tx.so replaces the reference's `impl Regressor for
+ * TxModel` (src/tx.rs:457-1116) behind the `Regressor` trait
+ * (src/lib.rs:616-651).  One tx_ctx is one model: it owns every parameter
+ * table in GPU memory.  All host buffers belong to the caller; the library
+ * copies what it needs during the call and never keeps a host pointer.
+ *
+ * Conventions:
+ *   - every entry point returns 0 on success and a negative tx_status on
+ *     failure; tx_last_error() then describes the failure (per thread);
+ *   - calls on one tx_ctx are not re-entrant; use one ctx per host thread;
+ *   - integer/byte inputs use the reference's on-disk dtypes (user i32,
+ *     item i32 after widening, date i16, rating i8, is_test i8).
+ */
+#ifndef TSVD_TX_H
+#define TSVD_TX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tx_ctx tx_ctx;
+
+enum tx_status {
+    TX_OK = 0,
+    TX_EINVAL = -1,     /* bad argument or configuration */
+    TX_EDEVICE = -2,    /* HIP runtime / kernel launch failure */
+    TX_ENOMEM = -3,     /* allocation failed */
+    TX_EUNSUPPORTED = -4
+};
+
+/* Mirror of TxConfig (src/tx.rs:128-204) plus OrdinalHeadConfig
+ * (src/lib.rs:502-507).  Booleans are int32 0/1.  The neighbourhood knobs
+ * (lr_w .. reg_c_day, max_neighbors, lambda1/2) belong to the tsvdx5/6 modes
+ * and must be 0 here; tx_create rejects anything else. */
+typedef struct tx_config {
+    uint64_t seed;
+    int32_t n_feat, n_epochs, n_time_bins, n_freq_bins;
+    int32_t reset_u_epoch, max_neighbors;
+    int32_t shuffle_users, sum_err_bug, low_memory, full_su, save_ifeat, has_ordinal;
+    float beta;
+    float lr_u, lr_ud, lr_u2, lr_ub, lr_ubd, lr_i, lr_ib, lr_y, lr_yb, lr_yd;
+    float lr_tu, lr_ti, lr_ta, lr_ibf, lr_iqf, lr_cu;
+    float reg_iqf, reg_cu, reg_u, reg_u2, reg_ud, reg_i, reg_y, reg_yd;
+    float sigma_iqf, sigma_u, sigma_i, sigma_y, sigma_yd;
+    float lr_w, lr_c, reg_w, reg_c, lr_w_day, lr_c_day, reg_w_day, reg_c_day;
+    float w_bias, w_factor, w_nbr;
+    float lambda1, lambda2;
+    float th_init[4];
+    float th_gap, lr_t, reg_t;
+    /* Tsvdx4pConfig extras (src/tsvdx4p.rs:128-187); unused by TxModel. */
+    float nsvd_norm_exp;
+    int32_t n_threads;
+    int32_t seq_epochs[5];
+} tx_config;
+
+/* A rating set: Dataset (src/lib.rs:227-243).  For the held-out set passed
+ * to tx_create (the reference's MaskedDataset, src/lib.rs:421-435)
+ * residuals and raw_ratings are NULL. is_test may be NULL (all zero). */
+typedef struct tx_ratings {
+    int64_t n_ratings;
+    int32_t n_users, n_items;
+    const int32_t* user_idxs;
+    const int32_t* item_idxs;
+    const int16_t* dates;
+    const float* residuals;
+    const int8_t* raw_ratings;
+    const int8_t* is_test;
+    const int32_t* user_cnts;   /* [n_users] */
+} tx_ratings;
+
+/* Training schedule on the GPU.
+ *   TX_SCHED_SERIAL: one wavefront walks users in the reference's shuffled
+ *     order with the reference's per-rating arithmetic; results are
+ *     identical to the serial reference trainer (src/tx.rs:734-1115).
+ *   TX_SCHED_STRATA: all users in parallel, one team of lanes per user, items
+ *     split into n_strata blocks processed as a rotating user-block x
+ *     item-block schedule (n_strata = 1: every item in one stratum). */
+enum tx_sched_mode { TX_SCHED_SERIAL = 0, TX_SCHED_STRATA = 1 };
+
+typedef struct tx_schedule {
+    int32_t mode;
+    int32_t n_strata;        /* STRATA only: 1 or a multiple of 8 */
+    int32_t teams_per_block; /* STRATA only: 0 = tuned default */
+    int32_t blocks;          /* STRATA only: persistent grid size, 0 = auto */
+    int32_t use_graph;       /* capture each epoch's launches in a hipGraph */
+    int32_t reserved[3];
+} tx_schedule;
+
+/* Per-epoch device timing (hipEvents on the library's stream). */
+typedef struct tx_epoch_stats {
+    double ms_total;        /* whole fit_epoch on the GPU (train + ycache rebuild) */
+    double ms_sgd;          /* SGD kernels */
+    double ms_rebuild;      /* ycache (+ ycache_day) rebuild */
+    double ms_other;        /* merges / queue resets */
+    int64_t n_ratings;      /* training ratings visited */
+    int32_t n_launches;
+    int32_t graph;          /* 1 when the epoch ran as a hipGraph */
+} tx_epoch_stats;
+
+/* Parameter ids for tx_get_param / tx_set_param, the tables of TxModel
+ * (src/tx.rs:210-272).  Shapes: U = users, I = items, f = n_feat,
+ * F = n_freq_bins, B = n_time_bins, D = (user, day) pairs. */
+enum tx_param {
+    TX_P_UBIAS = 0,      /* [U]          ubias        */
+    TX_P_IBIAS,          /* [I]          ibias        */
+    TX_P_UFEAT,          /* [U, f]       ufeat  (p)   */
+    TX_P_UFEAT2,         /* [U, f]       ufeat2 (p2)  */
+    TX_P_IFEAT,          /* [I, f]       ifeat  (q)   */
+    TX_P_YFEAT,          /* [I, f]       yfeat  (y)   */
+    TX_P_YCACHE,         /* [U, f]       ycache (s)   */
+    TX_P_YFEAT_DAY,      /* [I, f]       (0 elements when low_memory) */
+    TX_P_TU_MEAN,        /* [U]          */
+    TX_P_ALPHA_U,        /* [U]          */
+    TX_P_BUT_BIN,        /* [U, B]       */
+    TX_P_BIT_BIN,        /* [I, B]       */
+    TX_P_YBIAS,          /* [I]          */
+    TX_P_YCACHE_BIAS,    /* [U]          */
+    TX_P_UBIAS_DAY,      /* [D]          */
+    TX_P_UFEAT_DAY,      /* [D, f]       (0 elements when low_memory) */
+    TX_P_YCACHE_DAY,     /* [D, f]       (0 elements when low_memory) */
+    TX_P_IBIAS_FREQ,     /* [I, F]       */
+    TX_P_IFEAT_FREQ,     /* [I*F, f]     */
+    TX_P_CU,             /* [U]          */
+    TX_P_CUT,            /* [D]          */
+    TX_P_GBIAS,          /* [1]          */
+    TX_P_THRESHOLDS,     /* [4]          ordinal head thresholds */
+    TX_P_COUNT
+};
+
+/* TxModel::new (src/tx.rs:460-649): builds SparseUD, tu_mean, day_range,
+ * gbias, draws the initial factors from the reference's RNG stream
+ * (StdRng seeded with cfg->seed) and uploads everything to the GPU. */
+int tx_create(const tx_config* cfg, const tx_ratings* train, const tx_ratings* heldout, tx_ctx** out);
+
+/* Drop the model and its GPU memory. */
+int tx_destroy(tx_ctx* ctx);
+
+/* Choose the training schedule (default: TX_SCHED_STRATA, n_strata = 1). */
+int tx_set_schedule(tx_ctx* ctx, const tx_schedule* sched);
+
+/* Regressor::fit_epoch (src/lib.rs:630, src/tx.rs:734-1115); epoch is
+ * 1-based.  Includes the epoch-end ycache rebuild (src/tx.rs:1109-1110). */
+int tx_fit_epoch(tx_ctx* ctx, int32_t epoch);
+
+/* Timing of the last tx_fit_epoch. */
+int tx_last_epoch_stats(tx_ctx* ctx, tx_epoch_stats* out);
+
+/* Regressor::predict (src/tx.rs:713-724) for n (user, item, day) triples
+ * held in host memory. */
+int tx_predict(tx_ctx* ctx, int64_t n, const int32_t* users, const int32_t* items,
+               const int16_t* days, float* out);
+
+/* Same, with every buffer already in GPU memory; runs on `stream` (a
+ * hipStream_t, NULL = the library's stream) and returns without waiting. */
+int tx_predict_device(tx_ctx* ctx, int64_t n, const int32_t* d_users, const int32_t* d_items,
+                      const int16_t* d_days, float* d_out, void* stream);
+
+/* Regressor::predict_subscores (src/tx.rs:663-711): bias, mf, nsvd1. */
+int tx_predict_subscores(tx_ctx* ctx, int64_t n, const int32_t* users, const int32_t* items,
+                         const int16_t* days, float* bias, float* mf, float* nsvd1);
+
+/* calc_rmse (src/lib.rs:659-673): FP64 RMSE of predict against residuals
+ * over rows with is_test == 0. */
+int tx_calc_rmse(tx_ctx* ctx, const tx_ratings* ds, double* out);
+
+/* Element count of a parameter table. */
+int tx_param_len(tx_ctx* ctx, int32_t param_id, int64_t* n);
+
+/* Copy a parameter table to / from host memory (n must equal tx_param_len).
+ * Used for the .ifeat artifact (save_artifacts, src/tx.rs:726-732) and to
+ * load identical parameters into two implementations. */
+int tx_get_param(tx_ctx* ctx, int32_t param_id, float* out, int64_t n);
+int tx_set_param(tx_ctx* ctx, int32_t param_id, const float* in, int64_t n);
+
+/* get_users (src/lib.rs:486-494): the per-epoch user order. Host only. */
+int tx_get_users(int64_t n, int32_t shuffle, uint64_t seed, int64_t epoch, int64_t* out);
+
+/* rand_array2 (src/lib.rs:64-67): n draws of Normal(0, sigma) from
+ * StdRng::seed_from_u64(seed), row-major. Host only. */
+int tx_rand_normal(uint64_t seed, float sigma, int64_t n, float* out);
+
+/* Description of the last failure on this thread ("" if none). */
+const char* tx_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TSVD_TX_H */

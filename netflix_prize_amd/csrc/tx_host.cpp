@@ -1,0 +1,1055 @@
+// Host side of libtsvd_tx.so: the C ABI declared in include/tsvd_tx.h.
+//
+// tx_create restates TxModel::new (src/tx.rs:460-649): it derives every static
+// per-rating quantity once (SparseUD index, time bin, dev_u(t), run bounds and
+// norms), draws the initial factors from the reference's RNG stream, and
+// uploads one device-resident model.  tx_fit_epoch then runs the chosen GPU
+// schedule plus the epoch-end ycache rebuild; predictions, RMSE and parameter
+// access are batched device calls.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "tsvd_tx.h"
+#include "tx_launch.hpp"
+#include "tx_rng.hpp"
+#include "tx_types.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPC(expr)                                                                          \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess) return fail(TX_EDEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+// Run fn(lo, hi) over [0, n) on up to hardware_concurrency threads.
+template <typename Fn>
+void parallel_for(int64_t n, Fn fn) {
+    unsigned nt = std::thread::hardware_concurrency();
+    if (nt == 0) nt = 1;
+    if (nt > 16) nt = 16;
+    if (n < 100000 || nt == 1) {
+        fn(int64_t{0}, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    const int64_t chunk = (n + nt - 1) / nt;
+    for (unsigned k = 0; k < nt; ++k) {
+        const int64_t lo = k * chunk, hi = std::min<int64_t>(n, lo + chunk);
+        if (lo >= hi) break;
+        th.emplace_back([=] { fn(lo, hi); });
+    }
+    for (auto& t : th) t.join();
+}
+
+// freq_bin (src/tx.rs:87-90), evaluated with the host libm like the reference.
+int freq_bin_host(uint32_t cnt, int nfb) {
+    if (cnt <= 1) return 0;
+    const float v = std::floor(std::log2(static_cast<float>(cnt)));
+    const float cap = static_cast<float>(nfb - 1);
+    return static_cast<int>(v < cap ? v : cap);
+}
+
+// dev (src/tx.rs:292-300)
+float dev_host(float tu_mean, int day, float beta) {
+    const float dt = static_cast<float>(day) - tu_mean;
+    if (dt == 0.0f) return 0.0f;
+    const float s = dt > 0.0f ? 1.0f : -1.0f;
+    return s * std::pow(std::fabs(dt), beta);
+}
+
+struct Shape {
+    int team, vec;
+};
+
+bool pick_shape(int f, Shape* s) {
+    const int lanes = (std::max(f, 1) + 3) / 4;
+    if (lanes <= 4) *s = {4, 4};
+    else if (lanes <= 8) *s = {8, 4};
+    else if (lanes <= 16) *s = {16, 4};
+    else if (lanes <= 32) *s = {32, 4};
+    else if (lanes <= 64) *s = {64, 4};
+    else if (f <= 512) *s = {64, 8};
+    else if (f <= 1024) *s = {64, 16};
+    else return false;
+    return true;
+}
+
+}  // namespace
+
+struct tx_ctx {
+    tx_config cfg{};
+    int32_t U = 0, I = 0, f = 0, fs = 0, F = 0, B = 0;
+    int64_t N = 0, NH = 0, n_ud = 0;
+    Shape shape{4, 4};
+    hipStream_t st = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    tx::DevModel dm{};
+    std::vector<void*> allocs;
+
+    // host copies kept to build the stratified layout
+    std::vector<uint32_t> t_off, t_item, t_ud, h_start;
+    std::vector<float> t_dev, t_r;
+    std::vector<uint8_t> t_bin;
+    std::vector<int8_t> t_raw;
+
+    tx_schedule sched{TX_SCHED_STRATA, 1, 0, 0, 0, {0, 0, 0}};
+    int built_S = 0;
+    tx::StrataView sv{};
+    std::vector<void*> strata_allocs;
+    hipGraphExec_t graph = nullptr;
+    int graph_S = 0;
+
+    int64_t* d_order = nullptr;
+    float* d_scratch = nullptr;
+    double* d_acc = nullptr;
+    // staging for host-buffer predict / rmse calls
+    void* d_stage = nullptr;
+    size_t stage_bytes = 0;
+    tx_epoch_stats last{};
+
+    template <typename T>
+    int alloc(T** p, size_t n, std::vector<void*>* list = nullptr) {
+        void* q = nullptr;
+        const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+        hipError_t e = hipMalloc(&q, bytes);
+        if (e != hipSuccess) return fail(TX_ENOMEM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+        (list ? list : &allocs)->push_back(q);
+        *p = static_cast<T*>(q);
+        return 0;
+    }
+
+    template <typename T>
+    int upload(T** p, const T* src, size_t n, std::vector<void*>* list = nullptr) {
+        if (int rc = alloc(p, n, list)) return rc;
+        if (n) HIPC(hipMemcpy(*p, src, n * sizeof(T), hipMemcpyHostToDevice));
+        return 0;
+    }
+
+    int stage(size_t bytes) {
+        if (bytes <= stage_bytes) return 0;
+        if (d_stage) (void)hipFree(d_stage);
+        d_stage = nullptr;
+        stage_bytes = 0;
+        hipError_t e = hipMalloc(&d_stage, bytes);
+        if (e != hipSuccess) return fail(TX_ENOMEM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+        stage_bytes = bytes;
+        return 0;
+    }
+
+    void free_strata() {
+        for (void* p : strata_allocs) (void)hipFree(p);
+        strata_allocs.clear();
+        built_S = 0;
+        if (graph) (void)hipGraphExecDestroy(graph);
+        graph = nullptr;
+        graph_S = 0;
+    }
+
+    ~tx_ctx() {
+        free_strata();
+        if (d_stage) (void)hipFree(d_stage);
+        for (void* p : allocs) (void)hipFree(p);
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (st) (void)hipStreamDestroy(st);
+    }
+};
+
+namespace {
+
+int check_ratings(const tx_ratings* r, const char* what, bool need_targets) {
+    if (!r) return fail(TX_EINVAL, "%s: NULL rating set", what);
+    if (r->n_ratings < 0 || r->n_users < 0 || r->n_items < 0) return fail(TX_EINVAL, "%s: negative size", what);
+    if (r->n_ratings > 0 && (!r->user_idxs || !r->item_idxs || !r->dates))
+        return fail(TX_EINVAL, "%s: missing user/item/date arrays", what);
+    if (!r->user_cnts && r->n_users > 0) return fail(TX_EINVAL, "%s: missing user_cnts", what);
+    if (need_targets && r->n_ratings > 0 && !r->residuals) return fail(TX_EINVAL, "%s: missing residuals", what);
+    for (int64_t t = 0; t < r->n_ratings; ++t) {
+        if (r->user_idxs[t] < 0 || r->user_idxs[t] >= r->n_users)
+            return fail(TX_EINVAL, "%s: user index %d out of range at row %lld", what, r->user_idxs[t], (long long)t);
+        if (r->item_idxs[t] < 0 || r->item_idxs[t] >= r->n_items)
+            return fail(TX_EINVAL, "%s: item index %d out of range at row %lld", what, r->item_idxs[t], (long long)t);
+    }
+    return 0;
+}
+
+tx::Hyper make_hyper(const tx_config& c) {
+    tx::Hyper h{};
+    h.lr_u = c.lr_u; h.lr_ud = c.lr_ud; h.lr_u2 = c.lr_u2; h.lr_ub = c.lr_ub; h.lr_ubd = c.lr_ubd;
+    h.lr_i = c.lr_i; h.lr_ib = c.lr_ib; h.lr_y = c.lr_y; h.lr_yb = c.lr_yb; h.lr_yd = c.lr_yd;
+    h.lr_tu = c.lr_tu; h.lr_ti = c.lr_ti; h.lr_ta = c.lr_ta; h.lr_ibf = c.lr_ibf; h.lr_iqf = c.lr_iqf;
+    h.lr_cu = c.lr_cu;
+    h.reg_iqf = c.reg_iqf; h.reg_cu = c.reg_cu; h.reg_u = c.reg_u; h.reg_u2 = c.reg_u2; h.reg_ud = c.reg_ud;
+    h.reg_i = c.reg_i; h.reg_y = c.reg_y; h.reg_yd = c.reg_yd;
+    h.w_bias = c.w_bias; h.w_factor = c.w_factor;
+    h.beta = c.beta;
+    h.lr_t = c.lr_t; h.reg_t = c.reg_t;
+    return h;
+}
+
+// Upload a dense [rows, f] host table into a zero-padded [rows, fs] device table.
+int upload_padded(tx_ctx* c, float** dst, const float* src, int64_t rows) {
+    if (int rc = c->alloc(dst, static_cast<size_t>(rows) * c->fs)) return rc;
+    HIPC(hipMemset(*dst, 0, std::max<int64_t>(rows, 1) * c->fs * sizeof(float)));
+    if (rows > 0 && c->f > 0 && src)
+        HIPC(hipMemcpy2D(*dst, c->fs * sizeof(float), src, c->f * sizeof(float), c->f * sizeof(float), rows,
+                         hipMemcpyHostToDevice));
+    return 0;
+}
+
+int zeros(tx_ctx* c, float** dst, int64_t n) {
+    if (int rc = c->alloc(dst, static_cast<size_t>(n))) return rc;
+    HIPC(hipMemset(*dst, 0, std::max<int64_t>(n, 1) * sizeof(float)));
+    return 0;
+}
+
+struct ParamDesc {
+    float* p;
+    int64_t rows, cols;  // cols == f means a padded factor table
+    bool padded;
+};
+
+ParamDesc param_desc(tx_ctx* c, int id) {
+    const auto& m = c->dm;
+    const int64_t U = c->U, I = c->I, D = c->n_ud, f = c->f;
+    const bool lm = c->cfg.low_memory != 0;
+    switch (id) {
+    case TX_P_UBIAS: return {m.ubias, U, 1, false};
+    case TX_P_IBIAS: return {m.ibias, I, 1, false};
+    case TX_P_UFEAT: return {m.ufeat, U, f, true};
+    case TX_P_UFEAT2: return {m.ufeat2, U, f, true};
+    case TX_P_IFEAT: return {m.ifeat, I, f, true};
+    case TX_P_YFEAT: return {m.yfeat, I, f, true};
+    case TX_P_YCACHE: return {m.ycache, U, f, true};
+    case TX_P_YFEAT_DAY: return {m.yfeat_day, lm ? 0 : I, f, true};
+    case TX_P_TU_MEAN: return {m.tu_mean, U, 1, false};
+    case TX_P_ALPHA_U: return {m.alpha_u, U, 1, false};
+    case TX_P_BUT_BIN: return {m.but_bin, U * c->B, 1, false};
+    case TX_P_BIT_BIN: return {m.bit_bin, I * c->B, 1, false};
+    case TX_P_YBIAS: return {m.ybias, I, 1, false};
+    case TX_P_YCACHE_BIAS: return {m.ycache_bias, U, 1, false};
+    case TX_P_UBIAS_DAY: return {m.ubias_day, D, 1, false};
+    case TX_P_UFEAT_DAY: return {m.ufeat_day, lm ? 0 : D, f, true};
+    case TX_P_YCACHE_DAY: return {m.ycache_day, lm ? 0 : D, f, true};
+    case TX_P_IBIAS_FREQ: return {m.ibias_freq, I * c->F, 1, false};
+    case TX_P_IFEAT_FREQ: return {m.ifeat_freq, I * c->F, f, true};
+    case TX_P_CU: return {m.cu, U, 1, false};
+    case TX_P_CUT: return {m.cut, D, 1, false};
+    case TX_P_GBIAS: return {nullptr, 1, 1, false};
+    case TX_P_THRESHOLDS: return {m.thresholds, 4, 1, false};
+    default: return {nullptr, -1, 0, false};
+    }
+}
+
+// Build the stratified layout: item blocks and user blocks balanced by rating
+// count (largest first to the lightest block), segments of each user's
+// ratings per item block in the rotation order of its block, and per
+// (stratum, cell) user queues sorted by segment length (longest first).
+int build_strata(tx_ctx* c, int S) {
+    c->free_strata();
+    const int64_t U = c->U, I = c->I, N = c->N;
+    const int C = S;  // one cell per user block
+    std::vector<int32_t> iblock(I, 0), ublock(U, 0);
+    if (S > 1) {
+        std::vector<int64_t> icnt(I, 0);
+        for (int64_t t = 0; t < N; ++t) icnt[c->t_item[t]]++;
+        std::vector<int32_t> ord(I);
+        std::iota(ord.begin(), ord.end(), 0);
+        std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t b) { return icnt[a] > icnt[b]; });
+        std::vector<int64_t> load(S, 0);
+        for (int32_t i : ord) {
+            const int k = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+            iblock[i] = k;
+            load[k] += icnt[i];
+        }
+        std::vector<int32_t> uord(U);
+        std::iota(uord.begin(), uord.end(), 0);
+        std::stable_sort(uord.begin(), uord.end(), [&](int32_t a, int32_t b) {
+            return (c->t_off[a + 1] - c->t_off[a]) > (c->t_off[b + 1] - c->t_off[b]);
+        });
+        std::fill(load.begin(), load.end(), 0);
+        for (int32_t u : uord) {
+            const int k = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+            ublock[u] = k;
+            load[k] += c->t_off[u + 1] - c->t_off[u];
+        }
+    }
+    // segments
+    std::vector<uint32_t> seg(static_cast<size_t>(U) * (S + 1));
+    std::vector<uint32_t> perm(N);
+    parallel_for(U, [&](int64_t lo, int64_t hi) {
+        std::vector<uint32_t> cnt(S);
+        for (int64_t u = lo; u < hi; ++u) {
+            const uint32_t a = c->t_off[u], b = c->t_off[u + 1];
+            std::fill(cnt.begin(), cnt.end(), 0u);
+            for (uint32_t t = a; t < b; ++t) {
+                const int s = S > 1 ? (iblock[c->t_item[t]] - ublock[u] + S) % S : 0;
+                cnt[s]++;
+            }
+            uint32_t* so = &seg[static_cast<size_t>(u) * (S + 1)];
+            so[0] = a;
+            for (int s = 0; s < S; ++s) so[s + 1] = so[s] + cnt[s];
+            std::vector<uint32_t> pos(so, so + S);
+            for (uint32_t t = a; t < b; ++t) {
+                const int s = S > 1 ? (iblock[c->t_item[t]] - ublock[u] + S) % S : 0;
+                perm[pos[s]++] = t;
+            }
+        }
+    });
+    // queues
+    std::vector<uint32_t> qstart(static_cast<size_t>(S) * C + 1, 0);
+    std::vector<std::vector<uint32_t>> qs(static_cast<size_t>(S) * C);
+    for (int64_t u = 0; u < U; ++u) {
+        const uint32_t* so = &seg[static_cast<size_t>(u) * (S + 1)];
+        if (so[S] == so[0]) continue;
+        for (int s = 0; s < S; ++s) {
+            // the first and last strata visit every user (NSVD1 gather and the y step)
+            if (so[s + 1] > so[s] || s == 0 || s == S - 1) qs[static_cast<size_t>(s) * C + ublock[u]].push_back(static_cast<uint32_t>(u));
+        }
+    }
+    std::vector<uint32_t> qusers;
+    for (size_t k = 0; k < qs.size(); ++k) {
+        const int s = static_cast<int>(k / C);
+        auto& q = qs[k];
+        std::stable_sort(q.begin(), q.end(), [&](uint32_t a, uint32_t b) {
+            const uint32_t* sa = &seg[static_cast<size_t>(a) * (S + 1)];
+            const uint32_t* sb = &seg[static_cast<size_t>(b) * (S + 1)];
+            // the full-user work of the first/last stratum dominates there
+            const uint32_t la = (s == 0 || s == S - 1) ? sa[S] - sa[0] : sa[s + 1] - sa[s];
+            const uint32_t lb = (s == 0 || s == S - 1) ? sb[S] - sb[0] : sb[s + 1] - sb[s];
+            return la > lb;
+        });
+        qstart[k + 1] = qstart[k] + static_cast<uint32_t>(q.size());
+        qusers.insert(qusers.end(), q.begin(), q.end());
+    }
+    auto* L = &c->strata_allocs;
+    tx::StrataView& sv = c->sv;
+    sv = tx::StrataView{};
+    sv.n_strata = S;
+    sv.n_cells = C;
+    uint32_t *d_seg, *d_qu, *d_qs, *d_qh;
+    if (int rc = c->upload(&d_seg, seg.data(), seg.size(), L)) return rc;
+    if (int rc = c->upload(&d_qu, qusers.data(), qusers.size(), L)) return rc;
+    if (int rc = c->upload(&d_qs, qstart.data(), qstart.size(), L)) return rc;
+    if (int rc = c->alloc(&d_qh, qstart.size(), L)) return rc;
+    sv.seg_off = d_seg;
+    sv.q_users = d_qu;
+    sv.q_start = d_qs;
+    sv.q_head = d_qh;
+    if (S == 1) {
+        sv.p_item = c->dm.t_item;
+        sv.p_ud = c->dm.t_ud;
+        sv.p_dev = c->dm.t_dev;
+        sv.p_r = c->dm.t_r;
+        sv.p_bin = c->dm.t_bin;
+        sv.p_raw = c->dm.t_raw;
+        sv.gacc = nullptr;
+        sv.gacc_b = nullptr;
+    } else {
+        std::vector<uint32_t> a(N), b(N);
+        std::vector<float> d(N), r(N);
+        std::vector<uint8_t> bn(N);
+        std::vector<int8_t> rw(N);
+        parallel_for(N, [&](int64_t lo, int64_t hi) {
+            for (int64_t k = lo; k < hi; ++k) {
+                const uint32_t t = perm[k];
+                a[k] = c->t_item[t]; b[k] = c->t_ud[t]; d[k] = c->t_dev[t]; r[k] = c->t_r[t];
+                bn[k] = c->t_bin[t]; rw[k] = c->t_raw.empty() ? 0 : c->t_raw[t];
+            }
+        });
+        uint32_t *pa, *pb;
+        float *pd, *pr, *ga, *gb;
+        uint8_t* pbn;
+        int8_t* prw;
+        if (int rc = c->upload(&pa, a.data(), a.size(), L)) return rc;
+        if (int rc = c->upload(&pb, b.data(), b.size(), L)) return rc;
+        if (int rc = c->upload(&pd, d.data(), d.size(), L)) return rc;
+        if (int rc = c->upload(&pr, r.data(), r.size(), L)) return rc;
+        if (int rc = c->upload(&pbn, bn.data(), bn.size(), L)) return rc;
+        if (int rc = c->upload(&prw, rw.data(), rw.size(), L)) return rc;
+        if (int rc = c->alloc(&ga, static_cast<size_t>(U) * c->fs, L)) return rc;
+        if (int rc = c->alloc(&gb, static_cast<size_t>(U), L)) return rc;
+        sv.p_item = pa; sv.p_ud = pb; sv.p_dev = pd; sv.p_r = pr; sv.p_bin = pbn; sv.p_raw = prw;
+        sv.gacc = ga;
+        sv.gacc_b = gb;
+    }
+    c->built_S = S;
+    return 0;
+}
+
+int default_blocks(tx_ctx* c) {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        cus = 256;
+    // a few resident 256-thread workgroups per CU; a multiple of 8 keeps the
+    // cell -> XCD mapping of the round-robin dispatch
+    int blocks = cus * 4;
+    return (blocks + 7) / 8 * 8;
+}
+
+// Enqueue one parallel epoch (queue reset, S strata, ycache rebuild).
+int enqueue_strata_epoch(tx_ctx* c, hipStream_t st) {
+    const int S = c->built_S;
+    const bool qf = c->cfg.lr_iqf != 0.0f || c->cfg.sigma_iqf != 0.0f;
+    const bool p2 = c->cfg.lr_u2 != 0.0f;
+    const bool ord = c->cfg.has_ordinal != 0;
+    const int blocks = c->sched.blocks > 0 ? c->sched.blocks : default_blocks(c);
+    HIPC(tx::launch_fill_u32(c->sv.q_head, 0u, static_cast<int64_t>(S) * c->sv.n_cells, st));
+    for (int s = 0; s < S; ++s)
+        HIPC(tx::launch_sgd_par(c->dm, c->sv, s, c->shape.team, c->shape.vec, blocks, qf, p2, ord, st));
+    HIPC(tx::launch_rebuild(c->dm, c->n_ud, c->shape.team, c->shape.vec, st));
+    return 0;
+}
+
+int enforce_thresholds(tx_ctx* c) {
+    if (!c->cfg.has_ordinal) return 0;
+    float th[4];
+    HIPC(hipMemcpyAsync(th, c->dm.thresholds, sizeof th, hipMemcpyDeviceToHost, c->st));
+    HIPC(hipStreamSynchronize(c->st));
+    // OrdinalHead::enforce_sorted_with_gap (src/lib.rs:519-525)
+    std::sort(th, th + 4);
+    for (int k = 1; k < 4; ++k) {
+        const float min_ok = th[k - 1] + c->cfg.th_gap;
+        if (th[k] < min_ok) th[k] = min_ok;
+    }
+    HIPC(hipMemcpyAsync(c->dm.thresholds, th, sizeof th, hipMemcpyHostToDevice, c->st));
+    HIPC(hipStreamSynchronize(c->st));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* tx_last_error(void) { return g_err.c_str(); }
+
+int tx_get_users(int64_t n, int32_t shuffle, uint64_t seed, int64_t epoch, int64_t* out) {
+    g_err.clear();
+    if (n < 0 || (n > 0 && !out)) return fail(TX_EINVAL, "tx_get_users: bad arguments");
+    tx::user_order(n, shuffle != 0, seed, epoch, out);
+    return 0;
+}
+
+int tx_rand_normal(uint64_t seed, float sigma, int64_t n, float* out) {
+    g_err.clear();
+    if (n < 0 || (n > 0 && !out) || !(sigma >= 0.0f) || !std::isfinite(sigma))
+        return fail(TX_EINVAL, "tx_rand_normal: bad arguments");
+    tx::ChaChaRng rng(seed);
+    tx::fill_normal(rng, out, n, sigma);
+    return 0;
+}
+
+int tx_create(const tx_config* cfg, const tx_ratings* tr, const tx_ratings* pr, tx_ctx** out) {
+    g_err.clear();
+    if (!cfg || !out) return fail(TX_EINVAL, "tx_create: NULL argument");
+    *out = nullptr;
+    if (cfg->lr_w != 0.0f || cfg->lr_c != 0.0f || cfg->lr_w_day != 0.0f || cfg->lr_c_day != 0.0f)
+        return fail(TX_EUNSUPPORTED, "tx_create: neighbourhood terms (tsvdx5/6) are not part of the tsvdx4 path");
+    if (cfg->n_feat < 0 || cfg->n_time_bins < 1 || cfg->n_freq_bins < 1 || cfg->n_epochs < 0)
+        return fail(TX_EINVAL, "tx_create: n_feat >= 0, n_time_bins >= 1, n_freq_bins >= 1 required");
+    if (int rc = check_ratings(tr, "train", true)) return rc;
+    if (int rc = check_ratings(pr, "heldout", false)) return rc;
+    if (pr->n_users != tr->n_users || pr->n_items != tr->n_items)
+        return fail(TX_EINVAL, "tx_create: train and held-out sets disagree on n_users/n_items");
+    if (cfg->has_ordinal && tr->n_ratings > 0 && !tr->raw_ratings)
+        return fail(TX_EINVAL, "tx_create: the ordinal head needs train raw_ratings");
+    if (tr->n_ratings + pr->n_ratings >= (int64_t{1} << 32))
+        return fail(TX_EUNSUPPORTED, "tx_create: more than 2^32 ratings");
+    Shape shape;
+    if (!pick_shape(cfg->n_feat, &shape)) return fail(TX_EUNSUPPORTED, "tx_create: n_feat > 1024");
+
+    auto ctx = std::make_unique<tx_ctx>();
+    tx_ctx* c = ctx.get();
+    c->cfg = *cfg;
+    c->U = tr->n_users;
+    c->I = tr->n_items;
+    c->f = cfg->n_feat;
+    c->shape = shape;
+    c->fs = shape.team * shape.vec;
+    c->F = cfg->n_freq_bins;
+    c->B = cfg->n_time_bins;
+    c->N = tr->n_ratings;
+    c->NH = pr->n_ratings;
+    const int64_t U = c->U, I = c->I, N = c->N, NH = c->NH;
+    const int f = c->f, F = c->F, B = c->B;
+
+    // calc_user_offsets (src/lib.rs:469-483): user-sorted input, spans = counts
+    for (int64_t t = 1; t < N; ++t)
+        if (tr->user_idxs[t] < tr->user_idxs[t - 1])
+            return fail(TX_EINVAL, "dataset 'train' is not user-sorted at index %lld: user %d follows user %d",
+                        (long long)t, tr->user_idxs[t], tr->user_idxs[t - 1]);
+    c->t_off.assign(U + 1, 0);
+    {
+        std::vector<int64_t> seen(U, 0);
+        for (int64_t t = 0; t < N; ++t) seen[tr->user_idxs[t]]++;
+        for (int64_t u = 0; u < U; ++u) {
+            if (seen[u] != tr->user_cnts[u])
+                return fail(TX_EINVAL, "train user_cnts[%lld] = %d but the set holds %lld ratings of that user",
+                            (long long)u, tr->user_cnts[u], (long long)seen[u]);
+            c->t_off[u + 1] = c->t_off[u] + static_cast<uint32_t>(seen[u]);
+        }
+    }
+
+    // tu_mean and day_range (src/tx.rs:465-475): f32 sums in file order
+    std::vector<float> tu_mean(U, 0.0f);
+    int32_t day_range = 0;
+    for (int64_t t = 0; t < N; ++t) {
+        tu_mean[tr->user_idxs[t]] += static_cast<float>(tr->dates[t]);
+        day_range = std::max<int32_t>(day_range, static_cast<int32_t>(tr->dates[t]) + 1);
+    }
+    for (int64_t u = 0; u < U; ++u)
+        if (tr->user_cnts[u] > 0) tu_mean[u] /= static_cast<float>(tr->user_cnts[u]);
+    if (day_range <= 0) day_range = 1;
+
+    // held-out rows grouped by user, in held-out file order
+    c->h_start.assign(U + 1, 0);
+    for (int64_t t = 0; t < NH; ++t) c->h_start[pr->user_idxs[t] + 1]++;
+    for (int64_t u = 0; u < U; ++u) c->h_start[u + 1] += c->h_start[u];
+    std::vector<uint32_t> h_item(NH);
+    {
+        std::vector<uint32_t> pos(c->h_start.begin(), c->h_start.end() - 1);
+        for (int64_t t = 0; t < NH; ++t) h_item[pos[pr->user_idxs[t]]++] = static_cast<uint32_t>(pr->item_idxs[t]);
+    }
+
+    // SparseUD (src/tx.rs:31-69): per user the sorted distinct days of train
+    // and held-out ratings, with their counts
+    std::vector<uint32_t> sud_start(U + 1, 0);
+    std::vector<int16_t> sud_date;
+    std::vector<uint32_t> sud_cnt;
+    {
+        std::vector<uint32_t> dstart(U + 1, 0);
+        for (int64_t u = 0; u < U; ++u) dstart[u + 1] = dstart[u] + (c->t_off[u + 1] - c->t_off[u]) + (c->h_start[u + 1] - c->h_start[u]);
+        std::vector<int16_t> all(dstart[U]);
+        std::vector<uint32_t> pos(dstart.begin(), dstart.end() - 1);
+        for (int64_t t = 0; t < N; ++t) all[pos[tr->user_idxs[t]]++] = tr->dates[t];
+        for (int64_t t = 0; t < NH; ++t) all[pos[pr->user_idxs[t]]++] = pr->dates[t];
+        std::vector<uint32_t> ndistinct(U, 0);
+        parallel_for(U, [&](int64_t lo, int64_t hi) {
+            for (int64_t u = lo; u < hi; ++u) {
+                std::sort(all.begin() + dstart[u], all.begin() + dstart[u + 1]);
+                uint32_t n = 0;
+                for (uint32_t k = dstart[u]; k < dstart[u + 1]; ++k)
+                    if (k == dstart[u] || all[k] != all[k - 1]) ++n;
+                ndistinct[u] = n;
+            }
+        });
+        for (int64_t u = 0; u < U; ++u) sud_start[u + 1] = sud_start[u] + ndistinct[u];
+        c->n_ud = sud_start[U];
+        sud_date.resize(c->n_ud);
+        sud_cnt.resize(c->n_ud);
+        parallel_for(U, [&](int64_t lo, int64_t hi) {
+            for (int64_t u = lo; u < hi; ++u) {
+                uint32_t w = sud_start[u];
+                for (uint32_t k = dstart[u]; k < dstart[u + 1]; ++k) {
+                    if (k == dstart[u] || all[k] != all[k - 1]) {
+                        sud_date[w] = all[k];
+                        sud_cnt[w] = 1;
+                        ++w;
+                    } else {
+                        sud_cnt[w - 1]++;
+                    }
+                }
+            }
+        });
+    }
+    const int64_t D = c->n_ud;
+    auto sud_index = [&](int64_t u, int16_t day) -> int64_t {
+        auto b = sud_date.begin() + sud_start[u], e = sud_date.begin() + sud_start[u + 1];
+        auto it = std::lower_bound(b, e, day);
+        return (it != e && *it == day) ? (it - sud_date.begin()) : -1;
+    };
+
+    // per training rating: item, SparseUD index, time bin, dev, target
+    c->t_item.resize(N);
+    c->t_ud.resize(N);
+    c->t_dev.resize(N);
+    c->t_r.resize(N);
+    c->t_bin.resize(N);
+    if (cfg->has_ordinal) c->t_raw.resize(N);
+    if (B > 256) return fail(TX_EUNSUPPORTED, "tx_create: n_time_bins > 256");
+    parallel_for(N, [&](int64_t lo, int64_t hi) {
+        for (int64_t t = lo; t < hi; ++t) {
+            const int32_t u = tr->user_idxs[t];
+            const int day = tr->dates[t];
+            c->t_item[t] = static_cast<uint32_t>(tr->item_idxs[t]);
+            c->t_ud[t] = static_cast<uint32_t>(sud_index(u, tr->dates[t]));
+            c->t_dev[t] = dev_host(tu_mean[u], day, cfg->beta);
+            c->t_r[t] = tr->residuals[t];
+            const int64_t b = (static_cast<int64_t>(day) * B) / day_range;
+            c->t_bin[t] = static_cast<uint8_t>(std::min<int64_t>(b, B - 1));
+            if (cfg->has_ordinal) c->t_raw[t] = tr->raw_ratings[t];
+        }
+    });
+
+    std::vector<uint8_t> sud_fbin(D);
+    for (int64_t d = 0; d < D; ++d) sud_fbin[d] = static_cast<uint8_t>(freq_bin_host(sud_cnt[d], F));
+
+    // norms: training su (src/tx.rs:808-819) and epoch-end rebuild (src/tx.rs:410-416)
+    std::vector<float> norm_train(U), norm_rebuild(U);
+    for (int64_t u = 0; u < U; ++u) {
+        const int64_t cnt = c->t_off[u + 1] - c->t_off[u];
+        const int64_t tot = cnt + (cfg->full_su ? (c->h_start[u + 1] - c->h_start[u]) : 0);
+        norm_train[u] = std::sqrt(static_cast<float>(tot));
+        const int64_t rc = static_cast<int64_t>(tr->user_cnts[u]) + pr->user_cnts[u];
+        norm_rebuild[u] = rc > 0 ? std::sqrt(static_cast<float>(rc)) : 0.0f;
+    }
+
+    // gbias (src/lib.rs:463-466): f64 mean of the residuals, sequential
+    double rsum = 0.0;
+    for (int64_t t = 0; t < N; ++t) rsum += static_cast<double>(tr->residuals[t]);
+    const float gbias = N > 0 ? static_cast<float>(rsum / static_cast<double>(N)) : 0.0f;
+
+    // day-term structures (non-low-memory): held-out / train items per
+    // (user, day), same-day runs with their norms
+    const bool low_mem = cfg->low_memory != 0;
+    std::vector<uint32_t> hd_start, hd_item, td_start, td_item, run_stop;
+    std::vector<float> run_norm, sud_norm;
+    if (!low_mem) {
+        hd_start.assign(D + 1, 0);
+        td_start.assign(D + 1, 0);
+        std::vector<int64_t> h_ud(NH);
+        for (int64_t t = 0; t < NH; ++t) {
+            h_ud[t] = sud_index(pr->user_idxs[t], pr->dates[t]);
+            hd_start[h_ud[t] + 1]++;
+        }
+        for (int64_t t = 0; t < N; ++t) td_start[c->t_ud[t] + 1]++;
+        for (int64_t d = 0; d < D; ++d) {
+            hd_start[d + 1] += hd_start[d];
+            td_start[d + 1] += td_start[d];
+        }
+        hd_item.resize(NH);
+        td_item.resize(N);
+        {
+            std::vector<uint32_t> ph(hd_start.begin(), hd_start.end() - 1), pt(td_start.begin(), td_start.end() - 1);
+            for (int64_t t = 0; t < NH; ++t) hd_item[ph[h_ud[t]]++] = static_cast<uint32_t>(pr->item_idxs[t]);
+            for (int64_t t = 0; t < N; ++t) td_item[pt[c->t_ud[t]]++] = c->t_item[t];
+        }
+        sud_norm.resize(D);
+        for (int64_t d = 0; d < D; ++d) {
+            float cnt = 0.0f;
+            for (uint32_t k = hd_start[d]; k < hd_start[d + 1]; ++k) cnt += 1.0f;
+            for (uint32_t k = td_start[d]; k < td_start[d + 1]; ++k) cnt += 1.0f;
+            sud_norm[d] = cnt > 0.0f ? std::sqrt(cnt) : 0.0f;
+        }
+        // day_ranges (src/tx.rs:93-105) within each user's span
+        run_stop.resize(N);
+        run_norm.resize(N);
+        for (int64_t u = 0; u < U; ++u) {
+            const uint32_t a = c->t_off[u], b = c->t_off[u + 1];
+            uint32_t rs = a;
+            for (uint32_t t = a; t < b; ++t) {
+                if (t + 1 == b || tr->dates[t + 1] != tr->dates[t]) {
+                    float day_cnt = static_cast<float>(t - rs + 1);
+                    if (cfg->full_su) {
+                        const uint32_t d = c->t_ud[rs];
+                        for (uint32_t k = hd_start[d]; k < hd_start[d + 1]; ++k) day_cnt += 1.0f;
+                    }
+                    const float nd = std::sqrt(day_cnt);
+                    for (uint32_t k = rs; k <= t; ++k) {
+                        run_stop[k] = t;
+                        run_norm[k] = nd;
+                    }
+                    rs = t + 1;
+                }
+            }
+        }
+    }
+
+    // initial factors in the reference's draw order (src/tx.rs:606-630):
+    // ufeat, ifeat, yfeat, [yfeat_day], ifeat_freq
+    tx::ChaChaRng rng(cfg->seed);
+    std::vector<float> h_ufeat(static_cast<size_t>(U) * f), h_ifeat(static_cast<size_t>(I) * f),
+        h_yfeat(static_cast<size_t>(I) * f), h_yday, h_iqf(static_cast<size_t>(I) * F * f);
+    tx::fill_normal(rng, h_ufeat.data(), static_cast<int64_t>(h_ufeat.size()), cfg->sigma_u);
+    tx::fill_normal(rng, h_ifeat.data(), static_cast<int64_t>(h_ifeat.size()), cfg->sigma_i);
+    tx::fill_normal(rng, h_yfeat.data(), static_cast<int64_t>(h_yfeat.size()), cfg->sigma_y);
+    if (!low_mem) {
+        h_yday.resize(static_cast<size_t>(I) * f);
+        tx::fill_normal(rng, h_yday.data(), static_cast<int64_t>(h_yday.size()), cfg->sigma_yd);
+    }
+    tx::fill_normal(rng, h_iqf.data(), static_cast<int64_t>(h_iqf.size()), cfg->sigma_iqf);
+
+    // ---- device ----
+    HIPC(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+    for (auto& e : c->ev) HIPC(hipEventCreate(&e));
+    tx::DevModel& m = c->dm;
+    m.U = c->U; m.I = c->I; m.f = f; m.fs = c->fs; m.F = F; m.B = B;
+    m.day_range = day_range;
+    m.low_mem = low_mem ? 1 : 0;
+    m.full_su = cfg->full_su ? 1 : 0;
+    m.has_ord = cfg->has_ordinal ? 1 : 0;
+    m.sum_err_bug = cfg->sum_err_bug ? 1 : 0;
+    m.gbias = gbias;
+    m.h = make_hyper(*cfg);
+#define RC(x) \
+    if (int rc_ = (x)) return rc_
+    RC(zeros(c, &m.ubias, U));
+    RC(zeros(c, &m.ibias, I));
+    RC(upload_padded(c, &m.ufeat, h_ufeat.data(), U));
+    RC(upload_padded(c, &m.ufeat2, nullptr, U));
+    RC(upload_padded(c, &m.ifeat, h_ifeat.data(), I));
+    RC(upload_padded(c, &m.yfeat, h_yfeat.data(), I));
+    RC(upload_padded(c, &m.ycache, nullptr, U));
+    RC(upload_padded(c, &m.yfeat_day, low_mem ? nullptr : h_yday.data(), low_mem ? 0 : I));
+    RC(c->upload(&m.tu_mean, tu_mean.data(), U));
+    RC(zeros(c, &m.alpha_u, U));
+    RC(zeros(c, &m.but_bin, U * B));
+    RC(zeros(c, &m.bit_bin, I * B));
+    RC(zeros(c, &m.ybias, I));
+    RC(zeros(c, &m.ycache_bias, U));
+    RC(zeros(c, &m.ubias_day, D));
+    RC(upload_padded(c, &m.ufeat_day, nullptr, low_mem ? 0 : D));
+    RC(upload_padded(c, &m.ycache_day, nullptr, low_mem ? 0 : D));
+    RC(zeros(c, &m.ibias_freq, I * F));
+    RC(upload_padded(c, &m.ifeat_freq, h_iqf.data(), I * F));
+    {
+        std::vector<float> ones(U, 1.0f);
+        RC(c->upload(&m.cu, ones.data(), U));
+    }
+    RC(zeros(c, &m.cut, D));
+    RC(c->upload(&m.thresholds, cfg->th_init, 4));
+
+    uint32_t* p32;
+    RC(c->upload(&p32, sud_start.data(), sud_start.size())); m.sud_start = p32;
+    int16_t* p16;
+    RC(c->upload(&p16, sud_date.data(), sud_date.size())); m.sud_date = p16;
+    uint8_t* p8;
+    RC(c->upload(&p8, sud_fbin.data(), sud_fbin.size())); m.sud_fbin = p8;
+    float* pf;
+    RC(c->upload(&pf, sud_norm.data(), sud_norm.size())); m.sud_norm = pf;
+    RC(c->upload(&p32, c->t_off.data(), c->t_off.size())); m.t_off = p32;
+    RC(c->upload(&p32, c->t_item.data(), c->t_item.size())); m.t_item = p32;
+    RC(c->upload(&p32, c->t_ud.data(), c->t_ud.size())); m.t_ud = p32;
+    RC(c->upload(&pf, c->t_dev.data(), c->t_dev.size())); m.t_dev = pf;
+    RC(c->upload(&pf, c->t_r.data(), c->t_r.size())); m.t_r = pf;
+    RC(c->upload(&p8, c->t_bin.data(), c->t_bin.size())); m.t_bin = p8;
+    int8_t* pi8;
+    RC(c->upload(&pi8, c->t_raw.data(), c->t_raw.size())); m.t_raw = pi8;
+    RC(c->upload(&p32, run_stop.data(), run_stop.size())); m.t_run_stop = p32;
+    RC(c->upload(&pf, run_norm.data(), run_norm.size())); m.t_run_norm = pf;
+    RC(c->upload(&p32, c->h_start.data(), c->h_start.size())); m.h_start = p32;
+    RC(c->upload(&p32, h_item.data(), h_item.size())); m.h_item = p32;
+    RC(c->upload(&p32, hd_start.data(), hd_start.size())); m.hd_start = p32;
+    RC(c->upload(&p32, hd_item.data(), hd_item.size())); m.hd_item = p32;
+    RC(c->upload(&p32, td_start.data(), td_start.size())); m.td_start = p32;
+    RC(c->upload(&p32, td_item.data(), td_item.size())); m.td_item = p32;
+    RC(c->upload(&pf, norm_train.data(), norm_train.size())); m.norm_train = pf;
+    RC(c->upload(&pf, norm_rebuild.data(), norm_rebuild.size())); m.norm_rebuild = pf;
+    RC(c->alloc(&c->d_order, static_cast<size_t>(U)));
+    RC(c->alloc(&c->d_scratch, static_cast<size_t>(4) * c->fs + 4));
+    RC(c->alloc(&c->d_acc, 2));
+#undef RC
+    HIPC(hipDeviceSynchronize());
+    *out = ctx.release();
+    return 0;
+}
+
+int tx_destroy(tx_ctx* ctx) {
+    g_err.clear();
+    delete ctx;
+    return 0;
+}
+
+int tx_set_schedule(tx_ctx* c, const tx_schedule* s) {
+    g_err.clear();
+    if (!c || !s) return fail(TX_EINVAL, "tx_set_schedule: NULL argument");
+    if (s->mode != TX_SCHED_SERIAL && s->mode != TX_SCHED_STRATA) return fail(TX_EINVAL, "unknown schedule mode %d", s->mode);
+    if (s->mode == TX_SCHED_STRATA) {
+        if (s->n_strata < 1 || (s->n_strata > 1 && s->n_strata % 8 != 0) || s->n_strata > 64)
+            return fail(TX_EINVAL, "n_strata must be 1 or a multiple of 8 (<= 64), got %d", s->n_strata);
+        if (!c->cfg.low_memory)
+            return fail(TX_EUNSUPPORTED, "the parallel schedule trains low_memory models; use TX_SCHED_SERIAL for day terms");
+        if (c->cfg.has_ordinal && c->cfg.lr_t != 0.0f)
+            return fail(TX_EUNSUPPORTED, "the parallel schedule keeps ordinal thresholds fixed (lr_t must be 0)");
+        if (s->blocks < 0 || (s->blocks > 0 && s->blocks < s->n_strata))
+            return fail(TX_EINVAL, "blocks must be 0 (auto) or at least n_strata");
+    }
+    const bool relayout = s->mode == TX_SCHED_STRATA && s->n_strata != c->built_S;
+    c->sched = *s;
+    if (relayout) {
+        if (int rc = build_strata(c, s->n_strata)) return rc;
+    }
+    if (c->graph) {
+        (void)hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+        c->graph_S = 0;
+    }
+    return 0;
+}
+
+int tx_fit_epoch(tx_ctx* c, int32_t epoch) {
+    g_err.clear();
+    if (!c) return fail(TX_EINVAL, "tx_fit_epoch: NULL ctx");
+    if (epoch < 1) return fail(TX_EINVAL, "tx_fit_epoch: epochs are 1-based");
+    hipStream_t st = c->st;
+    tx_epoch_stats stats{};
+    stats.n_ratings = c->N;
+    HIPC(hipEventRecord(c->ev[0], st));
+    if (epoch == c->cfg.reset_u_epoch) {
+        HIPC(tx::launch_fill_f32(c->dm.ufeat, 0.0f, static_cast<int64_t>(c->U) * c->fs, st));
+        HIPC(tx::launch_fill_f32(c->dm.ufeat2, 0.0f, static_cast<int64_t>(c->U) * c->fs, st));
+    }
+    if (c->sched.mode == TX_SCHED_SERIAL) {
+        std::vector<int64_t> order(c->U);
+        tx::user_order(c->U, c->cfg.shuffle_users != 0, c->cfg.seed, epoch, order.data());
+        HIPC(hipMemcpyAsync(c->d_order, order.data(), sizeof(int64_t) * c->U, hipMemcpyHostToDevice, st));
+        HIPC(hipEventRecord(c->ev[1], st));
+        HIPC(tx::launch_sgd_serial(c->dm, c->d_order, c->d_scratch, st));
+        HIPC(hipEventRecord(c->ev[2], st));
+        HIPC(tx::launch_rebuild(c->dm, c->n_ud, c->shape.team, c->shape.vec, st));
+        HIPC(hipEventRecord(c->ev[3], st));
+        HIPC(hipEventSynchronize(c->ev[3]));
+        float a = 0, b = 0, t = 0;
+        HIPC(hipEventElapsedTime(&a, c->ev[1], c->ev[2]));
+        HIPC(hipEventElapsedTime(&b, c->ev[2], c->ev[3]));
+        HIPC(hipEventElapsedTime(&t, c->ev[0], c->ev[3]));
+        stats.ms_sgd = a;
+        stats.ms_rebuild = b;
+        stats.ms_total = t;
+        stats.ms_other = t - a - b;
+        stats.n_launches = 2;
+        if (int rc = enforce_thresholds(c)) return rc;
+    } else {
+        if (c->built_S == 0)
+            if (int rc = build_strata(c, c->sched.n_strata > 0 ? c->sched.n_strata : 1)) return rc;
+        if (c->sched.use_graph) {
+            if (!c->graph || c->graph_S != c->built_S) {
+                if (c->graph) (void)hipGraphExecDestroy(c->graph);
+                c->graph = nullptr;
+                hipGraph_t g;
+                HIPC(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                int rc = enqueue_strata_epoch(c, st);
+                hipError_t e = hipStreamEndCapture(st, &g);
+                if (rc) return rc;
+                if (e != hipSuccess) return fail(TX_EDEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+                e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                if (e != hipSuccess) return fail(TX_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
+                c->graph_S = c->built_S;
+            }
+            HIPC(hipEventRecord(c->ev[1], st));
+            HIPC(hipGraphLaunch(c->graph, st));
+            HIPC(hipEventRecord(c->ev[3], st));
+            HIPC(hipEventSynchronize(c->ev[3]));
+            float a = 0, t = 0;
+            HIPC(hipEventElapsedTime(&a, c->ev[1], c->ev[3]));
+            HIPC(hipEventElapsedTime(&t, c->ev[0], c->ev[3]));
+            stats.ms_sgd = a;
+            stats.ms_total = t;
+            stats.graph = 1;
+            stats.n_launches = c->built_S + 2;
+        } else {
+            const int S = c->built_S;
+            const bool qf = c->cfg.lr_iqf != 0.0f || c->cfg.sigma_iqf != 0.0f;
+            const bool p2 = c->cfg.lr_u2 != 0.0f;
+            const bool ord = c->cfg.has_ordinal != 0;
+            const int blocks = c->sched.blocks > 0 ? c->sched.blocks : default_blocks(c);
+            HIPC(tx::launch_fill_u32(c->sv.q_head, 0u, static_cast<int64_t>(S) * c->sv.n_cells, st));
+            HIPC(hipEventRecord(c->ev[1], st));
+            for (int s = 0; s < S; ++s)
+                HIPC(tx::launch_sgd_par(c->dm, c->sv, s, c->shape.team, c->shape.vec, blocks, qf, p2, ord, st));
+            HIPC(hipEventRecord(c->ev[2], st));
+            HIPC(tx::launch_rebuild(c->dm, c->n_ud, c->shape.team, c->shape.vec, st));
+            HIPC(hipEventRecord(c->ev[3], st));
+            HIPC(hipEventSynchronize(c->ev[3]));
+            float a = 0, b = 0, t = 0;
+            HIPC(hipEventElapsedTime(&a, c->ev[1], c->ev[2]));
+            HIPC(hipEventElapsedTime(&b, c->ev[2], c->ev[3]));
+            HIPC(hipEventElapsedTime(&t, c->ev[0], c->ev[3]));
+            stats.ms_sgd = a;
+            stats.ms_rebuild = b;
+            stats.ms_total = t;
+            stats.ms_other = t - a - b;
+            stats.n_launches = S + 2;
+        }
+    }
+    c->last = stats;
+    return 0;
+}
+
+int tx_last_epoch_stats(tx_ctx* c, tx_epoch_stats* out) {
+    g_err.clear();
+    if (!c || !out) return fail(TX_EINVAL, "tx_last_epoch_stats: NULL argument");
+    *out = c->last;
+    return 0;
+}
+
+int tx_predict_device(tx_ctx* c, int64_t n, const int32_t* d_u, const int32_t* d_i, const int16_t* d_d, float* d_out,
+                      void* stream) {
+    g_err.clear();
+    if (!c || n < 0 || (n > 0 && (!d_u || !d_i || !d_d || !d_out))) return fail(TX_EINVAL, "tx_predict_device: bad arguments");
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->st;
+    HIPC(tx::launch_predict(c->dm, 0, n, d_u, d_i, d_d, d_out, nullptr, nullptr, nullptr, nullptr, nullptr,
+                            c->shape.team, c->shape.vec, st));
+    return 0;
+}
+
+namespace {
+
+int check_queries(tx_ctx* c, int64_t n, const int32_t* u, const int32_t* i) {
+    for (int64_t k = 0; k < n; ++k) {
+        if (u[k] < 0 || u[k] >= c->U) return fail(TX_EINVAL, "user index %d out of range at %lld", u[k], (long long)k);
+        if (i[k] < 0 || i[k] >= c->I) return fail(TX_EINVAL, "item index %d out of range at %lld", i[k], (long long)k);
+    }
+    return 0;
+}
+
+// Host-buffer predict / subscores / rmse in chunks through the staging buffer.
+int run_host(tx_ctx* c, int mode, int64_t n, const int32_t* u, const int32_t* i, const int16_t* d, float* o0, float* o1,
+             float* o2, const float* target, const int8_t* is_test, double* acc) {
+    if (int rc = check_queries(c, n, u, i)) return rc;
+    const int64_t CH = int64_t{1} << 22;
+    const int64_t chunk = std::min<int64_t>(std::max<int64_t>(n, 1), CH);
+    const size_t per = sizeof(int32_t) * 2 + sizeof(int16_t) + sizeof(float) * 3 + sizeof(int8_t) + 16;
+    if (int rc = c->stage(static_cast<size_t>(chunk) * per + 256)) return rc;
+    char* base = static_cast<char*>(c->d_stage);
+    auto carve = [&](size_t bytes) {
+        char* p = base;
+        base += (bytes + 255) / 256 * 256;
+        return p;
+    };
+    int32_t* du = reinterpret_cast<int32_t*>(carve(chunk * 4));
+    int32_t* di = reinterpret_cast<int32_t*>(carve(chunk * 4));
+    int16_t* dd = reinterpret_cast<int16_t*>(carve(chunk * 2));
+    float* d0 = reinterpret_cast<float*>(carve(chunk * 4));
+    float* d1 = reinterpret_cast<float*>(carve(chunk * 4));
+    float* d2 = reinterpret_cast<float*>(carve(chunk * 4));
+    int8_t* dt = reinterpret_cast<int8_t*>(carve(chunk));
+    if (mode == 2) HIPC(tx::launch_fill_f64(c->d_acc, 0.0, 2, c->st));
+    for (int64_t lo = 0; lo < n; lo += chunk) {
+        const int64_t m = std::min(chunk, n - lo);
+        HIPC(hipMemcpyAsync(du, u + lo, m * 4, hipMemcpyHostToDevice, c->st));
+        HIPC(hipMemcpyAsync(di, i + lo, m * 4, hipMemcpyHostToDevice, c->st));
+        HIPC(hipMemcpyAsync(dd, d + lo, m * 2, hipMemcpyHostToDevice, c->st));
+        if (mode == 2) {
+            HIPC(hipMemcpyAsync(d0, target + lo, m * 4, hipMemcpyHostToDevice, c->st));
+            if (is_test) HIPC(hipMemcpyAsync(dt, is_test + lo, m, hipMemcpyHostToDevice, c->st));
+        }
+        HIPC(tx::launch_predict(c->dm, mode, m, du, di, dd, d0, d1, d2, mode == 2 ? d0 : nullptr,
+                                (mode == 2 && is_test) ? dt : nullptr, c->d_acc, c->shape.team, c->shape.vec, c->st));
+        if (mode == 0) HIPC(hipMemcpyAsync(o0 + lo, d0, m * 4, hipMemcpyDeviceToHost, c->st));
+        if (mode == 1) {
+            HIPC(hipMemcpyAsync(o0 + lo, d0, m * 4, hipMemcpyDeviceToHost, c->st));
+            HIPC(hipMemcpyAsync(o1 + lo, d1, m * 4, hipMemcpyDeviceToHost, c->st));
+            HIPC(hipMemcpyAsync(o2 + lo, d2, m * 4, hipMemcpyDeviceToHost, c->st));
+        }
+        HIPC(hipStreamSynchronize(c->st));
+    }
+    if (mode == 2) {
+        HIPC(hipMemcpyAsync(acc, c->d_acc, 2 * sizeof(double), hipMemcpyDeviceToHost, c->st));
+        HIPC(hipStreamSynchronize(c->st));
+    }
+    return 0;
+}
+
+}  // namespace
+
+int tx_predict(tx_ctx* c, int64_t n, const int32_t* u, const int32_t* i, const int16_t* d, float* out) {
+    g_err.clear();
+    if (!c || n < 0 || (n > 0 && (!u || !i || !d || !out))) return fail(TX_EINVAL, "tx_predict: bad arguments");
+    return run_host(c, 0, n, u, i, d, out, nullptr, nullptr, nullptr, nullptr, nullptr);
+}
+
+int tx_predict_subscores(tx_ctx* c, int64_t n, const int32_t* u, const int32_t* i, const int16_t* d, float* bias,
+                         float* mf, float* nsvd1) {
+    g_err.clear();
+    if (!c || n < 0 || (n > 0 && (!u || !i || !d || !bias || !mf || !nsvd1)))
+        return fail(TX_EINVAL, "tx_predict_subscores: bad arguments");
+    return run_host(c, 1, n, u, i, d, bias, mf, nsvd1, nullptr, nullptr, nullptr);
+}
+
+int tx_calc_rmse(tx_ctx* c, const tx_ratings* ds, double* out) {
+    g_err.clear();
+    if (!c || !ds || !out) return fail(TX_EINVAL, "tx_calc_rmse: NULL argument");
+    if (ds->n_ratings > 0 && (!ds->user_idxs || !ds->item_idxs || !ds->dates || !ds->residuals))
+        return fail(TX_EINVAL, "tx_calc_rmse: rating set lacks indices or residuals");
+    double acc[2] = {0.0, 0.0};
+    if (int rc = run_host(c, 2, ds->n_ratings, ds->user_idxs, ds->item_idxs, ds->dates, nullptr, nullptr, nullptr,
+                          ds->residuals, ds->is_test, acc))
+        return rc;
+    *out = std::sqrt(acc[0] / acc[1]);
+    return 0;
+}
+
+int tx_param_len(tx_ctx* c, int32_t id, int64_t* n) {
+    g_err.clear();
+    if (!c || !n) return fail(TX_EINVAL, "tx_param_len: NULL argument");
+    const ParamDesc p = param_desc(c, id);
+    if (p.rows < 0) return fail(TX_EINVAL, "unknown parameter id %d", id);
+    *n = p.rows * p.cols;
+    return 0;
+}
+
+int tx_get_param(tx_ctx* c, int32_t id, float* out, int64_t n) {
+    g_err.clear();
+    if (!c || (n > 0 && !out)) return fail(TX_EINVAL, "tx_get_param: NULL argument");
+    const ParamDesc p = param_desc(c, id);
+    if (p.rows < 0) return fail(TX_EINVAL, "unknown parameter id %d", id);
+    if (n != p.rows * p.cols) return fail(TX_EINVAL, "parameter %d holds %lld elements, caller passed %lld", id,
+                                          (long long)(p.rows * p.cols), (long long)n);
+    if (id == TX_P_GBIAS) {
+        out[0] = c->dm.gbias;
+        return 0;
+    }
+    if (n == 0) return 0;
+    HIPC(hipStreamSynchronize(c->st));
+    if (p.padded)
+        HIPC(hipMemcpy2D(out, p.cols * sizeof(float), p.p, c->fs * sizeof(float), p.cols * sizeof(float), p.rows,
+                         hipMemcpyDeviceToHost));
+    else
+        HIPC(hipMemcpy(out, p.p, n * sizeof(float), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int tx_set_param(tx_ctx* c, int32_t id, const float* in, int64_t n) {
+    g_err.clear();
+    if (!c || (n > 0 && !in)) return fail(TX_EINVAL, "tx_set_param: NULL argument");
+    const ParamDesc p = param_desc(c, id);
+    if (p.rows < 0) return fail(TX_EINVAL, "unknown parameter id %d", id);
+    if (n != p.rows * p.cols) return fail(TX_EINVAL, "parameter %d holds %lld elements, caller passed %lld", id,
+                                          (long long)(p.rows * p.cols), (long long)n);
+    if (id == TX_P_GBIAS) {
+        c->dm.gbias = in[0];
+        return 0;
+    }
+    if (n == 0) return 0;
+    HIPC(hipStreamSynchronize(c->st));
+    if (p.padded)
+        HIPC(hipMemcpy2D(p.p, c->fs * sizeof(float), in, p.cols * sizeof(float), p.cols * sizeof(float), p.rows,
+                         hipMemcpyHostToDevice));
+    else
+        HIPC(hipMemcpy(p.p, in, n * sizeof(float), hipMemcpyHostToDevice));
+    // a captured epoch embeds the model by value: re-capture after a change
+    if (c->graph) {
+        (void)hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+        c->graph_S = 0;
+    }
+    return 0;
+}
+
+}  // extern "C"

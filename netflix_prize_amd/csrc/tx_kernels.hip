@@ -1,0 +1,823 @@
+// HIP kernels of the MI355X tsvdx4 (timeSVD++) path, gfx950.
+//
+//   k_sgd_serial     one wavefront replays TxModel::fit_epoch
+//                    (src/tx.rs:734-1115) user by user in the reference's
+//                    order with the reference's float operation order.
+//   k_sgd_par        all users in parallel: one team of TEAM lanes per user,
+//                    VEC factor elements per lane, user state in registers,
+//                    item updates as device-scope atomic deltas; run once
+//                    per stratum of the rotating user-block x item-block
+//                    schedule.
+//   k_rebuild_ycache[_day]  epoch-end NSVD1 caches (src/tx.rs:391-454).
+//   k_predict        predict / predict_subscores / calc_rmse
+//                    (src/tx.rs:303-339, 663-724; src/lib.rs:659-673).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "tx_launch.hpp"
+#include "tx_types.hpp"
+
+namespace tx {
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+
+template <int TEAM>
+__device__ __forceinline__ float team_sum(float v) {
+#pragma unroll
+    for (int o = TEAM / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, TEAM);
+    return v;
+}
+
+template <int TEAM>
+__device__ __forceinline__ double team_sum_d(double v) {
+#pragma unroll
+    for (int o = TEAM / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, TEAM);
+    return v;
+}
+
+template <int VEC>
+struct Vec {
+    float v[VEC];
+};
+
+template <int VEC>
+__device__ __forceinline__ void load_vec(const float* __restrict__ p, float (&out)[VEC]) {
+    if constexpr (VEC % 4 == 0) {
+#pragma unroll
+        for (int c = 0; c < VEC; c += 4) {
+            const float4 x = *reinterpret_cast<const float4*>(p + c);
+            out[c] = x.x; out[c + 1] = x.y; out[c + 2] = x.z; out[c + 3] = x.w;
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < VEC; ++c) out[c] = p[c];
+    }
+}
+
+template <int VEC>
+__device__ __forceinline__ void store_vec(float* __restrict__ p, const float (&in)[VEC]) {
+    if constexpr (VEC % 4 == 0) {
+#pragma unroll
+        for (int c = 0; c < VEC; c += 4)
+            *reinterpret_cast<float4*>(p + c) = make_float4(in[c], in[c + 1], in[c + 2], in[c + 3]);
+    } else {
+#pragma unroll
+        for (int c = 0; c < VEC; ++c) p[c] = in[c];
+    }
+}
+
+__device__ __forceinline__ void atomic_add_dev(float* p, float v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// time_bin (src/tx.rs:285-289)
+__device__ __forceinline__ int time_bin_of(int day, int B, int day_range) {
+    const long long b = (static_cast<long long>(day) * B) / day_range;
+    return static_cast<int>(b < B - 1 ? b : B - 1);
+}
+
+// dev (src/tx.rs:292-300); powf here is the device libm (within 1-2 ulp of the
+// host's), used only for arbitrary predict queries.  Training ratings carry a
+// host-evaluated dev.
+__device__ __forceinline__ float dev_of(float tu_mean, int day, float beta) {
+    const float dt = static_cast<float>(day) - tu_mean;
+    if (dt == 0.0f) return 0.0f;
+    const float s = dt > 0.0f ? 1.0f : -1.0f;
+    return s * powf(fabsf(dt), beta);
+}
+
+// SparseUD::index (src/tx.rs:76-83): binary search of the user's days.
+__device__ __forceinline__ int64_t sud_find(const DevModel& m, int u, int16_t day) {
+    uint32_t lo = m.sud_start[u], hi = m.sud_start[u + 1];
+    while (lo < hi) {
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        const int16_t d = m.sud_date[mid];
+        if (d < day) lo = mid + 1;
+        else if (d > day) hi = mid;
+        else return mid;
+    }
+    return -1;
+}
+
+__device__ __forceinline__ float sigmoid_f(float x) { return 1.0f / (1.0f + expf(-x)); }
+
+// OrdinalHead::grad (src/lib.rs:565-599); returns d loss / d score and the
+// threshold gradient.
+__device__ __forceinline__ float ordinal_grad(const float th[4], float score, int y, float g_t[4]) {
+    float F[6], a[6];
+    F[0] = 0.0f; F[5] = 1.0f;
+#pragma unroll
+    for (int k = 1; k < 5; ++k) F[k] = sigmoid_f(th[k - 1] - score);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) a[k] = F[k] * (1.0f - F[k]);
+    float p_y = F[y] - F[y - 1];
+    if (!(p_y >= 1e-6f)) p_y = 1e-6f;
+    const float g_ds = (a[y] - a[y - 1]) / p_y;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) g_t[k] = 0.0f;
+    if (y >= 2) g_t[y - 2] += a[y - 1] / p_y;
+    if (y <= 4) g_t[y - 1] -= a[y] / p_y;
+    return g_ds;
+}
+
+// OrdinalHead::predict_probs expectation (src/lib.rs:536-559, src/tx.rs:717-720)
+__device__ __forceinline__ float ordinal_expect(const float th[4], float score) {
+    float F[6];
+    F[0] = 0.0f; F[5] = 1.0f;
+#pragma unroll
+    for (int k = 1; k < 5; ++k) F[k] = sigmoid_f(th[k - 1] - score);
+    float p[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) p[k] = F[k + 1] - F[k];
+    return 1.0f * p[0] + 2.0f * p[1] + 3.0f * p[2] + 4.0f * p[3] + 5.0f * p[4];
+}
+
+// ---------------------------------------------------------------------------
+// Serial schedule: the reference trainer, one wavefront.
+// Lane l owns factor elements k = l, l+64, ...; scalars are evaluated by every
+// lane from the same loads and stored by lane 0.  The dot product is summed
+// sequentially over k by lane 0, so every value matches the reference bit for
+// bit (contraction is off for this kernel).
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(64) void k_sgd_serial(DevModel m, const int64_t* __restrict__ order,
+                                                   float* __restrict__ scratch) {
+#pragma clang fp contract(off)
+    extern __shared__ float prod[];  // [fs + 1]
+    const int lane = threadIdx.x;
+    const int f = m.f, fs = m.fs, F = m.F, B = m.B;
+    const bool low_mem = m.low_mem != 0;
+    const Hyper h = m.h;
+    float* su = scratch;
+    float* G = scratch + fs;
+    float* Gd = scratch + 2 * fs;
+    float* sud = scratch + 3 * fs;
+    const bool has_ud = !low_mem && h.lr_ud != 0.0f;
+
+    for (int ui = 0; ui < m.U; ++ui) {
+        const int u = static_cast<int>(order[ui]);
+        const uint32_t start = m.t_off[u], end = m.t_off[u + 1];
+        if (start == end) continue;
+        const float norm = m.norm_train[u];
+
+        // NSVD1 user aggregation (src/tx.rs:801-822)
+        for (int k = lane; k < f; k += 64) {
+            float acc = 0.0f;
+            for (uint32_t t = start; t < end; ++t) acc += m.yfeat[static_cast<size_t>(m.t_item[t]) * fs + k];
+            if (m.full_su)
+                for (uint32_t q = m.h_start[u]; q < m.h_start[u + 1]; ++q)
+                    acc += m.yfeat[static_cast<size_t>(m.h_item[q]) * fs + k];
+            acc = acc / norm;
+            su[k] = acc;
+            m.ycache[static_cast<size_t>(u) * fs + k] = acc;
+            G[k] = 0.0f;
+            Gd[k] = 0.0f;
+            sud[k] = 0.0f;
+        }
+        float su_bias = 0.0f;
+        for (uint32_t t = start; t < end; ++t) su_bias += m.ybias[m.t_item[t]];
+        if (m.full_su)
+            for (uint32_t q = m.h_start[u]; q < m.h_start[u + 1]; ++q) su_bias += m.ybias[m.h_item[q]];
+        su_bias = su_bias / norm;
+        if (lane == 0) m.ycache_bias[u] = su_bias;
+
+        float sum_err = 0.0f;
+        float norm_day = 0.0f;
+        int cur_fb = 0;
+        uint32_t run_begin = start;
+
+        for (uint32_t t = start; t < end; ++t) {
+            const uint32_t i = m.t_item[t];
+            const float r = m.t_r[t];
+            const int b = m.t_bin[t];
+            const float dev = m.t_dev[t];
+            const uint32_t ud = m.t_ud[t];
+
+            if (!low_mem && (t == start || m.t_run_stop[t - 1] != m.t_run_stop[t])) {
+                // start of a same-day run (src/tx.rs:846-871)
+                run_begin = t;
+                const uint32_t stop = m.t_run_stop[t];
+                cur_fb = m.sud_fbin[ud];
+                for (int k = lane; k < f; k += 64) {
+                    Gd[k] = 0.0f;
+                    float acc = 0.0f;
+                    for (uint32_t td = t; td <= stop; ++td)
+                        acc += m.yfeat_day[static_cast<size_t>(m.t_item[td]) * fs + k];
+                    if (m.full_su)
+                        for (uint32_t q = m.hd_start[ud]; q < m.hd_start[ud + 1]; ++q)
+                            acc += m.yfeat_day[static_cast<size_t>(m.hd_item[q]) * fs + k];
+                    sud[k] = acc / m.t_run_norm[t];
+                }
+                norm_day = m.t_run_norm[t];
+            }
+            const int fb = low_mem ? 0 : cur_fb;
+
+            // forward (src/tx.rs:878-901)
+            const float bu_day = m.ubias_day[ud];
+            const float bu_t = m.ubias[u] + m.but_bin[static_cast<size_t>(u) * B + b] + m.alpha_u[u] * dev + bu_day
+                               + m.ycache_bias[u];
+            const float bi_t = m.ibias[i] + m.bit_bin[static_cast<size_t>(i) * B + b]
+                               + m.ibias_freq[static_cast<size_t>(i) * F + fb];
+            const float cu_t = m.cu[u] + m.cut[ud];
+            const size_t qrow = static_cast<size_t>(i) * fs;
+            const size_t qfrow = (static_cast<size_t>(i) * F + fb) * fs;
+            const size_t prow = static_cast<size_t>(u) * fs;
+            for (int k = lane; k < f; k += 64) {
+                const float qi_eff = m.ifeat[qrow + k] + m.ifeat_freq[qfrow + k];
+                float pu_eff = m.ufeat[prow + k] + su[k] + dev * m.ufeat2[prow + k];
+                if (!low_mem) {
+                    pu_eff += m.ufeat_day[static_cast<size_t>(ud) * fs + k];
+                    pu_eff += sud[k] + 0.0f;
+                }
+                prod[k] = pu_eff * qi_eff;
+            }
+            __syncthreads();
+            if (lane == 0) {
+                float d = 0.0f;
+                for (int k = 0; k < f; ++k) d += prod[k];
+                prod[fs] = d;
+            }
+            __syncthreads();
+            const float dot = prod[fs];
+            __syncthreads();
+            const float bias_part = m.gbias + bu_t + bi_t * cu_t;
+            const float score = bias_part + dot + 0.0f;
+
+            float err;
+            if (m.has_ord) {
+                float th[4], g_t[4];
+                for (int k = 0; k < 4; ++k) th[k] = m.thresholds[k];
+                err = ordinal_grad(th, score, static_cast<int>(m.t_raw[t]), g_t);
+                if (lane == 0)
+                    for (int k = 0; k < 4; ++k) m.thresholds[k] = th[k] - h.lr_t * (g_t[k] + h.reg_t * th[k]);
+            } else {
+                err = score - r;
+            }
+            const float err_b = err * h.w_bias;
+            const float err_d = err * h.w_factor;
+
+            // bias steps (src/tx.rs:981-990)
+            if (lane == 0) {
+                m.ubias[u] = m.ubias[u] - h.lr_ub * err_b;
+                float* bb = &m.but_bin[static_cast<size_t>(u) * B + b];
+                *bb = *bb - h.lr_tu * err_b;
+                m.alpha_u[u] = m.alpha_u[u] - h.lr_ta * err_b * dev;
+                m.ubias_day[ud] = m.ubias_day[ud] - h.lr_ubd * err_b;
+                m.ibias[i] = m.ibias[i] - h.lr_ib * (err_b * cu_t);
+                float* tb = &m.bit_bin[static_cast<size_t>(i) * B + b];
+                *tb = *tb - h.lr_ti * (err_b * cu_t);
+                float* fbp = &m.ibias_freq[static_cast<size_t>(i) * F + fb];
+                *fbp = *fbp - h.lr_ibf * (err_b * cu_t);
+                m.cu[u] = m.cu[u] - h.lr_cu * (err_b * bi_t + h.reg_cu * (m.cu[u] - 1.0f));
+            }
+
+            // factor steps (src/tx.rs:995-1023)
+            if (!m.sum_err_bug) sum_err += err_b;
+            else
+                for (int k = 0; k < f; ++k) sum_err += err_b;
+            for (int k = lane; k < f; k += 64) {
+                const float qi_k = m.ifeat[qrow + k];
+                const float qf_k = m.ifeat_freq[qfrow + k];
+                const float qi_eff_k = qi_k + qf_k;
+                const float pu_k = m.ufeat[prow + k];
+                const float pu2_k = m.ufeat2[prow + k];
+                G[k] += err_d * qi_eff_k;
+                if (!low_mem) Gd[k] += err_d * qi_eff_k;
+                m.ufeat[prow + k] = pu_k - h.lr_u * (err_d * qi_eff_k + h.reg_u * pu_k);
+                m.ufeat2[prow + k] = pu2_k - h.lr_u2 * (err_d * qi_eff_k * dev + h.reg_u2 * pu2_k);
+                float pud_k = 0.0f;
+                if (has_ud) {
+                    float* pd = &m.ufeat_day[static_cast<size_t>(ud) * fs + k];
+                    const float pud = *pd;
+                    *pd = pud - h.lr_ud * (err_d * qi_eff_k + h.reg_ud * pud);
+                    pud_k = pud;
+                }
+                const float pu_eff = pu_k + dev * pu2_k + pud_k + su[k] + (low_mem ? 0.0f : sud[k] + 0.0f);
+                m.ifeat[qrow + k] = qi_k - h.lr_i * (err_d * pu_eff + h.reg_i * qi_k);
+                m.ifeat_freq[qfrow + k] = qf_k - h.lr_iqf * (err_d * pu_eff + h.reg_iqf * qf_k);
+            }
+
+            // end of a same-day run (src/tx.rs:1025-1036)
+            if (!low_mem && t == m.t_run_stop[t]) {
+                for (uint32_t td = run_begin; td <= t; ++td) {
+                    const size_t yr = static_cast<size_t>(m.t_item[td]) * fs;
+                    for (int k = lane; k < f; k += 64) {
+                        const float yj = m.yfeat_day[yr + k];
+                        m.yfeat_day[yr + k] = yj - h.lr_yd * (Gd[k] / norm_day + h.reg_yd * yj);
+                    }
+                }
+            }
+        }
+
+        // end of user (src/tx.rs:1099-1106)
+        for (uint32_t t = start; t < end; ++t) {
+            const uint32_t j = m.t_item[t];
+            const size_t yr = static_cast<size_t>(j) * fs;
+            for (int k = lane; k < f; k += 64) {
+                const float yj = m.yfeat[yr + k];
+                m.yfeat[yr + k] = yj - h.lr_y * (G[k] / norm + h.reg_y * yj);
+            }
+            if (lane == 0) m.ybias[j] = m.ybias[j] - h.lr_yb * (sum_err / norm);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Parallel schedule.  One team of TEAM lanes owns one user segment; lane l
+// holds factor elements [l*VEC, l*VEC + VEC).  Users of a cell are pulled from
+// a per-(stratum, cell) queue in chunks; cell = block index mod n_cells, so a
+// cell's workgroups share one XCD under the round-robin workgroup dispatch.
+// ---------------------------------------------------------------------------
+
+template <int TEAM, int VEC, bool QF, bool P2, bool ORD>
+__global__ __launch_bounds__(256) void k_sgd_par(DevModel m, StrataView sv, int stratum) {
+    constexpr int CHUNK = 4;
+    const int lane = threadIdx.x % TEAM;
+    const int S = sv.n_strata;
+    const int cell = sv.n_cells > 1 ? static_cast<int>(blockIdx.x % sv.n_cells) : 0;
+    const int qid = stratum * sv.n_cells + cell;
+    const uint32_t qbeg = sv.q_start[qid];
+    const uint32_t qlen = sv.q_start[qid + 1] - qbeg;
+    const int fs = m.fs, F = m.F, B = m.B;
+    const Hyper h = m.h;
+    const int off = lane * VEC;
+    const bool first = (stratum == 0);
+    const bool last = (stratum == S - 1);
+
+    float th[4] = {0.f, 0.f, 0.f, 0.f};
+    if constexpr (ORD) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) th[k] = m.thresholds[k];
+    }
+
+    for (;;) {
+        uint32_t w0 = 0;
+        if (lane == 0) w0 = __hip_atomic_fetch_add(&sv.q_head[qid], static_cast<uint32_t>(CHUNK), __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+        w0 = __shfl(w0, 0, TEAM);
+        if (w0 >= qlen) break;
+        const uint32_t w1 = w0 + CHUNK < qlen ? w0 + CHUNK : qlen;
+        for (uint32_t w = w0; w < w1; ++w) {
+            const uint32_t u = sv.q_users[qbeg + w];
+            const uint32_t s0 = sv.seg_off[static_cast<size_t>(u) * (S + 1) + stratum];
+            const uint32_t s1 = sv.seg_off[static_cast<size_t>(u) * (S + 1) + stratum + 1];
+            const size_t prow = static_cast<size_t>(u) * fs + off;
+            const uint32_t ts = m.t_off[u], te = m.t_off[u + 1];
+            const float norm = m.norm_train[u];
+
+            float p[VEC], p2[VEC], s[VEC], G[VEC];
+            load_vec<VEC>(m.ufeat + prow, p);
+            if constexpr (P2) {
+                load_vec<VEC>(m.ufeat2 + prow, p2);
+            } else {
+#pragma unroll
+                for (int c = 0; c < VEC; ++c) p2[c] = 0.0f;
+            }
+            float su_b;
+            if (first) {
+                // NSVD1 user aggregation from the current y (src/tx.rs:801-822)
+#pragma unroll
+                for (int c = 0; c < VEC; ++c) s[c] = 0.0f;
+                float sb = 0.0f;
+                for (uint32_t t = ts; t < te; ++t) {
+                    const uint32_t j = m.t_item[t];
+                    float y[VEC];
+                    load_vec<VEC>(m.yfeat + static_cast<size_t>(j) * fs + off, y);
+#pragma unroll
+                    for (int c = 0; c < VEC; ++c) s[c] += y[c];
+                    if (lane == (t & (TEAM - 1))) sb += m.ybias[j];
+                }
+                if (m.full_su) {
+                    for (uint32_t q = m.h_start[u]; q < m.h_start[u + 1]; ++q) {
+                        const uint32_t j = m.h_item[q];
+                        float y[VEC];
+                        load_vec<VEC>(m.yfeat + static_cast<size_t>(j) * fs + off, y);
+#pragma unroll
+                        for (int c = 0; c < VEC; ++c) s[c] += y[c];
+                        if (lane == (q & (TEAM - 1))) sb += m.ybias[j];
+                    }
+                }
+                const float inv = 1.0f / norm;
+#pragma unroll
+                for (int c = 0; c < VEC; ++c) s[c] *= inv;
+                su_b = team_sum<TEAM>(sb) * inv;
+                if (S > 1) {
+                    store_vec<VEC>(m.ycache + prow, s);
+                    if (lane == 0) m.ycache_bias[u] = su_b;
+                }
+#pragma unroll
+                for (int c = 0; c < VEC; ++c) G[c] = 0.0f;
+            } else {
+                load_vec<VEC>(m.ycache + prow, s);
+                su_b = m.ycache_bias[u];
+                load_vec<VEC>(sv.gacc + prow, G);
+            }
+            float Gb = first ? 0.0f : sv.gacc_b[u];
+            float ubias = m.ubias[u], alpha = m.alpha_u[u], cu = m.cu[u];
+
+            for (uint32_t t = s0; t < s1; ++t) {
+                const uint32_t i = sv.p_item[t];
+                const uint32_t ud = sv.p_ud[t];
+                const float dev = sv.p_dev[t];
+                const float r = sv.p_r[t];
+                const int b = sv.p_bin[t];
+                const size_t qrow = static_cast<size_t>(i) * fs + off;
+                const size_t qfrow = static_cast<size_t>(i) * F * fs + off;  // freq bin 0 (low-memory training)
+                float q[VEC], qf[VEC];
+                load_vec<VEC>(m.ifeat + qrow, q);
+                if constexpr (QF) {
+                    load_vec<VEC>(m.ifeat_freq + qfrow, qf);
+                } else {
+#pragma unroll
+                    for (int c = 0; c < VEC; ++c) qf[c] = 0.0f;
+                }
+                float* bb = &m.but_bin[static_cast<size_t>(u) * B + b];
+                const float but = *bb;
+                const float bud = m.ubias_day[ud];
+                const float bi_t = m.ibias[i] + m.bit_bin[static_cast<size_t>(i) * B + b]
+                                   + m.ibias_freq[static_cast<size_t>(i) * F];
+                const float cu_t = cu + m.cut[ud];
+                const float bu_t = ubias + but + alpha * dev + bud + su_b;
+
+                float part = 0.0f;
+#pragma unroll
+                for (int c = 0; c < VEC; ++c) part += (q[c] + qf[c]) * (p[c] + s[c] + dev * p2[c]);
+                const float dot = team_sum<TEAM>(part);
+                const float score = m.gbias + bu_t + bi_t * cu_t + dot;
+                float err;
+                if constexpr (ORD) {
+                    float g_t[4];
+                    err = ordinal_grad(th, score, static_cast<int>(sv.p_raw[t]), g_t);
+                } else {
+                    err = score - r;
+                }
+                const float err_b = err * h.w_bias;
+                const float err_d = err * h.w_factor;
+
+                ubias -= h.lr_ub * err_b;
+                alpha -= h.lr_ta * err_b * dev;
+                if (lane == 0) {
+                    *bb = but - h.lr_tu * err_b;
+                    m.ubias_day[ud] = bud - h.lr_ubd * err_b;
+                    const float gi = err_b * cu_t;
+                    atomic_add_dev(&m.ibias[i], -h.lr_ib * gi);
+                    atomic_add_dev(&m.bit_bin[static_cast<size_t>(i) * B + b], -h.lr_ti * gi);
+                    atomic_add_dev(&m.ibias_freq[static_cast<size_t>(i) * F], -h.lr_ibf * gi);
+                }
+                cu -= h.lr_cu * (err_b * bi_t + h.reg_cu * (cu - 1.0f));
+                Gb += m.sum_err_bug ? err_b * static_cast<float>(m.f) : err_b;
+#pragma unroll
+                for (int c = 0; c < VEC; ++c) {
+                    const float qe = q[c] + qf[c];
+                    G[c] += err_d * qe;
+                    const float pe = p[c] + dev * p2[c] + s[c];
+                    p[c] -= h.lr_u * (err_d * qe + h.reg_u * p[c]);
+                    if constexpr (P2) p2[c] -= h.lr_u2 * (err_d * qe * dev + h.reg_u2 * p2[c]);
+                    atomic_add_dev(&m.ifeat[qrow + c], -h.lr_i * (err_d * pe + h.reg_i * q[c]));
+                    if constexpr (QF) atomic_add_dev(&m.ifeat_freq[qfrow + c], -h.lr_iqf * (err_d * pe + h.reg_iqf * qf[c]));
+                }
+            }
+
+            store_vec<VEC>(m.ufeat + prow, p);
+            if constexpr (P2) store_vec<VEC>(m.ufeat2 + prow, p2);
+            if (lane == 0) {
+                m.ubias[u] = ubias;
+                m.alpha_u[u] = alpha;
+                m.cu[u] = cu;
+            }
+            if (last) {
+                // end-of-user NSVD1 step over R(u) (src/tx.rs:1099-1106)
+                const float inv = 1.0f / norm;
+                const float gb = h.lr_yb * (Gb * inv);
+                for (uint32_t t = ts; t < te; ++t) {
+                    const uint32_t j = m.t_item[t];
+                    float* yr = m.yfeat + static_cast<size_t>(j) * fs + off;
+                    float y[VEC];
+                    load_vec<VEC>(yr, y);
+#pragma unroll
+                    for (int c = 0; c < VEC; ++c) atomic_add_dev(yr + c, -h.lr_y * (G[c] * inv + h.reg_y * y[c]));
+                    if (lane == (t & (TEAM - 1)) && gb != 0.0f) atomic_add_dev(&m.ybias[j], -gb);
+                }
+            } else {
+                store_vec<VEC>(sv.gacc + prow, G);
+                if (lane == 0) sv.gacc_b[u] = Gb;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Epoch-end caches.  Per element the sums run over the same rows in the same
+// order as the reference (train rows, then held-out rows), so the result is
+// bit-identical.
+// ---------------------------------------------------------------------------
+
+template <int TEAM, int VEC>
+__global__ __launch_bounds__(256) void k_rebuild_ycache(DevModel m) {
+#pragma clang fp contract(off)
+    const int team = (blockIdx.x * blockDim.x + threadIdx.x) / TEAM;
+    const int lane = threadIdx.x % TEAM;
+    const int nteams = gridDim.x * blockDim.x / TEAM;
+    const int fs = m.fs, off = lane * VEC;
+    for (int u = team; u < m.U; u += nteams) {
+        float acc[VEC];
+#pragma unroll
+        for (int c = 0; c < VEC; ++c) acc[c] = 0.0f;
+        float sb = 0.0f;
+        const uint32_t ts = m.t_off[u], te = m.t_off[u + 1];
+        for (uint32_t t = ts; t < te; ++t) {
+            const uint32_t j = m.t_item[t];
+            float y[VEC];
+            load_vec<VEC>(m.yfeat + static_cast<size_t>(j) * fs + off, y);
+#pragma unroll
+            for (int c = 0; c < VEC; ++c) acc[c] += y[c];
+            if (lane == 0) sb += m.ybias[j];
+        }
+        for (uint32_t q = m.h_start[u]; q < m.h_start[u + 1]; ++q) {
+            const uint32_t j = m.h_item[q];
+            float y[VEC];
+            load_vec<VEC>(m.yfeat + static_cast<size_t>(j) * fs + off, y);
+#pragma unroll
+            for (int c = 0; c < VEC; ++c) acc[c] += y[c];
+            if (lane == 0) sb += m.ybias[j];
+        }
+        const float nrm = m.norm_rebuild[u];
+        if (nrm > 0.0f) {
+#pragma unroll
+            for (int c = 0; c < VEC; ++c) acc[c] = acc[c] / nrm;
+            sb = sb / nrm;
+        }
+        store_vec<VEC>(m.ycache + static_cast<size_t>(u) * fs + off, acc);
+        if (lane == 0) m.ycache_bias[u] = sb;
+    }
+}
+
+template <int TEAM, int VEC>
+__global__ __launch_bounds__(256) void k_rebuild_ycache_day(DevModel m, int64_t n_ud) {
+#pragma clang fp contract(off)
+    const int64_t team = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / TEAM;
+    const int lane = threadIdx.x % TEAM;
+    const int64_t nteams = static_cast<int64_t>(gridDim.x) * blockDim.x / TEAM;
+    const int fs = m.fs, off = lane * VEC;
+    for (int64_t d = team; d < n_ud; d += nteams) {
+        float acc[VEC];
+#pragma unroll
+        for (int c = 0; c < VEC; ++c) acc[c] = 0.0f;
+        for (uint32_t q = m.hd_start[d]; q < m.hd_start[d + 1]; ++q) {
+            float y[VEC];
+            load_vec<VEC>(m.yfeat_day + static_cast<size_t>(m.hd_item[q]) * fs + off, y);
+#pragma unroll
+            for (int c = 0; c < VEC; ++c) acc[c] += y[c];
+        }
+        for (uint32_t q = m.td_start[d]; q < m.td_start[d + 1]; ++q) {
+            float y[VEC];
+            load_vec<VEC>(m.yfeat_day + static_cast<size_t>(m.td_item[q]) * fs + off, y);
+#pragma unroll
+            for (int c = 0; c < VEC; ++c) acc[c] += y[c];
+        }
+        const float nrm = m.sud_norm[d];
+        if (nrm > 0.0f) {
+#pragma unroll
+            for (int c = 0; c < VEC; ++c) acc[c] = acc[c] / nrm;
+        }
+        store_vec<VEC>(m.ycache_day + static_cast<size_t>(d) * fs + off, acc);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Predict: one team per (user, item, day) query.
+//   MODE 0: score (ordinal expectation when the head is on)
+//   MODE 1: subscores bias / mf / nsvd1
+//   MODE 2: squared error vs. target, FP64 sum into out_d[0], count in out_d[1]
+// ---------------------------------------------------------------------------
+
+template <int TEAM, int VEC, int MODE>
+__global__ __launch_bounds__(256) void k_predict(DevModel m, int64_t n, const int32_t* __restrict__ us,
+                                                 const int32_t* __restrict__ is, const int16_t* __restrict__ ds,
+                                                 float* __restrict__ out0, float* __restrict__ out1,
+                                                 float* __restrict__ out2, const float* __restrict__ target,
+                                                 const int8_t* __restrict__ is_test, double* __restrict__ out_d) {
+    const int64_t team = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / TEAM;
+    const int lane = threadIdx.x % TEAM;
+    const int64_t nteams = static_cast<int64_t>(gridDim.x) * blockDim.x / TEAM;
+    const int fs = m.fs, F = m.F, B = m.B, off = lane * VEC;
+    double sse = 0.0, cnt = 0.0;
+    for (int64_t qn = team; qn < n; qn += nteams) {
+        if (MODE == 2 && is_test != nullptr && is_test[qn] != 0) continue;
+        const int u = us[qn], i = is[qn];
+        const int day = ds[qn];
+        const int b = time_bin_of(day, B, m.day_range);
+        const float dev = dev_of(m.tu_mean[u], day, m.h.beta);
+        const int64_t ud = sud_find(m, u, static_cast<int16_t>(day));
+        const int fb = ud >= 0 ? m.sud_fbin[ud] : 0;
+        const float bu_day = ud >= 0 ? m.ubias_day[ud] : 0.0f;
+        const float bi_t = m.ibias[i] + m.bit_bin[static_cast<size_t>(i) * B + b] + m.ibias_freq[static_cast<size_t>(i) * F + fb];
+        const float cu_t = m.cu[u] + (ud >= 0 ? m.cut[ud] : 0.0f);
+        float q[VEC], qf[VEC], p[VEC], p2[VEC], s[VEC];
+        load_vec<VEC>(m.ifeat + static_cast<size_t>(i) * fs + off, q);
+        load_vec<VEC>(m.ifeat_freq + (static_cast<size_t>(i) * F + fb) * fs + off, qf);
+        load_vec<VEC>(m.ufeat + static_cast<size_t>(u) * fs + off, p);
+        load_vec<VEC>(m.ufeat2 + static_cast<size_t>(u) * fs + off, p2);
+        load_vec<VEC>(m.ycache + static_cast<size_t>(u) * fs + off, s);
+        float pd[VEC], sd[VEC];
+        const bool day_terms = !m.low_mem && ud >= 0;
+        if (day_terms) {
+            load_vec<VEC>(m.ufeat_day + static_cast<size_t>(ud) * fs + off, pd);
+            load_vec<VEC>(m.ycache_day + static_cast<size_t>(ud) * fs + off, sd);
+        } else {
+#pragma unroll
+            for (int c = 0; c < VEC; ++c) { pd[c] = 0.0f; sd[c] = 0.0f; }
+        }
+        if (MODE == 1) {
+            float mf = 0.0f, ns = 0.0f;
+#pragma unroll
+            for (int c = 0; c < VEC; ++c) {
+                const float qe = q[c] + qf[c];
+                mf += (p[c] + dev * p2[c] + pd[c]) * qe;
+                ns += (s[c] + sd[c]) * qe;
+            }
+            mf = team_sum<TEAM>(mf);
+            ns = team_sum<TEAM>(ns);
+            if (lane == 0) {
+                const float bias = m.gbias + m.ubias[u] + m.ibias[i] + m.ibias_freq[static_cast<size_t>(i) * F + fb]
+                                   + m.but_bin[static_cast<size_t>(u) * B + b] + m.alpha_u[u] * dev + bu_day
+                                   + m.bit_bin[static_cast<size_t>(i) * B + b] + bi_t * (cu_t - 1.0f) + m.ycache_bias[u];
+                out0[qn] = bias;
+                out1[qn] = mf;
+                out2[qn] = ns;
+            }
+            continue;
+        }
+        float part = 0.0f;
+#pragma unroll
+        for (int c = 0; c < VEC; ++c) part += (q[c] + qf[c]) * (p[c] + s[c] + dev * p2[c] + pd[c] + sd[c]);
+        const float dot = team_sum<TEAM>(part);
+        const float bu_t = m.ubias[u] + m.but_bin[static_cast<size_t>(u) * B + b] + m.alpha_u[u] * dev + bu_day
+                           + m.ycache_bias[u];
+        float score = m.gbias + bu_t + bi_t * cu_t + dot;
+        if (m.has_ord) {
+            const float th[4] = {m.thresholds[0], m.thresholds[1], m.thresholds[2], m.thresholds[3]};
+            score = ordinal_expect(th, score);
+        }
+        if (MODE == 0) {
+            if (lane == 0) out0[qn] = score;
+        } else {
+            const double e = static_cast<double>(score - target[qn]);
+            sse += e * e;
+            cnt += 1.0;
+        }
+    }
+    if (MODE == 2) {
+        // lane 0 of every team holds the same sums as its team-mates; reduce
+        // over the block's teams, then one FP64 atomic per block.
+        __shared__ double red[2][256];
+        red[0][threadIdx.x] = lane == 0 ? sse : 0.0;
+        red[1][threadIdx.x] = lane == 0 ? cnt : 0.0;
+        __syncthreads();
+        for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+            if (static_cast<int>(threadIdx.x) < o) {
+                red[0][threadIdx.x] += red[0][threadIdx.x + o];
+                red[1][threadIdx.x] += red[1][threadIdx.x + o];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            atomicAdd(&out_d[0], red[0][0]);
+            atomicAdd(&out_d[1], red[1][0]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// fills (used instead of memsets so a captured epoch holds only kernels)
+// ---------------------------------------------------------------------------
+
+__global__ void k_fill_f32(float* p, float v, int64_t n) {
+    for (int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < n;
+         k += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        p[k] = v;
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, int64_t n) {
+    for (int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < n;
+         k += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        p[k] = v;
+}
+
+__global__ void k_fill_f64(double* p, double v, int64_t n) {
+    for (int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < n;
+         k += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        p[k] = v;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+
+static inline int grid_for(int64_t teams, int team) {
+    const int64_t threads = teams * team;
+    int64_t blocks = (threads + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    return static_cast<int>(blocks < 1 ? 1 : blocks);
+}
+
+// Dispatch on (TEAM, VEC): the shapes the host picks for a padded row stride.
+#define TX_SHAPES(X)  \
+    X(4, 4)           \
+    X(8, 4)           \
+    X(16, 4)          \
+    X(32, 4)          \
+    X(64, 4)          \
+    X(64, 8)          \
+    X(64, 16)
+
+hipError_t launch_sgd_serial(const DevModel& m, const int64_t* order, float* scratch, hipStream_t st) {
+    const size_t shm = sizeof(float) * (static_cast<size_t>(m.fs) + 1);
+    hipLaunchKernelGGL(k_sgd_serial, dim3(1), dim3(64), shm, st, m, order, scratch);
+    return hipGetLastError();
+}
+
+template <int TEAM, int VEC>
+static hipError_t sgd_par_tv(const DevModel& m, const StrataView& sv, int stratum, int blocks, bool qf, bool p2,
+                             bool ord, hipStream_t st) {
+#define TX_PAR(QF, P2, ORD) \
+    hipLaunchKernelGGL((k_sgd_par<TEAM, VEC, QF, P2, ORD>), dim3(blocks), dim3(256), 0, st, m, sv, stratum)
+    if (ord) {
+        if (qf) { if (p2) TX_PAR(true, true, true); else TX_PAR(true, false, true); }
+        else { if (p2) TX_PAR(false, true, true); else TX_PAR(false, false, true); }
+    } else {
+        if (qf) { if (p2) TX_PAR(true, true, false); else TX_PAR(true, false, false); }
+        else { if (p2) TX_PAR(false, true, false); else TX_PAR(false, false, false); }
+    }
+#undef TX_PAR
+    return hipGetLastError();
+}
+
+hipError_t launch_sgd_par(const DevModel& m, const StrataView& sv, int stratum, int team, int vec, int blocks,
+                          bool qf, bool p2, bool ord, hipStream_t st) {
+#define X(T, V) \
+    if (team == T && vec == V) return sgd_par_tv<T, V>(m, sv, stratum, blocks, qf, p2, ord, st);
+    TX_SHAPES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_rebuild(const DevModel& m, int64_t n_ud, int team, int vec, hipStream_t st) {
+#define X(T, V)                                                                                           \
+    if (team == T && vec == V) {                                                                           \
+        hipLaunchKernelGGL((k_rebuild_ycache<T, V>), dim3(grid_for(m.U, T)), dim3(256), 0, st, m);         \
+        if (!m.low_mem && n_ud > 0)                                                                        \
+            hipLaunchKernelGGL((k_rebuild_ycache_day<T, V>), dim3(grid_for(n_ud, T)), dim3(256), 0, st, m, \
+                               n_ud);                                                                      \
+        return hipGetLastError();                                                                          \
+    }
+    TX_SHAPES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_predict(const DevModel& m, int mode, int64_t n, const int32_t* us, const int32_t* is,
+                          const int16_t* ds, float* o0, float* o1, float* o2, const float* target,
+                          const int8_t* is_test, double* out_d, int team, int vec, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+#define X(T, V)                                                                                            \
+    if (team == T && vec == V) {                                                                            \
+        const int g = grid_for(n, T);                                                                       \
+        if (mode == 0)                                                                                      \
+            hipLaunchKernelGGL((k_predict<T, V, 0>), dim3(g), dim3(256), 0, st, m, n, us, is, ds, o0, o1, o2, \
+                               target, is_test, out_d);                                                     \
+        else if (mode == 1)                                                                                 \
+            hipLaunchKernelGGL((k_predict<T, V, 1>), dim3(g), dim3(256), 0, st, m, n, us, is, ds, o0, o1, o2, \
+                               target, is_test, out_d);                                                     \
+        else                                                                                                \
+            hipLaunchKernelGGL((k_predict<T, V, 2>), dim3(g), dim3(256), 0, st, m, n, us, is, ds, o0, o1, o2, \
+                               target, is_test, out_d);                                                     \
+        return hipGetLastError();                                                                           \
+    }
+    TX_SHAPES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_fill_f32(float* p, float v, int64_t n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(n, 1) > 4096 ? 4096 : grid_for(n, 1)), dim3(256), 0, st, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 1) > 4096 ? 4096 : grid_for(n, 1)), dim3(256), 0, st, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_f64(double* p, double v, int64_t n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(64), 0, st, p, v, n);
+    return hipGetLastError();
+}
+
+}  // namespace tx

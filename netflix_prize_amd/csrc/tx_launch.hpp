@@ -1,0 +1,23 @@
+// Kernel launchers (defined in tx_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "tx_types.hpp"
+
+namespace tx {
+
+hipError_t launch_sgd_serial(const DevModel& m, const int64_t* order, float* scratch, hipStream_t st);
+hipError_t launch_sgd_par(const DevModel& m, const StrataView& sv, int stratum, int team, int vec, int blocks,
+                          bool qf, bool p2, bool ord, hipStream_t st);
+hipError_t launch_rebuild(const DevModel& m, int64_t n_ud, int team, int vec, hipStream_t st);
+hipError_t launch_predict(const DevModel& m, int mode, int64_t n, const int32_t* us, const int32_t* is,
+                          const int16_t* ds, float* o0, float* o1, float* o2, const float* target,
+                          const int8_t* is_test, double* out_d, int team, int vec, hipStream_t st);
+hipError_t launch_fill_f32(float* p, float v, int64_t n, hipStream_t st);
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st);
+hipError_t launch_fill_f64(double* p, double v, int64_t n, hipStream_t st);
+
+}  // namespace tx

@@ -1,0 +1,82 @@
+// Device-side view of one TxModel: parameter tables, the static per-rating
+// records derived from the training set, and the hyper-parameters, passed by
+// value to every kernel.
+#pragma once
+
+#include <cstdint>
+
+namespace tx {
+
+// Hyper-parameters used inside kernels (a subset of TxConfig).
+struct Hyper {
+    float lr_u, lr_ud, lr_u2, lr_ub, lr_ubd, lr_i, lr_ib, lr_y, lr_yb, lr_yd;
+    float lr_tu, lr_ti, lr_ta, lr_ibf, lr_iqf, lr_cu;
+    float reg_iqf, reg_cu, reg_u, reg_u2, reg_ud, reg_i, reg_y, reg_yd;
+    float w_bias, w_factor;
+    float beta;
+    float lr_t, reg_t;
+};
+
+struct DevModel {
+    // sizes
+    int32_t U, I, f, fs, F, B;  // f = n_feat, fs = padded row stride
+    int32_t day_range;
+    int32_t low_mem, full_su, has_ord, sum_err_bug;
+    float gbias;
+    Hyper h;
+
+    // parameter tables (row stride fs for the factor tables)
+    float *ubias, *ibias, *ufeat, *ufeat2, *ifeat, *yfeat, *ycache, *yfeat_day;
+    float *tu_mean, *alpha_u, *but_bin, *bit_bin, *ybias, *ycache_bias;
+    float *ubias_day, *ufeat_day, *ycache_day, *ibias_freq, *ifeat_freq, *cu, *cut;
+    float* thresholds;  // [4]
+
+    // SparseUD (src/tx.rs:21-84): per-user sorted unique days
+    const uint32_t* sud_start;  // [U+1]
+    const int16_t* sud_date;    // [D]
+    const uint8_t* sud_fbin;    // [D] freq_bin(day_cnt)
+    const float* sud_norm;      // [D] sqrt(#train+#heldout ratings on that day)
+
+    // training ratings, reference (user-sorted file) order
+    const uint32_t* t_off;   // [U+1]
+    const uint32_t* t_item;  // [N]
+    const uint32_t* t_ud;    // [N] SparseUD index of (user, date)
+    const float* t_dev;      // [N] dev_u(t), evaluated once on the host
+    const float* t_r;        // [N] residual target
+    const uint8_t* t_bin;    // [N] time bin
+    const int8_t* t_raw;     // [N] raw rating (ordinal head)
+    const uint32_t* t_run_stop;  // [N] last index of the same-day run (day terms)
+    const float* t_run_norm;     // [N] sqrt(#ratings of the run + #heldout that day)
+
+    // held-out rows grouped by user / by (user, day), held-out file order
+    const uint32_t* h_start;  // [U+1]
+    const uint32_t* h_item;
+    const uint32_t* hd_start; // [D+1]
+    const uint32_t* hd_item;
+    // training items grouped by (user, day), training file order
+    const uint32_t* td_start; // [D+1]
+    const uint32_t* td_item;
+
+    const float* norm_train;    // [U] sqrt(|R(u)| + |P(u)| if full_su)
+    const float* norm_rebuild;  // [U] sqrt(train_cnt + heldout_cnt)
+};
+
+// Parallel-schedule state (stratified user x item-block rotation).
+struct StrataView {
+    int32_t n_strata;
+    int32_t n_cells;            // user blocks = item blocks = n_strata (1 => one cell)
+    const uint32_t* seg_off;    // [U * (n_strata + 1)] offsets into the permuted records
+    const uint32_t* p_item;     // permuted training records
+    const uint32_t* p_ud;
+    const float* p_dev;
+    const float* p_r;
+    const uint8_t* p_bin;
+    const int8_t* p_raw;
+    const uint32_t* q_users;    // per (stratum, cell) queues of users, longest segment first
+    const uint32_t* q_start;    // [n_strata * n_cells + 1]
+    uint32_t* q_head;           // [n_strata * n_cells] work counters (reset per epoch)
+    float* gacc;                // [U * fs] per-user sum_err_q across strata
+    float* gacc_b;              // [U]      per-user sum_err across strata
+};
+
+}  // namespace tx

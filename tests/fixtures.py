@@ -38,3 +38,33 @@ def tiny():
     tr = dataset_from_rows(fx["train"], fx["n_users"], fx["n_items"], "tiny_train")
     pr = dataset_from_rows(fx["probe"], fx["n_users"], fx["n_items"], "tiny_probe")
     return fx, tr, pr
+
+
+_SYNTH_CACHE = {}
+
+
+def synthetic(name="tiny_synth", seed=7):
+    """Seeded synthetic (train, heldout) sets, cached per process.
+
+    tiny_synth: 600 users x 150 items x 30k ratings (serial-schedule checks)
+    small:      the 10k x 2k x 1M benchmark shape
+    """
+    from netflix_prize_amd import synth
+    key = (name, seed)
+    if key not in _SYNTH_CACHE:
+        if name == "tiny_synth":
+            spec = synth.SynthSpec(n_users=600, n_items=150, n_ratings=30_000, user_median=30, user_max=150,
+                                   item_sigma=1.0, item_min=3, item_max=5_000)
+        elif name == "small":
+            spec = synth.SMALL
+        else:
+            raise KeyError(name)
+        _SYNTH_CACHE[key] = synth.generate(spec, seed)
+    return _SYNTH_CACHE[key]
+
+
+def lm_config(**kw):
+    """The tsvdx4 lm hyper-parameters (src/bin/tsvdx4-new.rs:130-154) with
+    per-test overrides, as a TxConfig."""
+    from netflix_prize_amd.tx import TxConfig
+    return TxConfig(**kw)

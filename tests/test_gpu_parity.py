@@ -1,0 +1,103 @@
+"""HIP path against the oracle, through the C ABI (needs an MI355X)."""
+import numpy as np
+import pytest
+
+from netflix_prize_amd import tx
+from oracle import oracle as O
+from tests.fixtures import lm_config, synthetic, tiny
+
+pytestmark = pytest.mark.gpu
+
+ALL_PARAMS = [p for p in tx.PARAMS if p != "thresholds"]
+
+
+def _oracle_and_gpu(cfg, tr, pr, **sched):
+    m_o = O.OracleModel(cfg, tr, pr)
+    m_g = tx.TxModel(tr, pr, cfg)
+    if sched:
+        m_g.set_schedule(**sched)
+    return m_o, m_g
+
+
+def _assert_params_equal(m_o, m_g, names=ALL_PARAMS, atol=0.0):
+    for n in names:
+        a, b = m_o.get(n), m_g.get(n)
+        assert a.shape == b.shape, n
+        if atol == 0.0:
+            np.testing.assert_array_equal(a, b, err_msg=n)
+        else:
+            np.testing.assert_allclose(b, a, rtol=0, atol=atol, err_msg=n)
+
+
+def test_tiny_kat_on_gpu_serial(gpu):
+    """The reference's tsvdx4 KAT (tests/regressors_test.rs:75-93), trained on
+    the GPU in the serial schedule, evaluated by the GPU calc_rmse."""
+    fx, tr, pr = tiny()
+    cfg = tx.TxConfig.from_dict(fx["kat"]["tsvdx4"]["config"])
+    m = tx.TxModel(tr, pr, cfg)
+    m.set_schedule(mode=tx.SCHED_SERIAL)
+    for e in range(1, cfg.n_epochs + 1):
+        m.fit_epoch(tr, pr, e)
+    rmse = m.calc_rmse(pr)
+    assert abs(rmse - fx["kat"]["tsvdx4"]["probe_rmse"]) < 1e-7, rmse
+
+
+def test_tiny_serial_params_bit_identical(gpu):
+    fx, tr, pr = tiny()
+    cfg = tx.TxConfig.from_dict(fx["kat"]["tsvdx4"]["config"])
+    m_o, m_g = _oracle_and_gpu(cfg, tr, pr, mode=tx.SCHED_SERIAL)
+    _assert_params_equal(m_o, m_g)          # init draws
+    for e in range(1, cfg.n_epochs + 1):
+        m_o.fit_epoch(e)
+        m_g.fit_epoch(epoch=e)
+        _assert_params_equal(m_o, m_g)
+
+
+@pytest.mark.parametrize("low_memory", [True, False])
+def test_serial_schedule_matches_oracle(gpu, low_memory):
+    tr, pr = synthetic("tiny_synth")
+    cfg = lm_config(n_feat=24, n_epochs=2, n_time_bins=30, low_memory=low_memory, reset_u_epoch=2)
+    m_o, m_g = _oracle_and_gpu(cfg, tr, pr, mode=tx.SCHED_SERIAL)
+    for e in (1, 2):
+        m_o.fit_epoch(e)
+        m_g.fit_epoch(epoch=e)
+    _assert_params_equal(m_o, m_g)
+    assert m_g.calc_rmse(pr) == pytest.approx(m_o.calc_rmse(), abs=1e-9)
+
+
+def test_predict_matches_oracle_and_fp64(gpu):
+    """Identical parameters -> predictions within 1e-5 of the FP32 predictor
+    and within 1e-6 RMS of the FP64 predict oracle."""
+    tr, pr = synthetic("small")
+    cfg = lm_config(n_feat=32, n_epochs=1, n_time_bins=30)
+    m_o, m_g = _oracle_and_gpu(cfg, tr, pr)
+    m_o.fit_epoch(1)
+    for n in ALL_PARAMS:
+        m_g.set(n, m_o.get(n))
+    for ds in (pr, tr):
+        u, i, d = ds.user_idxs, ds.item_idxs, ds.dates
+        g = m_g.predict(u, i, d)
+        o = m_o.predict(u, i, d)
+        o64 = m_o.predict_f64(u, i, d)
+        assert np.max(np.abs(g - o)) <= 1e-5
+        assert np.sqrt(np.mean((g.astype(np.float64) - o64) ** 2)) <= 1e-6
+    b, mf, ns = m_g.predict_subscores(pr.user_idxs, pr.item_idxs, pr.dates)
+    ob, omf, ons = m_o.predict_subscores(pr.user_idxs, pr.item_idxs, pr.dates)
+    for x, y in ((b, ob), (mf, omf), (ns, ons)):
+        assert np.max(np.abs(x - y)) <= 1e-5
+    assert m_g.calc_rmse(pr) == pytest.approx(m_o.calc_rmse(), abs=1e-6)
+
+
+@pytest.mark.parametrize("n_strata", [1, 8])
+def test_parallel_schedule_holdout_rmse(gpu, n_strata):
+    """Parallel schedules train to the serial reference's holdout RMSE
+    (within 0.001) on the 10k x 2k x 1M shape."""
+    tr, pr = synthetic("small")
+    cfg = lm_config(n_feat=32, n_epochs=5, n_time_bins=30)
+    m_o, m_g = _oracle_and_gpu(cfg, tr, pr, mode=tx.SCHED_STRATA, n_strata=n_strata)
+    for e in range(1, cfg.n_epochs + 1):
+        m_o.fit_epoch(e)
+        m_g.fit_epoch(epoch=e)
+    r_o, r_g = m_o.calc_rmse(), m_g.calc_rmse(pr)
+    print(f"strata={n_strata}: oracle {r_o:.6f} gpu {r_g:.6f}")
+    assert abs(r_g - r_o) <= 1e-3
