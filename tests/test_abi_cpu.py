@@ -27,12 +27,21 @@ def test_library_exports_every_declared_entry_point():
         assert hasattr(L, n), n
 
 
-def test_struct_layouts_match_header():
-    # tx_config: u64 + 12 i32 + 43 f32 + th_init[4] + 4 f32 + i32 + 5 i32
-    assert C.sizeof(tx.CConfig) == 8 + 12 * 4 + 43 * 4 + 16 + 3 * 4 + 4 + 4 + 20
-    assert C.sizeof(tx.CRatings) == 8 + 4 + 4 + 7 * 8
-    assert C.sizeof(tx.CSchedule) == 8 * 4
-    assert C.sizeof(tx.CEpochStats) == 4 * 8 + 8 + 4 + 4
+def test_struct_layouts_match_header(tmp_path):
+    """ctypes mirrors have the sizes and field offsets the C header gives."""
+    src = tmp_path / "probe.c"
+    src.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "tsvd_tx.h"\n'
+        'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu\\n", sizeof(tx_config), sizeof(tx_ratings),'
+        ' sizeof(tx_schedule), sizeof(tx_epoch_stats), offsetof(tx_config, beta),'
+        ' offsetof(tx_config, th_init), offsetof(tx_config, seq_epochs)); return 0;}\n')
+    exe = tmp_path / "probe"
+    import subprocess
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    want = [C.sizeof(tx.CConfig), C.sizeof(tx.CRatings), C.sizeof(tx.CSchedule), C.sizeof(tx.CEpochStats),
+            tx.CConfig.beta.offset, tx.CConfig.th_init.offset, tx.CConfig.seq_epochs.offset]
+    assert got == want
 
 
 @pytest.mark.parametrize("seed,sigma", [(42, 0.005), (8003, 0.0), (2400, 1.0)])
