@@ -16,6 +16,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(PKG, "libtsvd_tx.so")
+DRIVER = os.path.join(PKG, "tsvdx4-new")
 OBJ = os.path.join(PKG, "_obj")
 
 ARCH = os.environ.get("TSVD_OFFLOAD_ARCH", "gfx950")
@@ -73,6 +74,13 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread"])
     os.replace(tmp, LIB)
+    # job-level driver (plain C++ over the C ABI), rpath to the library
+    tmp = DRIVER + ".tmp"
+    dt = _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{INCLUDE}", f"-I{CSRC}", os.path.join(CSRC, "tsvdx4_new.cpp"),
+               "-o", tmp, f"-L{PKG}", "-ltsvd_tx", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, DRIVER)
+    if verbose:
+        print(f"tsvdx4_new.cpp {dt:.1f}s")
     return LIB
 
 
