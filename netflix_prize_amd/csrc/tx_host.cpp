@@ -213,8 +213,9 @@ tx::Hyper make_hyper(const tx_config& c) {
 
 // Upload a dense [rows, f] host table into a zero-padded [rows, fs] device table.
 int upload_padded(tx_ctx* c, float** dst, const float* src, int64_t rows) {
-    if (int rc = c->alloc(dst, static_cast<size_t>(rows) * c->fs)) return rc;
-    HIPC(hipMemset(*dst, 0, std::max<int64_t>(rows, 1) * c->fs * sizeof(float)));
+    const int64_t alloc_rows = std::max<int64_t>(rows, 1);
+    if (int rc = c->alloc(dst, static_cast<size_t>(alloc_rows) * c->fs)) return rc;
+    HIPC(hipMemset(*dst, 0, alloc_rows * c->fs * sizeof(float)));
     if (rows > 0 && c->f > 0 && src)
         HIPC(hipMemcpy2D(*dst, c->fs * sizeof(float), src, c->f * sizeof(float), c->f * sizeof(float), rows,
                          hipMemcpyHostToDevice));
@@ -924,12 +925,14 @@ int run_host(tx_ctx* c, int mode, int64_t n, const int32_t* u, const int32_t* i,
     if (int rc = check_queries(c, n, u, i)) return rc;
     const int64_t CH = int64_t{1} << 22;
     const int64_t chunk = std::min<int64_t>(std::max<int64_t>(n, 1), CH);
-    const size_t per = sizeof(int32_t) * 2 + sizeof(int16_t) + sizeof(float) * 3 + sizeof(int8_t) + 16;
-    if (int rc = c->stage(static_cast<size_t>(chunk) * per + 256)) return rc;
+    auto rounded = [](size_t bytes) { return (bytes + 255) / 256 * 256; };
+    const size_t ch = static_cast<size_t>(chunk);
+    const size_t need = rounded(ch * 4) * 2 + rounded(ch * 2) + rounded(ch * 4) * 3 + rounded(ch);
+    if (int rc = c->stage(need)) return rc;
     char* base = static_cast<char*>(c->d_stage);
     auto carve = [&](size_t bytes) {
         char* p = base;
-        base += (bytes + 255) / 256 * 256;
+        base += rounded(bytes);
         return p;
     };
     int32_t* du = reinterpret_cast<int32_t*>(carve(chunk * 4));

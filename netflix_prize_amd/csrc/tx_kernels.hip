@@ -736,6 +736,7 @@ static inline int grid_for(int64_t teams, int team) {
 
 hipError_t launch_sgd_serial(const DevModel& m, const int64_t* order, float* scratch, hipStream_t st) {
     const size_t shm = sizeof(float) * (static_cast<size_t>(m.fs) + 1);
+    (void)hipGetLastError();
     hipLaunchKernelGGL(k_sgd_serial, dim3(1), dim3(64), shm, st, m, order, scratch);
     return hipGetLastError();
 }
@@ -743,6 +744,7 @@ hipError_t launch_sgd_serial(const DevModel& m, const int64_t* order, float* scr
 template <int TEAM, int VEC>
 static hipError_t sgd_par_tv(const DevModel& m, const StrataView& sv, int stratum, int blocks, bool qf, bool p2,
                              bool ord, hipStream_t st) {
+    (void)hipGetLastError();
 #define TX_PAR(QF, P2, ORD) \
     hipLaunchKernelGGL((k_sgd_par<TEAM, VEC, QF, P2, ORD>), dim3(blocks), dim3(256), 0, st, m, sv, stratum)
     if (ord) {
@@ -768,6 +770,7 @@ hipError_t launch_sgd_par(const DevModel& m, const StrataView& sv, int stratum, 
 hipError_t launch_rebuild(const DevModel& m, int64_t n_ud, int team, int vec, hipStream_t st) {
 #define X(T, V)                                                                                           \
     if (team == T && vec == V) {                                                                           \
+        (void)hipGetLastError();                                                                           \
         hipLaunchKernelGGL((k_rebuild_ycache<T, V>), dim3(grid_for(m.U, T)), dim3(256), 0, st, m);         \
         if (!m.low_mem && n_ud > 0)                                                                        \
             hipLaunchKernelGGL((k_rebuild_ycache_day<T, V>), dim3(grid_for(n_ud, T)), dim3(256), 0, st, m, \
@@ -786,6 +789,7 @@ hipError_t launch_predict(const DevModel& m, int mode, int64_t n, const int32_t*
 #define X(T, V)                                                                                            \
     if (team == T && vec == V) {                                                                            \
         const int g = grid_for(n, T);                                                                       \
+        (void)hipGetLastError();                                                                            \
         if (mode == 0)                                                                                      \
             hipLaunchKernelGGL((k_predict<T, V, 0>), dim3(g), dim3(256), 0, st, m, n, us, is, ds, o0, o1, o2, \
                                target, is_test, out_d);                                                     \
@@ -804,18 +808,21 @@ hipError_t launch_predict(const DevModel& m, int mode, int64_t n, const int32_t*
 
 hipError_t launch_fill_f32(float* p, float v, int64_t n, hipStream_t st) {
     if (n <= 0) return hipSuccess;
+    (void)hipGetLastError();
     hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(n, 1) > 4096 ? 4096 : grid_for(n, 1)), dim3(256), 0, st, p, v, n);
     return hipGetLastError();
 }
 
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st) {
     if (n <= 0) return hipSuccess;
+    (void)hipGetLastError();
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 1) > 4096 ? 4096 : grid_for(n, 1)), dim3(256), 0, st, p, v, n);
     return hipGetLastError();
 }
 
 hipError_t launch_fill_f64(double* p, double v, int64_t n, hipStream_t st) {
     if (n <= 0) return hipSuccess;
+    (void)hipGetLastError();
     hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(64), 0, st, p, v, n);
     return hipGetLastError();
 }

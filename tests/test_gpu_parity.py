@@ -62,7 +62,8 @@ def test_serial_schedule_matches_oracle(gpu, low_memory):
         m_o.fit_epoch(e)
         m_g.fit_epoch(epoch=e)
     _assert_params_equal(m_o, m_g)
-    assert m_g.calc_rmse(pr) == pytest.approx(m_o.calc_rmse(), abs=1e-9)
+    # parameters are bit-identical; predict reduces the dot product across lanes
+    assert m_g.calc_rmse(pr) == pytest.approx(m_o.calc_rmse(), abs=1e-7)
 
 
 def test_predict_matches_oracle_and_fp64(gpu):
