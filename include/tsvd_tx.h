@@ -90,8 +90,20 @@ typedef struct tx_schedule {
     int32_t teams_per_block; /* STRATA only: 0 = tuned default */
     int32_t blocks;          /* STRATA only: persistent grid size, 0 = auto */
     int32_t use_graph;       /* capture each epoch's launches in a hipGraph */
-    int32_t reserved[3];
+    int32_t update_flags;    /* STRATA only: how shared rows are updated, see below */
+    int32_t reserved[2];
 } tx_schedule;
+
+/* update_flags: bits 0-1 item rows/biases (q, qf, b_i, b_i,bin, b_i,f),
+ * bits 2-3 NSVD1 rows (y, y-bias):
+ *   0 = device-scope atomic add of the SGD delta,
+ *   1 = XCD-local atomic add (coherent only when one XCD owns the rows,
+ *       i.e. item rows with n_strata >= 8),
+ *   2 = plain read-modify-write (lock-free "Hogwild" updates).
+ * Default (tx_create): atomic item rows, plain NSVD1 rows. */
+#define TX_UPD_ATOMIC 0
+#define TX_UPD_XCD_ATOMIC 1
+#define TX_UPD_PLAIN 2
 
 /* Per-epoch device timing (hipEvents on the library's stream). */
 typedef struct tx_epoch_stats {
@@ -142,7 +154,8 @@ int tx_create(const tx_config* cfg, const tx_ratings* train, const tx_ratings* h
 /* Drop the model and its GPU memory. */
 int tx_destroy(tx_ctx* ctx);
 
-/* Choose the training schedule (default: TX_SCHED_STRATA, n_strata = 1). */
+/* Choose the training schedule (default: TX_SCHED_STRATA, n_strata = 1,
+ * update_flags = TX_UPD_ATOMIC | TX_UPD_PLAIN << 2). */
 int tx_set_schedule(tx_ctx* ctx, const tx_schedule* sched);
 
 /* Regressor::fit_epoch (src/lib.rs:630, src/tx.rs:734-1115); epoch is

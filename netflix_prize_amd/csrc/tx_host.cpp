@@ -116,7 +116,8 @@ struct tx_ctx {
     std::vector<uint8_t> t_bin;
     std::vector<int8_t> t_raw;
 
-    tx_schedule sched{TX_SCHED_STRATA, 1, 0, 0, 0, {0, 0, 0}};
+    // default: one stratum, atomic item updates, lock-free NSVD1 row updates
+    tx_schedule sched{TX_SCHED_STRATA, 1, 0, 0, 0, TX_UPD_ATOMIC | (TX_UPD_PLAIN << 2), {0, 0}};
     int built_S = 0;
     tx::StrataView sv{};
     std::vector<void*> strata_allocs;
@@ -350,6 +351,8 @@ int build_strata(tx_ctx* c, int S) {
     auto* L = &c->strata_allocs;
     tx::StrataView& sv = c->sv;
     sv = tx::StrataView{};
+    sv.q_mode = c->sched.update_flags & 3;
+    sv.y_mode = (c->sched.update_flags >> 2) & 3;
     sv.n_strata = S;
     sv.n_cells = C;
     uint32_t *d_seg, *d_qu, *d_qs, *d_qh;
@@ -789,11 +792,15 @@ int tx_set_schedule(tx_ctx* c, const tx_schedule* s) {
         if (s->blocks < 0 || (s->blocks > 0 && s->blocks < s->n_strata))
             return fail(TX_EINVAL, "blocks must be 0 (auto) or at least n_strata");
     }
+    const int qm = s->update_flags & 3, ym = (s->update_flags >> 2) & 3;
+    if (qm > 2 || ym > 2 || (s->update_flags >> 4) != 0) return fail(TX_EINVAL, "bad update_flags %d", s->update_flags);
     const bool relayout = s->mode == TX_SCHED_STRATA && s->n_strata != c->built_S;
     c->sched = *s;
     if (relayout) {
         if (int rc = build_strata(c, s->n_strata)) return rc;
     }
+    c->sv.q_mode = qm;
+    c->sv.y_mode = ym;
     if (c->graph) {
         (void)hipGraphExecDestroy(c->graph);
         c->graph = nullptr;

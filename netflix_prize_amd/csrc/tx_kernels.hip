@@ -69,6 +69,20 @@ __device__ __forceinline__ void store_vec(float* __restrict__ p, const float (&i
     }
 }
 
+// Strided element mapping: lane l of a team holds elements l, l + TEAM, ...,
+// so each per-element instruction of a team touches TEAM consecutive floats.
+template <int VEC, int TEAM>
+__device__ __forceinline__ void load_str(const float* __restrict__ p, float (&out)[VEC]) {
+#pragma unroll
+    for (int c = 0; c < VEC; ++c) out[c] = p[c * TEAM];
+}
+
+template <int VEC, int TEAM>
+__device__ __forceinline__ void store_str(float* __restrict__ p, const float (&in)[VEC]) {
+#pragma unroll
+    for (int c = 0; c < VEC; ++c) p[c * TEAM] = in[c];
+}
+
 __device__ __forceinline__ void atomic_add_dev(float* p, float v) {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -328,13 +342,66 @@ __global__ __launch_bounds__(64) void k_sgd_serial(DevModel m, const int64_t* __
 // ---------------------------------------------------------------------------
 // Parallel schedule.  One team of TEAM lanes owns one user segment; lane l
 // holds factor elements [l*VEC, l*VEC + VEC).  Users of a cell are pulled from
-// a per-(stratum, cell) queue in chunks; cell = block index mod n_cells, so a
-// cell's workgroups share one XCD under the round-robin workgroup dispatch.
+// a per-(stratum, cell) queue in chunks; cell = block index mod n_cells, and
+// workgroups are dispatched to the 8 XCDs round-robin, so with n_cells = 8 a
+// cell's workgroups all run on one XCD and share its L2.
 // ---------------------------------------------------------------------------
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// Sum over the TEAM lanes of a team; every lane of the team gets the result.
+// Within a 16-lane row: quad_perm [1,0,3,2], quad_perm [2,3,0,1],
+// row_half_mirror, row_mirror (each lane ends with the row sum); across rows
+// one bpermute (TEAM = 32) or four readlanes (TEAM = 64).
+template <int TEAM>
+__device__ __forceinline__ float team_sum_dpp(float v) {
+    if constexpr (TEAM >= 2) v += dpp_f<0xB1>(v);
+    if constexpr (TEAM >= 4) v += dpp_f<0x4E>(v);
+    if constexpr (TEAM >= 8) v += dpp_f<0x141>(v);
+    if constexpr (TEAM >= 16) v += dpp_f<0x140>(v);
+    if constexpr (TEAM == 32) v += __shfl_xor(v, 16, 32);
+    if constexpr (TEAM == 64) {
+        const int b = __float_as_int(v);
+        v = (__int_as_float(__builtin_amdgcn_readlane(b, 0)) + __int_as_float(__builtin_amdgcn_readlane(b, 16))) +
+            (__int_as_float(__builtin_amdgcn_readlane(b, 32)) + __int_as_float(__builtin_amdgcn_readlane(b, 48)));
+    }
+    return v;
+}
+
+// Apply an SGD delta to a shared row element (see tx_schedule.update_flags).
+__device__ __forceinline__ void apply_delta(float* p, float old, float delta, int mode) {
+    if (mode == 0) __hip_atomic_fetch_add(p, delta, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else if (mode == 1) __hip_atomic_fetch_add(p, delta, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    else *p = old + delta;
+}
+
+template <int VEC, int TEAM>
+__device__ __forceinline__ void apply_vec(float* p, const float (&old)[VEC], const float (&delta)[VEC], int mode) {
+#pragma unroll
+    for (int c = 0; c < VEC; ++c) apply_delta(p + c * TEAM, old[c], delta[c], mode);
+}
+
+// The next training rating and everything read for it before its update.
+template <int VEC>
+struct Pending {
+    uint32_t i, ud;
+    float dev, r;
+    int b, raw;
+    float q[VEC], qf[VEC];
+    float ib, bt, ibf, cut, but, bud;
+};
+
+#ifndef TX_PREFETCH
+#define TX_PREFETCH 2
+#endif
 
 template <int TEAM, int VEC, bool QF, bool P2, bool ORD>
 __global__ __launch_bounds__(256) void k_sgd_par(DevModel m, StrataView sv, int stratum) {
     constexpr int CHUNK = 4;
+    constexpr int DEPTH = TX_PREFETCH;
     const int lane = threadIdx.x % TEAM;
     const int S = sv.n_strata;
     const int cell = sv.n_cells > 1 ? static_cast<int>(blockIdx.x % sv.n_cells) : 0;
@@ -343,9 +410,10 @@ __global__ __launch_bounds__(256) void k_sgd_par(DevModel m, StrataView sv, int 
     const uint32_t qlen = sv.q_start[qid + 1] - qbeg;
     const int fs = m.fs, F = m.F, B = m.B;
     const Hyper h = m.h;
-    const int off = lane * VEC;
+    const int off = lane;  // strided: element c*TEAM + lane
     const bool first = (stratum == 0);
     const bool last = (stratum == S - 1);
+    const int qm = sv.q_mode, ym = sv.y_mode;
 
     float th[4] = {0.f, 0.f, 0.f, 0.f};
     if constexpr (ORD) {
@@ -367,11 +435,40 @@ __global__ __launch_bounds__(256) void k_sgd_par(DevModel m, StrataView sv, int 
             const size_t prow = static_cast<size_t>(u) * fs + off;
             const uint32_t ts = m.t_off[u], te = m.t_off[u + 1];
             const float norm = m.norm_train[u];
+            const size_t ubrow = static_cast<size_t>(u) * B;
+
+            auto fetch = [&](uint32_t t, Pending<VEC>& n) {
+                n.i = sv.p_item[t];
+                n.ud = sv.p_ud[t];
+                n.dev = sv.p_dev[t];
+                n.r = sv.p_r[t];
+                n.b = sv.p_bin[t];
+                n.raw = ORD ? static_cast<int>(sv.p_raw[t]) : 0;
+                load_str<VEC, TEAM>(m.ifeat + static_cast<size_t>(n.i) * fs + off, n.q);
+                if constexpr (QF) {
+                    load_str<VEC, TEAM>(m.ifeat_freq + static_cast<size_t>(n.i) * F * fs + off, n.qf);
+                } else {
+#pragma unroll
+                    for (int c = 0; c < VEC; ++c) n.qf[c] = 0.0f;
+                }
+                n.ib = m.ibias[n.i];
+                n.bt = m.bit_bin[static_cast<size_t>(n.i) * B + n.b];
+                n.ibf = m.ibias_freq[static_cast<size_t>(n.i) * F];
+                n.cut = m.cut[n.ud];
+                n.but = m.but_bin[ubrow + n.b];
+                n.bud = m.ubias_day[n.ud];
+            };
+            // ring of the next DEPTH ratings, loaded before the current rating's
+            // atomics so their latency overlaps DEPTH ratings of compute
+            Pending<VEC> ring[DEPTH];
+#pragma unroll
+            for (int k = 0; k < DEPTH; ++k)
+                if (s0 + k < s1) fetch(s0 + k, ring[k]);
 
             float p[VEC], p2[VEC], s[VEC], G[VEC];
-            load_vec<VEC>(m.ufeat + prow, p);
+            load_str<VEC, TEAM>(m.ufeat + prow, p);
             if constexpr (P2) {
-                load_vec<VEC>(m.ufeat2 + prow, p2);
+                load_str<VEC, TEAM>(m.ufeat2 + prow, p2);
             } else {
 #pragma unroll
                 for (int c = 0; c < VEC; ++c) p2[c] = 0.0f;
@@ -379,110 +476,117 @@ __global__ __launch_bounds__(256) void k_sgd_par(DevModel m, StrataView sv, int 
             float su_b;
             if (first) {
                 // NSVD1 user aggregation from the current y (src/tx.rs:801-822)
-#pragma unroll
-                for (int c = 0; c < VEC; ++c) s[c] = 0.0f;
+                float acc[4][VEC];
                 float sb = 0.0f;
-                for (uint32_t t = ts; t < te; ++t) {
-                    const uint32_t j = m.t_item[t];
-                    float y[VEC];
-                    load_vec<VEC>(m.yfeat + static_cast<size_t>(j) * fs + off, y);
 #pragma unroll
-                    for (int c = 0; c < VEC; ++c) s[c] += y[c];
-                    if (lane == (t & (TEAM - 1))) sb += m.ybias[j];
-                }
-                if (m.full_su) {
-                    for (uint32_t q = m.h_start[u]; q < m.h_start[u + 1]; ++q) {
-                        const uint32_t j = m.h_item[q];
-                        float y[VEC];
-                        load_vec<VEC>(m.yfeat + static_cast<size_t>(j) * fs + off, y);
+                for (int k = 0; k < 4; ++k)
 #pragma unroll
-                        for (int c = 0; c < VEC; ++c) s[c] += y[c];
-                        if (lane == (q & (TEAM - 1))) sb += m.ybias[j];
+                    for (int c = 0; c < VEC; ++c) acc[k][c] = 0.0f;
+                const uint32_t nh = m.full_su ? m.h_start[u + 1] - m.h_start[u] : 0;
+                const uint32_t ntot = (te - ts) + nh;
+                const uint32_t* hitems = m.h_item + m.h_start[u];
+                for (uint32_t k0 = 0; k0 < ntot; k0 += 4) {
+                    float y[4][VEC];
+                    uint32_t jj[4];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        const uint32_t kk = k0 + k;
+                        jj[k] = kk < ntot ? (kk < te - ts ? m.t_item[ts + kk] : hitems[kk - (te - ts)]) : 0xffffffffu;
+                        if (jj[k] != 0xffffffffu) load_str<VEC, TEAM>(m.yfeat + static_cast<size_t>(jj[k]) * fs + off, y[k]);
+                    }
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        if (jj[k] == 0xffffffffu) continue;
+#pragma unroll
+                        for (int c = 0; c < VEC; ++c) acc[k][c] += y[k][c];
+                        if (lane == ((k0 + k) & (TEAM - 1))) sb += m.ybias[jj[k]];
                     }
                 }
                 const float inv = 1.0f / norm;
 #pragma unroll
-                for (int c = 0; c < VEC; ++c) s[c] *= inv;
-                su_b = team_sum<TEAM>(sb) * inv;
+                for (int c = 0; c < VEC; ++c) s[c] = ((acc[0][c] + acc[1][c]) + (acc[2][c] + acc[3][c])) * inv;
+                su_b = team_sum_dpp<TEAM>(sb) * inv;
                 if (S > 1) {
-                    store_vec<VEC>(m.ycache + prow, s);
+                    store_str<VEC, TEAM>(m.ycache + prow, s);
                     if (lane == 0) m.ycache_bias[u] = su_b;
                 }
 #pragma unroll
                 for (int c = 0; c < VEC; ++c) G[c] = 0.0f;
             } else {
-                load_vec<VEC>(m.ycache + prow, s);
+                load_str<VEC, TEAM>(m.ycache + prow, s);
                 su_b = m.ycache_bias[u];
-                load_vec<VEC>(sv.gacc + prow, G);
+                load_str<VEC, TEAM>(sv.gacc + prow, G);
             }
             float Gb = first ? 0.0f : sv.gacc_b[u];
             float ubias = m.ubias[u], alpha = m.alpha_u[u], cu = m.cu[u];
 
-            for (uint32_t t = s0; t < s1; ++t) {
-                const uint32_t i = sv.p_item[t];
-                const uint32_t ud = sv.p_ud[t];
-                const float dev = sv.p_dev[t];
-                const float r = sv.p_r[t];
-                const int b = sv.p_bin[t];
-                const size_t qrow = static_cast<size_t>(i) * fs + off;
-                const size_t qfrow = static_cast<size_t>(i) * F * fs + off;  // freq bin 0 (low-memory training)
-                float q[VEC], qf[VEC];
-                load_vec<VEC>(m.ifeat + qrow, q);
-                if constexpr (QF) {
-                    load_vec<VEC>(m.ifeat_freq + qfrow, qf);
-                } else {
+            for (uint32_t t0 = s0; t0 < s1; t0 += DEPTH) {
 #pragma unroll
-                    for (int c = 0; c < VEC; ++c) qf[c] = 0.0f;
-                }
-                float* bb = &m.but_bin[static_cast<size_t>(u) * B + b];
-                const float but = *bb;
-                const float bud = m.ubias_day[ud];
-                const float bi_t = m.ibias[i] + m.bit_bin[static_cast<size_t>(i) * B + b]
-                                   + m.ibias_freq[static_cast<size_t>(i) * F];
-                const float cu_t = cu + m.cut[ud];
-                const float bu_t = ubias + but + alpha * dev + bud + su_b;
+                for (int k = 0; k < DEPTH; ++k) {
+                const uint32_t t = t0 + k;
+                if (t >= s1) break;
+                const Pending<VEC> cur = ring[k];
+                if (t + DEPTH < s1) fetch(t + DEPTH, ring[k]);
+                const uint32_t i = cur.i;
+                const float dev = cur.dev;
+                const float bi_t = cur.ib + cur.bt + cur.ibf;
+                const float cu_t = cu + cur.cut;
+                const float bu_t = ubias + cur.but + alpha * dev + cur.bud + su_b;
 
                 float part = 0.0f;
 #pragma unroll
-                for (int c = 0; c < VEC; ++c) part += (q[c] + qf[c]) * (p[c] + s[c] + dev * p2[c]);
-                const float dot = team_sum<TEAM>(part);
+                for (int c = 0; c < VEC; ++c) part += (cur.q[c] + cur.qf[c]) * (p[c] + s[c] + dev * p2[c]);
+                const float dot = team_sum_dpp<TEAM>(part);
                 const float score = m.gbias + bu_t + bi_t * cu_t + dot;
                 float err;
                 if constexpr (ORD) {
                     float g_t[4];
-                    err = ordinal_grad(th, score, static_cast<int>(sv.p_raw[t]), g_t);
+                    err = ordinal_grad(th, score, cur.raw, g_t);
                 } else {
-                    err = score - r;
+                    err = score - cur.r;
                 }
                 const float err_b = err * h.w_bias;
                 const float err_d = err * h.w_factor;
 
+                // user-owned biases: registers, or written back now and forwarded
+                // to every prefetched rating that reads the same slot
                 ubias -= h.lr_ub * err_b;
                 alpha -= h.lr_ta * err_b * dev;
+                const float but_new = cur.but - h.lr_tu * err_b;
+                const float bud_new = cur.bud - h.lr_ubd * err_b;
+#pragma unroll
+                for (int j = 0; j < DEPTH; ++j) {
+                    if (ring[j].b == cur.b) ring[j].but = but_new;
+                    if (ring[j].ud == cur.ud) ring[j].bud = bud_new;
+                }
                 if (lane == 0) {
-                    *bb = but - h.lr_tu * err_b;
-                    m.ubias_day[ud] = bud - h.lr_ubd * err_b;
+                    m.but_bin[ubrow + cur.b] = but_new;
+                    m.ubias_day[cur.ud] = bud_new;
                     const float gi = err_b * cu_t;
-                    atomic_add_dev(&m.ibias[i], -h.lr_ib * gi);
-                    atomic_add_dev(&m.bit_bin[static_cast<size_t>(i) * B + b], -h.lr_ti * gi);
-                    atomic_add_dev(&m.ibias_freq[static_cast<size_t>(i) * F], -h.lr_ibf * gi);
+                    apply_delta(&m.ibias[i], cur.ib, -h.lr_ib * gi, qm);
+                    apply_delta(&m.bit_bin[static_cast<size_t>(i) * B + cur.b], cur.bt, -h.lr_ti * gi, qm);
+                    apply_delta(&m.ibias_freq[static_cast<size_t>(i) * F], cur.ibf, -h.lr_ibf * gi, qm);
                 }
                 cu -= h.lr_cu * (err_b * bi_t + h.reg_cu * (cu - 1.0f));
                 Gb += m.sum_err_bug ? err_b * static_cast<float>(m.f) : err_b;
+                float dq[VEC], dqf[VEC];
 #pragma unroll
                 for (int c = 0; c < VEC; ++c) {
-                    const float qe = q[c] + qf[c];
+                    const float qe = cur.q[c] + cur.qf[c];
                     G[c] += err_d * qe;
                     const float pe = p[c] + dev * p2[c] + s[c];
                     p[c] -= h.lr_u * (err_d * qe + h.reg_u * p[c]);
                     if constexpr (P2) p2[c] -= h.lr_u2 * (err_d * qe * dev + h.reg_u2 * p2[c]);
-                    atomic_add_dev(&m.ifeat[qrow + c], -h.lr_i * (err_d * pe + h.reg_i * q[c]));
-                    if constexpr (QF) atomic_add_dev(&m.ifeat_freq[qfrow + c], -h.lr_iqf * (err_d * pe + h.reg_iqf * qf[c]));
+                    dq[c] = -h.lr_i * (err_d * pe + h.reg_i * cur.q[c]);
+                    if constexpr (QF) dqf[c] = -h.lr_iqf * (err_d * pe + h.reg_iqf * cur.qf[c]);
+                }
+                apply_vec<VEC, TEAM>(m.ifeat + static_cast<size_t>(i) * fs + off, cur.q, dq, qm);
+                if constexpr (QF) apply_vec<VEC, TEAM>(m.ifeat_freq + static_cast<size_t>(i) * F * fs + off, cur.qf, dqf, qm);
                 }
             }
 
-            store_vec<VEC>(m.ufeat + prow, p);
-            if constexpr (P2) store_vec<VEC>(m.ufeat2 + prow, p2);
+            store_str<VEC, TEAM>(m.ufeat + prow, p);
+            if constexpr (P2) store_str<VEC, TEAM>(m.ufeat2 + prow, p2);
             if (lane == 0) {
                 m.ubias[u] = ubias;
                 m.alpha_u[u] = alpha;
@@ -491,18 +595,42 @@ __global__ __launch_bounds__(256) void k_sgd_par(DevModel m, StrataView sv, int 
             if (last) {
                 // end-of-user NSVD1 step over R(u) (src/tx.rs:1099-1106)
                 const float inv = 1.0f / norm;
-                const float gb = h.lr_yb * (Gb * inv);
-                for (uint32_t t = ts; t < te; ++t) {
-                    const uint32_t j = m.t_item[t];
-                    float* yr = m.yfeat + static_cast<size_t>(j) * fs + off;
-                    float y[VEC];
-                    load_vec<VEC>(yr, y);
+                float gq[VEC];
 #pragma unroll
-                    for (int c = 0; c < VEC; ++c) atomic_add_dev(yr + c, -h.lr_y * (G[c] * inv + h.reg_y * y[c]));
-                    if (lane == (t & (TEAM - 1)) && gb != 0.0f) atomic_add_dev(&m.ybias[j], -gb);
+                for (int c = 0; c < VEC; ++c) gq[c] = G[c] * inv;
+                const float gb = h.lr_yb * (Gb * inv);
+                // rows of batch b+1 are loaded before batch b's updates are issued
+                float ya[4][VEC], yb[4][VEC];
+                uint32_t ja[4], jb[4];
+                auto load_batch = [&](uint32_t t0, float (&y)[4][VEC], uint32_t (&jj)[4]) {
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        jj[k] = t0 + k < te ? m.t_item[t0 + k] : 0xffffffffu;
+                        if (jj[k] != 0xffffffffu) load_str<VEC, TEAM>(m.yfeat + static_cast<size_t>(jj[k]) * fs + off, y[k]);
+                    }
+                };
+                auto update_batch = [&](uint32_t t0, const float (&y)[4][VEC], const uint32_t (&jj)[4]) {
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        if (jj[k] == 0xffffffffu) continue;
+                        float dy[VEC];
+#pragma unroll
+                        for (int c = 0; c < VEC; ++c) dy[c] = -h.lr_y * (gq[c] + h.reg_y * y[k][c]);
+                        apply_vec<VEC, TEAM>(m.yfeat + static_cast<size_t>(jj[k]) * fs + off, y[k], dy, ym);
+                        if (gb != 0.0f && lane == ((t0 + k) & (TEAM - 1)))
+                            apply_delta(&m.ybias[jj[k]], m.ybias[jj[k]], -gb, ym);
+                    }
+                };
+                load_batch(ts, ya, ja);
+                for (uint32_t t0 = ts; t0 < te; t0 += 8) {
+                    if (t0 + 4 < te) load_batch(t0 + 4, yb, jb);
+                    update_batch(t0, ya, ja);
+                    if (t0 + 4 >= te) break;
+                    if (t0 + 8 < te) load_batch(t0 + 8, ya, ja);
+                    update_batch(t0 + 4, yb, jb);
                 }
             } else {
-                store_vec<VEC>(sv.gacc + prow, G);
+                store_str<VEC, TEAM>(sv.gacc + prow, G);
                 if (lane == 0) sv.gacc_b[u] = Gb;
             }
         }

@@ -77,6 +77,8 @@ struct StrataView {
     uint32_t* q_head;           // [n_strata * n_cells] work counters (reset per epoch)
     float* gacc;                // [U * fs] per-user sum_err_q across strata
     float* gacc_b;              // [U]      per-user sum_err across strata
+    int32_t q_mode;             // item-row update: 0 device atomic, 1 XCD-local atomic, 2 plain
+    int32_t y_mode;             // NSVD1-row update, same codes
 };
 
 }  // namespace tx
