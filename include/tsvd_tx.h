@@ -91,7 +91,9 @@ typedef struct tx_schedule {
     int32_t blocks;          /* STRATA only: persistent grid size, 0 = auto */
     int32_t use_graph;       /* capture each epoch's launches in a hipGraph */
     int32_t update_flags;    /* STRATA only: how shared rows are updated, see below */
-    int32_t reserved[2];
+    int32_t user_order;      /* STRATA only: 0 = the reference's per-epoch shuffled user
+                                order (get_users), 1 = longest user first */
+    int32_t reserved[1];
 } tx_schedule;
 
 /* update_flags: bits 0-1 item rows/biases (q, qf, b_i, b_i,bin, b_i,f),

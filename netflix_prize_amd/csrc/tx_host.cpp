@@ -115,15 +115,17 @@ struct tx_ctx {
     std::vector<float> t_dev, t_r;
     std::vector<uint8_t> t_bin;
     std::vector<int8_t> t_raw;
+    uint4* d_rec = nullptr;  // packed records, reference order
 
     // default: one stratum, atomic item updates, lock-free NSVD1 row updates
-    tx_schedule sched{TX_SCHED_STRATA, 1, 0, 0, 0, TX_UPD_ATOMIC | (TX_UPD_PLAIN << 2), {0, 0}};
+    tx_schedule sched{TX_SCHED_STRATA, 1, 0, 0, 0, TX_UPD_ATOMIC | (TX_UPD_PLAIN << 2), 0, {0}};
     int built_S = 0;
     tx::StrataView sv{};
     std::vector<void*> strata_allocs;
     hipGraphExec_t graph = nullptr;
     int graph_S = 0;
 
+    std::vector<uint32_t> h_queue;  // pinned order of the current epoch's queue
     int64_t* d_order = nullptr;
     float* d_scratch = nullptr;
     double* d_acc = nullptr;
@@ -267,6 +269,17 @@ ParamDesc param_desc(tx_ctx* c, int id) {
     }
 }
 
+// Packed training record (see StrataView::p_rec).
+inline uint4 pack_rec(const tx_ctx* c, int64_t t) {
+    uint4 r;
+    const uint32_t raw = c->t_raw.empty() ? 0u : static_cast<uint32_t>(c->t_raw[t]) & 15u;
+    r.x = c->t_item[t] | (raw << 20) | (static_cast<uint32_t>(c->t_bin[t]) << 24);
+    r.y = c->t_ud[t];
+    std::memcpy(&r.z, &c->t_dev[t], 4);
+    std::memcpy(&r.w, &c->t_r[t], 4);
+    return r;
+}
+
 // Build the stratified layout: item blocks and user blocks balanced by rating
 // count (largest first to the lightest block), segments of each user's
 // ratings per item block in the rotation order of its block, and per
@@ -365,39 +378,20 @@ int build_strata(tx_ctx* c, int S) {
     sv.q_start = d_qs;
     sv.q_head = d_qh;
     if (S == 1) {
-        sv.p_item = c->dm.t_item;
-        sv.p_ud = c->dm.t_ud;
-        sv.p_dev = c->dm.t_dev;
-        sv.p_r = c->dm.t_r;
-        sv.p_bin = c->dm.t_bin;
-        sv.p_raw = c->dm.t_raw;
+        sv.p_rec = c->d_rec;
         sv.gacc = nullptr;
         sv.gacc_b = nullptr;
     } else {
-        std::vector<uint32_t> a(N), b(N);
-        std::vector<float> d(N), r(N);
-        std::vector<uint8_t> bn(N);
-        std::vector<int8_t> rw(N);
+        std::vector<uint4> rec(N);
         parallel_for(N, [&](int64_t lo, int64_t hi) {
-            for (int64_t k = lo; k < hi; ++k) {
-                const uint32_t t = perm[k];
-                a[k] = c->t_item[t]; b[k] = c->t_ud[t]; d[k] = c->t_dev[t]; r[k] = c->t_r[t];
-                bn[k] = c->t_bin[t]; rw[k] = c->t_raw.empty() ? 0 : c->t_raw[t];
-            }
+            for (int64_t k = lo; k < hi; ++k) rec[k] = pack_rec(c, perm[k]);
         });
-        uint32_t *pa, *pb;
-        float *pd, *pr, *ga, *gb;
-        uint8_t* pbn;
-        int8_t* prw;
-        if (int rc = c->upload(&pa, a.data(), a.size(), L)) return rc;
-        if (int rc = c->upload(&pb, b.data(), b.size(), L)) return rc;
-        if (int rc = c->upload(&pd, d.data(), d.size(), L)) return rc;
-        if (int rc = c->upload(&pr, r.data(), r.size(), L)) return rc;
-        if (int rc = c->upload(&pbn, bn.data(), bn.size(), L)) return rc;
-        if (int rc = c->upload(&prw, rw.data(), rw.size(), L)) return rc;
+        uint4* pr;
+        float *ga, *gb;
+        if (int rc = c->upload(&pr, rec.data(), rec.size(), L)) return rc;
         if (int rc = c->alloc(&ga, static_cast<size_t>(U) * c->fs, L)) return rc;
         if (int rc = c->alloc(&gb, static_cast<size_t>(U), L)) return rc;
-        sv.p_item = pa; sv.p_ud = pb; sv.p_dev = pd; sv.p_r = pr; sv.p_bin = pbn; sv.p_raw = prw;
+        sv.p_rec = pr;
         sv.gacc = ga;
         sv.gacc_b = gb;
     }
@@ -707,6 +701,9 @@ int tx_create(const tx_config* cfg, const tx_ratings* tr, const tx_ratings* pr, 
     m.full_su = cfg->full_su ? 1 : 0;
     m.has_ord = cfg->has_ordinal ? 1 : 0;
     m.sum_err_bug = cfg->sum_err_bug ? 1 : 0;
+    m.skip_cut = 1;                              // cut is never trained (zero until set)
+    m.skip_ibf = cfg->lr_ibf == 0.0f ? 1 : 0;    // zero-initialised, trained only by lr_ibf
+    m.skip_but = cfg->lr_tu == 0.0f ? 1 : 0;
     m.gbias = gbias;
     m.h = make_hyper(*cfg);
 #define RC(x) \
@@ -753,6 +750,14 @@ int tx_create(const tx_config* cfg, const tx_ratings* tr, const tx_ratings* pr, 
     RC(c->upload(&p8, c->t_bin.data(), c->t_bin.size())); m.t_bin = p8;
     int8_t* pi8;
     RC(c->upload(&pi8, c->t_raw.data(), c->t_raw.size())); m.t_raw = pi8;
+    {
+        if (c->I > (1 << 20)) return fail(TX_EUNSUPPORTED, "tx_create: more than 2^20 items");
+        std::vector<uint4> rec(N);
+        parallel_for(N, [&](int64_t lo, int64_t hi) {
+            for (int64_t k = lo; k < hi; ++k) rec[k] = pack_rec(c, k);
+        });
+        RC(c->upload(&c->d_rec, rec.data(), rec.size()));
+    }
     RC(c->upload(&p32, run_stop.data(), run_stop.size())); m.t_run_stop = p32;
     RC(c->upload(&pf, run_norm.data(), run_norm.size())); m.t_run_norm = pf;
     RC(c->upload(&p32, c->h_start.data(), c->h_start.size())); m.h_start = p32;
@@ -792,6 +797,7 @@ int tx_set_schedule(tx_ctx* c, const tx_schedule* s) {
         if (s->blocks < 0 || (s->blocks > 0 && s->blocks < s->n_strata))
             return fail(TX_EINVAL, "blocks must be 0 (auto) or at least n_strata");
     }
+    if (s->user_order != 0 && s->user_order != 1) return fail(TX_EINVAL, "user_order must be 0 or 1");
     const int qm = s->update_flags & 3, ym = (s->update_flags >> 2) & 3;
     if (qm > 2 || ym > 2 || (s->update_flags >> 4) != 0) return fail(TX_EINVAL, "bad update_flags %d", s->update_flags);
     const bool relayout = s->mode == TX_SCHED_STRATA && s->n_strata != c->built_S;
@@ -844,6 +850,16 @@ int tx_fit_epoch(tx_ctx* c, int32_t epoch) {
     } else {
         if (c->built_S == 0)
             if (int rc = build_strata(c, c->sched.n_strata > 0 ? c->sched.n_strata : 1)) return rc;
+        if (c->built_S == 1 && c->sched.user_order == 0) {
+            // users in the reference's order for this epoch (get_users, src/lib.rs:486-494)
+            std::vector<int64_t> order(c->U);
+            tx::user_order(c->U, c->cfg.shuffle_users != 0, c->cfg.seed, epoch, order.data());
+            c->h_queue.clear();
+            for (int64_t u : order)
+                if (c->t_off[u + 1] > c->t_off[u]) c->h_queue.push_back(static_cast<uint32_t>(u));
+            HIPC(hipMemcpyAsync(const_cast<uint32_t*>(c->sv.q_users), c->h_queue.data(),
+                                c->h_queue.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+        }
         if (c->sched.use_graph) {
             if (!c->graph || c->graph_S != c->built_S) {
                 if (c->graph) (void)hipGraphExecDestroy(c->graph);
@@ -1053,6 +1069,15 @@ int tx_set_param(tx_ctx* c, int32_t id, const float* in, int64_t n) {
                          hipMemcpyHostToDevice));
     else
         HIPC(hipMemcpy(p.p, in, n * sizeof(float), hipMemcpyHostToDevice));
+    // tables the parallel kernel may skip while they are all zero
+    auto all_zero = [&]() {
+        for (int64_t k = 0; k < n; ++k)
+            if (in[k] != 0.0f) return false;
+        return true;
+    };
+    if (id == TX_P_CUT) c->dm.skip_cut = all_zero() ? 1 : 0;
+    if (id == TX_P_IBIAS_FREQ) c->dm.skip_ibf = (c->cfg.lr_ibf == 0.0f && all_zero()) ? 1 : 0;
+    if (id == TX_P_BUT_BIN) c->dm.skip_but = (c->cfg.lr_tu == 0.0f && all_zero()) ? 1 : 0;
     // a captured epoch embeds the model by value: re-capture after a change
     if (c->graph) {
         (void)hipGraphExecDestroy(c->graph);

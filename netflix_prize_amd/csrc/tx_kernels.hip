@@ -371,6 +371,13 @@ __device__ __forceinline__ float team_sum_dpp(float v) {
     return v;
 }
 
+// Value of lane j (team-relative, uniform across the team) to the whole team.
+template <int TEAM>
+__device__ __forceinline__ uint32_t team_bcast(uint32_t v, uint32_t j) {
+    if constexpr (TEAM == 64) return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), static_cast<int>(j)));
+    else return static_cast<uint32_t>(__shfl(static_cast<int>(v), static_cast<int>(j), TEAM));
+}
+
 // Apply an SGD delta to a shared row element (see tx_schedule.update_flags).
 __device__ __forceinline__ void apply_delta(float* p, float old, float delta, int mode) {
     if (mode == 0) __hip_atomic_fetch_add(p, delta, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -398,8 +405,12 @@ struct Pending {
 #define TX_PREFETCH 2
 #endif
 
+#ifndef TX_SGD_WAVES
+#define TX_SGD_WAVES 1
+#endif
+
 template <int TEAM, int VEC, bool QF, bool P2, bool ORD>
-__global__ __launch_bounds__(256) void k_sgd_par(DevModel m, StrataView sv, int stratum) {
+__global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, StrataView sv, int stratum) {
     constexpr int CHUNK = 4;
     constexpr int DEPTH = TX_PREFETCH;
     const int lane = threadIdx.x % TEAM;
@@ -437,13 +448,14 @@ __global__ __launch_bounds__(256) void k_sgd_par(DevModel m, StrataView sv, int 
             const float norm = m.norm_train[u];
             const size_t ubrow = static_cast<size_t>(u) * B;
 
-            auto fetch = [&](uint32_t t, Pending<VEC>& n) {
-                n.i = sv.p_item[t];
-                n.ud = sv.p_ud[t];
-                n.dev = sv.p_dev[t];
-                n.r = sv.p_r[t];
-                n.b = sv.p_bin[t];
-                n.raw = ORD ? static_cast<int>(sv.p_raw[t]) : 0;
+            // issue the loads one rating needs, from its packed record
+            auto fetch = [&](uint4 rc, Pending<VEC>& n) {
+                n.i = rc.x & 0xFFFFFu;
+                n.raw = static_cast<int>((rc.x >> 20) & 15u);
+                n.b = static_cast<int>(rc.x >> 24);
+                n.ud = rc.y;
+                n.dev = __uint_as_float(rc.z);
+                n.r = __uint_as_float(rc.w);
                 load_str<VEC, TEAM>(m.ifeat + static_cast<size_t>(n.i) * fs + off, n.q);
                 if constexpr (QF) {
                     load_str<VEC, TEAM>(m.ifeat_freq + static_cast<size_t>(n.i) * F * fs + off, n.qf);
@@ -453,18 +465,11 @@ __global__ __launch_bounds__(256) void k_sgd_par(DevModel m, StrataView sv, int 
                 }
                 n.ib = m.ibias[n.i];
                 n.bt = m.bit_bin[static_cast<size_t>(n.i) * B + n.b];
-                n.ibf = m.ibias_freq[static_cast<size_t>(n.i) * F];
-                n.cut = m.cut[n.ud];
-                n.but = m.but_bin[ubrow + n.b];
+                n.ibf = m.skip_ibf ? 0.0f : m.ibias_freq[static_cast<size_t>(n.i) * F];
+                n.cut = m.skip_cut ? 0.0f : m.cut[n.ud];
+                n.but = m.skip_but ? 0.0f : m.but_bin[ubrow + n.b];
                 n.bud = m.ubias_day[n.ud];
             };
-            // ring of the next DEPTH ratings, loaded before the current rating's
-            // atomics so their latency overlaps DEPTH ratings of compute
-            Pending<VEC> ring[DEPTH];
-#pragma unroll
-            for (int k = 0; k < DEPTH; ++k)
-                if (s0 + k < s1) fetch(s0 + k, ring[k]);
-
             float p[VEC], p2[VEC], s[VEC], G[VEC];
             load_str<VEC, TEAM>(m.ufeat + prow, p);
             if constexpr (P2) {
@@ -520,13 +525,35 @@ __global__ __launch_bounds__(256) void k_sgd_par(DevModel m, StrataView sv, int 
             float Gb = first ? 0.0f : sv.gacc_b[u];
             float ubias = m.ubias[u], alpha = m.alpha_u[u], cu = m.cu[u];
 
-            for (uint32_t t0 = s0; t0 < s1; t0 += DEPTH) {
+            // records arrive TEAM at a time (lane l holds record tb + l); the
+            // batch after the current one is always in flight
+            auto load_recs = [&](uint32_t tb) {
+                const uint32_t t = tb + static_cast<uint32_t>(lane);
+                return t < s1 ? sv.p_rec[t] : make_uint4(0u, 0u, 0u, 0u);
+            };
+            uint4 cb = load_recs(s0), nb = load_recs(s0 + TEAM);
+            auto rec_at = [&](uint32_t j) {  // j in [0, 2*TEAM) relative to the current batch
+                const uint4 src = j < TEAM ? cb : nb;
+                const uint32_t l = j < TEAM ? j : j - TEAM;
+                return make_uint4(team_bcast<TEAM>(src.x, l), team_bcast<TEAM>(src.y, l), team_bcast<TEAM>(src.z, l),
+                                  team_bcast<TEAM>(src.w, l));
+            };
+            // ring of the next DEPTH ratings, loaded before the current rating's
+            // updates so their latency overlaps DEPTH ratings of compute
+            Pending<VEC> ring[DEPTH];
+#pragma unroll
+            for (int k = 0; k < DEPTH; ++k)
+                if (s0 + k < s1) fetch(rec_at(k), ring[k]);
+
+            static_assert(TEAM % DEPTH == 0, "prefetch depth must divide the team size");
+            for (uint32_t tb = s0; tb < s1; tb += TEAM) {
+            for (uint32_t t0 = tb; t0 < tb + TEAM && t0 < s1; t0 += DEPTH) {
 #pragma unroll
                 for (int k = 0; k < DEPTH; ++k) {
                 const uint32_t t = t0 + k;
                 if (t >= s1) break;
                 const Pending<VEC> cur = ring[k];
-                if (t + DEPTH < s1) fetch(t + DEPTH, ring[k]);
+                if (t + DEPTH < s1) fetch(rec_at(t + DEPTH - tb), ring[k]);
                 const uint32_t i = cur.i;
                 const float dev = cur.dev;
                 const float bi_t = cur.ib + cur.bt + cur.ibf;
@@ -560,12 +587,12 @@ __global__ __launch_bounds__(256) void k_sgd_par(DevModel m, StrataView sv, int 
                     if (ring[j].ud == cur.ud) ring[j].bud = bud_new;
                 }
                 if (lane == 0) {
-                    m.but_bin[ubrow + cur.b] = but_new;
+                    if (!m.skip_but) m.but_bin[ubrow + cur.b] = but_new;
                     m.ubias_day[cur.ud] = bud_new;
                     const float gi = err_b * cu_t;
                     apply_delta(&m.ibias[i], cur.ib, -h.lr_ib * gi, qm);
                     apply_delta(&m.bit_bin[static_cast<size_t>(i) * B + cur.b], cur.bt, -h.lr_ti * gi, qm);
-                    apply_delta(&m.ibias_freq[static_cast<size_t>(i) * F], cur.ibf, -h.lr_ibf * gi, qm);
+                    if (!m.skip_ibf) apply_delta(&m.ibias_freq[static_cast<size_t>(i) * F], cur.ibf, -h.lr_ibf * gi, qm);
                 }
                 cu -= h.lr_cu * (err_b * bi_t + h.reg_cu * (cu - 1.0f));
                 Gb += m.sum_err_bug ? err_b * static_cast<float>(m.f) : err_b;
@@ -583,6 +610,9 @@ __global__ __launch_bounds__(256) void k_sgd_par(DevModel m, StrataView sv, int 
                 apply_vec<VEC, TEAM>(m.ifeat + static_cast<size_t>(i) * fs + off, cur.q, dq, qm);
                 if constexpr (QF) apply_vec<VEC, TEAM>(m.ifeat_freq + static_cast<size_t>(i) * F * fs + off, cur.qf, dqf, qm);
                 }
+            }
+            cb = nb;
+            nb = load_recs(tb + 2 * TEAM);
             }
 
             store_str<VEC, TEAM>(m.ufeat + prow, p);

@@ -22,6 +22,8 @@ struct DevModel {
     int32_t U, I, f, fs, F, B;  // f = n_feat, fs = padded row stride
     int32_t day_range;
     int32_t low_mem, full_su, has_ord, sum_err_bug;
+    // tables that are all zero and never trained: the parallel kernel skips them
+    int32_t skip_cut, skip_ibf, skip_but;
     float gbias;
     Hyper h;
 
@@ -66,12 +68,9 @@ struct StrataView {
     int32_t n_strata;
     int32_t n_cells;            // user blocks = item blocks = n_strata (1 => one cell)
     const uint32_t* seg_off;    // [U * (n_strata + 1)] offsets into the permuted records
-    const uint32_t* p_item;     // permuted training records
-    const uint32_t* p_ud;
-    const float* p_dev;
-    const float* p_r;
-    const uint8_t* p_bin;
-    const int8_t* p_raw;
+    // packed training records in segment order:
+    //   x = item | (raw & 15) << 20 | bin << 24, y = SparseUD index, z = dev, w = target
+    const uint4* p_rec;
     const uint32_t* q_users;    // per (stratum, cell) queues of users, longest segment first
     const uint32_t* q_start;    // [n_strata * n_cells + 1]
     uint32_t* q_head;           // [n_strata * n_cells] work counters (reset per epoch)

@@ -159,7 +159,7 @@ class CRatings(C.Structure):
 class CSchedule(C.Structure):
     _fields_ = [("mode", C.c_int32), ("n_strata", C.c_int32), ("teams_per_block", C.c_int32),
                 ("blocks", C.c_int32), ("use_graph", C.c_int32), ("update_flags", C.c_int32),
-                ("reserved", C.c_int32 * 2)]
+                ("user_order", C.c_int32), ("reserved", C.c_int32 * 1)]
 
 
 class CEpochStats(C.Structure):
@@ -376,8 +376,10 @@ class TxModel:
         return ["bias", "mf", "nsvd1"]
 
     def set_schedule(self, mode: int = SCHED_STRATA, n_strata: int = 1, blocks: int = 0, use_graph: bool = False,
-                     teams_per_block: int = 0, q_update: int = 0, y_update: int = 2) -> None:
+                     teams_per_block: int = 0, q_update: int = 0, y_update: int = 2,
+                     user_order: int = 0) -> None:
         s = CSchedule()
+        s.user_order = int(user_order)
         s.update_flags = int(q_update) | (int(y_update) << 2)
         s.mode = int(mode)
         s.n_strata = int(n_strata)
