@@ -399,14 +399,17 @@ int build_strata(tx_ctx* c, int S) {
     return 0;
 }
 
+// Teams of the parallel schedule in flight at once.  A step reads an item row
+// a few ratings before writing it, so the updates it misses grow with the
+// number of concurrent teams; about 4096 keeps the holdout RMSE on the
+// serial reference's curve at the full Netflix shape.
+constexpr int kTargetTeams = 4096;
+
 int default_blocks(tx_ctx* c) {
-    int dev = 0, cus = 0;
-    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-        cus = 256;
-    // a few resident 256-thread workgroups per CU; a multiple of 8 keeps the
-    // cell -> XCD mapping of the round-robin dispatch
-    int blocks = cus * 4;
-    return (blocks + 7) / 8 * 8;
+    const int teams_per_block = 256 / c->shape.team;
+    int blocks = (kTargetTeams + teams_per_block - 1) / teams_per_block;
+    // a multiple of 8 keeps the cell -> XCD mapping of the round-robin dispatch
+    return std::max(8, (blocks + 7) / 8 * 8);
 }
 
 // Enqueue one parallel epoch (queue reset, S strata, ycache rebuild).
@@ -704,6 +707,7 @@ int tx_create(const tx_config* cfg, const tx_ratings* tr, const tx_ratings* pr, 
     m.skip_cut = 1;                              // cut is never trained (zero until set)
     m.skip_ibf = cfg->lr_ibf == 0.0f ? 1 : 0;    // zero-initialised, trained only by lr_ibf
     m.skip_but = cfg->lr_tu == 0.0f ? 1 : 0;
+    m.skip_yb = cfg->lr_yb == 0.0f ? 1 : 0;
     m.gbias = gbias;
     m.h = make_hyper(*cfg);
 #define RC(x) \
@@ -1078,6 +1082,7 @@ int tx_set_param(tx_ctx* c, int32_t id, const float* in, int64_t n) {
     if (id == TX_P_CUT) c->dm.skip_cut = all_zero() ? 1 : 0;
     if (id == TX_P_IBIAS_FREQ) c->dm.skip_ibf = (c->cfg.lr_ibf == 0.0f && all_zero()) ? 1 : 0;
     if (id == TX_P_BUT_BIN) c->dm.skip_but = (c->cfg.lr_tu == 0.0f && all_zero()) ? 1 : 0;
+    if (id == TX_P_YBIAS) c->dm.skip_yb = (c->cfg.lr_yb == 0.0f && all_zero()) ? 1 : 0;
     // a captured epoch embeds the model by value: re-capture after a change
     if (c->graph) {
         (void)hipGraphExecDestroy(c->graph);

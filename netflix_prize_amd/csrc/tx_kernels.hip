@@ -378,6 +378,11 @@ __device__ __forceinline__ uint32_t team_bcast(uint32_t v, uint32_t j) {
     else return static_cast<uint32_t>(__shfl(static_cast<int>(v), static_cast<int>(j), TEAM));
 }
 
+template <int TEAM>
+__device__ __forceinline__ float team_bcast_f(float v, uint32_t j) {
+    return __uint_as_float(team_bcast<TEAM>(__float_as_uint(v), j));
+}
+
 // Apply an SGD delta to a shared row element (see tx_schedule.update_flags).
 __device__ __forceinline__ void apply_delta(float* p, float old, float delta, int mode) {
     if (mode == 0) __hip_atomic_fetch_add(p, delta, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -392,13 +397,16 @@ __device__ __forceinline__ void apply_vec(float* p, const float (&old)[VEC], con
 }
 
 // The next training rating and everything read for it before its update.
+// The rating's scalars arrive in one load spread over team lanes:
+//   sc  lane 0 b_i, 1 b_i,bin, 2 b_u,day, 3 b_i,f(0)
+//   sc2 lane 0 c_u,day, 1 b_u,bin
 template <int VEC>
 struct Pending {
     uint32_t i, ud;
     float dev, r;
     int b, raw;
     float q[VEC], qf[VEC];
-    float ib, bt, ibf, cut, but, bud;
+    float sc, sc2;
 };
 
 #ifndef TX_PREFETCH
@@ -413,6 +421,7 @@ template <int TEAM, int VEC, bool QF, bool P2, bool ORD>
 __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, StrataView sv, int stratum) {
     constexpr int CHUNK = 4;
     constexpr int DEPTH = TX_PREFETCH;
+    __shared__ float xbuf[256 * VEC];  // per-team transpose slot, TEAM*VEC floats each
     const int lane = threadIdx.x % TEAM;
     const int S = sv.n_strata;
     const int cell = sv.n_cells > 1 ? static_cast<int>(blockIdx.x % sv.n_cells) : 0;
@@ -463,12 +472,22 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
 #pragma unroll
                     for (int c = 0; c < VEC; ++c) n.qf[c] = 0.0f;
                 }
-                n.ib = m.ibias[n.i];
-                n.bt = m.bit_bin[static_cast<size_t>(n.i) * B + n.b];
-                n.ibf = m.skip_ibf ? 0.0f : m.ibias_freq[static_cast<size_t>(n.i) * F];
-                n.cut = m.skip_cut ? 0.0f : m.cut[n.ud];
-                n.but = m.skip_but ? 0.0f : m.but_bin[ubrow + n.b];
-                n.bud = m.ubias_day[n.ud];
+                {
+                    const float* a = nullptr;
+                    if (lane == 0) a = &m.ibias[n.i];
+                    else if (lane == 1) a = &m.bit_bin[static_cast<size_t>(n.i) * B + n.b];
+                    else if (lane == 2) a = &m.ubias_day[n.ud];
+                    else if (lane == 3 && !m.skip_ibf) a = &m.ibias_freq[static_cast<size_t>(n.i) * F];
+                    n.sc = a ? *a : 0.0f;
+                }
+                if (!(m.skip_cut && m.skip_but)) {
+                    const float* a = nullptr;
+                    if (lane == 0 && !m.skip_cut) a = &m.cut[n.ud];
+                    else if (lane == 1 && !m.skip_but) a = &m.but_bin[ubrow + n.b];
+                    n.sc2 = a ? *a : 0.0f;
+                } else {
+                    n.sc2 = 0.0f;
+                }
             };
             float p[VEC], p2[VEC], s[VEC], G[VEC];
             load_str<VEC, TEAM>(m.ufeat + prow, p);
@@ -480,37 +499,50 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
             }
             float su_b;
             if (first) {
-                // NSVD1 user aggregation from the current y (src/tx.rs:801-822)
-                float acc[4][VEC];
+                // NSVD1 user aggregation from the current y (src/tx.rs:801-822):
+                // rows read contiguously (lane l: elements l*VEC..), then
+                // transposed to the strided layout through the team's LDS slot
+                float acc[2][VEC];
                 float sb = 0.0f;
 #pragma unroll
-                for (int k = 0; k < 4; ++k)
+                for (int k = 0; k < 2; ++k)
 #pragma unroll
                     for (int c = 0; c < VEC; ++c) acc[k][c] = 0.0f;
                 const uint32_t nh = m.full_su ? m.h_start[u + 1] - m.h_start[u] : 0;
-                const uint32_t ntot = (te - ts) + nh;
+                const uint32_t ntr = te - ts;
+                const uint32_t ntot = ntr + nh;
                 const uint32_t* hitems = m.h_item + m.h_start[u];
+                const int coff = lane * VEC;
                 for (uint32_t k0 = 0; k0 < ntot; k0 += 4) {
                     float y[4][VEC];
                     uint32_t jj[4];
 #pragma unroll
                     for (int k = 0; k < 4; ++k) {
                         const uint32_t kk = k0 + k;
-                        jj[k] = kk < ntot ? (kk < te - ts ? m.t_item[ts + kk] : hitems[kk - (te - ts)]) : 0xffffffffu;
-                        if (jj[k] != 0xffffffffu) load_str<VEC, TEAM>(m.yfeat + static_cast<size_t>(jj[k]) * fs + off, y[k]);
+                        jj[k] = kk < ntot ? (kk < ntr ? m.t_item[ts + kk] : hitems[kk - ntr]) : 0xffffffffu;
+                        if (jj[k] != 0xffffffffu) load_vec<VEC>(m.yfeat + static_cast<size_t>(jj[k]) * fs + coff, y[k]);
                     }
 #pragma unroll
                     for (int k = 0; k < 4; ++k) {
                         if (jj[k] == 0xffffffffu) continue;
 #pragma unroll
-                        for (int c = 0; c < VEC; ++c) acc[k][c] += y[k][c];
-                        if (lane == ((k0 + k) & (TEAM - 1))) sb += m.ybias[jj[k]];
+                        for (int c = 0; c < VEC; ++c) acc[k & 1][c] += y[k][c];
+                        if (!m.skip_yb && lane == ((k0 + k) & (TEAM - 1))) sb += m.ybias[jj[k]];
                     }
                 }
                 const float inv = 1.0f / norm;
+                float* x = xbuf + (threadIdx.x / TEAM) * (TEAM * VEC);
 #pragma unroll
-                for (int c = 0; c < VEC; ++c) s[c] = ((acc[0][c] + acc[1][c]) + (acc[2][c] + acc[3][c])) * inv;
-                su_b = team_sum_dpp<TEAM>(sb) * inv;
+                for (int c = 0; c < VEC; ++c) x[coff + c] = (acc[0][c] + acc[1][c]) * inv;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+                for (int c = 0; c < VEC; ++c) s[c] = x[c * TEAM + lane];
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                su_b = m.skip_yb ? 0.0f : team_sum_dpp<TEAM>(sb) * inv;
                 if (S > 1) {
                     store_str<VEC, TEAM>(m.ycache + prow, s);
                     if (lane == 0) m.ycache_bias[u] = su_b;
@@ -556,9 +588,12 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
                 if (t + DEPTH < s1) fetch(rec_at(t + DEPTH - tb), ring[k]);
                 const uint32_t i = cur.i;
                 const float dev = cur.dev;
-                const float bi_t = cur.ib + cur.bt + cur.ibf;
-                const float cu_t = cu + cur.cut;
-                const float bu_t = ubias + cur.but + alpha * dev + cur.bud + su_b;
+                const float c_ib = team_bcast_f<TEAM>(cur.sc, 0), c_bt = team_bcast_f<TEAM>(cur.sc, 1);
+                const float c_bud = team_bcast_f<TEAM>(cur.sc, 2), c_ibf = team_bcast_f<TEAM>(cur.sc, 3);
+                const float c_cut = team_bcast_f<TEAM>(cur.sc2, 0), c_but = team_bcast_f<TEAM>(cur.sc2, 1);
+                const float bi_t = c_ib + c_bt + c_ibf;
+                const float cu_t = cu + c_cut;
+                const float bu_t = ubias + c_but + alpha * dev + c_bud + su_b;
 
                 float part = 0.0f;
 #pragma unroll
@@ -579,20 +614,27 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
                 // to every prefetched rating that reads the same slot
                 ubias -= h.lr_ub * err_b;
                 alpha -= h.lr_ta * err_b * dev;
-                const float but_new = cur.but - h.lr_tu * err_b;
-                const float bud_new = cur.bud - h.lr_ubd * err_b;
+                const float but_new = c_but - h.lr_tu * err_b;
+                const float bud_new = c_bud - h.lr_ubd * err_b;
 #pragma unroll
                 for (int j = 0; j < DEPTH; ++j) {
-                    if (ring[j].b == cur.b) ring[j].but = but_new;
-                    if (ring[j].ud == cur.ud) ring[j].bud = bud_new;
+                    if (ring[j].b == cur.b && lane == 1) ring[j].sc2 = but_new;
+                    if (ring[j].ud == cur.ud && lane == 2) ring[j].sc = bud_new;
                 }
-                if (lane == 0) {
-                    if (!m.skip_but) m.but_bin[ubrow + cur.b] = but_new;
-                    m.ubias_day[cur.ud] = bud_new;
+                {
+                    // item biases in one instruction (lanes 0, 1, 3), user biases in one (lanes 1, 2)
                     const float gi = err_b * cu_t;
-                    apply_delta(&m.ibias[i], cur.ib, -h.lr_ib * gi, qm);
-                    apply_delta(&m.bit_bin[static_cast<size_t>(i) * B + cur.b], cur.bt, -h.lr_ti * gi, qm);
-                    if (!m.skip_ibf) apply_delta(&m.ibias_freq[static_cast<size_t>(i) * F], cur.ibf, -h.lr_ibf * gi, qm);
+                    float* a = nullptr;
+                    float d = 0.0f;
+                    if (lane == 0) { a = &m.ibias[i]; d = -h.lr_ib * gi; }
+                    else if (lane == 1) { a = &m.bit_bin[static_cast<size_t>(i) * B + cur.b]; d = -h.lr_ti * gi; }
+                    else if (lane == 3 && !m.skip_ibf) { a = &m.ibias_freq[static_cast<size_t>(i) * F]; d = -h.lr_ibf * gi; }
+                    if (a) apply_delta(a, cur.sc, d, qm);
+                    float* w = nullptr;
+                    float wv = 0.0f;
+                    if (lane == 2) { w = &m.ubias_day[cur.ud]; wv = bud_new; }
+                    else if (lane == 1 && !m.skip_but) { w = &m.but_bin[ubrow + cur.b]; wv = but_new; }
+                    if (w) *w = wv;
                 }
                 cu -= h.lr_cu * (err_b * bi_t + h.reg_cu * (cu - 1.0f));
                 Gb += m.sum_err_bug ? err_b * static_cast<float>(m.f) : err_b;
@@ -629,35 +671,71 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
 #pragma unroll
                 for (int c = 0; c < VEC; ++c) gq[c] = G[c] * inv;
                 const float gb = h.lr_yb * (Gb * inv);
+                if (ym == 2) {
+                    // lock-free rows: contiguous float4 read-modify-write, with the
+                    // user's gradient moved to the contiguous layout through LDS
+                    const int coff = lane * VEC;
+                    float* x = xbuf + (threadIdx.x / TEAM) * (TEAM * VEC);
+#pragma unroll
+                    for (int c = 0; c < VEC; ++c) x[c * TEAM + lane] = gq[c];
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    float gc[VEC];
+#pragma unroll
+                    for (int c = 0; c < VEC; ++c) gc[c] = x[coff + c];
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    for (uint32_t t0 = ts; t0 < te; t0 += 4) {
+                        float y[4][VEC];
+                        uint32_t jj[4];
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            jj[k] = t0 + k < te ? m.t_item[t0 + k] : 0xffffffffu;
+                            if (jj[k] != 0xffffffffu) load_vec<VEC>(m.yfeat + static_cast<size_t>(jj[k]) * fs + coff, y[k]);
+                        }
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            if (jj[k] == 0xffffffffu) continue;
+                            float ny[VEC];
+#pragma unroll
+                            for (int c = 0; c < VEC; ++c) ny[c] = y[k][c] - h.lr_y * (gc[c] + h.reg_y * y[k][c]);
+                            store_vec<VEC>(m.yfeat + static_cast<size_t>(jj[k]) * fs + coff, ny);
+                            if (gb != 0.0f && lane == ((t0 + k) & (TEAM - 1))) m.ybias[jj[k]] = m.ybias[jj[k]] - gb;
+                        }
+                    }
+                } else {
                 // rows of batch b+1 are loaded before batch b's updates are issued
-                float ya[4][VEC], yb[4][VEC];
-                uint32_t ja[4], jb[4];
-                auto load_batch = [&](uint32_t t0, float (&y)[4][VEC], uint32_t (&jj)[4]) {
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        jj[k] = t0 + k < te ? m.t_item[t0 + k] : 0xffffffffu;
-                        if (jj[k] != 0xffffffffu) load_str<VEC, TEAM>(m.yfeat + static_cast<size_t>(jj[k]) * fs + off, y[k]);
+                    float ya[4][VEC], yb[4][VEC];
+                    uint32_t ja[4], jb[4];
+                    auto load_batch = [&](uint32_t t0, float (&y)[4][VEC], uint32_t (&jj)[4]) {
+    #pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            jj[k] = t0 + k < te ? m.t_item[t0 + k] : 0xffffffffu;
+                            if (jj[k] != 0xffffffffu) load_str<VEC, TEAM>(m.yfeat + static_cast<size_t>(jj[k]) * fs + off, y[k]);
+                        }
+                    };
+                    auto update_batch = [&](uint32_t t0, const float (&y)[4][VEC], const uint32_t (&jj)[4]) {
+    #pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            if (jj[k] == 0xffffffffu) continue;
+                            float dy[VEC];
+    #pragma unroll
+                            for (int c = 0; c < VEC; ++c) dy[c] = -h.lr_y * (gq[c] + h.reg_y * y[k][c]);
+                            apply_vec<VEC, TEAM>(m.yfeat + static_cast<size_t>(jj[k]) * fs + off, y[k], dy, ym);
+                            if (gb != 0.0f && lane == ((t0 + k) & (TEAM - 1)))
+                                apply_delta(&m.ybias[jj[k]], m.ybias[jj[k]], -gb, ym);
+                        }
+                    };
+                    load_batch(ts, ya, ja);
+                    for (uint32_t t0 = ts; t0 < te; t0 += 8) {
+                        if (t0 + 4 < te) load_batch(t0 + 4, yb, jb);
+                        update_batch(t0, ya, ja);
+                        if (t0 + 4 >= te) break;
+                        if (t0 + 8 < te) load_batch(t0 + 8, ya, ja);
+                        update_batch(t0 + 4, yb, jb);
                     }
-                };
-                auto update_batch = [&](uint32_t t0, const float (&y)[4][VEC], const uint32_t (&jj)[4]) {
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        if (jj[k] == 0xffffffffu) continue;
-                        float dy[VEC];
-#pragma unroll
-                        for (int c = 0; c < VEC; ++c) dy[c] = -h.lr_y * (gq[c] + h.reg_y * y[k][c]);
-                        apply_vec<VEC, TEAM>(m.yfeat + static_cast<size_t>(jj[k]) * fs + off, y[k], dy, ym);
-                        if (gb != 0.0f && lane == ((t0 + k) & (TEAM - 1)))
-                            apply_delta(&m.ybias[jj[k]], m.ybias[jj[k]], -gb, ym);
-                    }
-                };
-                load_batch(ts, ya, ja);
-                for (uint32_t t0 = ts; t0 < te; t0 += 8) {
-                    if (t0 + 4 < te) load_batch(t0 + 4, yb, jb);
-                    update_batch(t0, ya, ja);
-                    if (t0 + 4 >= te) break;
-                    if (t0 + 8 < te) load_batch(t0 + 8, ya, ja);
-                    update_batch(t0 + 4, yb, jb);
                 }
             } else {
                 store_str<VEC, TEAM>(sv.gacc + prow, G);

@@ -23,7 +23,7 @@ struct DevModel {
     int32_t day_range;
     int32_t low_mem, full_su, has_ord, sum_err_bug;
     // tables that are all zero and never trained: the parallel kernel skips them
-    int32_t skip_cut, skip_ibf, skip_but;
+    int32_t skip_cut, skip_ibf, skip_but, skip_yb;
     float gbias;
     Hyper h;
 

@@ -102,3 +102,25 @@ def test_parallel_schedule_holdout_rmse(gpu, n_strata):
     r_o, r_g = m_o.calc_rmse(), m_g.calc_rmse(pr)
     print(f"strata={n_strata}: oracle {r_o:.6f} gpu {r_g:.6f}")
     assert abs(r_g - r_o) <= 1e-3
+
+
+def test_full_shape_holdout_rmse_matches_serial_reference(gpu):
+    """Parity (b) at the full Netflix shape: plain form, f=64, 20 epochs in the
+    default parallel schedule vs the serial oracle's holdout RMSE
+    (tests/golden/oracle_curve_full_f64.json, made by make_oracle_curves.py)."""
+    import json
+    import os
+    from netflix_prize_amd import synth
+    golden = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "oracle_curve_full_f64.json")))
+    tr, pr = synth.generate(synth.FULL, golden["seed"])
+    assert tr.n_ratings == golden["n_train"] and pr.n_ratings == golden["n_heldout"]
+    cfg = tx.plain_form(lm_config(n_feat=64, n_epochs=20, n_time_bins=30, reset_u_epoch=10_000))
+    with tx.TxModel(tr, pr, cfg) as m:
+        curve = []
+        for e in range(1, 21):
+            m.fit_epoch(epoch=e)
+            curve.append(m.calc_rmse(pr))
+    ref = [x["holdout_rmse"] for x in golden["epochs"]]
+    print("gpu", [round(x, 5) for x in curve])
+    print("ref", [round(x, 5) for x in ref])
+    assert abs(curve[-1] - ref[-1]) <= 1e-3
