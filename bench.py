@@ -111,6 +111,7 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-workers", type=int, default=0, help="threads for the parallel CPU variant (0 = auto)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--ttr-epochs", type=int, default=20, help="epochs of the time-to-RMSE run (0 = skip)")
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -159,6 +160,33 @@ def main(argv=None) -> int:
     value = n_train / (ms_per_step / 1000.0)
     rmse = model.calc_rmse(pr)
     log(f"bench: holdout RMSE after {epoch} epochs: {rmse:.5f}")
+
+    # time-to-holdout-RMSE: a fresh model trained for the full epoch budget,
+    # GPU epoch time accumulated until the RMSE is within 0.001 of its final value
+    ttr = None
+    if args.ttr_epochs > 0:
+        m2 = tx.TxModel(tr, pr, cfg)
+        m2.set_schedule(mode=tx.SCHED_STRATA, n_strata=args.strata, use_graph=bool(args.graph), blocks=args.blocks)
+        curve, secs = [], []
+        for e in range(1, args.ttr_epochs + 1):
+            m2.fit_epoch(epoch=e)
+            secs.append(m2.last_epoch_stats()["ms_total"] / 1000.0)
+            curve.append(m2.calc_rmse(pr))
+        m2.close()
+        target = curve[-1] + 0.001
+        hit = next(k for k, r in enumerate(curve) if r <= target)
+        ref = None
+        gpath = os.path.join(ROOT, "tests", "golden", f"oracle_curve_{args.shape}_f{cfg.n_feat}.json")
+        if os.path.exists(gpath):
+            g = json.load(open(gpath))
+            if g.get("seed") == args.seed and len(g["epochs"]) >= args.ttr_epochs:
+                ref = g["epochs"][args.ttr_epochs - 1]["holdout_rmse"]
+        ttr = {"epochs": args.ttr_epochs, "final_holdout_rmse": round(curve[-1], 5),
+               "target_rmse": round(target, 5), "seconds_to_target": round(sum(secs[:hit + 1]), 3),
+               "epochs_to_target": hit + 1, "seconds_total": round(sum(secs), 3),
+               "serial_reference_rmse": None if ref is None else round(ref, 5),
+               "curve": [round(r, 5) for r in curve]}
+        log(f"bench: time-to-RMSE {ttr}")
 
     # roofline of the dominant kernel (SGD), from the library's events on its stream
     qf = cfg.lr_iqf != 0.0 or cfg.sigma_iqf != 0.0
@@ -209,6 +237,7 @@ def main(argv=None) -> int:
                    "n_train": n_train, "n_heldout": pr.n_ratings, "holdout_rmse": round(rmse, 5),
                    "epochs_trained": epoch},
         "roofline": roofline,
+        "time_to_rmse": ttr,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -383,23 +383,102 @@ __device__ __forceinline__ float team_bcast_f(float v, uint32_t j) {
     return __uint_as_float(team_bcast<TEAM>(__float_as_uint(v), j));
 }
 
-// Apply an SGD delta to a shared row element (see tx_schedule.update_flags).
+// Apply an SGD delta to a shared row element (see tx_schedule.update_flags):
+// mode 2 writes old + delta, modes 0 and 1 add atomically (gfx950 float
+// atomics carry no scope bits, so both compile to the same instruction).
 __device__ __forceinline__ void apply_delta(float* p, float old, float delta, int mode) {
-    if (mode == 0) __hip_atomic_fetch_add(p, delta, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    else if (mode == 1) __hip_atomic_fetch_add(p, delta, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    else *p = old + delta;
+    if (mode == 2) *p = old + delta;
+    else __hip_atomic_fetch_add(p, delta, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// The mode test is hoisted out of the element loop: both sides issue VEC
+// stores, so the wait counters stay exact after the branch joins.
 template <int VEC, int TEAM>
 __device__ __forceinline__ void apply_vec(float* p, const float (&old)[VEC], const float (&delta)[VEC], int mode) {
+    if (mode == 2) {
 #pragma unroll
-    for (int c = 0; c < VEC; ++c) apply_delta(p + c * TEAM, old[c], delta[c], mode);
+        for (int c = 0; c < VEC; ++c) p[c * TEAM] = old[c] + delta[c];
+    } else {
+#pragma unroll
+        for (int c = 0; c < VEC; ++c) __hip_atomic_fetch_add(p + c * TEAM, delta[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// A copy of a wave-uniform value that the compiler must keep in a VGPR.
+__device__ __forceinline__ float in_vgpr(float x) {
+    float r;
+    asm("v_mov_b32 %0, %1" : "=v"(r) : "s"(x));
+    return r;
+}
+
+// Walks rows 0..n-1 of a user's item list in order.  Row r's item index is
+// *idx_ptr(r): the indices arrive TEAM at a time (lane l loads index b0 + l),
+// a batch ahead.  The rows are read contiguously (lane l: elements l*VEC ..),
+// GB at a time and double-buffered, so use() of one group overlaps the loads
+// of the next and a store issued by use() is never waited for by a later
+// load.  Every load is unconditional, indexes past n clamped to row n-1 (read,
+// never used).  batch(b0, ib) runs once per index batch: lane l holds the
+// index of row b0 + l in ib.
+template <int TEAM, int VEC, int GB, class IdxPtr, class Batch, class Use>
+__device__ __forceinline__ void for_rows(uint32_t n, int lane, const float* yfeat, int fs, IdxPtr idx_ptr,
+                                         Batch batch, Use use) {
+    static_assert(TEAM % (2 * GB) == 0, "two row groups must tile an index batch");
+    if (n == 0) return;
+    auto idx_load = [&](uint32_t r) { return *idx_ptr(r < n ? r : n - 1); };
+    const int coff = lane * VEC;
+    uint32_t b0 = 0;
+    uint32_t ib = idx_load(static_cast<uint32_t>(lane)), ibn = idx_load(TEAM + static_cast<uint32_t>(lane));
+    auto jrow = [&](uint32_t r) -> uint32_t {  // item of row r, r - b0 < 2*TEAM
+        const uint32_t d = r - b0;
+        if (d < TEAM) return team_bcast<TEAM>(ib, d);
+        asm volatile("");  // keep the branch: a select would wait for ibn
+        return team_bcast<TEAM>(ibn, d - TEAM);
+    };
+    float ya[GB][VEC], yb[GB][VEC];
+    uint32_t ja[GB], jb[GB];
+    auto load = [&](uint32_t r0, float (&y)[GB][VEC], uint32_t (&jj)[GB]) {
+#pragma unroll
+        for (int k = 0; k < GB; ++k) {
+            jj[k] = jrow(r0 + k);
+            load_vec<VEC>(yfeat + static_cast<size_t>(jj[k]) * fs + coff, y[k]);
+        }
+    };
+    batch(b0, ib);
+    load(0, ya, ja);
+    for (uint32_t r0 = 0;; r0 += 2 * GB) {
+        load(r0 + GB, yb, jb);
+#pragma unroll
+        for (int k = 0; k < GB; ++k)
+            if (r0 + k < n) use(r0 + k, ja[k], ya[k]);
+        if (r0 + GB >= n) break;
+        load(r0 + 2 * GB, ya, ja);
+#pragma unroll
+        for (int k = 0; k < GB; ++k)
+            if (r0 + GB + k < n) use(r0 + GB + k, jb[k], yb[k]);
+        if (r0 + 2 * GB >= n) break;
+        if (r0 + 2 * GB - b0 == TEAM) {
+            b0 += TEAM;
+            ib = ibn;
+            ibn = idx_load(b0 + TEAM + static_cast<uint32_t>(lane));
+            batch(b0, ib);
+        }
+    }
+}
+
+// Rows per group of for_rows: 32 floats of row data per lane per group.
+template <int TEAM, int VEC>
+constexpr int row_group() {
+    return (TEAM / 2 < 32 / VEC) ? TEAM / 2 : 32 / VEC;
 }
 
 // The next training rating and everything read for it before its update.
 // The rating's scalars arrive in one load spread over team lanes:
 //   sc  lane 0 b_i, 1 b_i,bin, 2 b_u,day, 3 b_i,f(0)
 //   sc2 lane 0 c_u,day, 1 b_u,bin
+// A user-owned bias written by an earlier rating after this one was fetched
+// is forwarded in bud/but (has_bud/has_but set) rather than written into
+// sc/sc2: those registers may still be waiting for their load, and writing
+// them would stall the wave until every younger memory operation returned.
 template <int VEC>
 struct Pending {
     uint32_t i, ud;
@@ -407,10 +486,13 @@ struct Pending {
     int b, raw;
     float q[VEC], qf[VEC];
     float sc, sc2;
+    float* sc_addr;  // this lane's slot of sc (read at fetch, updated after the rating)
+    float bud, but;
+    bool has_bud, has_but;
 };
 
 #ifndef TX_PREFETCH
-#define TX_PREFETCH 2
+#define TX_PREFETCH 3
 #endif
 
 #ifndef TX_SGD_WAVES
@@ -429,11 +511,34 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
     const uint32_t qbeg = sv.q_start[qid];
     const uint32_t qlen = sv.q_start[qid + 1] - qbeg;
     const int fs = m.fs, F = m.F, B = m.B;
-    const Hyper h = m.h;
+    // The rating loop's hyper-parameters live in VGPRs (an opaque copy, so the
+    // compiler cannot fold them back): VALU operands either way, and the
+    // scalar registers stay free for the loop state instead of spilling.
+    Hyper h = m.h;
+    h.lr_ub = in_vgpr(h.lr_ub); h.lr_ta = in_vgpr(h.lr_ta); h.lr_tu = in_vgpr(h.lr_tu);
+    h.lr_ubd = in_vgpr(h.lr_ubd); h.lr_ib = in_vgpr(h.lr_ib); h.lr_ti = in_vgpr(h.lr_ti);
+    h.lr_ibf = in_vgpr(h.lr_ibf); h.lr_cu = in_vgpr(h.lr_cu); h.reg_cu = in_vgpr(h.reg_cu);
+    h.lr_u = in_vgpr(h.lr_u); h.reg_u = in_vgpr(h.reg_u); h.lr_i = in_vgpr(h.lr_i); h.reg_i = in_vgpr(h.reg_i);
+    h.w_bias = in_vgpr(h.w_bias); h.w_factor = in_vgpr(h.w_factor);
+    if constexpr (P2) { h.lr_u2 = in_vgpr(h.lr_u2); h.reg_u2 = in_vgpr(h.reg_u2); }
+    if constexpr (QF) { h.lr_iqf = in_vgpr(h.lr_iqf); h.reg_iqf = in_vgpr(h.reg_iqf); }
     const int off = lane;  // strided: element c*TEAM + lane
     const bool first = (stratum == 0);
     const bool last = (stratum == S - 1);
     const int qm = sv.q_mode, ym = sv.y_mode;
+
+    // Per-lane slot of the rating's scalars (sc): lane 0 b_i, 1 b_i,bin,
+    // 2 b_u,day, 3 b_i,f(0); other lanes, and lane 3 when b_i,f is skipped,
+    // re-read b_i.  Address = base + i*imul + (b & bm) + (ud & um); all
+    // lane-dependent, so they live in VGPRs and cost no scalar registers.
+    const bool l3 = lane == 3 && !m.skip_ibf;
+    const bool sc_item = lane == 0 || lane == 1 || l3;
+    float* const sc_base = lane == 1 ? m.bit_bin : lane == 2 ? m.ubias_day : l3 ? m.ibias_freq : m.ibias;
+    const uint32_t sc_imul = lane == 1 ? static_cast<uint32_t>(B) : lane == 2 ? 0u : l3 ? static_cast<uint32_t>(F) : 1u;
+    const uint32_t sc_bm = lane == 1 ? 0xffffffffu : 0u;
+    const uint32_t sc_um = lane == 2 ? 0xffffffffu : 0u;
+    const float sc_lr = lane == 0 ? h.lr_ib : lane == 1 ? h.lr_ti : h.lr_ibf;
+    float* const q_base = m.ifeat + off;
 
     float th[4] = {0.f, 0.f, 0.f, 0.f};
     if constexpr (ORD) {
@@ -445,7 +550,11 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
         uint32_t w0 = 0;
         if (lane == 0) w0 = __hip_atomic_fetch_add(&sv.q_head[qid], static_cast<uint32_t>(CHUNK), __ATOMIC_RELAXED,
                                                    __HIP_MEMORY_SCOPE_AGENT);
-        w0 = __shfl(w0, 0, TEAM);
+        // a full-wave team takes the value with readfirstlane so the compiler
+        // sees it (and the user, segment bounds and loop trip counts derived
+        // from it) as wave-uniform: scalar loop control, no exec-mask loops
+        if constexpr (TEAM == 64) w0 = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(w0)));
+        else w0 = __shfl(w0, 0, TEAM);
         if (w0 >= qlen) break;
         const uint32_t w1 = w0 + CHUNK < qlen ? w0 + CHUNK : qlen;
         for (uint32_t w = w0; w < w1; ++w) {
@@ -465,26 +574,22 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
                 n.ud = rc.y;
                 n.dev = __uint_as_float(rc.z);
                 n.r = __uint_as_float(rc.w);
-                load_str<VEC, TEAM>(m.ifeat + static_cast<size_t>(n.i) * fs + off, n.q);
+                n.has_bud = n.has_but = false;
+                n.bud = n.but = 0.0f;
+                load_str<VEC, TEAM>(q_base + static_cast<size_t>(n.i) * fs, n.q);
                 if constexpr (QF) {
                     load_str<VEC, TEAM>(m.ifeat_freq + static_cast<size_t>(n.i) * F * fs + off, n.qf);
                 } else {
 #pragma unroll
                     for (int c = 0; c < VEC; ++c) n.qf[c] = 0.0f;
                 }
-                {
-                    const float* a = nullptr;
-                    if (lane == 0) a = &m.ibias[n.i];
-                    else if (lane == 1) a = &m.bit_bin[static_cast<size_t>(n.i) * B + n.b];
-                    else if (lane == 2) a = &m.ubias_day[n.ud];
-                    else if (lane == 3 && !m.skip_ibf) a = &m.ibias_freq[static_cast<size_t>(n.i) * F];
-                    n.sc = a ? *a : 0.0f;
-                }
+                n.sc_addr = sc_base + static_cast<size_t>(n.i) * sc_imul + (static_cast<uint32_t>(n.b) & sc_bm) +
+                            (n.ud & sc_um);
+                n.sc = *n.sc_addr;
                 if (!(m.skip_cut && m.skip_but)) {
-                    const float* a = nullptr;
-                    if (lane == 0 && !m.skip_cut) a = &m.cut[n.ud];
-                    else if (lane == 1 && !m.skip_but) a = &m.but_bin[ubrow + n.b];
-                    n.sc2 = a ? *a : 0.0f;
+                    const float* a = m.skip_cut ? &m.but_bin[ubrow + n.b] : &m.cut[n.ud];
+                    a = (lane == 1 && !m.skip_but) ? &m.but_bin[ubrow + n.b] : a;
+                    n.sc2 = *a;
                 } else {
                     n.sc2 = 0.0f;
                 }
@@ -500,40 +605,33 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
             float su_b;
             if (first) {
                 // NSVD1 user aggregation from the current y (src/tx.rs:801-822):
-                // rows read contiguously (lane l: elements l*VEC..), then
-                // transposed to the strided layout through the team's LDS slot
-                float acc[2][VEC];
+                // rows summed in the contiguous layout, then transposed to the
+                // strided layout through the team's LDS slot
+                float acc[VEC];
                 float sb = 0.0f;
 #pragma unroll
-                for (int k = 0; k < 2; ++k)
-#pragma unroll
-                    for (int c = 0; c < VEC; ++c) acc[k][c] = 0.0f;
+                for (int c = 0; c < VEC; ++c) acc[c] = 0.0f;
                 const uint32_t nh = m.full_su ? m.h_start[u + 1] - m.h_start[u] : 0;
                 const uint32_t ntr = te - ts;
                 const uint32_t ntot = ntr + nh;
-                const uint32_t* hitems = m.h_item + m.h_start[u];
+                const uint32_t* titems = m.t_item + ts;
+                const uint32_t* hitems = m.h_item + m.h_start[u] - ntr;  // row r >= ntr is hitems[r]
                 const int coff = lane * VEC;
-                for (uint32_t k0 = 0; k0 < ntot; k0 += 4) {
-                    float y[4][VEC];
-                    uint32_t jj[4];
+                const bool yb_on = !m.skip_yb;
+                for_rows<TEAM, VEC, row_group<TEAM, VEC>()>(
+                    ntot, lane, m.yfeat, fs,
+                    [&](uint32_t r) { return r < ntr ? titems + r : hitems + r; },
+                    [&](uint32_t b0, uint32_t ib) {
+                        if (yb_on && b0 + static_cast<uint32_t>(lane) < ntot) sb += m.ybias[ib];
+                    },
+                    [&](uint32_t, uint32_t, const float (&y)[VEC]) {
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        const uint32_t kk = k0 + k;
-                        jj[k] = kk < ntot ? (kk < ntr ? m.t_item[ts + kk] : hitems[kk - ntr]) : 0xffffffffu;
-                        if (jj[k] != 0xffffffffu) load_vec<VEC>(m.yfeat + static_cast<size_t>(jj[k]) * fs + coff, y[k]);
-                    }
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        if (jj[k] == 0xffffffffu) continue;
-#pragma unroll
-                        for (int c = 0; c < VEC; ++c) acc[k & 1][c] += y[k][c];
-                        if (!m.skip_yb && lane == ((k0 + k) & (TEAM - 1))) sb += m.ybias[jj[k]];
-                    }
-                }
+                        for (int c = 0; c < VEC; ++c) acc[c] += y[c];
+                    });
                 const float inv = 1.0f / norm;
                 float* x = xbuf + (threadIdx.x / TEAM) * (TEAM * VEC);
 #pragma unroll
-                for (int c = 0; c < VEC; ++c) x[coff + c] = (acc[0][c] + acc[1][c]) * inv;
+                for (int c = 0; c < VEC; ++c) x[coff + c] = acc[c] * inv;
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -557,40 +655,70 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
             float Gb = first ? 0.0f : sv.gacc_b[u];
             float ubias = m.ubias[u], alpha = m.alpha_u[u], cu = m.cu[u];
 
-            // records arrive TEAM at a time (lane l holds record tb + l); the
-            // batch after the current one is always in flight
+            // records arrive TEAM at a time (lane l holds record tb + l); the two
+            // batches after the current one are in flight, so the copies at a
+            // batch boundary only touch registers loaded a batch or more ago
+            // (copying the batch just requested would wait for it).  Every load in
+            // the rating loop is unconditional, indexes past the segment end
+            // being clamped to its last record (re-read, never consumed): a
+            // path that skipped them would leave the hardware's in-order wait
+            // counter nothing younger than the prefetched rows, and each use
+            // of a prefetched row would then wait for all memory traffic.
             auto load_recs = [&](uint32_t tb) {
                 const uint32_t t = tb + static_cast<uint32_t>(lane);
-                return t < s1 ? sv.p_rec[t] : make_uint4(0u, 0u, 0u, 0u);
+                return sv.p_rec[t < s1 ? t : s1 - 1];
             };
-            uint4 cb = load_recs(s0), nb = load_recs(s0 + TEAM);
-            auto rec_at = [&](uint32_t j) {  // j in [0, 2*TEAM) relative to the current batch
-                const uint4 src = j < TEAM ? cb : nb;
-                const uint32_t l = j < TEAM ? j : j - TEAM;
-                return make_uint4(team_bcast<TEAM>(src.x, l), team_bcast<TEAM>(src.y, l), team_bcast<TEAM>(src.z, l),
-                                  team_bcast<TEAM>(src.w, l));
+            if (s0 < s1) {
+            uint4 cb = load_recs(s0), nb = load_recs(s0 + TEAM), fb = load_recs(s0 + 2 * TEAM);
+            uint32_t tb = s0;
+            // record t of the current batch or the next.  A branch, not a select:
+            // a select would read nb at every rating and wait for it; the empty
+            // asm keeps the compiler from turning the branch back into a select.
+            auto rec_at = [&](uint32_t t) {
+                const uint32_t j = t - tb;
+                uint4 r;
+                if (j < TEAM) {
+                    r = make_uint4(team_bcast<TEAM>(cb.x, j), team_bcast<TEAM>(cb.y, j), team_bcast<TEAM>(cb.z, j),
+                                   team_bcast<TEAM>(cb.w, j));
+                } else {
+                    asm volatile("");
+                    const uint32_t l = j - TEAM;
+                    r = make_uint4(team_bcast<TEAM>(nb.x, l), team_bcast<TEAM>(nb.y, l), team_bcast<TEAM>(nb.z, l),
+                                   team_bcast<TEAM>(nb.w, l));
+                }
+                return r;
             };
             // ring of the next DEPTH ratings, loaded before the current rating's
-            // updates so their latency overlaps DEPTH ratings of compute
-            Pending<VEC> ring[DEPTH];
+            // updates so their latency overlaps DEPTH ratings of compute.  The
+            // ring has DEPTH+1 slots and the loop body covers all of them, so a
+            // load lands in the slot consumed one step earlier, whose value is
+            // dead: no register copy (and no wait for the load) at the back-edge.
+            constexpr int RING = DEPTH + 1;
+            Pending<VEC> ring[RING];
 #pragma unroll
-            for (int k = 0; k < DEPTH; ++k)
-                if (s0 + k < s1) fetch(rec_at(k), ring[k]);
+            for (int k = 0; k < RING; ++k) {  // slots not yet fetched match no rating
+                ring[k].b = -1;
+                ring[k].ud = 0xffffffffu;
+            }
+#pragma unroll
+            for (int k = 0; k < DEPTH; ++k) fetch(rec_at(s0 + k < s1 ? s0 + k : s1 - 1), ring[k]);
 
-            static_assert(TEAM % DEPTH == 0, "prefetch depth must divide the team size");
-            for (uint32_t tb = s0; tb < s1; tb += TEAM) {
-            for (uint32_t t0 = tb; t0 < tb + TEAM && t0 < s1; t0 += DEPTH) {
+            static_assert(TEAM % RING == 0, "the prefetch depth plus one must divide the team size");
+            for (; tb < s1; tb += TEAM) {
+            for (uint32_t t0 = tb; t0 < tb + TEAM && t0 < s1; t0 += RING) {
 #pragma unroll
-                for (int k = 0; k < DEPTH; ++k) {
+                for (int k = 0; k < RING; ++k) {
                 const uint32_t t = t0 + k;
                 if (t >= s1) break;
-                const Pending<VEC> cur = ring[k];
-                if (t + DEPTH < s1) fetch(rec_at(t + DEPTH - tb), ring[k]);
+                const Pending<VEC>& cur = ring[k];
+                fetch(rec_at(t + DEPTH < s1 ? t + DEPTH : s1 - 1), ring[(k + DEPTH) % RING]);
                 const uint32_t i = cur.i;
                 const float dev = cur.dev;
                 const float c_ib = team_bcast_f<TEAM>(cur.sc, 0), c_bt = team_bcast_f<TEAM>(cur.sc, 1);
-                const float c_bud = team_bcast_f<TEAM>(cur.sc, 2), c_ibf = team_bcast_f<TEAM>(cur.sc, 3);
-                const float c_cut = team_bcast_f<TEAM>(cur.sc2, 0), c_but = team_bcast_f<TEAM>(cur.sc2, 1);
+                const float c_ibf = m.skip_ibf ? 0.0f : team_bcast_f<TEAM>(cur.sc, 3);
+                const float c_bud = cur.has_bud ? cur.bud : team_bcast_f<TEAM>(cur.sc, 2);
+                const float c_cut = m.skip_cut ? 0.0f : team_bcast_f<TEAM>(cur.sc2, 0);
+                const float c_but = cur.has_but ? cur.but : (m.skip_but ? 0.0f : team_bcast_f<TEAM>(cur.sc2, 1));
                 const float bi_t = c_ib + c_bt + c_ibf;
                 const float cu_t = cu + c_cut;
                 const float bu_t = ubias + c_but + alpha * dev + c_bud + su_b;
@@ -617,25 +745,15 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
                 const float but_new = c_but - h.lr_tu * err_b;
                 const float bud_new = c_bud - h.lr_ubd * err_b;
 #pragma unroll
-                for (int j = 0; j < DEPTH; ++j) {
-                    if (ring[j].b == cur.b && lane == 1) ring[j].sc2 = but_new;
-                    if (ring[j].ud == cur.ud && lane == 2) ring[j].sc = bud_new;
+                for (int jj = 1; jj <= DEPTH; ++jj) {
+                    Pending<VEC>& nx = ring[(k + jj) % RING];  // ratings t+1 .. t+DEPTH
+                    if (nx.b == cur.b) { nx.has_but = true; nx.but = but_new; }
+                    if (nx.ud == cur.ud) { nx.has_bud = true; nx.bud = bud_new; }
                 }
-                {
-                    // item biases in one instruction (lanes 0, 1, 3), user biases in one (lanes 1, 2)
-                    const float gi = err_b * cu_t;
-                    float* a = nullptr;
-                    float d = 0.0f;
-                    if (lane == 0) { a = &m.ibias[i]; d = -h.lr_ib * gi; }
-                    else if (lane == 1) { a = &m.bit_bin[static_cast<size_t>(i) * B + cur.b]; d = -h.lr_ti * gi; }
-                    else if (lane == 3 && !m.skip_ibf) { a = &m.ibias_freq[static_cast<size_t>(i) * F]; d = -h.lr_ibf * gi; }
-                    if (a) apply_delta(a, cur.sc, d, qm);
-                    float* w = nullptr;
-                    float wv = 0.0f;
-                    if (lane == 2) { w = &m.ubias_day[cur.ud]; wv = bud_new; }
-                    else if (lane == 1 && !m.skip_but) { w = &m.but_bin[ubrow + cur.b]; wv = but_new; }
-                    if (w) *w = wv;
-                }
+                // item biases in one instruction (lanes 0, 1, 3), b_u,day in one (lane 2)
+                if (sc_item) apply_delta(cur.sc_addr, cur.sc, -sc_lr * (err_b * cu_t), qm);
+                else if (lane == 2) *cur.sc_addr = bud_new;
+                if (lane == 1 && !m.skip_but) m.but_bin[ubrow + cur.b] = but_new;
                 cu -= h.lr_cu * (err_b * bi_t + h.reg_cu * (cu - 1.0f));
                 Gb += m.sum_err_bug ? err_b * static_cast<float>(m.f) : err_b;
                 float dq[VEC], dqf[VEC];
@@ -649,13 +767,15 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
                     dq[c] = -h.lr_i * (err_d * pe + h.reg_i * cur.q[c]);
                     if constexpr (QF) dqf[c] = -h.lr_iqf * (err_d * pe + h.reg_iqf * cur.qf[c]);
                 }
-                apply_vec<VEC, TEAM>(m.ifeat + static_cast<size_t>(i) * fs + off, cur.q, dq, qm);
+                apply_vec<VEC, TEAM>(q_base + static_cast<size_t>(i) * fs, cur.q, dq, qm);
                 if constexpr (QF) apply_vec<VEC, TEAM>(m.ifeat_freq + static_cast<size_t>(i) * F * fs + off, cur.qf, dqf, qm);
                 }
             }
             cb = nb;
-            nb = load_recs(tb + 2 * TEAM);
+            nb = fb;
+            fb = load_recs(tb + 3 * TEAM);
             }
+            }  // s0 < s1
 
             store_str<VEC, TEAM>(m.ufeat + prow, p);
             if constexpr (P2) store_str<VEC, TEAM>(m.ufeat2 + prow, p2);
@@ -687,24 +807,18 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                     __builtin_amdgcn_wave_barrier();
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                    for (uint32_t t0 = ts; t0 < te; t0 += 4) {
-                        float y[4][VEC];
-                        uint32_t jj[4];
-#pragma unroll
-                        for (int k = 0; k < 4; ++k) {
-                            jj[k] = t0 + k < te ? m.t_item[t0 + k] : 0xffffffffu;
-                            if (jj[k] != 0xffffffffu) load_vec<VEC>(m.yfeat + static_cast<size_t>(jj[k]) * fs + coff, y[k]);
-                        }
-#pragma unroll
-                        for (int k = 0; k < 4; ++k) {
-                            if (jj[k] == 0xffffffffu) continue;
+                    const uint32_t* titems = m.t_item + ts;
+                    for_rows<TEAM, VEC, row_group<TEAM, VEC>()>(
+                        te - ts, lane, m.yfeat, fs, [&](uint32_t r) { return titems + r; },
+                        [&](uint32_t b0, uint32_t ib) {
+                            if (gb != 0.0f && b0 + static_cast<uint32_t>(lane) < te - ts) m.ybias[ib] = m.ybias[ib] - gb;
+                        },
+                        [&](uint32_t, uint32_t j, const float (&y)[VEC]) {
                             float ny[VEC];
 #pragma unroll
-                            for (int c = 0; c < VEC; ++c) ny[c] = y[k][c] - h.lr_y * (gc[c] + h.reg_y * y[k][c]);
-                            store_vec<VEC>(m.yfeat + static_cast<size_t>(jj[k]) * fs + coff, ny);
-                            if (gb != 0.0f && lane == ((t0 + k) & (TEAM - 1))) m.ybias[jj[k]] = m.ybias[jj[k]] - gb;
-                        }
-                    }
+                            for (int c = 0; c < VEC; ++c) ny[c] = y[c] - h.lr_y * (gc[c] + h.reg_y * y[c]);
+                            store_vec<VEC>(m.yfeat + static_cast<size_t>(j) * fs + coff, ny);
+                        });
                 } else {
                 // rows of batch b+1 are loaded before batch b's updates are issued
                     float ya[4][VEC], yb[4][VEC];
@@ -961,6 +1075,8 @@ static inline int grid_for(int64_t teams, int team) {
 }
 
 // Dispatch on (TEAM, VEC): the shapes the host picks for a padded row stride.
+// (A build may predefine TX_SHAPES to a subset, e.g. to inspect one kernel.)
+#ifndef TX_SHAPES
 #define TX_SHAPES(X)  \
     X(4, 4)           \
     X(8, 4)           \
@@ -969,6 +1085,7 @@ static inline int grid_for(int64_t teams, int team) {
     X(64, 4)          \
     X(64, 8)          \
     X(64, 16)
+#endif
 
 hipError_t launch_sgd_serial(const DevModel& m, const int64_t* order, float* scratch, hipStream_t st) {
     const size_t shm = sizeof(float) * (static_cast<size_t>(m.fs) + 1);

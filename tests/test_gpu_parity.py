@@ -104,17 +104,30 @@ def test_parallel_schedule_holdout_rmse(gpu, n_strata):
     assert abs(r_g - r_o) <= 1e-3
 
 
-def test_full_shape_holdout_rmse_matches_serial_reference(gpu):
-    """Parity (b) at the full Netflix shape: plain form, f=64, 20 epochs in the
+_FULL = {}
+
+
+def _full_data(seed):
+    """The full-shape synthetic set, generated once per process."""
+    if seed not in _FULL:
+        from netflix_prize_amd import synth
+        _FULL[seed] = synth.generate(synth.FULL, seed)
+    return _FULL[seed]
+
+
+@pytest.mark.parametrize("f", [64, 256])
+def test_full_shape_holdout_rmse_matches_serial_reference(gpu, f):
+    """Parity (b) at the full Netflix shape: plain form, 20 epochs in the
     default parallel schedule vs the serial oracle's holdout RMSE
-    (tests/golden/oracle_curve_full_f64.json, made by make_oracle_curves.py)."""
+    (tests/golden/oracle_curve_full_f{64,256}.json, made by
+    make_oracle_curves.py): within 1e-3 after the last epoch."""
     import json
     import os
-    from netflix_prize_amd import synth
-    golden = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "oracle_curve_full_f64.json")))
-    tr, pr = synth.generate(synth.FULL, golden["seed"])
+    golden = json.load(open(os.path.join(os.path.dirname(__file__), "golden", f"oracle_curve_full_f{f}.json")))
+    assert len(golden["epochs"]) == 20
+    tr, pr = _full_data(golden["seed"])
     assert tr.n_ratings == golden["n_train"] and pr.n_ratings == golden["n_heldout"]
-    cfg = tx.plain_form(lm_config(n_feat=64, n_epochs=20, n_time_bins=30, reset_u_epoch=10_000))
+    cfg = tx.plain_form(lm_config(n_feat=f, n_epochs=20, n_time_bins=30, reset_u_epoch=10_000))
     with tx.TxModel(tr, pr, cfg) as m:
         curve = []
         for e in range(1, 21):
