@@ -38,6 +38,20 @@ def algorithmic_bytes_per_rating(f: int, n_train: int, n_users: int, qf: bool, p
     return {"sgd": sgd, "rebuild_per_row": s * f, "ycache_per_user": s * f}
 
 
+def profiled_traffic(f: int, args) -> float | None:
+    """HBM bytes per SGD launch (FETCH_SIZE + WRITE_SIZE, calibrated against a
+    plain copy) from the rocprofv3 summary of this workload in profiles/, or
+    None for a workload that has no such summary."""
+    path = os.path.join(ROOT, "profiles", f"k_sgd_par_{args.shape}_f{f}.md")
+    if not os.path.exists(path) or args.strata != 1:
+        return None
+    text = open(path).read()
+    if "```json" not in text:
+        return None
+    block = text.split("```json", 1)[1].split("```", 1)[0]
+    return float(json.loads(block)["traffic_bytes_per_launch"])
+
+
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
@@ -207,7 +221,7 @@ def main(argv=None) -> int:
     bytes_per_launch = ab["sgd"] * n_train / launches
     achieved = bytes_per_launch / (avg_launch_ms / 1000.0) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(cfg.n_feat, args),
                 "kernel": "k_sgd_par", "bytes_per_rating": round(ab["sgd"], 1),
                 "avg_launch_ms": round(avg_launch_ms, 3), "launches_per_epoch": launches}
 
