@@ -81,8 +81,14 @@ typedef struct tx_ratings {
  *     identical to the serial reference trainer (src/tx.rs:734-1115).
  *   TX_SCHED_STRATA: all users in parallel, one team of lanes per user, items
  *     split into n_strata blocks processed as a rotating user-block x
- *     item-block schedule (n_strata = 1: every item in one stratum). */
-enum tx_sched_mode { TX_SCHED_SERIAL = 0, TX_SCHED_STRATA = 1 };
+ *     item-block schedule (n_strata = 1: every item in one stratum).
+ *   TX_SCHED_TSVDX4P: the epoch of Tsvdx4pModel::fit_epoch
+ *     (src/tsvdx4p.rs:798-805): a user pass with item tables frozen and the
+ *     NSVD1 gradients accumulated per item, the delta apply divided by
+ *     n_contrib^nsvd_norm_exp, the ycache rebuild, then an item pass with user
+ *     tables frozen; epochs listed in cfg.seq_epochs run the TX_SCHED_STRATA
+ *     epoch instead.  Low-memory models without q_f stream or ordinal head. */
+enum tx_sched_mode { TX_SCHED_SERIAL = 0, TX_SCHED_STRATA = 1, TX_SCHED_TSVDX4P = 2 };
 
 typedef struct tx_schedule {
     int32_t mode;
@@ -164,7 +170,14 @@ int tx_set_schedule(tx_ctx* ctx, const tx_schedule* sched);
  * 1-based.  Includes the epoch-end ycache rebuild (src/tx.rs:1109-1110). */
 int tx_fit_epoch(tx_ctx* ctx, int32_t epoch);
 
-/* Timing of the last tx_fit_epoch. */
+/* One user-only epoch: every user's parameters (p, p2, b_u, b_u,bin, alpha_u,
+ * b_u,day, c_u) trained by SGD over its training ratings with all item
+ * tables and y frozen, then the ycache rebuild.  This is the user pass of
+ * Tsvdx4pModel (fit_epoch_user, src/tsvdx4p.rs:390-639) without its NSVD1
+ * accumulators: folding users into fixed item tables. */
+int tx_fit_users_epoch(tx_ctx* ctx, int32_t epoch);
+
+/* Timing of the last tx_fit_epoch / tx_fit_users_epoch. */
 int tx_last_epoch_stats(tx_ctx* ctx, tx_epoch_stats* out);
 
 /* Regressor::predict (src/tx.rs:713-724) for n (user, item, day) triples

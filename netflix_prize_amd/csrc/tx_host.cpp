@@ -45,6 +45,9 @@ int fail(int code, const char* fmt, ...) {
         if (e_ != hipSuccess) return fail(TX_EDEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
     } while (0)
 
+#define RC(x) \
+    if (int rc_ = (x)) return rc_
+
 // Run fn(lo, hi) over [0, n) on up to hardware_concurrency threads.
 template <typename Fn>
 void parallel_for(int64_t n, Fn fn) {
@@ -115,7 +118,13 @@ struct tx_ctx {
     std::vector<float> t_dev, t_r;
     std::vector<uint8_t> t_bin;
     std::vector<int8_t> t_raw;
+    std::vector<uint8_t> sud_fbin;
     uint4* d_rec = nullptr;  // packed records, reference order
+
+    // tsvdx4p schedule: ratings by item, NSVD1 accumulators, per-item scales
+    bool iv_built = false;
+    tx::ItemView iv{};
+    float* d_scale = nullptr;
 
     // default: one stratum, atomic item updates, lock-free NSVD1 row updates
     tx_schedule sched{TX_SCHED_STRATA, 1, 0, 0, 0, TX_UPD_ATOMIC | (TX_UPD_PLAIN << 2), 0, {0}};
@@ -417,6 +426,120 @@ int default_blocks(tx_ctx* c) {
     return std::max(8, (blocks + 7) / 8 * 8);
 }
 
+// Users of the epoch in the reference's order (get_users, src/lib.rs:486-494)
+// into the one-stratum queue.
+int prepare_user_queue(tx_ctx* c, int32_t epoch, hipStream_t st) {
+    std::vector<int64_t> order(c->U);
+    tx::user_order(c->U, c->cfg.shuffle_users != 0, c->cfg.seed, epoch, order.data());
+    c->h_queue.clear();
+    for (int64_t u : order)
+        if (c->t_off[u + 1] > c->t_off[u]) c->h_queue.push_back(static_cast<uint32_t>(u));
+    HIPC(hipMemcpyAsync(const_cast<uint32_t*>(c->sv.q_users), c->h_queue.data(), c->h_queue.size() * sizeof(uint32_t),
+                        hipMemcpyHostToDevice, st));
+    return 0;
+}
+
+// The tsvdx4p layout: ratings grouped by item in training-file order
+// (ByItemIndex, src/tsvdx4p.rs:29-57) with the user, bins, dev and target of
+// each; items queued most ratings first; NSVD1 accumulators; per-item scale
+// n_contrib^nsvd_norm_exp (src/tsvdx4p.rs:356-365) with the host libm powf.
+int build_item_view(tx_ctx* c) {
+    if (c->iv_built) return 0;
+    const int64_t I = c->I, N = c->N;
+    if (c->U > (1 << 20) || c->B > 256 || c->F > 16)
+        return fail(TX_EUNSUPPORTED, "tsvdx4p schedule: needs < 2^20 users, <= 256 time bins, <= 16 freq bins");
+    std::vector<uint32_t> start(I + 1, 0);
+    for (int64_t t = 0; t < N; ++t) start[c->t_item[t] + 1]++;
+    for (int64_t i = 0; i < I; ++i) start[i + 1] += start[i];
+    std::vector<uint32_t> pos(start.begin(), start.end() - 1);
+    std::vector<uint4> rec(std::max<int64_t>(N, 1));
+    for (int64_t u = 0; u < c->U; ++u)
+        for (uint32_t t = c->t_off[u]; t < c->t_off[u + 1]; ++t) {
+            const uint32_t i = c->t_item[t], ud = c->t_ud[t];
+            uint4 r;
+            r.x = static_cast<uint32_t>(u) | (static_cast<uint32_t>(c->t_bin[t]) << 20) |
+                  (static_cast<uint32_t>(c->sud_fbin[ud]) << 28);
+            r.y = ud;
+            std::memcpy(&r.z, &c->t_dev[t], 4);
+            std::memcpy(&r.w, &c->t_r[t], 4);
+            rec[pos[i]++] = r;
+        }
+    std::vector<uint32_t> items;
+    for (int64_t i = 0; i < I; ++i)
+        if (start[i + 1] > start[i]) items.push_back(static_cast<uint32_t>(i));
+    std::stable_sort(items.begin(), items.end(),
+                     [&](uint32_t a, uint32_t b) { return start[a + 1] - start[a] > start[b + 1] - start[b]; });
+    std::vector<float> scale(std::max<int64_t>(I, 1), 0.0f);
+    const float ex = c->cfg.nsvd_norm_exp;
+    for (int64_t i = 0; i < I; ++i) {
+        const uint32_t nc = start[i + 1] - start[i];
+        if (nc) scale[i] = ex == 0.0f ? 1.0f : std::pow(static_cast<float>(nc), ex);
+    }
+    uint32_t *d_start = nullptr, *d_items = nullptr, *d_head = nullptr;
+    uint4* d_bi = nullptr;
+    RC(c->upload(&d_start, start.data(), start.size()));
+    RC(c->upload(&d_bi, rec.data(), rec.size()));
+    if (items.empty()) items.push_back(0);
+    RC(c->upload(&d_items, items.data(), items.size()));
+    RC(c->alloc(&d_head, 1));
+    RC(c->upload(&c->d_scale, scale.data(), scale.size()));
+    float *dy = nullptr, *dyb = nullptr;
+    RC(c->alloc(&dy, static_cast<size_t>(std::max<int64_t>(I, 1)) * c->fs));
+    RC(c->alloc(&dyb, static_cast<size_t>(std::max<int64_t>(I, 1))));
+    HIPC(hipMemset(dy, 0, sizeof(float) * static_cast<size_t>(std::max<int64_t>(I, 1)) * c->fs));
+    HIPC(hipMemset(dyb, 0, sizeof(float) * static_cast<size_t>(std::max<int64_t>(I, 1))));
+    c->iv.q_items = d_items;
+    c->iv.n_items = start[I] ? static_cast<uint32_t>(items.size()) : 0u;
+    c->iv.q_head = d_head;
+    c->iv.bi_start = d_start;
+    c->iv.bi_rec = d_bi;
+    c->sv.d_y = dy;
+    c->sv.d_yb = dyb;
+    c->iv_built = true;
+    return 0;
+}
+
+int user_pass_blocks(tx_ctx* c) {
+    // users are independent in the user passes: fill the device
+    const int teams_per_block = 256 / c->shape.team;
+    const int blocks = (4096 + teams_per_block - 1) / teams_per_block;
+    return std::max(8, (blocks + 7) / 8 * 8);
+}
+
+// One Tsvdx4pModel epoch (src/tsvdx4p.rs:798-805): user pass with NSVD1
+// accumulators, delta apply, ycache rebuild, item pass.
+int tsvdx4p_epoch(tx_ctx* c, int32_t epoch, tx_epoch_stats& stats, hipStream_t st) {
+    const bool qf = c->cfg.lr_iqf != 0.0f || c->cfg.sigma_iqf != 0.0f;
+    const bool p2 = c->cfg.lr_u2 != 0.0f;
+    if (!c->cfg.low_memory || qf || c->cfg.has_ordinal)
+        return fail(TX_EUNSUPPORTED, "tsvdx4p schedule: low-memory form without the q_f stream or ordinal head only");
+    if (c->built_S != 1) {
+        if (c->built_S != 0) return fail(TX_EUNSUPPORTED, "tsvdx4p schedule: needs the one-stratum layout");
+        if (int rc = build_strata(c, 1)) return rc;
+    }
+    if (int rc = build_item_view(c)) return rc;
+    if (int rc = prepare_user_queue(c, epoch, st)) return rc;
+    const int sgpr_blocks = c->sched.blocks > 0 ? c->sched.blocks : user_pass_blocks(c);
+    HIPC(tx::launch_fill_u32(c->sv.q_head, 0u, c->sv.n_cells, st));
+    HIPC(tx::launch_fill_u32(c->iv.q_head, 0u, 1, st));
+    HIPC(hipEventRecord(c->ev[1], st));
+    HIPC(tx::launch_sgd_par(c->dm, c->sv, 0, c->shape.team, c->shape.vec, sgpr_blocks, false, p2, false, 1, st));
+    HIPC(tx::launch_apply_nsvd(c->dm, c->sv.d_y, c->sv.d_yb, c->d_scale, st));
+    HIPC(hipEventRecord(c->ev[2], st));
+    HIPC(tx::launch_rebuild(c->dm, c->n_ud, c->shape.team, c->shape.vec, st));
+    HIPC(hipEventRecord(c->ev[3], st));
+    HIPC(tx::launch_item_pass(c->dm, c->iv, c->shape.team, c->shape.vec, p2, user_pass_blocks(c), st));
+    HIPC(hipEventSynchronize(c->ev[3]));
+    HIPC(hipStreamSynchronize(st));
+    float a = 0, b = 0;
+    HIPC(hipEventElapsedTime(&a, c->ev[1], c->ev[2]));
+    HIPC(hipEventElapsedTime(&b, c->ev[2], c->ev[3]));
+    stats.ms_sgd = a;
+    stats.ms_rebuild = b;
+    stats.n_launches = 4;
+    return 0;
+}
+
 // Enqueue one parallel epoch (queue reset, S strata, ycache rebuild).
 int enqueue_strata_epoch(tx_ctx* c, hipStream_t st) {
     const int S = c->built_S;
@@ -426,7 +549,7 @@ int enqueue_strata_epoch(tx_ctx* c, hipStream_t st) {
     const int blocks = c->sched.blocks > 0 ? c->sched.blocks : default_blocks(c);
     HIPC(tx::launch_fill_u32(c->sv.q_head, 0u, static_cast<int64_t>(S) * c->sv.n_cells, st));
     for (int s = 0; s < S; ++s)
-        HIPC(tx::launch_sgd_par(c->dm, c->sv, s, c->shape.team, c->shape.vec, blocks, qf, p2, ord, st));
+        HIPC(tx::launch_sgd_par(c->dm, c->sv, s, c->shape.team, c->shape.vec, blocks, qf, p2, ord, 0, st));
     HIPC(tx::launch_rebuild(c->dm, c->n_ud, c->shape.team, c->shape.vec, st));
     return 0;
 }
@@ -611,7 +734,8 @@ int tx_create(const tx_config* cfg, const tx_ratings* tr, const tx_ratings* pr, 
         }
     });
 
-    std::vector<uint8_t> sud_fbin(D);
+    std::vector<uint8_t>& sud_fbin = c->sud_fbin;
+    sud_fbin.assign(D, 0);
     for (int64_t d = 0; d < D; ++d) sud_fbin[d] = static_cast<uint8_t>(freq_bin_host(sud_cnt[d], F));
 
     // norms: training su (src/tx.rs:808-819) and epoch-end rebuild (src/tx.rs:410-416)
@@ -715,8 +839,6 @@ int tx_create(const tx_config* cfg, const tx_ratings* tr, const tx_ratings* pr, 
     m.skip_yb = cfg->lr_yb == 0.0f ? 1 : 0;
     m.gbias = gbias;
     m.h = make_hyper(*cfg);
-#define RC(x) \
-    if (int rc_ = (x)) return rc_
     RC(zeros(c, &m.ubias, U));
     RC(zeros(c, &m.ibias, I));
     RC(upload_padded(c, &m.ufeat, h_ufeat.data(), U));
@@ -795,7 +917,14 @@ int tx_destroy(tx_ctx* ctx) {
 int tx_set_schedule(tx_ctx* c, const tx_schedule* s) {
     g_err.clear();
     if (!c || !s) return fail(TX_EINVAL, "tx_set_schedule: NULL argument");
-    if (s->mode != TX_SCHED_SERIAL && s->mode != TX_SCHED_STRATA) return fail(TX_EINVAL, "unknown schedule mode %d", s->mode);
+    if (s->mode != TX_SCHED_SERIAL && s->mode != TX_SCHED_STRATA && s->mode != TX_SCHED_TSVDX4P)
+        return fail(TX_EINVAL, "unknown schedule mode %d", s->mode);
+    if (s->mode == TX_SCHED_TSVDX4P) {
+        if (!c->cfg.low_memory || c->cfg.has_ordinal || c->cfg.lr_iqf != 0.0f || c->cfg.sigma_iqf != 0.0f)
+            return fail(TX_EUNSUPPORTED,
+                        "the tsvdx4p schedule trains low_memory models without the q_f stream or ordinal head");
+        if (s->blocks < 0) return fail(TX_EINVAL, "blocks must be 0 (auto) or positive");
+    }
     if (s->mode == TX_SCHED_STRATA) {
         if (s->n_strata < 1 || (s->n_strata > 1 && s->n_strata % 8 != 0) || s->n_strata > 64)
             return fail(TX_EINVAL, "n_strata must be 1 or a multiple of 8 (<= 64), got %d", s->n_strata);
@@ -809,10 +938,12 @@ int tx_set_schedule(tx_ctx* c, const tx_schedule* s) {
     if (s->user_order != 0 && s->user_order != 1) return fail(TX_EINVAL, "user_order must be 0 or 1");
     const int qm = s->update_flags & 3, ym = (s->update_flags >> 2) & 3;
     if (qm > 2 || ym > 2 || (s->update_flags >> 4) != 0) return fail(TX_EINVAL, "bad update_flags %d", s->update_flags);
-    const bool relayout = s->mode == TX_SCHED_STRATA && s->n_strata != c->built_S;
+    const int want_S = s->mode == TX_SCHED_TSVDX4P ? 1 : s->n_strata;
+    const bool relayout = s->mode != TX_SCHED_SERIAL && want_S != c->built_S;
     c->sched = *s;
+    if (s->mode == TX_SCHED_TSVDX4P) c->sched.n_strata = 1;
     if (relayout) {
-        if (int rc = build_strata(c, s->n_strata)) return rc;
+        if (int rc = build_strata(c, want_S)) return rc;
     }
     c->sv.q_mode = qm;
     c->sv.y_mode = ym;
@@ -836,7 +967,17 @@ int tx_fit_epoch(tx_ctx* c, int32_t epoch) {
         HIPC(tx::launch_fill_f32(c->dm.ufeat, 0.0f, static_cast<int64_t>(c->U) * c->fs, st));
         HIPC(tx::launch_fill_f32(c->dm.ufeat2, 0.0f, static_cast<int64_t>(c->U) * c->fs, st));
     }
-    if (c->sched.mode == TX_SCHED_SERIAL) {
+    bool seq_epoch = false;  // Tsvdx4pConfig::seq_epochs run the TxModel epoch (src/tsvdx4p.rs:799-800)
+    for (int k = 0; k < 5; ++k) seq_epoch = seq_epoch || c->cfg.seq_epochs[k] == epoch;
+    if (c->sched.mode == TX_SCHED_TSVDX4P && !seq_epoch) {
+        if (int rc = tsvdx4p_epoch(c, epoch, stats, st)) return rc;
+        HIPC(hipEventRecord(c->ev[3], st));
+        HIPC(hipEventSynchronize(c->ev[3]));
+        float t = 0;
+        HIPC(hipEventElapsedTime(&t, c->ev[0], c->ev[3]));
+        stats.ms_total = t;
+        stats.ms_other = t - stats.ms_sgd - stats.ms_rebuild;
+    } else if (c->sched.mode == TX_SCHED_SERIAL) {
         std::vector<int64_t> order(c->U);
         tx::user_order(c->U, c->cfg.shuffle_users != 0, c->cfg.seed, epoch, order.data());
         HIPC(hipMemcpyAsync(c->d_order, order.data(), sizeof(int64_t) * c->U, hipMemcpyHostToDevice, st));
@@ -904,7 +1045,7 @@ int tx_fit_epoch(tx_ctx* c, int32_t epoch) {
             HIPC(tx::launch_fill_u32(c->sv.q_head, 0u, static_cast<int64_t>(S) * c->sv.n_cells, st));
             HIPC(hipEventRecord(c->ev[1], st));
             for (int s = 0; s < S; ++s)
-                HIPC(tx::launch_sgd_par(c->dm, c->sv, s, c->shape.team, c->shape.vec, blocks, qf, p2, ord, st));
+                HIPC(tx::launch_sgd_par(c->dm, c->sv, s, c->shape.team, c->shape.vec, blocks, qf, p2, ord, 0, st));
             HIPC(hipEventRecord(c->ev[2], st));
             HIPC(tx::launch_rebuild(c->dm, c->n_ud, c->shape.team, c->shape.vec, st));
             HIPC(hipEventRecord(c->ev[3], st));
@@ -920,6 +1061,43 @@ int tx_fit_epoch(tx_ctx* c, int32_t epoch) {
             stats.n_launches = S + 2;
         }
     }
+    c->last = stats;
+    return 0;
+}
+
+int tx_fit_users_epoch(tx_ctx* c, int32_t epoch) {
+    g_err.clear();
+    if (!c) return fail(TX_EINVAL, "tx_fit_users_epoch: NULL ctx");
+    if (epoch < 1) return fail(TX_EINVAL, "tx_fit_users_epoch: epochs are 1-based");
+    if (!c->cfg.low_memory || c->cfg.has_ordinal)
+        return fail(TX_EUNSUPPORTED, "tx_fit_users_epoch: low-memory models without ordinal head only");
+    if (c->built_S > 1) return fail(TX_EUNSUPPORTED, "tx_fit_users_epoch: the model uses a stratified layout");
+    if (c->built_S == 0)
+        if (int rc = build_strata(c, 1)) return rc;
+    hipStream_t st = c->st;
+    tx_epoch_stats stats{};
+    stats.n_ratings = c->N;
+    HIPC(hipEventRecord(c->ev[0], st));
+    if (int rc = prepare_user_queue(c, epoch, st)) return rc;
+    const bool qf = c->cfg.lr_iqf != 0.0f || c->cfg.sigma_iqf != 0.0f;
+    const bool p2 = c->cfg.lr_u2 != 0.0f;
+    HIPC(tx::launch_fill_u32(c->sv.q_head, 0u, c->sv.n_cells, st));
+    HIPC(hipEventRecord(c->ev[1], st));
+    const int blocks = c->sched.blocks > 0 ? c->sched.blocks : user_pass_blocks(c);
+    HIPC(tx::launch_sgd_par(c->dm, c->sv, 0, c->shape.team, c->shape.vec, blocks, qf, p2, false, 2, st));
+    HIPC(hipEventRecord(c->ev[2], st));
+    HIPC(tx::launch_rebuild(c->dm, c->n_ud, c->shape.team, c->shape.vec, st));
+    HIPC(hipEventRecord(c->ev[3], st));
+    HIPC(hipEventSynchronize(c->ev[3]));
+    float a = 0, b = 0, t = 0;
+    HIPC(hipEventElapsedTime(&a, c->ev[1], c->ev[2]));
+    HIPC(hipEventElapsedTime(&b, c->ev[2], c->ev[3]));
+    HIPC(hipEventElapsedTime(&t, c->ev[0], c->ev[3]));
+    stats.ms_sgd = a;
+    stats.ms_rebuild = b;
+    stats.ms_total = t;
+    stats.ms_other = t - a - b;
+    stats.n_launches = 2;
     c->last = stats;
     return 0;
 }

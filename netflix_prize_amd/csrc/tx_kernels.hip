@@ -499,8 +499,15 @@ struct Pending {
 #define TX_SGD_WAVES 1
 #endif
 
-template <int TEAM, int VEC, bool QF, bool P2, bool ORD>
+// PASS 0: one SGD epoch of TxModel::fit_epoch (src/tx.rs:734-1115).
+// PASS 1: the user pass of Tsvdx4pModel (src/tsvdx4p.rs:390-639): item tables
+//         frozen, user parameters trained, each user's NSVD1 gradient added to
+//         per-item accumulators (sv.d_y, sv.d_yb) that a later kernel applies.
+// PASS 2: the same user pass without the accumulators (fold-in of users
+//         against fixed item tables).
+template <int TEAM, int VEC, bool QF, bool P2, bool ORD, int PASS>
 __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, StrataView sv, int stratum) {
+    static_assert(PASS == 0 || !ORD, "the user passes have no ordinal head");
     constexpr int CHUNK = 4;
     constexpr int DEPTH = TX_PREFETCH;
     __shared__ float xbuf[256 * VEC];  // per-team transpose slot, TEAM*VEC floats each
@@ -735,8 +742,9 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
                 } else {
                     err = score - cur.r;
                 }
-                const float err_b = err * h.w_bias;
-                const float err_d = err * h.w_factor;
+                // the user passes weight bias and factor errors alike (src/tsvdx4p.rs:560-600)
+                const float err_b = PASS == 0 ? err * h.w_bias : err;
+                const float err_d = PASS == 0 ? err * h.w_factor : err;
 
                 // user-owned biases: registers, or written back now and forwarded
                 // to every prefetched rating that reads the same slot
@@ -751,11 +759,11 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
                     if (nx.ud == cur.ud) { nx.has_bud = true; nx.bud = bud_new; }
                 }
                 // item biases in one instruction (lanes 0, 1, 3), b_u,day in one (lane 2)
-                if (sc_item) apply_delta(cur.sc_addr, cur.sc, -sc_lr * (err_b * cu_t), qm);
+                if (PASS == 0 && sc_item) apply_delta(cur.sc_addr, cur.sc, -sc_lr * (err_b * cu_t), qm);
                 else if (lane == 2) *cur.sc_addr = bud_new;
                 if (lane == 1 && !m.skip_but) m.but_bin[ubrow + cur.b] = but_new;
                 cu -= h.lr_cu * (err_b * bi_t + h.reg_cu * (cu - 1.0f));
-                Gb += m.sum_err_bug ? err_b * static_cast<float>(m.f) : err_b;
+                Gb += (PASS == 0 && m.sum_err_bug) ? err_b * static_cast<float>(m.f) : err_b;
                 float dq[VEC], dqf[VEC];
 #pragma unroll
                 for (int c = 0; c < VEC; ++c) {
@@ -767,8 +775,10 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
                     dq[c] = -h.lr_i * (err_d * pe + h.reg_i * cur.q[c]);
                     if constexpr (QF) dqf[c] = -h.lr_iqf * (err_d * pe + h.reg_iqf * cur.qf[c]);
                 }
-                apply_vec<VEC, TEAM>(q_base + static_cast<size_t>(i) * fs, cur.q, dq, qm);
-                if constexpr (QF) apply_vec<VEC, TEAM>(m.ifeat_freq + static_cast<size_t>(i) * F * fs + off, cur.qf, dqf, qm);
+                if constexpr (PASS == 0) {
+                    apply_vec<VEC, TEAM>(q_base + static_cast<size_t>(i) * fs, cur.q, dq, qm);
+                    if constexpr (QF) apply_vec<VEC, TEAM>(m.ifeat_freq + static_cast<size_t>(i) * F * fs + off, cur.qf, dqf, qm);
+                }
                 }
             }
             cb = nb;
@@ -784,7 +794,30 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
                 m.alpha_u[u] = alpha;
                 m.cu[u] = cu;
             }
-            if (last) {
+            if constexpr (PASS == 1) {
+                // the user's NSVD1 gradient into the per-item accumulators
+                // (NsvdDelta, src/tsvdx4p.rs:622-630): row j += G/norm, ybias
+                // j += sum_err/norm for every training item j of the user
+                float gq[VEC];
+#pragma unroll
+                for (int c = 0; c < VEC; ++c) gq[c] = G[c] / norm;
+                const uint32_t ntr = te - ts;
+                for (uint32_t b0 = 0; b0 < ntr; b0 += TEAM) {
+                    const uint32_t r = b0 + static_cast<uint32_t>(lane);
+                    const uint32_t jl = m.t_item[ts + (r < ntr ? r : ntr - 1)];
+                    if (r < ntr) __hip_atomic_fetch_add(&sv.d_yb[jl], Gb / norm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    const uint32_t nb = ntr - b0 < static_cast<uint32_t>(TEAM) ? ntr - b0 : static_cast<uint32_t>(TEAM);
+                    for (uint32_t k = 0; k < nb; ++k) {
+                        const uint32_t j = team_bcast<TEAM>(jl, k);
+                        float* row = sv.d_y + static_cast<size_t>(j) * fs + off;
+#pragma unroll
+                        for (int c = 0; c < VEC; ++c)
+                            __hip_atomic_fetch_add(row + c * TEAM, gq[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                }
+            } else if (PASS == 2) {
+                // fold-in: nothing beyond the user's own parameters
+            } else if (last) {
                 // end-of-user NSVD1 step over R(u) (src/tx.rs:1099-1106)
                 const float inv = 1.0f / norm;
                 float gq[VEC];
@@ -856,6 +889,150 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
                 if (lane == 0) sv.gacc_b[u] = Gb;
             }
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Tsvdx4pModel: NSVD1 delta apply and the item pass
+// ---------------------------------------------------------------------------
+
+// apply_nsvd_deltas (src/tsvdx4p.rs:341-385): for every item with
+// contributors, y_j -= lr_y (D_j / s_j + reg_y y_j) and ybias_j -= lr_yb
+// D^b_j / s_j, with s_j = n_j^nsvd_norm_exp evaluated on the host (0 marks an
+// item without contributors); the accumulators are cleared for the next pass.
+__global__ __launch_bounds__(256) void k_apply_nsvd(DevModel m, float* d_y, float* d_yb, const float* scale) {
+#pragma clang fp contract(off)
+    const int64_t n = static_cast<int64_t>(m.I) * m.fs;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < n; e += stride) {
+        const int64_t j = e / m.fs;
+        const int k = static_cast<int>(e - j * m.fs);
+        const float sc = scale[j];
+        if (sc > 0.0f && k < m.f) {
+            const float y = m.yfeat[e];
+            m.yfeat[e] = y - m.h.lr_y * (d_y[e] / sc + m.h.reg_y * y);
+        }
+        d_y[e] = 0.0f;
+        if (k == 0) {
+            if (sc > 0.0f) m.ybias[j] = m.ybias[j] - m.h.lr_yb * (d_yb[j] / sc);
+            d_yb[j] = 0.0f;
+        }
+    }
+}
+
+// One rating of the item pass and its frozen user-side inputs.
+//   sc:  lane 0 b_u, 1 b_u,bin, 2 alpha_u, 3 b_u,day
+//   sc2: lane 0 ycache_bias, 1 c_u, 2 c_u,day
+template <int VEC, bool P2>
+struct ItemPending {
+    uint32_t b, fb;
+    float dev, r;
+    float pe[VEC];  // p + ycache + dev * p2, contiguous layout
+    float sc, sc2;
+};
+
+// fit_epoch_item (src/tsvdx4p.rs:644-762): one team per item (longest first;
+// items are independent), the item's ratings in training-file order with every
+// user-side value frozen.  q_i stays in registers and the item's bias slots
+// (b_i, b_{i,bin}[B], b_{i,f}[F]) in the team's LDS for the whole item; both
+// are written back once.  Low-memory form without the q_f stream.
+template <int TEAM, int VEC, bool P2>
+__global__ __launch_bounds__(256) void k_item_pass(DevModel m, ItemView iv) {
+    constexpr int DEPTH = TX_PREFETCH;
+    constexpr int RING = DEPTH + 1;
+    static_assert(TEAM % RING == 0, "the prefetch depth plus one must divide the team size");
+    extern __shared__ float ish[];
+    const int lane = threadIdx.x % TEAM;
+    const int B = m.B, F = m.F, fs = m.fs;
+    float* bt = ish + (threadIdx.x / TEAM) * (B + F);  // b_{i,bin}[B], then b_{i,f}[F]
+    float* bq = bt + B;
+    const int coff = lane * VEC;
+    const Hyper h = m.h;
+    // lane-indexed user scalar slots: base + u*umul + (b & bm) + (ud & um)
+    const float* sc_base = lane == 1 ? m.but_bin : lane == 2 ? m.alpha_u : lane == 3 ? m.ubias_day : m.ubias;
+    const uint32_t sc_umul = lane == 1 ? static_cast<uint32_t>(B) : lane == 3 ? 0u : 1u;
+    const uint32_t sc_bm = lane == 1 ? 0xffffffffu : 0u;
+    const uint32_t sc_um = lane == 3 ? 0xffffffffu : 0u;
+    const float* sc2_base = lane == 1 ? m.cu : lane == 2 ? m.cut : m.ycache_bias;
+    const uint32_t sc2_umul = lane == 2 ? 0u : 1u;
+    const uint32_t sc2_um = lane == 2 ? 0xffffffffu : 0u;
+
+    for (;;) {
+        uint32_t w = 0;
+        if (lane == 0) w = __hip_atomic_fetch_add(iv.q_head, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if constexpr (TEAM == 64) w = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(w)));
+        else w = __shfl(w, 0, TEAM);
+        if (w >= iv.n_items) break;
+        const uint32_t i = iv.q_items[w];
+        const uint32_t r0 = iv.bi_start[i], r1 = iv.bi_start[i + 1];
+        float q[VEC];
+        load_vec<VEC>(m.ifeat + static_cast<size_t>(i) * fs + coff, q);
+        float ib = m.ibias[i];
+        for (int l = lane; l < B; l += TEAM) bt[l] = m.bit_bin[static_cast<size_t>(i) * B + l];
+        for (int l = lane; l < F; l += TEAM) bq[l] = m.ibias_freq[static_cast<size_t>(i) * F + l];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+        auto fetch = [&](uint32_t t, ItemPending<VEC, P2>& n) {
+            const uint4 rc = iv.bi_rec[t];  // same address in every lane of the team
+            const uint32_t u = rc.x & 0xFFFFFu;
+            n.b = (rc.x >> 20) & 0xFFu;
+            n.fb = rc.x >> 28;
+            n.dev = __uint_as_float(rc.z);
+            n.r = __uint_as_float(rc.w);
+            float p[VEC], s[VEC], p2[VEC];
+            load_vec<VEC>(m.ufeat + static_cast<size_t>(u) * fs + coff, p);
+            load_vec<VEC>(m.ycache + static_cast<size_t>(u) * fs + coff, s);
+            if constexpr (P2) load_vec<VEC>(m.ufeat2 + static_cast<size_t>(u) * fs + coff, p2);
+#pragma unroll
+            for (int c = 0; c < VEC; ++c) n.pe[c] = P2 ? p[c] + s[c] + n.dev * p2[c] : p[c] + s[c];
+            n.sc = sc_base[static_cast<size_t>(u) * sc_umul + (n.b & sc_bm) + (rc.y & sc_um)];
+            n.sc2 = sc2_base[static_cast<size_t>(u) * sc2_umul + (rc.y & sc2_um)];
+        };
+        if (r0 < r1) {
+            ItemPending<VEC, P2> ring[RING];
+#pragma unroll
+            for (int k = 0; k < DEPTH; ++k) fetch(r0 + k < r1 ? r0 + k : r1 - 1, ring[k]);
+            for (uint32_t t0 = r0; t0 < r1; t0 += RING) {
+#pragma unroll
+                for (int k = 0; k < RING; ++k) {
+                    const uint32_t t = t0 + k;
+                    if (t >= r1) break;
+                    const ItemPending<VEC, P2>& cur = ring[k];
+                    fetch(t + DEPTH < r1 ? t + DEPTH : r1 - 1, ring[(k + DEPTH) % RING]);
+                    const float c_ub = team_bcast_f<TEAM>(cur.sc, 0), c_but = team_bcast_f<TEAM>(cur.sc, 1);
+                    const float c_al = team_bcast_f<TEAM>(cur.sc, 2), c_bud = team_bcast_f<TEAM>(cur.sc, 3);
+                    const float c_ycb = team_bcast_f<TEAM>(cur.sc2, 0), c_cu = team_bcast_f<TEAM>(cur.sc2, 1);
+                    const float c_cut = team_bcast_f<TEAM>(cur.sc2, 2);
+                    const float c_bt = bt[cur.b], c_bq = bq[cur.fb];
+                    const float bu_t = c_ub + c_but + c_al * cur.dev + c_bud + c_ycb;
+                    const float bi_t = ib + c_bt + c_bq;
+                    const float cu_t = c_cu + c_cut;
+                    float part = 0.0f;
+#pragma unroll
+                    for (int c = 0; c < VEC; ++c) part += cur.pe[c] * q[c];
+                    const float dot = team_sum_dpp<TEAM>(part);
+                    const float err = m.gbias + bu_t + bi_t * cu_t + dot - cur.r;
+                    const float gi = err * cu_t;
+                    ib -= h.lr_ib * gi;
+                    bt[cur.b] = c_bt - h.lr_ti * gi;   // every lane writes the same value
+                    bq[cur.fb] = c_bq - h.lr_ibf * gi;
+#pragma unroll
+                    for (int c = 0; c < VEC; ++c) q[c] -= h.lr_i * (err * cur.pe[c] + h.reg_i * q[c]);
+                }
+            }
+        }
+        store_vec<VEC>(m.ifeat + static_cast<size_t>(i) * fs + coff, q);
+        if (lane == 0) m.ibias[i] = ib;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int l = lane; l < B; l += TEAM) m.bit_bin[static_cast<size_t>(i) * B + l] = bt[l];
+        for (int l = lane; l < F; l += TEAM) m.ibias_freq[static_cast<size_t>(i) * F + l] = bq[l];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 
@@ -1094,27 +1271,65 @@ hipError_t launch_sgd_serial(const DevModel& m, const int64_t* order, float* scr
     return hipGetLastError();
 }
 
-template <int TEAM, int VEC>
-static hipError_t sgd_par_tv(const DevModel& m, const StrataView& sv, int stratum, int blocks, bool qf, bool p2,
-                             bool ord, hipStream_t st) {
+template <int TEAM, int VEC, int PASS>
+static hipError_t sgd_par_tvp(const DevModel& m, const StrataView& sv, int stratum, int blocks, bool qf, bool p2,
+                              bool ord, hipStream_t st) {
     (void)hipGetLastError();
 #define TX_PAR(QF, P2, ORD) \
-    hipLaunchKernelGGL((k_sgd_par<TEAM, VEC, QF, P2, ORD>), dim3(blocks), dim3(256), 0, st, m, sv, stratum)
-    if (ord) {
-        if (qf) { if (p2) TX_PAR(true, true, true); else TX_PAR(true, false, true); }
-        else { if (p2) TX_PAR(false, true, true); else TX_PAR(false, false, true); }
+    hipLaunchKernelGGL((k_sgd_par<TEAM, VEC, QF, P2, ORD, PASS>), dim3(blocks), dim3(256), 0, st, m, sv, stratum)
+    if constexpr (PASS == 0) {
+        if (ord) {
+            if (qf) { if (p2) TX_PAR(true, true, true); else TX_PAR(true, false, true); }
+            else { if (p2) TX_PAR(false, true, true); else TX_PAR(false, false, true); }
+            return hipGetLastError();
+        }
     } else {
-        if (qf) { if (p2) TX_PAR(true, true, false); else TX_PAR(true, false, false); }
-        else { if (p2) TX_PAR(false, true, false); else TX_PAR(false, false, false); }
+        if (ord) return hipErrorInvalidValue;
     }
+    if (qf) { if (p2) TX_PAR(true, true, false); else TX_PAR(true, false, false); }
+    else { if (p2) TX_PAR(false, true, false); else TX_PAR(false, false, false); }
 #undef TX_PAR
     return hipGetLastError();
 }
 
+template <int TEAM, int VEC>
+static hipError_t sgd_par_tv(const DevModel& m, const StrataView& sv, int stratum, int blocks, bool qf, bool p2,
+                             bool ord, int pass, hipStream_t st) {
+    if (pass == 0) return sgd_par_tvp<TEAM, VEC, 0>(m, sv, stratum, blocks, qf, p2, ord, st);
+    if (pass == 1) return sgd_par_tvp<TEAM, VEC, 1>(m, sv, stratum, blocks, qf, p2, ord, st);
+    if (pass == 2) return sgd_par_tvp<TEAM, VEC, 2>(m, sv, stratum, blocks, qf, p2, ord, st);
+    return hipErrorInvalidValue;
+}
+
 hipError_t launch_sgd_par(const DevModel& m, const StrataView& sv, int stratum, int team, int vec, int blocks,
-                          bool qf, bool p2, bool ord, hipStream_t st) {
+                          bool qf, bool p2, bool ord, int pass, hipStream_t st) {
 #define X(T, V) \
-    if (team == T && vec == V) return sgd_par_tv<T, V>(m, sv, stratum, blocks, qf, p2, ord, st);
+    if (team == T && vec == V) return sgd_par_tv<T, V>(m, sv, stratum, blocks, qf, p2, ord, pass, st);
+    TX_SHAPES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_apply_nsvd(const DevModel& m, float* d_y, float* d_yb, const float* scale, hipStream_t st) {
+    const int64_t n = static_cast<int64_t>(m.I) * m.fs;
+    if (n <= 0) return hipSuccess;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(k_apply_nsvd, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, m, d_y, d_yb, scale);
+    return hipGetLastError();
+}
+
+hipError_t launch_item_pass(const DevModel& m, const ItemView& iv, int team, int vec, bool p2, int blocks,
+                            hipStream_t st) {
+    const size_t shm = sizeof(float) * static_cast<size_t>(m.B + m.F) * (256 / team);
+#define X(T, V)                                                                                          \
+    if (team == T && vec == V) {                                                                          \
+        (void)hipGetLastError();                                                                          \
+        if (p2) hipLaunchKernelGGL((k_item_pass<T, V, true>), dim3(blocks), dim3(256), shm, st, m, iv);   \
+        else hipLaunchKernelGGL((k_item_pass<T, V, false>), dim3(blocks), dim3(256), shm, st, m, iv);     \
+        return hipGetLastError();                                                                         \
+    }
     TX_SHAPES(X)
 #undef X
     return hipErrorInvalidValue;

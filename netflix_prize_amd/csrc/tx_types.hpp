@@ -78,6 +78,19 @@ struct StrataView {
     float* gacc_b;              // [U]      per-user sum_err across strata
     int32_t q_mode;             // item-row update: 0 device atomic, 1 XCD-local atomic, 2 plain
     int32_t y_mode;             // NSVD1-row update, same codes
+    float* d_y;                 // [I * fs] NSVD1 gradient accumulators of the user pass (tsvdx4p)
+    float* d_yb;                // [I]
+};
+
+// Item pass of the tsvdx4p schedule: ratings grouped by item.
+struct ItemView {
+    const uint32_t* q_items;   // items with ratings, most ratings first
+    uint32_t n_items;          // length of q_items
+    uint32_t* q_head;          // work counter (reset per pass)
+    const uint32_t* bi_start;  // [I+1] offsets into bi_rec
+    // per rating, training-file order within an item (ByItemIndex, src/tsvdx4p.rs:29-57):
+    //   x = user | bin << 20 | freq_bin << 28, y = SparseUD index, z = dev, w = target
+    const uint4* bi_rec;
 };
 
 }  // namespace tx

@@ -36,6 +36,7 @@ PARAM_ID = {n: k for k, n in enumerate(PARAMS)}
 
 SCHED_SERIAL = 0
 SCHED_STRATA = 1
+SCHED_TSVDX4P = 2  # Tsvdx4pModel epochs (src/tsvdx4p.rs:798-805)
 
 
 class TxError(RuntimeError):
@@ -110,6 +111,10 @@ class TxConfig:
     save_ifeat: bool = False
     low_memory: bool = True
     full_su: bool = True
+    # Tsvdx4pConfig extras (src/tsvdx4p.rs:175-186), used by SCHED_TSVDX4P
+    nsvd_norm_exp: float = 0.0
+    n_threads: int = 0
+    seq_epochs: tuple = (0, 0, 0, 0, 0)
 
     def replace(self, **kw) -> "TxConfig":
         return dataclasses.replace(self, **kw)
@@ -178,6 +183,11 @@ def config_struct(cfg: TxConfig) -> CConfig:
     c.has_ordinal = int(cfg.ordinal_head is not None)
     for n in _F32:
         setattr(c, n, float(getattr(cfg, n)))
+    c.nsvd_norm_exp = float(cfg.nsvd_norm_exp)
+    c.n_threads = int(cfg.n_threads)
+    se = list(cfg.seq_epochs or ()) + [0] * 5
+    for k in range(5):
+        c.seq_epochs[k] = int(se[k])
     if cfg.ordinal_head is not None:
         for k in range(4):
             c.th_init[k] = float(cfg.ordinal_head.th_init[k])
@@ -203,6 +213,7 @@ def lib() -> C.CDLL:
         L.tx_destroy.argtypes = [vp]
         L.tx_set_schedule.argtypes = [vp, vp]
         L.tx_fit_epoch.argtypes = [vp, i32]
+        L.tx_fit_users_epoch.argtypes = [vp, i32]
         L.tx_last_epoch_stats.argtypes = [vp, vp]
         L.tx_predict.argtypes = [vp, i64, vp, vp, vp, vp]
         L.tx_predict_device.argtypes = [vp, i64, vp, vp, vp, vp, vp]
@@ -213,7 +224,8 @@ def lib() -> C.CDLL:
         L.tx_set_param.argtypes = [vp, i32, vp, i64]
         L.tx_get_users.argtypes = [i64, i32, C.c_uint64, i64, vp]
         L.tx_rand_normal.argtypes = [C.c_uint64, C.c_float, i64, vp]
-        for name in ["tx_create", "tx_destroy", "tx_set_schedule", "tx_fit_epoch", "tx_last_epoch_stats",
+        for name in ["tx_create", "tx_destroy", "tx_set_schedule", "tx_fit_epoch", "tx_fit_users_epoch",
+                     "tx_last_epoch_stats",
                      "tx_predict", "tx_predict_device", "tx_predict_subscores", "tx_calc_rmse", "tx_param_len",
                      "tx_get_param", "tx_set_param", "tx_get_users", "tx_rand_normal"]:
             getattr(L, name).restype = C.c_int
@@ -392,6 +404,12 @@ class TxModel:
         """Regressor::fit_epoch; the training sets were fixed at creation, so
         tr / pr are accepted for interface parity only."""
         _check(lib().tx_fit_epoch(self._h, int(epoch)), "tx_fit_epoch")
+
+    def fit_users_epoch(self, epoch: int = 1) -> None:
+        """One user-only epoch with item tables and y frozen (the user pass of
+        Tsvdx4pModel, src/tsvdx4p.rs:390-639, without its NSVD1 accumulators):
+        folds users into fixed item tables."""
+        _check(lib().tx_fit_users_epoch(self._h, int(epoch)), "tx_fit_users_epoch")
 
     def last_epoch_stats(self) -> dict:
         s = CEpochStats()

@@ -114,6 +114,7 @@ def lib():
         L.or_fit_epoch.argtypes = [vp, vp, vp, i64]
         L.or_fit_epoch_order.argtypes = [vp, vp, vp, i64, vp]
         L.or4p_fit_epoch.argtypes = [vp, vp, vp, i64, C.c_int]
+        L.or_fit_users_epoch.argtypes = [vp, vp, vp, i64]
         L.or_rebuild_ycache.argtypes = [vp, vp, vp]
         L.or_predict.argtypes = [vp, i64, vp, vp, vp, vp]
         L.or_predict_subscores.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp]
@@ -207,6 +208,11 @@ class OracleModel:
         else:
             u = np.ascontiguousarray(users, dtype=np.int64)
             lib().or_fit_epoch_order(self._m, self.tr.ref, self.pr.ref, int(epoch), u.ctypes.data)
+
+    def fit_users_epoch(self, epoch: int):
+        """The tsvdx4p user pass without its NSVD1 accumulators, then the ycache
+        rebuild (users folded into fixed item tables)."""
+        lib().or_fit_users_epoch(self._m, self.tr.ref, self.pr.ref, int(epoch))
 
     def fit_epoch_p(self, epoch: int, workers: int = 1):
         """Tsvdx4pModel::fit_epoch with `workers` OS threads."""

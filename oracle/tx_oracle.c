@@ -1313,6 +1313,27 @@ OR_API void or4p_fit_epoch(or_model* m, const or_dataset* tr, const or_dataset* 
     free(ij);
 }
 
+/* The user pass alone (fit_epoch_user, src/tsvdx4p.rs:390-639) with its NSVD1
+ * accumulators discarded, then rebuild_ycache: users folded into fixed item
+ * tables and y.  Users in get_users order, one chunk. */
+OR_API void or_fit_users_epoch(or_model* m, const or_dataset* tr, const or_dataset* pr, int64_t epoch) {
+    const or_config c = m->cfg;
+    const int f = m->f, I = m->n_items;
+    int64_t* users = malloc(sizeof(int64_t) * (size_t)(m->n_users ? m->n_users : 1));
+    get_users(m->n_users, c.shuffle_users, c.seed, epoch, users);
+    p_job job;
+    memset(&job, 0, sizeof(job));
+    job.m = m; job.tr = tr; job.users = users; job.lo = 0; job.hi = m->n_users;
+    job.d.d_yfeat = zalloc((int64_t)I * f);
+    job.d.d_ybias = zalloc(I);
+    job.d.d_yfeat_day = c.low_memory ? NULL : zalloc((int64_t)I * f);
+    job.d.n_contrib = calloc((size_t)(I ? I : 1), sizeof(uint32_t));
+    p_user_chunk(&job);
+    free(job.d.d_yfeat); free(job.d.d_ybias); free(job.d.d_yfeat_day); free(job.d.n_contrib); free(users);
+    rebuild_ycache(m, tr, pr);
+    if (!c.low_memory) rebuild_ycache_day(m, tr, pr);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Parameter access and RNG hooks for the checks                            */
 /* ------------------------------------------------------------------------ */
