@@ -177,6 +177,11 @@ int tx_fit_epoch(tx_ctx* ctx, int32_t epoch);
  * accumulators: folding users into fixed item tables. */
 int tx_fit_users_epoch(tx_ctx* ctx, int32_t epoch);
 
+/* rebuild_ycache (and rebuild_ycache_day when not low_memory) from the
+ * current y tables (src/tx.rs:391-454): after tx_set_param of yfeat / ybias,
+ * predictions use the new NSVD1 sums only once this has run. */
+int tx_rebuild_ycache(tx_ctx* ctx);
+
 /* Timing of the last tx_fit_epoch / tx_fit_users_epoch. */
 int tx_last_epoch_stats(tx_ctx* ctx, tx_epoch_stats* out);
 

@@ -1,4 +1,5 @@
-"""Build libtsvd_tx.so (HIP kernels + C ABI) in-tree with hipcc for gfx950.
+"""Build libtsvd_tx.so and libtsvd_enc.so (HIP kernels + C ABIs) in-tree with
+hipcc for gfx950.
 
 Run as ``python -m netflix_prize_amd.build``; ``__graft_entry__.build()``
 calls :func:`build`.
@@ -16,6 +17,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(PKG, "libtsvd_tx.so")
+ENC_LIB = os.path.join(PKG, "libtsvd_enc.so")
 DRIVER = os.path.join(PKG, "tsvdx4-new")
 OBJ = os.path.join(PKG, "_obj")
 
@@ -74,6 +76,12 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread"])
     os.replace(tmp, LIB)
+    # fold-in encoder kernels: their own library, C ABI in include/tsvd_enc.h
+    tmp = ENC_LIB + ".tmp"
+    dt = _run([hipcc, *HIP_FLAGS, *inc, "-shared", os.path.join(CSRC, "enc_kernels.hip"), "-o", tmp])
+    os.replace(tmp, ENC_LIB)
+    if verbose:
+        print(f"enc_kernels.hip {dt:.1f}s")
     # job-level driver (plain C++ over the C ABI), rpath to the library
     tmp = DRIVER + ".tmp"
     dt = _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{INCLUDE}", f"-I{CSRC}", os.path.join(CSRC, "tsvdx4_new.cpp"),

@@ -1102,6 +1102,14 @@ int tx_fit_users_epoch(tx_ctx* c, int32_t epoch) {
     return 0;
 }
 
+int tx_rebuild_ycache(tx_ctx* c) {
+    g_err.clear();
+    if (!c) return fail(TX_EINVAL, "tx_rebuild_ycache: NULL ctx");
+    HIPC(tx::launch_rebuild(c->dm, c->n_ud, c->shape.team, c->shape.vec, c->st));
+    HIPC(hipStreamSynchronize(c->st));
+    return 0;
+}
+
 int tx_last_epoch_stats(tx_ctx* c, tx_epoch_stats* out) {
     g_err.clear();
     if (!c || !out) return fail(TX_EINVAL, "tx_last_epoch_stats: NULL argument");

@@ -27,6 +27,16 @@ def test_library_exports_every_declared_entry_point():
         assert hasattr(L, n), n
 
 
+def test_encoder_library_exports_every_declared_entry_point():
+    text = open(os.path.join(ROOT, "include", "tsvd_enc.h")).read()
+    names = sorted(set(re.findall(r"^(?:int|const char\*)\s+(enc_\w+)\(", text, re.M)))
+    assert len(names) >= 7
+    from netflix_prize_amd import encoder
+    L = C.CDLL(encoder.ENC_LIB_PATH)
+    for n in names:
+        assert hasattr(L, n), n
+
+
 def test_struct_layouts_match_header(tmp_path):
     """ctypes mirrors have the sizes and field offsets the C header gives."""
     src = tmp_path / "probe.c"
