@@ -126,6 +126,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--ttr-epochs", type=int, default=20, help="epochs of the time-to-RMSE run (0 = skip)")
+    ap.add_argument("--fold-in-users", type=int, default=80_189,
+                    help="users held out of training and folded in afterwards (0 = skip)")
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -133,6 +135,7 @@ def main(argv=None) -> int:
               "(see the sharding design notes)", file=sys.stderr)
         return 2
 
+    import torch  # first: the libraries then share torch's HIP runtime
     from netflix_prize_amd import synth, tx
 
     t0 = time.time()
@@ -147,8 +150,7 @@ def main(argv=None) -> int:
     model.set_schedule(mode=tx.SCHED_STRATA, n_strata=args.strata, use_graph=bool(args.graph), blocks=args.blocks)
     log(f"bench: model created ({time.time() - t0:.1f}s)")
 
-    import torch  # synchronisation of the whole device around the timed region
-    have_torch_gpu = torch.cuda.is_available()
+    have_torch_gpu = torch.cuda.is_available()  # synchronisation of the device around the timed region
 
     epoch = 0
     for _ in range(args.warmup):
@@ -202,6 +204,15 @@ def main(argv=None) -> int:
                "curve": [round(r, 5) for r in curve]}
         log(f"bench: time-to-RMSE {ttr}")
 
+    # new users folded into a model trained on the other users: the encoder's
+    # warm start against the reference's random start
+    fold = None
+    if args.fold_in_users > 0:
+        from netflix_prize_amd import encoder as E
+        n_new = args.fold_in_users if args.fold_in_users < spec.n_users else spec.n_users // 6
+        log(f"bench: fold-in of {n_new} new users (encoder vs random start)")
+        fold = E.evaluate_new_users(tr, pr, cfg, spec.n_users - n_new, args.ttr_epochs or 20, 5, 20, log=log)
+
     # roofline of the dominant kernel (SGD), from the library's events on its stream
     qf = cfg.lr_iqf != 0.0 or cfg.sigma_iqf != 0.0
     p2 = cfg.lr_u2 != 0.0
@@ -252,6 +263,7 @@ def main(argv=None) -> int:
                    "epochs_trained": epoch},
         "roofline": roofline,
         "time_to_rmse": ttr,
+        "fold_in": fold,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -25,6 +25,9 @@ import time
 from dataclasses import dataclass
 
 import numpy as np
+# torch first: its HIP runtime must be the one this process loads, before
+# libtsvd_tx.so / libtsvd_enc.so bind theirs (same soname, so they then share it)
+import torch  # noqa: F401
 
 from . import tx
 
@@ -388,3 +391,57 @@ def warm_start(m: "tx.TxModel", enc: FoldInEncoder, X: np.ndarray) -> float:
     m.set("ubias", Y[:, f])
     m.rebuild_ycache()
     return time.perf_counter() - t0
+
+
+def evaluate_new_users(tr, pr, cfg, n_train: int, epochs: int, fold_epochs: int = 5, random_epochs: int = 20,
+                       enc_cfg: EncoderConfig | None = None, log=None) -> dict:
+    """Users [0, n_train) train the model (default parallel schedule, `epochs`
+    epochs) and the encoder (on that model's p_u, b_u); users [n_train, U) are
+    new: folded in with user-only epochs from the reference's random start
+    (0..random_epochs epochs) and from the encoder's warm start
+    (0..fold_epochs).  Holdout RMSE of the new users along both, and the time
+    each start takes."""
+    log = log or (lambda s: None)
+    (tr_a, pr_a), (tr_b, pr_b) = split_users(tr, pr, n_train)
+    t0 = time.perf_counter()
+    base = tx.TxModel(tr_a, pr_a, cfg)
+    gpu_s = 0.0
+    for e in range(1, epochs + 1):
+        base.fit_epoch(epoch=e)
+        gpu_s += base.last_epoch_stats()["ms_total"] / 1000.0
+    log(f"base model: {epochs} epochs on {n_train} users, {gpu_s:.2f} s GPU, holdout {base.calc_rmse(pr_a):.5f}")
+    Xa = user_features(base, tr_a)
+    Ta = sgd_targets(base)
+    enc = FoldInEncoder(Xa.shape[1], Ta.shape[1], enc_cfg)
+    t1 = time.perf_counter()
+    hist = enc.fit(Xa, Ta, log=log)
+    _torch().cuda.synchronize()
+    fit_s = time.perf_counter() - t1
+    rand = fold_in_model(base, tr_b, pr_b, cfg)
+    r_rand, s_rand = fold_in_curve(rand, pr_b, max(random_epochs, fold_epochs))
+    rand.close()
+    warm = fold_in_model(base, tr_b, pr_b, cfg)
+    t2 = time.perf_counter()
+    Xb = user_features(base, tr_b)
+    feat_s = time.perf_counter() - t2
+    ws_s = warm_start(warm, enc, Xb)
+    r_warm, s_warm = fold_in_curve(warm, pr_b, fold_epochs)
+    warm.close()
+    base.close()
+    # user-only epochs the random start needs to reach the warm start's RMSE
+    # before any epoch (within 1e-3), and the GPU seconds those epochs take
+    target = r_warm[0] + 1e-3
+    k_eq = next((k for k, r in enumerate(r_rand) if r <= target), None)
+    return {
+        "train_users": n_train, "new_users": tr_b.n_users, "base_epochs": epochs,
+        "encoder": {"dtype": enc.cfg.dtype, "layers": [enc.d_in, enc.hid, enc.d_out], "outputs": enc.n_out,
+                    "epochs": enc.cfg.epochs, "batch": enc.cfg.batch, "loss_first_last": [round(hist[0], 4), round(hist[-1], 4)],
+                    "train_s": round(fit_s, 2)},
+        "holdout_rmse_random_start": [round(x, 5) for x in r_rand],
+        "holdout_rmse_encoder_start": [round(x, 5) for x in r_warm],
+        "fold_epoch_s": [round(x, 4) for x in s_rand[1:]],
+        "encoder_start_s": round(feat_s + ws_s, 3),
+        "random_start_epochs_to_encoder_rmse": k_eq,
+        "random_start_s_to_encoder_rmse": None if k_eq is None else round(sum(s_rand[1:k_eq + 1]), 4),
+        "wall_s": round(time.perf_counter() - t0, 1),
+    }
