@@ -177,6 +177,14 @@ int tx_fit_epoch(tx_ctx* ctx, int32_t epoch);
  * accumulators: folding users into fixed item tables. */
 int tx_fit_users_epoch(tx_ctx* ctx, int32_t epoch);
 
+/* Precision of the factor tables that predictions read (tx_predict*,
+ * tx_calc_rmse): TX_PREC_F32 (default) or TX_PREC_BF16, BF16 copies of q,
+ * q_f, p, p2 and ycache (round to nearest even) refreshed from the FP32
+ * master tables after any change; training always updates the FP32 masters.
+ * No reference counterpart (the reference computes in FP32 throughout). */
+enum tx_precision { TX_PREC_F32 = 0, TX_PREC_BF16 = 1 };
+int tx_set_table_precision(tx_ctx* ctx, int32_t precision);
+
 /* rebuild_ycache (and rebuild_ycache_day when not low_memory) from the
  * current y tables (src/tx.rs:391-454): after tx_set_param of yfeat / ybias,
  * predictions use the new NSVD1 sums only once this has run. */

@@ -121,6 +121,10 @@ struct tx_ctx {
     std::vector<uint8_t> sud_fbin;
     uint4* d_rec = nullptr;  // packed records, reference order
 
+    // BF16 read copies of the factor tables for predictions (FP32 masters)
+    bool bf16 = false, bf16_stale = true;
+    uint16_t *b_ifeat = nullptr, *b_ifeat_freq = nullptr, *b_ufeat = nullptr, *b_ufeat2 = nullptr, *b_ycache = nullptr;
+
     // tsvdx4p schedule: ratings by item, NSVD1 accumulators, per-item scales
     bool iv_built = false;
     tx::ItemView iv{};
@@ -424,6 +428,19 @@ int default_blocks(tx_ctx* c) {
     int blocks = (want + teams_per_block - 1) / teams_per_block;
     // a multiple of 8 keeps the cell -> XCD mapping of the round-robin dispatch
     return std::max(8, (blocks + 7) / 8 * 8);
+}
+
+// Refresh the BF16 copies when predictions read them and a parameter changed.
+int ensure_bf16(tx_ctx* c, hipStream_t st) {
+    if (!c->bf16 || !c->bf16_stale) return 0;
+    const int64_t U = c->U, I = c->I, fs = c->fs, F = c->F;
+    HIPC(tx::launch_to_bf16(c->dm.ifeat, c->b_ifeat, I * fs, st));
+    HIPC(tx::launch_to_bf16(c->dm.ifeat_freq, c->b_ifeat_freq, I * F * fs, st));
+    HIPC(tx::launch_to_bf16(c->dm.ufeat, c->b_ufeat, U * fs, st));
+    HIPC(tx::launch_to_bf16(c->dm.ufeat2, c->b_ufeat2, U * fs, st));
+    HIPC(tx::launch_to_bf16(c->dm.ycache, c->b_ycache, U * fs, st));
+    c->bf16_stale = false;
+    return 0;
 }
 
 // Users of the epoch in the reference's order (get_users, src/lib.rs:486-494)
@@ -902,7 +919,6 @@ int tx_create(const tx_config* cfg, const tx_ratings* tr, const tx_ratings* pr, 
     RC(c->alloc(&c->d_order, static_cast<size_t>(U)));
     RC(c->alloc(&c->d_scratch, static_cast<size_t>(4) * c->fs + 4));
     RC(c->alloc(&c->d_acc, 2));
-#undef RC
     HIPC(hipDeviceSynchronize());
     *out = ctx.release();
     return 0;
@@ -958,6 +974,7 @@ int tx_set_schedule(tx_ctx* c, const tx_schedule* s) {
 int tx_fit_epoch(tx_ctx* c, int32_t epoch) {
     g_err.clear();
     if (!c) return fail(TX_EINVAL, "tx_fit_epoch: NULL ctx");
+    c->bf16_stale = true;
     if (epoch < 1) return fail(TX_EINVAL, "tx_fit_epoch: epochs are 1-based");
     hipStream_t st = c->st;
     tx_epoch_stats stats{};
@@ -1068,6 +1085,7 @@ int tx_fit_epoch(tx_ctx* c, int32_t epoch) {
 int tx_fit_users_epoch(tx_ctx* c, int32_t epoch) {
     g_err.clear();
     if (!c) return fail(TX_EINVAL, "tx_fit_users_epoch: NULL ctx");
+    c->bf16_stale = true;
     if (epoch < 1) return fail(TX_EINVAL, "tx_fit_users_epoch: epochs are 1-based");
     if (!c->cfg.low_memory || c->cfg.has_ordinal)
         return fail(TX_EUNSUPPORTED, "tx_fit_users_epoch: low-memory models without ordinal head only");
@@ -1102,9 +1120,34 @@ int tx_fit_users_epoch(tx_ctx* c, int32_t epoch) {
     return 0;
 }
 
+int tx_set_table_precision(tx_ctx* c, int32_t precision) {
+    g_err.clear();
+    if (!c) return fail(TX_EINVAL, "tx_set_table_precision: NULL ctx");
+    if (precision != TX_PREC_F32 && precision != TX_PREC_BF16)
+        return fail(TX_EINVAL, "tx_set_table_precision: unknown precision %d", precision);
+    if (precision == TX_PREC_BF16 && !c->b_ifeat) {
+        const size_t U = std::max<int64_t>(c->U, 1), I = std::max<int64_t>(c->I, 1), fs = c->fs, F = c->F;
+        RC(c->alloc(&c->b_ifeat, I * fs));
+        RC(c->alloc(&c->b_ifeat_freq, I * F * fs));
+        RC(c->alloc(&c->b_ufeat, U * fs));
+        RC(c->alloc(&c->b_ufeat2, U * fs));
+        RC(c->alloc(&c->b_ycache, U * fs));
+        c->dm.b_ifeat = c->b_ifeat;
+        c->dm.b_ifeat_freq = c->b_ifeat_freq;
+        c->dm.b_ufeat = c->b_ufeat;
+        c->dm.b_ufeat2 = c->b_ufeat2;
+        c->dm.b_ycache = c->b_ycache;
+    }
+    c->bf16 = precision == TX_PREC_BF16;
+    c->dm.bf16_pred = c->bf16 ? 1 : 0;
+    c->bf16_stale = true;
+    return 0;
+}
+
 int tx_rebuild_ycache(tx_ctx* c) {
     g_err.clear();
     if (!c) return fail(TX_EINVAL, "tx_rebuild_ycache: NULL ctx");
+    c->bf16_stale = true;
     HIPC(tx::launch_rebuild(c->dm, c->n_ud, c->shape.team, c->shape.vec, c->st));
     HIPC(hipStreamSynchronize(c->st));
     return 0;
@@ -1122,6 +1165,7 @@ int tx_predict_device(tx_ctx* c, int64_t n, const int32_t* d_u, const int32_t* d
     g_err.clear();
     if (!c || n < 0 || (n > 0 && (!d_u || !d_i || !d_d || !d_out))) return fail(TX_EINVAL, "tx_predict_device: bad arguments");
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->st;
+    if (int rc = ensure_bf16(c, st)) return rc;
     HIPC(tx::launch_predict(c->dm, 0, n, d_u, d_i, d_d, d_out, nullptr, nullptr, nullptr, nullptr, nullptr,
                             c->shape.team, c->shape.vec, st));
     return 0;
@@ -1141,6 +1185,7 @@ int check_queries(tx_ctx* c, int64_t n, const int32_t* u, const int32_t* i) {
 int run_host(tx_ctx* c, int mode, int64_t n, const int32_t* u, const int32_t* i, const int16_t* d, float* o0, float* o1,
              float* o2, const float* target, const int8_t* is_test, double* acc) {
     if (int rc = check_queries(c, n, u, i)) return rc;
+    if (int rc = ensure_bf16(c, c->st)) return rc;
     const int64_t CH = int64_t{1} << 22;
     const int64_t chunk = std::min<int64_t>(std::max<int64_t>(n, 1), CH);
     auto rounded = [](size_t bytes) { return (bytes + 255) / 256 * 256; };
@@ -1249,6 +1294,7 @@ int tx_get_param(tx_ctx* c, int32_t id, float* out, int64_t n) {
 int tx_set_param(tx_ctx* c, int32_t id, const float* in, int64_t n) {
     g_err.clear();
     if (!c || (n > 0 && !in)) return fail(TX_EINVAL, "tx_set_param: NULL argument");
+    c->bf16_stale = true;
     const ParamDesc p = param_desc(c, id);
     if (p.rows < 0) return fail(TX_EINVAL, "unknown parameter id %d", id);
     if (n != p.rows * p.cols) return fail(TX_EINVAL, "parameter %d holds %lld elements, caller passed %lld", id,

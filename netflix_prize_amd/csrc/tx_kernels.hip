@@ -71,6 +71,27 @@ __device__ __forceinline__ void store_vec(float* __restrict__ p, const float (&i
 
 // Strided element mapping: lane l of a team holds elements l, l + TEAM, ...,
 // so each per-element instruction of a team touches TEAM consecutive floats.
+// VEC consecutive BF16 values widened to FP32 (exact).
+template <int VEC>
+__device__ __forceinline__ void load_bf16(const uint16_t* __restrict__ p, float (&out)[VEC]) {
+#pragma unroll
+    for (int c = 0; c < VEC; ++c) out[c] = __uint_as_float(static_cast<uint32_t>(p[c]) << 16);
+}
+
+// FP32 -> BF16, round to nearest even (NaN stays NaN).
+__device__ __forceinline__ uint16_t to_bf16(float x) {
+    uint32_t u = __float_as_uint(x);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return static_cast<uint16_t>((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return static_cast<uint16_t>(u >> 16);
+}
+
+__global__ __launch_bounds__(256) void k_to_bf16(const float* __restrict__ x, uint16_t* __restrict__ y, int64_t n) {
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < n;
+         e += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        y[e] = to_bf16(x[e]);
+}
+
 template <int VEC, int TEAM>
 __device__ __forceinline__ void load_str(const float* __restrict__ p, float (&out)[VEC]) {
 #pragma unroll
@@ -1144,11 +1165,19 @@ __global__ __launch_bounds__(256) void k_predict(DevModel m, int64_t n, const in
         const float bi_t = m.ibias[i] + m.bit_bin[static_cast<size_t>(i) * B + b] + m.ibias_freq[static_cast<size_t>(i) * F + fb];
         const float cu_t = m.cu[u] + (ud >= 0 ? m.cut[ud] : 0.0f);
         float q[VEC], qf[VEC], p[VEC], p2[VEC], s[VEC];
-        load_vec<VEC>(m.ifeat + static_cast<size_t>(i) * fs + off, q);
-        load_vec<VEC>(m.ifeat_freq + (static_cast<size_t>(i) * F + fb) * fs + off, qf);
-        load_vec<VEC>(m.ufeat + static_cast<size_t>(u) * fs + off, p);
-        load_vec<VEC>(m.ufeat2 + static_cast<size_t>(u) * fs + off, p2);
-        load_vec<VEC>(m.ycache + static_cast<size_t>(u) * fs + off, s);
+        if (m.bf16_pred) {
+            load_bf16<VEC>(m.b_ifeat + static_cast<size_t>(i) * fs + off, q);
+            load_bf16<VEC>(m.b_ifeat_freq + (static_cast<size_t>(i) * F + fb) * fs + off, qf);
+            load_bf16<VEC>(m.b_ufeat + static_cast<size_t>(u) * fs + off, p);
+            load_bf16<VEC>(m.b_ufeat2 + static_cast<size_t>(u) * fs + off, p2);
+            load_bf16<VEC>(m.b_ycache + static_cast<size_t>(u) * fs + off, s);
+        } else {
+            load_vec<VEC>(m.ifeat + static_cast<size_t>(i) * fs + off, q);
+            load_vec<VEC>(m.ifeat_freq + (static_cast<size_t>(i) * F + fb) * fs + off, qf);
+            load_vec<VEC>(m.ufeat + static_cast<size_t>(u) * fs + off, p);
+            load_vec<VEC>(m.ufeat2 + static_cast<size_t>(u) * fs + off, p2);
+            load_vec<VEC>(m.ycache + static_cast<size_t>(u) * fs + off, s);
+        }
         float pd[VEC], sd[VEC];
         const bool day_terms = !m.low_mem && ud >= 0;
         if (day_terms) {
@@ -1372,6 +1401,15 @@ hipError_t launch_predict(const DevModel& m, int mode, int64_t n, const int32_t*
     TX_SHAPES(X)
 #undef X
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_to_bf16(const float* x, uint16_t* y, int64_t n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(k_to_bf16, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, x, y, n);
+    return hipGetLastError();
 }
 
 hipError_t launch_fill_f32(float* p, float v, int64_t n, hipStream_t st) {

@@ -13,6 +13,7 @@ hipError_t launch_sgd_serial(const DevModel& m, const int64_t* order, float* scr
 // pass: 0 SGD epoch, 1 tsvdx4p user pass (NSVD1 accumulators), 2 user-only fold-in pass
 hipError_t launch_sgd_par(const DevModel& m, const StrataView& sv, int stratum, int team, int vec, int blocks,
                           bool qf, bool p2, bool ord, int pass, hipStream_t st);
+hipError_t launch_to_bf16(const float* x, uint16_t* y, int64_t n, hipStream_t st);
 hipError_t launch_apply_nsvd(const DevModel& m, float* d_y, float* d_yb, const float* scale, hipStream_t st);
 hipError_t launch_item_pass(const DevModel& m, const ItemView& iv, int team, int vec, bool p2, int blocks,
                             hipStream_t st);

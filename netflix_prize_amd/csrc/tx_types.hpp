@@ -61,6 +61,11 @@ struct DevModel {
 
     const float* norm_train;    // [U] sqrt(|R(u)| + |P(u)| if full_su)
     const float* norm_rebuild;  // [U] sqrt(train_cnt + heldout_cnt)
+
+    // BF16 read copies of the factor tables (row stride fs; FP32 tables stay
+    // the masters); predictions read them when bf16_pred is set
+    int32_t bf16_pred;
+    const uint16_t *b_ifeat, *b_ifeat_freq, *b_ufeat, *b_ufeat2, *b_ycache;
 };
 
 // Parallel-schedule state (stratified user x item-block rotation).

@@ -215,6 +215,7 @@ def lib() -> C.CDLL:
         L.tx_fit_epoch.argtypes = [vp, i32]
         L.tx_fit_users_epoch.argtypes = [vp, i32]
         L.tx_rebuild_ycache.argtypes = [vp]
+        L.tx_set_table_precision.argtypes = [vp, i32]
         L.tx_last_epoch_stats.argtypes = [vp, vp]
         L.tx_predict.argtypes = [vp, i64, vp, vp, vp, vp]
         L.tx_predict_device.argtypes = [vp, i64, vp, vp, vp, vp, vp]
@@ -226,7 +227,7 @@ def lib() -> C.CDLL:
         L.tx_get_users.argtypes = [i64, i32, C.c_uint64, i64, vp]
         L.tx_rand_normal.argtypes = [C.c_uint64, C.c_float, i64, vp]
         for name in ["tx_create", "tx_destroy", "tx_set_schedule", "tx_fit_epoch", "tx_fit_users_epoch",
-                     "tx_rebuild_ycache",
+                     "tx_rebuild_ycache", "tx_set_table_precision",
                      "tx_last_epoch_stats",
                      "tx_predict", "tx_predict_device", "tx_predict_subscores", "tx_calc_rmse", "tx_param_len",
                      "tx_get_param", "tx_set_param", "tx_get_users", "tx_rand_normal"]:
@@ -412,6 +413,12 @@ class TxModel:
         Tsvdx4pModel, src/tsvdx4p.rs:390-639, without its NSVD1 accumulators):
         folds users into fixed item tables."""
         _check(lib().tx_fit_users_epoch(self._h, int(epoch)), "tx_fit_users_epoch")
+
+    def set_table_precision(self, precision: str) -> None:
+        """"f32" (default) or "bf16": the factor tables predictions read
+        (BF16 copies of the FP32 masters, refreshed after every change)."""
+        code = {"f32": 0, "bf16": 1}[precision]
+        _check(lib().tx_set_table_precision(self._h, code), "tx_set_table_precision")
 
     def rebuild_ycache(self) -> None:
         """rebuild_ycache (src/tx.rs:391-418) from the current y tables."""
