@@ -98,8 +98,11 @@ typedef struct tx_schedule {
     int32_t use_graph;       /* capture each epoch's launches in a hipGraph */
     int32_t update_flags;    /* STRATA only: how shared rows are updated, see below */
     int32_t user_order;      /* STRATA only: 0 = the reference's per-epoch shuffled user
-                                order (get_users), 1 = longest user first */
-    int32_t reserved[1];
+                                order (get_users), 1 = longest user first, 2 = length
+                                buckets: the reference's order with the users of at
+                                least heavy_min ratings spread evenly over the first
+                                half of the epoch (so no long user starts near its end) */
+    int32_t heavy_min;       /* user_order 2: bound of the long-user bucket, 0 = 1024 */
 } tx_schedule;
 
 /* update_flags: bits 0-1 item rows/biases (q, qf, b_i, b_i,bin, b_i,f),
@@ -163,8 +166,19 @@ int tx_create(const tx_config* cfg, const tx_ratings* train, const tx_ratings* h
 int tx_destroy(tx_ctx* ctx);
 
 /* Choose the training schedule (default: TX_SCHED_STRATA, n_strata = 1,
- * update_flags = TX_UPD_ATOMIC | TX_UPD_PLAIN << 2). */
+ * blocks = 0, update_flags = TX_UPD_ATOMIC | TX_UPD_PLAIN << 2, user_order
+ * = 2, heavy_min = 1024). */
 int tx_set_schedule(tx_ctx* ctx, const tx_schedule* sched);
+
+/* Autotuner of the one-stratum parallel schedule (no reference counterpart):
+ * times trial_epochs epochs for each candidate of teams in flight (1/2, 3/4
+ * and all of the measured fidelity bound for this f) x user order (the
+ * reference's, or length buckets with heavy_min 256 / 1024 / 4096), each
+ * from the same snapshot of every parameter table, then restores the tables
+ * and applies the fastest candidate (use_graph kept).  The model's parameters
+ * are unchanged; *best (may be NULL) receives the chosen schedule.  Needs
+ * device memory for one copy of the parameter tables. */
+int tx_autotune(tx_ctx* ctx, int32_t trial_epochs, tx_schedule* best);
 
 /* Regressor::fit_epoch (src/lib.rs:630, src/tx.rs:734-1115); epoch is
  * 1-based.  Includes the epoch-end ycache rebuild (src/tx.rs:1109-1110). */

@@ -130,8 +130,9 @@ struct tx_ctx {
     tx::ItemView iv{};
     float* d_scale = nullptr;
 
-    // default: one stratum, atomic item updates, lock-free NSVD1 row updates
-    tx_schedule sched{TX_SCHED_STRATA, 1, 0, 0, 0, TX_UPD_ATOMIC | (TX_UPD_PLAIN << 2), 0, {0}};
+    // default: one stratum, atomic item updates, lock-free NSVD1 row updates,
+    // the reference's user order in length buckets
+    tx_schedule sched{TX_SCHED_STRATA, 1, 0, 0, 0, TX_UPD_ATOMIC | (TX_UPD_PLAIN << 2), 2, 1024};
     int built_S = 0;
     tx::StrataView sv{};
     std::vector<void*> strata_allocs;
@@ -417,10 +418,11 @@ int build_strata(tx_ctx* c, int S) {
 // seeing each other's steps; on popular items that delayed feedback grows
 // with the number of teams and, past a point, makes the y step overshoot
 // (item and user factors tolerate far more concurrency).  The tolerable
-// count falls with f: at the full Netflix shape, 4096 teams at f=64 and 1024
-// at f=256 keep the holdout RMSE of 20 epochs within 1e-3 of the serial
-// trainer, 4096 at f=256 does not.
-int target_teams(int f) { return std::min(2048, std::max(512, 262144 / std::max(f, 1))); }
+// count falls with f.  At the full Netflix shape with the length-bucket order
+// (long users started in the first half of the epoch), 3072 teams at f=64 and
+// 1536 at f=256 keep the holdout RMSE of 20 epochs within 1e-3 of the serial
+// trainer (within 1e-4 measured); 4096 at f=64 and 3072 at f=256 do not.
+int target_teams(int f) { return std::min(3072, std::max(768, 393216 / std::max(f, 1))); }
 
 int default_blocks(tx_ctx* c) {
     const int teams_per_block = 256 / c->shape.team;
@@ -444,13 +446,30 @@ int ensure_bf16(tx_ctx* c, hipStream_t st) {
 }
 
 // Users of the epoch in the reference's order (get_users, src/lib.rs:486-494)
-// into the one-stratum queue.
+// into the one-stratum queue; with length buckets (user_order 2) the users of
+// at least heavy_min ratings keep their relative order but are spread evenly
+// over the first half of the queue.
 int prepare_user_queue(tx_ctx* c, int32_t epoch, hipStream_t st) {
     std::vector<int64_t> order(c->U);
     tx::user_order(c->U, c->cfg.shuffle_users != 0, c->cfg.seed, epoch, order.data());
     c->h_queue.clear();
-    for (int64_t u : order)
-        if (c->t_off[u + 1] > c->t_off[u]) c->h_queue.push_back(static_cast<uint32_t>(u));
+    if (c->sched.user_order == 2) {
+        const uint32_t hmin = c->sched.heavy_min > 0 ? static_cast<uint32_t>(c->sched.heavy_min) : 1024u;
+        std::vector<uint32_t> heavy, light;
+        for (int64_t u : order) {
+            const uint32_t cnt = c->t_off[u + 1] - c->t_off[u];
+            if (cnt) (cnt >= hmin ? heavy : light).push_back(static_cast<uint32_t>(u));
+        }
+        const size_t n = heavy.size() + light.size(), half = n / 2;
+        size_t k = 0, l = 0;
+        for (size_t pos = 0; pos < n; ++pos) {
+            const bool take_heavy = k < heavy.size() && (l >= light.size() || pos * heavy.size() >= k * half);
+            c->h_queue.push_back(take_heavy ? heavy[k++] : light[l++]);
+        }
+    } else {
+        for (int64_t u : order)
+            if (c->t_off[u + 1] > c->t_off[u]) c->h_queue.push_back(static_cast<uint32_t>(u));
+    }
     HIPC(hipMemcpyAsync(const_cast<uint32_t*>(c->sv.q_users), c->h_queue.data(), c->h_queue.size() * sizeof(uint32_t),
                         hipMemcpyHostToDevice, st));
     return 0;
@@ -951,7 +970,8 @@ int tx_set_schedule(tx_ctx* c, const tx_schedule* s) {
         if (s->blocks < 0 || (s->blocks > 0 && s->blocks < s->n_strata))
             return fail(TX_EINVAL, "blocks must be 0 (auto) or at least n_strata");
     }
-    if (s->user_order != 0 && s->user_order != 1) return fail(TX_EINVAL, "user_order must be 0 or 1");
+    if (s->user_order < 0 || s->user_order > 2) return fail(TX_EINVAL, "user_order must be 0, 1 or 2");
+    if (s->heavy_min < 0) return fail(TX_EINVAL, "heavy_min must be >= 0");
     const int qm = s->update_flags & 3, ym = (s->update_flags >> 2) & 3;
     if (qm > 2 || ym > 2 || (s->update_flags >> 4) != 0) return fail(TX_EINVAL, "bad update_flags %d", s->update_flags);
     const int want_S = s->mode == TX_SCHED_TSVDX4P ? 1 : s->n_strata;
@@ -1017,16 +1037,8 @@ int tx_fit_epoch(tx_ctx* c, int32_t epoch) {
     } else {
         if (c->built_S == 0)
             if (int rc = build_strata(c, c->sched.n_strata > 0 ? c->sched.n_strata : 1)) return rc;
-        if (c->built_S == 1 && c->sched.user_order == 0) {
-            // users in the reference's order for this epoch (get_users, src/lib.rs:486-494)
-            std::vector<int64_t> order(c->U);
-            tx::user_order(c->U, c->cfg.shuffle_users != 0, c->cfg.seed, epoch, order.data());
-            c->h_queue.clear();
-            for (int64_t u : order)
-                if (c->t_off[u + 1] > c->t_off[u]) c->h_queue.push_back(static_cast<uint32_t>(u));
-            HIPC(hipMemcpyAsync(const_cast<uint32_t*>(c->sv.q_users), c->h_queue.data(),
-                                c->h_queue.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
-        }
+        if (c->built_S == 1 && c->sched.user_order != 1)
+            if (int rc = prepare_user_queue(c, epoch, st)) return rc;
         if (c->sched.use_graph) {
             if (!c->graph || c->graph_S != c->built_S) {
                 if (c->graph) (void)hipGraphExecDestroy(c->graph);
@@ -1117,6 +1129,87 @@ int tx_fit_users_epoch(tx_ctx* c, int32_t epoch) {
     stats.ms_other = t - a - b;
     stats.n_launches = 2;
     c->last = stats;
+    return 0;
+}
+
+int tx_autotune(tx_ctx* c, int32_t trial_epochs, tx_schedule* best_out) {
+    g_err.clear();
+    if (!c) return fail(TX_EINVAL, "tx_autotune: NULL ctx");
+    if (c->sched.mode != TX_SCHED_STRATA || c->sched.n_strata != 1)
+        return fail(TX_EUNSUPPORTED, "tx_autotune tunes the one-stratum parallel schedule");
+    if (trial_epochs < 1) trial_epochs = 1;
+    hipStream_t st = c->st;
+    // snapshot of every parameter table (device to device)
+    struct Snap { float* src; float* copy; size_t n; };
+    std::vector<Snap> snaps;
+    auto release = [&]() {
+        for (auto& s : snaps) (void)hipFree(s.copy);
+        snaps.clear();
+    };
+    for (int id = 0; id < TX_P_COUNT; ++id) {
+        const ParamDesc pd = param_desc(c, id);
+        if (!pd.p || pd.rows <= 0) continue;
+        const size_t n = static_cast<size_t>(pd.rows) * (pd.padded ? c->fs : pd.cols);
+        float* cp = nullptr;
+        if (hipMalloc(&cp, n * sizeof(float)) != hipSuccess) {
+            release();
+            return fail(TX_EDEVICE, "tx_autotune: no device memory for the parameter snapshot");
+        }
+        snaps.push_back({pd.p, cp, n});
+        if (hipMemcpyAsync(cp, pd.p, n * sizeof(float), hipMemcpyDeviceToDevice, st) != hipSuccess) {
+            release();
+            return fail(TX_EDEVICE, "tx_autotune: snapshot copy failed");
+        }
+    }
+    auto restore = [&]() -> int {
+        for (auto& s : snaps)
+            HIPC(hipMemcpyAsync(s.src, s.copy, s.n * sizeof(float), hipMemcpyDeviceToDevice, st));
+        HIPC(hipStreamSynchronize(st));
+        return 0;
+    };
+    const tx_schedule orig = c->sched;
+    const int tpb = 256 / c->shape.team, cap = target_teams(c->cfg.n_feat);
+    std::vector<tx_schedule> cands;
+    for (int quarters : {2, 3, 4}) {
+        int blocks = (cap * quarters / 4 + tpb - 1) / tpb;
+        blocks = std::max(8, (blocks + 7) / 8 * 8);
+        const int orders[4][2] = {{0, 0}, {2, 256}, {2, 1024}, {2, 4096}};
+        for (const auto& o : orders) {
+            tx_schedule s = orig;
+            s.blocks = blocks;
+            s.user_order = o[0];
+            s.heavy_min = o[1];
+            s.use_graph = 0;
+            cands.push_back(s);
+        }
+    }
+    double best_ms = 1e30;
+    tx_schedule best = orig;
+    int rc = 0;
+    for (const auto& s : cands) {
+        if ((rc = tx_set_schedule(c, &s))) break;
+        std::vector<double> ms;
+        for (int e = 1; e <= trial_epochs && !rc; ++e) {
+            if ((rc = restore())) break;
+            if ((rc = tx_fit_epoch(c, e))) break;
+            ms.push_back(c->last.ms_total);
+        }
+        if (rc) break;
+        std::sort(ms.begin(), ms.end());
+        const double med = ms[ms.size() / 2];
+        if (med < best_ms) {
+            best_ms = med;
+            best = s;
+        }
+    }
+    const int rc2 = restore();
+    release();
+    if (rc) return rc;
+    if (rc2) return rc2;
+    best.use_graph = orig.use_graph;
+    if (int r = tx_set_schedule(c, &best)) return r;
+    c->bf16_stale = true;
+    if (best_out) *best_out = best;
     return 0;
 }
 

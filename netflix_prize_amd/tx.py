@@ -164,7 +164,7 @@ class CRatings(C.Structure):
 class CSchedule(C.Structure):
     _fields_ = [("mode", C.c_int32), ("n_strata", C.c_int32), ("teams_per_block", C.c_int32),
                 ("blocks", C.c_int32), ("use_graph", C.c_int32), ("update_flags", C.c_int32),
-                ("user_order", C.c_int32), ("reserved", C.c_int32 * 1)]
+                ("user_order", C.c_int32), ("heavy_min", C.c_int32)]
 
 
 class CEpochStats(C.Structure):
@@ -216,6 +216,7 @@ def lib() -> C.CDLL:
         L.tx_fit_users_epoch.argtypes = [vp, i32]
         L.tx_rebuild_ycache.argtypes = [vp]
         L.tx_set_table_precision.argtypes = [vp, i32]
+        L.tx_autotune.argtypes = [vp, i32, vp]
         L.tx_last_epoch_stats.argtypes = [vp, vp]
         L.tx_predict.argtypes = [vp, i64, vp, vp, vp, vp]
         L.tx_predict_device.argtypes = [vp, i64, vp, vp, vp, vp, vp]
@@ -227,7 +228,7 @@ def lib() -> C.CDLL:
         L.tx_get_users.argtypes = [i64, i32, C.c_uint64, i64, vp]
         L.tx_rand_normal.argtypes = [C.c_uint64, C.c_float, i64, vp]
         for name in ["tx_create", "tx_destroy", "tx_set_schedule", "tx_fit_epoch", "tx_fit_users_epoch",
-                     "tx_rebuild_ycache", "tx_set_table_precision",
+                     "tx_rebuild_ycache", "tx_set_table_precision", "tx_autotune",
                      "tx_last_epoch_stats",
                      "tx_predict", "tx_predict_device", "tx_predict_subscores", "tx_calc_rmse", "tx_param_len",
                      "tx_get_param", "tx_set_param", "tx_get_users", "tx_rand_normal"]:
@@ -392,9 +393,10 @@ class TxModel:
 
     def set_schedule(self, mode: int = SCHED_STRATA, n_strata: int = 1, blocks: int = 0, use_graph: bool = False,
                      teams_per_block: int = 0, q_update: int = 0, y_update: int = 2,
-                     user_order: int = 0) -> None:
+                     user_order: int = 2, heavy_min: int = 1024) -> None:
         s = CSchedule()
         s.user_order = int(user_order)
+        s.heavy_min = int(heavy_min)
         s.update_flags = int(q_update) | (int(y_update) << 2)
         s.mode = int(mode)
         s.n_strata = int(n_strata)
@@ -413,6 +415,14 @@ class TxModel:
         Tsvdx4pModel, src/tsvdx4p.rs:390-639, without its NSVD1 accumulators):
         folds users into fixed item tables."""
         _check(lib().tx_fit_users_epoch(self._h, int(epoch)), "tx_fit_users_epoch")
+
+    def autotune(self, trial_epochs: int = 1) -> dict:
+        """Pick teams in flight and the length-bucket bound by timing trial
+        epochs from a snapshot of the parameters (restored afterwards); the
+        chosen schedule is applied and returned."""
+        s = CSchedule()
+        _check(lib().tx_autotune(self._h, int(trial_epochs), C.byref(s)), "tx_autotune")
+        return {k: getattr(s, k) for k, _ in CSchedule._fields_}
 
     def set_table_precision(self, precision: str) -> None:
         """"f32" (default) or "bf16": the factor tables predictions read

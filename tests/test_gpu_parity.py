@@ -137,3 +137,22 @@ def test_full_shape_holdout_rmse_matches_serial_reference(gpu, f):
     print("gpu", [round(x, 5) for x in curve])
     print("ref", [round(x, 5) for x in ref])
     assert abs(curve[-1] - ref[-1]) <= 1e-3
+
+
+def test_autotune_leaves_parameters_and_picks_bounded_schedule(gpu):
+    """The autotuner times candidate schedules from a snapshot of every table
+    and restores it: the model's parameters are bit-identical afterwards, the
+    chosen schedule stays within the fidelity bound on teams in flight, and
+    training then proceeds as usual."""
+    tr, pr = synthetic("small")
+    cfg = lm_config(n_feat=32, n_epochs=2, n_time_bins=30)
+    m = tx.TxModel(tr, pr, cfg)
+    m.fit_epoch(epoch=1)
+    before = {n: m.get(n) for n in ALL_PARAMS}
+    best = m.autotune(trial_epochs=1)
+    for n, v in before.items():
+        np.testing.assert_array_equal(m.get(n), v, err_msg=n)
+    teams = best["blocks"] * (256 // 8)          # f=32: 8-lane teams, 32 per block
+    assert best["user_order"] in (0, 2) and 0 < teams <= 3072 + 256, best
+    m.fit_epoch(epoch=2)
+    assert m.calc_rmse(pr) < 1.1
