@@ -1070,28 +1070,30 @@ __global__ __launch_bounds__(256) void k_rebuild_ycache(DevModel m) {
     const int lane = threadIdx.x % TEAM;
     const int nteams = gridDim.x * blockDim.x / TEAM;
     const int fs = m.fs, off = lane * VEC;
+    const bool yb_on = !m.skip_yb;
     for (int u = team; u < m.U; u += nteams) {
         float acc[VEC];
 #pragma unroll
         for (int c = 0; c < VEC; ++c) acc[c] = 0.0f;
         float sb = 0.0f;
-        const uint32_t ts = m.t_off[u], te = m.t_off[u + 1];
-        for (uint32_t t = ts; t < te; ++t) {
-            const uint32_t j = m.t_item[t];
-            float y[VEC];
-            load_vec<VEC>(m.yfeat + static_cast<size_t>(j) * fs + off, y);
+        const uint32_t ts = m.t_off[u], ntr = m.t_off[u + 1] - ts;
+        const uint32_t n = ntr + (m.h_start[u + 1] - m.h_start[u]);
+        const uint32_t* titems = m.t_item + ts;
+        const uint32_t* hitems = m.h_item + m.h_start[u] - ntr;  // row r >= ntr is hitems[r]
+        // rows streamed in order (training rows, then held-out rows); every
+        // per-element sum and the ybias sum run sequentially in that order
+        for_rows<TEAM, VEC, row_group<TEAM, VEC>()>(
+            n, lane, m.yfeat, fs, [&](uint32_t r) { return r < ntr ? titems + r : hitems + r; },
+            [&](uint32_t b0, uint32_t ib) {
+                if (!yb_on) return;
+                const float ybl = b0 + static_cast<uint32_t>(lane) < n ? m.ybias[ib] : 0.0f;
+                const uint32_t nb = n - b0 < static_cast<uint32_t>(TEAM) ? n - b0 : static_cast<uint32_t>(TEAM);
+                for (uint32_t k = 0; k < nb; ++k) sb += team_bcast_f<TEAM>(ybl, k);
+            },
+            [&](uint32_t, uint32_t, const float (&y)[VEC]) {
 #pragma unroll
-            for (int c = 0; c < VEC; ++c) acc[c] += y[c];
-            if (lane == 0) sb += m.ybias[j];
-        }
-        for (uint32_t q = m.h_start[u]; q < m.h_start[u + 1]; ++q) {
-            const uint32_t j = m.h_item[q];
-            float y[VEC];
-            load_vec<VEC>(m.yfeat + static_cast<size_t>(j) * fs + off, y);
-#pragma unroll
-            for (int c = 0; c < VEC; ++c) acc[c] += y[c];
-            if (lane == 0) sb += m.ybias[j];
-        }
+                for (int c = 0; c < VEC; ++c) acc[c] += y[c];
+            });
         const float nrm = m.norm_rebuild[u];
         if (nrm > 0.0f) {
 #pragma unroll
