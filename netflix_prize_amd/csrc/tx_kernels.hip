@@ -614,12 +614,15 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
                 n.sc_addr = sc_base + static_cast<size_t>(n.i) * sc_imul + (static_cast<uint32_t>(n.b) & sc_bm) +
                             (n.ud & sc_um);
                 n.sc = *n.sc_addr;
-                if (!(m.skip_cut && m.skip_but)) {
+                {
+                    // always one load: with both terms skipped it re-reads the sc
+                    // slot just fetched (cache-hot, ignored).  A branch that set
+                    // sc2 = 0 instead would write a register whose earlier load
+                    // may be in flight, and the wait for it drains the pipeline.
                     const float* a = m.skip_cut ? &m.but_bin[ubrow + n.b] : &m.cut[n.ud];
                     a = (lane == 1 && !m.skip_but) ? &m.but_bin[ubrow + n.b] : a;
+                    a = (m.skip_cut && m.skip_but) ? n.sc_addr : a;
                     n.sc2 = *a;
-                } else {
-                    n.sc2 = 0.0f;
                 }
             };
             float p[VEC], p2[VEC], s[VEC], G[VEC];
@@ -730,6 +733,16 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
             }
 #pragma unroll
             for (int k = 0; k < DEPTH; ++k) fetch(rec_at(s0 + k < s1 ? s0 + k : s1 - 1), ring[k]);
+#ifdef TX_PROLOGUE_PAD
+            {
+                // re-read the q row of the first rating TX_PROLOGUE_PAD times (L2-hot):
+                // the first steps then see as many younger memory operations as
+                // the steady state, so the compiler's merged wait counts rise
+                const volatile float* pq = q_base + static_cast<size_t>(ring[0].i) * fs;
+#pragma unroll
+                for (int k = 0; k < TX_PROLOGUE_PAD; ++k) (void)pq[(k & (VEC - 1)) * TEAM];
+            }
+#endif
 
             static_assert(TEAM % RING == 0, "the prefetch depth plus one must divide the team size");
             for (; tb < s1; tb += TEAM) {
