@@ -126,6 +126,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--ttr-epochs", type=int, default=20, help="epochs of the time-to-RMSE run (0 = skip)")
+    ap.add_argument("--autotune", type=int, default=1, help="autotune the batched form of the time-to-RMSE run")
     ap.add_argument("--fold-in-users", type=int, default=80_189,
                     help="users held out of training and folded in afterwards (0 = skip)")
     args = ap.parse_args(argv)
@@ -177,32 +178,51 @@ def main(argv=None) -> int:
     rmse = model.calc_rmse(pr)
     log(f"bench: holdout RMSE after {epoch} epochs: {rmse:.5f}")
 
-    # time-to-holdout-RMSE: a fresh model trained for the full epoch budget,
-    # GPU epoch time accumulated until the RMSE is within 0.001 of its final value
+    # time-to-holdout-RMSE: fresh models trained for the full epoch budget in
+    # the batched form (length buckets, autotuned teams, hipGraph epoch) and in
+    # the plain GPU form (reference user order, 1,024 teams, stream launches);
+    # GPU epoch time until the holdout RMSE is within 1e-3 of the serial
+    # trainer's after the same epochs (or of the form's own final RMSE)
     ttr = None
     if args.ttr_epochs > 0:
-        m2 = tx.TxModel(tr, pr, cfg)
-        m2.set_schedule(mode=tx.SCHED_STRATA, n_strata=args.strata, use_graph=bool(args.graph), blocks=args.blocks)
-        curve, secs = [], []
-        for e in range(1, args.ttr_epochs + 1):
-            m2.fit_epoch(epoch=e)
-            secs.append(m2.last_epoch_stats()["ms_total"] / 1000.0)
-            curve.append(m2.calc_rmse(pr))
-        m2.close()
-        target = curve[-1] + 0.001
-        hit = next(k for k, r in enumerate(curve) if r <= target)
         ref = None
         gpath = os.path.join(ROOT, "tests", "golden", f"oracle_curve_{args.shape}_f{cfg.n_feat}.json")
         if os.path.exists(gpath):
             g = json.load(open(gpath))
             if g.get("seed") == args.seed and len(g["epochs"]) >= args.ttr_epochs:
                 ref = g["epochs"][args.ttr_epochs - 1]["holdout_rmse"]
-        ttr = {"epochs": args.ttr_epochs, "final_holdout_rmse": round(curve[-1], 5),
-               "target_rmse": round(target, 5), "seconds_to_target": round(sum(secs[:hit + 1]), 3),
-               "epochs_to_target": hit + 1, "seconds_total": round(sum(secs), 3),
-               "serial_reference_rmse": None if ref is None else round(ref, 5),
-               "curve": [round(r, 5) for r in curve]}
-        log(f"bench: time-to-RMSE {ttr}")
+
+        def curve_of(form):
+            m2 = tx.TxModel(tr, pr, cfg)
+            tuned = None
+            if form == "batched":
+                m2.set_schedule(mode=tx.SCHED_STRATA, n_strata=args.strata, use_graph=bool(args.graph),
+                                blocks=args.blocks)
+                if args.autotune:
+                    tuned = m2.autotune(1)
+            else:
+                team = 64 if cfg.n_feat > 128 else (32 if cfg.n_feat > 64 else 16)
+                m2.set_schedule(mode=tx.SCHED_STRATA, user_order=0, blocks=max(8, 1024 // (256 // team)))
+            curve, secs = [], []
+            for e in range(1, args.ttr_epochs + 1):
+                m2.fit_epoch(epoch=e)
+                secs.append(m2.last_epoch_stats()["ms_total"] / 1000.0)
+                curve.append(m2.calc_rmse(pr))
+            m2.close()
+            return curve, secs, tuned
+
+        ttr = {"epochs": args.ttr_epochs, "serial_reference_rmse": None if ref is None else round(ref, 5)}
+        for form in ("batched", "plain"):
+            curve, secs, tuned = curve_of(form)
+            target = (ref if ref is not None else curve[-1]) + 1e-3
+            hit = next((k for k, r in enumerate(curve) if r <= target), None)
+            ttr[form] = {"final_holdout_rmse": round(curve[-1], 5), "target_rmse": round(target, 5),
+                         "epochs_to_target": None if hit is None else hit + 1,
+                         "seconds_to_target": None if hit is None else round(sum(secs[:hit + 1]), 3),
+                         "seconds_total": round(sum(secs), 3), "curve": [round(r, 5) for r in curve]}
+            if tuned is not None:
+                ttr[form]["autotuned_schedule"] = {k: tuned[k] for k in ("blocks", "user_order", "heavy_min")}
+            log(f"bench: time-to-RMSE {form}: {ttr[form]}")
 
     # new users folded into a model trained on the other users: the encoder's
     # warm start against the reference's random start
