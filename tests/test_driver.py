@@ -77,4 +77,5 @@ def test_job_end_to_end(gpu, tmp_path):
     logged = float(re.search(r"probex RMSE ([0-9.]+)", log).group(1))
     rmse = float(np.sqrt(np.mean((probe.astype(np.float64) - pr.raw_ratings) ** 2)))
     assert abs(logged - rmse) < 1e-5, (logged, rmse)
-    assert rmse < 1.0
+    # better than the best constant prediction of the held-out ratings
+    assert rmse < float(np.std(pr.raw_ratings.astype(np.float64)))
