@@ -129,6 +129,8 @@ def main(argv=None) -> int:
     ap.add_argument("--autotune", type=int, default=1, help="autotune the batched form of the time-to-RMSE run")
     ap.add_argument("--fold-in-users", type=int, default=80_189,
                     help="users held out of training and folded in afterwards (0 = skip)")
+    ap.add_argument("--day-epochs", type=int, default=3,
+                    help="epochs of the tsvdx4-60 job's model (day terms, f=60) in the parallel schedule (0 = skip)")
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -201,8 +203,8 @@ def main(argv=None) -> int:
                 if args.autotune:
                     tuned = m2.autotune(1)
             else:
-                team = 64 if cfg.n_feat > 128 else (32 if cfg.n_feat > 64 else 16)
-                m2.set_schedule(mode=tx.SCHED_STRATA, user_order=0, blocks=max(8, 1024 // (256 // team)))
+                # 1,024 one-wave teams, four per 256-thread block
+                m2.set_schedule(mode=tx.SCHED_STRATA, user_order=0, blocks=1024 // 4)
             curve, secs = [], []
             for e in range(1, args.ttr_epochs + 1):
                 m2.fit_epoch(epoch=e)
@@ -232,6 +234,35 @@ def main(argv=None) -> int:
         n_new = args.fold_in_users if args.fold_in_users < spec.n_users else spec.n_users // 6
         log(f"bench: fold-in of {n_new} new users (encoder vs random start)")
         fold = E.evaluate_new_users(tr, pr, cfg, spec.n_users - n_new, args.ttr_epochs or 20, 5, 20, log=log)
+
+    # the tsvdx4-60 job's model (lm set with day terms, low_memory false, f=60,
+    # 29 time bins, seed 42) in the default parallel schedule: epoch times and
+    # the holdout RMSE against the serial oracle's after the same epochs
+    day = None
+    if args.day_epochs > 0:
+        dcfg = tx.TxConfig(n_feat=60, n_epochs=20, seed=42, n_time_bins=29, low_memory=False)
+        t0 = time.time()
+        dm = tx.TxModel(tr, pr, dcfg)
+        dm.set_schedule(mode=tx.SCHED_STRATA, use_graph=bool(args.graph))
+        log(f"bench: day-term model created ({time.time() - t0:.1f}s)")
+        dst, dcurve = [], []
+        for e in range(1, args.day_epochs + 1):
+            dm.fit_epoch(epoch=e)
+            dst.append(dm.last_epoch_stats())
+            dcurve.append(dm.calc_rmse(pr))
+            log(f"bench: day-term epoch {e}: {dst[-1]['ms_total']:.1f} ms, holdout RMSE {dcurve[-1]:.5f}")
+        dm.close()
+        gpath = os.path.join(ROOT, "tests", "golden", f"oracle_curve_{args.shape}_job60.json")
+        dref = None
+        if os.path.exists(gpath):
+            g = json.load(open(gpath))
+            if g.get("seed") == args.seed and len(g["epochs"]) >= args.day_epochs:
+                dref = [x["holdout_rmse"] for x in g["epochs"][:args.day_epochs]]
+        ms = statistics.median(x["ms_total"] for x in dst[1:] or dst)
+        day = {"workload": "tsvdx4-60 model: lm set, day terms, f=60, 29 time bins, 1 stratum",
+               "ms_per_epoch": round(ms, 2), "ratings_per_s": round(n_train / (ms / 1000.0), 1),
+               "holdout_rmse": [round(r, 5) for r in dcurve],
+               "serial_reference_rmse": None if dref is None else [round(r, 5) for r in dref]}
 
     # roofline of the dominant kernel (SGD), from the library's events on its stream
     qf = cfg.lr_iqf != 0.0 or cfg.sigma_iqf != 0.0
@@ -284,6 +315,7 @@ def main(argv=None) -> int:
         "roofline": roofline,
         "time_to_rmse": ttr,
         "fold_in": fold,
+        "day_terms": day,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

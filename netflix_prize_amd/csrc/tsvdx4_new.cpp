@@ -10,8 +10,9 @@
 //
 // Environment knobs (the reference has none; they select GPU schedules and
 // let synthetic data of another shape through Dataset::load):
-//   TSVD_SCHEDULE = strata | serial   (default: strata for low-memory jobs,
-//                                      serial for jobs with day terms)
+//   TSVD_SCHEDULE = strata | serial   (default: the library's, one-stratum
+//                                      parallel; serial for day terms on
+//                                      ratings not sorted by date per user)
 //   TSVD_STRATA   = 1 | 8 | ...       (strata of the parallel schedule)
 //   TSVD_N_USERS / TSVD_N_ITEMS       (default 480189 / 17770, as the reference)
 //   TSVD_DATA_DIR                     (default "data")
@@ -310,12 +311,18 @@ struct Model {
     Model(const Dataset& tr, const Dataset& pr, const tx_config& c) : cfg(c), n_items(tr.n_items) {
         const tx_ratings t = tr.view(false), p = pr.view(true);
         check(tx_create(&cfg, &t, &p, &ctx), "TxModel::new");
-        tx_schedule s{};
-        const std::string mode = env("TSVD_SCHEDULE", cfg.low_memory ? "strata" : "serial");
-        s.mode = mode == "serial" ? TX_SCHED_SERIAL : TX_SCHED_STRATA;
+        // the library's default schedule, epochs captured in a graph, mode and
+        // strata from the environment
+        const std::string mode = env("TSVD_SCHEDULE", "");
+        tx_schedule s{TX_SCHED_STRATA, 1, 0, 0, 1, TX_UPD_ATOMIC | (TX_UPD_PLAIN << 2), 2, 1024};
+        if (mode == "serial") s.mode = TX_SCHED_SERIAL;
         s.n_strata = std::atoi(env("TSVD_STRATA", "1").c_str());
-        s.use_graph = 1;
-        check(tx_set_schedule(ctx, &s), "tx_set_schedule");
+        int rc = tx_set_schedule(ctx, &s);
+        if (rc == TX_EUNSUPPORTED && mode.empty() && !cfg.low_memory) {
+            s.mode = TX_SCHED_SERIAL;  // day terms on ratings not sorted by date
+            rc = tx_set_schedule(ctx, &s);
+        }
+        check(rc, "tx_set_schedule");
     }
     ~Model() { tx_destroy(ctx); }
     void fit_epoch(int epoch) { check(tx_fit_epoch(ctx, epoch), "fit_epoch"); }

@@ -88,13 +88,13 @@ struct Shape {
     int team, vec;
 };
 
+// One wave (64 lanes) per user at every f: at f=60, four 16-lane teams per
+// wave took 95 ms per epoch of the full-shape lm model against 69 ms with one
+// element per lane (each wave ran its four users' loops to the longest).
 bool pick_shape(int f, Shape* s) {
-    const int lanes = (std::max(f, 1) + 3) / 4;
-    if (lanes <= 4) *s = {4, 4};
-    else if (lanes <= 8) *s = {8, 4};
-    else if (lanes <= 16) *s = {16, 4};
-    else if (lanes <= 32) *s = {32, 4};
-    else if (lanes <= 64) *s = {64, 4};
+    if (f <= 64) *s = {64, 1};
+    else if (f <= 128) *s = {64, 2};
+    else if (f <= 256) *s = {64, 4};
     else if (f <= 512) *s = {64, 8};
     else if (f <= 1024) *s = {64, 16};
     else return false;
@@ -120,6 +120,7 @@ struct tx_ctx {
     std::vector<int8_t> t_raw;
     std::vector<uint8_t> sud_fbin;
     uint4* d_rec = nullptr;  // packed records, reference order
+    bool days_sorted = true;  // every user's training ratings in date order (day terms in parallel)
 
     // BF16 read copies of the factor tables for predictions (FP32 masters)
     bool bf16 = false, bf16_stale = true;
@@ -288,7 +289,9 @@ inline uint4 pack_rec(const tx_ctx* c, int64_t t) {
     uint4 r;
     const uint32_t raw = c->t_raw.empty() ? 0u : static_cast<uint32_t>(c->t_raw[t]) & 15u;
     r.x = c->t_item[t] | (raw << 20) | (static_cast<uint32_t>(c->t_bin[t]) << 24);
-    r.y = c->t_ud[t];
+    // training uses freq bin 0 in low-memory mode (src/tx.rs:838-842), the day's bin otherwise
+    const uint32_t fb = c->cfg.low_memory ? 0u : static_cast<uint32_t>(c->sud_fbin[c->t_ud[t]]);
+    r.y = c->t_ud[t] | (fb << 28);
     std::memcpy(&r.z, &c->t_dev[t], 4);
     std::memcpy(&r.w, &c->t_r[t], 4);
     return r;
@@ -395,6 +398,11 @@ int build_strata(tx_ctx* c, int S) {
         sv.p_rec = c->d_rec;
         sv.gacc = nullptr;
         sv.gacc_b = nullptr;
+        if (!c->cfg.low_memory && c->n_ud > 0) {
+            float* da;
+            if (int rc = c->alloc(&da, static_cast<size_t>(c->n_ud) * c->fs, L)) return rc;
+            sv.day_acc = da;
+        }
     } else {
         std::vector<uint4> rec(N);
         parallel_for(N, [&](int64_t lo, int64_t hi) {
@@ -559,7 +567,7 @@ int tsvdx4p_epoch(tx_ctx* c, int32_t epoch, tx_epoch_stats& stats, hipStream_t s
     HIPC(tx::launch_fill_u32(c->sv.q_head, 0u, c->sv.n_cells, st));
     HIPC(tx::launch_fill_u32(c->iv.q_head, 0u, 1, st));
     HIPC(hipEventRecord(c->ev[1], st));
-    HIPC(tx::launch_sgd_par(c->dm, c->sv, 0, c->shape.team, c->shape.vec, sgpr_blocks, false, p2, false, 1, st));
+    HIPC(tx::launch_sgd_par(c->dm, c->sv, 0, c->shape.team, c->shape.vec, sgpr_blocks, false, p2, false, 1, false, st));
     HIPC(tx::launch_apply_nsvd(c->dm, c->sv.d_y, c->sv.d_yb, c->d_scale, st));
     HIPC(hipEventRecord(c->ev[2], st));
     HIPC(tx::launch_rebuild(c->dm, c->n_ud, c->shape.team, c->shape.vec, st));
@@ -582,10 +590,11 @@ int enqueue_strata_epoch(tx_ctx* c, hipStream_t st) {
     const bool qf = c->cfg.lr_iqf != 0.0f || c->cfg.sigma_iqf != 0.0f;
     const bool p2 = c->cfg.lr_u2 != 0.0f;
     const bool ord = c->cfg.has_ordinal != 0;
+    const bool day = c->cfg.low_memory == 0;
     const int blocks = c->sched.blocks > 0 ? c->sched.blocks : default_blocks(c);
     HIPC(tx::launch_fill_u32(c->sv.q_head, 0u, static_cast<int64_t>(S) * c->sv.n_cells, st));
     for (int s = 0; s < S; ++s)
-        HIPC(tx::launch_sgd_par(c->dm, c->sv, s, c->shape.team, c->shape.vec, blocks, qf, p2, ord, 0, st));
+        HIPC(tx::launch_sgd_par(c->dm, c->sv, s, c->shape.team, c->shape.vec, blocks, qf, p2, ord, 0, day, st));
     HIPC(tx::launch_rebuild(c->dm, c->n_ud, c->shape.team, c->shape.vec, st));
     return 0;
 }
@@ -773,6 +782,15 @@ int tx_create(const tx_config* cfg, const tx_ratings* tr, const tx_ratings* pr, 
     std::vector<uint8_t>& sud_fbin = c->sud_fbin;
     sud_fbin.assign(D, 0);
     for (int64_t d = 0; d < D; ++d) sud_fbin[d] = static_cast<uint8_t>(freq_bin_host(sud_cnt[d], F));
+    if (D >= (int64_t{1} << 28)) return fail(TX_EUNSUPPORTED, "tx_create: 2^28 or more (user, day) pairs");
+    for (int64_t u = 0; u < U && c->days_sorted; ++u)
+        for (uint32_t t = c->t_off[u] + 1; t < c->t_off[u + 1]; ++t)
+            if (c->t_ud[t] < c->t_ud[t - 1]) {
+                c->days_sorted = false;
+                break;
+            }
+    // day terms train in the parallel schedule when the ratings are date-ordered
+    if (!cfg->low_memory && !c->days_sorted) c->sched.mode = TX_SCHED_SERIAL;
 
     // norms: training su (src/tx.rs:808-819) and epoch-end rebuild (src/tx.rs:410-416)
     std::vector<float> norm_train(U), norm_rebuild(U);
@@ -933,6 +951,41 @@ int tx_create(const tx_config* cfg, const tx_ratings* tr, const tx_ratings* pr, 
     RC(c->upload(&p32, hd_item.data(), hd_item.size())); m.hd_item = p32;
     RC(c->upload(&p32, td_start.data(), td_start.size())); m.td_start = p32;
     RC(c->upload(&p32, td_item.data(), td_item.size())); m.td_item = p32;
+    if (!low_mem && c->days_sorted) {
+        // day streams of the parallel day-term kernel (see DevModel::dg_item)
+        std::vector<uint32_t> dg_off(U + 1, 0), dy_off(U + 1, 0);
+        for (int64_t u = 0; u < U; ++u) {
+            uint32_t ng = 0, ny = 0;
+            for (uint32_t d = sud_start[u]; d < sud_start[u + 1]; ++d) {
+                const uint32_t nt = td_start[d + 1] - td_start[d];
+                if (nt == 0) continue;
+                ng += nt + (cfg->full_su ? hd_start[d + 1] - hd_start[d] : 0) + 1;
+                ny += nt + 1;
+            }
+            if (uint64_t{dg_off[u]} + ng > 0xffffffffu) return fail(TX_EUNSUPPORTED, "tx_create: day stream over 2^32 rows");
+            dg_off[u + 1] = dg_off[u] + ng;
+            dy_off[u + 1] = dy_off[u] + ny;
+        }
+        std::vector<uint32_t> dg(dg_off[U]), dy(dy_off[U]);
+        parallel_for(U, [&](int64_t lo, int64_t hi) {
+            for (int64_t u = lo; u < hi; ++u) {
+                uint32_t* g = dg.data() + dg_off[u];
+                uint32_t* y = dy.data() + dy_off[u];
+                for (uint32_t d = sud_start[u]; d < sud_start[u + 1]; ++d) {
+                    if (td_start[d + 1] == td_start[d]) continue;
+                    *y++ = d | 0x80000000u;
+                    for (uint32_t k = td_start[d]; k < td_start[d + 1]; ++k) *g++ = *y++ = td_item[k];
+                    if (cfg->full_su)
+                        for (uint32_t k = hd_start[d]; k < hd_start[d + 1]; ++k) *g++ = hd_item[k];
+                    *g++ = d | 0x80000000u;
+                }
+            }
+        });
+        RC(c->upload(&p32, dg_off.data(), dg_off.size())); m.dg_off = p32;
+        RC(c->upload(&p32, dg.data(), dg.size())); m.dg_item = p32;
+        RC(c->upload(&p32, dy_off.data(), dy_off.size())); m.dy_off = p32;
+        RC(c->upload(&p32, dy.data(), dy.size())); m.dy_item = p32;
+    }
     RC(c->upload(&pf, norm_train.data(), norm_train.size())); m.norm_train = pf;
     RC(c->upload(&pf, norm_rebuild.data(), norm_rebuild.size())); m.norm_rebuild = pf;
     RC(c->alloc(&c->d_order, static_cast<size_t>(U)));
@@ -963,8 +1016,12 @@ int tx_set_schedule(tx_ctx* c, const tx_schedule* s) {
     if (s->mode == TX_SCHED_STRATA) {
         if (s->n_strata < 1 || (s->n_strata > 1 && s->n_strata % 8 != 0) || s->n_strata > 64)
             return fail(TX_EINVAL, "n_strata must be 1 or a multiple of 8 (<= 64), got %d", s->n_strata);
-        if (!c->cfg.low_memory)
-            return fail(TX_EUNSUPPORTED, "the parallel schedule trains low_memory models; use TX_SCHED_SERIAL for day terms");
+        if (!c->cfg.low_memory && s->n_strata != 1)
+            return fail(TX_EUNSUPPORTED, "day terms (low_memory = false) train in the one-stratum parallel schedule");
+        if (!c->cfg.low_memory && !c->days_sorted)
+            return fail(TX_EUNSUPPORTED,
+                        "the parallel schedule trains day terms when every user's training ratings are sorted by "
+                        "date; use TX_SCHED_SERIAL");
         if (c->cfg.has_ordinal && c->cfg.lr_t != 0.0f)
             return fail(TX_EUNSUPPORTED, "the parallel schedule keeps ordinal thresholds fixed (lr_t must be 0)");
         if (s->blocks < 0 || (s->blocks > 0 && s->blocks < s->n_strata))
@@ -1070,11 +1127,12 @@ int tx_fit_epoch(tx_ctx* c, int32_t epoch) {
             const bool qf = c->cfg.lr_iqf != 0.0f || c->cfg.sigma_iqf != 0.0f;
             const bool p2 = c->cfg.lr_u2 != 0.0f;
             const bool ord = c->cfg.has_ordinal != 0;
+            const bool day = c->cfg.low_memory == 0;
             const int blocks = c->sched.blocks > 0 ? c->sched.blocks : default_blocks(c);
             HIPC(tx::launch_fill_u32(c->sv.q_head, 0u, static_cast<int64_t>(S) * c->sv.n_cells, st));
             HIPC(hipEventRecord(c->ev[1], st));
             for (int s = 0; s < S; ++s)
-                HIPC(tx::launch_sgd_par(c->dm, c->sv, s, c->shape.team, c->shape.vec, blocks, qf, p2, ord, 0, st));
+                HIPC(tx::launch_sgd_par(c->dm, c->sv, s, c->shape.team, c->shape.vec, blocks, qf, p2, ord, 0, day, st));
             HIPC(hipEventRecord(c->ev[2], st));
             HIPC(tx::launch_rebuild(c->dm, c->n_ud, c->shape.team, c->shape.vec, st));
             HIPC(hipEventRecord(c->ev[3], st));
@@ -1114,7 +1172,7 @@ int tx_fit_users_epoch(tx_ctx* c, int32_t epoch) {
     HIPC(tx::launch_fill_u32(c->sv.q_head, 0u, c->sv.n_cells, st));
     HIPC(hipEventRecord(c->ev[1], st));
     const int blocks = c->sched.blocks > 0 ? c->sched.blocks : user_pass_blocks(c);
-    HIPC(tx::launch_sgd_par(c->dm, c->sv, 0, c->shape.team, c->shape.vec, blocks, qf, p2, false, 2, st));
+    HIPC(tx::launch_sgd_par(c->dm, c->sv, 0, c->shape.team, c->shape.vec, blocks, qf, p2, false, 2, false, st));
     HIPC(hipEventRecord(c->ev[2], st));
     HIPC(tx::launch_rebuild(c->dm, c->n_ud, c->shape.team, c->shape.vec, st));
     HIPC(hipEventRecord(c->ev[3], st));

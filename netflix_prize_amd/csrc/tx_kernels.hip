@@ -492,6 +492,57 @@ constexpr int row_group() {
     return (TEAM / 2 < 32 / VEC) ? TEAM / 2 : 32 / VEC;
 }
 
+// for_rows over a row stream whose entries may name rows of different
+// tables: entry j's row starts at row_ptr(j); use(j, row) sees the entry and
+// the row in the contiguous layout.  Same pipeline: indices a batch ahead,
+// rows GB at a time double-buffered, every load unconditional (indexes past n
+// clamped).  (Row memory is the natural element order in every layout, so a
+// row this walker stores contiguously reads back strided elsewhere.)
+template <int TEAM, int VEC, int GB, class RowPtr, class Use>
+__device__ __forceinline__ void for_rows_at(uint32_t n, int lane, const uint32_t* idx, RowPtr row_ptr, Use use) {
+    static_assert(TEAM % (2 * GB) == 0, "two row groups must tile an index batch");
+    if (n == 0) return;
+    auto idx_load = [&](uint32_t r) { return idx[r < n ? r : n - 1]; };
+    uint32_t b0 = 0;
+    uint32_t ib = idx_load(static_cast<uint32_t>(lane)), ibn = idx_load(TEAM + static_cast<uint32_t>(lane));
+    auto jrow = [&](uint32_t r) -> uint32_t {
+        const uint32_t d = r - b0;
+        if (d < TEAM) return team_bcast<TEAM>(ib, d);
+        asm volatile("");
+        return team_bcast<TEAM>(ibn, d - TEAM);
+    };
+    float ya[GB][VEC], yb[GB][VEC];
+    uint32_t ja[GB], jb[GB];
+    auto load = [&](uint32_t r0, float (&y)[GB][VEC], uint32_t (&jj)[GB]) {
+#pragma unroll
+        for (int k = 0; k < GB; ++k) {
+            jj[k] = jrow(r0 + k);
+            load_vec<VEC>(row_ptr(jj[k]) + lane * VEC, y[k]);
+        }
+    };
+    load(0, ya, ja);
+    for (uint32_t r0 = 0;; r0 += 2 * GB) {
+        load(r0 + GB, yb, jb);
+#pragma unroll
+        for (int k = 0; k < GB; ++k)
+            if (r0 + k < n) use(ja[k], ya[k]);
+        if (r0 + GB >= n) break;
+        load(r0 + 2 * GB, ya, ja);
+#pragma unroll
+        for (int k = 0; k < GB; ++k)
+            if (r0 + GB + k < n) use(jb[k], yb[k]);
+        if (r0 + 2 * GB >= n) break;
+        if (r0 + 2 * GB - b0 == TEAM) {
+            b0 += TEAM;
+            ib = ibn;
+            ibn = idx_load(b0 + TEAM + static_cast<uint32_t>(lane));
+        }
+    }
+}
+
+// Day-stream entries with this bit set are day headers (index = SparseUD day).
+constexpr uint32_t kDayHead = 0x80000000u;
+
 // The next training rating and everything read for it before its update.
 // The rating's scalars arrive in one load spread over team lanes:
 //   sc  lane 0 b_i, 1 b_i,bin, 2 b_u,day, 3 b_i,f(0)
@@ -500,12 +551,15 @@ constexpr int row_group() {
 // is forwarded in bud/but (has_bud/has_but set) rather than written into
 // sc/sc2: those registers may still be waiting for their load, and writing
 // them would stall the wave until every younger memory operation returned.
-template <int VEC>
+template <int VEC, bool DAY>
 struct Pending {
-    uint32_t i, ud;
+    uint32_t i, ud, fb;
     float dev, r;
     int b, raw;
     float q[VEC], qf[VEC];
+    // day terms: p_u,day, the day's NSVD1 sum over y_day (strided) and the run norm
+    float pd[DAY ? VEC : 1], sd[DAY ? VEC : 1];
+    float nd;
     float sc, sc2;
     float* sc_addr;  // this lane's slot of sc (read at fetch, updated after the rating)
     float bud, but;
@@ -526,9 +580,21 @@ struct Pending {
 //         per-item accumulators (sv.d_y, sv.d_yb) that a later kernel applies.
 // PASS 2: the same user pass without the accumulators (fold-in of users
 //         against fixed item tables).
-template <int TEAM, int VEC, bool QF, bool P2, bool ORD, int PASS>
+// DAY:    the day terms of a model that is not low-memory (src/tx.rs:846-871,
+//         1025-1036), one stratum only.  A user's training ratings are sorted
+//         by day, so each SparseUD day is one same-day run.  At user start one
+//         row stream (m.dg_item) sums every training day's y_day rows (its
+//         training items, plus its held-out items with full_su) into
+//         sv.day_acc; a run divides that sum by the run norm, carries p_u,day
+//         and its gradient sum in registers and leaves them in ufeat_day and
+//         sv.day_acc; at user end a second stream (m.dy_item) applies the y_day
+//         step of every run.  A user's days have disjoint item sets (each item
+//         is rated once), so only other users' concurrent steps separate this
+//         order from the reference's gather-at-run-start, step-at-run-end.
+template <int TEAM, int VEC, bool QF, bool P2, bool ORD, int PASS, bool DAY>
 __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, StrataView sv, int stratum) {
     static_assert(PASS == 0 || !ORD, "the user passes have no ordinal head");
+    static_assert(PASS == 0 || !DAY, "the user passes train low-memory models");
     constexpr int CHUNK = 4;
     constexpr int DEPTH = TX_PREFETCH;
     __shared__ float xbuf[256 * VEC];  // per-team transpose slot, TEAM*VEC floats each
@@ -550,6 +616,8 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
     h.w_bias = in_vgpr(h.w_bias); h.w_factor = in_vgpr(h.w_factor);
     if constexpr (P2) { h.lr_u2 = in_vgpr(h.lr_u2); h.reg_u2 = in_vgpr(h.reg_u2); }
     if constexpr (QF) { h.lr_iqf = in_vgpr(h.lr_iqf); h.reg_iqf = in_vgpr(h.reg_iqf); }
+    const bool has_ud = h.lr_ud != 0.0f;  // p_u,day trained (src/tx.rs:1005-1012)
+    if constexpr (DAY) { h.lr_ud = in_vgpr(h.lr_ud); h.reg_ud = in_vgpr(h.reg_ud); }
     const int off = lane;  // strided: element c*TEAM + lane
     const bool first = (stratum == 0);
     const bool last = (stratum == S - 1);
@@ -565,6 +633,7 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
     const uint32_t sc_imul = lane == 1 ? static_cast<uint32_t>(B) : lane == 2 ? 0u : l3 ? static_cast<uint32_t>(F) : 1u;
     const uint32_t sc_bm = lane == 1 ? 0xffffffffu : 0u;
     const uint32_t sc_um = lane == 2 ? 0xffffffffu : 0u;
+    const uint32_t sc_fm = (DAY && l3) ? 0xffffffffu : 0u;  // b_i,f(t): the day's freq bin
     const float sc_lr = lane == 0 ? h.lr_ib : lane == 1 ? h.lr_ti : h.lr_ibf;
     float* const q_base = m.ifeat + off;
 
@@ -595,24 +664,32 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
             const size_t ubrow = static_cast<size_t>(u) * B;
 
             // issue the loads one rating needs, from its packed record
-            auto fetch = [&](uint4 rc, Pending<VEC>& n) {
+            auto fetch = [&](uint4 rc, Pending<VEC, DAY>& n) {
                 n.i = rc.x & 0xFFFFFu;
                 n.raw = static_cast<int>((rc.x >> 20) & 15u);
                 n.b = static_cast<int>(rc.x >> 24);
-                n.ud = rc.y;
+                // the freq bin is packed above the day index for day-term models
+                // only (low-memory records carry bin 0, so y is the index)
+                n.ud = DAY ? (rc.y & 0x0FFFFFFFu) : rc.y;
+                n.fb = DAY ? (rc.y >> 28) : 0u;
                 n.dev = __uint_as_float(rc.z);
                 n.r = __uint_as_float(rc.w);
                 n.has_bud = n.has_but = false;
                 n.bud = n.but = 0.0f;
                 load_str<VEC, TEAM>(q_base + static_cast<size_t>(n.i) * fs, n.q);
                 if constexpr (QF) {
-                    load_str<VEC, TEAM>(m.ifeat_freq + static_cast<size_t>(n.i) * F * fs + off, n.qf);
+                    const size_t qfr = static_cast<size_t>(n.i) * F + (DAY ? n.fb : 0u);
+                    load_str<VEC, TEAM>(m.ifeat_freq + qfr * fs + off, n.qf);
                 } else {
 #pragma unroll
                     for (int c = 0; c < VEC; ++c) n.qf[c] = 0.0f;
                 }
+                if constexpr (DAY) {
+                    load_str<VEC, TEAM>(m.ufeat_day + static_cast<size_t>(n.ud) * fs + off, n.pd);
+                    load_str<VEC, TEAM>(sv.day_acc + static_cast<size_t>(n.ud) * fs + off, n.sd);
+                }
                 n.sc_addr = sc_base + static_cast<size_t>(n.i) * sc_imul + (static_cast<uint32_t>(n.b) & sc_bm) +
-                            (n.ud & sc_um);
+                            (n.ud & sc_um) + (n.fb & sc_fm);
                 n.sc = *n.sc_addr;
                 {
                     // always one load: with both terms skipped it re-reads the sc
@@ -672,6 +749,31 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 su_b = m.skip_yb ? 0.0f : team_sum_dpp<TEAM>(sb) * inv;
+                if constexpr (DAY) {
+                    // every training day's sum over y_day of its training items and
+                    // (full_su) held-out items, in one stream of the user's days:
+                    // [rows of day d][header d] ...  (src/tx.rs:846-871)
+                    const uint32_t g0 = m.dg_off[u];
+                    float dacc[VEC];
+#pragma unroll
+                    for (int c = 0; c < VEC; ++c) dacc[c] = 0.0f;
+                    for_rows_at<TEAM, VEC, row_group<TEAM, VEC>()>(
+                        m.dg_off[u + 1] - g0, lane, m.dg_item + g0,
+                        [&](uint32_t j) {
+                            return (j & kDayHead) ? sv.day_acc + static_cast<size_t>(j & ~kDayHead) * fs
+                                                  : m.yfeat_day + static_cast<size_t>(j) * fs;
+                        },
+                        [&](uint32_t j, const float (&y)[VEC]) {
+                            if (j & kDayHead) {
+                                store_vec<VEC>(sv.day_acc + static_cast<size_t>(j & ~kDayHead) * fs + lane * VEC, dacc);
+#pragma unroll
+                                for (int c = 0; c < VEC; ++c) dacc[c] = 0.0f;
+                            } else {
+#pragma unroll
+                                for (int c = 0; c < VEC; ++c) dacc[c] += y[c];
+                            }
+                        });
+                }
                 if (S > 1) {
                     store_str<VEC, TEAM>(m.ycache + prow, s);
                     if (lane == 0) m.ycache_bias[u] = su_b;
@@ -725,14 +827,23 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
             // load lands in the slot consumed one step earlier, whose value is
             // dead: no register copy (and no wait for the load) at the back-edge.
             constexpr int RING = DEPTH + 1;
-            Pending<VEC> ring[RING];
+            Pending<VEC, DAY> ring[RING];
+            // day terms of the current same-day run (DAY): p_u,day, the day's
+            // NSVD1 sum and the run's gradient sum, and the run's day
+            float pud[VEC], sdr[VEC], Gd[VEC];
+            uint32_t run_ud = 0xffffffffu;
+#pragma unroll
+            for (int c = 0; c < VEC; ++c) pud[c] = sdr[c] = Gd[c] = 0.0f;
 #pragma unroll
             for (int k = 0; k < RING; ++k) {  // slots not yet fetched match no rating
                 ring[k].b = -1;
                 ring[k].ud = 0xffffffffu;
             }
 #pragma unroll
-            for (int k = 0; k < DEPTH; ++k) fetch(rec_at(s0 + k < s1 ? s0 + k : s1 - 1), ring[k]);
+            for (int k = 0; k < DEPTH; ++k) {
+                fetch(rec_at(s0 + k < s1 ? s0 + k : s1 - 1), ring[k]);
+                if constexpr (DAY) ring[k].nd = m.t_run_norm[s0 + k < s1 ? s0 + k : s1 - 1];
+            }
 #ifdef TX_PROLOGUE_PAD
             {
                 // re-read the q row of the first rating TX_PROLOGUE_PAD times (L2-hot):
@@ -751,8 +862,9 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
                 for (int k = 0; k < RING; ++k) {
                 const uint32_t t = t0 + k;
                 if (t >= s1) break;
-                const Pending<VEC>& cur = ring[k];
+                const Pending<VEC, DAY>& cur = ring[k];
                 fetch(rec_at(t + DEPTH < s1 ? t + DEPTH : s1 - 1), ring[(k + DEPTH) % RING]);
+                if constexpr (DAY) ring[(k + DEPTH) % RING].nd = m.t_run_norm[t + DEPTH < s1 ? t + DEPTH : s1 - 1];
                 const uint32_t i = cur.i;
                 const float dev = cur.dev;
                 const float c_ib = team_bcast_f<TEAM>(cur.sc, 0), c_bt = team_bcast_f<TEAM>(cur.sc, 1);
@@ -763,10 +875,34 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
                 const float bi_t = c_ib + c_bt + c_ibf;
                 const float cu_t = cu + c_cut;
                 const float bu_t = ubias + c_but + alpha * dev + c_bud + su_b;
+                if constexpr (DAY) {
+                    // a new run takes the prefetched day rows (they arrived with
+                    // this rating's q row): p_u,day, and the day's y_day sum divided
+                    // by the run norm (src/tx.rs:859-866)
+                    if (cur.ud != run_ud) {
+#pragma unroll
+                        for (int c = 0; c < VEC; ++c) {
+                            pud[c] = cur.pd[c];
+                            sdr[c] = cur.sd[c] / cur.nd;
+                            Gd[c] = 0.0f;
+                        }
+                    }
+                    run_ud = cur.ud;
+                }
 
                 float part = 0.0f;
+                if constexpr (DAY) {
 #pragma unroll
-                for (int c = 0; c < VEC; ++c) part += (cur.q[c] + cur.qf[c]) * (p[c] + s[c] + dev * p2[c]);
+                    for (int c = 0; c < VEC; ++c) {
+                        float pu = p[c] + s[c] + dev * p2[c];
+                        pu += pud[c];
+                        pu += sdr[c];
+                        part += (cur.q[c] + cur.qf[c]) * pu;
+                    }
+                } else {
+#pragma unroll
+                    for (int c = 0; c < VEC; ++c) part += (cur.q[c] + cur.qf[c]) * (p[c] + s[c] + dev * p2[c]);
+                }
                 const float dot = team_sum_dpp<TEAM>(part);
                 const float score = m.gbias + bu_t + bi_t * cu_t + dot;
                 float err;
@@ -788,7 +924,7 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
                 const float bud_new = c_bud - h.lr_ubd * err_b;
 #pragma unroll
                 for (int jj = 1; jj <= DEPTH; ++jj) {
-                    Pending<VEC>& nx = ring[(k + jj) % RING];  // ratings t+1 .. t+DEPTH
+                    Pending<VEC, DAY>& nx = ring[(k + jj) % RING];  // ratings t+1 .. t+DEPTH
                     if (nx.b == cur.b) { nx.has_but = true; nx.but = but_new; }
                     if (nx.ud == cur.ud) { nx.has_bud = true; nx.bud = bud_new; }
                 }
@@ -803,7 +939,13 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
                 for (int c = 0; c < VEC; ++c) {
                     const float qe = cur.q[c] + cur.qf[c];
                     G[c] += err_d * qe;
-                    const float pe = p[c] + dev * p2[c] + s[c];
+                    const float pe = DAY ? p[c] + dev * p2[c] + (has_ud ? pud[c] : 0.0f) + s[c] + sdr[c]
+                                         : p[c] + dev * p2[c] + s[c];
+                    if constexpr (DAY) {
+                        Gd[c] += err_d * qe;
+                        const float pd_new = pud[c] - h.lr_ud * (err_d * qe + h.reg_ud * pud[c]);
+                        pud[c] = has_ud ? pd_new : pud[c];
+                    }
                     p[c] -= h.lr_u * (err_d * qe + h.reg_u * p[c]);
                     if constexpr (P2) p2[c] -= h.lr_u2 * (err_d * qe * dev + h.reg_u2 * p2[c]);
                     dq[c] = -h.lr_i * (err_d * pe + h.reg_i * cur.q[c]);
@@ -811,7 +953,21 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
                 }
                 if constexpr (PASS == 0) {
                     apply_vec<VEC, TEAM>(q_base + static_cast<size_t>(i) * fs, cur.q, dq, qm);
-                    if constexpr (QF) apply_vec<VEC, TEAM>(m.ifeat_freq + static_cast<size_t>(i) * F * fs + off, cur.qf, dqf, qm);
+                    if constexpr (QF) {
+                        const size_t qfr = static_cast<size_t>(i) * F + (DAY ? cur.fb : 0u);
+                        apply_vec<VEC, TEAM>(m.ifeat_freq + qfr * fs + off, cur.qf, dqf, qm);
+                    }
+                }
+                if constexpr (DAY) {
+                    // end of the run: p_u,day and the run's gradient sum to memory
+                    // (the next rating's record is already decoded in the ring)
+                    if (t + 1 >= s1 || ring[(k + 1) % RING].ud != cur.ud) {
+                        float gd[VEC];
+#pragma unroll
+                        for (int c = 0; c < VEC; ++c) gd[c] = Gd[c] / cur.nd;
+                        store_str<VEC, TEAM>(m.ufeat_day + static_cast<size_t>(cur.ud) * fs + off, pud);
+                        store_str<VEC, TEAM>(sv.day_acc + static_cast<size_t>(cur.ud) * fs + off, gd);
+                    }
                 }
                 }
             }
@@ -921,6 +1077,41 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
             } else {
                 store_str<VEC, TEAM>(sv.gacc + prow, G);
                 if (lane == 0) sv.gacc_b[u] = Gb;
+            }
+            if constexpr (DAY) {
+                // y_day step of every run (src/tx.rs:1025-1036) in one stream of
+                // the user's days: [header d][training items of d] ...; the header
+                // row is the run's gradient sum already divided by its norm, and
+                // y_j -= lr_yd (g + reg_yd y_j), rows written like the NSVD1 rows
+                // (plain or atomic per the y update mode)
+                const uint32_t y0 = m.dy_off[u];
+                float g[VEC];
+#pragma unroll
+                for (int c = 0; c < VEC; ++c) g[c] = 0.0f;
+                for_rows_at<TEAM, VEC, row_group<TEAM, VEC>()>(
+                    m.dy_off[u + 1] - y0, lane, m.dy_item + y0,
+                    [&](uint32_t j) {
+                        return (j & kDayHead) ? sv.day_acc + static_cast<size_t>(j & ~kDayHead) * fs
+                                              : m.yfeat_day + static_cast<size_t>(j) * fs;
+                    },
+                    [&](uint32_t j, const float (&y)[VEC]) {
+                        if (j & kDayHead) {
+#pragma unroll
+                            for (int c = 0; c < VEC; ++c) g[c] = y[c];
+                        } else {
+                            float* row = m.yfeat_day + static_cast<size_t>(j) * fs + lane * VEC;
+                            if (ym == 2) {
+                                float ny[VEC];
+#pragma unroll
+                                for (int c = 0; c < VEC; ++c) ny[c] = y[c] - h.lr_yd * (g[c] + h.reg_yd * y[c]);
+                                store_vec<VEC>(row, ny);
+                            } else {
+#pragma unroll
+                                for (int c = 0; c < VEC; ++c)
+                                    apply_delta(row + c, y[c], -h.lr_yd * (g[c] + h.reg_yd * y[c]), ym);
+                            }
+                        }
+                    });
             }
         }
     }
@@ -1295,17 +1486,27 @@ static inline int grid_for(int64_t teams, int team) {
     return static_cast<int>(blocks < 1 ? 1 : blocks);
 }
 
-// Dispatch on (TEAM, VEC): the shapes the host picks for a padded row stride.
+// Dispatch on (TEAM, VEC): the shapes the host picks for a padded row stride
+// (one wave per user: four 16-lane teams in a wave would run their users'
+// loops to the longest of the four).  The per-day rebuild sums a few rows per
+// day and takes four elements per lane instead (TX_DAY_SHAPES).
 // (A build may predefine TX_SHAPES to a subset, e.g. to inspect one kernel.)
 #ifndef TX_SHAPES
 #define TX_SHAPES(X)  \
-    X(4, 4)           \
-    X(8, 4)           \
-    X(16, 4)          \
-    X(32, 4)          \
+    X(64, 1)          \
+    X(64, 2)          \
     X(64, 4)          \
     X(64, 8)          \
     X(64, 16)
+#define TX_DAY_SHAPES(X) \
+    X(16, 4)             \
+    X(32, 4)             \
+    X(64, 4)             \
+    X(64, 8)             \
+    X(64, 16)
+#endif
+#ifndef TX_DAY_SHAPES
+#define TX_DAY_SHAPES(X) TX_SHAPES(X)
 #endif
 
 hipError_t launch_sgd_serial(const DevModel& m, const int64_t* order, float* scratch, hipStream_t st) {
@@ -1315,12 +1516,12 @@ hipError_t launch_sgd_serial(const DevModel& m, const int64_t* order, float* scr
     return hipGetLastError();
 }
 
-template <int TEAM, int VEC, int PASS>
+template <int TEAM, int VEC, int PASS, bool DAY>
 static hipError_t sgd_par_tvp(const DevModel& m, const StrataView& sv, int stratum, int blocks, bool qf, bool p2,
                               bool ord, hipStream_t st) {
     (void)hipGetLastError();
 #define TX_PAR(QF, P2, ORD) \
-    hipLaunchKernelGGL((k_sgd_par<TEAM, VEC, QF, P2, ORD, PASS>), dim3(blocks), dim3(256), 0, st, m, sv, stratum)
+    hipLaunchKernelGGL((k_sgd_par<TEAM, VEC, QF, P2, ORD, PASS, DAY>), dim3(blocks), dim3(256), 0, st, m, sv, stratum)
     if constexpr (PASS == 0) {
         if (ord) {
             if (qf) { if (p2) TX_PAR(true, true, true); else TX_PAR(true, false, true); }
@@ -1338,17 +1539,21 @@ static hipError_t sgd_par_tvp(const DevModel& m, const StrataView& sv, int strat
 
 template <int TEAM, int VEC>
 static hipError_t sgd_par_tv(const DevModel& m, const StrataView& sv, int stratum, int blocks, bool qf, bool p2,
-                             bool ord, int pass, hipStream_t st) {
-    if (pass == 0) return sgd_par_tvp<TEAM, VEC, 0>(m, sv, stratum, blocks, qf, p2, ord, st);
-    if (pass == 1) return sgd_par_tvp<TEAM, VEC, 1>(m, sv, stratum, blocks, qf, p2, ord, st);
-    if (pass == 2) return sgd_par_tvp<TEAM, VEC, 2>(m, sv, stratum, blocks, qf, p2, ord, st);
+                             bool ord, int pass, bool day, hipStream_t st) {
+    if (day) {
+        if (pass != 0 || sv.n_strata != 1 || !sv.day_acc) return hipErrorInvalidValue;
+        return sgd_par_tvp<TEAM, VEC, 0, true>(m, sv, stratum, blocks, qf, p2, ord, st);
+    }
+    if (pass == 0) return sgd_par_tvp<TEAM, VEC, 0, false>(m, sv, stratum, blocks, qf, p2, ord, st);
+    if (pass == 1) return sgd_par_tvp<TEAM, VEC, 1, false>(m, sv, stratum, blocks, qf, p2, ord, st);
+    if (pass == 2) return sgd_par_tvp<TEAM, VEC, 2, false>(m, sv, stratum, blocks, qf, p2, ord, st);
     return hipErrorInvalidValue;
 }
 
 hipError_t launch_sgd_par(const DevModel& m, const StrataView& sv, int stratum, int team, int vec, int blocks,
-                          bool qf, bool p2, bool ord, int pass, hipStream_t st) {
+                          bool qf, bool p2, bool ord, int pass, bool day, hipStream_t st) {
 #define X(T, V) \
-    if (team == T && vec == V) return sgd_par_tv<T, V>(m, sv, stratum, blocks, qf, p2, ord, pass, st);
+    if (team == T && vec == V) return sgd_par_tv<T, V>(m, sv, stratum, blocks, qf, p2, ord, pass, day, st);
     TX_SHAPES(X)
 #undef X
     return hipErrorInvalidValue;
@@ -1380,16 +1585,27 @@ hipError_t launch_item_pass(const DevModel& m, const ItemView& iv, int team, int
 }
 
 hipError_t launch_rebuild(const DevModel& m, int64_t n_ud, int team, int vec, hipStream_t st) {
-#define X(T, V)                                                                                           \
-    if (team == T && vec == V) {                                                                           \
-        (void)hipGetLastError();                                                                           \
-        hipLaunchKernelGGL((k_rebuild_ycache<T, V>), dim3(grid_for(m.U, T)), dim3(256), 0, st, m);         \
-        if (!m.low_mem && n_ud > 0)                                                                        \
-            hipLaunchKernelGGL((k_rebuild_ycache_day<T, V>), dim3(grid_for(n_ud, T)), dim3(256), 0, st, m, \
-                               n_ud);                                                                      \
-        return hipGetLastError();                                                                          \
+    bool done = false;
+#define X(T, V)                                                                                   \
+    if (!done && team == T && vec == V) {                                                          \
+        (void)hipGetLastError();                                                                   \
+        hipLaunchKernelGGL((k_rebuild_ycache<T, V>), dim3(grid_for(m.U, T)), dim3(256), 0, st, m); \
+        if (hipError_t e = hipGetLastError()) return e;                                            \
+        done = true;                                                                               \
     }
     TX_SHAPES(X)
+#undef X
+    if (!done) return hipErrorInvalidValue;
+    if (m.low_mem || n_ud <= 0) return hipSuccess;
+    // per-day sums: four elements per lane, up to 64 lanes
+    const int dt = m.fs / 4 <= 64 ? m.fs / 4 : 64, dv = m.fs / dt;
+#define X(T, V)                                                                                                 \
+    if (dt == T && dv == V) {                                                                                    \
+        (void)hipGetLastError();                                                                                 \
+        hipLaunchKernelGGL((k_rebuild_ycache_day<T, V>), dim3(grid_for(n_ud, T)), dim3(256), 0, st, m, n_ud);    \
+        return hipGetLastError();                                                                                \
+    }
+    TX_DAY_SHAPES(X)
 #undef X
     return hipErrorInvalidValue;
 }

@@ -10,9 +10,10 @@
 namespace tx {
 
 hipError_t launch_sgd_serial(const DevModel& m, const int64_t* order, float* scratch, hipStream_t st);
-// pass: 0 SGD epoch, 1 tsvdx4p user pass (NSVD1 accumulators), 2 user-only fold-in pass
+// pass: 0 SGD epoch, 1 tsvdx4p user pass (NSVD1 accumulators), 2 user-only fold-in pass;
+// day: day terms (pass 0, one stratum, sv.day_acc allocated)
 hipError_t launch_sgd_par(const DevModel& m, const StrataView& sv, int stratum, int team, int vec, int blocks,
-                          bool qf, bool p2, bool ord, int pass, hipStream_t st);
+                          bool qf, bool p2, bool ord, int pass, bool day, hipStream_t st);
 hipError_t launch_to_bf16(const float* x, uint16_t* y, int64_t n, hipStream_t st);
 hipError_t launch_apply_nsvd(const DevModel& m, float* d_y, float* d_yb, const float* scale, hipStream_t st);
 hipError_t launch_item_pass(const DevModel& m, const ItemView& iv, int team, int vec, bool p2, int blocks,

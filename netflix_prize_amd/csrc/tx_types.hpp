@@ -58,6 +58,16 @@ struct DevModel {
     // training items grouped by (user, day), training file order
     const uint32_t* td_start; // [D+1]
     const uint32_t* td_item;
+    // per-user day streams of the parallel day-term kernel (date-ordered
+    // ratings only); entries with bit 31 set are day headers (SparseUD index):
+    //   dg: for each training day d: its training items, (full_su) its
+    //       held-out items, then header d  -- the day sums
+    //   dy: for each training day d: header d, then its training items  -- the
+    //       y_day steps
+    const uint32_t* dg_off;   // [U+1]
+    const uint32_t* dg_item;
+    const uint32_t* dy_off;   // [U+1]
+    const uint32_t* dy_item;
 
     const float* norm_train;    // [U] sqrt(|R(u)| + |P(u)| if full_su)
     const float* norm_rebuild;  // [U] sqrt(train_cnt + heldout_cnt)
@@ -74,7 +84,8 @@ struct StrataView {
     int32_t n_cells;            // user blocks = item blocks = n_strata (1 => one cell)
     const uint32_t* seg_off;    // [U * (n_strata + 1)] offsets into the permuted records
     // packed training records in segment order:
-    //   x = item | (raw & 15) << 20 | bin << 24, y = SparseUD index, z = dev, w = target
+    //   x = item | (raw & 15) << 20 | bin << 24, y = SparseUD index | freq bin << 28,
+    //   z = dev, w = target (freq bin 0 for low-memory training)
     const uint4* p_rec;
     const uint32_t* q_users;    // per (stratum, cell) queues of users, longest segment first
     const uint32_t* q_start;    // [n_strata * n_cells + 1]
@@ -85,6 +96,7 @@ struct StrataView {
     int32_t y_mode;             // NSVD1-row update, same codes
     float* d_y;                 // [I * fs] NSVD1 gradient accumulators of the user pass (tsvdx4p)
     float* d_yb;                // [I]
+    float* day_acc;             // [D * fs] day terms: a day's y_day sum (user start), then its gradient sum
 };
 
 // Item pass of the tsvdx4p schedule: ratings grouped by item.
