@@ -241,3 +241,26 @@ def test_day_terms_schedule_rules(gpu):
         m.fit_epoch(epoch=1)
         m_o.fit_epoch(1)
         _assert_params_equal(m_o, m)    # the default ran the serial schedule: bit-identical
+
+
+def test_full_shape_day_model_matches_serial_reference(gpu):
+    """The tsvdx4-60 job's model (lm set with day terms, low_memory = false,
+    f=60, 29 time bins, seed 42; src/bin/tsvdx4-new.rs:18) for 20 epochs in
+    the default parallel schedule at the full Netflix shape, against the serial
+    oracle's holdout RMSE (tests/golden/oracle_curve_full_job60.json): within
+    1e-3 after the last epoch."""
+    import json
+    import os
+    golden = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "oracle_curve_full_job60.json")))
+    assert len(golden["epochs"]) == 20
+    tr, pr = _full_data(golden["seed"])
+    cfg = tx.TxConfig(n_feat=60, n_epochs=20, seed=42, n_time_bins=29, low_memory=False)
+    with tx.TxModel(tr, pr, cfg) as m:
+        curve = []
+        for e in range(1, 21):
+            m.fit_epoch(epoch=e)
+            curve.append(m.calc_rmse(pr))
+    ref = [x["holdout_rmse"] for x in golden["epochs"]]
+    print("gpu", [round(x, 5) for x in curve])
+    print("ref", [round(x, 5) for x in ref])
+    assert abs(curve[-1] - ref[-1]) <= 1e-3
