@@ -38,11 +38,20 @@ def algorithmic_bytes_per_rating(f: int, n_train: int, n_users: int, qf: bool, p
     return {"sgd": sgd, "rebuild_per_row": s * f, "ycache_per_user": s * f}
 
 
-def profiled_traffic(f: int, args) -> float | None:
+def day_bytes_per_rating(f: int, n_train: int, n_heldout: int, n_days: int, s: int = 4) -> float:
+    """Algorithmic HBM bytes per training rating that the day terms add to the
+    SGD kernel: y_day gather of every training and (full_su) held-out row,
+    y_day read and write per training row, and per training day p_day read and
+    write, the day sum written and read back, the run gradient written and
+    read back; plus the run norm read per rating."""
+    return 3 * s * f + s * f * n_heldout / n_train + 6 * s * f * n_days / n_train + s
+
+
+def profiled_traffic(f: int, args, kind: str = "") -> float | None:
     """HBM bytes per SGD launch (FETCH_SIZE + WRITE_SIZE, calibrated against a
     plain copy) from the rocprofv3 summary of this workload in profiles/, or
     None for a workload that has no such summary."""
-    path = os.path.join(ROOT, "profiles", f"k_sgd_par_{args.shape}_f{f}.md")
+    path = os.path.join(ROOT, "profiles", f"k_sgd_par_{kind}{args.shape}_f{f}.md")
     if not os.path.exists(path) or args.strata != 1:
         return None
     text = open(path).read()
@@ -243,7 +252,7 @@ def main(argv=None) -> int:
         dcfg = tx.TxConfig(n_feat=60, n_epochs=20, seed=42, n_time_bins=29, low_memory=False)
         t0 = time.time()
         dm = tx.TxModel(tr, pr, dcfg)
-        dm.set_schedule(mode=tx.SCHED_STRATA, use_graph=bool(args.graph))
+        dm.set_schedule(mode=tx.SCHED_STRATA)   # stream launches: ms_sgd is the SGD launch alone
         log(f"bench: day-term model created ({time.time() - t0:.1f}s)")
         dst, dcurve = [], []
         for e in range(1, args.day_epochs + 1):
@@ -259,10 +268,22 @@ def main(argv=None) -> int:
             if g.get("seed") == args.seed and len(g["epochs"]) >= args.day_epochs:
                 dref = [x["holdout_rmse"] for x in g["epochs"][:args.day_epochs]]
         ms = statistics.median(x["ms_total"] for x in dst[1:] or dst)
+        sgd = statistics.median(x["ms_sgd"] for x in dst[1:] or dst)
+        # training days: the rows are sorted by (user, date)
+        u, d = tr.user_idxs, tr.dates
+        n_days = int(np.count_nonzero((u[1:] != u[:-1]) | (d[1:] != d[:-1]))) + 1
+        qf = dcfg.lr_iqf != 0.0 or dcfg.sigma_iqf != 0.0
+        bpr = (algorithmic_bytes_per_rating(60, n_train, tr.n_users, qf, dcfg.lr_u2 != 0.0)["sgd"]
+               + day_bytes_per_rating(60, n_train, pr.n_ratings, n_days))
+        ach = bpr * n_train / (sgd / 1000.0) / 1e9
         day = {"workload": "tsvdx4-60 model: lm set, day terms, f=60, 29 time bins, 1 stratum",
                "ms_per_epoch": round(ms, 2), "ratings_per_s": round(n_train / (ms / 1000.0), 1),
                "holdout_rmse": [round(r, 5) for r in dcurve],
-               "serial_reference_rmse": None if dref is None else [round(r, 5) for r in dref]}
+               "serial_reference_rmse": None if dref is None else [round(r, 5) for r in dref],
+               "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(60, args, "day_"),
+                            "kernel": "k_sgd_par (day terms)", "bytes_per_rating": round(bpr, 1),
+                            "avg_launch_ms": round(sgd, 3), "training_days": n_days}}
 
     # roofline of the dominant kernel (SGD), from the library's events on its stream
     qf = cfg.lr_iqf != 0.0 or cfg.sigma_iqf != 0.0

@@ -93,7 +93,7 @@ enum tx_sched_mode { TX_SCHED_SERIAL = 0, TX_SCHED_STRATA = 1, TX_SCHED_TSVDX4P 
 typedef struct tx_schedule {
     int32_t mode;
     int32_t n_strata;        /* STRATA only: 1 or a multiple of 8 */
-    int32_t teams_per_block; /* STRATA only: 0 = tuned default */
+    int32_t teams_per_block; /* reserved (one 64-lane team per user at every f) */
     int32_t blocks;          /* STRATA only: persistent grid size, 0 = auto */
     int32_t use_graph;       /* capture each epoch's launches in a hipGraph */
     int32_t update_flags;    /* STRATA only: how shared rows are updated, see below */
@@ -167,7 +167,10 @@ int tx_destroy(tx_ctx* ctx);
 
 /* Choose the training schedule (default: TX_SCHED_STRATA, n_strata = 1,
  * blocks = 0, update_flags = TX_UPD_ATOMIC | TX_UPD_PLAIN << 2, user_order
- * = 2, heavy_min = 1024). */
+ * = 2, heavy_min = 1024).  Day terms (low_memory = 0) train in TX_SCHED_STRATA
+ * with n_strata = 1 when every user's training ratings are in date order;
+ * otherwise the default is TX_SCHED_SERIAL and TX_SCHED_STRATA returns
+ * TX_EUNSUPPORTED. */
 int tx_set_schedule(tx_ctx* ctx, const tx_schedule* sched);
 
 /* Autotuner of the one-stratum parallel schedule (no reference counterpart):
