@@ -87,7 +87,9 @@ typedef struct tx_ratings {
  *     NSVD1 gradients accumulated per item, the delta apply divided by
  *     n_contrib^nsvd_norm_exp, the ycache rebuild, then an item pass with user
  *     tables frozen; epochs listed in cfg.seq_epochs run the TX_SCHED_STRATA
- *     epoch instead.  Low-memory models without q_f stream or ordinal head. */
+ *     epoch instead.  Low-memory models without ordinal head; the q_f stream
+ *     reads bin 0 in the user pass and the day's real freq bin in the item
+ *     pass, as the reference does. */
 enum tx_sched_mode { TX_SCHED_SERIAL = 0, TX_SCHED_STRATA = 1, TX_SCHED_TSVDX4P = 2 };
 
 typedef struct tx_schedule {

@@ -555,8 +555,8 @@ int user_pass_blocks(tx_ctx* c) {
 int tsvdx4p_epoch(tx_ctx* c, int32_t epoch, tx_epoch_stats& stats, hipStream_t st) {
     const bool qf = c->cfg.lr_iqf != 0.0f || c->cfg.sigma_iqf != 0.0f;
     const bool p2 = c->cfg.lr_u2 != 0.0f;
-    if (!c->cfg.low_memory || qf || c->cfg.has_ordinal)
-        return fail(TX_EUNSUPPORTED, "tsvdx4p schedule: low-memory form without the q_f stream or ordinal head only");
+    if (!c->cfg.low_memory || c->cfg.has_ordinal)
+        return fail(TX_EUNSUPPORTED, "tsvdx4p schedule: low-memory form without ordinal head only");
     if (c->built_S != 1) {
         if (c->built_S != 0) return fail(TX_EUNSUPPORTED, "tsvdx4p schedule: needs the one-stratum layout");
         if (int rc = build_strata(c, 1)) return rc;
@@ -567,12 +567,12 @@ int tsvdx4p_epoch(tx_ctx* c, int32_t epoch, tx_epoch_stats& stats, hipStream_t s
     HIPC(tx::launch_fill_u32(c->sv.q_head, 0u, c->sv.n_cells, st));
     HIPC(tx::launch_fill_u32(c->iv.q_head, 0u, 1, st));
     HIPC(hipEventRecord(c->ev[1], st));
-    HIPC(tx::launch_sgd_par(c->dm, c->sv, 0, c->shape.team, c->shape.vec, sgpr_blocks, false, p2, false, 1, false, st));
+    HIPC(tx::launch_sgd_par(c->dm, c->sv, 0, c->shape.team, c->shape.vec, sgpr_blocks, qf, p2, false, 1, false, st));
     HIPC(tx::launch_apply_nsvd(c->dm, c->sv.d_y, c->sv.d_yb, c->d_scale, st));
     HIPC(hipEventRecord(c->ev[2], st));
     HIPC(tx::launch_rebuild(c->dm, c->n_ud, c->shape.team, c->shape.vec, st));
     HIPC(hipEventRecord(c->ev[3], st));
-    HIPC(tx::launch_item_pass(c->dm, c->iv, c->shape.team, c->shape.vec, p2, user_pass_blocks(c), st));
+    HIPC(tx::launch_item_pass(c->dm, c->iv, c->shape.team, c->shape.vec, p2, qf, user_pass_blocks(c), st));
     HIPC(hipEventSynchronize(c->ev[3]));
     HIPC(hipStreamSynchronize(st));
     float a = 0, b = 0;
@@ -1008,9 +1008,8 @@ int tx_set_schedule(tx_ctx* c, const tx_schedule* s) {
     if (s->mode != TX_SCHED_SERIAL && s->mode != TX_SCHED_STRATA && s->mode != TX_SCHED_TSVDX4P)
         return fail(TX_EINVAL, "unknown schedule mode %d", s->mode);
     if (s->mode == TX_SCHED_TSVDX4P) {
-        if (!c->cfg.low_memory || c->cfg.has_ordinal || c->cfg.lr_iqf != 0.0f || c->cfg.sigma_iqf != 0.0f)
-            return fail(TX_EUNSUPPORTED,
-                        "the tsvdx4p schedule trains low_memory models without the q_f stream or ordinal head");
+        if (!c->cfg.low_memory || c->cfg.has_ordinal)
+            return fail(TX_EUNSUPPORTED, "the tsvdx4p schedule trains low_memory models without ordinal head");
         if (s->blocks < 0) return fail(TX_EINVAL, "blocks must be 0 (auto) or positive");
     }
     if (s->mode == TX_SCHED_STRATA) {
@@ -1147,6 +1146,9 @@ int tx_fit_epoch(tx_ctx* c, int32_t epoch) {
             stats.ms_other = t - a - b;
             stats.n_launches = S + 2;
         }
+        // OrdinalHead::enforce_sorted_with_gap at epoch end (src/tx.rs:1113):
+        // the parallel schedule keeps lr_t = 0, so this only acts on th_init
+        if (int rc = enforce_thresholds(c)) return rc;
     }
     c->last = stats;
     return 0;

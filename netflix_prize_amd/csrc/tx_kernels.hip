@@ -1148,20 +1148,27 @@ __global__ __launch_bounds__(256) void k_apply_nsvd(DevModel m, float* d_y, floa
 // One rating of the item pass and its frozen user-side inputs.
 //   sc:  lane 0 b_u, 1 b_u,bin, 2 alpha_u, 3 b_u,day
 //   sc2: lane 0 ycache_bias, 1 c_u, 2 c_u,day
-template <int VEC, bool P2>
+template <int VEC, bool P2, bool QF>
 struct ItemPending {
     uint32_t b, fb;
     float dev, r;
     float pe[VEC];  // p + ycache + dev * p2, contiguous layout
     float sc, sc2;
+    // q_f row of the rating's freq bin as loaded, and the row written by an
+    // earlier rating of the item after that load (has_qf), forwarded here so
+    // the loaded registers are never written while their load is in flight
+    float qf[QF ? VEC : 1], qfo[QF ? VEC : 1];
+    bool has_qf;
 };
 
 // fit_epoch_item (src/tsvdx4p.rs:644-762): one team per item (longest first;
 // items are independent), the item's ratings in training-file order with every
 // user-side value frozen.  q_i stays in registers and the item's bias slots
 // (b_i, b_{i,bin}[B], b_{i,f}[F]) in the team's LDS for the whole item; both
-// are written back once.  Low-memory form without the q_f stream.
-template <int TEAM, int VEC, bool P2>
+// are written back once.  Low-memory form; with QF the q_f rows of the day's
+// freq bin (the real bin here, bin 0 in the user pass: src/tsvdx4p.rs:497,
+// 708) are read and written per rating.
+template <int TEAM, int VEC, bool P2, bool QF>
 __global__ __launch_bounds__(256) void k_item_pass(DevModel m, ItemView iv) {
     constexpr int DEPTH = TX_PREFETCH;
     constexpr int RING = DEPTH + 1;
@@ -1199,11 +1206,15 @@ __global__ __launch_bounds__(256) void k_item_pass(DevModel m, ItemView iv) {
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
-        auto fetch = [&](uint32_t t, ItemPending<VEC, P2>& n) {
+        auto fetch = [&](uint32_t t, ItemPending<VEC, P2, QF>& n) {
             const uint4 rc = iv.bi_rec[t];  // same address in every lane of the team
             const uint32_t u = rc.x & 0xFFFFFu;
             n.b = (rc.x >> 20) & 0xFFu;
             n.fb = rc.x >> 28;
+            if constexpr (QF) {
+                load_vec<VEC>(m.ifeat_freq + (static_cast<size_t>(i) * F + n.fb) * fs + coff, n.qf);
+                n.has_qf = false;
+            }
             n.dev = __uint_as_float(rc.z);
             n.r = __uint_as_float(rc.w);
             float p[VEC], s[VEC], p2[VEC];
@@ -1216,7 +1227,7 @@ __global__ __launch_bounds__(256) void k_item_pass(DevModel m, ItemView iv) {
             n.sc2 = sc2_base[static_cast<size_t>(u) * sc2_umul + (rc.y & sc2_um)];
         };
         if (r0 < r1) {
-            ItemPending<VEC, P2> ring[RING];
+            ItemPending<VEC, P2, QF> ring[RING];
 #pragma unroll
             for (int k = 0; k < DEPTH; ++k) fetch(r0 + k < r1 ? r0 + k : r1 - 1, ring[k]);
             for (uint32_t t0 = r0; t0 < r1; t0 += RING) {
@@ -1224,7 +1235,7 @@ __global__ __launch_bounds__(256) void k_item_pass(DevModel m, ItemView iv) {
                 for (int k = 0; k < RING; ++k) {
                     const uint32_t t = t0 + k;
                     if (t >= r1) break;
-                    const ItemPending<VEC, P2>& cur = ring[k];
+                    const ItemPending<VEC, P2, QF>& cur = ring[k];
                     fetch(t + DEPTH < r1 ? t + DEPTH : r1 - 1, ring[(k + DEPTH) % RING]);
                     const float c_ub = team_bcast_f<TEAM>(cur.sc, 0), c_but = team_bcast_f<TEAM>(cur.sc, 1);
                     const float c_al = team_bcast_f<TEAM>(cur.sc, 2), c_bud = team_bcast_f<TEAM>(cur.sc, 3);
@@ -1234,9 +1245,17 @@ __global__ __launch_bounds__(256) void k_item_pass(DevModel m, ItemView iv) {
                     const float bu_t = c_ub + c_but + c_al * cur.dev + c_bud + c_ycb;
                     const float bi_t = ib + c_bt + c_bq;
                     const float cu_t = c_cu + c_cut;
+                    float qf[VEC];  // the q_f row (zero without QF)
+                    if constexpr (QF) {
+#pragma unroll
+                        for (int c = 0; c < VEC; ++c) qf[c] = cur.has_qf ? cur.qfo[c] : cur.qf[c];
+                    } else {
+#pragma unroll
+                        for (int c = 0; c < VEC; ++c) qf[c] = 0.0f;
+                    }
                     float part = 0.0f;
 #pragma unroll
-                    for (int c = 0; c < VEC; ++c) part += cur.pe[c] * q[c];
+                    for (int c = 0; c < VEC; ++c) part += cur.pe[c] * (q[c] + qf[c]);
                     const float dot = team_sum_dpp<TEAM>(part);
                     const float err = m.gbias + bu_t + bi_t * cu_t + dot - cur.r;
                     const float gi = err * cu_t;
@@ -1245,6 +1264,22 @@ __global__ __launch_bounds__(256) void k_item_pass(DevModel m, ItemView iv) {
                     bq[cur.fb] = c_bq - h.lr_ibf * gi;
 #pragma unroll
                     for (int c = 0; c < VEC; ++c) q[c] -= h.lr_i * (err * cur.pe[c] + h.reg_i * q[c]);
+                    if constexpr (QF) {
+                        float nq[VEC];
+#pragma unroll
+                        for (int c = 0; c < VEC; ++c) nq[c] = qf[c] - h.lr_iqf * (err * cur.pe[c] + h.reg_iqf * qf[c]);
+                        store_vec<VEC>(m.ifeat_freq + (static_cast<size_t>(i) * F + cur.fb) * fs + coff, nq);
+                        // ratings t+1 .. t+DEPTH were loaded before this store
+#pragma unroll
+                        for (int jj = 1; jj <= DEPTH; ++jj) {
+                            ItemPending<VEC, P2, QF>& nx = ring[(k + jj) % RING];
+                            if (nx.fb == cur.fb) {
+                                nx.has_qf = true;
+#pragma unroll
+                                for (int c = 0; c < VEC; ++c) nx.qfo[c] = nq[c];
+                            }
+                        }
+                    }
                 }
             }
         }
@@ -1569,18 +1604,21 @@ hipError_t launch_apply_nsvd(const DevModel& m, float* d_y, float* d_yb, const f
     return hipGetLastError();
 }
 
-hipError_t launch_item_pass(const DevModel& m, const ItemView& iv, int team, int vec, bool p2, int blocks,
-                            hipStream_t st) {
+hipError_t launch_item_pass(const DevModel& m, const ItemView& iv, int team, int vec, bool p2, bool qf,
+                            int blocks, hipStream_t st) {
     const size_t shm = sizeof(float) * static_cast<size_t>(m.B + m.F) * (256 / team);
-#define X(T, V)                                                                                          \
-    if (team == T && vec == V) {                                                                          \
-        (void)hipGetLastError();                                                                          \
-        if (p2) hipLaunchKernelGGL((k_item_pass<T, V, true>), dim3(blocks), dim3(256), shm, st, m, iv);   \
-        else hipLaunchKernelGGL((k_item_pass<T, V, false>), dim3(blocks), dim3(256), shm, st, m, iv);     \
-        return hipGetLastError();                                                                         \
+#define TX_IP(T, V, P2, QF) \
+    hipLaunchKernelGGL((k_item_pass<T, V, P2, QF>), dim3(blocks), dim3(256), shm, st, m, iv)
+#define X(T, V)                                                                     \
+    if (team == T && vec == V) {                                                     \
+        (void)hipGetLastError();                                                     \
+        if (qf) { if (p2) TX_IP(T, V, true, true); else TX_IP(T, V, false, true); }   \
+        else { if (p2) TX_IP(T, V, true, false); else TX_IP(T, V, false, false); }    \
+        return hipGetLastError();                                                    \
     }
     TX_SHAPES(X)
 #undef X
+#undef TX_IP
     return hipErrorInvalidValue;
 }
 

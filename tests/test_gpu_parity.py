@@ -181,15 +181,17 @@ def _disjoint_users(n_users=48, per_user=40, n_train=32, seed=3):
     return out
 
 
-@pytest.mark.parametrize("f", [24, 256])
-def test_parallel_day_terms_match_serial_on_disjoint_users(gpu, f):
+@pytest.mark.parametrize("f,ordinal", [(24, False), (256, False), (60, True)])
+def test_parallel_day_terms_match_serial_on_disjoint_users(gpu, f, ordinal):
     """Day terms (low_memory = false) in the parallel schedule against the
     serial oracle where users share no item: every table within 1e-3 of its
     movement over two epochs (a dropped or misplaced day term moves a table
-    by a large part of that)."""
+    by a large part of that).  The ordinal case is the *o jobs' head with
+    fixed thresholds (src/bin/tsvdx4-new.rs:187-192)."""
     tr, pr = _disjoint_users()
+    head = tx.OrdinalHeadConfig(th_init=(0.5, 1.25, 3.75, 5.5), th_gap=0.001, lr_t=0.0, reg_t=0.0) if ordinal else None
     cfg = lm_config(n_feat=f, n_epochs=2, n_time_bins=30, low_memory=False, lr_ud=0.01, lr_yd=0.01,
-                    sigma_yd=0.05, lr_iqf=0.005, lr_ibf=0.005, reset_u_epoch=100)
+                    sigma_yd=0.05, lr_iqf=0.005, lr_ibf=0.005, reset_u_epoch=100, ordinal_head=head)
     m_o, m_g = _oracle_and_gpu(cfg, tr, pr)   # default schedule: parallel, day terms in the kernel
     init = {n: m_o.get(n).copy() for n in ALL_PARAMS}
     for e in (1, 2):

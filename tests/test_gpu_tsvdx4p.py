@@ -16,11 +16,13 @@ pytestmark = pytest.mark.gpu
 ALL_PARAMS = [p for p in tx.PARAMS if p != "thresholds"]
 
 
-def _p_config(base: dict, **kw):
-    """A Tsvdx4p config in the low-memory form the GPU schedule runs (no q_f
-    stream), every epoch parallel (seq_epochs empty)."""
+def _p_config(base: dict, qf: bool = False, **kw):
+    """A Tsvdx4p config in the low-memory form the GPU schedule runs, every
+    epoch parallel (seq_epochs empty); without the q_f stream unless qf."""
     d = dict(base)
-    d.update(low_memory=True, lr_iqf=0.0, sigma_iqf=0.0, seq_epochs=[0, 0, 0, 0, 0])
+    d.update(low_memory=True, seq_epochs=[0, 0, 0, 0, 0])
+    if not qf:
+        d.update(lr_iqf=0.0, sigma_iqf=0.0)
     d.update(kw)
     return tx.TxConfig.from_dict(d)
 
@@ -33,11 +35,13 @@ def _compare(m_o, m_g, atol):
             np.testing.assert_allclose(b, a, rtol=0, atol=atol, err_msg=n)
 
 
-def test_tsvdx4p_tiny_matches_oracle(gpu):
+@pytest.mark.parametrize("qf", [False, True])
+def test_tsvdx4p_tiny_matches_oracle(gpu, qf):
     """Three Tsvdx4pModel epochs on the reference's tiny fixture (its tsvdx4p
-    KAT configuration, low-memory form): every table within 2e-5."""
+    KAT configuration, low-memory form, with and without the q_f stream):
+    every table within 2e-5."""
     fx, tr, pr = tiny()
-    cfg = _p_config(fx["kat"]["tsvdx4p"]["config"])
+    cfg = _p_config(fx["kat"]["tsvdx4p"]["config"], qf=qf, lr_iqf=0.01 if qf else 0.0)
     m_o = O.OracleModel(cfg, tr, pr)
     m_g = tx.TxModel(tr, pr, cfg)
     m_g.set_schedule(mode=tx.SCHED_TSVDX4P)
@@ -48,12 +52,15 @@ def test_tsvdx4p_tiny_matches_oracle(gpu):
     assert m_g.calc_rmse(pr) == pytest.approx(m_o.calc_rmse(), abs=1e-6)
 
 
-def test_tsvdx4p_synthetic_matches_oracle(gpu):
-    """Tsvdx4pModel on the 10k x 2k x 1M shape (f=32, nsvd_norm_exp 0.5):
-    tables within 1e-4 and holdout RMSE within 1e-5 after 3 epochs."""
+@pytest.mark.parametrize("qf", [False, True])
+def test_tsvdx4p_synthetic_matches_oracle(gpu, qf):
+    """Tsvdx4pModel on the 10k x 2k x 1M shape (f=32, nsvd_norm_exp 0.5; the
+    lm set's q_f stream on or off): tables within 1e-4 and holdout RMSE within
+    1e-5 after 3 epochs."""
     tr, pr = synthetic("small")
+    off = {} if qf else dict(lr_iqf=0.0, sigma_iqf=0.0)
     cfg = lm_config(n_feat=32, n_epochs=3, n_time_bins=30, nsvd_norm_exp=0.5, n_threads=8,
-                    seq_epochs=[0, 0, 0, 0, 0], lr_iqf=0.0, sigma_iqf=0.0)
+                    seq_epochs=[0, 0, 0, 0, 0], **off)
     m_o = O.OracleModel(cfg, tr, pr)
     m_g = tx.TxModel(tr, pr, cfg)
     m_g.set_schedule(mode=tx.SCHED_TSVDX4P)
@@ -68,10 +75,11 @@ def test_tsvdx4p_synthetic_matches_oracle(gpu):
 
 def test_tsvdx4p_rejects_unsupported_forms(gpu):
     tr, pr = synthetic("tiny_synth")
-    cfg = lm_config(n_feat=16, n_epochs=1, n_time_bins=30)   # q_f stream on
-    m = tx.TxModel(tr, pr, cfg)
-    with pytest.raises(RuntimeError, match="q_f"):
-        m.set_schedule(mode=tx.SCHED_TSVDX4P)
+    for kw in (dict(low_memory=False), dict(ordinal_head=tx.OrdinalHeadConfig())):
+        m = tx.TxModel(tr, pr, lm_config(n_feat=16, n_epochs=1, n_time_bins=30, **kw))
+        with pytest.raises(RuntimeError, match="low_memory|ordinal"):
+            m.set_schedule(mode=tx.SCHED_TSVDX4P)
+        m.close()
 
 
 @pytest.mark.parametrize("shape", ["tiny_synth", "small"])
