@@ -87,6 +87,11 @@ def build(verbose: bool = False) -> str:
     dt = _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{INCLUDE}", f"-I{CSRC}", os.path.join(CSRC, "tsvdx4_new.cpp"),
                "-o", tmp, f"-L{PKG}", "-ltsvd_tx", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, DRIVER)
+    # the tsvdx4p-new jobs run from the same binary under the reference's name
+    link = os.path.join(PKG, "tsvdx4p-new")
+    if os.path.lexists(link):
+        os.remove(link)
+    os.symlink(os.path.basename(DRIVER), link)
     if verbose:
         print(f"tsvdx4_new.cpp {dt:.1f}s")
     return LIB

@@ -1,5 +1,7 @@
 // tsvdx4-new: job-level drop-in for the reference's `tsvdx4-new <job>`
-// (src/bin/tsvdx4-new.rs), running TxModel on the GPU through libtsvd_tx.so.
+// (src/bin/tsvdx4-new.rs), running TxModel on the GPU through libtsvd_tx.so;
+// the same binary (also installed as tsvdx4p-new) runs the `tsvdx4p-new <job>`
+// jobs (src/bin/tsvdx4p-new.rs) as Tsvdx4pModel epochs.
 //
 // It restates the reference driver: Dataset::load with target specs
 // (src/lib.rs:252-290), fit2_inner phase 1 split.tr -> split.pr and phase 2
@@ -12,7 +14,8 @@
 // let synthetic data of another shape through Dataset::load):
 //   TSVD_SCHEDULE = strata | serial   (default: the library's, one-stratum
 //                                      parallel; serial for day terms on
-//                                      ratings not sorted by date per user)
+//                                      ratings not sorted by date per user;
+//                                      Tsvdx4pModel epochs for tsvdx4p jobs)
 //   TSVD_STRATA   = 1 | 8 | ...       (strata of the parallel schedule)
 //   TSVD_N_USERS / TSVD_N_ITEMS       (default 480189 / 17770, as the reference)
 //   TSVD_DATA_DIR                     (default "data")
@@ -151,6 +154,36 @@ std::string debug_config(const tx_config& c) {
     s += std::string("save_ifeat: ") + (c.save_ifeat ? "true" : "false") + ", low_memory: " +
          (c.low_memory ? "true" : "false") + ", full_su: " + (c.full_su ? "true" : "false") + " }";
     return s;
+}
+
+// Debug rendering of Tsvdx4pConfig (src/tsvdx4p.rs:128-186): its own field
+// order, the TxConfig fields it shares, then the three tsvdx4p extras.
+bool g_tsvdx4p = false;  // the job trains a Tsvdx4pModel (tsvdx4p-new jobs)
+
+std::string debug_config_p(const tx_config& c) {
+    static const char* kOrder[] = {"n_feat", "n_epochs", "seed", "shuffle_users", "n_time_bins", "beta",
+        "n_freq_bins", "lr_u", "lr_ud", "lr_u2", "lr_ub", "lr_ubd", "lr_i", "lr_ib", "lr_y", "lr_yb", "lr_yd",
+        "lr_tu", "lr_ti", "lr_ta", "lr_ibf", "lr_iqf", "reg_iqf", "sigma_iqf", "lr_cu", "reg_cu", "reg_u",
+        "reg_u2", "reg_ud", "reg_i", "reg_y", "reg_yd", "sigma_u", "sigma_i", "sigma_y", "sigma_yd",
+        "reset_u_epoch"};
+    std::string s = "Tsvdx4pConfig { ";
+    const char* base = reinterpret_cast<const char*>(&c);
+    for (const char* name : kOrder) {
+        const Field& f = *field(name);
+        s += f.name;
+        s += ": ";
+        if (f.kind == 'i') s += std::to_string(*reinterpret_cast<const int32_t*>(base + f.off));
+        else if (f.kind == 'b') s += *reinterpret_cast<const int32_t*>(base + f.off) ? "true" : "false";
+        else if (f.kind == 'u') s += std::to_string(*reinterpret_cast<const uint64_t*>(base + f.off));
+        else s += rust_f32(*reinterpret_cast<const float*>(base + f.off));
+        s += ", ";
+    }
+    s += std::string("save_ifeat: ") + (c.save_ifeat ? "true" : "false") + ", low_memory: " +
+         (c.low_memory ? "true" : "false") + ", full_su: " + (c.full_su ? "true" : "false") +
+         ", nsvd_norm_exp: " + rust_f32(c.nsvd_norm_exp) + ", n_threads: " + std::to_string(c.n_threads) +
+         ", seq_epochs: [";
+    for (int k = 0; k < 5; ++k) s += std::to_string(c.seq_epochs[k]) + (k < 4 ? ", " : "");
+    return s + "] }";
 }
 
 tx_config make_config(std::initializer_list<std::pair<const char*, double>> kv) {
@@ -311,11 +344,13 @@ struct Model {
     Model(const Dataset& tr, const Dataset& pr, const tx_config& c) : cfg(c), n_items(tr.n_items) {
         const tx_ratings t = tr.view(false), p = pr.view(true);
         check(tx_create(&cfg, &t, &p, &ctx), "TxModel::new");
-        // the library's default schedule, epochs captured in a graph, mode and
-        // strata from the environment
+        // the library's default schedule (Tsvdx4pModel epochs for tsvdx4p
+        // jobs), epochs captured in a graph, mode and strata from the environment
         const std::string mode = env("TSVD_SCHEDULE", "");
         tx_schedule s{TX_SCHED_STRATA, 1, 0, 0, 1, TX_UPD_ATOMIC | (TX_UPD_PLAIN << 2), 2, 1024};
+        if (g_tsvdx4p) s.mode = TX_SCHED_TSVDX4P;
         if (mode == "serial") s.mode = TX_SCHED_SERIAL;
+        else if (mode == "strata") s.mode = TX_SCHED_STRATA;
         s.n_strata = std::atoi(env("TSVD_STRATA", "1").c_str());
         int rc = tx_set_schedule(ctx, &s);
         if (rc == TX_EUNSUPPORTED && mode.empty() && !cfg.low_memory) {
@@ -455,7 +490,7 @@ void fit2(const tx_config& cfg, const std::string& target, const std::string& jo
     }
     tee("[%s]\n", job.c_str());
     tee("target = \"%s\"\n", target.c_str());
-    tee("%s\n", debug_config(cfg).c_str());
+    tee("%s\n", (g_tsvdx4p ? debug_config_p(cfg) : debug_config(cfg)).c_str());
     {
         tee("%s => %s%s\n", sp.tr, sp.pr, o.transpose ? " (transposed)" : "");
         Dataset tr = load_dataset(sp.tr, target, preds_dir);
@@ -712,6 +747,37 @@ bool find_job(const std::string& job, Job* out) {
     return false;
 }
 
+// tsvdx4p-new job table (src/bin/tsvdx4p-new.rs:12-26, run_lm :30-85): the lm
+// set with lr_yb 2e-4, reset at epoch 1024, n^1.0 NSVD1 normalisation, 16
+// threads, low-memory; save_train jobs write train predictions, the others
+// every epoch's probe predictions.
+bool find_job_p(const std::string& job, Job* out) {
+    struct P { const char* name; int f; bool save_ifeat; int seq[5]; bool save_train; };
+    static const P kJobs[] = {
+        {"tsvdx4p-1000lm", 1000, true, {1, 10, 20, 0, 0}, true},
+        {"tsvdx4p-2000lm", 2000, true, {1, 10, 20, 0, 0}, true},
+        {"tsvdx4p-4000lm", 4000, false, {1, 10, 20, 0, 0}, false},
+        {"tsvdx4p-6000lm", 6000, true, {0, 0, 0, 0, 0}, true},
+        {"tsvdx4p-8000lm", 8000, true, {1, 10, 20, 0, 0}, true},
+        {"tsvdx4p-16000lm", 16000, false, {1, 10, 20, 0, 0}, false},
+    };
+    for (const P& p : kJobs) {
+        if (job != p.name) continue;
+        tx_config c = lm(p.f, 20, 42, 29, p.save_ifeat, true);
+        c.lr_yb = 0.0002f;
+        c.reset_u_epoch = 1024;
+        c.nsvd_norm_exp = 1.0f;
+        c.n_threads = 16;
+        for (int k = 0; k < 5; ++k) c.seq_epochs[k] = p.seq[k];
+        Opts o;
+        o.save_train = p.save_train;
+        o.save_probe_each_epoch = !p.save_train;
+        *out = {c, o};
+        return true;
+    }
+    return false;
+}
+
 bool find_eblend(const std::string& job, std::vector<double>* w, double* off) {
     static const std::map<std::string, std::pair<std::vector<double>, double>> table = {
         {"tsvdx4-60__epochs",
@@ -749,9 +815,10 @@ int main(int argc, char** argv) {
     const std::string job = argv[1];
     const bool print_only = argc > 2 && std::string(argv[2]) == "--print-config";
     Job j;
-    if (find_job(job, &j)) {
+    g_tsvdx4p = find_job_p(job, &j);
+    if (g_tsvdx4p || find_job(job, &j)) {
         if (print_only) {
-            std::printf("%s\n", debug_config(j.cfg).c_str());
+            std::printf("%s\n", (g_tsvdx4p ? debug_config_p(j.cfg) : debug_config(j.cfg)).c_str());
             return 0;
         }
         fit2(j.cfg, "rtg", job, SPLIT_NEW, j.opts);

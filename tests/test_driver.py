@@ -35,6 +35,21 @@ def test_print_config_matches_reference_job_table():
     assert (c["n_feat"], c["low_memory"], c["save_ifeat"]) == ("60", "false", "false")
 
 
+def test_print_config_tsvdx4p_job():
+    """tsvdx4p-new jobs (src/bin/tsvdx4p-new.rs:14, 30-85) from the same binary
+    under the reference's name: the Tsvdx4pConfig of tsvdx4p-1000lm."""
+    out = subprocess.run([os.path.join(ROOT, "netflix_prize_amd", "tsvdx4p-new"), "tsvdx4p-1000lm", "--print-config"],
+                         capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.startswith("Tsvdx4pConfig { n_feat: 1000, n_epochs: 20, seed: 42,")
+    c = dict(re.findall(r"(\w+): ([^,{}\[\]]+)", out.stdout))
+    expect = {"lr_yb": "0.0002", "lr_iqf": "5e-6", "sigma_iqf": "0.005", "reset_u_epoch": "1024",
+              "save_ifeat": "true", "low_memory": "true", "nsvd_norm_exp": "1.0", "n_threads": "16"}
+    for k, v in expect.items():
+        assert c[k].strip() == v, (k, c[k])
+    assert "seq_epochs: [1, 10, 20, 0, 0] }" in out.stdout
+
+
 def test_unknown_job_is_an_error():
     out = subprocess.run([DRIVER, "tsvdx4-no-such-job"], capture_output=True, text=True, timeout=60)
     assert out.returncode != 0
@@ -78,4 +93,30 @@ def test_job_end_to_end(gpu, tmp_path):
     rmse = float(np.sqrt(np.mean((probe.astype(np.float64) - pr.raw_ratings) ** 2)))
     assert abs(logged - rmse) < 1e-5, (logged, rmse)
     # better than the best constant prediction of the held-out ratings
+    assert rmse < float(np.std(pr.raw_ratings.astype(np.float64)))
+
+
+@pytest.mark.gpu
+def test_tsvdx4p_job_end_to_end(gpu, tmp_path):
+    """tsvdx4p-1000lm (Tsvdx4pModel epochs with the q_f stream, f=1000,
+    seq_epochs 1/10/20) on the 10k x 2k x 1M split: its output files and the
+    logged probe RMSE equal to the RMSE of the saved predictions."""
+    tr, pr = synthetic("small")
+    _write_split(str(tmp_path), tr, pr)
+    env = dict(os.environ, TSVD_N_USERS=str(tr.n_users), TSVD_N_ITEMS=str(tr.n_items),
+               TSVD_DATA_DIR=str(tmp_path / "data"))
+    job = "tsvdx4p-1000lm"
+    out = subprocess.run([DRIVER, job], cwd=tmp_path, env=env, capture_output=True, text=True, timeout=600)
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
+    pd = tmp_path / "preds_new"
+    probe = np.load(pd / f"{job}.probex.npy")
+    assert probe.shape == (pr.n_ratings,) and np.all(np.isfinite(probe))
+    assert np.load(pd / f"{job}.trainx.npy").shape == (tr.n_ratings,)
+    assert np.load(pd / f"{job}.qual.npy").shape == (pr.n_ratings,)
+    assert np.load(pd / f"{job}.ifeat.trainx.npy").shape == (tr.n_items, 1000)
+    log = (pd / f"{job}.out").read_text()
+    assert log.splitlines()[2].startswith("Tsvdx4pConfig {")
+    logged = float(re.search(r"probex RMSE ([0-9.]+)", log).group(1))
+    rmse = float(np.sqrt(np.mean((probe.astype(np.float64) - pr.raw_ratings) ** 2)))
+    assert abs(logged - rmse) < 1e-5, (logged, rmse)
     assert rmse < float(np.std(pr.raw_ratings.astype(np.float64)))
