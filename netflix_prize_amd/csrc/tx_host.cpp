@@ -86,6 +86,7 @@ float dev_host(float tu_mean, int day, float beta) {
 
 struct Shape {
     int team, vec;
+    int waves = 1;  // waves per user in the SGD kernel (f > 1024: chunks of 1024 elements)
 };
 
 // One wave (64 lanes) per user at every f: at f=60, four 16-lane teams per
@@ -97,6 +98,7 @@ bool pick_shape(int f, Shape* s) {
     else if (f <= 256) *s = {64, 4};
     else if (f <= 512) *s = {64, 8};
     else if (f <= 1024) *s = {64, 16};
+    else if (f <= 8192) *s = {64, 16, (f + 1023) / 1024};
     else return false;
     return true;
 }
@@ -433,7 +435,7 @@ int build_strata(tx_ctx* c, int S) {
 int target_teams(int f) { return std::min(3072, std::max(768, 393216 / std::max(f, 1))); }
 
 int default_blocks(tx_ctx* c) {
-    const int teams_per_block = 256 / c->shape.team;
+    const int teams_per_block = c->shape.waves > 1 ? 1 : 256 / c->shape.team;
     const int want = target_teams(c->cfg.n_feat);
     int blocks = (want + teams_per_block - 1) / teams_per_block;
     // a multiple of 8 keeps the cell -> XCD mapping of the round-robin dispatch
@@ -557,6 +559,7 @@ int tsvdx4p_epoch(tx_ctx* c, int32_t epoch, tx_epoch_stats& stats, hipStream_t s
     const bool p2 = c->cfg.lr_u2 != 0.0f;
     if (!c->cfg.low_memory || c->cfg.has_ordinal)
         return fail(TX_EUNSUPPORTED, "tsvdx4p schedule: low-memory form without ordinal head only");
+    if (c->shape.waves > 1) return fail(TX_EUNSUPPORTED, "tsvdx4p schedule: n_feat <= 1024 only");
     if (c->built_S != 1) {
         if (c->built_S != 0) return fail(TX_EUNSUPPORTED, "tsvdx4p schedule: needs the one-stratum layout");
         if (int rc = build_strata(c, 1)) return rc;
@@ -593,8 +596,11 @@ int enqueue_strata_epoch(tx_ctx* c, hipStream_t st) {
     const bool day = c->cfg.low_memory == 0;
     const int blocks = c->sched.blocks > 0 ? c->sched.blocks : default_blocks(c);
     HIPC(tx::launch_fill_u32(c->sv.q_head, 0u, static_cast<int64_t>(S) * c->sv.n_cells, st));
-    for (int s = 0; s < S; ++s)
-        HIPC(tx::launch_sgd_par(c->dm, c->sv, s, c->shape.team, c->shape.vec, blocks, qf, p2, ord, 0, day, st));
+    if (c->shape.waves > 1)
+        HIPC(tx::launch_sgd_par_mw(c->dm, c->sv, blocks, c->shape.waves, qf, p2, ord, day, st));
+    else
+        for (int s = 0; s < S; ++s)
+            HIPC(tx::launch_sgd_par(c->dm, c->sv, s, c->shape.team, c->shape.vec, blocks, qf, p2, ord, 0, day, st));
     HIPC(tx::launch_rebuild(c->dm, c->n_ud, c->shape.team, c->shape.vec, st));
     return 0;
 }
@@ -654,7 +660,7 @@ int tx_create(const tx_config* cfg, const tx_ratings* tr, const tx_ratings* pr, 
     if (tr->n_ratings + pr->n_ratings >= (int64_t{1} << 32))
         return fail(TX_EUNSUPPORTED, "tx_create: more than 2^32 ratings");
     Shape shape;
-    if (!pick_shape(cfg->n_feat, &shape)) return fail(TX_EUNSUPPORTED, "tx_create: n_feat > 1024");
+    if (!pick_shape(cfg->n_feat, &shape)) return fail(TX_EUNSUPPORTED, "tx_create: n_feat > 8192");
 
     auto ctx = std::make_unique<tx_ctx>();
     tx_ctx* c = ctx.get();
@@ -663,7 +669,7 @@ int tx_create(const tx_config* cfg, const tx_ratings* tr, const tx_ratings* pr, 
     c->I = tr->n_items;
     c->f = cfg->n_feat;
     c->shape = shape;
-    c->fs = shape.team * shape.vec;
+    c->fs = shape.team * shape.vec * shape.waves;
     c->F = cfg->n_freq_bins;
     c->B = cfg->n_time_bins;
     c->N = tr->n_ratings;
@@ -1010,6 +1016,7 @@ int tx_set_schedule(tx_ctx* c, const tx_schedule* s) {
     if (s->mode == TX_SCHED_TSVDX4P) {
         if (!c->cfg.low_memory || c->cfg.has_ordinal)
             return fail(TX_EUNSUPPORTED, "the tsvdx4p schedule trains low_memory models without ordinal head");
+        if (c->shape.waves > 1) return fail(TX_EUNSUPPORTED, "the tsvdx4p schedule trains n_feat <= 1024");
         if (s->blocks < 0) return fail(TX_EINVAL, "blocks must be 0 (auto) or positive");
     }
     if (s->mode == TX_SCHED_STRATA) {
@@ -1017,6 +1024,8 @@ int tx_set_schedule(tx_ctx* c, const tx_schedule* s) {
             return fail(TX_EINVAL, "n_strata must be 1 or a multiple of 8 (<= 64), got %d", s->n_strata);
         if (!c->cfg.low_memory && s->n_strata != 1)
             return fail(TX_EUNSUPPORTED, "day terms (low_memory = false) train in the one-stratum parallel schedule");
+        if (c->shape.waves > 1 && s->n_strata != 1)
+            return fail(TX_EUNSUPPORTED, "n_feat > 1024 trains in the one-stratum parallel schedule");
         if (!c->cfg.low_memory && !c->days_sorted)
             return fail(TX_EUNSUPPORTED,
                         "the parallel schedule trains day terms when every user's training ratings are sorted by "
@@ -1130,8 +1139,12 @@ int tx_fit_epoch(tx_ctx* c, int32_t epoch) {
             const int blocks = c->sched.blocks > 0 ? c->sched.blocks : default_blocks(c);
             HIPC(tx::launch_fill_u32(c->sv.q_head, 0u, static_cast<int64_t>(S) * c->sv.n_cells, st));
             HIPC(hipEventRecord(c->ev[1], st));
-            for (int s = 0; s < S; ++s)
-                HIPC(tx::launch_sgd_par(c->dm, c->sv, s, c->shape.team, c->shape.vec, blocks, qf, p2, ord, 0, day, st));
+            if (c->shape.waves > 1)
+                HIPC(tx::launch_sgd_par_mw(c->dm, c->sv, blocks, c->shape.waves, qf, p2, ord, day, st));
+            else
+                for (int s = 0; s < S; ++s)
+                    HIPC(tx::launch_sgd_par(c->dm, c->sv, s, c->shape.team, c->shape.vec, blocks, qf, p2, ord, 0, day,
+                                            st));
             HIPC(hipEventRecord(c->ev[2], st));
             HIPC(tx::launch_rebuild(c->dm, c->n_ud, c->shape.team, c->shape.vec, st));
             HIPC(hipEventRecord(c->ev[3], st));
@@ -1161,6 +1174,7 @@ int tx_fit_users_epoch(tx_ctx* c, int32_t epoch) {
     if (epoch < 1) return fail(TX_EINVAL, "tx_fit_users_epoch: epochs are 1-based");
     if (!c->cfg.low_memory || c->cfg.has_ordinal)
         return fail(TX_EUNSUPPORTED, "tx_fit_users_epoch: low-memory models without ordinal head only");
+    if (c->shape.waves > 1) return fail(TX_EUNSUPPORTED, "tx_fit_users_epoch: n_feat <= 1024 only");
     if (c->built_S > 1) return fail(TX_EUNSUPPORTED, "tx_fit_users_epoch: the model uses a stratified layout");
     if (c->built_S == 0)
         if (int rc = build_strata(c, 1)) return rc;
@@ -1228,7 +1242,7 @@ int tx_autotune(tx_ctx* c, int32_t trial_epochs, tx_schedule* best_out) {
         return 0;
     };
     const tx_schedule orig = c->sched;
-    const int tpb = 256 / c->shape.team, cap = target_teams(c->cfg.n_feat);
+    const int tpb = c->shape.waves > 1 ? 1 : 256 / c->shape.team, cap = target_teams(c->cfg.n_feat);
     std::vector<tx_schedule> cands;
     for (int quarters : {2, 3, 4}) {
         int blocks = (cap * quarters / 4 + tpb - 1) / tpb;

@@ -540,6 +540,13 @@ __device__ __forceinline__ void for_rows_at(uint32_t n, int lane, const uint32_t
     }
 }
 
+// Workgroup barrier that orders LDS only: the waves of a multi-wave team meet
+// here once per rating, and a release fence at workgroup scope would also wait
+// for every global load in flight (the whole prefetch ring).
+__device__ __forceinline__ void wg_barrier() {
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 // Day-stream entries with this bit set are day headers (index = SparseUD day).
 constexpr uint32_t kDayHead = 0x80000000u;
 
@@ -591,14 +598,33 @@ struct Pending {
 //         step of every run.  A user's days have disjoint item sets (each item
 //         is rated once), so only other users' concurrent steps separate this
 //         order from the reference's gather-at-run-start, step-at-run-end.
-template <int TEAM, int VEC, bool QF, bool P2, bool ORD, int PASS, bool DAY>
-__global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, StrataView sv, int stratum) {
+// MW:     a team of W = blockDim.x / 64 waves per user, W <= MW (4 or 8; 0 =
+//         one wave; f > 1024, one team per block, one stratum, PASS 0).
+//         Wave w owns elements [w*64*VEC,
+//         (w+1)*64*VEC) of every row and runs the one-wave code on that chunk;
+//         the waves meet only in the rating's dot product (per-wave sums
+//         through LDS and a barrier, added in wave order) and the queue grab.
+//         Every wave computes the same scalars; the item-bias atomics and
+//         the shared NSVD1-bias writes are issued by wave 0 alone, user-owned
+//         slots by every wave (each wave's own later loads then see them).
+template <int TEAM, int VEC, bool QF, bool P2, bool ORD, int PASS, bool DAY, int MW>
+__global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_sgd_par(DevModel m, StrataView sv, int stratum) {
     static_assert(PASS == 0 || !ORD, "the user passes have no ordinal head");
     static_assert(PASS == 0 || !DAY, "the user passes train low-memory models");
+    static_assert(!MW || (TEAM == 64 && PASS == 0), "multi-wave teams: 64-lane waves, SGD pass");
     constexpr int CHUNK = 4;
-    constexpr int DEPTH = TX_PREFETCH;
-    __shared__ float xbuf[256 * VEC];  // per-team transpose slot, TEAM*VEC floats each
+    // up to 4 waves share a SIMD's register file with nothing else; with 8,
+    // two waves share each SIMD and one rating ahead keeps them closer to it
+    constexpr int DEPTH = MW == 8 ? 1 : TX_PREFETCH;
+    __shared__ float xbuf[(MW == 8 ? 512 : 256) * VEC];  // per-wave transpose slot, TEAM*VEC floats each
+    __shared__ float dred[2][MW ? 9 : 1];            // per-wave dot products + wave 0's b_i(t) (MW), double-buffered
+    __shared__ uint32_t qgrab[1];                    // the team's queue grab (MW)
     const int lane = threadIdx.x % TEAM;
+    const int wv = MW ? static_cast<int>(threadIdx.x / 64) : 0;  // wave of the team
+    const int nwv = MW ? static_cast<int>(blockDim.x / 64) : 1;
+    const int cbase = wv * TEAM * VEC;                            // first element of the wave's chunk
+    const bool lead = wv == 0;                                    // issues the shared one-off writes
+    uint32_t dpar = 0;
     const int S = sv.n_strata;
     const int cell = sv.n_cells > 1 ? static_cast<int>(blockIdx.x % sv.n_cells) : 0;
     const int qid = stratum * sv.n_cells + cell;
@@ -618,7 +644,7 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
     if constexpr (QF) { h.lr_iqf = in_vgpr(h.lr_iqf); h.reg_iqf = in_vgpr(h.reg_iqf); }
     const bool has_ud = h.lr_ud != 0.0f;  // p_u,day trained (src/tx.rs:1005-1012)
     if constexpr (DAY) { h.lr_ud = in_vgpr(h.lr_ud); h.reg_ud = in_vgpr(h.reg_ud); }
-    const int off = lane;  // strided: element c*TEAM + lane
+    const int off = cbase + lane;  // strided: element cbase + c*TEAM + lane
     const bool first = (stratum == 0);
     const bool last = (stratum == S - 1);
     const int qm = sv.q_mode, ym = sv.y_mode;
@@ -645,8 +671,18 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
 
     for (;;) {
         uint32_t w0 = 0;
-        if (lane == 0) w0 = __hip_atomic_fetch_add(&sv.q_head[qid], static_cast<uint32_t>(CHUNK), __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_AGENT);
+        if constexpr (MW) {
+            // one grab for the team, passed to its waves through LDS
+            wg_barrier();
+            if (lead && lane == 0)
+                qgrab[0] = __hip_atomic_fetch_add(&sv.q_head[qid], static_cast<uint32_t>(CHUNK), __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT);
+            wg_barrier();
+            w0 = qgrab[0];
+        } else if (lane == 0) {
+            w0 = __hip_atomic_fetch_add(&sv.q_head[qid], static_cast<uint32_t>(CHUNK), __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT);
+        }
         // a full-wave team takes the value with readfirstlane so the compiler
         // sees it (and the user, segment bounds and loop trip counts derived
         // from it) as wave-uniform: scalar loop control, no exec-mask loops
@@ -727,7 +763,7 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
                 const int coff = lane * VEC;
                 const bool yb_on = !m.skip_yb;
                 for_rows<TEAM, VEC, row_group<TEAM, VEC>()>(
-                    ntot, lane, m.yfeat, fs,
+                    ntot, lane, m.yfeat + cbase, fs,
                     [&](uint32_t r) { return r < ntr ? titems + r : hitems + r; },
                     [&](uint32_t b0, uint32_t ib) {
                         if (yb_on && b0 + static_cast<uint32_t>(lane) < ntot) sb += m.ybias[ib];
@@ -760,12 +796,12 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
                     for_rows_at<TEAM, VEC, row_group<TEAM, VEC>()>(
                         m.dg_off[u + 1] - g0, lane, m.dg_item + g0,
                         [&](uint32_t j) {
-                            return (j & kDayHead) ? sv.day_acc + static_cast<size_t>(j & ~kDayHead) * fs
-                                                  : m.yfeat_day + static_cast<size_t>(j) * fs;
+                            return (j & kDayHead) ? sv.day_acc + static_cast<size_t>(j & ~kDayHead) * fs + cbase
+                                                  : m.yfeat_day + static_cast<size_t>(j) * fs + cbase;
                         },
                         [&](uint32_t j, const float (&y)[VEC]) {
                             if (j & kDayHead) {
-                                store_vec<VEC>(sv.day_acc + static_cast<size_t>(j & ~kDayHead) * fs + lane * VEC, dacc);
+                                store_vec<VEC>(sv.day_acc + static_cast<size_t>(j & ~kDayHead) * fs + cbase + lane * VEC, dacc);
 #pragma unroll
                                 for (int c = 0; c < VEC; ++c) dacc[c] = 0.0f;
                             } else {
@@ -872,7 +908,7 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
                 const float c_bud = cur.has_bud ? cur.bud : team_bcast_f<TEAM>(cur.sc, 2);
                 const float c_cut = m.skip_cut ? 0.0f : team_bcast_f<TEAM>(cur.sc2, 0);
                 const float c_but = cur.has_but ? cur.but : (m.skip_but ? 0.0f : team_bcast_f<TEAM>(cur.sc2, 1));
-                const float bi_t = c_ib + c_bt + c_ibf;
+                float bi_t = c_ib + c_bt + c_ibf;
                 const float cu_t = cu + c_cut;
                 const float bu_t = ubias + c_but + alpha * dev + c_bud + su_b;
                 if constexpr (DAY) {
@@ -903,7 +939,20 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
 #pragma unroll
                     for (int c = 0; c < VEC; ++c) part += (cur.q[c] + cur.qf[c]) * (p[c] + s[c] + dev * p2[c]);
                 }
-                const float dot = team_sum_dpp<TEAM>(part);
+                float dot = team_sum_dpp<TEAM>(part);
+                if constexpr (MW) {
+                    // the waves' chunk sums, added in wave order by every wave, and
+                    // wave 0's item biases (other teams' atomics may land between
+                    // the waves' loads; one value keeps the waves' scalars equal)
+                    if (lane == 0) dred[dpar][wv] = dot;
+                    if (lane == 0 && lead) dred[dpar][8] = bi_t;
+                    wg_barrier();
+                    float d = 0.0f;
+                    for (int w = 0; w < nwv; ++w) d += dred[dpar][w];
+                    dot = d;
+                    bi_t = dred[dpar][8];
+                    dpar ^= 1u;
+                }
                 const float score = m.gbias + bu_t + bi_t * cu_t + dot;
                 float err;
                 if constexpr (ORD) {
@@ -929,8 +978,11 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
                     if (nx.ud == cur.ud) { nx.has_bud = true; nx.bud = bud_new; }
                 }
                 // item biases in one instruction (lanes 0, 1, 3), b_u,day in one (lane 2)
-                if (PASS == 0 && sc_item) apply_delta(cur.sc_addr, cur.sc, -sc_lr * (err_b * cu_t), qm);
-                else if (lane == 2) *cur.sc_addr = bud_new;
+                if (PASS == 0 && sc_item) {
+                    if (lead) apply_delta(cur.sc_addr, cur.sc, -sc_lr * (err_b * cu_t), qm);
+                } else if (lane == 2) {
+                    *cur.sc_addr = bud_new;
+                }
                 if (lane == 1 && !m.skip_but) m.but_bin[ubrow + cur.b] = but_new;
                 cu -= h.lr_cu * (err_b * bi_t + h.reg_cu * (cu - 1.0f));
                 Gb += (PASS == 0 && m.sum_err_bug) ? err_b * static_cast<float>(m.f) : err_b;
@@ -979,7 +1031,7 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
 
             store_str<VEC, TEAM>(m.ufeat + prow, p);
             if constexpr (P2) store_str<VEC, TEAM>(m.ufeat2 + prow, p2);
-            if (lane == 0) {
+            if (lane == 0 && lead) {
                 m.ubias[u] = ubias;
                 m.alpha_u[u] = alpha;
                 m.cu[u] = cu;
@@ -1032,15 +1084,16 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                     const uint32_t* titems = m.t_item + ts;
                     for_rows<TEAM, VEC, row_group<TEAM, VEC>()>(
-                        te - ts, lane, m.yfeat, fs, [&](uint32_t r) { return titems + r; },
+                        te - ts, lane, m.yfeat + cbase, fs, [&](uint32_t r) { return titems + r; },
                         [&](uint32_t b0, uint32_t ib) {
-                            if (gb != 0.0f && b0 + static_cast<uint32_t>(lane) < te - ts) m.ybias[ib] = m.ybias[ib] - gb;
+                            if (gb != 0.0f && lead && b0 + static_cast<uint32_t>(lane) < te - ts)
+                                m.ybias[ib] = m.ybias[ib] - gb;
                         },
                         [&](uint32_t, uint32_t j, const float (&y)[VEC]) {
                             float ny[VEC];
 #pragma unroll
                             for (int c = 0; c < VEC; ++c) ny[c] = y[c] - h.lr_y * (gc[c] + h.reg_y * y[c]);
-                            store_vec<VEC>(m.yfeat + static_cast<size_t>(j) * fs + coff, ny);
+                            store_vec<VEC>(m.yfeat + static_cast<size_t>(j) * fs + cbase + coff, ny);
                         });
                 } else {
                 // rows of batch b+1 are loaded before batch b's updates are issued
@@ -1061,7 +1114,7 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
     #pragma unroll
                             for (int c = 0; c < VEC; ++c) dy[c] = -h.lr_y * (gq[c] + h.reg_y * y[k][c]);
                             apply_vec<VEC, TEAM>(m.yfeat + static_cast<size_t>(jj[k]) * fs + off, y[k], dy, ym);
-                            if (gb != 0.0f && lane == ((t0 + k) & (TEAM - 1)))
+                            if (gb != 0.0f && lead && lane == ((t0 + k) & (TEAM - 1)))
                                 apply_delta(&m.ybias[jj[k]], m.ybias[jj[k]], -gb, ym);
                         }
                     };
@@ -1091,15 +1144,15 @@ __global__ __launch_bounds__(256, TX_SGD_WAVES) void k_sgd_par(DevModel m, Strat
                 for_rows_at<TEAM, VEC, row_group<TEAM, VEC>()>(
                     m.dy_off[u + 1] - y0, lane, m.dy_item + y0,
                     [&](uint32_t j) {
-                        return (j & kDayHead) ? sv.day_acc + static_cast<size_t>(j & ~kDayHead) * fs
-                                              : m.yfeat_day + static_cast<size_t>(j) * fs;
+                        return (j & kDayHead) ? sv.day_acc + static_cast<size_t>(j & ~kDayHead) * fs + cbase
+                                              : m.yfeat_day + static_cast<size_t>(j) * fs + cbase;
                     },
                     [&](uint32_t j, const float (&y)[VEC]) {
                         if (j & kDayHead) {
 #pragma unroll
                             for (int c = 0; c < VEC; ++c) g[c] = y[c];
                         } else {
-                            float* row = m.yfeat_day + static_cast<size_t>(j) * fs + lane * VEC;
+                            float* row = m.yfeat_day + static_cast<size_t>(j) * fs + cbase + lane * VEC;
                             if (ym == 2) {
                                 float ny[VEC];
 #pragma unroll
@@ -1309,8 +1362,10 @@ __global__ __launch_bounds__(256) void k_rebuild_ycache(DevModel m) {
     const int lane = threadIdx.x % TEAM;
     const int nteams = gridDim.x * blockDim.x / TEAM;
     const int fs = m.fs, off = lane * VEC;
-    const bool yb_on = !m.skip_yb;
     for (int u = team; u < m.U; u += nteams) {
+    // rows wider than the team (f > 1024) are summed a TEAM*VEC chunk at a time
+    for (int cb = 0; cb < fs; cb += TEAM * VEC) {
+        const bool yb_on = !m.skip_yb && cb == 0;
         float acc[VEC];
 #pragma unroll
         for (int c = 0; c < VEC; ++c) acc[c] = 0.0f;
@@ -1322,7 +1377,7 @@ __global__ __launch_bounds__(256) void k_rebuild_ycache(DevModel m) {
         // rows streamed in order (training rows, then held-out rows); every
         // per-element sum and the ybias sum run sequentially in that order
         for_rows<TEAM, VEC, row_group<TEAM, VEC>()>(
-            n, lane, m.yfeat, fs, [&](uint32_t r) { return r < ntr ? titems + r : hitems + r; },
+            n, lane, m.yfeat + cb, fs, [&](uint32_t r) { return r < ntr ? titems + r : hitems + r; },
             [&](uint32_t b0, uint32_t ib) {
                 if (!yb_on) return;
                 const float ybl = b0 + static_cast<uint32_t>(lane) < n ? m.ybias[ib] : 0.0f;
@@ -1339,8 +1394,9 @@ __global__ __launch_bounds__(256) void k_rebuild_ycache(DevModel m) {
             for (int c = 0; c < VEC; ++c) acc[c] = acc[c] / nrm;
             sb = sb / nrm;
         }
-        store_vec<VEC>(m.ycache + static_cast<size_t>(u) * fs + off, acc);
-        if (lane == 0) m.ycache_bias[u] = sb;
+        store_vec<VEC>(m.ycache + static_cast<size_t>(u) * fs + cb + off, acc);
+        if (lane == 0 && cb == 0) m.ycache_bias[u] = sb;
+    }
     }
 }
 
@@ -1350,8 +1406,9 @@ __global__ __launch_bounds__(256) void k_rebuild_ycache_day(DevModel m, int64_t 
     const int64_t team = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / TEAM;
     const int lane = threadIdx.x % TEAM;
     const int64_t nteams = static_cast<int64_t>(gridDim.x) * blockDim.x / TEAM;
-    const int fs = m.fs, off = lane * VEC;
+    const int fs = m.fs;
     for (int64_t d = team; d < n_ud; d += nteams) {
+    for (int off = lane * VEC; off < fs; off += TEAM * VEC) {  // chunks of rows wider than the team
         float acc[VEC];
 #pragma unroll
         for (int c = 0; c < VEC; ++c) acc[c] = 0.0f;
@@ -1373,6 +1430,7 @@ __global__ __launch_bounds__(256) void k_rebuild_ycache_day(DevModel m, int64_t 
             for (int c = 0; c < VEC; ++c) acc[c] = acc[c] / nrm;
         }
         store_vec<VEC>(m.ycache_day + static_cast<size_t>(d) * fs + off, acc);
+    }
     }
 }
 
@@ -1405,37 +1463,46 @@ __global__ __launch_bounds__(256) void k_predict(DevModel m, int64_t n, const in
         const float bu_day = ud >= 0 ? m.ubias_day[ud] : 0.0f;
         const float bi_t = m.ibias[i] + m.bit_bin[static_cast<size_t>(i) * B + b] + m.ibias_freq[static_cast<size_t>(i) * F + fb];
         const float cu_t = m.cu[u] + (ud >= 0 ? m.cut[ud] : 0.0f);
+        const bool day_terms = !m.low_mem && ud >= 0;
+        // partial sums over the row, a TEAM*VEC chunk at a time (one chunk up to f=1024)
+        float part = 0.0f, mf = 0.0f, ns = 0.0f;
+        for (int cb = 0; cb < fs; cb += TEAM * VEC) {
+        const int o = cb + off;
         float q[VEC], qf[VEC], p[VEC], p2[VEC], s[VEC];
         if (m.bf16_pred) {
-            load_bf16<VEC>(m.b_ifeat + static_cast<size_t>(i) * fs + off, q);
-            load_bf16<VEC>(m.b_ifeat_freq + (static_cast<size_t>(i) * F + fb) * fs + off, qf);
-            load_bf16<VEC>(m.b_ufeat + static_cast<size_t>(u) * fs + off, p);
-            load_bf16<VEC>(m.b_ufeat2 + static_cast<size_t>(u) * fs + off, p2);
-            load_bf16<VEC>(m.b_ycache + static_cast<size_t>(u) * fs + off, s);
+            load_bf16<VEC>(m.b_ifeat + static_cast<size_t>(i) * fs + o, q);
+            load_bf16<VEC>(m.b_ifeat_freq + (static_cast<size_t>(i) * F + fb) * fs + o, qf);
+            load_bf16<VEC>(m.b_ufeat + static_cast<size_t>(u) * fs + o, p);
+            load_bf16<VEC>(m.b_ufeat2 + static_cast<size_t>(u) * fs + o, p2);
+            load_bf16<VEC>(m.b_ycache + static_cast<size_t>(u) * fs + o, s);
         } else {
-            load_vec<VEC>(m.ifeat + static_cast<size_t>(i) * fs + off, q);
-            load_vec<VEC>(m.ifeat_freq + (static_cast<size_t>(i) * F + fb) * fs + off, qf);
-            load_vec<VEC>(m.ufeat + static_cast<size_t>(u) * fs + off, p);
-            load_vec<VEC>(m.ufeat2 + static_cast<size_t>(u) * fs + off, p2);
-            load_vec<VEC>(m.ycache + static_cast<size_t>(u) * fs + off, s);
+            load_vec<VEC>(m.ifeat + static_cast<size_t>(i) * fs + o, q);
+            load_vec<VEC>(m.ifeat_freq + (static_cast<size_t>(i) * F + fb) * fs + o, qf);
+            load_vec<VEC>(m.ufeat + static_cast<size_t>(u) * fs + o, p);
+            load_vec<VEC>(m.ufeat2 + static_cast<size_t>(u) * fs + o, p2);
+            load_vec<VEC>(m.ycache + static_cast<size_t>(u) * fs + o, s);
         }
         float pd[VEC], sd[VEC];
-        const bool day_terms = !m.low_mem && ud >= 0;
         if (day_terms) {
-            load_vec<VEC>(m.ufeat_day + static_cast<size_t>(ud) * fs + off, pd);
-            load_vec<VEC>(m.ycache_day + static_cast<size_t>(ud) * fs + off, sd);
+            load_vec<VEC>(m.ufeat_day + static_cast<size_t>(ud) * fs + o, pd);
+            load_vec<VEC>(m.ycache_day + static_cast<size_t>(ud) * fs + o, sd);
         } else {
 #pragma unroll
             for (int c = 0; c < VEC; ++c) { pd[c] = 0.0f; sd[c] = 0.0f; }
         }
         if (MODE == 1) {
-            float mf = 0.0f, ns = 0.0f;
 #pragma unroll
             for (int c = 0; c < VEC; ++c) {
                 const float qe = q[c] + qf[c];
                 mf += (p[c] + dev * p2[c] + pd[c]) * qe;
                 ns += (s[c] + sd[c]) * qe;
             }
+        } else {
+#pragma unroll
+            for (int c = 0; c < VEC; ++c) part += (q[c] + qf[c]) * (p[c] + s[c] + dev * p2[c] + pd[c] + sd[c]);
+        }
+        }
+        if (MODE == 1) {
             mf = team_sum<TEAM>(mf);
             ns = team_sum<TEAM>(ns);
             if (lane == 0) {
@@ -1448,9 +1515,6 @@ __global__ __launch_bounds__(256) void k_predict(DevModel m, int64_t n, const in
             }
             continue;
         }
-        float part = 0.0f;
-#pragma unroll
-        for (int c = 0; c < VEC; ++c) part += (q[c] + qf[c]) * (p[c] + s[c] + dev * p2[c] + pd[c] + sd[c]);
         const float dot = team_sum<TEAM>(part);
         const float bu_t = m.ubias[u] + m.but_bin[static_cast<size_t>(u) * B + b] + m.alpha_u[u] * dev + bu_day
                            + m.ycache_bias[u];
@@ -1556,7 +1620,7 @@ static hipError_t sgd_par_tvp(const DevModel& m, const StrataView& sv, int strat
                               bool ord, hipStream_t st) {
     (void)hipGetLastError();
 #define TX_PAR(QF, P2, ORD) \
-    hipLaunchKernelGGL((k_sgd_par<TEAM, VEC, QF, P2, ORD, PASS, DAY>), dim3(blocks), dim3(256), 0, st, m, sv, stratum)
+    hipLaunchKernelGGL((k_sgd_par<TEAM, VEC, QF, P2, ORD, PASS, DAY, 0>), dim3(blocks), dim3(256), 0, st, m, sv, stratum)
     if constexpr (PASS == 0) {
         if (ord) {
             if (qf) { if (p2) TX_PAR(true, true, true); else TX_PAR(true, false, true); }
@@ -1592,6 +1656,35 @@ hipError_t launch_sgd_par(const DevModel& m, const StrataView& sv, int stratum, 
     TX_SHAPES(X)
 #undef X
     return hipErrorInvalidValue;
+}
+
+template <bool DAY, int MW>
+static hipError_t sgd_par_mw(const DevModel& m, const StrataView& sv, int teams, int waves, bool qf, bool p2, bool ord,
+                             hipStream_t st) {
+    (void)hipGetLastError();
+#define TX_MW(QF, P2, ORD)                                                                                    \
+    hipLaunchKernelGGL((k_sgd_par<64, 16, QF, P2, ORD, 0, DAY, MW>), dim3(teams), dim3(64 * waves), 0, st, m, \
+                       sv, 0)
+    if (ord) {
+        if (qf) { if (p2) TX_MW(true, true, true); else TX_MW(true, false, true); }
+        else { if (p2) TX_MW(false, true, true); else TX_MW(false, false, true); }
+    } else {
+        if (qf) { if (p2) TX_MW(true, true, false); else TX_MW(true, false, false); }
+        else { if (p2) TX_MW(false, true, false); else TX_MW(false, false, false); }
+    }
+#undef TX_MW
+    return hipGetLastError();
+}
+
+hipError_t launch_sgd_par_mw(const DevModel& m, const StrataView& sv, int teams, int waves, bool qf, bool p2, bool ord,
+                             bool day, hipStream_t st) {
+    if (waves < 2 || waves > 8 || m.fs != 1024 * waves || sv.n_strata != 1 || (day && !sv.day_acc))
+        return hipErrorInvalidValue;
+    if (waves <= 4)
+        return day ? sgd_par_mw<true, 4>(m, sv, teams, waves, qf, p2, ord, st)
+                   : sgd_par_mw<false, 4>(m, sv, teams, waves, qf, p2, ord, st);
+    return day ? sgd_par_mw<true, 8>(m, sv, teams, waves, qf, p2, ord, st)
+               : sgd_par_mw<false, 8>(m, sv, teams, waves, qf, p2, ord, st);
 }
 
 hipError_t launch_apply_nsvd(const DevModel& m, float* d_y, float* d_yb, const float* scale, hipStream_t st) {
@@ -1635,8 +1728,9 @@ hipError_t launch_rebuild(const DevModel& m, int64_t n_ud, int team, int vec, hi
 #undef X
     if (!done) return hipErrorInvalidValue;
     if (m.low_mem || n_ud <= 0) return hipSuccess;
-    // per-day sums: four elements per lane, up to 64 lanes
-    const int dt = m.fs / 4 <= 64 ? m.fs / 4 : 64, dv = m.fs / dt;
+    // per-day sums: four elements per lane, up to 64 lanes (wider rows in
+    // chunks of 64 x 16)
+    const int dt = m.fs / 4 <= 64 ? m.fs / 4 : 64, dv = m.fs / dt <= 16 ? m.fs / dt : 16;
 #define X(T, V)                                                                                                 \
     if (dt == T && dv == V) {                                                                                    \
         (void)hipGetLastError();                                                                                 \

@@ -14,6 +14,10 @@ hipError_t launch_sgd_serial(const DevModel& m, const int64_t* order, float* scr
 // day: day terms (pass 0, one stratum, sv.day_acc allocated)
 hipError_t launch_sgd_par(const DevModel& m, const StrataView& sv, int stratum, int team, int vec, int blocks,
                           bool qf, bool p2, bool ord, int pass, bool day, hipStream_t st);
+// multi-wave teams (f > 1024): `teams` workgroups of 64*waves threads, one user
+// each, fs = 1024*waves; SGD pass, one stratum
+hipError_t launch_sgd_par_mw(const DevModel& m, const StrataView& sv, int teams, int waves, bool qf, bool p2, bool ord,
+                             bool day, hipStream_t st);
 hipError_t launch_to_bf16(const float* x, uint16_t* y, int64_t n, hipStream_t st);
 hipError_t launch_apply_nsvd(const DevModel& m, float* d_y, float* d_yb, const float* scale, hipStream_t st);
 hipError_t launch_item_pass(const DevModel& m, const ItemView& iv, int team, int vec, bool p2, bool qf,

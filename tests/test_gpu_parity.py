@@ -206,6 +206,29 @@ def test_parallel_day_terms_match_serial_on_disjoint_users(gpu, f, ordinal):
     assert m_g.calc_rmse(pr) == pytest.approx(m_o.calc_rmse(), abs=1e-5)
 
 
+@pytest.mark.parametrize("f,low_memory", [(1500, True), (1500, False), (4500, True), (4500, False)])
+def test_wide_rows_match_serial_on_disjoint_users(gpu, f, low_memory):
+    """f > 1024: teams of 2 and 5 waves (1,024 elements of each row per wave,
+    the dot product joined through LDS) against the serial oracle on users
+    that share no item: every table within 1e-3 of its movement over two
+    epochs, with and without day terms."""
+    tr, pr = _disjoint_users()
+    cfg = lm_config(n_feat=f, n_epochs=2, n_time_bins=30, low_memory=low_memory, lr_ud=0.01, lr_yd=0.01,
+                    sigma_yd=0.05, lr_iqf=0.005, lr_ibf=0.005, lr_y=0.005, reset_u_epoch=100)
+    m_o, m_g = _oracle_and_gpu(cfg, tr, pr)
+    init = {n: m_o.get(n).copy() for n in ALL_PARAMS}
+    for e in (1, 2):
+        m_o.fit_epoch(e)
+        m_g.fit_epoch(epoch=e)
+    for n in ALL_PARAMS:
+        o, g = m_o.get(n), m_g.get(n)
+        move = float(np.max(np.abs(o.astype(np.float64) - init[n]))) if o.size else 0.0
+        err = float(np.max(np.abs(g.astype(np.float64) - o))) if o.size else 0.0
+        print(f"{n}: movement {move:.3e} gpu-oracle {err:.3e}")
+        assert err <= 1e-3 * move + 1e-6, n
+    assert m_g.calc_rmse(pr) == pytest.approx(m_o.calc_rmse(), abs=1e-5)
+
+
 def test_parallel_day_terms_holdout_rmse(gpu):
     """Day terms in the parallel schedule train to the serial oracle's
     holdout RMSE (within 0.001) on the 10k x 2k x 1M shape."""
