@@ -547,7 +547,7 @@ int build_item_view(tx_ctx* c) {
 
 int user_pass_blocks(tx_ctx* c) {
     // users are independent in the user passes: fill the device
-    const int teams_per_block = 256 / c->shape.team;
+    const int teams_per_block = c->shape.waves > 1 ? 1 : 256 / c->shape.team;
     const int blocks = (4096 + teams_per_block - 1) / teams_per_block;
     return std::max(8, (blocks + 7) / 8 * 8);
 }
@@ -559,7 +559,6 @@ int tsvdx4p_epoch(tx_ctx* c, int32_t epoch, tx_epoch_stats& stats, hipStream_t s
     const bool p2 = c->cfg.lr_u2 != 0.0f;
     if (!c->cfg.low_memory || c->cfg.has_ordinal)
         return fail(TX_EUNSUPPORTED, "tsvdx4p schedule: low-memory form without ordinal head only");
-    if (c->shape.waves > 1) return fail(TX_EUNSUPPORTED, "tsvdx4p schedule: n_feat <= 1024 only");
     if (c->built_S != 1) {
         if (c->built_S != 0) return fail(TX_EUNSUPPORTED, "tsvdx4p schedule: needs the one-stratum layout");
         if (int rc = build_strata(c, 1)) return rc;
@@ -570,12 +569,16 @@ int tsvdx4p_epoch(tx_ctx* c, int32_t epoch, tx_epoch_stats& stats, hipStream_t s
     HIPC(tx::launch_fill_u32(c->sv.q_head, 0u, c->sv.n_cells, st));
     HIPC(tx::launch_fill_u32(c->iv.q_head, 0u, 1, st));
     HIPC(hipEventRecord(c->ev[1], st));
-    HIPC(tx::launch_sgd_par(c->dm, c->sv, 0, c->shape.team, c->shape.vec, sgpr_blocks, qf, p2, false, 1, false, st));
+    if (c->shape.waves > 1)
+        HIPC(tx::launch_sgd_par_mw(c->dm, c->sv, sgpr_blocks, c->shape.waves, qf, p2, false, false, st, 1));
+    else
+        HIPC(tx::launch_sgd_par(c->dm, c->sv, 0, c->shape.team, c->shape.vec, sgpr_blocks, qf, p2, false, 1, false, st));
     HIPC(tx::launch_apply_nsvd(c->dm, c->sv.d_y, c->sv.d_yb, c->d_scale, st));
     HIPC(hipEventRecord(c->ev[2], st));
     HIPC(tx::launch_rebuild(c->dm, c->n_ud, c->shape.team, c->shape.vec, st));
     HIPC(hipEventRecord(c->ev[3], st));
-    HIPC(tx::launch_item_pass(c->dm, c->iv, c->shape.team, c->shape.vec, p2, qf, user_pass_blocks(c), st));
+    HIPC(tx::launch_item_pass(c->dm, c->iv, c->shape.team, c->shape.vec, p2, qf, user_pass_blocks(c), st,
+                              c->shape.waves));
     HIPC(hipEventSynchronize(c->ev[3]));
     HIPC(hipStreamSynchronize(st));
     float a = 0, b = 0;
@@ -1016,7 +1019,6 @@ int tx_set_schedule(tx_ctx* c, const tx_schedule* s) {
     if (s->mode == TX_SCHED_TSVDX4P) {
         if (!c->cfg.low_memory || c->cfg.has_ordinal)
             return fail(TX_EUNSUPPORTED, "the tsvdx4p schedule trains low_memory models without ordinal head");
-        if (c->shape.waves > 1) return fail(TX_EUNSUPPORTED, "the tsvdx4p schedule trains n_feat <= 1024");
         if (s->blocks < 0) return fail(TX_EINVAL, "blocks must be 0 (auto) or positive");
     }
     if (s->mode == TX_SCHED_STRATA) {
@@ -1174,7 +1176,6 @@ int tx_fit_users_epoch(tx_ctx* c, int32_t epoch) {
     if (epoch < 1) return fail(TX_EINVAL, "tx_fit_users_epoch: epochs are 1-based");
     if (!c->cfg.low_memory || c->cfg.has_ordinal)
         return fail(TX_EUNSUPPORTED, "tx_fit_users_epoch: low-memory models without ordinal head only");
-    if (c->shape.waves > 1) return fail(TX_EUNSUPPORTED, "tx_fit_users_epoch: n_feat <= 1024 only");
     if (c->built_S > 1) return fail(TX_EUNSUPPORTED, "tx_fit_users_epoch: the model uses a stratified layout");
     if (c->built_S == 0)
         if (int rc = build_strata(c, 1)) return rc;
@@ -1188,7 +1189,10 @@ int tx_fit_users_epoch(tx_ctx* c, int32_t epoch) {
     HIPC(tx::launch_fill_u32(c->sv.q_head, 0u, c->sv.n_cells, st));
     HIPC(hipEventRecord(c->ev[1], st));
     const int blocks = c->sched.blocks > 0 ? c->sched.blocks : user_pass_blocks(c);
-    HIPC(tx::launch_sgd_par(c->dm, c->sv, 0, c->shape.team, c->shape.vec, blocks, qf, p2, false, 2, false, st));
+    if (c->shape.waves > 1)
+        HIPC(tx::launch_sgd_par_mw(c->dm, c->sv, blocks, c->shape.waves, qf, p2, false, false, st, 2));
+    else
+        HIPC(tx::launch_sgd_par(c->dm, c->sv, 0, c->shape.team, c->shape.vec, blocks, qf, p2, false, 2, false, st));
     HIPC(hipEventRecord(c->ev[2], st));
     HIPC(tx::launch_rebuild(c->dm, c->n_ud, c->shape.team, c->shape.vec, st));
     HIPC(hipEventRecord(c->ev[3], st));

@@ -611,7 +611,7 @@ template <int TEAM, int VEC, bool QF, bool P2, bool ORD, int PASS, bool DAY, int
 __global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_sgd_par(DevModel m, StrataView sv, int stratum) {
     static_assert(PASS == 0 || !ORD, "the user passes have no ordinal head");
     static_assert(PASS == 0 || !DAY, "the user passes train low-memory models");
-    static_assert(!MW || (TEAM == 64 && PASS == 0), "multi-wave teams: 64-lane waves, SGD pass");
+    static_assert(!MW || TEAM == 64, "multi-wave teams: 64-lane waves");
     constexpr int CHUNK = 4;
     // up to 4 waves share a SIMD's register file with nothing else; with 8,
     // two waves share each SIMD and one rating ahead keeps them closer to it
@@ -1047,7 +1047,8 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_
                 for (uint32_t b0 = 0; b0 < ntr; b0 += TEAM) {
                     const uint32_t r = b0 + static_cast<uint32_t>(lane);
                     const uint32_t jl = m.t_item[ts + (r < ntr ? r : ntr - 1)];
-                    if (r < ntr) __hip_atomic_fetch_add(&sv.d_yb[jl], Gb / norm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (r < ntr && lead)
+                        __hip_atomic_fetch_add(&sv.d_yb[jl], Gb / norm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     const uint32_t nb = ntr - b0 < static_cast<uint32_t>(TEAM) ? ntr - b0 : static_cast<uint32_t>(TEAM);
                     for (uint32_t k = 0; k < nb; ++k) {
                         const uint32_t j = team_bcast<TEAM>(jl, k);
@@ -1221,17 +1222,28 @@ struct ItemPending {
 // are written back once.  Low-memory form; with QF the q_f rows of the day's
 // freq bin (the real bin here, bin 0 in the user pass: src/tsvdx4p.rs:497,
 // 708) are read and written per rating.
-template <int TEAM, int VEC, bool P2, bool QF>
-__global__ __launch_bounds__(256) void k_item_pass(DevModel m, ItemView iv) {
-    constexpr int DEPTH = TX_PREFETCH;
+// MW (4 or 8; 0 = one wave): a team of blockDim.x / 64 waves per item, wave w
+// holding elements [1024w, 1024w + 1024) of the rows (f > 1024); the waves
+// join each rating's dot product through LDS, keep their own copies of the
+// item's bias slots (they compute the same values) and wave 0 writes them back.
+template <int TEAM, int VEC, bool P2, bool QF, int MW>
+__global__ __launch_bounds__(MW == 8 ? 512 : 256) void k_item_pass(DevModel m, ItemView iv) {
+    static_assert(!MW || TEAM == 64, "multi-wave teams: 64-lane waves");
+    constexpr int DEPTH = MW == 8 ? 1 : TX_PREFETCH;
     constexpr int RING = DEPTH + 1;
     static_assert(TEAM % RING == 0, "the prefetch depth plus one must divide the team size");
     extern __shared__ float ish[];
+    __shared__ float dred[2][MW ? 8 : 1];
+    __shared__ uint32_t qgrab[1];
     const int lane = threadIdx.x % TEAM;
+    const int wv = MW ? static_cast<int>(threadIdx.x / 64) : 0;
+    const int nwv = MW ? static_cast<int>(blockDim.x / 64) : 1;
+    const bool lead = wv == 0;
+    uint32_t dpar = 0;
     const int B = m.B, F = m.F, fs = m.fs;
-    float* bt = ish + (threadIdx.x / TEAM) * (B + F);  // b_{i,bin}[B], then b_{i,f}[F]
+    float* bt = ish + (threadIdx.x / TEAM) * (B + F);  // per wave: b_{i,bin}[B], then b_{i,f}[F]
     float* bq = bt + B;
-    const int coff = lane * VEC;
+    const int coff = wv * TEAM * VEC + lane * VEC;
     const Hyper h = m.h;
     // lane-indexed user scalar slots: base + u*umul + (b & bm) + (ud & um)
     const float* sc_base = lane == 1 ? m.but_bin : lane == 2 ? m.alpha_u : lane == 3 ? m.ubias_day : m.ubias;
@@ -1244,7 +1256,14 @@ __global__ __launch_bounds__(256) void k_item_pass(DevModel m, ItemView iv) {
 
     for (;;) {
         uint32_t w = 0;
-        if (lane == 0) w = __hip_atomic_fetch_add(iv.q_head, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if constexpr (MW) {
+            wg_barrier();
+            if (lead && lane == 0) qgrab[0] = __hip_atomic_fetch_add(iv.q_head, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            wg_barrier();
+            w = qgrab[0];
+        } else if (lane == 0) {
+            w = __hip_atomic_fetch_add(iv.q_head, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
         if constexpr (TEAM == 64) w = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(w)));
         else w = __shfl(w, 0, TEAM);
         if (w >= iv.n_items) break;
@@ -1309,7 +1328,15 @@ __global__ __launch_bounds__(256) void k_item_pass(DevModel m, ItemView iv) {
                     float part = 0.0f;
 #pragma unroll
                     for (int c = 0; c < VEC; ++c) part += cur.pe[c] * (q[c] + qf[c]);
-                    const float dot = team_sum_dpp<TEAM>(part);
+                    float dot = team_sum_dpp<TEAM>(part);
+                    if constexpr (MW) {
+                        if (lane == 0) dred[dpar][wv] = dot;
+                        wg_barrier();
+                        float d = 0.0f;
+                        for (int k2 = 0; k2 < nwv; ++k2) d += dred[dpar][k2];
+                        dot = d;
+                        dpar ^= 1u;
+                    }
                     const float err = m.gbias + bu_t + bi_t * cu_t + dot - cur.r;
                     const float gi = err * cu_t;
                     ib -= h.lr_ib * gi;
@@ -1337,12 +1364,14 @@ __global__ __launch_bounds__(256) void k_item_pass(DevModel m, ItemView iv) {
             }
         }
         store_vec<VEC>(m.ifeat + static_cast<size_t>(i) * fs + coff, q);
-        if (lane == 0) m.ibias[i] = ib;
+        if (lane == 0 && lead) m.ibias[i] = ib;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        for (int l = lane; l < B; l += TEAM) m.bit_bin[static_cast<size_t>(i) * B + l] = bt[l];
-        for (int l = lane; l < F; l += TEAM) m.ibias_freq[static_cast<size_t>(i) * F + l] = bq[l];
+        if (lead) {
+            for (int l = lane; l < B; l += TEAM) m.bit_bin[static_cast<size_t>(i) * B + l] = bt[l];
+            for (int l = lane; l < F; l += TEAM) m.ibias_freq[static_cast<size_t>(i) * F + l] = bq[l];
+        }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -1658,14 +1687,19 @@ hipError_t launch_sgd_par(const DevModel& m, const StrataView& sv, int stratum, 
     return hipErrorInvalidValue;
 }
 
-template <bool DAY, int MW>
+template <bool DAY, int MW, int PASS = 0>
 static hipError_t sgd_par_mw(const DevModel& m, const StrataView& sv, int teams, int waves, bool qf, bool p2, bool ord,
                              hipStream_t st) {
     (void)hipGetLastError();
-#define TX_MW(QF, P2, ORD)                                                                                    \
-    hipLaunchKernelGGL((k_sgd_par<64, 16, QF, P2, ORD, 0, DAY, MW>), dim3(teams), dim3(64 * waves), 0, st, m, \
+#define TX_MW(QF, P2, ORD)                                                                                       \
+    hipLaunchKernelGGL((k_sgd_par<64, 16, QF, P2, ORD, PASS, DAY, MW>), dim3(teams), dim3(64 * waves), 0, st, m, \
                        sv, 0)
-    if (ord) {
+    if constexpr (PASS != 0) {
+        if (ord || DAY) return hipErrorInvalidValue;
+        if (qf) { if (p2) TX_MW(true, true, false); else TX_MW(true, false, false); }
+        else { if (p2) TX_MW(false, true, false); else TX_MW(false, false, false); }
+        return hipGetLastError();
+    } else if (ord) {
         if (qf) { if (p2) TX_MW(true, true, true); else TX_MW(true, false, true); }
         else { if (p2) TX_MW(false, true, true); else TX_MW(false, false, true); }
     } else {
@@ -1677,9 +1711,18 @@ static hipError_t sgd_par_mw(const DevModel& m, const StrataView& sv, int teams,
 }
 
 hipError_t launch_sgd_par_mw(const DevModel& m, const StrataView& sv, int teams, int waves, bool qf, bool p2, bool ord,
-                             bool day, hipStream_t st) {
+                             bool day, hipStream_t st, int pass) {
     if (waves < 2 || waves > 8 || m.fs != 1024 * waves || sv.n_strata != 1 || (day && !sv.day_acc))
         return hipErrorInvalidValue;
+    if (pass == 1 || pass == 2) {
+        if (day || ord) return hipErrorInvalidValue;
+        if (waves <= 4)
+            return pass == 1 ? sgd_par_mw<false, 4, 1>(m, sv, teams, waves, qf, p2, false, st)
+                             : sgd_par_mw<false, 4, 2>(m, sv, teams, waves, qf, p2, false, st);
+        return pass == 1 ? sgd_par_mw<false, 8, 1>(m, sv, teams, waves, qf, p2, false, st)
+                         : sgd_par_mw<false, 8, 2>(m, sv, teams, waves, qf, p2, false, st);
+    }
+    if (pass != 0) return hipErrorInvalidValue;
     if (waves <= 4)
         return day ? sgd_par_mw<true, 4>(m, sv, teams, waves, qf, p2, ord, st)
                    : sgd_par_mw<false, 4>(m, sv, teams, waves, qf, p2, ord, st);
@@ -1698,10 +1741,27 @@ hipError_t launch_apply_nsvd(const DevModel& m, float* d_y, float* d_yb, const f
 }
 
 hipError_t launch_item_pass(const DevModel& m, const ItemView& iv, int team, int vec, bool p2, bool qf,
-                            int blocks, hipStream_t st) {
+                            int blocks, hipStream_t st, int waves) {
+    if (waves > 1) {
+        // multi-wave teams (f > 1024): one item per workgroup of 64*waves threads
+        if (team != 64 || vec != 16 || waves > 8 || m.fs != 1024 * waves) return hipErrorInvalidValue;
+        const size_t shm = sizeof(float) * static_cast<size_t>(m.B + m.F) * waves;
+        (void)hipGetLastError();
+#define TX_IPW(P2, QF, MW) \
+    hipLaunchKernelGGL((k_item_pass<64, 16, P2, QF, MW>), dim3(blocks), dim3(64 * waves), shm, st, m, iv)
+        if (waves <= 4) {
+            if (qf) { if (p2) TX_IPW(true, true, 4); else TX_IPW(false, true, 4); }
+            else { if (p2) TX_IPW(true, false, 4); else TX_IPW(false, false, 4); }
+        } else {
+            if (qf) { if (p2) TX_IPW(true, true, 8); else TX_IPW(false, true, 8); }
+            else { if (p2) TX_IPW(true, false, 8); else TX_IPW(false, false, 8); }
+        }
+#undef TX_IPW
+        return hipGetLastError();
+    }
     const size_t shm = sizeof(float) * static_cast<size_t>(m.B + m.F) * (256 / team);
 #define TX_IP(T, V, P2, QF) \
-    hipLaunchKernelGGL((k_item_pass<T, V, P2, QF>), dim3(blocks), dim3(256), shm, st, m, iv)
+    hipLaunchKernelGGL((k_item_pass<T, V, P2, QF, 0>), dim3(blocks), dim3(256), shm, st, m, iv)
 #define X(T, V)                                                                     \
     if (team == T && vec == V) {                                                     \
         (void)hipGetLastError();                                                     \

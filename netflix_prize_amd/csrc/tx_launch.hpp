@@ -15,13 +15,14 @@ hipError_t launch_sgd_serial(const DevModel& m, const int64_t* order, float* scr
 hipError_t launch_sgd_par(const DevModel& m, const StrataView& sv, int stratum, int team, int vec, int blocks,
                           bool qf, bool p2, bool ord, int pass, bool day, hipStream_t st);
 // multi-wave teams (f > 1024): `teams` workgroups of 64*waves threads, one user
-// each, fs = 1024*waves; SGD pass, one stratum
+// each, fs = 1024*waves, one stratum; pass as for launch_sgd_par
 hipError_t launch_sgd_par_mw(const DevModel& m, const StrataView& sv, int teams, int waves, bool qf, bool p2, bool ord,
-                             bool day, hipStream_t st);
+                             bool day, hipStream_t st, int pass = 0);
 hipError_t launch_to_bf16(const float* x, uint16_t* y, int64_t n, hipStream_t st);
 hipError_t launch_apply_nsvd(const DevModel& m, float* d_y, float* d_yb, const float* scale, hipStream_t st);
+// waves > 1: multi-wave teams (f > 1024), one item per block of 64*waves threads
 hipError_t launch_item_pass(const DevModel& m, const ItemView& iv, int team, int vec, bool p2, bool qf,
-                            int blocks, hipStream_t st);
+                            int blocks, hipStream_t st, int waves = 1);
 hipError_t launch_rebuild(const DevModel& m, int64_t n_ud, int team, int vec, hipStream_t st);
 hipError_t launch_predict(const DevModel& m, int mode, int64_t n, const int32_t* us, const int32_t* is,
                           const int16_t* ds, float* o0, float* o1, float* o2, const float* target,

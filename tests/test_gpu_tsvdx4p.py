@@ -82,12 +82,31 @@ def test_tsvdx4p_rejects_unsupported_forms(gpu):
         m.close()
 
 
-@pytest.mark.parametrize("shape", ["tiny_synth", "small"])
-def test_fold_in_epochs_match_oracle(gpu, shape):
+@pytest.mark.parametrize("f", [1500, 4500])
+def test_tsvdx4p_wide_rows_match_oracle(gpu, f):
+    """Tsvdx4pModel epochs at f > 1024 (user and item passes on teams of 2 and
+    5 waves, the lm set's q_f stream on) on the 600 x 150 x 30k set: tables
+    within 1e-4 and holdout RMSE within 1e-5 after 2 epochs."""
+    tr, pr = synthetic("tiny_synth")
+    cfg = lm_config(n_feat=f, n_epochs=2, n_time_bins=30, nsvd_norm_exp=1.0, n_threads=8, lr_yb=0.0002,
+                    seq_epochs=[0, 0, 0, 0, 0], lr_iqf=0.005, reset_u_epoch=1024)
+    m_o = O.OracleModel(cfg, tr, pr)
+    m_g = tx.TxModel(tr, pr, cfg)
+    m_g.set_schedule(mode=tx.SCHED_TSVDX4P)
+    for e in (1, 2):
+        m_o.fit_epoch_p(e, 8)
+        m_g.fit_epoch(epoch=e)
+    _compare(m_o, m_g, 1e-4)
+    assert m_g.calc_rmse(pr) == pytest.approx(m_o.calc_rmse(), abs=1e-5)
+
+
+@pytest.mark.parametrize("shape,f", [("tiny_synth", 24), ("small", 24), ("tiny_synth", 1500)])
+def test_fold_in_epochs_match_oracle(gpu, shape, f):
     """User-only epochs (items and y frozen) after one regular epoch: the GPU
-    fold-in pass against the oracle's user pass without accumulators."""
+    fold-in pass against the oracle's user pass without accumulators (f=1500:
+    two-wave teams)."""
     tr, pr = synthetic(shape)
-    cfg = lm_config(n_feat=24, n_epochs=3, n_time_bins=30)
+    cfg = lm_config(n_feat=f, n_epochs=3, n_time_bins=30)
     m_o = O.OracleModel(cfg, tr, pr)
     m_g = tx.TxModel(tr, pr, cfg)
     m_g.set_schedule(mode=tx.SCHED_SERIAL)
