@@ -11,6 +11,7 @@ import shutil
 import subprocess
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
@@ -60,19 +61,18 @@ def build(verbose: bool = False) -> str:
     os.makedirs(OBJ, exist_ok=True)
     rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
     inc = [f"-I{INCLUDE}", f"-I{CSRC}", f"-I{rocm}/include"]
-    objs = []
-    # host C++ (hip runtime API only): -x c++ keeps it out of device compilation
-    host_o = os.path.join(OBJ, "tx_host.o")
-    dt = _run([hipcc, "-x", "c++", *HOST_FLAGS, "-D__HIP_PLATFORM_AMD__", *inc, "-c",
-               os.path.join(CSRC, "tx_host.cpp"), "-o", host_o])
-    objs.append(host_o)
+    # every object compiles in its own hipcc process, all at once
+    host = lambda src, obj: [hipcc, "-x", "c++", *HOST_FLAGS, "-D__HIP_PLATFORM_AMD__", *inc, "-c",
+                             os.path.join(CSRC, src), "-o", os.path.join(OBJ, obj)]
+    dev = lambda src, obj: [hipcc, *HIP_FLAGS, *inc, "-c", os.path.join(CSRC, src), "-o", os.path.join(OBJ, obj)]
+    jobs = [host("tx_host.cpp", "tx_host.o"), host("knn_host.cpp", "knn_host.o"),
+            dev("tx_kernels.hip", "tx_kernels.o"), dev("knn_kernels.hip", "knn_kernels.o")]
+    with ThreadPoolExecutor(max_workers=len(jobs)) as ex:
+        times = list(ex.map(_run, jobs))
     if verbose:
-        print(f"tx_host.cpp {dt:.1f}s")
-    kern_o = os.path.join(OBJ, "tx_kernels.o")
-    dt = _run([hipcc, *HIP_FLAGS, *inc, "-c", os.path.join(CSRC, "tx_kernels.hip"), "-o", kern_o])
-    objs.append(kern_o)
-    if verbose:
-        print(f"tx_kernels.hip {dt:.1f}s")
+        for cmd, dt in zip(jobs, times):
+            print(f"{os.path.basename(cmd[-3])} {dt:.1f}s")
+    objs = [cmd[-1] for cmd in jobs]
     tmp = LIB + ".tmp"
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread"])
     os.replace(tmp, LIB)

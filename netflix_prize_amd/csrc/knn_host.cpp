@@ -1,0 +1,184 @@
+// Host side of the GPU KnnfModel: the C ABI declared in include/tsvd_knn.h.
+//
+// knnf_create restates KnnfModel::new_with_factors (src/knnf.rs:60-90): the
+// factor table goes to the device zero-padded to a multiple of 64 rows and 32
+// columns, row norms and the item-item similarity max(0, cos) are computed
+// there (knn_kernels.hip), and each user's training ratings (item, residual,
+// day) are kept as CSR arrays.  knnf_predict runs KnnfModel::predict for a
+// batch of queries.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "knn_launch.hpp"
+#include "tsvd_knn.h"
+
+namespace tx {
+void set_error(const std::string& msg);
+void clear_error();
+}  // namespace tx
+
+struct knnf_ctx {
+    knnf_config cfg{};
+    int32_t n_items = 0, n_users = 0;
+    int ip = 0, fp = 0;
+    hipStream_t st = nullptr;
+    std::vector<void*> allocs;
+    knn::KnnView v{};
+    ~knnf_ctx() {
+        for (void* p : allocs) (void)hipFree(p);
+        if (st) (void)hipStreamDestroy(st);
+    }
+    template <typename T>
+    hipError_t alloc(T** p, size_t n) {
+        void* q = nullptr;
+        hipError_t e = hipMalloc(&q, sizeof(T) * (n > 0 ? n : 1));
+        if (e == hipSuccess) {
+            allocs.push_back(q);
+            *p = static_cast<T*>(q);
+        }
+        return e;
+    }
+};
+
+namespace {
+
+int fail(int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    tx::set_error(buf);
+    return code;
+}
+
+#define HIPC(expr)                                                                              \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess) return fail(TX_EDEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+}  // namespace
+
+extern "C" {
+
+int knnf_create(const knnf_config* cfg, const float* factors, int32_t n_items, int32_t n_feat,
+                const tx_ratings* train, knnf_ctx** out) {
+    tx::clear_error();
+    if (!cfg || !factors || !train || !out) return fail(TX_EINVAL, "knnf_create: NULL argument");
+    if (cfg->k < 1) return fail(TX_EINVAL, "knnf_create: k must be >= 1");
+    if (n_items < 1 || n_feat < 1) return fail(TX_EINVAL, "knnf_create: empty factor table");
+    if (train->n_items > n_items)
+        return fail(TX_EINVAL, "knnf_create: factor table has %d rows for %d items", n_items, train->n_items);
+    const int64_t N = train->n_ratings;
+    const int32_t U = train->n_users;
+    auto c = std::make_unique<knnf_ctx>();
+    c->cfg = *cfg;
+    c->n_items = n_items;
+    c->n_users = U;
+    c->ip = (n_items + 63) / 64 * 64;
+    c->fp = (n_feat + 31) / 32 * 32;
+    if (static_cast<int64_t>(c->ip) * c->ip > (int64_t{1} << 34))
+        return fail(TX_EUNSUPPORTED, "knnf_create: %d items exceed the similarity matrix budget", n_items);
+
+    // per-user CSR of the training ratings (user-sorted input, as the reference asserts)
+    std::vector<uint32_t> off(static_cast<size_t>(U) + 1, 0);
+    for (int64_t t = 0; t < N; ++t) {
+        const int32_t u = train->user_idxs[t];
+        if (u < 0 || u >= U) return fail(TX_EINVAL, "knnf_create: user index %d out of range", u);
+        if (t > 0 && u < train->user_idxs[t - 1]) return fail(TX_EINVAL, "knnf_create: training rows not user-sorted");
+        const int32_t i = train->item_idxs[t];
+        if (i < 0 || i >= n_items) return fail(TX_EINVAL, "knnf_create: item index %d out of range", i);
+        off[u + 1]++;
+    }
+    for (int32_t u = 0; u < U; ++u) off[u + 1] += off[u];
+    std::vector<float> day(static_cast<size_t>(N));
+    for (int64_t t = 0; t < N; ++t) day[t] = static_cast<float>(train->dates[t]);
+
+    HIPC(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+    // padded factors
+    std::vector<float> F(static_cast<size_t>(c->ip) * c->fp, 0.0f);
+    for (int32_t i = 0; i < n_items; ++i)
+        for (int32_t k = 0; k < n_feat; ++k) F[static_cast<size_t>(i) * c->fp + k] = factors[static_cast<size_t>(i) * n_feat + k];
+    float *dF = nullptr, *dn = nullptr, *dS = nullptr, *dres = nullptr, *dday = nullptr;
+    uint32_t* doff = nullptr;
+    int32_t* ditem = nullptr;
+    HIPC(c->alloc(&dF, F.size()));
+    HIPC(c->alloc(&dn, static_cast<size_t>(c->ip)));
+    HIPC(c->alloc(&dS, static_cast<size_t>(c->ip) * c->ip));
+    HIPC(c->alloc(&doff, off.size()));
+    HIPC(c->alloc(&ditem, static_cast<size_t>(N)));
+    HIPC(c->alloc(&dres, static_cast<size_t>(N)));
+    HIPC(c->alloc(&dday, static_cast<size_t>(N)));
+    HIPC(hipMemcpyAsync(dF, F.data(), sizeof(float) * F.size(), hipMemcpyHostToDevice, c->st));
+    HIPC(hipMemcpyAsync(doff, off.data(), sizeof(uint32_t) * off.size(), hipMemcpyHostToDevice, c->st));
+    if (N > 0) {
+        HIPC(hipMemcpyAsync(ditem, train->item_idxs, sizeof(int32_t) * N, hipMemcpyHostToDevice, c->st));
+        HIPC(hipMemcpyAsync(dres, train->residuals, sizeof(float) * N, hipMemcpyHostToDevice, c->st));
+        HIPC(hipMemcpyAsync(dday, day.data(), sizeof(float) * N, hipMemcpyHostToDevice, c->st));
+    }
+    HIPC(knn::launch_norms(dF, c->fp, c->ip, dn, c->st));
+    HIPC(knn::launch_cosine(dF, c->fp, dn, dS, c->ip, c->st));
+    HIPC(hipStreamSynchronize(c->st));
+    c->v = knn::KnnView{dS, c->ip, doff, ditem, dres, dday, cfg->k, cfg->scaling, cfg->tau};
+    *out = c.release();
+    return 0;
+}
+
+int knnf_predict(knnf_ctx* c, int64_t n, const int32_t* users, const int32_t* items, const int16_t* days, float* out) {
+    tx::clear_error();
+    if (!c || (n > 0 && (!users || !items || !days || !out))) return fail(TX_EINVAL, "knnf_predict: NULL argument");
+    for (int64_t q = 0; q < n; ++q) {
+        if (users[q] < 0 || users[q] >= c->n_users) return fail(TX_EINVAL, "knnf_predict: user %d out of range", users[q]);
+        if (items[q] < 0 || items[q] >= c->n_items) return fail(TX_EINVAL, "knnf_predict: item %d out of range", items[q]);
+    }
+    const int64_t chunk = int64_t{1} << 22;
+    int32_t *du = nullptr, *di = nullptr;
+    int16_t* dd = nullptr;
+    float* dout = nullptr;
+    const size_t cap = static_cast<size_t>(n < chunk ? (n > 0 ? n : 1) : chunk);
+    HIPC(hipMalloc(&du, sizeof(int32_t) * cap));
+    std::unique_ptr<void, void (*)(void*)> g1(du, [](void* p) { (void)hipFree(p); });
+    HIPC(hipMalloc(&di, sizeof(int32_t) * cap));
+    std::unique_ptr<void, void (*)(void*)> g2(di, [](void* p) { (void)hipFree(p); });
+    HIPC(hipMalloc(&dd, sizeof(int16_t) * cap));
+    std::unique_ptr<void, void (*)(void*)> g3(dd, [](void* p) { (void)hipFree(p); });
+    HIPC(hipMalloc(&dout, sizeof(float) * cap));
+    std::unique_ptr<void, void (*)(void*)> g4(dout, [](void* p) { (void)hipFree(p); });
+    for (int64_t q0 = 0; q0 < n; q0 += chunk) {
+        const int64_t m = n - q0 < chunk ? n - q0 : chunk;
+        HIPC(hipMemcpyAsync(du, users + q0, sizeof(int32_t) * m, hipMemcpyHostToDevice, c->st));
+        HIPC(hipMemcpyAsync(di, items + q0, sizeof(int32_t) * m, hipMemcpyHostToDevice, c->st));
+        HIPC(hipMemcpyAsync(dd, days + q0, sizeof(int16_t) * m, hipMemcpyHostToDevice, c->st));
+        HIPC(knn::launch_knnf_predict(c->v, m, du, di, dd, dout, c->st));
+        HIPC(hipMemcpyAsync(out + q0, dout, sizeof(float) * m, hipMemcpyDeviceToHost, c->st));
+        HIPC(hipStreamSynchronize(c->st));
+    }
+    return 0;
+}
+
+int knnf_similarity(knnf_ctx* c, int64_t n, const int32_t* i, const int32_t* j, float* out) {
+    tx::clear_error();
+    if (!c || (n > 0 && (!i || !j || !out))) return fail(TX_EINVAL, "knnf_similarity: NULL argument");
+    for (int64_t q = 0; q < n; ++q) {
+        if (i[q] < 0 || i[q] >= c->n_items || j[q] < 0 || j[q] >= c->n_items)
+            return fail(TX_EINVAL, "knnf_similarity: item out of range");
+        HIPC(hipMemcpyAsync(out + q, c->v.S + static_cast<size_t>(i[q]) * c->ip + j[q], sizeof(float),
+                            hipMemcpyDeviceToHost, c->st));
+    }
+    HIPC(hipStreamSynchronize(c->st));
+    return 0;
+}
+
+int knnf_destroy(knnf_ctx* c) {
+    tx::clear_error();
+    delete c;
+    return 0;
+}
+
+}  // extern "C"

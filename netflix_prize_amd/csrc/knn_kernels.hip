@@ -1,0 +1,241 @@
+// KnnfModel on the GPU (src/knnf.rs): the item-item cosine similarity matrix
+// of a factor table by FP32 MFMA, and the per-query temporal top-k weighted
+// mean of the user's residuals.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "knn_launch.hpp"
+
+namespace knn {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// S = max(0, F F^T / (n_i n_j)) over the padded Ip x Ip matrix (rows of F are
+// items, fp >= f columns zero-padded, Ip a multiple of 64).  256 threads =
+// 4 waves per 64 x 64 tile of S, each wave a 32 x 32 quarter as 2 x 2 MFMA
+// 16 x 16 x 4 blocks; K advances 32 at a time through LDS.  Operand layout of
+// v_mfma_f32_16x16x4f32 (wave64, lane l): A[l%16][l/16], B[l/16][l%16],
+// result D[4*(l/16)+e][l%16].  A zero norm gives similarity 0, as
+// KnnfModel::sim (src/knnf.rs:46-52).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_cosine(const float* __restrict__ F, int fp, const float* __restrict__ norm,
+                                                float* __restrict__ S, int ip) {
+    constexpr int BM = 64, BK = 32;
+    __shared__ float As[BM][BK + 1];
+    __shared__ float Bs[BM][BK + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BM;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    v4f acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k0 = 0; k0 < fp; k0 += BK) {
+#pragma unroll
+        for (int r = 0; r < (BM * BK) / 256; ++r) {
+            const int e = tid + 256 * r;
+            const int k = e % BK, m = e / BK;  // consecutive threads: consecutive columns of a row
+            As[m][k] = F[static_cast<size_t>(m0 + m) * fp + k0 + k];
+            Bs[m][k] = F[static_cast<size_t>(n0 + m) * fp + k0 + k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < BK; ks += 4) {
+            float a[2], b[2];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                a[t] = As[wm + t * 16 + (lane & 15)][ks + (lane >> 4)];
+                b[t] = Bs[wn + t * 16 + (lane & 15)][ks + (lane >> 4)];
+            }
+#pragma unroll
+            for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+                for (int tj = 0; tj < 2; ++tj)
+                    acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[ti], b[tj], acc[ti][tj], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int r = m0 + wm + ti * 16 + 4 * (lane >> 4) + e;
+                const int c = n0 + wn + tj * 16 + (lane & 15);
+                const float nr = norm[r], nc = norm[c];
+                float v = 0.0f;
+                if (nr != 0.0f && nc != 0.0f) {
+                    v = acc[ti][tj][e] / (nr * nc);
+                    v = v > 0.0f ? v : 0.0f;
+                }
+                S[static_cast<size_t>(r) * ip + c] = v;
+            }
+}
+
+// Row norms sqrt(sum f^2), one wave per row.
+__global__ __launch_bounds__(256) void k_norms(const float* __restrict__ F, int fp, int rows, float* __restrict__ norm) {
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+    if (wave >= rows) return;
+    float s = 0.0f;
+    for (int k = lane; k < fp; k += 64) {
+        const float x = F[static_cast<size_t>(wave) * fp + k];
+        s += x * x;
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) norm[wave] = __builtin_sqrtf(s);
+}
+
+// ---------------------------------------------------------------------------
+// KnnfModel::predict (src/knnf.rs:101-141), one wave per query (u, i, day).
+// Candidates are the user's training ratings t (item j, residual r, day d_j)
+// with j != i and weight w = sim(i, j)^scaling / (1 + tau |day - d_j|) > 0;
+// the k largest weights are kept and the prediction is sum w r / sum w (0
+// without candidates).  The k-th largest weight is found exactly by a radix
+// select over the weights' bit patterns (positive floats order as unsigned
+// integers), 8 bits per pass through a 256-bin LDS histogram; weights equal
+// to it are taken in rating order until k are kept (the reference's unstable
+// selection takes any of them).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float knn_weight(const KnnView& v, const float* srow, int32_t i, float day, uint32_t t) {
+    const int32_t j = v.t_item[t];
+    if (j == i) return 0.0f;
+    float sim = srow[j];
+    if (v.scaling != 1.0f) sim = __builtin_powf(sim, v.scaling);
+    const float dt = __builtin_fabsf(day - v.t_day[t]);
+    return sim / (1.0f + v.tau * dt);
+}
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t x) {
+    for (int o = 32; o > 0; o >>= 1) x += static_cast<uint32_t>(__shfl_xor(static_cast<int>(x), o, 64));
+    return x;
+}
+
+__device__ __forceinline__ float wave_sum_f32(float x) {
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+
+__global__ __launch_bounds__(256) void k_knnf_predict(KnnView v, int64_t n, const int32_t* __restrict__ us,
+                                                      const int32_t* __restrict__ is, const int16_t* __restrict__ ds,
+                                                      float* __restrict__ out) {
+    __shared__ uint32_t hist_all[4][256];
+    const int lane = threadIdx.x & 63, wslot = threadIdx.x >> 6;
+    uint32_t* hist = hist_all[wslot];
+    const int64_t nwaves = static_cast<int64_t>(gridDim.x) * (blockDim.x >> 6);
+    for (int64_t q = static_cast<int64_t>(blockIdx.x) * (blockDim.x >> 6) + wslot; q < n; q += nwaves) {
+        const int32_t u = us[q], i = is[q];
+        const float day = static_cast<float>(ds[q]);
+        const uint32_t t0 = v.t_off[u], t1 = v.t_off[u + 1];
+        const float* srow = v.S + static_cast<size_t>(i) * v.ip;
+        // positive candidates
+        uint32_t m = 0;
+        for (uint32_t t = t0 + lane; t < t1; t += 64) m += knn_weight(v, srow, i, day, t) > 0.0f ? 1u : 0u;
+        m = wave_sum_u32(m);
+        uint32_t thr = 0, take_eq = 0xffffffffu;  // keep w > thr, and the first take_eq weights == thr
+        if (m > static_cast<uint32_t>(v.k)) {
+            uint32_t prefix = 0, mask = 0, krem = static_cast<uint32_t>(v.k);
+            for (int shift = 24; shift >= 0; shift -= 8) {
+                for (int b = lane; b < 256; b += 64) hist[b] = 0;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                for (uint32_t t = t0 + lane; t < t1; t += 64) {
+                    const float w = knn_weight(v, srow, i, day, t);
+                    const uint32_t bits = __float_as_uint(w);
+                    if (w > 0.0f && (bits & mask) == prefix) atomicAdd(&hist[(bits >> shift) & 255u], 1u);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                // lane l holds bins 255-4l .. 252-4l (descending); inclusive scan in lane order
+                uint32_t c[4], own = 0;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    c[e] = hist[255 - 4 * lane - e];
+                    own += c[e];
+                }
+                uint32_t incl = own;
+                for (int o = 1; o < 64; o <<= 1) {
+                    const uint32_t up = static_cast<uint32_t>(__shfl_up(static_cast<int>(incl), o, 64));
+                    if (lane >= o) incl += up;
+                }
+                const uint64_t hit = __ballot(incl >= krem);
+                const int cl = __builtin_ctzll(hit);  // first lane whose bins reach the k-th weight
+                uint32_t digit = 0, above = 0;
+                if (lane == cl) {
+                    uint32_t cum = incl - own;
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        if (cum + c[e] >= krem) {
+                            digit = 255u - 4u * static_cast<uint32_t>(lane) - static_cast<uint32_t>(e);
+                            above = cum;
+                            break;
+                        }
+                        cum += c[e];
+                    }
+                }
+                digit = static_cast<uint32_t>(__shfl(static_cast<int>(digit), cl, 64));
+                above = static_cast<uint32_t>(__shfl(static_cast<int>(above), cl, 64));
+                krem -= above;
+                prefix |= digit << shift;
+                mask |= 255u << shift;
+            }
+            thr = prefix;
+            take_eq = krem;
+        }
+        // weighted mean over the kept candidates
+        float num = 0.0f, den = 0.0f;
+        uint32_t eq_seen = 0;
+        for (uint32_t tb = t0; tb < t1; tb += 64) {
+            const uint32_t t = tb + lane;
+            float w = 0.0f, r = 0.0f;
+            if (t < t1) {
+                w = knn_weight(v, srow, i, day, t);
+                r = v.t_res[t];
+            }
+            const uint32_t bits = __float_as_uint(w);
+            bool keep = w > 0.0f && (m <= static_cast<uint32_t>(v.k) || bits > thr);
+            const bool eq = w > 0.0f && m > static_cast<uint32_t>(v.k) && bits == thr;
+            const uint64_t eqm = __ballot(eq);
+            const uint32_t rank = eq_seen + static_cast<uint32_t>(__builtin_popcountll(eqm & ((1ull << lane) - 1ull)));
+            if (eq && rank < take_eq) keep = true;
+            eq_seen += static_cast<uint32_t>(__builtin_popcountll(eqm));
+            if (keep) {
+                num += w * r;
+                den += w;
+            }
+        }
+        num = wave_sum_f32(num);
+        den = wave_sum_f32(den);
+        if (lane == 0) out[q] = m == 0 ? 0.0f : num / den;
+    }
+}
+
+hipError_t launch_norms(const float* F, int fp, int rows, float* norm, hipStream_t st) {
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(k_norms, dim3((rows + 3) / 4), dim3(256), 0, st, F, fp, rows, norm);
+    return hipGetLastError();
+}
+
+hipError_t launch_cosine(const float* F, int fp, const float* norm, float* S, int ip, hipStream_t st) {
+    if (ip % 64 != 0 || fp % 32 != 0) return hipErrorInvalidValue;
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(k_cosine, dim3(ip / 64, ip / 64), dim3(256), 0, st, F, fp, norm, S, ip);
+    return hipGetLastError();
+}
+
+hipError_t launch_knnf_predict(const KnnView& v, int64_t n, const int32_t* us, const int32_t* is, const int16_t* ds,
+                               float* out, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    int64_t blocks = (n + 3) / 4;
+    if (blocks > 65536) blocks = 65536;
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(k_knnf_predict, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, v, n, us, is, ds, out);
+    return hipGetLastError();
+}
+
+}  // namespace knn

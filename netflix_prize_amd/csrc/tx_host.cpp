@@ -39,6 +39,16 @@ int fail(int code, const char* fmt, ...) {
     return code;
 }
 
+}  // namespace
+
+// The library's last-error string, shared with the other C-ABI sources.
+namespace tx {
+void set_error(const std::string& msg) { g_err = msg; }
+void clear_error() { g_err.clear(); }
+}  // namespace tx
+
+namespace {
+
 #define HIPC(expr)                                                                          \
     do {                                                                                    \
         hipError_t e_ = (expr);                                                             \

@@ -1,0 +1,66 @@
+"""The knnf oracle (oracle/knnf_ref.py) pinned on the CPU: the reference's KAT
+and closed-form cases that each fail for a plausible mistake (decay sign or
+form, the j == i skip, the cosine clamp, top-k direction, the weighted mean)."""
+import numpy as np
+import pytest
+
+from oracle.knnf_ref import KnnfRef, sim_matrix
+from tests.fixtures import dataset_from_rows, tiny
+
+# tests/regressors_test.rs:403-422 (knnf_probe_rmse_regression)
+KNNF_KAT_RMSE = 1.9509877627957246
+
+
+def kat_factors():
+    i, k = np.meshgrid(np.arange(10), np.arange(4), indexing="ij")
+    return (((i * 13 + k * 7 + 3) % 17).astype(np.float32) / np.float32(17.0) - np.float32(0.5)).astype(np.float32)
+
+
+def rmse(pred, ds):
+    e = np.asarray(pred, np.float64) - np.asarray(ds.residuals, np.float64)
+    return float(np.sqrt(np.mean(e[np.asarray(ds.is_test) == 0] ** 2)))
+
+
+def test_oracle_reproduces_reference_kat():
+    _, tr, pr = tiny()
+    ref = KnnfRef(tr, kat_factors(), k=3, scaling=1.0, tau=0.025)
+    r = rmse(ref.predict(pr.user_idxs, pr.item_idxs, pr.dates), pr)
+    assert r == pytest.approx(KNNF_KAT_RMSE, abs=1e-6), r
+
+
+def _one_user(rows, n_items):
+    return dataset_from_rows([(0, i, r, d) for i, r, d in rows], 1, n_items)
+
+
+def test_weighted_mean_with_temporal_decay():
+    # item 0 at 60 degrees from item 1 (cos 0.5) and parallel to item 2 (cos 1)
+    F = np.array([[1.0, 0.0], [0.5, np.sqrt(3) / 2], [2.0, 0.0]], np.float32)
+    tr = _one_user([(1, 4, 100), (2, 1, 140)], 3)
+    ref = KnnfRef(tr, F, k=25, scaling=1.0, tau=0.025)
+    # day 100: w1 = 0.5 / 1, w2 = 1 / (1 + 0.025 * 40) = 0.5
+    assert ref.predict_one(0, 0, 100.0) == pytest.approx((0.5 * 4 + 0.5 * 1) / 1.0, abs=1e-6)
+    # day 140: w1 = 0.5 / 2, w2 = 1
+    assert ref.predict_one(0, 0, 140.0) == pytest.approx((0.25 * 4 + 1.0 * 1) / 1.25, abs=1e-6)
+    # k = 1 keeps the larger weight only
+    assert KnnfRef(tr, F, k=1).predict_one(0, 0, 140.0) == pytest.approx(1.0)
+    assert KnnfRef(tr, F, k=1).predict_one(0, 0, 100.0 - 1000.0) == pytest.approx(1.0)
+    # scaling 2 squares the cosine: w1 = 0.25, w2 = 0.5 at day 100
+    assert KnnfRef(tr, F, k=25, scaling=2.0).predict_one(0, 0, 100.0) == pytest.approx((0.25 * 4 + 0.5) / 0.75)
+
+
+def test_self_item_skipped_and_negative_cosine_clamped():
+    F = np.array([[1.0, 0.0], [-1.0, 0.1], [0.0, 1.0]], np.float32)
+    tr = _one_user([(0, 5, 10), (1, 3, 10), (2, 2, 10)], 3)
+    ref = KnnfRef(tr, F)
+    # querying item 0: item 0 itself is skipped, item 1 has cos < 0, item 2 cos 0 -> no candidate
+    assert ref.predict_one(0, 0, 10.0) == 0.0
+    S = sim_matrix(F)
+    assert S[0, 1] == 0.0 and S[0, 0] == pytest.approx(1.0)
+
+
+def test_constant_residuals_give_the_constant():
+    rng = np.random.default_rng(0)
+    F = np.abs(rng.normal(size=(30, 8))).astype(np.float32)
+    tr = _one_user([(j, 3, int(d)) for j, d in zip(range(1, 30), rng.integers(0, 2000, 29))], 30)
+    ref = KnnfRef(tr, F, k=7)
+    assert ref.predict_one(0, 0, 1000.0) == pytest.approx(3.0, abs=1e-12)
