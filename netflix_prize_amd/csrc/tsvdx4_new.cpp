@@ -34,6 +34,7 @@
 #include <vector>
 
 #include "npy.hpp"
+#include "tsvd_knn.h"
 #include "tsvd_tx.h"
 
 namespace {
@@ -537,6 +538,55 @@ void fit2(const tx_config& cfg, const std::string& target, const std::string& jo
     }
 }
 
+// `<base>__knnf` jobs (run_knnf, src/bin/tsvdx4-new.rs:55-57, 576-588 and
+// src/bin/tsvdx4p-new.rs:24, 90-105): fit2 of a KnnfModel with the default
+// KnnfConfig over the base job's saved item factors
+// `{preds}/{base}.ifeat.{train set}.npy`, target `1.0*{base}`.  KnnfModel has
+// no epochs (Regressor::n_epochs() = 0), subscores or artifacts.
+void knnf_fit(const std::string& base, const std::string& target, const std::string& tr_set,
+              const std::string& pr_set, const std::string& job, const std::string& preds_dir, bool phase1) {
+    tee("%s => %s\n", tr_set.c_str(), pr_set.c_str());
+    Dataset tr = load_dataset(tr_set, target, preds_dir);
+    Dataset pr = load_dataset(pr_set, target, preds_dir);
+    const std::string fpath = preds_dir + "/" + base + ".ifeat." + tr_set + ".npy";
+    npy::Array fa;
+    try {
+        fa = npy::read(fpath);
+    } catch (const std::exception& e) {
+        panic(e.what());
+    }
+    if (fa.shape.size() != 2) panic(fpath + ": expected a 2-D factor table");
+    const std::vector<float> F = npy::as<float>(fa, fpath);
+    const knnf_config kc{25, 1.0f, 0.025f};  // KnnfConfig::with_factors (src/knnf.rs:23-26)
+    knnf_ctx* ctx = nullptr;
+    const tx_ratings t = tr.view(false);
+    check(knnf_create(&kc, F.data(), static_cast<int32_t>(fa.shape[0]), static_cast<int32_t>(fa.shape[1]), &t, &ctx),
+          "KnnfModel::new");
+    std::vector<float> p(pr.n_ratings());
+    check(knnf_predict(ctx, pr.n_ratings(), pr.user_idxs.data(), pr.item_idxs.data(), pr.dates.data(), p.data()),
+          "predict");
+    knnf_destroy(ctx);
+    for (int64_t k = 0; k < pr.n_ratings(); ++k) p[k] = p[k] + (static_cast<float>(pr.raw_ratings[k]) - pr.residuals[k]);
+    if (phase1) tee("%s RMSE %.5f\n", pr_set.c_str(), rmse_from_preds(p, pr));
+    write_preds(preds_dir + "/" + job + "." + pr_set + ".npy", p);
+}
+
+void fit2_knnf(const std::string& job, const std::string& base, const Split& sp) {
+    const std::string preds_dir = sp.preds_dir;
+    mkdir(preds_dir.c_str(), 0755);
+    g_log = std::fopen((preds_dir + "/" + job + ".out").c_str(), "w");
+    if (!g_log) panic(preds_dir + "/" + job + ".out: cannot create");
+    const std::string target = "1.0*" + base;
+    tee("[%s]\n", job.c_str());
+    tee("target = \"%s\"\n", target.c_str());
+    tee("KnnfConfig { factors: \"%s/%s.ifeat\", k: 25, scaling: 1.0, tau: 0.025 }\n", preds_dir.c_str(), base.c_str());
+    knnf_fit(base, target, sp.tr, sp.pr, job, preds_dir, true);
+    knnf_fit(base, target, sp.fulltrain_tr, sp.fulltrain_pr, job, preds_dir, false);
+    tee("\n");
+    std::fclose(g_log);
+    g_log = nullptr;
+}
+
 // epoch_blend_apply (src/lib.rs:1181-1235)
 void epoch_blend_apply(const std::string& model, const std::vector<double>& w, double offset, const Split& sp) {
     const std::string preds_dir = sp.preds_dir;
@@ -831,7 +881,15 @@ int main(int argc, char** argv) {
         epoch_blend_apply(job.substr(0, job.size() - std::string("__epochs").size()), w, off, SPLIT_NEW);
         return 0;
     }
+    static const char* kKnnf[] = {"tsvdx4-16-t__knnf", "tsvdx4-400lm__knnf", "tsvdx4-4800lm__knnf",
+                                  "tsvdx4p-8000lm__knnf"};
+    for (const char* name : kKnnf) {
+        if (job != name) continue;
+        if (print_only) return 0;
+        fit2_knnf(job, job.substr(0, job.size() - std::string("__knnf").size()), SPLIT_NEW);
+        return 0;
+    }
     if (job.find("__knn") != std::string::npos)
-        panic("job " + job + ": kNN post-processing chains are outside the tsvdx4 path of this build");
+        panic("job " + job + ": this kNN chain (knn3 / knns / knn-d) is outside this build");
     panic("invalid job name: " + job);
 }

@@ -95,6 +95,29 @@ def test_job_end_to_end(gpu, tmp_path):
     # better than the best constant prediction of the held-out ratings
     assert rmse < float(np.std(pr.raw_ratings.astype(np.float64)))
 
+    # the job's kNN chain over its saved item factors (run_knnf, src/bin/tsvdx4-new.rs:576-588):
+    # predictions = base job's + KnnfModel on the base job's residuals, equal to the oracle's
+    knn_job = job + "__knnf"
+    out = subprocess.run([DRIVER, knn_job], cwd=tmp_path, env=env, capture_output=True, text=True, timeout=600)
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
+    kp = np.load(pd / f"{knn_job}.probex.npy")
+    assert kp.shape == (pr.n_ratings,) and np.all(np.isfinite(kp))
+    assert np.load(pd / f"{knn_job}.qual.npy").shape == (pr.n_ratings,)
+    klog = (pd / f"{knn_job}.out").read_text()
+    assert f'target = "1.0*{job}"' in klog and "KnnfConfig { factors:" in klog
+    klogged = float(re.search(r"probex RMSE ([0-9.]+)", klog).group(1))
+    krmse = float(np.sqrt(np.mean((kp.astype(np.float64) - pr.raw_ratings) ** 2)))
+    assert abs(klogged - krmse) < 1e-5, (klogged, krmse)
+    from oracle.knnf_ref import KnnfRef
+    res_tr = tr.raw_ratings.astype(np.float32) - np.load(pd / f"{job}.trainx.npy")
+    tr_k = tx.Dataset.from_arrays(tr.user_idxs, tr.item_idxs, tr.raw_ratings, tr.dates, tr.n_users, tr.n_items,
+                                  residuals=res_tr)
+    ref = KnnfRef(tr_k, np.load(pd / f"{job}.ifeat.trainx.npy"))
+    q = np.arange(0, pr.n_ratings, 50)
+    want = ref.predict(pr.user_idxs[q], pr.item_idxs[q], pr.dates[q]) + probe[q]
+    clear = ref.margins(pr.user_idxs[q], pr.item_idxs[q], pr.dates[q]) > 1e-4
+    assert np.max(np.abs(kp[q][clear] - want[clear])) < 1e-4
+
 
 @pytest.mark.gpu
 def test_tsvdx4p_job_end_to_end(gpu, tmp_path):
