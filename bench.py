@@ -138,6 +138,10 @@ def main(argv=None) -> int:
     ap.add_argument("--autotune", type=int, default=1, help="autotune the batched form of the time-to-RMSE run")
     ap.add_argument("--fold-in-users", type=int, default=80_189,
                     help="users held out of training and folded in afterwards (0 = skip)")
+    ap.add_argument("--knnf", type=int, default=1,
+                    help="time KnnfModel (similarity matrix + held-out predictions) over the model's item factors")
+    ap.add_argument("--knnf-f", type=int, default=0,
+                    help="use random item factors of this width instead of the trained model's (0 = trained)")
     ap.add_argument("--day-epochs", type=int, default=3,
                     help="epochs of the tsvdx4-60 job's model (day terms, f=60) in the parallel schedule (0 = skip)")
     args = ap.parse_args(argv)
@@ -244,6 +248,30 @@ def main(argv=None) -> int:
         log(f"bench: fold-in of {n_new} new users (encoder vs random start)")
         fold = E.evaluate_new_users(tr, pr, cfg, spec.n_users - n_new, args.ttr_epochs or 20, 5, 20, log=log)
 
+    # KnnfModel (the *__knnf jobs) over the trained model's item factors: the
+    # cosine similarity matrix by FP32 MFMA at create, then every held-out query
+    knnf = None
+    if args.knnf:
+        from netflix_prize_amd import knn
+        if args.knnf_f > 0:
+            F = np.random.default_rng(0).normal(size=(tr.n_items, args.knnf_f)).astype(np.float32)
+        else:
+            F = model.get("ifeat").reshape(tr.n_items, cfg.n_feat)
+        t0 = time.perf_counter()
+        km = knn.KnnfModel(tr, F)
+        t_create = time.perf_counter() - t0
+        km.predict(pr.user_idxs, pr.item_idxs, pr.dates)
+        t0 = time.perf_counter()
+        km.predict(pr.user_idxs, pr.item_idxs, pr.dates)
+        t_pred = time.perf_counter() - t0
+        km.close()
+        ip, fp = (tr.n_items + 63) // 64 * 64, (F.shape[1] + 31) // 32 * 32
+        knnf = {"workload": f"KnnfModel k=25 over {tr.n_items} x {F.shape[1]} item factors "
+                            f"({'random' if args.knnf_f else 'trained'}), {pr.n_ratings} held-out queries",
+                "create_s": round(t_create, 4), "cosine_gflop": round(2.0 * ip * ip * fp / 1e9, 1),
+                "predict_s": round(t_pred, 4), "queries_per_s": round(pr.n_ratings / t_pred, 1)}
+        log(f"bench: knnf {knnf}")
+
     # the tsvdx4-60 job's model (lm set with day terms, low_memory false, f=60,
     # 29 time bins, seed 42) in the default parallel schedule: epoch times and
     # the holdout RMSE against the serial oracle's after the same epochs
@@ -337,6 +365,7 @@ def main(argv=None) -> int:
         "time_to_rmse": ttr,
         "fold_in": fold,
         "day_terms": day,
+        "knnf": knnf,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
