@@ -1,8 +1,9 @@
-/* C ABI of the GPU KnnfModel (libtsvd_tx.so): item-item kNN over a factor
- * table produced by an MF-family model (the `.ifeat` a tsvdx4 job saves),
- * with temporal decay at predict time.  Drop-in for KnnfModel
- * (src/knnf.rs:1-144) behind the reference's Regressor trait; used by the
- * `*__knnf` jobs (src/bin/tsvdx4-new.rs:55-57, src/bin/tsvdx4p-new.rs:24).
+/* C ABI of the GPU KnnfModel and KnnsModel (libtsvd_tx.so): item-item kNN
+ * with temporal decay at predict time, over the cosine of a factor table (the
+ * `.ifeat` a tsvdx4 job saves; KnnfModel, src/knnf.rs:1-144) or over the
+ * shrunk lift of item co-occurrence (KnnsModel, src/knns.rs:1-149).  Drop-ins
+ * behind the reference's Regressor trait for the `*__knnf` and `*__knns` jobs
+ * (src/bin/tsvdx4-new.rs:49-57, src/bin/tsvdx4p-new.rs:24).
  *
  * Conventions as include/tsvd_tx.h: calls return 0 or a negative TX_E* code
  * with tx_last_error() describing it; host buffers are copied, never kept. */
@@ -38,6 +39,28 @@ int knnf_create(const knnf_config* cfg, const float* factors, int32_t n_items, i
 /* KnnfModel::predict (src/knnf.rs:101-141) for n queries, out[n] on the host. */
 int knnf_predict(knnf_ctx* ctx, int64_t n, const int32_t* users, const int32_t* items, const int16_t* days,
                  float* out);
+
+/* KnnsConfig (src/knns.rs:19-43) with SuppSource::Compute, the source the
+ * reference's jobs use (src/bin/tsvdx4-new.rs:563-574). */
+typedef struct knns_config {
+    int32_t k;
+    float shrinkage;
+    float scaling;
+    float tau;
+} knns_config;
+
+/* KnnsModel::new with SuppSource::Compute (src/knns.rs:78-100): the support
+ * matrix of build_bin_nbstats(train, heldout, "supp") (src/nbstats.rs:73-117,
+ * every ordered pair of each user's training and held-out items) and the
+ * similarity phi n / (n + shrinkage), phi = n N_USERS / (supp_ii supp_jj)
+ * (src/knns.rs:64-70), built on the GPU.  heldout needs only users and items
+ * (MaskedDataset). */
+int knns_create(const knns_config* cfg, const tx_ratings* train, const tx_ratings* heldout, knnf_ctx** out);
+
+/* KnnsModel::predict (src/knns.rs:106-147): the same rule as knnf_predict. */
+int knns_predict(knnf_ctx* ctx, int64_t n, const int32_t* users, const int32_t* items, const int16_t* days,
+                 float* out);
+int knns_destroy(knnf_ctx* ctx);
 
 /* The similarity of items i and j as the model uses it (tests and tools). */
 int knnf_similarity(knnf_ctx* ctx, int64_t n, const int32_t* i, const int32_t* j, float* out);

@@ -1,11 +1,15 @@
-// Host side of the GPU KnnfModel: the C ABI declared in include/tsvd_knn.h.
+// Host side of the GPU KnnfModel and KnnsModel: the C ABI declared in
+// include/tsvd_knn.h.
 //
 // knnf_create restates KnnfModel::new_with_factors (src/knnf.rs:60-90): the
 // factor table goes to the device zero-padded to a multiple of 64 rows and 32
 // columns, row norms and the item-item similarity max(0, cos) are computed
-// there (knn_kernels.hip), and each user's training ratings (item, residual,
-// day) are kept as CSR arrays.  knnf_predict runs KnnfModel::predict for a
-// batch of queries.
+// there (knn_kernels.hip).  knns_create restates KnnsModel::new with
+// SuppSource::Compute (src/knns.rs:78-100): the co-occurrence support of the
+// training and held-out items per user, turned into the shrunk-lift
+// similarity on the device.  Both keep each user's training ratings (item,
+// residual, day) as CSR arrays; predictions (KnnfModel::predict and
+// KnnsModel::predict are the same rule over their similarity) run batched.
 #include <hip/hip_runtime.h>
 
 #include <cstdarg>
@@ -29,6 +33,7 @@ struct knnf_ctx {
     hipStream_t st = nullptr;
     std::vector<void*> allocs;
     knn::KnnView v{};
+    std::vector<uint32_t> h_off;  // host copy of the training CSR offsets
     ~knnf_ctx() {
         for (void* p : allocs) (void)hipFree(p);
         if (st) (void)hipStreamDestroy(st);
@@ -63,6 +68,48 @@ int fail(int code, const char* fmt, ...) {
         if (e_ != hipSuccess) return fail(TX_EDEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
     } while (0)
 
+// Training ratings as per-user CSR on the device (user-sorted input, as the
+// reference asserts) and the similarity matrix allocation (ip x ip).
+int setup_train(knnf_ctx* c, const tx_ratings* train, int32_t n_items, const char* what) {
+    const int64_t N = train->n_ratings;
+    const int32_t U = train->n_users;
+    c->n_items = n_items;
+    c->n_users = U;
+    c->ip = (n_items + 63) / 64 * 64;
+    if (static_cast<int64_t>(c->ip) * c->ip > (int64_t{1} << 34))
+        return fail(TX_EUNSUPPORTED, "%s: %d items exceed the similarity matrix budget", what, n_items);
+    c->h_off.assign(static_cast<size_t>(U) + 1, 0);
+    for (int64_t t = 0; t < N; ++t) {
+        const int32_t u = train->user_idxs[t];
+        if (u < 0 || u >= U) return fail(TX_EINVAL, "%s: user index %d out of range", what, u);
+        if (t > 0 && u < train->user_idxs[t - 1]) return fail(TX_EINVAL, "%s: training rows not user-sorted", what);
+        const int32_t i = train->item_idxs[t];
+        if (i < 0 || i >= n_items) return fail(TX_EINVAL, "%s: item index %d out of range", what, i);
+        c->h_off[u + 1]++;
+    }
+    for (int32_t u = 0; u < U; ++u) c->h_off[u + 1] += c->h_off[u];
+    std::vector<float> day(static_cast<size_t>(N));
+    for (int64_t t = 0; t < N; ++t) day[t] = static_cast<float>(train->dates[t]);
+    HIPC(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+    float *dS = nullptr, *dres = nullptr, *dday = nullptr;
+    uint32_t* doff = nullptr;
+    int32_t* ditem = nullptr;
+    HIPC(c->alloc(&dS, static_cast<size_t>(c->ip) * c->ip));
+    HIPC(c->alloc(&doff, c->h_off.size()));
+    HIPC(c->alloc(&ditem, static_cast<size_t>(N)));
+    HIPC(c->alloc(&dres, static_cast<size_t>(N)));
+    HIPC(c->alloc(&dday, static_cast<size_t>(N)));
+    HIPC(hipMemcpyAsync(doff, c->h_off.data(), sizeof(uint32_t) * c->h_off.size(), hipMemcpyHostToDevice, c->st));
+    if (N > 0) {
+        HIPC(hipMemcpyAsync(ditem, train->item_idxs, sizeof(int32_t) * N, hipMemcpyHostToDevice, c->st));
+        HIPC(hipMemcpyAsync(dres, train->residuals, sizeof(float) * N, hipMemcpyHostToDevice, c->st));
+        HIPC(hipMemcpyAsync(dday, day.data(), sizeof(float) * N, hipMemcpyHostToDevice, c->st));
+    }
+    HIPC(hipStreamSynchronize(c->st));  // day is a host temporary
+    c->v = knn::KnnView{dS, c->ip, doff, ditem, dres, dday, 1, 1.0f, 0.0f};
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -75,60 +122,88 @@ int knnf_create(const knnf_config* cfg, const float* factors, int32_t n_items, i
     if (n_items < 1 || n_feat < 1) return fail(TX_EINVAL, "knnf_create: empty factor table");
     if (train->n_items > n_items)
         return fail(TX_EINVAL, "knnf_create: factor table has %d rows for %d items", n_items, train->n_items);
-    const int64_t N = train->n_ratings;
-    const int32_t U = train->n_users;
+    if (!train->residuals) return fail(TX_EINVAL, "knnf_create: training residuals missing");
     auto c = std::make_unique<knnf_ctx>();
     c->cfg = *cfg;
-    c->n_items = n_items;
-    c->n_users = U;
-    c->ip = (n_items + 63) / 64 * 64;
     c->fp = (n_feat + 31) / 32 * 32;
-    if (static_cast<int64_t>(c->ip) * c->ip > (int64_t{1} << 34))
-        return fail(TX_EUNSUPPORTED, "knnf_create: %d items exceed the similarity matrix budget", n_items);
-
-    // per-user CSR of the training ratings (user-sorted input, as the reference asserts)
-    std::vector<uint32_t> off(static_cast<size_t>(U) + 1, 0);
-    for (int64_t t = 0; t < N; ++t) {
-        const int32_t u = train->user_idxs[t];
-        if (u < 0 || u >= U) return fail(TX_EINVAL, "knnf_create: user index %d out of range", u);
-        if (t > 0 && u < train->user_idxs[t - 1]) return fail(TX_EINVAL, "knnf_create: training rows not user-sorted");
-        const int32_t i = train->item_idxs[t];
-        if (i < 0 || i >= n_items) return fail(TX_EINVAL, "knnf_create: item index %d out of range", i);
-        off[u + 1]++;
-    }
-    for (int32_t u = 0; u < U; ++u) off[u + 1] += off[u];
-    std::vector<float> day(static_cast<size_t>(N));
-    for (int64_t t = 0; t < N; ++t) day[t] = static_cast<float>(train->dates[t]);
-
-    HIPC(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+    if (int rc = setup_train(c.get(), train, n_items, "knnf_create")) return rc;
     // padded factors
     std::vector<float> F(static_cast<size_t>(c->ip) * c->fp, 0.0f);
     for (int32_t i = 0; i < n_items; ++i)
         for (int32_t k = 0; k < n_feat; ++k) F[static_cast<size_t>(i) * c->fp + k] = factors[static_cast<size_t>(i) * n_feat + k];
-    float *dF = nullptr, *dn = nullptr, *dS = nullptr, *dres = nullptr, *dday = nullptr;
-    uint32_t* doff = nullptr;
-    int32_t* ditem = nullptr;
+    float *dF = nullptr, *dn = nullptr;
     HIPC(c->alloc(&dF, F.size()));
     HIPC(c->alloc(&dn, static_cast<size_t>(c->ip)));
-    HIPC(c->alloc(&dS, static_cast<size_t>(c->ip) * c->ip));
-    HIPC(c->alloc(&doff, off.size()));
-    HIPC(c->alloc(&ditem, static_cast<size_t>(N)));
-    HIPC(c->alloc(&dres, static_cast<size_t>(N)));
-    HIPC(c->alloc(&dday, static_cast<size_t>(N)));
     HIPC(hipMemcpyAsync(dF, F.data(), sizeof(float) * F.size(), hipMemcpyHostToDevice, c->st));
-    HIPC(hipMemcpyAsync(doff, off.data(), sizeof(uint32_t) * off.size(), hipMemcpyHostToDevice, c->st));
-    if (N > 0) {
-        HIPC(hipMemcpyAsync(ditem, train->item_idxs, sizeof(int32_t) * N, hipMemcpyHostToDevice, c->st));
-        HIPC(hipMemcpyAsync(dres, train->residuals, sizeof(float) * N, hipMemcpyHostToDevice, c->st));
-        HIPC(hipMemcpyAsync(dday, day.data(), sizeof(float) * N, hipMemcpyHostToDevice, c->st));
-    }
     HIPC(knn::launch_norms(dF, c->fp, c->ip, dn, c->st));
-    HIPC(knn::launch_cosine(dF, c->fp, dn, dS, c->ip, c->st));
+    HIPC(knn::launch_cosine(dF, c->fp, dn, const_cast<float*>(c->v.S), c->ip, c->st));
     HIPC(hipStreamSynchronize(c->st));
-    c->v = knn::KnnView{dS, c->ip, doff, ditem, dres, dday, cfg->k, cfg->scaling, cfg->tau};
+    c->v.k = cfg->k;
+    c->v.scaling = cfg->scaling;
+    c->v.tau = cfg->tau;
     *out = c.release();
     return 0;
 }
+
+int knns_create(const knns_config* cfg, const tx_ratings* train, const tx_ratings* heldout, knnf_ctx** out) {
+    tx::clear_error();
+    if (!cfg || !train || !heldout || !out) return fail(TX_EINVAL, "knns_create: NULL argument");
+    if (cfg->k < 1) return fail(TX_EINVAL, "knns_create: k must be >= 1");
+    if (heldout->n_users != train->n_users) return fail(TX_EINVAL, "knns_create: user counts differ");
+    const int32_t I = train->n_items > heldout->n_items ? train->n_items : heldout->n_items;
+    if (I < 1) return fail(TX_EINVAL, "knns_create: no items");
+    if (!train->residuals) return fail(TX_EINVAL, "knns_create: training residuals missing");
+    auto c = std::make_unique<knnf_ctx>();
+    if (int rc = setup_train(c.get(), train, I, "knns_create")) return rc;
+    // per-user item lists: training items, then held-out items (build_bin_nbstats)
+    const int32_t U = train->n_users;
+    const int64_t NH = heldout->n_ratings;
+    std::vector<uint32_t> hcnt(static_cast<size_t>(U) + 1, 0);
+    for (int64_t t = 0; t < NH; ++t) {
+        const int32_t u = heldout->user_idxs[t], i = heldout->item_idxs[t];
+        if (u < 0 || u >= U || i < 0 || i >= I) return fail(TX_EINVAL, "knns_create: held-out row %lld out of range",
+                                                            static_cast<long long>(t));
+        hcnt[u + 1]++;
+    }
+    std::vector<uint32_t> uoff(static_cast<size_t>(U) + 1, 0);
+    // a user without training ratings is skipped, held-out items too (src/nbstats.rs:79-80)
+    for (int32_t u = 0; u < U; ++u)
+        uoff[u + 1] = uoff[u] + (c->h_off[u + 1] > c->h_off[u] ? (c->h_off[u + 1] - c->h_off[u]) + hcnt[u + 1] : 0);
+    std::vector<int32_t> items(uoff[U]);
+    std::vector<uint32_t> pos(uoff.begin(), uoff.end() - 1);
+    for (int32_t u = 0; u < U; ++u)
+        for (uint32_t t = c->h_off[u]; t < c->h_off[u + 1]; ++t) items[pos[u]++] = train->item_idxs[t];
+    for (int64_t t = 0; t < NH; ++t) {
+        const int32_t u = heldout->user_idxs[t];
+        if (c->h_off[u + 1] > c->h_off[u]) items[pos[u]++] = heldout->item_idxs[t];
+    }
+    uint32_t* duoff = nullptr;
+    int32_t* ditems = nullptr;
+    float* ddiag = nullptr;
+    HIPC(c->alloc(&duoff, uoff.size()));
+    HIPC(c->alloc(&ditems, items.size()));
+    HIPC(c->alloc(&ddiag, static_cast<size_t>(c->ip)));
+    HIPC(hipMemcpyAsync(duoff, uoff.data(), sizeof(uint32_t) * uoff.size(), hipMemcpyHostToDevice, c->st));
+    if (!items.empty())
+        HIPC(hipMemcpyAsync(ditems, items.data(), sizeof(int32_t) * items.size(), hipMemcpyHostToDevice, c->st));
+    float* S = const_cast<float*>(c->v.S);
+    HIPC(hipMemsetAsync(S, 0, sizeof(float) * static_cast<size_t>(c->ip) * c->ip, c->st));
+    HIPC(knn::launch_knns_sim(duoff, ditems, U, static_cast<int64_t>(items.size()), S, ddiag, c->ip, cfg->shrinkage,
+                              c->st));
+    HIPC(hipStreamSynchronize(c->st));
+    c->v.k = cfg->k;
+    c->v.scaling = cfg->scaling;
+    c->v.tau = cfg->tau;
+    *out = c.release();
+    return 0;
+}
+
+int knns_predict(knnf_ctx* ctx, int64_t n, const int32_t* users, const int32_t* items, const int16_t* days,
+                 float* out) {
+    return knnf_predict(ctx, n, users, items, days, out);
+}
+
+int knns_destroy(knnf_ctx* ctx) { return knnf_destroy(ctx); }
 
 int knnf_predict(knnf_ctx* c, int64_t n, const int32_t* users, const int32_t* items, const int16_t* days, float* out) {
     tx::clear_error();

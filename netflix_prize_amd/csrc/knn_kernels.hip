@@ -96,9 +96,12 @@ __global__ __launch_bounds__(256) void k_norms(const float* __restrict__ F, int 
 // the k largest weights are kept and the prediction is sum w r / sum w (0
 // without candidates).  The k-th largest weight is found exactly by a radix
 // select over the weights' bit patterns (positive floats order as unsigned
-// integers), 8 bits per pass through a 256-bin LDS histogram; weights equal
-// to it are taken in rating order until k are kept (the reference's unstable
-// selection takes any of them).
+// integers), 8 bits per pass through a 256-bin LDS histogram.  Equal weights
+// at the k-th place follow the reference's select_nth_unstable_by: with k + 1
+// candidates it drops the first smallest (core::slice::select, index ==
+// len - 1: max_index keeps the first of equals), with up to 16 its insertion
+// sort keeps equal weights in rating order, which is also taken beyond 16
+// (there the reference's quickselect partitioning decides).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float knn_weight(const KnnView& v, const float* srow, int32_t i, float day, uint32_t t) {
     const int32_t j = v.t_item[t];
@@ -111,6 +114,14 @@ __device__ __forceinline__ float knn_weight(const KnnView& v, const float* srow,
 
 __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t x) {
     for (int o = 32; o > 0; o >>= 1) x += static_cast<uint32_t>(__shfl_xor(static_cast<int>(x), o, 64));
+    return x;
+}
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t x) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t y = static_cast<uint32_t>(__shfl_xor(static_cast<int>(x), o, 64));
+        x = y < x ? y : x;
+    }
     return x;
 }
 
@@ -136,7 +147,16 @@ __global__ __launch_bounds__(256) void k_knnf_predict(KnnView v, int64_t n, cons
         for (uint32_t t = t0 + lane; t < t1; t += 64) m += knn_weight(v, srow, i, day, t) > 0.0f ? 1u : 0u;
         m = wave_sum_u32(m);
         uint32_t thr = 0, take_eq = 0xffffffffu;  // keep w > thr, and the first take_eq weights == thr
-        if (m > static_cast<uint32_t>(v.k)) {
+        bool drop_first = false;                   // k + 1 candidates: keep all but the first smallest
+        if (m == static_cast<uint32_t>(v.k) + 1u) {
+            uint32_t mn = 0xffffffffu;
+            for (uint32_t t = t0 + lane; t < t1; t += 64) {
+                const float w = knn_weight(v, srow, i, day, t);
+                if (w > 0.0f && __float_as_uint(w) < mn) mn = __float_as_uint(w);
+            }
+            thr = wave_min_u32(mn);
+            drop_first = true;
+        } else if (m > static_cast<uint32_t>(v.k)) {
             uint32_t prefix = 0, mask = 0, krem = static_cast<uint32_t>(v.k);
             for (int shift = 24; shift >= 0; shift -= 8) {
                 for (int b = lane; b < 256; b += 64) hist[b] = 0;
@@ -202,7 +222,7 @@ __global__ __launch_bounds__(256) void k_knnf_predict(KnnView v, int64_t n, cons
             const bool eq = w > 0.0f && m > static_cast<uint32_t>(v.k) && bits == thr;
             const uint64_t eqm = __ballot(eq);
             const uint32_t rank = eq_seen + static_cast<uint32_t>(__builtin_popcountll(eqm & ((1ull << lane) - 1ull)));
-            if (eq && rank < take_eq) keep = true;
+            if (eq && (drop_first ? rank >= 1u : rank < take_eq)) keep = true;
             eq_seen += static_cast<uint32_t>(__builtin_popcountll(eqm));
             if (keep) {
                 num += w * r;
@@ -213,6 +233,64 @@ __global__ __launch_bounds__(256) void k_knnf_predict(KnnView v, int64_t n, cons
         den = wave_sum_f32(den);
         if (lane == 0) out[q] = m == 0 ? 0.0f : num / den;
     }
+}
+
+// ---------------------------------------------------------------------------
+// KnnsModel (src/knns.rs): the support matrix of build_bin_nbstats(.., "supp")
+// (src/nbstats.rs:73-117): supp[i][j] += 1 for every ordered pair of items of
+// each user's training and held-out items (diagonal included).  One wave per
+// row r of the per-user item lists adds 1 to supp[item r][item s] for every
+// item s of the same user; FP32 adds of 1 stay exact below 2^24.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_supp(const uint32_t* __restrict__ off, const int32_t* __restrict__ items,
+                                              int32_t n_users, int64_t n_rows, float* __restrict__ supp, int ip) {
+    const int lane = threadIdx.x & 63;
+    const int64_t nwaves = static_cast<int64_t>(gridDim.x) * (blockDim.x >> 6);
+    for (int64_t r = static_cast<int64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6); r < n_rows; r += nwaves) {
+        // the row's user: last u with off[u] <= r
+        int32_t lo = 0, hi = n_users;
+        while (hi - lo > 1) {
+            const int32_t mid = (lo + hi) >> 1;
+            if (off[mid] <= static_cast<uint64_t>(r)) lo = mid;
+            else hi = mid;
+        }
+        const uint32_t a = off[lo], b = off[lo + 1];
+        float* row = supp + static_cast<size_t>(items[r]) * ip;
+        for (uint32_t s = a + lane; s < b; s += 64) atomicAdd(row + items[s], 1.0f);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_diag(const float* __restrict__ supp, int ip, float* __restrict__ diag) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ip) diag[i] = supp[static_cast<size_t>(i) * ip + i];
+}
+
+// KnnsModel::sim (src/knns.rs:64-70), in place over the support matrix:
+// phi = n * N_USERS / (supp_ii supp_jj) (0 if the product is 0), then
+// phi * n / (n + shrinkage).  N_USERS is the reference's constant 480,189.
+__global__ __launch_bounds__(256) void k_knns_sim(float* __restrict__ S, const float* __restrict__ diag, int ip,
+                                                  float shrinkage) {
+#pragma clang fp contract(off)
+    const int64_t total = static_cast<int64_t>(ip) * ip;
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
+         e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int i = static_cast<int>(e / ip), j = static_cast<int>(e % ip);
+        const float n = S[e];
+        const float den = diag[i] * diag[j];
+        const float phi = den > 0.0f ? n * 480189.0f / den : 0.0f;
+        S[e] = phi * n / (n + shrinkage);
+    }
+}
+
+hipError_t launch_knns_sim(const uint32_t* off, const int32_t* items, int32_t n_users, int64_t n_rows, float* S,
+                           float* diag, int ip, float shrinkage, hipStream_t st) {
+    (void)hipGetLastError();
+    int64_t blocks = (n_rows + 3) / 4;
+    if (blocks > 65536) blocks = 65536;
+    if (blocks > 0) hipLaunchKernelGGL(k_supp, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, off, items, n_users, n_rows, S, ip);
+    hipLaunchKernelGGL(k_diag, dim3((ip + 255) / 256), dim3(256), 0, st, S, ip, diag);
+    hipLaunchKernelGGL(k_knns_sim, dim3(65536), dim3(256), 0, st, S, diag, ip, shrinkage);
+    return hipGetLastError();
 }
 
 hipError_t launch_norms(const float* F, int fp, int rows, float* norm, hipStream_t st) {

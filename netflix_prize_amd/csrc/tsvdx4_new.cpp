@@ -538,30 +538,41 @@ void fit2(const tx_config& cfg, const std::string& target, const std::string& jo
     }
 }
 
-// `<base>__knnf` jobs (run_knnf, src/bin/tsvdx4-new.rs:55-57, 576-588 and
-// src/bin/tsvdx4p-new.rs:24, 90-105): fit2 of a KnnfModel with the default
-// KnnfConfig over the base job's saved item factors
-// `{preds}/{base}.ifeat.{train set}.npy`, target `1.0*{base}`.  KnnfModel has
-// no epochs (Regressor::n_epochs() = 0), subscores or artifacts.
-void knnf_fit(const std::string& base, const std::string& target, const std::string& tr_set,
-              const std::string& pr_set, const std::string& job, const std::string& preds_dir, bool phase1) {
+// kNN chain jobs: fit2 of a model without epochs (Regressor::n_epochs() = 0),
+// subscores or artifacts, on the base job's residuals.
+//   `<base>__knnf` (run_knnf, src/bin/tsvdx4-new.rs:55-57, 576-588 and
+//   src/bin/tsvdx4p-new.rs:24, 90-105): KnnfModel with the default KnnfConfig
+//   over the base job's saved item factors `{preds}/{base}.ifeat.{train set}.npy`,
+//   target `1.0*{base}`.
+//   `<base>__knns` (run_knns, src/bin/tsvdx4-new.rs:49-52, 563-574): KnnsModel
+//   with SuppSource::Compute and shrinkage 100, target `{w}*{base}`.
+// One fit phase: knns when ks is given, else knnf.
+void knn_fit(const std::string& base, const std::string& target, const std::string& tr_set,
+             const std::string& pr_set, const std::string& job, const std::string& preds_dir, bool phase1,
+             const knns_config* ks) {
     tee("%s => %s\n", tr_set.c_str(), pr_set.c_str());
     Dataset tr = load_dataset(tr_set, target, preds_dir);
     Dataset pr = load_dataset(pr_set, target, preds_dir);
-    const std::string fpath = preds_dir + "/" + base + ".ifeat." + tr_set + ".npy";
-    npy::Array fa;
-    try {
-        fa = npy::read(fpath);
-    } catch (const std::exception& e) {
-        panic(e.what());
-    }
-    if (fa.shape.size() != 2) panic(fpath + ": expected a 2-D factor table");
-    const std::vector<float> F = npy::as<float>(fa, fpath);
-    const knnf_config kc{25, 1.0f, 0.025f};  // KnnfConfig::with_factors (src/knnf.rs:23-26)
     knnf_ctx* ctx = nullptr;
     const tx_ratings t = tr.view(false);
-    check(knnf_create(&kc, F.data(), static_cast<int32_t>(fa.shape[0]), static_cast<int32_t>(fa.shape[1]), &t, &ctx),
-          "KnnfModel::new");
+    if (ks) {
+        const tx_ratings h = pr.view(true);
+        check(knns_create(ks, &t, &h, &ctx), "KnnsModel::new");
+    } else {
+        const std::string fpath = preds_dir + "/" + base + ".ifeat." + tr_set + ".npy";
+        npy::Array fa;
+        try {
+            fa = npy::read(fpath);
+        } catch (const std::exception& e) {
+            panic(e.what());
+        }
+        if (fa.shape.size() != 2) panic(fpath + ": expected a 2-D factor table");
+        const std::vector<float> F = npy::as<float>(fa, fpath);
+        const knnf_config kc{25, 1.0f, 0.025f};  // KnnfConfig::with_factors (src/knnf.rs:23-26)
+        check(knnf_create(&kc, F.data(), static_cast<int32_t>(fa.shape[0]), static_cast<int32_t>(fa.shape[1]), &t,
+                          &ctx),
+              "KnnfModel::new");
+    }
     std::vector<float> p(pr.n_ratings());
     check(knnf_predict(ctx, pr.n_ratings(), pr.user_idxs.data(), pr.item_idxs.data(), pr.dates.data(), p.data()),
           "predict");
@@ -571,17 +582,22 @@ void knnf_fit(const std::string& base, const std::string& target, const std::str
     write_preds(preds_dir + "/" + job + "." + pr_set + ".npy", p);
 }
 
-void fit2_knnf(const std::string& job, const std::string& base, const Split& sp) {
+void fit2_knn(const std::string& job, const std::string& base, const std::string& target, const Split& sp,
+              const knns_config* ks) {
     const std::string preds_dir = sp.preds_dir;
     mkdir(preds_dir.c_str(), 0755);
     g_log = std::fopen((preds_dir + "/" + job + ".out").c_str(), "w");
     if (!g_log) panic(preds_dir + "/" + job + ".out: cannot create");
-    const std::string target = "1.0*" + base;
     tee("[%s]\n", job.c_str());
     tee("target = \"%s\"\n", target.c_str());
-    tee("KnnfConfig { factors: \"%s/%s.ifeat\", k: 25, scaling: 1.0, tau: 0.025 }\n", preds_dir.c_str(), base.c_str());
-    knnf_fit(base, target, sp.tr, sp.pr, job, preds_dir, true);
-    knnf_fit(base, target, sp.fulltrain_tr, sp.fulltrain_pr, job, preds_dir, false);
+    if (ks)
+        tee("KnnsConfig { k: %d, shrinkage: %s, scaling: %s, tau: %s, supp_source: Compute }\n", ks->k,
+            rust_f32(ks->shrinkage).c_str(), rust_f32(ks->scaling).c_str(), rust_f32(ks->tau).c_str());
+    else
+        tee("KnnfConfig { factors: \"%s/%s.ifeat\", k: 25, scaling: 1.0, tau: 0.025 }\n", preds_dir.c_str(),
+            base.c_str());
+    knn_fit(base, target, sp.tr, sp.pr, job, preds_dir, true, ks);
+    knn_fit(base, target, sp.fulltrain_tr, sp.fulltrain_pr, job, preds_dir, false, ks);
     tee("\n");
     std::fclose(g_log);
     g_log = nullptr;
@@ -886,10 +902,24 @@ int main(int argc, char** argv) {
     for (const char* name : kKnnf) {
         if (job != name) continue;
         if (print_only) return 0;
-        fit2_knnf(job, job.substr(0, job.size() - std::string("__knnf").size()), SPLIT_NEW);
+        const std::string base = job.substr(0, job.size() - std::string("__knnf").size());
+        fit2_knn(job, base, "1.0*" + base, SPLIT_NEW, nullptr);
+        return 0;
+    }
+    struct KnnsJob { const char* name; int k; float scaling, tau; const char* w; };
+    static const KnnsJob kKnns[] = {
+        {"tsvdx4-60__knns", 25, 1.0f, 0.025f, "1.0"}, {"tsvdx4-60o__knns", 50, 0.5f, 0.0f, "1.0"},
+        {"tsvdx4-2400lm__knns", 15, 1.0f, 0.5f, "1.2"}, {"tsvdx4-4800lm__knns", 40, 3.0f, 0.05f, "0.9"},
+    };
+    for (const KnnsJob& kj : kKnns) {
+        if (job != kj.name) continue;
+        if (print_only) return 0;
+        const std::string base = job.substr(0, job.size() - std::string("__knns").size());
+        const knns_config ks{kj.k, 100.0f, kj.scaling, kj.tau};
+        fit2_knn(job, base, std::string(kj.w) + "*" + base, SPLIT_NEW, &ks);
         return 0;
     }
     if (job.find("__knn") != std::string::npos)
-        panic("job " + job + ": this kNN chain (knn3 / knns / knn-d) is outside this build");
+        panic("job " + job + ": this kNN chain (knn3 / knn-d) is outside this build");
     panic("invalid job name: " + job);
 }

@@ -90,3 +90,38 @@ class KnnfModel:
             self.close()
         except Exception:
             pass
+
+
+@dataclass
+class KnnsConfig:
+    """KnnsConfig (src/knns.rs:19-43) with SuppSource::Compute; defaults of
+    KnnsConfig::default (src/knns.rs:35-43)."""
+    k: int = 25
+    shrinkage: float = 100.0
+    scaling: float = 1.0
+    tau: float = 0.025
+
+
+class _CKnns(C.Structure):
+    _fields_ = [("k", C.c_int32), ("shrinkage", C.c_float), ("scaling", C.c_float), ("tau", C.c_float)]
+
+
+class KnnsModel(KnnfModel):
+    """KnnsModel::new with SuppSource::Compute (src/knns.rs:78-100) on the GPU:
+    co-occurrence support of the training and held-out items, shrunk-lift
+    similarity; predictions as KnnfModel (the same rule)."""
+
+    def __init__(self, tr, pr, cfg: KnnsConfig = KnnsConfig()):
+        L = _lib()
+        if not getattr(L, "_knns_bound", False):
+            vp = C.c_void_p
+            L.knns_create.argtypes = [vp, vp, vp, C.POINTER(vp)]
+            L.knns_create.restype = C.c_int
+            L._knns_bound = True
+        self.cfg = cfg
+        self._tr = tx._RatingsRef(tr)
+        self._pr = tx._RatingsRef(pr, masked=True)
+        c = _CKnns(int(cfg.k), float(cfg.shrinkage), float(cfg.scaling), float(cfg.tau))
+        h = C.c_void_p()
+        tx._check(L.knns_create(C.byref(c), C.byref(self._tr.s), C.byref(self._pr.s), C.byref(h)), "knns_create")
+        self._h = h

@@ -1,5 +1,6 @@
-"""CPU restatement of KnnfModel (src/knnf.rs) -- test infrastructure only, the
-checker of the GPU knnf kernels; never imported by the product.
+"""CPU restatement of KnnfModel (src/knnf.rs) and KnnsModel (src/knns.rs) --
+test infrastructure only, the checker of the GPU kNN kernels; never imported
+by the product.
 
 Follows src/knnf.rs line by line:
   * factor_norms[i] = sqrt(row_i . row_i)                  (:64-68)
@@ -7,7 +8,12 @@ Follows src/knnf.rs line by line:
   * sim(i, j) = max(0, row_i . row_j / (n_i n_j)), 0 if a norm is 0   (:46-52)
   * predict: skip j == i; w = sim^scaling / (1 + tau |day - d_j|); skip
     w == 0; no candidates -> 0; keep the k largest w; sum(w r) / sum(w)
-    (:101-141).
+    (:101-141).  Equal weights at the k-th place as the reference's
+    select_nth_unstable_by (core::slice::select): with k + 1 candidates the
+    first smallest is dropped (index == len - 1, max_index keeps the first
+    of equals), otherwise equal weights are kept in rating order (its
+    insertion sort up to 16 elements; beyond that its quickselect
+    partitioning decides, which this restatement does not follow).
 Arithmetic is FP64 here (the reference is FP32): predictions agree to ~1e-6
 except where the k-th and (k+1)-th weights nearly tie, which the reference's
 unstable selection leaves open too; `margins` reports that gap per query.
@@ -60,7 +66,11 @@ class KnnfRef:
         w, r = self._weights(u, i, day)
         if len(w) == 0:
             return 0.0
-        if len(w) > self.k:
+        if len(w) == self.k + 1:
+            keep = np.ones(len(w), bool)
+            keep[np.argmin(w)] = False  # the first smallest
+            w, r = w[keep], r[keep]
+        elif len(w) > self.k:
             top = np.argsort(-w, kind="stable")[: self.k]
             w, r = w[top], r[top]
         return float(np.sum(w * r) / np.sum(w))
@@ -77,3 +87,33 @@ class KnnfRef:
                 s = np.sort(w)[::-1]
                 out[q] = (s[self.k - 1] - s[self.k]) / s[self.k - 1]
         return out
+
+
+N_USERS = 480_189  # src/lib.rs:15
+
+
+def knns_sim_matrix(tr, pr, n_items: int, shrinkage: float) -> np.ndarray:
+    """KnnsModel::sim (src/knns.rs:64-70) over build_bin_nbstats(tr, pr,
+    "supp") (src/nbstats.rs:73-117): supp[i, j] = number of users with both i
+    and j among their training and held-out items (diagonal: users of i)."""
+    n_users = int(max(tr.n_users, pr.n_users))
+    has_train = np.bincount(np.asarray(tr.user_idxs), minlength=n_users) > 0  # others skipped (:79-80)
+    pu = np.asarray(pr.user_idxs)
+    users = np.concatenate([np.asarray(tr.user_idxs), pu[has_train[pu]]]).astype(np.int64)
+    items = np.concatenate([np.asarray(tr.item_idxs), np.asarray(pr.item_idxs)[has_train[pu]]]).astype(np.int64)
+    B = np.zeros((n_users, n_items), np.float64)
+    np.add.at(B, (users, items), 1.0)
+    supp = B.T @ B
+    d = np.diag(supp).copy()
+    den = np.outer(d, d)
+    phi = np.zeros_like(supp)
+    ok = den > 0
+    phi[ok] = supp[ok] * N_USERS / den[ok]
+    return phi * supp / (supp + shrinkage)
+
+
+class KnnsRef(KnnfRef):
+    def __init__(self, tr, pr, n_items: int, k: int = 25, shrinkage: float = 100.0, scaling: float = 1.0,
+                 tau: float = 0.025):
+        super().__init__(tr, np.ones((n_items, 1), np.float32), k=k, scaling=scaling, tau=tau)
+        self.S = knns_sim_matrix(tr, pr, n_items, shrinkage)

@@ -143,3 +143,38 @@ def test_tsvdx4p_job_end_to_end(gpu, tmp_path):
     rmse = float(np.sqrt(np.mean((probe.astype(np.float64) - pr.raw_ratings) ** 2)))
     assert abs(logged - rmse) < 1e-5, (logged, rmse)
     assert rmse < float(np.std(pr.raw_ratings.astype(np.float64)))
+
+
+@pytest.mark.gpu
+def test_knns_job_end_to_end(gpu, tmp_path):
+    """tsvdx4-60 (day terms, parallel schedule) then its kNN chain
+    tsvdx4-60__knns (KnnsModel, support computed from each phase's training
+    and held-out sets; run_knns, src/bin/tsvdx4-new.rs:49, 563-574) on the
+    10k x 2k x 1M split: output files, logged RMSE, and predictions equal to
+    the oracle's on the base job's residuals."""
+    tr, pr = synthetic("small")
+    _write_split(str(tmp_path), tr, pr)
+    env = dict(os.environ, TSVD_N_USERS=str(tr.n_users), TSVD_N_ITEMS=str(tr.n_items),
+               TSVD_DATA_DIR=str(tmp_path / "data"))
+    for job in ("tsvdx4-60", "tsvdx4-60__knns"):
+        out = subprocess.run([DRIVER, job], cwd=tmp_path, env=env, capture_output=True, text=True, timeout=600)
+        assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
+    pd = tmp_path / "preds_new"
+    base = np.load(pd / "tsvdx4-60.probex.npy")
+    kp = np.load(pd / "tsvdx4-60__knns.probex.npy")
+    assert kp.shape == (pr.n_ratings,) and np.all(np.isfinite(kp))
+    assert np.load(pd / "tsvdx4-60__knns.qual.npy").shape == (pr.n_ratings,)
+    klog = (pd / "tsvdx4-60__knns.out").read_text()
+    assert "KnnsConfig { k: 25, shrinkage: 100.0, scaling: 1.0, tau: 0.025, supp_source: Compute }" in klog
+    logged = float(re.search(r"probex RMSE ([0-9.]+)", klog).group(1))
+    rmse = float(np.sqrt(np.mean((kp.astype(np.float64) - pr.raw_ratings) ** 2)))
+    assert abs(logged - rmse) < 1e-5, (logged, rmse)
+    from oracle.knnf_ref import KnnsRef
+    res_tr = tr.raw_ratings.astype(np.float32) - np.load(pd / "tsvdx4-60.trainx.npy")
+    tr_k = tx.Dataset.from_arrays(tr.user_idxs, tr.item_idxs, tr.raw_ratings, tr.dates, tr.n_users, tr.n_items,
+                                  residuals=res_tr)
+    ref = KnnsRef(tr_k, pr, tr.n_items, k=25, shrinkage=100.0, scaling=1.0, tau=0.025)
+    q = np.arange(0, pr.n_ratings, 50)
+    want = ref.predict(pr.user_idxs[q], pr.item_idxs[q], pr.dates[q]) + base[q]
+    clear = ref.margins(pr.user_idxs[q], pr.item_idxs[q], pr.dates[q]) > 1e-4
+    assert np.max(np.abs(kp[q][clear] - want[clear])) < 1e-4

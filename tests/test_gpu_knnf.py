@@ -61,3 +61,38 @@ def test_predictions_match_oracle(gpu, k, scaling, tau):
     err = np.abs(got.astype(np.float64) - want)
     print(f"k={k}: max |diff| {err[clear].max():.2e} over {clear.sum()} clear queries")
     assert err[clear].max() <= 1e-5
+
+
+def test_knns_reference_kat_on_gpu(gpu):
+    """tests/regressors_test.rs:218-233: KnnsModel (support computed from the
+    tiny fixture's training and probe sets, k=3, shrinkage 5) probe RMSE
+    1.2349219913552398, within 1e-6."""
+    from tests.test_knnf_cpu import KNNS_KAT_RMSE
+    _, tr, pr = tiny()
+    with knn.KnnsModel(tr, pr, knn.KnnsConfig(k=3, shrinkage=5.0, scaling=1.0, tau=0.025)) as m:
+        r = rmse(m.predict(pr.user_idxs, pr.item_idxs, pr.dates), pr)
+    assert r == pytest.approx(KNNS_KAT_RMSE, abs=1e-6), r
+
+
+@pytest.mark.parametrize("k,scaling,tau", [(25, 1.0, 0.025), (40, 3.0, 0.05)])
+def test_knns_matches_oracle(gpu, k, scaling, tau):
+    """10k x 2k x 1M set: the GPU support matrix (atomic pair counts over
+    training and held-out items) and shrunk-lift similarity within 1e-6
+    relative of the FP64 oracle; predictions as for knnf."""
+    from oracle.knnf_ref import KnnsRef
+    tr, pr = synthetic("small")
+    ref = KnnsRef(tr, pr, tr.n_items, k=k, shrinkage=100.0, scaling=scaling, tau=tau)
+    rng = np.random.default_rng(5)
+    q = rng.choice(pr.n_ratings, 20_000, replace=False)
+    u, i, d = pr.user_idxs[q], pr.item_idxs[q], pr.dates[q]
+    with knn.KnnsModel(tr, pr, knn.KnnsConfig(k=k, shrinkage=100.0, scaling=scaling, tau=tau)) as m:
+        a, b = rng.integers(0, tr.n_items, 5000), rng.integers(0, tr.n_items, 5000)
+        s = m.similarity(a, b)
+        got = m.predict(u, i, d)
+    np.testing.assert_allclose(s, ref.S[a, b], rtol=1e-6, atol=1e-9)
+    want = ref.predict(u, i, d)
+    clear = ref.margins(u, i, d) > 1e-4
+    assert clear.mean() > 0.99, clear.mean()
+    err = np.abs(got.astype(np.float64) - want)
+    print(f"knns k={k}: max |diff| {err[clear].max():.2e} over {clear.sum()} clear queries")
+    assert err[clear].max() <= 1e-5

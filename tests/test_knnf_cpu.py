@@ -64,3 +64,27 @@ def test_constant_residuals_give_the_constant():
     tr = _one_user([(j, 3, int(d)) for j, d in zip(range(1, 30), rng.integers(0, 2000, 29))], 30)
     ref = KnnfRef(tr, F, k=7)
     assert ref.predict_one(0, 0, 1000.0) == pytest.approx(3.0, abs=1e-12)
+
+
+# tests/regressors_test.rs:218-233 (knns_probe_rmse_regression)
+KNNS_KAT_RMSE = 1.2349219913552398
+
+
+def test_knns_oracle_reproduces_reference_kat():
+    from oracle.knnf_ref import KnnsRef
+    fx, tr, pr = tiny()
+    ref = KnnsRef(tr, pr, fx["n_items"], k=3, shrinkage=5.0, scaling=1.0, tau=0.025)
+    r = rmse(ref.predict(pr.user_idxs, pr.item_idxs, pr.dates), pr)
+    assert r == pytest.approx(KNNS_KAT_RMSE, abs=1e-6), r
+
+
+def test_knns_support_counts_training_and_heldout_items():
+    from oracle.knnf_ref import N_USERS, knns_sim_matrix
+    # user 0 rates items 0, 1 (train) and 2 (held out); user 1 rates 0 and 2
+    tr = dataset_from_rows([(0, 0, 4, 1), (0, 1, 3, 2), (1, 0, 5, 3)], 2, 3)
+    pr = dataset_from_rows([(0, 2, 1, 4), (1, 2, 2, 5)], 2, 3)
+    S = knns_sim_matrix(tr, pr, 3, shrinkage=1.0)
+    # supp: n(0)=2, n(1)=1, n(2)=2, n(0,2)=2, n(0,1)=1
+    assert S[0, 2] == pytest.approx(2 * N_USERS / (2 * 2) * 2 / (2 + 1.0))
+    assert S[0, 1] == pytest.approx(1 * N_USERS / (2 * 1) * 1 / (1 + 1.0))
+    assert S[1, 2] == pytest.approx(1 * N_USERS / (1 * 2) * 1 / (1 + 1.0))
