@@ -270,6 +270,19 @@ def main(argv=None) -> int:
                             f"({'random' if args.knnf_f else 'trained'}), {pr.n_ratings} held-out queries",
                 "create_s": round(t_create, 4), "cosine_gflop": round(2.0 * ip * ip * fp / 1e9, 1),
                 "predict_s": round(t_pred, 4), "queries_per_s": round(pr.n_ratings / t_pred, 1)}
+        # KnnsModel (the *__knns jobs): support of every user's training and
+        # held-out item pairs, then the same held-out queries
+        t0 = time.perf_counter()
+        km = knn.KnnsModel(tr, pr, knn.KnnsConfig())
+        t_create = time.perf_counter() - t0
+        km.predict(pr.user_idxs, pr.item_idxs, pr.dates)
+        t0 = time.perf_counter()
+        km.predict(pr.user_idxs, pr.item_idxs, pr.dates)
+        t_pred = time.perf_counter() - t0
+        km.close()
+        nu = np.bincount(tr.user_idxs, minlength=tr.n_users) + np.bincount(pr.user_idxs, minlength=tr.n_users)
+        knnf["knns"] = {"create_s": round(t_create, 4), "support_pairs": int(np.sum(nu.astype(np.int64) ** 2)),
+                        "predict_s": round(t_pred, 4), "queries_per_s": round(pr.n_ratings / t_pred, 1)}
         log(f"bench: knnf {knnf}")
 
     # the tsvdx4-60 job's model (lm set with day terms, low_memory false, f=60,
