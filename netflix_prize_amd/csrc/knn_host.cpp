@@ -12,10 +12,12 @@
 // KnnsModel::predict are the same rule over their similarity) run batched.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "knn_launch.hpp"
@@ -176,6 +178,17 @@ int knns_create(const knns_config* cfg, const tx_ratings* train, const tx_rating
     for (int64_t t = 0; t < NH; ++t) {
         const int32_t u = heldout->user_idxs[t];
         if (c->h_off[u + 1] > c->h_off[u]) items[pos[u]++] = heldout->item_idxs[t];
+    }
+    // sorted by item: the device counts each item pair once, in the upper triangle
+    {
+        unsigned nt = std::thread::hardware_concurrency();
+        nt = nt == 0 ? 1 : (nt > 16 ? 16 : nt);
+        std::vector<std::thread> pool;
+        for (unsigned w = 0; w < nt; ++w)
+            pool.emplace_back([&, w] {
+                for (int64_t u = w; u < U; u += nt) std::sort(items.begin() + uoff[u], items.begin() + uoff[u + 1]);
+            });
+        for (auto& th : pool) th.join();
     }
     uint32_t* duoff = nullptr;
     int32_t* ditems = nullptr;

@@ -238,9 +238,11 @@ __global__ __launch_bounds__(256) void k_knnf_predict(KnnView v, int64_t n, cons
 // ---------------------------------------------------------------------------
 // KnnsModel (src/knns.rs): the support matrix of build_bin_nbstats(.., "supp")
 // (src/nbstats.rs:73-117): supp[i][j] += 1 for every ordered pair of items of
-// each user's training and held-out items (diagonal included).  One wave per
-// row r of the per-user item lists adds 1 to supp[item r][item s] for every
-// item s of the same user; FP32 adds of 1 stay exact below 2^24.
+// each user's training and held-out items (diagonal included).  The matrix is
+// symmetric, so each user's list is sorted by item and one wave per list
+// position r adds the pairs (r, s), s >= r, to the upper triangle (an equal
+// item at s > r stands for both orders and adds 2); k_mirror then copies the
+// upper triangle down.  FP32 adds of small integers stay exact below 2^24.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_supp(const uint32_t* __restrict__ off, const int32_t* __restrict__ items,
                                               int32_t n_users, int64_t n_rows, float* __restrict__ supp, int ip) {
@@ -254,9 +256,28 @@ __global__ __launch_bounds__(256) void k_supp(const uint32_t* __restrict__ off, 
             if (off[mid] <= static_cast<uint64_t>(r)) lo = mid;
             else hi = mid;
         }
-        const uint32_t a = off[lo], b = off[lo + 1];
-        float* row = supp + static_cast<size_t>(items[r]) * ip;
-        for (uint32_t s = a + lane; s < b; s += 64) atomicAdd(row + items[s], 1.0f);
+        const uint32_t b = off[lo + 1];
+        const int32_t ir = items[r];
+        float* row = supp + static_cast<size_t>(ir) * ip;
+        for (uint32_t s = static_cast<uint32_t>(r) + lane; s < b; s += 64) {
+            const int32_t is = items[s];
+            atomicAdd(row + is, (is == ir && s != static_cast<uint32_t>(r)) ? 2.0f : 1.0f);
+        }
+    }
+}
+
+// Lower triangle from the upper one, 32 x 32 tiles through LDS (tile (bx, by)
+// with bx <= by reads upper tile (bx, by) and writes its transpose at (by, bx)).
+__global__ __launch_bounds__(256) void k_mirror(float* __restrict__ S, int ip) {
+    __shared__ float t[32][33];
+    const int bx = blockIdx.x, by = blockIdx.y;
+    if (bx > by) return;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    for (int r = ty; r < 32; r += 8) t[r][tx] = S[static_cast<size_t>(bx * 32 + r) * ip + by * 32 + tx];
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int row = by * 32 + r, col = bx * 32 + tx;  // lower element (row > col) or the diagonal tile's
+        if (row > col) S[static_cast<size_t>(row) * ip + col] = t[tx][r];
     }
 }
 
@@ -288,6 +309,7 @@ hipError_t launch_knns_sim(const uint32_t* off, const int32_t* items, int32_t n_
     int64_t blocks = (n_rows + 3) / 4;
     if (blocks > 65536) blocks = 65536;
     if (blocks > 0) hipLaunchKernelGGL(k_supp, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, off, items, n_users, n_rows, S, ip);
+    hipLaunchKernelGGL(k_mirror, dim3(ip / 32, ip / 32), dim3(256), 0, st, S, ip);
     hipLaunchKernelGGL(k_diag, dim3((ip + 255) / 256), dim3(256), 0, st, S, ip, diag);
     hipLaunchKernelGGL(k_knns_sim, dim3(65536), dim3(256), 0, st, S, diag, ip, shrinkage);
     return hipGetLastError();
