@@ -95,7 +95,9 @@ enum tx_sched_mode { TX_SCHED_SERIAL = 0, TX_SCHED_STRATA = 1, TX_SCHED_TSVDX4P 
 typedef struct tx_schedule {
     int32_t mode;
     int32_t n_strata;        /* STRATA only: 1 or a multiple of 8 */
-    int32_t teams_per_block; /* reserved (one 64-lane team per user at every f) */
+    int32_t teams_per_block; /* STRATA, f <= 1024: one-wave teams per 256-lane workgroup, 1..4
+                                (0 = 4); blocks = 1 with 1 runs the kernel as a single team in
+                                the user order (serial-debug order) */
     int32_t blocks;          /* STRATA only: persistent grid size, 0 = auto */
     int32_t use_graph;       /* capture each epoch's launches in a hipGraph */
     int32_t update_flags;    /* STRATA only: how shared rows are updated, see below */

@@ -395,6 +395,7 @@ int build_strata(tx_ctx* c, int S) {
     sv = tx::StrataView{};
     sv.q_mode = c->sched.update_flags & 3;
     sv.y_mode = (c->sched.update_flags >> 2) & 3;
+    sv.block_threads = (c->sched.teams_per_block > 0 ? c->sched.teams_per_block : 4) * 64;
     sv.n_strata = S;
     sv.n_cells = C;
     uint32_t *d_seg, *d_qu, *d_qs, *d_qh;
@@ -444,8 +445,15 @@ int build_strata(tx_ctx* c, int S) {
 // trainer (within 1e-4 measured); 4096 at f=64 and 3072 at f=256 do not.
 int target_teams(int f) { return std::min(3072, std::max(768, 393216 / std::max(f, 1))); }
 
+// One-wave teams per SGD workgroup: the schedule's teams_per_block (1-4), 4 by
+// default; a multi-wave team is a workgroup of its own.
+int teams_per_block(const tx_ctx* c) {
+    if (c->shape.waves > 1) return 1;
+    return c->sched.teams_per_block > 0 ? c->sched.teams_per_block : 256 / c->shape.team;
+}
+
 int default_blocks(tx_ctx* c) {
-    const int teams_per_block = c->shape.waves > 1 ? 1 : 256 / c->shape.team;
+    const int teams_per_block = ::teams_per_block(c);
     const int want = target_teams(c->cfg.n_feat);
     int blocks = (want + teams_per_block - 1) / teams_per_block;
     // a multiple of 8 keeps the cell -> XCD mapping of the round-robin dispatch
@@ -1048,6 +1056,8 @@ int tx_set_schedule(tx_ctx* c, const tx_schedule* s) {
             return fail(TX_EINVAL, "blocks must be 0 (auto) or at least n_strata");
     }
     if (s->user_order < 0 || s->user_order > 2) return fail(TX_EINVAL, "user_order must be 0, 1 or 2");
+    if (s->teams_per_block < 0 || s->teams_per_block > 4)
+        return fail(TX_EINVAL, "teams_per_block must be 0 (4) or 1..4, got %d", s->teams_per_block);
     if (s->heavy_min < 0) return fail(TX_EINVAL, "heavy_min must be >= 0");
     const int qm = s->update_flags & 3, ym = (s->update_flags >> 2) & 3;
     if (qm > 2 || ym > 2 || (s->update_flags >> 4) != 0) return fail(TX_EINVAL, "bad update_flags %d", s->update_flags);
@@ -1060,6 +1070,7 @@ int tx_set_schedule(tx_ctx* c, const tx_schedule* s) {
     }
     c->sv.q_mode = qm;
     c->sv.y_mode = ym;
+    c->sv.block_threads = (s->teams_per_block > 0 ? s->teams_per_block : 4) * 64;
     if (c->graph) {
         (void)hipGraphExecDestroy(c->graph);
         c->graph = nullptr;
@@ -1256,7 +1267,7 @@ int tx_autotune(tx_ctx* c, int32_t trial_epochs, tx_schedule* best_out) {
         return 0;
     };
     const tx_schedule orig = c->sched;
-    const int tpb = c->shape.waves > 1 ? 1 : 256 / c->shape.team, cap = target_teams(c->cfg.n_feat);
+    const int tpb = teams_per_block(c), cap = target_teams(c->cfg.n_feat);
     std::vector<tx_schedule> cands;
     for (int quarters : {2, 3, 4}) {
         int blocks = (cap * quarters / 4 + tpb - 1) / tpb;

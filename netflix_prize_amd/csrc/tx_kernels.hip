@@ -1649,7 +1649,8 @@ static hipError_t sgd_par_tvp(const DevModel& m, const StrataView& sv, int strat
                               bool ord, hipStream_t st) {
     (void)hipGetLastError();
 #define TX_PAR(QF, P2, ORD) \
-    hipLaunchKernelGGL((k_sgd_par<TEAM, VEC, QF, P2, ORD, PASS, DAY, 0>), dim3(blocks), dim3(256), 0, st, m, sv, stratum)
+    hipLaunchKernelGGL((k_sgd_par<TEAM, VEC, QF, P2, ORD, PASS, DAY, 0>), dim3(blocks),                          \
+                       dim3(sv.block_threads > 0 ? sv.block_threads : 256), 0, st, m, sv, stratum)
     if constexpr (PASS == 0) {
         if (ord) {
             if (qf) { if (p2) TX_PAR(true, true, true); else TX_PAR(true, false, true); }

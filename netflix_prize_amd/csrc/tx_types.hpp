@@ -97,6 +97,7 @@ struct StrataView {
     float* d_y;                 // [I * fs] NSVD1 gradient accumulators of the user pass (tsvdx4p)
     float* d_yb;                // [I]
     float* day_acc;             // [D * fs] day terms: a day's y_day sum (user start), then its gradient sum
+    int32_t block_threads;      // SGD workgroup size of the one-wave teams: 64 x teams per block (0 = 256)
 };
 
 // Item pass of the tsvdx4p schedule: ratings grouped by item.
