@@ -153,6 +153,9 @@ struct tx_ctx {
     int graph_S = 0;
 
     std::vector<uint32_t> h_queue;  // pinned order of the current epoch's queue
+    // the next epoch's queue, built on the host while the GPU runs this one
+    std::vector<uint32_t> h_queue_next;
+    int32_t next_epoch = -1, next_order = -1, next_hmin = -1;
     int64_t* d_order = nullptr;
     float* d_scratch = nullptr;
     double* d_acc = nullptr;
@@ -477,10 +480,13 @@ int ensure_bf16(tx_ctx* c, hipStream_t st) {
 // into the one-stratum queue; with length buckets (user_order 2) the users of
 // at least heavy_min ratings keep their relative order but are spread evenly
 // over the first half of the queue.
-int prepare_user_queue(tx_ctx* c, int32_t epoch, hipStream_t st) {
+// The queue of an epoch: the reference's per-epoch user order (get_users),
+// users without ratings dropped, with user_order 2 the long users spread over
+// the first half.
+void build_queue(const tx_ctx* c, int32_t epoch, std::vector<uint32_t>& q) {
     std::vector<int64_t> order(c->U);
     tx::user_order(c->U, c->cfg.shuffle_users != 0, c->cfg.seed, epoch, order.data());
-    c->h_queue.clear();
+    q.clear();
     if (c->sched.user_order == 2) {
         const uint32_t hmin = c->sched.heavy_min > 0 ? static_cast<uint32_t>(c->sched.heavy_min) : 1024u;
         std::vector<uint32_t> heavy, light;
@@ -492,12 +498,29 @@ int prepare_user_queue(tx_ctx* c, int32_t epoch, hipStream_t st) {
         size_t k = 0, l = 0;
         for (size_t pos = 0; pos < n; ++pos) {
             const bool take_heavy = k < heavy.size() && (l >= light.size() || pos * heavy.size() >= k * half);
-            c->h_queue.push_back(take_heavy ? heavy[k++] : light[l++]);
+            q.push_back(take_heavy ? heavy[k++] : light[l++]);
         }
     } else {
         for (int64_t u : order)
-            if (c->t_off[u + 1] > c->t_off[u]) c->h_queue.push_back(static_cast<uint32_t>(u));
+            if (c->t_off[u + 1] > c->t_off[u]) q.push_back(static_cast<uint32_t>(u));
     }
+}
+
+// Build the next epoch's queue ahead (called while the GPU runs an epoch).
+void build_next_queue(tx_ctx* c, int32_t epoch) {
+    build_queue(c, epoch, c->h_queue_next);
+    c->next_epoch = epoch;
+    c->next_order = c->sched.user_order;
+    c->next_hmin = c->sched.heavy_min;
+}
+
+int prepare_user_queue(tx_ctx* c, int32_t epoch, hipStream_t st) {
+    if (c->next_epoch == epoch && c->next_order == c->sched.user_order && c->next_hmin == c->sched.heavy_min) {
+        std::swap(c->h_queue, c->h_queue_next);
+    } else {
+        build_queue(c, epoch, c->h_queue);
+    }
+    c->next_epoch = -1;
     HIPC(hipMemcpyAsync(const_cast<uint32_t*>(c->sv.q_users), c->h_queue.data(), c->h_queue.size() * sizeof(uint32_t),
                         hipMemcpyHostToDevice, st));
     return 0;
@@ -1145,6 +1168,7 @@ int tx_fit_epoch(tx_ctx* c, int32_t epoch) {
             HIPC(hipEventRecord(c->ev[1], st));
             HIPC(hipGraphLaunch(c->graph, st));
             HIPC(hipEventRecord(c->ev[3], st));
+            if (c->built_S == 1 && c->sched.user_order != 1) build_next_queue(c, epoch + 1);
             HIPC(hipEventSynchronize(c->ev[3]));
             float a = 0, t = 0;
             HIPC(hipEventElapsedTime(&a, c->ev[1], c->ev[3]));
@@ -1171,6 +1195,7 @@ int tx_fit_epoch(tx_ctx* c, int32_t epoch) {
             HIPC(hipEventRecord(c->ev[2], st));
             HIPC(tx::launch_rebuild(c->dm, c->n_ud, c->shape.team, c->shape.vec, st));
             HIPC(hipEventRecord(c->ev[3], st));
+            if (c->built_S == 1 && c->sched.user_order != 1) build_next_queue(c, epoch + 1);
             HIPC(hipEventSynchronize(c->ev[3]));
             float a = 0, b = 0, t = 0;
             HIPC(hipEventElapsedTime(&a, c->ev[1], c->ev[2]));
