@@ -1,9 +1,11 @@
-/* C ABI of the GPU KnnfModel and KnnsModel (libtsvd_tx.so): item-item kNN
- * with temporal decay at predict time, over the cosine of a factor table (the
- * `.ifeat` a tsvdx4 job saves; KnnfModel, src/knnf.rs:1-144) or over the
- * shrunk lift of item co-occurrence (KnnsModel, src/knns.rs:1-149).  Drop-ins
- * behind the reference's Regressor trait for the `*__knnf` and `*__knns` jobs
- * (src/bin/tsvdx4-new.rs:49-57, src/bin/tsvdx4p-new.rs:24).
+/* C ABI of the GPU KnnfModel, KnnsModel and Knn3Model (libtsvd_tx.so):
+ * item-item kNN with temporal decay at predict time, over the cosine of a
+ * factor table (the `.ifeat` a tsvdx4 job saves; KnnfModel, src/knnf.rs:1-144)
+ * or over the shrunk lift of item co-occurrence (KnnsModel, src/knns.rs:1-149);
+ * and item-item regression weights by NNLS (Knn3Model, src/knn3.rs:1-396).
+ * Drop-ins behind the reference's Regressor trait for the `*__knnf`,
+ * `*__knns` and `*__knn3` jobs (src/bin/tsvdx4-new.rs:43-57,
+ * src/bin/tsvdx4p-new.rs:24).
  *
  * Conventions as include/tsvd_tx.h: calls return 0 or a negative TX_E* code
  * with tx_last_error() describing it; host buffers are copied, never kept. */
@@ -61,6 +63,45 @@ int knns_create(const knns_config* cfg, const tx_ratings* train, const tx_rating
 int knns_predict(knnf_ctx* ctx, int64_t n, const int32_t* users, const int32_t* items, const int16_t* days,
                  float* out);
 int knns_destroy(knnf_ctx* ctx);
+
+/* Knn3Config (src/knn3.rs:7-31).  Defaults (Knn3Config::default): threshold
+ * 0.25, k_min 10, k_max 60, shrinkage 20000, reg 0.01, x 0.8, bl_reg_m 25,
+ * bl_reg_u 10.  k_max above 64 is TX_EUNSUPPORTED. */
+typedef struct knn3_config {
+    float threshold;
+    int32_t k_min, k_max;
+    float shrinkage, reg, x, bl_reg_m, bl_reg_u;
+} knn3_config;
+
+typedef struct knn3_ctx knn3_ctx;
+
+/* Knn3Model::new (src/knn3.rs:350-362): over the non-test training ratings
+ * (train needs residuals and raw_ratings; is_test NULL means none), the
+ * regularised baseline (compute_baseline, :123-165), per item the candidates
+ * by shrunk lift of co-rating counts (:195-216), the mixed FP64 normal
+ * equations of the residual and baseline-corrected signals (:222-286) and
+ * their NNLS solution (solve_nnls, :79-120), built on the GPU.  One rating per
+ * user and item (TX_EINVAL otherwise); at most 32,768 items. */
+int knn3_create(const knn3_config* cfg, const tx_ratings* train, knn3_ctx** out);
+
+/* Knn3Model::predict (src/knn3.rs:379-389): sum of weight x residual over the
+ * item's weighted neighbours the user rated; days may be NULL (unused). */
+int knn3_predict(knn3_ctx* ctx, int64_t n, const int32_t* users, const int32_t* items, const int16_t* days,
+                 float* out);
+
+/* The model's neighbour table: counts[n_items], items and weights
+ * [n_items x 64], row m holding item m's positive weights in candidate order
+ * (tests and tools). */
+int knn3_weights(knn3_ctx* ctx, int32_t* counts, int32_t* items, float* weights);
+
+/* The candidate lists before the regression: counts[n_items], items
+ * [n_items x 64] by prox, largest first (tests and tools). */
+int knn3_candidates(knn3_ctx* ctx, int32_t* counts, int32_t* items);
+
+/* The regularised baseline: *mu, bm[n_items], bu[n_users] (tests and tools). */
+int knn3_baseline(knn3_ctx* ctx, float* mu, float* bm, float* bu);
+
+int knn3_destroy(knn3_ctx* ctx);
 
 /* The similarity of items i and j as the model uses it (tests and tools). */
 int knnf_similarity(knnf_ctx* ctx, int64_t n, const int32_t* i, const int32_t* j, float* out);

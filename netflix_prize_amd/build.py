@@ -65,8 +65,9 @@ def build(verbose: bool = False) -> str:
     host = lambda src, obj: [hipcc, "-x", "c++", *HOST_FLAGS, "-D__HIP_PLATFORM_AMD__", *inc, "-c",
                              os.path.join(CSRC, src), "-o", os.path.join(OBJ, obj)]
     dev = lambda src, obj: [hipcc, *HIP_FLAGS, *inc, "-c", os.path.join(CSRC, src), "-o", os.path.join(OBJ, obj)]
-    jobs = [host("tx_host.cpp", "tx_host.o"), host("knn_host.cpp", "knn_host.o"),
-            dev("tx_kernels.hip", "tx_kernels.o"), dev("knn_kernels.hip", "knn_kernels.o")]
+    jobs = [host("tx_host.cpp", "tx_host.o"), host("knn_host.cpp", "knn_host.o"), host("knn3_host.cpp", "knn3_host.o"),
+            dev("tx_kernels.hip", "tx_kernels.o"), dev("knn_kernels.hip", "knn_kernels.o"),
+            dev("knn3_kernels.hip", "knn3_kernels.o")]
     with ThreadPoolExecutor(max_workers=len(jobs)) as ex:
         times = list(ex.map(_run, jobs))
     if verbose:

@@ -315,6 +315,17 @@ hipError_t launch_knns_sim(const uint32_t* off, const int32_t* items, int32_t n_
     return hipGetLastError();
 }
 
+hipError_t launch_support(const uint32_t* off, const int32_t* items, int32_t n_users, int64_t n_rows, float* S,
+                          float* diag, int ip, hipStream_t st) {
+    (void)hipGetLastError();
+    int64_t blocks = (n_rows + 3) / 4;
+    if (blocks > 65536) blocks = 65536;
+    if (blocks > 0) hipLaunchKernelGGL(k_supp, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, off, items, n_users, n_rows, S, ip);
+    hipLaunchKernelGGL(k_mirror, dim3(ip / 32, ip / 32), dim3(256), 0, st, S, ip);
+    hipLaunchKernelGGL(k_diag, dim3((ip + 255) / 256), dim3(256), 0, st, S, ip, diag);
+    return hipGetLastError();
+}
+
 hipError_t launch_norms(const float* F, int fp, int rows, float* norm, hipStream_t st) {
     (void)hipGetLastError();
     hipLaunchKernelGGL(k_norms, dim3((rows + 3) / 4), dim3(256), 0, st, F, fp, rows, norm);

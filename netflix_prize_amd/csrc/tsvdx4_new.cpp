@@ -546,15 +546,44 @@ void fit2(const tx_config& cfg, const std::string& target, const std::string& jo
 //   target `1.0*{base}`.
 //   `<base>__knns` (run_knns, src/bin/tsvdx4-new.rs:49-52, 563-574): KnnsModel
 //   with SuppSource::Compute and shrinkage 100, target `{w}*{base}`.
-// One fit phase: knns when ks is given, else knnf.
+//   `<base>__knn3` (run_knn3, src/bin/tsvdx4-new.rs:43-46, 557-561): Knn3Model
+//   with Knn3Config::default, target `1.0*{base}`.
+// One fit phase: knn3 when k3 is given, knns when ks is given, else knnf.
 void knn_fit(const std::string& base, const std::string& target, const std::string& tr_set,
              const std::string& pr_set, const std::string& job, const std::string& preds_dir, bool phase1,
-             const knns_config* ks) {
+             const knns_config* ks, const knn3_config* k3 = nullptr) {
     tee("%s => %s\n", tr_set.c_str(), pr_set.c_str());
     Dataset tr = load_dataset(tr_set, target, preds_dir);
     Dataset pr = load_dataset(pr_set, target, preds_dir);
     knnf_ctx* ctx = nullptr;
     const tx_ratings t = tr.view(false);
+    if (k3) {
+        // Knn3Model::new (src/knn3.rs:350-362) and its progress lines (:292-340)
+        tee("  KNN3: computing baseline...\n");
+        tee("  KNN3: building inverted index...\n");
+        tee("  KNN3: computing per-item weights (%d items, parallel)...\n", tr.n_items);
+        knn3_ctx* c3 = nullptr;
+        check(knn3_create(k3, &t, &c3), "Knn3Model::new");
+        std::vector<int32_t> cnt(tr.n_items), it(static_cast<size_t>(tr.n_items) * 64);
+        std::vector<float> w(it.size());
+        check(knn3_weights(c3, cnt.data(), it.data(), w.data()), "Knn3Model::new");
+        int64_t total = 0, nonzero = 0;
+        for (int32_t c : cnt) {
+            total += c;
+            nonzero += c > 0;
+        }
+        tee("  KNN3: %lld items with neighbors, %lld total weights\n", static_cast<long long>(nonzero),
+            static_cast<long long>(total));
+        std::vector<float> p(pr.n_ratings());
+        check(knn3_predict(c3, pr.n_ratings(), pr.user_idxs.data(), pr.item_idxs.data(), pr.dates.data(), p.data()),
+              "predict");
+        knn3_destroy(c3);
+        for (int64_t k = 0; k < pr.n_ratings(); ++k)
+            p[k] = p[k] + (static_cast<float>(pr.raw_ratings[k]) - pr.residuals[k]);
+        if (phase1) tee("%s RMSE %.5f\n", pr_set.c_str(), rmse_from_preds(p, pr));
+        write_preds(preds_dir + "/" + job + "." + pr_set + ".npy", p);
+        return;
+    }
     if (ks) {
         const tx_ratings h = pr.view(true);
         check(knns_create(ks, &t, &h, &ctx), "KnnsModel::new");
@@ -583,21 +612,27 @@ void knn_fit(const std::string& base, const std::string& target, const std::stri
 }
 
 void fit2_knn(const std::string& job, const std::string& base, const std::string& target, const Split& sp,
-              const knns_config* ks) {
+              const knns_config* ks, const knn3_config* k3 = nullptr) {
     const std::string preds_dir = sp.preds_dir;
     mkdir(preds_dir.c_str(), 0755);
     g_log = std::fopen((preds_dir + "/" + job + ".out").c_str(), "w");
     if (!g_log) panic(preds_dir + "/" + job + ".out: cannot create");
     tee("[%s]\n", job.c_str());
     tee("target = \"%s\"\n", target.c_str());
-    if (ks)
+    if (k3)
+        tee("Knn3Config { threshold: %s, k_min: %d, k_max: %d, shrinkage: %s, reg: %s, x: %s, bl_reg_m: %s, "
+            "bl_reg_u: %s }\n",
+            rust_f32(k3->threshold).c_str(), k3->k_min, k3->k_max, rust_f32(k3->shrinkage).c_str(),
+            rust_f32(k3->reg).c_str(), rust_f32(k3->x).c_str(), rust_f32(k3->bl_reg_m).c_str(),
+            rust_f32(k3->bl_reg_u).c_str());
+    else if (ks)
         tee("KnnsConfig { k: %d, shrinkage: %s, scaling: %s, tau: %s, supp_source: Compute }\n", ks->k,
             rust_f32(ks->shrinkage).c_str(), rust_f32(ks->scaling).c_str(), rust_f32(ks->tau).c_str());
     else
         tee("KnnfConfig { factors: \"%s/%s.ifeat\", k: 25, scaling: 1.0, tau: 0.025 }\n", preds_dir.c_str(),
             base.c_str());
-    knn_fit(base, target, sp.tr, sp.pr, job, preds_dir, true, ks);
-    knn_fit(base, target, sp.fulltrain_tr, sp.fulltrain_pr, job, preds_dir, false, ks);
+    knn_fit(base, target, sp.tr, sp.pr, job, preds_dir, true, ks, k3);
+    knn_fit(base, target, sp.fulltrain_tr, sp.fulltrain_pr, job, preds_dir, false, ks, k3);
     tee("\n");
     std::fclose(g_log);
     g_log = nullptr;
@@ -919,7 +954,17 @@ int main(int argc, char** argv) {
         fit2_knn(job, base, std::string(kj.w) + "*" + base, SPLIT_NEW, &ks);
         return 0;
     }
+    static const char* kKnn3[] = {"tsvdx4-d0tb__knn3", "tsvdx4-h8nb__knn3", "tsvdx4-n8nbo__knn3",
+                                  "tsvdx4-p0tbo__knn3"};
+    for (const char* name : kKnn3) {
+        if (job != name) continue;
+        if (print_only) return 0;
+        const std::string base = job.substr(0, job.size() - std::string("__knn3").size());
+        const knn3_config k3{0.25f, 10, 60, 20000.0f, 0.01f, 0.8f, 25.0f, 10.0f};  // Knn3Config::default
+        fit2_knn(job, base, "1.0*" + base, SPLIT_NEW, nullptr, &k3);
+        return 0;
+    }
     if (job.find("__knn") != std::string::npos)
-        panic("job " + job + ": this kNN chain (knn3 / knn-d) is outside this build");
+        panic("job " + job + ": this kNN chain (knn-d) is outside this build");
     panic("invalid job name: " + job);
 }

@@ -1,7 +1,9 @@
-"""Python mirror of KnnfModel (src/knnf.rs) over the GPU library's C ABI
-(include/tsvd_knn.h): item-item kNN over a factor table (a tsvdx4 job's
-`.ifeat`), cosine similarity, temporal decay, top-k weighted mean of the
-user's residuals.  Used by the tests and the `*__knnf` jobs' Python path."""
+"""Python mirror of KnnfModel (src/knnf.rs), KnnsModel (src/knns.rs) and
+Knn3Model (src/knn3.rs) over the GPU library's C ABI (include/tsvd_knn.h):
+item-item kNN over a factor table (a tsvdx4 job's `.ifeat`) or co-occurrence
+support with temporal decay and a top-k weighted mean of the user's
+residuals, and item-item NNLS regression weights.  Used by the tests and the
+bench."""
 from __future__ import annotations
 
 import ctypes as C
@@ -125,3 +127,103 @@ class KnnsModel(KnnfModel):
         h = C.c_void_p()
         tx._check(L.knns_create(C.byref(c), C.byref(self._tr.s), C.byref(self._pr.s), C.byref(h)), "knns_create")
         self._h = h
+
+
+@dataclass
+class Knn3Config:
+    """Knn3Config (src/knn3.rs:7-31); defaults of Knn3Config::default."""
+    threshold: float = 0.25
+    k_min: int = 10
+    k_max: int = 60
+    shrinkage: float = 20000.0
+    reg: float = 0.01
+    x: float = 0.8
+    bl_reg_m: float = 25.0
+    bl_reg_u: float = 10.0
+
+
+class _CKnn3(C.Structure):
+    _fields_ = [("threshold", C.c_float), ("k_min", C.c_int32), ("k_max", C.c_int32), ("shrinkage", C.c_float),
+                ("reg", C.c_float), ("x", C.c_float), ("bl_reg_m", C.c_float), ("bl_reg_u", C.c_float)]
+
+
+def _lib3():
+    L = tx.lib()
+    if not getattr(L, "_knn3_bound", False):
+        vp = C.c_void_p
+        L.knn3_create.argtypes = [vp, vp, C.POINTER(vp)]
+        L.knn3_predict.argtypes = [vp, C.c_int64, vp, vp, vp, vp]
+        L.knn3_weights.argtypes = [vp, vp, vp, vp]
+        L.knn3_candidates.argtypes = [vp, vp, vp]
+        L.knn3_baseline.argtypes = [vp, vp, vp, vp]
+        L.knn3_destroy.argtypes = [vp]
+        for name in ("knn3_create", "knn3_predict", "knn3_weights", "knn3_candidates", "knn3_baseline",
+                     "knn3_destroy"):
+            getattr(L, name).restype = C.c_int
+        L._knn3_bound = True
+    return L
+
+
+class Knn3Model:
+    """Knn3Model::new (src/knn3.rs:350-362) on the GPU: per item, regression
+    weights on its most related items (NNLS over the item's users); no SGD
+    epochs.  predict(u, i) sums weight x residual over the neighbours u rated
+    (src/knn3.rs:379-389)."""
+
+    n_epochs = 0
+
+    def __init__(self, tr, cfg: Knn3Config = Knn3Config()):
+        self.cfg = cfg
+        self.n_items, self.n_users = int(tr.n_items), int(tr.n_users)
+        self._tr = tx._RatingsRef(tr)
+        c = _CKnn3(float(cfg.threshold), int(cfg.k_min), int(cfg.k_max), float(cfg.shrinkage), float(cfg.reg),
+                   float(cfg.x), float(cfg.bl_reg_m), float(cfg.bl_reg_u))
+        h = C.c_void_p()
+        tx._check(_lib3().knn3_create(C.byref(c), C.byref(self._tr.s), C.byref(h)), "knn3_create")
+        self._h = h
+
+    def predict(self, users, items, days=None) -> np.ndarray:
+        u = np.ascontiguousarray(users, np.int32)
+        i = np.ascontiguousarray(items, np.int32)
+        out = np.empty(len(u), np.float32)
+        tx._check(_lib3().knn3_predict(self._h, len(u), u.ctypes.data, i.ctypes.data, None, out.ctypes.data),
+                  "knn3_predict")
+        return out
+
+    def neighbors(self) -> list:
+        """[(item, weight), ...] per item, in candidate order."""
+        cnt = np.empty(self.n_items, np.int32)
+        it = np.empty((self.n_items, 64), np.int32)
+        w = np.empty((self.n_items, 64), np.float32)
+        tx._check(_lib3().knn3_weights(self._h, cnt.ctypes.data, it.ctypes.data, w.ctypes.data), "knn3_weights")
+        return [[(int(it[m, a]), float(w[m, a])) for a in range(cnt[m])] for m in range(self.n_items)]
+
+    def candidates(self) -> list:
+        cnt = np.empty(self.n_items, np.int32)
+        it = np.empty((self.n_items, 64), np.int32)
+        tx._check(_lib3().knn3_candidates(self._h, cnt.ctypes.data, it.ctypes.data), "knn3_candidates")
+        return [it[m, :cnt[m]].copy() for m in range(self.n_items)]
+
+    def baseline(self):
+        mu = C.c_float()
+        bm = np.empty(self.n_items, np.float32)
+        bu = np.empty(self.n_users, np.float32)
+        tx._check(_lib3().knn3_baseline(self._h, C.byref(mu), bm.ctypes.data, bu.ctypes.data), "knn3_baseline")
+        return np.float32(mu.value), bm, bu
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _lib3().knn3_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -27,6 +27,15 @@ def test_library_exports_every_declared_entry_point():
         assert hasattr(L, n), n
 
 
+def test_library_exports_every_declared_knn_entry_point():
+    text = open(os.path.join(ROOT, "include", "tsvd_knn.h")).read()
+    names = sorted(set(re.findall(r"^int\s+(knn\w+)\(", text, re.M)))
+    assert {"knnf_create", "knns_create", "knn3_create", "knn3_predict", "knn3_weights"} <= set(names)
+    L = C.CDLL(tx.LIB_PATH)
+    for n in names:
+        assert hasattr(L, n), n
+
+
 def test_encoder_library_exports_every_declared_entry_point():
     text = open(os.path.join(ROOT, "include", "tsvd_enc.h")).read()
     names = sorted(set(re.findall(r"^(?:int|const char\*)\s+(enc_\w+)\(", text, re.M)))

@@ -178,3 +178,38 @@ def test_knns_job_end_to_end(gpu, tmp_path):
     want = ref.predict(pr.user_idxs[q], pr.item_idxs[q], pr.dates[q]) + base[q]
     clear = ref.margins(pr.user_idxs[q], pr.item_idxs[q], pr.dates[q]) > 1e-4
     assert np.max(np.abs(kp[q][clear] - want[clear])) < 1e-4
+
+
+@pytest.mark.gpu
+def test_knn3_job_end_to_end(gpu, tmp_path):
+    """tsvdx4-d0tb (bias-only timeSVD++) then its kNN chain tsvdx4-d0tb__knn3
+    (Knn3Model with Knn3Config::default on the base job's residuals; run_knn3,
+    src/bin/tsvdx4-new.rs:43-46, 557-561) on the 10k x 2k x 1M split: output
+    files, the logged config and RMSE, and predictions equal to the oracle's."""
+    tr, pr = synthetic("small")
+    _write_split(str(tmp_path), tr, pr)
+    env = dict(os.environ, TSVD_N_USERS=str(tr.n_users), TSVD_N_ITEMS=str(tr.n_items),
+               TSVD_DATA_DIR=str(tmp_path / "data"))
+    for job in ("tsvdx4-d0tb", "tsvdx4-d0tb__knn3"):
+        out = subprocess.run([DRIVER, job], cwd=tmp_path, env=env, capture_output=True, text=True, timeout=600)
+        assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
+    pd = tmp_path / "preds_new"
+    base = np.load(pd / "tsvdx4-d0tb.probex.npy")
+    kp = np.load(pd / "tsvdx4-d0tb__knn3.probex.npy")
+    assert kp.shape == (pr.n_ratings,) and np.all(np.isfinite(kp))
+    assert np.load(pd / "tsvdx4-d0tb__knn3.qual.npy").shape == (pr.n_ratings,)
+    klog = (pd / "tsvdx4-d0tb__knn3.out").read_text()
+    assert ("Knn3Config { threshold: 0.25, k_min: 10, k_max: 60, shrinkage: 20000.0, reg: 0.01, x: 0.8, "
+            "bl_reg_m: 25.0, bl_reg_u: 10.0 }") in klog
+    assert "  KNN3: computing per-item weights (2000 items, parallel)..." in klog
+    logged = float(re.search(r"probex RMSE ([0-9.]+)", klog).group(1))
+    rmse = float(np.sqrt(np.mean((kp.astype(np.float64) - pr.raw_ratings) ** 2)))
+    assert abs(logged - rmse) < 1e-5, (logged, rmse)
+    from oracle.knn3_ref import Knn3Ref
+    res_tr = tr.raw_ratings.astype(np.float32) - np.load(pd / "tsvdx4-d0tb.trainx.npy")
+    tr_k = tx.Dataset.from_arrays(tr.user_idxs, tr.item_idxs, tr.raw_ratings, tr.dates, tr.n_users, tr.n_items,
+                                  residuals=res_tr)
+    ref = Knn3Ref(tr_k)
+    q = np.arange(0, pr.n_ratings, 50)
+    want = ref.predict(pr.user_idxs[q], pr.item_idxs[q]) + base[q]
+    assert np.max(np.abs(kp[q] - want)) < 1e-4
