@@ -283,6 +283,21 @@ def main(argv=None) -> int:
         nu = np.bincount(tr.user_idxs, minlength=tr.n_users) + np.bincount(pr.user_idxs, minlength=tr.n_users)
         knnf["knns"] = {"create_s": round(t_create, 4), "support_pairs": int(np.sum(nu.astype(np.int64) ** 2)),
                         "predict_s": round(t_pred, 4), "queries_per_s": round(pr.n_ratings / t_pred, 1)}
+        # Knn3Model (the *__knn3 jobs, Knn3Config::default) on the training
+        # set's residual targets: co-rating counts, candidates, the FP64 normal
+        # equations and NNLS of every item, then the held-out queries
+        t0 = time.perf_counter()
+        km = knn.Knn3Model(tr, knn.Knn3Config())
+        t_create = time.perf_counter() - t0
+        km.predict(pr.user_idxs, pr.item_idxs)
+        t0 = time.perf_counter()
+        km.predict(pr.user_idxs, pr.item_idxs)
+        t_pred = time.perf_counter() - t0
+        n_w = int(sum(len(x) for x in km.neighbors()))
+        km.close()
+        nt = np.bincount(tr.user_idxs, minlength=tr.n_users).astype(np.int64)
+        knnf["knn3"] = {"create_s": round(t_create, 4), "pair_lookups": int(np.sum(nt ** 2)), "weights": n_w,
+                        "predict_s": round(t_pred, 4), "queries_per_s": round(pr.n_ratings / t_pred, 1)}
         log(f"bench: knnf {knnf}")
 
     # the tsvdx4-60 job's model (lm set with day terms, low_memory false, f=60,
