@@ -249,10 +249,11 @@ int knn3_create(const knn3_config* cfg, const tx_ratings* train, knn3_ctx** out)
     HIPC(up(d_iuser, i_user.data(), sizeof(int32_t) * NR));
     HIPC(up(d_ierr, i_err.data(), sizeof(float) * NR));
     HIPC(up(d_irp, i_rp.data(), sizeof(float) * NR));
-    HIPC(hipMemsetAsync(cnt, 0, sizeof(float) * static_cast<size_t>(ip) * ip, c->st));
     HIPC(hipMemsetAsync(v.w_cnt, 0, sizeof(int32_t) * I, c->st));
     // co-rating counts of the non-test ratings (the diagonal: each item's users)
-    HIPC(knn::launch_support(d_uoff, d_uitem, U, static_cast<int64_t>(NR), cnt, diag, ip, c->st));
+    uint32_t* scratch = nullptr;
+    HIPC(c->alloc(&scratch, knn::support_scratch_words(static_cast<int64_t>(NR), ip)));
+    HIPC(knn::launch_support(d_uoff, d_uitem, U, static_cast<int64_t>(NR), cnt, diag, ip, scratch, c->st));
     v.cnt = cnt;
     v.diag = diag;
     v.u_off = d_uoff;
@@ -272,7 +273,7 @@ int knn3_create(const knn3_config* cfg, const tx_ratings* train, knn3_ctx** out)
     HIPC(knn::launch_knn3_build(v, c->st));
     HIPC(hipStreamSynchronize(c->st));  // host arrays above are temporaries
     // build-only device buffers
-    for (const void* p : {static_cast<const void*>(cnt), static_cast<const void*>(v.sys),
+    for (const void* p : {static_cast<const void*>(cnt), static_cast<const void*>(scratch), static_cast<const void*>(v.sys),
                           static_cast<const void*>(v.rhs), static_cast<const void*>(d_ioff),
                           static_cast<const void*>(d_iuser), static_cast<const void*>(d_ierr),
                           static_cast<const void*>(d_irp)})

@@ -200,9 +200,11 @@ int knns_create(const knns_config* cfg, const tx_ratings* train, const tx_rating
     if (!items.empty())
         HIPC(hipMemcpyAsync(ditems, items.data(), sizeof(int32_t) * items.size(), hipMemcpyHostToDevice, c->st));
     float* S = const_cast<float*>(c->v.S);
-    HIPC(hipMemsetAsync(S, 0, sizeof(float) * static_cast<size_t>(c->ip) * c->ip, c->st));
+    uint32_t* scratch = nullptr;
+    HIPC(hipMalloc(&scratch, sizeof(uint32_t) * knn::support_scratch_words(static_cast<int64_t>(items.size()), c->ip)));
+    std::unique_ptr<void, void (*)(void*)> g(scratch, [](void* p) { (void)hipFree(p); });
     HIPC(knn::launch_knns_sim(duoff, ditems, U, static_cast<int64_t>(items.size()), S, ddiag, c->ip, cfg->shrinkage,
-                              c->st));
+                              scratch, c->st));
     HIPC(hipStreamSynchronize(c->st));
     c->v.k = cfg->k;
     c->v.scaling = cfg->scaling;

@@ -238,31 +238,91 @@ __global__ __launch_bounds__(256) void k_knnf_predict(KnnView v, int64_t n, cons
 // ---------------------------------------------------------------------------
 // KnnsModel (src/knns.rs): the support matrix of build_bin_nbstats(.., "supp")
 // (src/nbstats.rs:73-117): supp[i][j] += 1 for every ordered pair of items of
-// each user's training and held-out items (diagonal included).  The matrix is
-// symmetric, so each user's list is sorted by item and one wave per list
-// position r adds the pairs (r, s), s >= r, to the upper triangle (an equal
-// item at s > r stands for both orders and adds 2); k_mirror then copies the
-// upper triangle down.  FP32 adds of small integers stay exact below 2^24.
+// each user's list (diagonal included).  The matrix is symmetric and each
+// user's list is sorted by item, so the entry of item i at list position t
+// contributes the pairs (t, s), s >= t, of its list tail to the upper
+// triangle (an equal item at s > t stands for both orders and adds 2);
+// k_mirror then copies the upper triangle down.
+// Row-owned counting: the list entries are indexed by item (k_item_hist,
+// k_scan_offsets, k_item_scatter: the position t and the list end of each
+// entry), then one block per row i counts the tails of i's entries with LDS
+// atomics over a kRowChunk-wide window of columns and writes the row once.
+// Integer counts, converted to FP32 exactly below 2^24.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_supp(const uint32_t* __restrict__ off, const int32_t* __restrict__ items,
-                                              int32_t n_users, int64_t n_rows, float* __restrict__ supp, int ip) {
+constexpr int kRowChunk = 20480;  // columns per LDS window (80 KB)
+
+__global__ __launch_bounds__(256) void k_item_hist(const int32_t* __restrict__ items, int64_t n_rows,
+                                                   uint32_t* __restrict__ icnt) {
+    for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < n_rows;
+         t += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        atomicAdd(&icnt[items[t] + 1], 1u);
+}
+
+// exclusive offsets in place over icnt[0..n] (icnt[0] = 0 on entry), copied to cursor
+__global__ __launch_bounds__(1024) void k_scan_offsets(uint32_t* __restrict__ icnt, uint32_t* __restrict__ cursor,
+                                                       int n) {
+    __shared__ uint32_t part[1024];
+    const int tid = threadIdx.x;
+    const int per = (n + 1 + 1023) / 1024;
+    const int a = tid * per, b = a + per < n + 1 ? a + per : n + 1;
+    uint32_t sum = 0;
+    for (int k = a; k < b; ++k) sum += icnt[k];
+    part[tid] = sum;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const uint32_t add = tid >= o ? part[tid - o] : 0u;
+        __syncthreads();
+        part[tid] += add;
+        __syncthreads();
+    }
+    uint32_t run = part[tid] - sum;  // exclusive prefix of this thread's chunk
+    for (int k = a; k < b; ++k) {
+        run += icnt[k];
+        icnt[k] = run;  // inclusive over icnt[0..k] = offset of item k (icnt[k] counts item k - 1)
+        cursor[k] = run;
+    }
+}
+
+// one wave per user: each list entry lands in its item's range with its position and list end
+__global__ __launch_bounds__(256) void k_item_scatter(const uint32_t* __restrict__ off, const int32_t* __restrict__ items,
+                                                      int32_t n_users, uint32_t* __restrict__ cursor,
+                                                      uint32_t* __restrict__ ient, uint32_t* __restrict__ iend) {
     const int lane = threadIdx.x & 63;
     const int64_t nwaves = static_cast<int64_t>(gridDim.x) * (blockDim.x >> 6);
-    for (int64_t r = static_cast<int64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6); r < n_rows; r += nwaves) {
-        // the row's user: last u with off[u] <= r
-        int32_t lo = 0, hi = n_users;
-        while (hi - lo > 1) {
-            const int32_t mid = (lo + hi) >> 1;
-            if (off[mid] <= static_cast<uint64_t>(r)) lo = mid;
-            else hi = mid;
+    for (int64_t u = static_cast<int64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6); u < n_users;
+         u += nwaves) {
+        const uint32_t e = off[u + 1];
+        for (uint32_t t = off[u] + lane; t < e; t += 64) {
+            const uint32_t p = atomicAdd(&cursor[items[t]], 1u);
+            ient[p] = t;
+            iend[p] = e;
         }
-        const uint32_t b = off[lo + 1];
-        const int32_t ir = items[r];
-        float* row = supp + static_cast<size_t>(ir) * ip;
-        for (uint32_t s = static_cast<uint32_t>(r) + lane; s < b; s += 64) {
-            const int32_t is = items[s];
-            atomicAdd(row + is, (is == ir && s != static_cast<uint32_t>(r)) ? 2.0f : 1.0f);
+    }
+}
+
+__global__ __launch_bounds__(512) void k_supp_rows(const int32_t* __restrict__ items, const uint32_t* __restrict__ ioff,
+                                                   const uint32_t* __restrict__ ient, const uint32_t* __restrict__ iend,
+                                                   float* __restrict__ supp, int ip) {
+    __shared__ uint32_t cnt[kRowChunk];
+    const int i = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const uint32_t q0 = ioff[i], q1 = ioff[i + 1];
+    float* row = supp + static_cast<size_t>(i) * ip;
+    for (int c0 = 0; c0 < ip; c0 += kRowChunk) {
+        const int w = ip - c0 < kRowChunk ? ip - c0 : kRowChunk;
+        for (int j = tid; j < w; j += 512) cnt[j] = 0u;
+        __syncthreads();
+        if (c0 + w > i) {  // the tails hold items >= i only
+            for (uint32_t q = q0 + wv; q < q1; q += 8) {
+                const uint32_t t = ient[q], e = iend[q];
+                for (uint32_t s = t + lane; s < e; s += 64) {
+                    const int j = items[s] - c0;
+                    if (j >= 0 && j < w) atomicAdd(&cnt[j], (j + c0 == i && s != t) ? 2u : 1u);
+                }
+            }
         }
+        __syncthreads();
+        for (int j = tid; j < w; j += 512) row[c0 + j] = static_cast<float>(cnt[j]);
+        __syncthreads();
     }
 }
 
@@ -303,26 +363,39 @@ __global__ __launch_bounds__(256) void k_knns_sim(float* __restrict__ S, const f
     }
 }
 
-hipError_t launch_knns_sim(const uint32_t* off, const int32_t* items, int32_t n_users, int64_t n_rows, float* S,
-                           float* diag, int ip, float shrinkage, hipStream_t st) {
-    (void)hipGetLastError();
-    int64_t blocks = (n_rows + 3) / 4;
-    if (blocks > 65536) blocks = 65536;
-    if (blocks > 0) hipLaunchKernelGGL(k_supp, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, off, items, n_users, n_rows, S, ip);
-    hipLaunchKernelGGL(k_mirror, dim3(ip / 32, ip / 32), dim3(256), 0, st, S, ip);
-    hipLaunchKernelGGL(k_diag, dim3((ip + 255) / 256), dim3(256), 0, st, S, ip, diag);
-    hipLaunchKernelGGL(k_knns_sim, dim3(65536), dim3(256), 0, st, S, diag, ip, shrinkage);
-    return hipGetLastError();
+size_t support_scratch_words(int64_t n_rows, int ip) {
+    return 2 * (static_cast<size_t>(ip) + 1) + 2 * static_cast<size_t>(n_rows);
 }
 
 hipError_t launch_support(const uint32_t* off, const int32_t* items, int32_t n_users, int64_t n_rows, float* S,
-                          float* diag, int ip, hipStream_t st) {
+                          float* diag, int ip, uint32_t* scratch, hipStream_t st) {
     (void)hipGetLastError();
-    int64_t blocks = (n_rows + 3) / 4;
+    uint32_t* icnt = scratch;
+    uint32_t* cursor = icnt + ip + 1;
+    uint32_t* ient = cursor + ip + 1;
+    uint32_t* iend = ient + n_rows;
+    hipError_t e = hipMemsetAsync(icnt, 0, sizeof(uint32_t) * (static_cast<size_t>(ip) + 1), st);
+    if (e != hipSuccess) return e;
+    int64_t blocks = (n_rows + 255) / 256;
     if (blocks > 65536) blocks = 65536;
-    if (blocks > 0) hipLaunchKernelGGL(k_supp, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, off, items, n_users, n_rows, S, ip);
+    if (blocks > 0) hipLaunchKernelGGL(k_item_hist, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, items, n_rows, icnt);
+    hipLaunchKernelGGL(k_scan_offsets, dim3(1), dim3(1024), 0, st, icnt, cursor, ip);
+    int64_t ublocks = (static_cast<int64_t>(n_users) + 3) / 4;
+    if (ublocks > 65536) ublocks = 65536;
+    if (ublocks > 0)
+        hipLaunchKernelGGL(k_item_scatter, dim3(static_cast<unsigned>(ublocks)), dim3(256), 0, st, off, items, n_users,
+                           cursor, ient, iend);
+    hipLaunchKernelGGL(k_supp_rows, dim3(ip), dim3(512), 0, st, items, icnt, ient, iend, S, ip);
     hipLaunchKernelGGL(k_mirror, dim3(ip / 32, ip / 32), dim3(256), 0, st, S, ip);
     hipLaunchKernelGGL(k_diag, dim3((ip + 255) / 256), dim3(256), 0, st, S, ip, diag);
+    return hipGetLastError();
+}
+
+hipError_t launch_knns_sim(const uint32_t* off, const int32_t* items, int32_t n_users, int64_t n_rows, float* S,
+                           float* diag, int ip, float shrinkage, uint32_t* scratch, hipStream_t st) {
+    hipError_t e = launch_support(off, items, n_users, n_rows, S, diag, ip, scratch, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_knns_sim, dim3(65536), dim3(256), 0, st, S, diag, ip, shrinkage);
     return hipGetLastError();
 }
 

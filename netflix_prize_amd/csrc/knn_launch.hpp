@@ -20,13 +20,15 @@ struct KnnView {
 
 hipError_t launch_norms(const float* F, int fp, int rows, float* norm, hipStream_t st);
 hipError_t launch_cosine(const float* F, int fp, const float* norm, float* S, int ip, hipStream_t st);
-// support matrix of the per-user item lists (off[U+1], items), then the
-// KnnsModel similarity in place; S is ip x ip and zeroed by the caller
+// support matrix of the per-user item lists (off[U+1], items sorted within
+// each list), then the KnnsModel similarity in place; S is ip x ip (every
+// element written); scratch holds support_scratch_words(n_rows, ip) words
+size_t support_scratch_words(int64_t n_rows, int ip);
 hipError_t launch_knns_sim(const uint32_t* off, const int32_t* items, int32_t n_users, int64_t n_rows, float* S,
-                           float* diag, int ip, float shrinkage, hipStream_t st);
+                           float* diag, int ip, float shrinkage, uint32_t* scratch, hipStream_t st);
 // the support matrix alone (the first two steps of launch_knns_sim) and its diagonal
 hipError_t launch_support(const uint32_t* off, const int32_t* items, int32_t n_users, int64_t n_rows, float* S,
-                          float* diag, int ip, hipStream_t st);
+                          float* diag, int ip, uint32_t* scratch, hipStream_t st);
 hipError_t launch_knnf_predict(const KnnView& v, int64_t n, const int32_t* us, const int32_t* is, const int16_t* ds,
                                float* out, hipStream_t st);
 
