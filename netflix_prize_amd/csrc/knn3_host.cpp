@@ -84,16 +84,25 @@ struct Rated {
     float err, rp;
 };
 
+unsigned worker_count() {
+    const unsigned nt = std::thread::hardware_concurrency();
+    return nt == 0 ? 1 : (nt > 16 ? 16 : nt);
+}
+
+template <typename F>
+void run_workers(unsigned nt, F&& f) {
+    std::vector<std::thread> pool;
+    for (unsigned w = 0; w < nt; ++w) pool.emplace_back([&, w] { f(w); });
+    for (auto& th : pool) th.join();
+}
+
+// f(u) for every user, users interleaved over the workers
 template <typename F>
 void parallel_users(int32_t U, F&& f) {
-    unsigned nt = std::thread::hardware_concurrency();
-    nt = nt == 0 ? 1 : (nt > 16 ? 16 : nt);
-    std::vector<std::thread> pool;
-    for (unsigned w = 0; w < nt; ++w)
-        pool.emplace_back([&, w] {
-            for (int64_t u = w; u < U; u += nt) f(static_cast<int32_t>(u));
-        });
-    for (auto& th : pool) th.join();
+    const unsigned nt = worker_count();
+    run_workers(nt, [&](unsigned w) {
+        for (int64_t u = w; u < U; u += nt) f(static_cast<int32_t>(u));
+    });
 }
 
 }  // namespace
@@ -146,72 +155,82 @@ int knn3_create(const knn3_config* cfg, const tx_ratings* train, knn3_ctx** out)
     for (int32_t m = 0; m < I; ++m)
         c->bm[m] = static_cast<float>(bm_sum[m] / (static_cast<double>(bm_cnt[m]) + static_cast<double>(cfg->bl_reg_m)));
     c->bu.assign(U, 0.0f);
-    std::vector<uint32_t> u_off(static_cast<size_t>(U) + 1, 0);
-    {
-        int64_t t = 0;
-        for (int32_t u = 0; u < U; ++u) {
-            double sum = 0.0;
-            int64_t cnt = 0;
-            for (; t < N && train->user_idxs[t] == u; ++t) {
-                if (test && test[t] != 0) continue;
-                sum += static_cast<double>((static_cast<float>(raw[t]) - mu) - c->bm[train->item_idxs[t]]);
-                ++cnt;
-            }
-            if (cnt > 0) c->bu[u] = static_cast<float>(sum / (static_cast<double>(cnt) + static_cast<double>(cfg->bl_reg_u)));
-            u_off[u + 1] = u_off[u] + static_cast<uint32_t>(cnt);
-        }
-    }
     c->mu = mu;
+    // bu per user (each sum sequential in rating order; users in parallel)
+    std::vector<int64_t> row_off(static_cast<size_t>(U) + 1, 0);
+    for (int64_t t = 0; t < N; ++t) row_off[train->user_idxs[t] + 1]++;
+    for (int32_t u = 0; u < U; ++u) row_off[u + 1] += row_off[u];
+    std::vector<uint32_t> u_off(static_cast<size_t>(U) + 1, 0);
+    parallel_users(U, [&](int32_t u) {
+        double sum = 0.0;
+        int64_t cnt = 0;
+        for (int64_t t = row_off[u]; t < row_off[u + 1]; ++t) {
+            if (test && test[t] != 0) continue;
+            sum += static_cast<double>((static_cast<float>(raw[t]) - mu) - c->bm[train->item_idxs[t]]);
+            ++cnt;
+        }
+        if (cnt > 0) c->bu[u] = static_cast<float>(sum / (static_cast<double>(cnt) + static_cast<double>(cfg->bl_reg_u)));
+        u_off[u + 1] = static_cast<uint32_t>(cnt);
+    });
+    for (int32_t u = 0; u < U; ++u) u_off[u + 1] += u_off[u];
     // per-user non-test ratings with err and r_prime (src/knn3.rs:295-300), sorted by item
     const uint32_t NR = u_off[U];
-    std::vector<Rated> rated(NR);
-    {
-        int64_t t = 0;
-        for (int32_t u = 0; u < U; ++u) {
-            uint32_t p = u_off[u];
-            for (; t < N && train->user_idxs[t] == u; ++t) {
-                if (test && test[t] != 0) continue;
-                const int32_t m = train->item_idxs[t];
-                const float rp = ((static_cast<float>(raw[t]) - mu) - c->bm[m]) - c->bu[u];
-                rated[p++] = Rated{m, res[t], rp};
-            }
-        }
-    }
+    std::vector<int32_t> u_item(NR);
+    std::vector<float> u_err(NR), u_rp(NR);
     std::atomic<int32_t> dup_user{-1};
     parallel_users(U, [&](int32_t u) {
-        auto b = rated.begin() + u_off[u], e = rated.begin() + u_off[u + 1];
-        std::sort(b, e, [](const Rated& x, const Rated& y) { return x.item < y.item; });
-        for (auto it = b; it + 1 < e; ++it)
-            if (it->item == (it + 1)->item) dup_user.store(u);  // any one of them is reported
+        thread_local std::vector<Rated> buf;
+        buf.clear();
+        for (int64_t t = row_off[u]; t < row_off[u + 1]; ++t) {
+            if (test && test[t] != 0) continue;
+            const int32_t m = train->item_idxs[t];
+            buf.push_back(Rated{m, res[t], ((static_cast<float>(raw[t]) - mu) - c->bm[m]) - c->bu[u]});
+        }
+        std::sort(buf.begin(), buf.end(), [](const Rated& x, const Rated& y) { return x.item < y.item; });
+        uint32_t p = u_off[u];
+        for (size_t k = 0; k < buf.size(); ++k, ++p) {
+            if (k > 0 && buf[k].item == buf[k - 1].item) dup_user.store(u);  // any one of them is reported
+            u_item[p] = buf[k].item;
+            u_err[p] = buf[k].err;
+            u_rp[p] = buf[k].rp;
+        }
     });
     if (dup_user.load() >= 0)
         return fail(TX_EINVAL, "knn3_create: user %d rates an item twice (one rating per user and item expected)",
                     dup_user.load());
-    std::vector<int32_t> u_item(NR);
-    std::vector<float> u_err(NR), u_rp(NR);
-    for (uint32_t p = 0; p < NR; ++p) {
-        u_item[p] = rated[p].item;
-        u_err[p] = rated[p].err;
-        u_rp[p] = rated[p].rp;
-    }
-    // each item's users in ascending order (the inverted index, src/knn3.rs:303-310)
+    // each item's users in ascending order (the inverted index, src/knn3.rs:303-310): a
+    // stable counting sort over contiguous user ranges, one per thread
     std::vector<uint32_t> i_off(static_cast<size_t>(I) + 1, 0);
-    for (uint32_t p = 0; p < NR; ++p) i_off[u_item[p] + 1]++;
-    for (int32_t m = 0; m < I; ++m) i_off[m + 1] += i_off[m];
     std::vector<int32_t> i_user(NR);
     std::vector<float> i_err(NR), i_rp(NR);
     {
-        std::vector<uint32_t> pos(i_off.begin(), i_off.end() - 1);
-        for (int32_t u = 0; u < U; ++u)
-            for (uint32_t p = u_off[u]; p < u_off[u + 1]; ++p) {
-                const uint32_t q = pos[u_item[p]]++;
-                i_user[q] = u;
-                i_err[q] = u_err[p];
-                i_rp[q] = u_rp[p];
+        const unsigned T = worker_count();
+        std::vector<std::vector<uint32_t>> tpos(T, std::vector<uint32_t>(static_cast<size_t>(I), 0));
+        auto range = [&](unsigned k) { return static_cast<int32_t>(static_cast<int64_t>(U) * k / T); };
+        run_workers(T, [&](unsigned k) {
+            for (uint32_t p = u_off[range(k)]; p < u_off[range(k + 1)]; ++p) tpos[k][u_item[p]]++;
+        });
+        uint32_t run = 0;
+        for (int32_t m = 0; m < I; ++m) {
+            i_off[m] = run;
+            for (unsigned k = 0; k < T; ++k) {
+                const uint32_t n = tpos[k][m];
+                tpos[k][m] = run;
+                run += n;
             }
+        }
+        i_off[I] = run;
+        run_workers(T, [&](unsigned k) {
+            std::vector<uint32_t>& pos = tpos[k];
+            for (int32_t u = range(k); u < range(k + 1); ++u)
+                for (uint32_t p = u_off[u]; p < u_off[u + 1]; ++p) {
+                    const uint32_t q = pos[u_item[p]]++;
+                    i_user[q] = u;
+                    i_err[q] = u_err[p];
+                    i_rp[q] = u_rp[p];
+                }
+        });
     }
-    rated.clear();
-    rated.shrink_to_fit();
 
     HIPC(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
     knn::Knn3View& v = c->v;

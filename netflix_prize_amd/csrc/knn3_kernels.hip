@@ -15,6 +15,8 @@ namespace {
 constexpr int kMaxK = 64;       // candidates per item (one wave of lanes)
 constexpr int kMapMax = 32768;  // items a k3_cov LDS map covers (one byte each)
 constexpr int kCh = 32;         // users staged per k3_cov round
+constexpr int kSlots = kCh / 4; // users per wave per round
+constexpr uint32_t kScanMax = 512;  // longer user lists are binary-searched
 
 __device__ __forceinline__ uint64_t lanes_below(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
 
@@ -173,9 +175,10 @@ __global__ __launch_bounds__(256) void k3_select(Knn3View v) {
 // (a = lane, wave g) owns C[a][16g .. 16g+15] of both signals, wave 0 also
 // d[a].  Every sum runs over the users in ascending order of exact FP64
 // products of FP32 values, so it equals the reference's sum bit for bit (its
-// skipped users add nothing; here they add +0).  A user list longer than 512
-// is probed by binary search per candidate, a shorter one scanned through an
-// LDS map item -> candidate.
+// skipped users add nothing; here they add +0).  A user list longer than
+// kScanMax is probed by binary search per candidate (a wave's long lists
+// searched together), a shorter one scanned through an LDS map item ->
+// candidate.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k3_cov(Knn3View v) {
 #pragma clang fp contract(off)
@@ -207,30 +210,74 @@ __global__ __launch_bounds__(256) void k3_cov(Knn3View v) {
             mr[tid] = v.i_rp[base + tid];
         }
         __syncthreads();
-        for (int r = wv; r < nr; r += 4) {
-            const int32_t u = v.i_user[base + r];
-            const uint32_t s = v.u_off[u], e = v.u_off[u + 1];
-            if (e - s > 512u) {
-                if (lane < K) {
-                    const int32_t want = nbi[lane];
-                    uint32_t lo = s, hi = e;
-                    while (lo < hi) {
-                        const uint32_t mid = (lo + hi) >> 1;
-                        if (v.u_item[mid] < want) lo = mid + 1;
-                        else hi = mid;
+        // the wave's users r = wv, wv + 4, ...: short lists scanned one after the
+        // other, long ones searched together (lane a looks for candidate a in
+        // every long list at once, kSlots independent load chains)
+        uint32_t lo[kSlots], hi[kSlots], en[kSlots];
+        const int32_t want = lane < K ? nbi[lane] : 0;
+#pragma unroll
+        for (int k = 0; k < kSlots; ++k) {
+            const int r = wv + 4 * k;
+            lo[k] = hi[k] = en[k] = 0;
+            if (r < nr) {
+                const int32_t u = v.i_user[base + r];
+                const uint32_t s = v.u_off[u], e = v.u_off[u + 1];
+                if (e - s > kScanMax) {
+                    en[k] = e;
+                    if (lane < K) {
+                        lo[k] = s;
+                        hi[k] = e;
                     }
-                    if (lo < e && v.u_item[lo] == want) {
-                        Xe[r][lane] = v.u_err[lo];
-                        Xr[r][lane] = v.u_rp[lo];
+                } else {
+                    // 256 entries a batch, every load unconditional (clamped into the list)
+                    for (uint32_t t0 = s; t0 < e; t0 += 256) {
+                        int32_t it[4];
+                        float ee[4], rr[4];
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const uint32_t t = t0 + q * 64 + lane;
+                            const uint32_t tc = t < e ? t : s;
+                            it[q] = v.u_item[tc];
+                            ee[q] = v.u_err[tc];
+                            rr[q] = v.u_rp[tc];
+                        }
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const uint8_t a = map[it[q]];
+                            if (t0 + q * 64 + lane < e && a != 0xFF) {
+                                Xe[r][a] = ee[q];
+                                Xr[r][a] = rr[q];
+                            }
+                        }
                     }
                 }
-            } else {
-                for (uint32_t t = s + lane; t < e; t += 64) {
-                    const uint8_t a = map[v.u_item[t]];
-                    if (a != 0xFF) {
-                        Xe[r][a] = v.u_err[t];
-                        Xr[r][a] = v.u_rp[t];
-                    }
+            }
+        }
+        for (;;) {
+            bool live = false;
+#pragma unroll
+            for (int k = 0; k < kSlots; ++k) live |= lo[k] < hi[k];
+            if (!__ballot(live)) break;
+            int32_t x[kSlots];
+#pragma unroll
+            for (int k = 0; k < kSlots; ++k) x[k] = v.u_item[lo[k] < hi[k] ? (lo[k] + hi[k]) >> 1 : 0u];
+#pragma unroll
+            for (int k = 0; k < kSlots; ++k) {
+                if (lo[k] < hi[k]) {
+                    const uint32_t mid = (lo[k] + hi[k]) >> 1;
+                    if (x[k] < want) lo[k] = mid + 1;
+                    else hi[k] = mid;
+                }
+            }
+        }
+        if (lane < K) {
+#pragma unroll
+            for (int k = 0; k < kSlots; ++k) {
+                const uint32_t t = lo[k] < en[k] ? lo[k] : 0u;
+                const bool hit = lo[k] < en[k] && v.u_item[t] == want;
+                if (hit) {
+                    Xe[wv + 4 * k][lane] = v.u_err[t];
+                    Xr[wv + 4 * k][lane] = v.u_rp[t];
                 }
             }
         }
