@@ -16,7 +16,7 @@ constexpr int kMaxK = 64;       // candidates per item (one wave of lanes)
 constexpr int kMapMax = 32768;  // items a k3_cov LDS map covers (one byte each)
 constexpr int kCh = 32;         // users staged per k3_cov round
 constexpr int kSlots = kCh / 4; // users per wave per round
-constexpr uint32_t kScanMax = 512;  // longer user lists are binary-searched
+constexpr uint32_t kScanMax = 256;  // longer user lists are binary-searched
 
 __device__ __forceinline__ uint64_t lanes_below(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
 
