@@ -268,6 +268,13 @@ int knn3_create(const knn3_config* cfg, const tx_ratings* train, knn3_ctx** out)
     HIPC(up(d_iuser, i_user.data(), sizeof(int32_t) * NR));
     HIPC(up(d_ierr, i_err.data(), sizeof(float) * NR));
     HIPC(up(d_irp, i_rp.data(), sizeof(float) * NR));
+    std::vector<int32_t> heavy(I);
+    for (int32_t m = 0; m < I; ++m) heavy[m] = m;
+    std::stable_sort(heavy.begin(), heavy.end(),
+                     [&](int32_t a, int32_t b) { return i_off[a + 1] - i_off[a] > i_off[b + 1] - i_off[b]; });
+    int32_t* d_heavy = nullptr;
+    HIPC(c->alloc(&d_heavy, static_cast<size_t>(I)));
+    HIPC(up(d_heavy, heavy.data(), sizeof(int32_t) * I));
     HIPC(hipMemsetAsync(v.w_cnt, 0, sizeof(int32_t) * I, c->st));
     // co-rating counts of the non-test ratings (the diagonal: each item's users)
     uint32_t* scratch = nullptr;
@@ -283,6 +290,7 @@ int knn3_create(const knn3_config* cfg, const tx_ratings* train, knn3_ctx** out)
     v.i_user = d_iuser;
     v.i_err = d_ierr;
     v.i_rp = d_irp;
+    v.heavy_first = d_heavy;
     v.threshold = cfg->threshold;
     v.shrinkage = cfg->shrinkage;
     v.k_min = cfg->k_min;
@@ -295,7 +303,7 @@ int knn3_create(const knn3_config* cfg, const tx_ratings* train, knn3_ctx** out)
     for (const void* p : {static_cast<const void*>(cnt), static_cast<const void*>(scratch), static_cast<const void*>(v.sys),
                           static_cast<const void*>(v.rhs), static_cast<const void*>(d_ioff),
                           static_cast<const void*>(d_iuser), static_cast<const void*>(d_ierr),
-                          static_cast<const void*>(d_irp)})
+                          static_cast<const void*>(d_irp), static_cast<const void*>(d_heavy)})
         c->release(p);
     v.cnt = nullptr;
     v.sys = nullptr;
@@ -304,6 +312,7 @@ int knn3_create(const knn3_config* cfg, const tx_ratings* train, knn3_ctx** out)
     v.i_user = nullptr;
     v.i_err = nullptr;
     v.i_rp = nullptr;
+    v.heavy_first = nullptr;
     *out = c.release();
     return 0;
 }

@@ -186,7 +186,8 @@ __global__ __launch_bounds__(256) void k3_cov(Knn3View v) {
     __shared__ int32_t nbi[kMaxK];
     __shared__ float Xe[kCh][kMaxK], Xr[kCh][kMaxK];
     __shared__ float me[kCh], mr[kCh];
-    const int m = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    // items with the most users first: their blocks set the kernel's length
+    const int m = v.heavy_first[blockIdx.x], tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int K = v.nb_cnt[m];
     if (K == 0) return;
     for (int j = tid; j < v.ip; j += 256) map[j] = 0xFF;

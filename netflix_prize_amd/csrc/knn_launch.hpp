@@ -47,6 +47,7 @@ struct Knn3View {
     const int32_t* i_user;   // item lists, users ascending
     const float* i_err;
     const float* i_rp;
+    const int32_t* heavy_first;  // [I] items by user count, largest first (k3_cov's block order)
     int32_t* nb_cnt;         // [I] candidates kept
     int32_t* nb_item;        // [I x 64] in candidate order
     double* sys;             // [I x 64 x 64] the mixed, regularised normal equations
