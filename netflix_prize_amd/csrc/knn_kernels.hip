@@ -3,7 +3,9 @@
 // mean of the user's residuals.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
+#include <vector>
 
 #include "knn_launch.hpp"
 
@@ -286,25 +288,29 @@ __global__ __launch_bounds__(1024) void k_scan_offsets(uint32_t* __restrict__ ic
 // one wave per user: each list entry lands in its item's range with its position and list end
 __global__ __launch_bounds__(256) void k_item_scatter(const uint32_t* __restrict__ off, const int32_t* __restrict__ items,
                                                       int32_t n_users, uint32_t* __restrict__ cursor,
-                                                      uint32_t* __restrict__ ient, uint32_t* __restrict__ iend) {
+                                                      uint32_t* __restrict__ ient, uint32_t* __restrict__ iend,
+                                                      float* __restrict__ work) {
     const int lane = threadIdx.x & 63;
     const int64_t nwaves = static_cast<int64_t>(gridDim.x) * (blockDim.x >> 6);
     for (int64_t u = static_cast<int64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6); u < n_users;
          u += nwaves) {
         const uint32_t e = off[u + 1];
         for (uint32_t t = off[u] + lane; t < e; t += 64) {
-            const uint32_t p = atomicAdd(&cursor[items[t]], 1u);
+            const int32_t it = items[t];
+            const uint32_t p = atomicAdd(&cursor[it], 1u);
             ient[p] = t;
             iend[p] = e;
+            atomicAdd(&work[it], static_cast<float>(e - t));  // the row's pair adds (ordering only)
         }
     }
 }
 
 __global__ __launch_bounds__(512) void k_supp_rows(const int32_t* __restrict__ items, const uint32_t* __restrict__ ioff,
                                                    const uint32_t* __restrict__ ient, const uint32_t* __restrict__ iend,
-                                                   float* __restrict__ supp, int ip) {
+                                                   const int32_t* __restrict__ order, float* __restrict__ supp, int ip) {
     __shared__ uint32_t cnt[kRowChunk];
-    const int i = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    // rows with the most pair adds first: their blocks set the kernel's length
+    const int i = order[blockIdx.x], tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const uint32_t q0 = ioff[i], q1 = ioff[i + 1];
     float* row = supp + static_cast<size_t>(i) * ip;
     for (int c0 = 0; c0 < ip; c0 += kRowChunk) {
@@ -364,9 +370,10 @@ __global__ __launch_bounds__(256) void k_knns_sim(float* __restrict__ S, const f
 }
 
 size_t support_scratch_words(int64_t n_rows, int ip) {
-    return 2 * (static_cast<size_t>(ip) + 1) + 2 * static_cast<size_t>(n_rows);
+    return 2 * (static_cast<size_t>(ip) + 1) + 2 * static_cast<size_t>(n_rows) + 2 * static_cast<size_t>(ip);
 }
 
+// Blocks on the stream once: the rows' work is read back to order the rows.
 hipError_t launch_support(const uint32_t* off, const int32_t* items, int32_t n_users, int64_t n_rows, float* S,
                           float* diag, int ip, uint32_t* scratch, hipStream_t st) {
     (void)hipGetLastError();
@@ -374,7 +381,10 @@ hipError_t launch_support(const uint32_t* off, const int32_t* items, int32_t n_u
     uint32_t* cursor = icnt + ip + 1;
     uint32_t* ient = cursor + ip + 1;
     uint32_t* iend = ient + n_rows;
+    float* work = reinterpret_cast<float*>(iend + n_rows);
+    int32_t* order = reinterpret_cast<int32_t*>(work + ip);
     hipError_t e = hipMemsetAsync(icnt, 0, sizeof(uint32_t) * (static_cast<size_t>(ip) + 1), st);
+    if (e == hipSuccess) e = hipMemsetAsync(work, 0, sizeof(float) * static_cast<size_t>(ip), st);
     if (e != hipSuccess) return e;
     int64_t blocks = (n_rows + 255) / 256;
     if (blocks > 65536) blocks = 65536;
@@ -384,11 +394,22 @@ hipError_t launch_support(const uint32_t* off, const int32_t* items, int32_t n_u
     if (ublocks > 65536) ublocks = 65536;
     if (ublocks > 0)
         hipLaunchKernelGGL(k_item_scatter, dim3(static_cast<unsigned>(ublocks)), dim3(256), 0, st, off, items, n_users,
-                           cursor, ient, iend);
-    hipLaunchKernelGGL(k_supp_rows, dim3(ip), dim3(512), 0, st, items, icnt, ient, iend, S, ip);
+                           cursor, ient, iend, work);
+    std::vector<float> w(static_cast<size_t>(ip));
+    e = hipMemcpyAsync(w.data(), work, sizeof(float) * w.size(), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return e;
+    std::vector<int32_t> rows(static_cast<size_t>(ip));
+    for (int r = 0; r < ip; ++r) rows[r] = r;
+    std::stable_sort(rows.begin(), rows.end(), [&](int32_t a, int32_t b) { return w[a] > w[b]; });
+    e = hipMemcpyAsync(order, rows.data(), sizeof(int32_t) * rows.size(), hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_supp_rows, dim3(ip), dim3(512), 0, st, items, icnt, ient, iend, order, S, ip);
     hipLaunchKernelGGL(k_mirror, dim3(ip / 32, ip / 32), dim3(256), 0, st, S, ip);
     hipLaunchKernelGGL(k_diag, dim3((ip + 255) / 256), dim3(256), 0, st, S, ip, diag);
-    return hipGetLastError();
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(st);  // rows is a host temporary
+    return e;
 }
 
 hipError_t launch_knns_sim(const uint32_t* off, const int32_t* items, int32_t n_users, int64_t n_rows, float* S,
