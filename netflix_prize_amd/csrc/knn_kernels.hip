@@ -105,6 +105,8 @@ __global__ __launch_bounds__(256) void k_norms(const float* __restrict__ F, int 
 // sort keeps equal weights in rating order, which is also taken beyond 16
 // (there the reference's quickselect partitioning decides).
 // ---------------------------------------------------------------------------
+constexpr int kWCache = 32; // candidate weights per lane kept in registers
+
 __device__ __forceinline__ float knn_weight(const KnnView& v, const float* srow, int32_t i, float day, uint32_t t) {
     const int32_t j = v.t_item[t];
     if (j == i) return 0.0f;
@@ -144,18 +146,42 @@ __global__ __launch_bounds__(256) void k_knnf_predict(KnnView v, int64_t n, cons
         const float day = static_cast<float>(ds[q]);
         const uint32_t t0 = v.t_off[u], t1 = v.t_off[u + 1];
         const float* srow = v.S + static_cast<size_t>(i) * v.ip;
+        // a list of up to 64 kWCache ratings keeps its weights and residuals in
+        // registers (kWCache per lane) for all passes; a longer one recomputes
+        // them per pass
+        const bool cached = t1 - t0 <= 64u * kWCache;
+        float wc[kWCache], rc[kWCache];
+#pragma unroll
+        for (int k = 0; k < kWCache; ++k) {
+            const uint32_t t = t0 + lane + 64u * k;
+            wc[k] = 0.0f;
+            rc[k] = 0.0f;
+            if (cached && t < t1) {
+                wc[k] = knn_weight(v, srow, i, day, t);
+                rc[k] = v.t_res[t];
+            }
+        }
+        // f(w) for every candidate weight of this lane (rating order not kept)
+        auto visit = [&](auto&& f) {
+            if (cached) {
+#pragma unroll
+                for (int k = 0; k < kWCache; ++k)
+                    if (t0 + lane + 64u * k < t1) f(wc[k]);
+            } else {
+                for (uint32_t t = t0 + lane; t < t1; t += 64) f(knn_weight(v, srow, i, day, t));
+            }
+        };
         // positive candidates
         uint32_t m = 0;
-        for (uint32_t t = t0 + lane; t < t1; t += 64) m += knn_weight(v, srow, i, day, t) > 0.0f ? 1u : 0u;
+        visit([&](float w) { m += w > 0.0f ? 1u : 0u; });
         m = wave_sum_u32(m);
         uint32_t thr = 0, take_eq = 0xffffffffu;  // keep w > thr, and the first take_eq weights == thr
         bool drop_first = false;                   // k + 1 candidates: keep all but the first smallest
         if (m == static_cast<uint32_t>(v.k) + 1u) {
             uint32_t mn = 0xffffffffu;
-            for (uint32_t t = t0 + lane; t < t1; t += 64) {
-                const float w = knn_weight(v, srow, i, day, t);
+            visit([&](float w) {
                 if (w > 0.0f && __float_as_uint(w) < mn) mn = __float_as_uint(w);
-            }
+            });
             thr = wave_min_u32(mn);
             drop_first = true;
         } else if (m > static_cast<uint32_t>(v.k)) {
@@ -165,11 +191,10 @@ __global__ __launch_bounds__(256) void k_knnf_predict(KnnView v, int64_t n, cons
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                for (uint32_t t = t0 + lane; t < t1; t += 64) {
-                    const float w = knn_weight(v, srow, i, day, t);
+                visit([&](float w) {
                     const uint32_t bits = __float_as_uint(w);
                     if (w > 0.0f && (bits & mask) == prefix) atomicAdd(&hist[(bits >> shift) & 255u], 1u);
-                }
+                });
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -209,16 +234,10 @@ __global__ __launch_bounds__(256) void k_knnf_predict(KnnView v, int64_t n, cons
             thr = prefix;
             take_eq = krem;
         }
-        // weighted mean over the kept candidates
+        // weighted mean over the kept candidates, batches of 64 in rating order
         float num = 0.0f, den = 0.0f;
         uint32_t eq_seen = 0;
-        for (uint32_t tb = t0; tb < t1; tb += 64) {
-            const uint32_t t = tb + lane;
-            float w = 0.0f, r = 0.0f;
-            if (t < t1) {
-                w = knn_weight(v, srow, i, day, t);
-                r = v.t_res[t];
-            }
+        auto batch = [&](float w, float r) {
             const uint32_t bits = __float_as_uint(w);
             bool keep = w > 0.0f && (m <= static_cast<uint32_t>(v.k) || bits > thr);
             const bool eq = w > 0.0f && m > static_cast<uint32_t>(v.k) && bits == thr;
@@ -229,6 +248,21 @@ __global__ __launch_bounds__(256) void k_knnf_predict(KnnView v, int64_t n, cons
             if (keep) {
                 num += w * r;
                 den += w;
+            }
+        };
+        if (cached) {
+#pragma unroll
+            for (int k = 0; k < kWCache; ++k)
+                if (t0 + 64u * k < t1) batch(wc[k], rc[k]);  // wc, rc are 0 past the list end
+        } else {
+            for (uint32_t tb = t0; tb < t1; tb += 64) {
+                const uint32_t t = tb + lane;
+                float w = 0.0f, r = 0.0f;
+                if (t < t1) {
+                    w = knn_weight(v, srow, i, day, t);
+                    r = v.t_res[t];
+                }
+                batch(w, r);
             }
         }
         num = wave_sum_f32(num);
