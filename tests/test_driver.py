@@ -213,3 +213,58 @@ def test_knn3_job_end_to_end(gpu, tmp_path):
     q = np.arange(0, pr.n_ratings, 50)
     want = ref.predict(pr.user_idxs[q], pr.item_idxs[q]) + base[q]
     assert np.max(np.abs(kp[q] - want)) < 1e-4
+
+
+def _job_config(job):
+    """The job's TxConfig as the driver prints it (--print-config)."""
+    out = subprocess.run([DRIVER, job, "--print-config"], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stderr
+    d = {}
+    for k, v in re.findall(r"(\w+): ([^,{}\[\]]+)", out.stdout):
+        v = v.strip()
+        if v in ("true", "false"):
+            d[k] = v == "true"
+        elif v != "None":
+            d[k] = int(v) if re.fullmatch(r"-?\d+", v) else float(v)
+    return tx.TxConfig.from_dict(d)
+
+
+@pytest.mark.gpu
+def test_transposed_job_end_to_end(gpu, tmp_path):
+    """tsvdx4-16-t (Dataset::transpose, src/lib.rs:295-402: items act as users,
+    training rows re-sorted by (item, date), held-out rows kept in their
+    order) on the 10k x 2k x 1M split: the transposed log line, the saved
+    item factors having one row per original user, probe predictions in the
+    original row order, and the holdout RMSE of a TxModel trained through the
+    Python API on the same transposition within 2e-3 (both parallel)."""
+    tr, pr = synthetic("small")
+    _write_split(str(tmp_path), tr, pr)
+    env = dict(os.environ, TSVD_N_USERS=str(tr.n_users), TSVD_N_ITEMS=str(tr.n_items),
+               TSVD_DATA_DIR=str(tmp_path / "data"))
+    job = "tsvdx4-16-t"
+    out = subprocess.run([DRIVER, job], cwd=tmp_path, env=env, capture_output=True, text=True, timeout=600)
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
+    pd = tmp_path / "preds_new"
+    log = (pd / f"{job}.out").read_text()
+    assert "trainx => probex (transposed)" in log
+    assert np.load(pd / f"{job}.ifeat.trainx.npy").shape == (tr.n_users, 16)
+    probe = np.load(pd / f"{job}.probex.npy")
+    assert probe.shape == (pr.n_ratings,) and np.all(np.isfinite(probe))
+    logged = float(re.search(r"probex RMSE ([0-9.]+)", log).group(1))
+    rmse = float(np.sqrt(np.mean((probe.astype(np.float64) - pr.raw_ratings) ** 2)))
+    assert abs(logged - rmse) < 1e-5, (logged, rmse)
+    # the same model through the Python API on a numpy transposition
+    order = np.lexsort((tr.dates, tr.item_idxs))
+    tt = tx.Dataset.from_arrays(tr.item_idxs[order], tr.user_idxs[order], tr.raw_ratings[order], tr.dates[order],
+                                tr.n_items, tr.n_users)
+    pt = tx.Dataset.from_arrays(pr.item_idxs, pr.user_idxs, pr.raw_ratings, pr.dates, tr.n_items, tr.n_users)
+    cfg = _job_config(job)
+    m = tx.TxModel(tt, pt, cfg)
+    for e in range(1, cfg.n_epochs + 1):
+        m.fit_epoch(epoch=e)
+    mine = m.predict(pt.user_idxs, pt.item_idxs, pt.dates)
+    m.close()
+    r_mine = float(np.sqrt(np.mean((mine.astype(np.float64) - pr.raw_ratings) ** 2)))
+    print(f"transposed job RMSE {rmse:.5f}, Python API {r_mine:.5f}")
+    assert abs(r_mine - rmse) < 2e-3
+    assert rmse < float(np.std(pr.raw_ratings.astype(np.float64)))
