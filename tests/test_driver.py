@@ -56,6 +56,39 @@ def test_unknown_job_is_an_error():
     assert "tsvdx4-no-such-job" in (out.stdout + out.stderr)
 
 
+def test_epoch_blend_job(tmp_path):
+    """tsvdx4-d0tb__epochs (run_eblend with the reference's weights and offset,
+    src/bin/tsvdx4-new.rs:85-88; epoch_blend_apply, src/lib.rs:1181-1235): the
+    FP64 weighted sum of the base job's per-epoch predictions plus the offset,
+    stored as FP32, for the probe and qual sets, and its log."""
+    w = [0.127633, 0.128381, 0.128978, 0.129290, 0.129347]
+    off = 0.978328
+    pd = tmp_path / "preds_new"
+    pd.mkdir()
+    rng = np.random.default_rng(3)
+    eps = {}
+    for ds in ("probex", "qual"):
+        eps[ds] = [rng.uniform(1, 5, 1000).astype(np.float32) for _ in w]
+        for e, a in enumerate(eps[ds], 1):
+            np.save(pd / f"tsvdx4-d0tb_ep{e:02d}.{ds}.npy", a)
+    out = subprocess.run([DRIVER, "tsvdx4-d0tb__epochs"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
+    for ds in ("probex", "qual"):
+        want = np.full(1000, off, np.float64)
+        for wi, a in zip(w, eps[ds]):
+            want = want + wi * a.astype(np.float64)
+        got = np.load(pd / f"tsvdx4-d0tb__epochs.{ds}.npy")
+        assert got.dtype == np.float32
+        np.testing.assert_array_equal(got, want.astype(np.float32))
+    log = (pd / "tsvdx4-d0tb__epochs.out").read_text().splitlines()
+    assert log[:3] == ["[tsvdx4-d0tb__epochs]", "eblend legacy mode: 5 epoch weights, constant offset=+0.978328",
+                       "  epoch  1 weight = +0.127633"]
+    # a missing epoch file is an error, as the reference's expect(path)
+    (pd / "tsvdx4-d0tb_ep03.qual.npy").unlink()
+    out = subprocess.run([DRIVER, "tsvdx4-d0tb__epochs"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert out.returncode != 0
+
+
 def _write_split(root, tr, pr):
     """The reference's SPLIT_NEW sets (trainx, probex, fulltrain, qual)."""
     full_u = np.concatenate([tr.user_idxs, pr.user_idxs])
