@@ -64,6 +64,30 @@ int knns_predict(knnf_ctx* ctx, int64_t n, const int32_t* users, const int32_t* 
                  float* out);
 int knns_destroy(knnf_ctx* ctx);
 
+/* KnnConfig (src/knn.rs:19-57) in the form the `*__knn-d` job uses
+ * (src/bin/tsvdx4-new.rs:590-610): SimType::Support over a loaded support
+ * stat, use_stat1 on, no threshold, no regression. */
+typedef struct knnd_config {
+    int32_t k;
+    float shrinkage;
+    float scaling;
+    float tau;
+} knnd_config;
+
+/* KnnModel::new (src/knn.rs:191-256) with stat0 and stat1 given as
+ * n_items x n_items row-major FP32 tables (the `{sim_dir}/{name}.{set}.npy`
+ * files it loads): similarity phi n / (n + shrinkage), phi = n 480,189 /
+ * (stat0_ii stat0_jj) (support_sim, :105-113), and the averaged value
+ * stat1_ij / (stat0_ij + shrinkage) in place of the residual (:321-334). */
+int knnd_create(const knnd_config* cfg, const float* stat0, const float* stat1, int32_t n_items,
+                const tx_ratings* train, knnf_ctx** out);
+
+/* KnnModel::predict (src/knn.rs:263-336) on the use_stat1 path; the same
+ * top-k rule as knnf_predict. */
+int knnd_predict(knnf_ctx* ctx, int64_t n, const int32_t* users, const int32_t* items, const int16_t* days,
+                 float* out);
+int knnd_destroy(knnf_ctx* ctx);
+
 /* Knn3Config (src/knn3.rs:7-31).  Defaults (Knn3Config::default): threshold
  * 0.25, k_min 10, k_max 60, shrinkage 20000, reg 0.01, x 0.8, bl_reg_m 25,
  * bl_reg_u 10.  k_max above 64 is TX_EUNSUPPORTED. */

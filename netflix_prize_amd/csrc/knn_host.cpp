@@ -213,6 +213,45 @@ int knns_create(const knns_config* cfg, const tx_ratings* train, const tx_rating
     return 0;
 }
 
+int knnd_create(const knnd_config* cfg, const float* stat0, const float* stat1, int32_t n_items,
+                const tx_ratings* train, knnf_ctx** out) {
+    tx::clear_error();
+    if (!cfg || !stat0 || !stat1 || !train || !out) return fail(TX_EINVAL, "knnd_create: NULL argument");
+    if (cfg->k < 1) return fail(TX_EINVAL, "knnd_create: k must be >= 1");
+    if (n_items < 1 || train->n_items > n_items)
+        return fail(TX_EINVAL, "knnd_create: stats have %d items for %d", n_items, train->n_items);
+    if (!train->residuals) return fail(TX_EINVAL, "knnd_create: training residuals missing");
+    auto c = std::make_unique<knnf_ctx>();
+    if (int rc = setup_train(c.get(), train, n_items, "knnd_create")) return rc;
+    const size_t ip = static_cast<size_t>(c->ip);
+    float *dD = nullptr, *ddiag = nullptr;
+    HIPC(c->alloc(&dD, ip * ip));
+    HIPC(c->alloc(&ddiag, ip));
+    float* S = const_cast<float*>(c->v.S);
+    HIPC(hipMemsetAsync(S, 0, sizeof(float) * ip * ip, c->st));
+    HIPC(hipMemsetAsync(dD, 0, sizeof(float) * ip * ip, c->st));
+    // the n_items x n_items tables into the padded ip x ip layout
+    HIPC(hipMemcpy2DAsync(S, sizeof(float) * ip, stat0, sizeof(float) * n_items, sizeof(float) * n_items, n_items,
+                          hipMemcpyHostToDevice, c->st));
+    HIPC(hipMemcpy2DAsync(dD, sizeof(float) * ip, stat1, sizeof(float) * n_items, sizeof(float) * n_items, n_items,
+                          hipMemcpyHostToDevice, c->st));
+    HIPC(knn::launch_stat_sim(S, dD, ddiag, c->ip, cfg->shrinkage, c->st));
+    HIPC(hipStreamSynchronize(c->st));
+    c->v.k = cfg->k;
+    c->v.scaling = cfg->scaling;
+    c->v.tau = cfg->tau;
+    c->v.D = dD;
+    *out = c.release();
+    return 0;
+}
+
+int knnd_predict(knnf_ctx* ctx, int64_t n, const int32_t* users, const int32_t* items, const int16_t* days,
+                 float* out) {
+    return knnf_predict(ctx, n, users, items, days, out);
+}
+
+int knnd_destroy(knnf_ctx* ctx) { return knnf_destroy(ctx); }
+
 int knns_predict(knnf_ctx* ctx, int64_t n, const int32_t* users, const int32_t* items, const int16_t* days,
                  float* out) {
     return knnf_predict(ctx, n, users, items, days, out);

@@ -116,6 +116,11 @@ __device__ __forceinline__ float knn_weight(const KnnView& v, const float* srow,
     return sim / (1.0f + v.tau * dt);
 }
 
+// the value averaged for candidate t: its residual, or D[i][j] (KnnModel use_stat1)
+__device__ __forceinline__ float knn_target(const KnnView& v, int32_t i, uint32_t t) {
+    return v.D ? v.D[static_cast<size_t>(i) * v.ip + v.t_item[t]] : v.t_res[t];
+}
+
 __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t x) {
     for (int o = 32; o > 0; o >>= 1) x += static_cast<uint32_t>(__shfl_xor(static_cast<int>(x), o, 64));
     return x;
@@ -158,7 +163,7 @@ __global__ __launch_bounds__(256) void k_knnf_predict(KnnView v, int64_t n, cons
             rc[k] = 0.0f;
             if (cached && t < t1) {
                 wc[k] = knn_weight(v, srow, i, day, t);
-                rc[k] = v.t_res[t];
+                rc[k] = knn_target(v, i, t);
             }
         }
         // f(w) for every candidate weight of this lane (rating order not kept)
@@ -260,7 +265,7 @@ __global__ __launch_bounds__(256) void k_knnf_predict(KnnView v, int64_t n, cons
                 float w = 0.0f, r = 0.0f;
                 if (t < t1) {
                     w = knn_weight(v, srow, i, day, t);
-                    r = v.t_res[t];
+                    r = knn_target(v, i, t);
                 }
                 batch(w, r);
             }
@@ -444,6 +449,25 @@ hipError_t launch_support(const uint32_t* off, const int32_t* items, int32_t n_u
     e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(st);  // rows is a host temporary
     return e;
+}
+
+// KnnModel (src/knn.rs:321-334): diff = stat1 / (stat0 + shrinkage), in place over stat1
+__global__ __launch_bounds__(256) void k_stat_diff(const float* __restrict__ S0, float* __restrict__ D, int ip,
+                                                   float shrinkage) {
+#pragma clang fp contract(off)
+    const int64_t total = static_cast<int64_t>(ip) * ip;
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
+         e += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        D[e] = D[e] / (S0[e] + shrinkage);
+}
+
+hipError_t launch_stat_sim(float* S, float* D, float* diag, int ip, float shrinkage, hipStream_t st) {
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(k_stat_diff, dim3(65536), dim3(256), 0, st, S, D, ip, shrinkage);
+    hipLaunchKernelGGL(k_diag, dim3((ip + 255) / 256), dim3(256), 0, st, S, ip, diag);
+    // KnnModel::support_sim (src/knn.rs:105-113) is KnnsModel::sim's formula over stat0
+    hipLaunchKernelGGL(k_knns_sim, dim3(65536), dim3(256), 0, st, S, diag, ip, shrinkage);
+    return hipGetLastError();
 }
 
 hipError_t launch_knns_sim(const uint32_t* off, const int32_t* items, int32_t n_users, int64_t n_rows, float* S,

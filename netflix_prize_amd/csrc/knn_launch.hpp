@@ -16,6 +16,7 @@ struct KnnView {
     const float* t_day;      // [N] day as f32
     int32_t k;               // neighbours kept
     float scaling, tau;      // similarity exponent, temporal decay
+    const float* D = nullptr;  // [ip x ip] per-pair target replacing the residual (KnnModel use_stat1), or none
 };
 
 hipError_t launch_norms(const float* F, int fp, int rows, float* norm, hipStream_t st);
@@ -29,6 +30,9 @@ hipError_t launch_knns_sim(const uint32_t* off, const int32_t* items, int32_t n_
 // the support matrix alone (the first two steps of launch_knns_sim) and its diagonal
 hipError_t launch_support(const uint32_t* off, const int32_t* items, int32_t n_users, int64_t n_rows, float* S,
                           float* diag, int ip, uint32_t* scratch, hipStream_t st);
+// KnnModel with SimType::Support and use_stat1 over loaded stats: S (stat0 in)
+// becomes the support similarity, D (stat1 in) becomes stat1 / (stat0 + shrinkage)
+hipError_t launch_stat_sim(float* S, float* D, float* diag, int ip, float shrinkage, hipStream_t st);
 hipError_t launch_knnf_predict(const KnnView& v, int64_t n, const int32_t* us, const int32_t* is, const int16_t* ds,
                                float* out, hipStream_t st);
 

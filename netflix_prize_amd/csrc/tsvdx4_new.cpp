@@ -548,16 +548,37 @@ void fit2(const tx_config& cfg, const std::string& target, const std::string& jo
 //   with SuppSource::Compute and shrinkage 100, target `{w}*{base}`.
 //   `<base>__knn3` (run_knn3, src/bin/tsvdx4-new.rs:43-46, 557-561): Knn3Model
 //   with Knn3Config::default, target `1.0*{base}`.
-// One fit phase: knn3 when k3 is given, knns when ks is given, else knnf.
+//   `tsvdx4-4800lm__knn-d` (run_knn_d, src/bin/tsvdx4-new.rs:60, 590-610):
+//   KnnModel, support similarity over sim/rtg_supp.{set}.npy, averaged value
+//   from sim/{base}_diff1.{set}.npy, target `1.0*{base}`.
+// One fit phase: knn-d when kd is given, knn3 when k3 is given, knns when ks
+// is given, else knnf.
 void knn_fit(const std::string& base, const std::string& target, const std::string& tr_set,
              const std::string& pr_set, const std::string& job, const std::string& preds_dir, bool phase1,
-             const knns_config* ks, const knn3_config* k3 = nullptr) {
+             const knns_config* ks, const knn3_config* k3 = nullptr, const knnd_config* kd = nullptr) {
     tee("%s => %s\n", tr_set.c_str(), pr_set.c_str());
     Dataset tr = load_dataset(tr_set, target, preds_dir);
     Dataset pr = load_dataset(pr_set, target, preds_dir);
     knnf_ctx* ctx = nullptr;
     const tx_ratings t = tr.view(false);
-    if (k3) {
+    if (kd) {
+        // KnnModel::new (src/knn.rs:191-199): stat0 and stat1 from {sim_dir}/{name}.{train set}.npy
+        auto stat = [&](const std::string& name) {
+            const std::string path = std::string("sim/") + name + "." + tr_set + ".npy";
+            npy::Array a;
+            try {
+                a = npy::read(path);
+            } catch (const std::exception& e) {
+                panic(e.what());
+            }
+            if (a.shape.size() != 2 || a.shape[0] != a.shape[1]) panic(path + ": expected a square item x item table");
+            return std::make_pair(npy::as<float>(a, path), static_cast<int32_t>(a.shape[0]));
+        };
+        const auto s0 = stat("rtg_supp");
+        const auto s1 = stat(base + "_diff1");
+        if (s0.second != s1.second) panic("stat tables of different sizes");
+        check(knnd_create(kd, s0.first.data(), s1.first.data(), s0.second, &t, &ctx), "KnnModel::new");
+    } else if (k3) {
         // Knn3Model::new (src/knn3.rs:350-362) and its progress lines (:292-340)
         tee("  KNN3: computing baseline...\n");
         tee("  KNN3: building inverted index...\n");
@@ -583,8 +604,7 @@ void knn_fit(const std::string& base, const std::string& target, const std::stri
         if (phase1) tee("%s RMSE %.5f\n", pr_set.c_str(), rmse_from_preds(p, pr));
         write_preds(preds_dir + "/" + job + "." + pr_set + ".npy", p);
         return;
-    }
-    if (ks) {
+    } else if (ks) {
         const tx_ratings h = pr.view(true);
         check(knns_create(ks, &t, &h, &ctx), "KnnsModel::new");
     } else {
@@ -612,14 +632,20 @@ void knn_fit(const std::string& base, const std::string& target, const std::stri
 }
 
 void fit2_knn(const std::string& job, const std::string& base, const std::string& target, const Split& sp,
-              const knns_config* ks, const knn3_config* k3 = nullptr) {
+              const knns_config* ks, const knn3_config* k3 = nullptr, const knnd_config* kd = nullptr) {
     const std::string preds_dir = sp.preds_dir;
     mkdir(preds_dir.c_str(), 0755);
     g_log = std::fopen((preds_dir + "/" + job + ".out").c_str(), "w");
     if (!g_log) panic(preds_dir + "/" + job + ".out: cannot create");
     tee("[%s]\n", job.c_str());
     tee("target = \"%s\"\n", target.c_str());
-    if (k3)
+    if (kd)
+        tee("KnnConfig { stat0: Some(\"rtg_supp\"), stat1: Some(\"%s_diff1\"), stat2: None, factors: None, "
+            "sim_type: Support, k: %d, sim_threshold: None, shrinkage: %s, scaling: %s, tau: %s, use_stat1: true, "
+            "regression: false, regression_lambda: 0.1, sim_dir: \"sim\", preds_dir: \"%s\" }\n",
+            base.c_str(), kd->k, rust_f32(kd->shrinkage).c_str(), rust_f32(kd->scaling).c_str(),
+            rust_f32(kd->tau).c_str(), preds_dir.c_str());
+    else if (k3)
         tee("Knn3Config { threshold: %s, k_min: %d, k_max: %d, shrinkage: %s, reg: %s, x: %s, bl_reg_m: %s, "
             "bl_reg_u: %s }\n",
             rust_f32(k3->threshold).c_str(), k3->k_min, k3->k_max, rust_f32(k3->shrinkage).c_str(),
@@ -631,8 +657,8 @@ void fit2_knn(const std::string& job, const std::string& base, const std::string
     else
         tee("KnnfConfig { factors: \"%s/%s.ifeat\", k: 25, scaling: 1.0, tau: 0.025 }\n", preds_dir.c_str(),
             base.c_str());
-    knn_fit(base, target, sp.tr, sp.pr, job, preds_dir, true, ks, k3);
-    knn_fit(base, target, sp.fulltrain_tr, sp.fulltrain_pr, job, preds_dir, false, ks, k3);
+    knn_fit(base, target, sp.tr, sp.pr, job, preds_dir, true, ks, k3, kd);
+    knn_fit(base, target, sp.fulltrain_tr, sp.fulltrain_pr, job, preds_dir, false, ks, k3, kd);
     tee("\n");
     std::fclose(g_log);
     g_log = nullptr;
@@ -964,7 +990,11 @@ int main(int argc, char** argv) {
         fit2_knn(job, base, "1.0*" + base, SPLIT_NEW, nullptr, &k3);
         return 0;
     }
-    if (job.find("__knn") != std::string::npos)
-        panic("job " + job + ": this kNN chain (knn-d) is outside this build");
+    if (job == "tsvdx4-4800lm__knn-d") {  // run_knn_d (src/bin/tsvdx4-new.rs:590-610)
+        if (print_only) return 0;
+        const knnd_config kd{15, 100.0f, 3.5f, 0.0f};
+        fit2_knn(job, "tsvdx4-4800lm", "1.0*tsvdx4-4800lm", SPLIT_NEW, nullptr, nullptr, &kd);
+        return 0;
+    }
     panic("invalid job name: " + job);
 }

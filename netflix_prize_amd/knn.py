@@ -130,6 +130,41 @@ class KnnsModel(KnnfModel):
 
 
 @dataclass
+class KnndConfig:
+    """KnnConfig (src/knn.rs:19-57) in the `*__knn-d` form: SimType::Support
+    over a loaded stat0, use_stat1; the job's values (src/bin/tsvdx4-new.rs:590-610)."""
+    k: int = 15
+    shrinkage: float = 100.0
+    scaling: float = 3.5
+    tau: float = 0.0
+
+
+class KnndModel(KnnfModel):
+    """KnnModel (src/knn.rs) on the GPU over stat tables given as n_items x
+    n_items arrays: support similarity from stat0, and stat1 / (stat0 +
+    shrinkage) averaged in place of the residual; the KnnfModel top-k rule."""
+
+    def __init__(self, tr, stat0: np.ndarray, stat1: np.ndarray, cfg: KnndConfig = KnndConfig()):
+        L = _lib()
+        if not getattr(L, "_knnd_bound", False):
+            vp = C.c_void_p
+            L.knnd_create.argtypes = [vp, vp, vp, C.c_int32, vp, C.POINTER(vp)]
+            L.knnd_create.restype = C.c_int
+            L._knnd_bound = True
+        S0 = np.ascontiguousarray(stat0, np.float32)
+        S1 = np.ascontiguousarray(stat1, np.float32)
+        if S0.ndim != 2 or S0.shape[0] != S0.shape[1] or S1.shape != S0.shape:
+            raise ValueError("stat0 and stat1 must be n_items x n_items")
+        self.cfg = cfg
+        self._tr = tx._RatingsRef(tr)
+        c = _CKnns(int(cfg.k), float(cfg.shrinkage), float(cfg.scaling), float(cfg.tau))
+        h = C.c_void_p()
+        tx._check(L.knnd_create(C.byref(c), S0.ctypes.data, S1.ctypes.data, S0.shape[0], C.byref(self._tr.s),
+                                C.byref(h)), "knnd_create")
+        self._h = h
+
+
+@dataclass
 class Knn3Config:
     """Knn3Config (src/knn3.rs:7-31); defaults of Knn3Config::default."""
     threshold: float = 0.25

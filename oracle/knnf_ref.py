@@ -1,4 +1,5 @@
-"""CPU restatement of KnnfModel (src/knnf.rs) and KnnsModel (src/knns.rs) --
+"""CPU restatement of KnnfModel (src/knnf.rs), KnnsModel (src/knns.rs) and the
+KnnModel form of the `*__knn-d` job (src/knn.rs, Support + use_stat1) --
 test infrastructure only, the checker of the GPU kNN kernels; never imported
 by the product.
 
@@ -117,3 +118,40 @@ class KnnsRef(KnnfRef):
                  tau: float = 0.025):
         super().__init__(tr, np.ones((n_items, 1), np.float32), k=k, scaling=scaling, tau=tau)
         self.S = knns_sim_matrix(tr, pr, n_items, shrinkage)
+
+
+def stat_sim_matrix(stat0: np.ndarray, shrinkage: float) -> np.ndarray:
+    """KnnModel::support_sim (src/knn.rs:105-113) over a loaded support stat:
+    phi = n * N_USERS / (s_ii s_jj) (0 if that product is 0), phi n / (n + shrinkage)."""
+    S0 = np.asarray(stat0, np.float64)
+    d = np.diag(S0).copy()
+    den = np.outer(d, d)
+    phi = np.zeros_like(S0)
+    ok = den > 0
+    phi[ok] = S0[ok] * N_USERS / den[ok]
+    return phi * S0 / (S0 + shrinkage)
+
+
+class KnnStatRef(KnnfRef):
+    """KnnModel (src/knn.rs) with SimType::Support and use_stat1, the form of the
+    `*__knn-d` job: weights as KnnfModel's over stat_sim_matrix, and the
+    averaged value stat1[i, j] / (stat0[i, j] + shrinkage) in place of the
+    residual (:321-334)."""
+
+    def __init__(self, tr, stat0, stat1, k: int = 15, shrinkage: float = 100.0, scaling: float = 3.5,
+                 tau: float = 0.0):
+        n = np.asarray(stat0).shape[0]
+        super().__init__(tr, np.ones((n, 1), np.float32), k=k, scaling=scaling, tau=tau)
+        self.S = stat_sim_matrix(stat0, shrinkage)
+        self.Dm = (np.asarray(stat1, np.float32) / (np.asarray(stat0, np.float32) + np.float32(shrinkage))).astype(
+            np.float64)
+
+    def _weights(self, u: int, i: int, day: float):
+        a, b = self.off[u], self.off[u + 1]
+        j, d = self.items[a:b], self.days[a:b]
+        s = self.S[i, j]
+        if self.scaling != 1.0:
+            s = s ** self.scaling
+        w = s / (1.0 + self.tau * np.abs(day - d))
+        keep = (j != i) & (w != 0.0)
+        return w[keep], self.Dm[i, j][keep]

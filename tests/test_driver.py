@@ -301,3 +301,60 @@ def test_transposed_job_end_to_end(gpu, tmp_path):
     print(f"transposed job RMSE {rmse:.5f}, Python API {r_mine:.5f}")
     assert abs(r_mine - rmse) < 2e-3
     assert rmse < float(np.std(pr.raw_ratings.astype(np.float64)))
+
+
+@pytest.mark.gpu
+def test_knn_d_job_end_to_end(gpu, tmp_path):
+    """tsvdx4-4800lm__knn-d (run_knn_d, src/bin/tsvdx4-new.rs:590-610: KnnModel
+    over sim/rtg_supp.{set}.npy and sim/tsvdx4-4800lm_diff1.{set}.npy, target
+    1.0*tsvdx4-4800lm) on the 10k x 2k x 1M split, with seeded base predictions
+    and stat tables written beforehand: output files, the logged config and
+    RMSE, and predictions equal to the oracle's."""
+    from oracle.knnf_ref import KnnStatRef
+    tr, pr = synthetic("small")
+    _write_split(str(tmp_path), tr, pr)
+    env = dict(os.environ, TSVD_N_USERS=str(tr.n_users), TSVD_N_ITEMS=str(tr.n_items),
+               TSVD_DATA_DIR=str(tmp_path / "data"))
+    pd, sd = tmp_path / "preds_new", tmp_path / "sim"
+    pd.mkdir()
+    sd.mkdir()
+    rng = np.random.default_rng(11)
+    base = {}
+    for name, n in (("trainx", tr.n_ratings), ("probex", pr.n_ratings), ("fulltrain", tr.n_ratings + pr.n_ratings),
+                    ("qual", pr.n_ratings)):
+        base[name] = rng.uniform(2.5, 4.5, n).astype(np.float32)
+        np.save(pd / f"tsvdx4-4800lm.{name}.npy", base[name])
+    full_u = np.concatenate([tr.user_idxs, pr.user_idxs])
+    full_i = np.concatenate([tr.item_idxs, pr.item_idxs])
+    stats = {}
+    for name, (u, i) in (("trainx", (tr.user_idxs, tr.item_idxs)), ("fulltrain", (full_u, full_i))):
+        B = np.zeros((tr.n_users, tr.n_items), np.float32)
+        B[u, i] = 1.0
+        stats[name] = ((B.T @ B).astype(np.float32), rng.normal(0, 50, (tr.n_items, tr.n_items)).astype(np.float32))
+        np.save(sd / f"rtg_supp.{name}.npy", stats[name][0])
+        np.save(sd / f"tsvdx4-4800lm_diff1.{name}.npy", stats[name][1])
+    job = "tsvdx4-4800lm__knn-d"
+    out = subprocess.run([DRIVER, job], cwd=tmp_path, env=env, capture_output=True, text=True, timeout=600)
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
+    kp = np.load(pd / f"{job}.probex.npy")
+    assert kp.shape == (pr.n_ratings,) and np.all(np.isfinite(kp))
+    assert np.load(pd / f"{job}.qual.npy").shape == (pr.n_ratings,)
+    klog = (pd / f"{job}.out").read_text()
+    assert ('KnnConfig { stat0: Some("rtg_supp"), stat1: Some("tsvdx4-4800lm_diff1"), stat2: None, factors: None, '
+            'sim_type: Support, k: 15, sim_threshold: None, shrinkage: 100.0, scaling: 3.5, tau: 0.0, use_stat1: true, '
+            'regression: false, regression_lambda: 0.1, sim_dir: "sim", preds_dir: "preds_new" }') in klog
+    logged = float(re.search(r"probex RMSE ([0-9.]+)", klog).group(1))
+    rmse = float(np.sqrt(np.mean((kp.astype(np.float64) - pr.raw_ratings) ** 2)))
+    assert abs(logged - rmse) < 1e-5, (logged, rmse)
+    res_tr = tr.raw_ratings.astype(np.float32) - base["trainx"]
+    tr_k = tx.Dataset.from_arrays(tr.user_idxs, tr.item_idxs, tr.raw_ratings, tr.dates, tr.n_users, tr.n_items,
+                                  residuals=res_tr)
+    ref = KnnStatRef(tr_k, *stats["trainx"], k=15, shrinkage=100.0, scaling=3.5, tau=0.0)
+    q = np.arange(0, pr.n_ratings, 50)
+    want = ref.predict(pr.user_idxs[q], pr.item_idxs[q], pr.dates[q]) + base["probex"][q]
+    clear = ref.margins(pr.user_idxs[q], pr.item_idxs[q], pr.dates[q]) > 1e-4
+    assert np.max(np.abs(kp[q][clear] - want[clear])) < 1e-4
+    # a missing stat file is an error, as the reference's unwrap
+    (sd / "rtg_supp.fulltrain.npy").unlink()
+    out = subprocess.run([DRIVER, job], cwd=tmp_path, env=env, capture_output=True, text=True, timeout=600)
+    assert out.returncode != 0

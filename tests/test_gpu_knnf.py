@@ -96,3 +96,32 @@ def test_knns_matches_oracle(gpu, k, scaling, tau):
     err = np.abs(got.astype(np.float64) - want)
     print(f"knns k={k}: max |diff| {err[clear].max():.2e} over {clear.sum()} clear queries")
     assert err[clear].max() <= 1e-5
+
+
+@pytest.mark.parametrize("scaling,tau", [(3.5, 0.0), (1.0, 0.025)])
+def test_knn_stat_matches_oracle(gpu, scaling, tau):
+    """KnnModel in the `*__knn-d` form over given stat tables (10k x 2k x 1M
+    set; stat0 = the training support, stat1 = seeded values): similarities
+    within 1e-6 relative and predictions within 1e-4 (values up to ~1) of the
+    FP64 oracle on clear queries."""
+    from oracle.knnf_ref import KnnStatRef
+    tr, pr = synthetic("small")
+    B = np.zeros((tr.n_users, tr.n_items), np.float32)
+    B[tr.user_idxs, tr.item_idxs] = 1.0
+    s0 = (B.T @ B).astype(np.float32)
+    s1 = np.random.default_rng(9).normal(0, 50, s0.shape).astype(np.float32)
+    ref = KnnStatRef(tr, s0, s1, k=15, shrinkage=100.0, scaling=scaling, tau=tau)
+    rng = np.random.default_rng(6)
+    q = rng.choice(pr.n_ratings, 20_000, replace=False)
+    u, i, d = pr.user_idxs[q], pr.item_idxs[q], pr.dates[q]
+    with knn.KnndModel(tr, s0, s1, knn.KnndConfig(k=15, shrinkage=100.0, scaling=scaling, tau=tau)) as m:
+        a, b = rng.integers(0, tr.n_items, 5000), rng.integers(0, tr.n_items, 5000)
+        s = m.similarity(a, b)
+        got = m.predict(u, i, d)
+    np.testing.assert_allclose(s, ref.S[a, b], rtol=1e-6, atol=1e-9)
+    want = ref.predict(u, i, d)
+    clear = ref.margins(u, i, d) > 1e-4
+    assert clear.mean() > 0.99, clear.mean()
+    err = np.abs(got.astype(np.float64) - want)
+    print(f"knn-d scaling={scaling}: max |diff| {err[clear].max():.2e} over {clear.sum()} clear queries")
+    assert err[clear].max() <= 1e-4

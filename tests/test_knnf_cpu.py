@@ -88,3 +88,23 @@ def test_knns_support_counts_training_and_heldout_items():
     assert S[0, 2] == pytest.approx(2 * N_USERS / (2 * 2) * 2 / (2 + 1.0))
     assert S[0, 1] == pytest.approx(1 * N_USERS / (2 * 1) * 1 / (1 + 1.0))
     assert S[1, 2] == pytest.approx(1 * N_USERS / (1 * 2) * 1 / (1 + 1.0))
+
+
+def test_knn_stat_oracle_closed_form():
+    """KnnModel's `*__knn-d` form (src/knn.rs:105-113, 263-336): one user
+    with items 1 and 2 rated; the prediction for item 0 is the weighted mean
+    of stat1 / (stat0 + shrinkage) with weights (support lift)^scaling."""
+    from oracle.knnf_ref import KnnStatRef
+    s0 = np.array([[3, 2, 1], [2, 4, 1], [1, 1, 5]], np.float32)
+    s1 = np.array([[0, 6, -4], [-6, 0, 2], [4, -2, 0]], np.float32)
+    tr = _one_user([(1, 4, 100), (2, 1, 140)], 3)
+    shr, sc = 10.0, 2.0
+    ref = KnnStatRef(tr, s0, s1, k=15, shrinkage=shr, scaling=sc, tau=0.0)
+    sim01 = (2 * 480189 / (3 * 4)) * 2 / (2 + shr)
+    sim02 = (1 * 480189 / (3 * 5)) * 1 / (1 + shr)
+    w1, w2 = sim01 ** sc, sim02 ** sc
+    want = (w1 * 6 / (2 + shr) + w2 * (-4) / (1 + shr)) / (w1 + w2)
+    assert ref.predict_one(0, 0, 120.0) == pytest.approx(want, rel=1e-6)
+    # k = 1 keeps the larger weight (item 1: sim01 > sim02)
+    assert KnnStatRef(tr, s0, s1, k=1, shrinkage=shr, scaling=sc).predict_one(0, 0, 0.0) == pytest.approx(
+        6 / (2 + shr), rel=1e-6)
