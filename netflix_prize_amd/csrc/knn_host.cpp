@@ -1,5 +1,5 @@
-// Host side of the GPU KnnfModel and KnnsModel: the C ABI declared in
-// include/tsvd_knn.h.
+// Host side of the GPU KnnfModel, KnnsModel and KnnModel (`__knn-d` form):
+// the C ABI declared in include/tsvd_knn.h (Knn3Model: knn3_host.cpp).
 //
 // knnf_create restates KnnfModel::new_with_factors (src/knnf.rs:60-90): the
 // factor table goes to the device zero-padded to a multiple of 64 rows and 32
@@ -7,7 +7,9 @@
 // there (knn_kernels.hip).  knns_create restates KnnsModel::new with
 // SuppSource::Compute (src/knns.rs:78-100): the co-occurrence support of the
 // training and held-out items per user, turned into the shrunk-lift
-// similarity on the device.  Both keep each user's training ratings (item,
+// similarity on the device.  knnd_create restates KnnModel::new for the
+// Support similarity with use_stat1 over two given stat tables.  All keep
+// each user's training ratings (item,
 // residual, day) as CSR arrays; predictions (KnnfModel::predict and
 // KnnsModel::predict are the same rule over their similarity) run batched.
 #include <hip/hip_runtime.h>

@@ -1,6 +1,8 @@
-// KnnfModel on the GPU (src/knnf.rs): the item-item cosine similarity matrix
-// of a factor table by FP32 MFMA, and the per-query temporal top-k weighted
-// mean of the user's residuals.
+// KnnfModel, KnnsModel and KnnModel (`__knn-d` form) on the GPU (src/knnf.rs,
+// src/knns.rs, src/knn.rs): the item-item cosine similarity matrix of a factor
+// table by FP32 MFMA, the co-occurrence support and its shrunk lift, and the
+// per-query temporal top-k weighted mean of the user's residuals (or of a
+// per-pair value).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
