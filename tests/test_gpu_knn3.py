@@ -90,3 +90,23 @@ def test_knn3_rejects_unsupported_input(gpu):
     rows = [(0, 1, 4, 1), (0, 1, 5, 2), (1, 0, 3, 3)]
     with pytest.raises(RuntimeError, match="twice"):
         Knn3Model(dataset_from_rows(rows, 2, 2), Knn3Config())
+
+
+def test_knn3_edge_cases(gpu):
+    """Items nobody rated (no candidates, predictions 0), a user whose every
+    rating is a test row (bu 0, no lookups), and k_min 0 with a threshold no
+    pair reaches (every item without neighbours)."""
+    tr, _ = synthetic("tiny_synth")
+    rows = np.stack([tr.user_idxs, tr.item_idxs, tr.raw_ratings, tr.dates], 1)
+    tr2 = dataset_from_rows(rows, tr.n_users, tr.n_items + 3)
+    tr2.residuals = (tr.raw_ratings.astype(np.float32) - np.float32(3.5)).astype(np.float32)
+    tr2.is_test = (tr.user_idxs == 0).astype(np.int8)
+    ref = _compare(tr2, dict(k_min=4, k_max=16))
+    for m in range(tr.n_items, tr.n_items + 3):
+        assert ref.neighbors[m] == []
+    with Knn3Model(tr2, Knn3Config(k_min=4, k_max=16)) as m:
+        p = m.predict(np.array([1, 2, 0], np.int32), np.array([tr.n_items, tr.n_items + 2, 5], np.int32))
+    assert p[0] == 0.0 and p[1] == 0.0 and p[2] == 0.0
+    with Knn3Model(tr2, Knn3Config(k_min=0, threshold=1e30)) as m:
+        assert all(len(x) == 0 for x in m.neighbors())
+        assert np.all(m.predict(tr2.user_idxs[:100], tr2.item_idxs[:100]) == 0.0)
