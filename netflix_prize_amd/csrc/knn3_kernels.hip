@@ -325,7 +325,8 @@ __global__ __launch_bounds__(256) void k3_cov(Knn3View v) {
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k3_nnls(Knn3View v) {
 #pragma clang fp contract(off)
-    __shared__ double Lm[kMaxK][kMaxK + 1];
+    __shared__ double Lp[kMaxK * (kMaxK + 1) / 2];  // the lower triangle, row after row
+    auto L = [&](int r, int c) -> double& { return Lp[r * (r + 1) / 2 + c]; };
     __shared__ double prod[kMaxK];
     __shared__ double resw[kMaxK];
     __shared__ int idxs[kMaxK];
@@ -350,15 +351,15 @@ __global__ __launch_bounds__(64) void k3_nnls(Knn3View v) {
         if (lane < k) {
             const int gi = idxs[lane];
             b = rhs[gi];
-            for (int rj = 0; rj < k; ++rj) Lm[lane][rj] = sys[static_cast<size_t>(gi) * kMaxK + idxs[rj]];
+            for (int rj = 0; rj <= lane; ++rj) L(lane, rj) = sys[static_cast<size_t>(gi) * kMaxK + idxs[rj]];
         }
         __syncthreads();
         bool fail = false;
         for (int j = 0; j < k; ++j) {
             double s = 0.0;
             if (lane >= j && lane < k) {
-                s = Lm[lane][j];
-                for (int kk = 0; kk < j; ++kk) s = s + (-Lm[j][kk]) * Lm[lane][kk];
+                s = L(lane, j);
+                for (int kk = 0; kk < j; ++kk) s = s + (-L(j, kk)) * L(lane, kk);
             }
             const double diag = __shfl(s, j, 64);
             if (!(diag > 0.0)) {
@@ -366,23 +367,23 @@ __global__ __launch_bounds__(64) void k3_nnls(Knn3View v) {
                 break;
             }
             const double den = sqrt(diag);
-            if (lane == j) Lm[j][j] = den;
-            else if (lane > j && lane < k) Lm[lane][j] = s / den;
+            if (lane == j) L(j, j) = den;
+            else if (lane > j && lane < k) L(lane, j) = s / den;
             __syncthreads();
         }
         if (fail) break;
         for (int i = 0; i < k; ++i) {  // L y = b
-            const double y = __shfl(b, i, 64) / Lm[i][i];
+            const double y = __shfl(b, i, 64) / L(i, i);
             if (lane == i) b = y;
-            else if (lane > i && lane < k) b = b + (-y) * Lm[lane][i];
+            else if (lane > i && lane < k) b = b + (-y) * L(lane, i);
         }
         for (int i = k - 1; i >= 0; --i) {  // L^T x = y
-            prod[lane] = (lane > i && lane < k) ? Lm[lane][i] * b : 0.0;
+            prod[lane] = (lane > i && lane < k) ? L(lane, i) * b : 0.0;
             __syncthreads();
             if (lane == i) {
                 double dot = 0.0;
                 for (int r = i + 1; r < k; ++r) dot = dot + prod[r];
-                b = (b - dot) / Lm[i][i];
+                b = (b - dot) / L(i, i);
             }
             __syncthreads();
         }
