@@ -293,6 +293,7 @@ __global__ __launch_bounds__(256) void k_knnf_predict(KnnView v, int64_t n, cons
 // Integer counts, converted to FP32 exactly below 2^24.
 // ---------------------------------------------------------------------------
 constexpr int kRowChunk = 20480;  // columns per LDS window (80 KB)
+constexpr int kRowThreads = 1024;  // threads per row block
 
 __global__ __launch_bounds__(256) void k_item_hist(const int32_t* __restrict__ items, int64_t n_rows,
                                                    uint32_t* __restrict__ icnt) {
@@ -346,7 +347,7 @@ __global__ __launch_bounds__(256) void k_item_scatter(const uint32_t* __restrict
     }
 }
 
-__global__ __launch_bounds__(512) void k_supp_rows(const int32_t* __restrict__ items, const uint32_t* __restrict__ ioff,
+__global__ __launch_bounds__(kRowThreads) void k_supp_rows(const int32_t* __restrict__ items, const uint32_t* __restrict__ ioff,
                                                    const uint32_t* __restrict__ ient, const uint32_t* __restrict__ iend,
                                                    const int32_t* __restrict__ order, float* __restrict__ supp, int ip) {
     __shared__ uint32_t cnt[kRowChunk];
@@ -356,10 +357,10 @@ __global__ __launch_bounds__(512) void k_supp_rows(const int32_t* __restrict__ i
     float* row = supp + static_cast<size_t>(i) * ip;
     for (int c0 = 0; c0 < ip; c0 += kRowChunk) {
         const int w = ip - c0 < kRowChunk ? ip - c0 : kRowChunk;
-        for (int j = tid; j < w; j += 512) cnt[j] = 0u;
+        for (int j = tid; j < w; j += kRowThreads) cnt[j] = 0u;
         __syncthreads();
         if (c0 + w > i) {  // the tails hold items >= i only
-            for (uint32_t q = q0 + wv; q < q1; q += 8) {
+            for (uint32_t q = q0 + wv; q < q1; q += kRowThreads / 64) {
                 const uint32_t t = ient[q], e = iend[q];
                 for (uint32_t s = t + lane; s < e; s += 64) {
                     const int j = items[s] - c0;
@@ -368,7 +369,7 @@ __global__ __launch_bounds__(512) void k_supp_rows(const int32_t* __restrict__ i
             }
         }
         __syncthreads();
-        for (int j = tid; j < w; j += 512) row[c0 + j] = static_cast<float>(cnt[j]);
+        for (int j = tid; j < w; j += kRowThreads) row[c0 + j] = static_cast<float>(cnt[j]);
         __syncthreads();
     }
 }
@@ -445,7 +446,7 @@ hipError_t launch_support(const uint32_t* off, const int32_t* items, int32_t n_u
     std::stable_sort(rows.begin(), rows.end(), [&](int32_t a, int32_t b) { return w[a] > w[b]; });
     e = hipMemcpyAsync(order, rows.data(), sizeof(int32_t) * rows.size(), hipMemcpyHostToDevice, st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_supp_rows, dim3(ip), dim3(512), 0, st, items, icnt, ient, iend, order, S, ip);
+    hipLaunchKernelGGL(k_supp_rows, dim3(ip), dim3(kRowThreads), 0, st, items, icnt, ient, iend, order, S, ip);
     hipLaunchKernelGGL(k_mirror, dim3(ip / 32, ip / 32), dim3(256), 0, st, S, ip);
     hipLaunchKernelGGL(k_diag, dim3((ip + 255) / 256), dim3(256), 0, st, S, ip, diag);
     e = hipGetLastError();
