@@ -319,9 +319,13 @@ __global__ __launch_bounds__(256) void k3_cov(Knn3View v) {
 // ---------------------------------------------------------------------------
 // solve_nnls (src/knn3.rs:79-120) for item m, one wave: on the active set,
 // lane r owns row r; left-looking Cholesky (diagonal not > 0: all weights 0),
-// forward and back substitution in the reference's order of operations, then
-// the most negative component (the first of equals) leaves the set.  The
-// positive weights are kept as FP32 in candidate order (:290-294).
+// forward substitution, and back substitution with each row's dot product
+// summed sequentially.  The reference's linear-algebra crate may accumulate
+// those dot products in another order, so the weights agree with the
+// reference within FP64 rounding, not bit for bit; a weight within rounding
+// of zero can then leave or stay in the active set differently.  The most
+// negative component (the first of equals) leaves the set.  The positive
+// weights are kept as FP32 in candidate order (:290-294).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k3_nnls(Knn3View v) {
 #pragma clang fp contract(off)

@@ -958,7 +958,10 @@ int main(int argc, char** argv) {
         epoch_blend_apply(job.substr(0, job.size() - std::string("__epochs").size()), w, off, SPLIT_NEW);
         return 0;
     }
-    static const char* kKnnf[] = {"tsvdx4-16-t__knnf", "tsvdx4-400lm__knnf", "tsvdx4-4800lm__knnf",
+    // run_knnf jobs of tsvdx4-new (src/bin/tsvdx4-new.rs:55-57) and tsvdx4p-new
+    // (src/bin/tsvdx4p-new.rs:21-24)
+    static const char* kKnnf[] = {"tsvdx4-16-t__knnf",    "tsvdx4-400lm__knnf",   "tsvdx4-4800lm__knnf",
+                                  "tsvdx4p-1000lm__knnf", "tsvdx4p-2000lm__knnf", "tsvdx4p-6000lm__knnf",
                                   "tsvdx4p-8000lm__knnf"};
     for (const char* name : kKnnf) {
         if (job != name) continue;

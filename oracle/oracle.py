@@ -139,6 +139,7 @@ def lib():
         L.or_dev.argtypes = [f32, i32, f32]
         L.or_freq_bin.restype = C.c_int
         L.or_freq_bin.argtypes = [C.c_uint32, C.c_int]
+        L.or_set_plant.argtypes = [C.c_int]
         _lib = L
     return _lib
 
@@ -272,6 +273,22 @@ class OracleModel:
         cnts = np.empty(n, np.uint32)
         lib().or_sud_arrays(self._m, starts.ctypes.data, dates.ctypes.data, cnts.ctypes.data)
         return starts, dates, cnts
+
+
+class planted:
+    """Context manager that plants mistake `k` in the oracle (see or_set_plant
+    in tx_oracle.c); for the tests that show an invariant catches it."""
+
+    def __init__(self, k: int):
+        self.k = int(k)
+
+    def __enter__(self):
+        lib().or_set_plant(self.k)
+        return self
+
+    def __exit__(self, *exc):
+        lib().or_set_plant(0)
+        return False
 
 
 def get_users(n, shuffle, seed, epoch):

@@ -24,7 +24,14 @@
  *     function; the ziggurat table is regenerated from the published
  *     construction and checked against the published table prefix.
  *   - low_memory=true (not covered by any reference test) is pinned by
- *     invariants and special cases in tests/test_oracle_invariants.py.
+ *     invariants and special cases in tests/test_oracle_invariants.py:
+ *     equivalence with the KAT-pinned low_memory=false path when the day
+ *     and frequency terms are inert, training on frequency bin 0 only
+ *     (quirk Q1), the init stream without the yfeat_day draws, the bias-only
+ *     model against an FP64 brute-force SGD, zero learning rates keeping
+ *     every table at init, and the ycache against the NSVD1 sum recomputed.
+ *     Each of those checks is shown to fail under a mistake planted through
+ *     or_set_plant (below), which only those tests call.
  *
  * Third-party algorithms restated here (absent from the reference tree, pinned
  * versions from Cargo.lock): rand 0.9.2 (StdRng, SliceRandom::shuffle via
@@ -438,6 +445,17 @@ static int cmp_f32(const void* a, const void* b) {
 
 static float* zalloc(int64_t n) { return calloc((size_t)(n > 0 ? n : 1), sizeof(float)); }
 
+/* Planted mistakes, for the tests that show an invariant catches them
+ * (0 = the restatement as written; nothing else sets it):
+ *   1  low-memory training uses the day's real frequency bin (not bin 0)
+ *   2  low-memory init draws the yfeat_day block anyway
+ *   3  rebuild_ycache leaves out the held-out rows
+ *   4  the end-of-user NSVD1-bias step is skipped
+ *   5  day_range one larger (the +1 span of other models)
+ *   6  the alpha_u step without the dev factor */
+static int g_plant = 0;
+OR_API void or_set_plant(int k) { g_plant = k; }
+
 OR_API or_model* or_new(const or_config* cfg, const or_dataset* tr, const or_dataset* pr) {
     zig_init();
     or_model* m = calloc(1, sizeof(or_model));
@@ -462,7 +480,7 @@ OR_API or_model* or_new(const or_config* cfg, const or_dataset* tr, const or_dat
         int32_t c = tr->user_cnts[u];
         if (c > 0) m->tu_mean[u] /= (float)c;
     }
-    m->day_range = day_range;
+    m->day_range = day_range + (g_plant == 5 ? 1 : 0);
     sud_build(&m->ud, tr, pr, U);
     const int64_t n_ud = m->ud.n_total;
 
@@ -524,6 +542,11 @@ OR_API or_model* or_new(const or_config* cfg, const or_dataset* tr, const or_dat
         rand_fill(&rng, m->yfeat_day, (int64_t)I * f, cfg->sigma_yd);
     } else {
         m->yfeat_day = zalloc(0);
+        if (g_plant == 2) {
+            float* waste = zalloc((int64_t)I * f);
+            rand_fill(&rng, waste, (int64_t)I * f, cfg->sigma_yd);
+            free(waste);
+        }
     }
     m->alpha_u = zalloc(U);
     m->but_bin = zalloc((int64_t)U * m->B);
@@ -572,7 +595,7 @@ static void rebuild_ycache(or_model* m, const or_dataset* tr, const or_dataset* 
         for (int k = 0; k < f; k++) su[k] += y[k];
         m->ycache_bias[u] += m->ybias[i];
     }
-    for (int64_t t = 0; t < pr->n_ratings; t++) {
+    for (int64_t t = 0; t < (g_plant == 3 ? 0 : pr->n_ratings); t++) {
         int32_t u = pr->user_idxs[t], i = pr->item_idxs[t];
         float* su = m->ycache + (int64_t)u * f;
         const float* y = m->yfeat + (int64_t)i * f;
@@ -730,7 +753,11 @@ static void fit_epoch_users(or_model* m, const or_dataset* tr, const or_dataset*
                 for (int k = 0; k < f; k++) su_day[k] /= norm_day;
                 for (int k = 0; k < f; k++) su_freq[k] /= norm_day;
             }
-            const int fb = cur_freq_bin;
+            int fb = cur_freq_bin;
+            if (low_mem && g_plant == 1) {
+                const int64_t pi = sud_index(&m->ud, u, day16);
+                fb = pi >= 0 ? freq_bin(m->ud.day_cnts[pi], F) : 0;
+            }
 
             /* forward (src/tx.rs:878-901) */
             const int64_t ud_idx = sud_index(&m->ud, u, day16);
@@ -773,7 +800,7 @@ static void fit_epoch_users(or_model* m, const or_dataset* tr, const or_dataset*
             /* bias steps (src/tx.rs:981-990) */
             m->ubias[u] -= c.lr_ub * err_b;
             m->but_bin[(int64_t)u * m->B + b] -= c.lr_tu * err_b;
-            m->alpha_u[u] -= c.lr_ta * err_b * dev;
+            m->alpha_u[u] -= c.lr_ta * err_b * (g_plant == 6 ? 1.0f : dev);
             if (ud_idx >= 0) m->ubias_day[ud_idx] -= c.lr_ubd * err_b;
             m->ibias[i] -= c.lr_ib * (err_b * cu_t);
             m->bit_bin[(int64_t)i * m->B + b] -= c.lr_ti * (err_b * cu_t);
@@ -830,7 +857,7 @@ static void fit_epoch_users(or_model* m, const or_dataset* tr, const or_dataset*
                 float yj = y[k];
                 y[k] -= c.lr_y * (sum_err_q[k] / norm + c.reg_y * yj);
             }
-            m->ybias[j] -= c.lr_yb * (sum_err / norm);
+            if (g_plant != 4) m->ybias[j] -= c.lr_yb * (sum_err / norm);
         }
     }
 

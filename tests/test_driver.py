@@ -50,6 +50,31 @@ def test_print_config_tsvdx4p_job():
     assert "seq_epochs: [1, 10, 20, 0, 0] }" in out.stdout
 
 
+# Every job name of the reference's dispatchers: the match arms of
+# src/bin/tsvdx4-new.rs:18-111 and src/bin/tsvdx4p-new.rs:14-24.
+REFERENCE_JOBS = (
+    ["tsvdx4-60", "tsvdx4-400lm", "tsvdx4-1600lm", "tsvdx4-2400lm", "tsvdx4-4800lm", "tsvdx4-60o", "tsvdx4-80o",
+     "tsvdx4-160o", "tsvdx4-d0tb", "tsvdx4-f1", "tsvdx4-h8nb", "tsvdx4-l32nb", "tsvdx4-m1n", "tsvdx4-q16tb",
+     "tsvdx4-s20fo", "tsvdx4-t8cu", "tsvdx4-16-t", "tsvdx4-n8nbo", "tsvdx4-p0tbo",
+     "tsvdx4-d0tb__knn3", "tsvdx4-h8nb__knn3", "tsvdx4-n8nbo__knn3", "tsvdx4-p0tbo__knn3",
+     "tsvdx4-60__knns", "tsvdx4-60o__knns", "tsvdx4-2400lm__knns", "tsvdx4-4800lm__knns",
+     "tsvdx4-16-t__knnf", "tsvdx4-400lm__knnf", "tsvdx4-4800lm__knnf", "tsvdx4-4800lm__knn-d",
+     "tsvdx4-60__epochs", "tsvdx4-60o__epochs", "tsvdx4-d0tb__epochs", "tsvdx4-f1__epochs",
+     "tsvdx4-q16tb__epochs", "tsvdx4-400lm__epochs"],
+    ["tsvdx4p-1000lm", "tsvdx4p-2000lm", "tsvdx4p-4000lm", "tsvdx4p-6000lm", "tsvdx4p-8000lm",
+     "tsvdx4p-16000lm", "tsvdx4p-1000lm__knnf", "tsvdx4p-2000lm__knnf", "tsvdx4p-6000lm__knnf",
+     "tsvdx4p-8000lm__knnf"],
+)
+
+
+@pytest.mark.parametrize("binary,job", [("tsvdx4-new", j) for j in REFERENCE_JOBS[0]] +
+                         [("tsvdx4p-new", j) for j in REFERENCE_JOBS[1]])
+def test_every_reference_job_name_is_in_the_table(binary, job):
+    out = subprocess.run([os.path.join(ROOT, "netflix_prize_amd", binary), job, "--print-config"],
+                         capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, (job, out.stderr)
+
+
 def test_unknown_job_is_an_error():
     out = subprocess.run([DRIVER, "tsvdx4-no-such-job"], capture_output=True, text=True, timeout=60)
     assert out.returncode != 0
