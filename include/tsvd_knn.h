@@ -8,7 +8,10 @@
  * src/bin/tsvdx4p-new.rs:24).
  *
  * Conventions as include/tsvd_tx.h: calls return 0 or a negative TX_E* code
- * with tx_last_error() describing it; host buffers are copied, never kept. */
+ * with tx_last_error() describing it; host buffers are copied, never kept.
+ * The *_predict calls stage through device memory allocated per call, so
+ * they may run concurrently from several threads on one ctx; create and
+ * destroy need the ctx to themselves. */
 #ifndef TSVD_KNN_H
 #define TSVD_KNN_H
 

@@ -11,7 +11,11 @@
  * Conventions:
  *   - every entry point returns 0 on success and a negative tx_status on
  *     failure; tx_last_error() then describes the failure (per thread);
- *   - calls on one tx_ctx are not re-entrant; use one ctx per host thread;
+ *   - tx_predict, tx_predict_subscores, tx_calc_rmse, tx_param_len and
+ *     tx_get_param may run concurrently from several host threads on one
+ *     tx_ctx (each call stages through device memory of its own and is
+ *     ordered after the training work queued before it); every other entry
+ *     point needs the ctx to itself, with no call in flight on it;
  *   - integer/byte inputs use the reference's on-disk dtypes (user i32,
  *     item i32 after widening, date i16, rating i8, is_test i8).
  */

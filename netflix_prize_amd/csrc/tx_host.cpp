@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <thread>
@@ -158,10 +159,19 @@ struct tx_ctx {
     int32_t next_epoch = -1, next_order = -1, next_hmin = -1;
     int64_t* d_order = nullptr;
     float* d_scratch = nullptr;
-    double* d_acc = nullptr;
-    // staging for host-buffer predict / rmse calls
-    void* d_stage = nullptr;
-    size_t stage_bytes = 0;
+    // Host-buffer predict / subscores / rmse calls stage through a stager
+    // checked out for the call: its own stream, device buffer and FP64 sums,
+    // so calls on one ctx from several threads never share device memory.
+    struct Stager {
+        hipStream_t st = nullptr;
+        hipEvent_t ready = nullptr;  // the model's stream up to the call
+        void* buf = nullptr;
+        size_t bytes = 0;
+        double* acc = nullptr;
+    };
+    std::mutex stage_mu;  // the stager pool and the BF16 refresh
+    std::vector<std::unique_ptr<Stager>> stagers;
+    std::vector<Stager*> free_stagers;
     tx_epoch_stats last{};
 
     template <typename T>
@@ -182,16 +192,6 @@ struct tx_ctx {
         return 0;
     }
 
-    int stage(size_t bytes) {
-        if (bytes <= stage_bytes) return 0;
-        if (d_stage) (void)hipFree(d_stage);
-        d_stage = nullptr;
-        stage_bytes = 0;
-        hipError_t e = hipMalloc(&d_stage, bytes);
-        if (e != hipSuccess) return fail(TX_ENOMEM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
-        stage_bytes = bytes;
-        return 0;
-    }
 
     void free_strata() {
         for (void* p : strata_allocs) (void)hipFree(p);
@@ -204,7 +204,13 @@ struct tx_ctx {
 
     ~tx_ctx() {
         free_strata();
-        if (d_stage) (void)hipFree(d_stage);
+        for (auto& g : stagers) {
+            if (g->st) (void)hipStreamSynchronize(g->st);
+            if (g->buf) (void)hipFree(g->buf);
+            if (g->acc) (void)hipFree(g->acc);
+            if (g->ready) (void)hipEventDestroy(g->ready);
+            if (g->st) (void)hipStreamDestroy(g->st);
+        }
         for (void* p : allocs) (void)hipFree(p);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
@@ -464,14 +470,20 @@ int default_blocks(tx_ctx* c) {
 }
 
 // Refresh the BF16 copies when predictions read them and a parameter changed.
-int ensure_bf16(tx_ctx* c, hipStream_t st) {
+// The copies are converted on the model's stream, which is then waited for, so
+// a prediction on any stream that starts after this returns reads them whole;
+// concurrent callers serialise on the ctx's staging lock.
+int ensure_bf16(tx_ctx* c) {
+    std::lock_guard<std::mutex> lock(c->stage_mu);
     if (!c->bf16 || !c->bf16_stale) return 0;
     const int64_t U = c->U, I = c->I, fs = c->fs, F = c->F;
+    hipStream_t st = c->st;
     HIPC(tx::launch_to_bf16(c->dm.ifeat, c->b_ifeat, I * fs, st));
     HIPC(tx::launch_to_bf16(c->dm.ifeat_freq, c->b_ifeat_freq, I * F * fs, st));
     HIPC(tx::launch_to_bf16(c->dm.ufeat, c->b_ufeat, U * fs, st));
     HIPC(tx::launch_to_bf16(c->dm.ufeat2, c->b_ufeat2, U * fs, st));
     HIPC(tx::launch_to_bf16(c->dm.ycache, c->b_ycache, U * fs, st));
+    HIPC(hipStreamSynchronize(st));
     c->bf16_stale = false;
     return 0;
 }
@@ -1040,7 +1052,6 @@ int tx_create(const tx_config* cfg, const tx_ratings* tr, const tx_ratings* pr, 
     RC(c->upload(&pf, norm_rebuild.data(), norm_rebuild.size())); m.norm_rebuild = pf;
     RC(c->alloc(&c->d_order, static_cast<size_t>(U)));
     RC(c->alloc(&c->d_scratch, static_cast<size_t>(4) * c->fs + 4));
-    RC(c->alloc(&c->d_acc, 2));
     HIPC(hipDeviceSynchronize());
     *out = ctx.release();
     return 0;
@@ -1382,7 +1393,7 @@ int tx_predict_device(tx_ctx* c, int64_t n, const int32_t* d_u, const int32_t* d
     g_err.clear();
     if (!c || n < 0 || (n > 0 && (!d_u || !d_i || !d_d || !d_out))) return fail(TX_EINVAL, "tx_predict_device: bad arguments");
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->st;
-    if (int rc = ensure_bf16(c, st)) return rc;
+    if (int rc = ensure_bf16(c)) return rc;
     HIPC(tx::launch_predict(c->dm, 0, n, d_u, d_i, d_d, d_out, nullptr, nullptr, nullptr, nullptr, nullptr,
                             c->shape.team, c->shape.vec, st));
     return 0;
@@ -1398,21 +1409,32 @@ int check_queries(tx_ctx* c, int64_t n, const int32_t* u, const int32_t* i) {
     return 0;
 }
 
-// Host-buffer predict / subscores / rmse in chunks through the staging buffer.
-int run_host(tx_ctx* c, int mode, int64_t n, const int32_t* u, const int32_t* i, const int16_t* d, float* o0, float* o1,
-             float* o2, const float* target, const int8_t* is_test, double* acc) {
-    if (int rc = check_queries(c, n, u, i)) return rc;
-    if (int rc = ensure_bf16(c, c->st)) return rc;
-    const int64_t CH = int64_t{1} << 22;
-    const int64_t chunk = std::min<int64_t>(std::max<int64_t>(n, 1), CH);
-    auto rounded = [](size_t bytes) { return (bytes + 255) / 256 * 256; };
-    const size_t ch = static_cast<size_t>(chunk);
-    const size_t need = rounded(ch * 4) * 2 + rounded(ch * 2) + rounded(ch * 4) * 3 + rounded(ch);
-    if (int rc = c->stage(need)) return rc;
-    char* base = static_cast<char*>(c->d_stage);
+// Device memory for host-buffer queries: one chunk of `rows` queries on a
+// stream, plus two FP64 sums for calc_rmse.
+struct Staging {
+    hipStream_t st;
+    char* base;
+    int64_t rows;
+    double* acc;
+};
+
+size_t round256(size_t bytes) { return (bytes + 255) / 256 * 256; }
+
+// Bytes of staging for `rows` queries: user, item (i32), day (i16), three
+// float outputs (the first doubles as calc_rmse's target), is_test (i8).
+size_t staging_bytes(int64_t rows) {
+    const size_t r = static_cast<size_t>(rows);
+    return round256(r * 4) * 2 + round256(r * 2) + round256(r * 4) * 3 + round256(r);
+}
+
+// Host-buffer predict / subscores / rmse in chunks through `sg`.
+int run_host(tx_ctx* c, const Staging& sg, int mode, int64_t n, const int32_t* u, const int32_t* i, const int16_t* d,
+             float* o0, float* o1, float* o2, const float* target, const int8_t* is_test, double* acc) {
+    const int64_t chunk = std::min<int64_t>(std::max<int64_t>(n, 1), sg.rows);
+    char* base = sg.base;
     auto carve = [&](size_t bytes) {
         char* p = base;
-        base += rounded(bytes);
+        base += round256(bytes);
         return p;
     };
     int32_t* du = reinterpret_cast<int32_t*>(carve(chunk * 4));
@@ -1422,31 +1444,92 @@ int run_host(tx_ctx* c, int mode, int64_t n, const int32_t* u, const int32_t* i,
     float* d1 = reinterpret_cast<float*>(carve(chunk * 4));
     float* d2 = reinterpret_cast<float*>(carve(chunk * 4));
     int8_t* dt = reinterpret_cast<int8_t*>(carve(chunk));
-    if (mode == 2) HIPC(tx::launch_fill_f64(c->d_acc, 0.0, 2, c->st));
+    hipStream_t st = sg.st;
+    if (mode == 2) HIPC(tx::launch_fill_f64(sg.acc, 0.0, 2, st));
     for (int64_t lo = 0; lo < n; lo += chunk) {
         const int64_t m = std::min(chunk, n - lo);
-        HIPC(hipMemcpyAsync(du, u + lo, m * 4, hipMemcpyHostToDevice, c->st));
-        HIPC(hipMemcpyAsync(di, i + lo, m * 4, hipMemcpyHostToDevice, c->st));
-        HIPC(hipMemcpyAsync(dd, d + lo, m * 2, hipMemcpyHostToDevice, c->st));
+        HIPC(hipMemcpyAsync(du, u + lo, m * 4, hipMemcpyHostToDevice, st));
+        HIPC(hipMemcpyAsync(di, i + lo, m * 4, hipMemcpyHostToDevice, st));
+        HIPC(hipMemcpyAsync(dd, d + lo, m * 2, hipMemcpyHostToDevice, st));
         if (mode == 2) {
-            HIPC(hipMemcpyAsync(d0, target + lo, m * 4, hipMemcpyHostToDevice, c->st));
-            if (is_test) HIPC(hipMemcpyAsync(dt, is_test + lo, m, hipMemcpyHostToDevice, c->st));
+            HIPC(hipMemcpyAsync(d0, target + lo, m * 4, hipMemcpyHostToDevice, st));
+            if (is_test) HIPC(hipMemcpyAsync(dt, is_test + lo, m, hipMemcpyHostToDevice, st));
         }
         HIPC(tx::launch_predict(c->dm, mode, m, du, di, dd, d0, d1, d2, mode == 2 ? d0 : nullptr,
-                                (mode == 2 && is_test) ? dt : nullptr, c->d_acc, c->shape.team, c->shape.vec, c->st));
-        if (mode == 0) HIPC(hipMemcpyAsync(o0 + lo, d0, m * 4, hipMemcpyDeviceToHost, c->st));
+                                (mode == 2 && is_test) ? dt : nullptr, sg.acc, c->shape.team, c->shape.vec, st));
+        if (mode == 0) HIPC(hipMemcpyAsync(o0 + lo, d0, m * 4, hipMemcpyDeviceToHost, st));
         if (mode == 1) {
-            HIPC(hipMemcpyAsync(o0 + lo, d0, m * 4, hipMemcpyDeviceToHost, c->st));
-            HIPC(hipMemcpyAsync(o1 + lo, d1, m * 4, hipMemcpyDeviceToHost, c->st));
-            HIPC(hipMemcpyAsync(o2 + lo, d2, m * 4, hipMemcpyDeviceToHost, c->st));
+            HIPC(hipMemcpyAsync(o0 + lo, d0, m * 4, hipMemcpyDeviceToHost, st));
+            HIPC(hipMemcpyAsync(o1 + lo, d1, m * 4, hipMemcpyDeviceToHost, st));
+            HIPC(hipMemcpyAsync(o2 + lo, d2, m * 4, hipMemcpyDeviceToHost, st));
         }
-        HIPC(hipStreamSynchronize(c->st));
+        HIPC(hipStreamSynchronize(st));
     }
     if (mode == 2) {
-        HIPC(hipMemcpyAsync(acc, c->d_acc, 2 * sizeof(double), hipMemcpyDeviceToHost, c->st));
-        HIPC(hipStreamSynchronize(c->st));
+        HIPC(hipMemcpyAsync(acc, sg.acc, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPC(hipStreamSynchronize(st));
     }
     return 0;
+}
+
+// Take a stager with room for a chunk of min(n, 4M) queries (a new one when
+// every stager is in use), ordered after the work queued on the model's
+// stream so far.
+int checkout(tx_ctx* c, int64_t n, tx_ctx::Stager** out) {
+    const int64_t rows = std::min<int64_t>(std::max<int64_t>(n, 1), int64_t{1} << 22);
+    const size_t need = staging_bytes(rows);
+    tx_ctx::Stager* g = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(c->stage_mu);
+        if (!c->free_stagers.empty()) {
+            g = c->free_stagers.back();
+            c->free_stagers.pop_back();
+        } else {
+            c->stagers.push_back(std::make_unique<tx_ctx::Stager>());
+            g = c->stagers.back().get();
+        }
+    }
+    auto give_back = [&](int rc) {
+        std::lock_guard<std::mutex> lock(c->stage_mu);
+        c->free_stagers.push_back(g);
+        return rc;
+    };
+    if (!g->st) {
+        if (hipStreamCreateWithFlags(&g->st, hipStreamNonBlocking) != hipSuccess)
+            return give_back(fail(TX_EDEVICE, "hipStreamCreate failed"));
+        if (hipEventCreateWithFlags(&g->ready, hipEventDisableTiming) != hipSuccess)
+            return give_back(fail(TX_EDEVICE, "hipEventCreate failed"));
+        if (hipMalloc(&g->acc, 2 * sizeof(double)) != hipSuccess)
+            return give_back(fail(TX_ENOMEM, "hipMalloc of the staging sums failed"));
+    }
+    if (g->bytes < need) {
+        if (g->buf) (void)hipFree(g->buf);
+        g->buf = nullptr;
+        g->bytes = 0;
+        hipError_t e = hipMalloc(&g->buf, need);
+        if (e != hipSuccess) return give_back(fail(TX_ENOMEM, "hipMalloc(%zu bytes): %s", need, hipGetErrorString(e)));
+        g->bytes = need;
+    }
+    if (hipEventRecord(g->ready, c->st) != hipSuccess || hipStreamWaitEvent(g->st, g->ready, 0) != hipSuccess)
+        return give_back(fail(TX_EDEVICE, "ordering the staging stream after the model's stream failed"));
+    *out = g;
+    return 0;
+}
+
+// run_host through a stager of the pool.
+int run_staged(tx_ctx* c, int mode, int64_t n, const int32_t* u, const int32_t* i, const int16_t* d, float* o0,
+               float* o1, float* o2, const float* target, const int8_t* is_test, double* acc) {
+    if (int rc = check_queries(c, n, u, i)) return rc;
+    if (int rc = ensure_bf16(c)) return rc;
+    tx_ctx::Stager* g = nullptr;
+    if (int rc = checkout(c, n, &g)) return rc;
+    const int64_t rows = std::min<int64_t>(std::max<int64_t>(n, 1), int64_t{1} << 22);
+    const Staging sg{g->st, static_cast<char*>(g->buf), rows, g->acc};
+    const int rc = run_host(c, sg, mode, n, u, i, d, o0, o1, o2, target, is_test, acc);
+    if (rc) (void)hipStreamSynchronize(g->st);
+    std::lock_guard<std::mutex> lock(c->stage_mu);
+    c->free_stagers.push_back(g);
+    return rc;
 }
 
 }  // namespace
@@ -1454,7 +1537,7 @@ int run_host(tx_ctx* c, int mode, int64_t n, const int32_t* u, const int32_t* i,
 int tx_predict(tx_ctx* c, int64_t n, const int32_t* u, const int32_t* i, const int16_t* d, float* out) {
     g_err.clear();
     if (!c || n < 0 || (n > 0 && (!u || !i || !d || !out))) return fail(TX_EINVAL, "tx_predict: bad arguments");
-    return run_host(c, 0, n, u, i, d, out, nullptr, nullptr, nullptr, nullptr, nullptr);
+    return run_staged(c, 0, n, u, i, d, out, nullptr, nullptr, nullptr, nullptr, nullptr);
 }
 
 int tx_predict_subscores(tx_ctx* c, int64_t n, const int32_t* u, const int32_t* i, const int16_t* d, float* bias,
@@ -1462,7 +1545,7 @@ int tx_predict_subscores(tx_ctx* c, int64_t n, const int32_t* u, const int32_t* 
     g_err.clear();
     if (!c || n < 0 || (n > 0 && (!u || !i || !d || !bias || !mf || !nsvd1)))
         return fail(TX_EINVAL, "tx_predict_subscores: bad arguments");
-    return run_host(c, 1, n, u, i, d, bias, mf, nsvd1, nullptr, nullptr, nullptr);
+    return run_staged(c, 1, n, u, i, d, bias, mf, nsvd1, nullptr, nullptr, nullptr);
 }
 
 int tx_calc_rmse(tx_ctx* c, const tx_ratings* ds, double* out) {
@@ -1471,7 +1554,7 @@ int tx_calc_rmse(tx_ctx* c, const tx_ratings* ds, double* out) {
     if (ds->n_ratings > 0 && (!ds->user_idxs || !ds->item_idxs || !ds->dates || !ds->residuals))
         return fail(TX_EINVAL, "tx_calc_rmse: rating set lacks indices or residuals");
     double acc[2] = {0.0, 0.0};
-    if (int rc = run_host(c, 2, ds->n_ratings, ds->user_idxs, ds->item_idxs, ds->dates, nullptr, nullptr, nullptr,
+    if (int rc = run_staged(c, 2, ds->n_ratings, ds->user_idxs, ds->item_idxs, ds->dates, nullptr, nullptr, nullptr,
                           ds->residuals, ds->is_test, acc))
         return rc;
     *out = std::sqrt(acc[0] / acc[1]);
