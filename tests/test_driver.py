@@ -328,6 +328,46 @@ def test_transposed_job_end_to_end(gpu, tmp_path):
     assert rmse < float(np.std(pr.raw_ratings.astype(np.float64)))
 
 
+def reference_transpose(ds, reorder):
+    """Dataset::transpose (src/lib.rs:295-402): users and items swap roles;
+    with reorder, rows are sorted by the key item * n_days + date with two
+    stable 16-bit radix passes, i.e. a stable sort by (item, date)."""
+    order = np.arange(ds.n_ratings)
+    if reorder:
+        n_days = int(ds.dates.max()) + 1
+        key = ds.item_idxs.astype(np.int64) * n_days + ds.dates.astype(np.int64)
+        order = np.argsort(key, kind="stable")
+    return tx.Dataset.from_arrays(ds.item_idxs[order], ds.user_idxs[order], ds.raw_ratings[order],
+                                  ds.dates[order], ds.n_items, ds.n_users, is_test=ds.is_test[order])
+
+
+@pytest.mark.gpu
+def test_transposed_job_serial_matches_oracle(gpu, tmp_path):
+    """tsvdx4-16-t in the serial schedule (TSVD_SCHEDULE=serial) against the
+    oracle trained on the reference's transposition of the same split: the
+    saved item factors (one row per original user) bit for bit, and the probe
+    predictions in their original order within the predict tolerance 1e-5."""
+    from oracle import oracle as O
+    tr, pr = synthetic("tiny_synth")
+    _write_split(str(tmp_path), tr, pr)
+    env = dict(os.environ, TSVD_N_USERS=str(tr.n_users), TSVD_N_ITEMS=str(tr.n_items),
+               TSVD_DATA_DIR=str(tmp_path / "data"), TSVD_SCHEDULE="serial")
+    job = "tsvdx4-16-t"
+    out = subprocess.run([DRIVER, job], cwd=tmp_path, env=env, capture_output=True, text=True, timeout=600)
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
+    tt, pt = reference_transpose(tr, True), reference_transpose(pr, False)
+    cfg = _job_config(job)
+    o = O.OracleModel(cfg, tt, pt)
+    for e in range(1, cfg.n_epochs + 1):
+        o.fit_epoch(e)
+    pd = tmp_path / "preds_new"
+    ifeat = np.load(pd / f"{job}.ifeat.trainx.npy")
+    np.testing.assert_array_equal(ifeat, o.get("ifeat").reshape(tr.n_users, cfg.n_feat))
+    probe = np.load(pd / f"{job}.probex.npy")
+    ref = o.predict(pt.user_idxs, pt.item_idxs, pt.dates)   # target rtg: no offset
+    assert np.max(np.abs(probe - ref)) <= 1e-5
+
+
 @pytest.mark.gpu
 def test_knn_d_job_end_to_end(gpu, tmp_path):
     """tsvdx4-4800lm__knn-d (run_knn_d, src/bin/tsvdx4-new.rs:590-610: KnnModel
