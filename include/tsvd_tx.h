@@ -228,6 +228,18 @@ int tx_predict(tx_ctx* ctx, int64_t n, const int32_t* users, const int32_t* item
 int tx_predict_device(tx_ctx* ctx, int64_t n, const int32_t* d_users, const int32_t* d_items,
                       const int16_t* d_days, float* d_out, void* stream);
 
+/* No counterpart in the reference.  tx_predict for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the same predictions, staged through a
+ * fixed scratch array of the library (23 MiB per device, held until the
+ * process ends) instead of memory allocated on first use.  At most
+ * TX_NOALLOC_MAX_ROWS queries per call (more: TX_EINVAL).  Every call on one
+ * device shares that array, whatever its ctx: calls that overlap on one
+ * device race. */
+#define TX_NOALLOC_MAX_ROWS 1048576
+int tx_predict_noalloc(tx_ctx* ctx, int64_t n, const int32_t* users, const int32_t* items, const int16_t* days,
+                       float* out);
+
 /* Regressor::predict_subscores (src/tx.rs:663-711): bias, mf, nsvd1. */
 int tx_predict_subscores(tx_ctx* ctx, int64_t n, const int32_t* users, const int32_t* items,
                          const int16_t* days, float* bias, float* mf, float* nsvd1);

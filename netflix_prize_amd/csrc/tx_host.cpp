@@ -159,6 +159,8 @@ struct tx_ctx {
     int32_t next_epoch = -1, next_order = -1, next_hmin = -1;
     int64_t* d_order = nullptr;
     float* d_scratch = nullptr;
+    int device = 0;              // the device current at tx_create
+    void* d_noalloc = nullptr;   // that device's tx_predict_noalloc scratch array
     // Host-buffer predict / subscores / rmse calls stage through a stager
     // checked out for the call: its own stream, device buffer and FP64 sums,
     // so calls on one ctx from several threads never share device memory.
@@ -1052,6 +1054,10 @@ int tx_create(const tx_config* cfg, const tx_ratings* tr, const tx_ratings* pr, 
     RC(c->upload(&pf, norm_rebuild.data(), norm_rebuild.size())); m.norm_rebuild = pf;
     RC(c->alloc(&c->d_order, static_cast<size_t>(U)));
     RC(c->alloc(&c->d_scratch, static_cast<size_t>(4) * c->fs + 4));
+    // the device's tx_predict_noalloc scratch array (its address looked up
+    // once per device, with the device current)
+    HIPC(hipGetDevice(&c->device));
+    HIPC(tx::noalloc_scratch(c->device, &c->d_noalloc));
     HIPC(hipDeviceSynchronize());
     *out = ctx.release();
     return 0;
@@ -1546,6 +1552,23 @@ int tx_predict_subscores(tx_ctx* c, int64_t n, const int32_t* u, const int32_t* 
     if (!c || n < 0 || (n > 0 && (!u || !i || !d || !bias || !mf || !nsvd1)))
         return fail(TX_EINVAL, "tx_predict_subscores: bad arguments");
     return run_staged(c, 1, n, u, i, d, bias, mf, nsvd1, nullptr, nullptr, nullptr);
+}
+
+int tx_predict_noalloc(tx_ctx* c, int64_t n, const int32_t* u, const int32_t* i, const int16_t* d, float* out) {
+    g_err.clear();
+    if (!c || n < 0 || (n > 0 && (!u || !i || !d || !out))) return fail(TX_EINVAL, "tx_predict_noalloc: bad arguments");
+    if (n > TX_NOALLOC_MAX_ROWS)
+        return fail(TX_EINVAL, "tx_predict_noalloc: %lld queries exceed the maximum of %d per call", (long long)n,
+                    TX_NOALLOC_MAX_ROWS);
+    if (!c->d_noalloc) return fail(TX_EDEVICE, "tx_predict_noalloc: no scratch array on device %d", c->device);
+    if (int rc = check_queries(c, n, u, i)) return rc;
+    if (int rc = ensure_bf16(c)) return rc;
+    // the staging of one chunk of TX_NOALLOC_MAX_ROWS queries, then the sums
+    char* base = static_cast<char*>(c->d_noalloc);
+    static_assert(tx::kNoAllocRows == TX_NOALLOC_MAX_ROWS, "scratch sized for the header's limit");
+    const Staging sg{c->st, base, TX_NOALLOC_MAX_ROWS,
+                     reinterpret_cast<double*>(base + staging_bytes(TX_NOALLOC_MAX_ROWS))};
+    return run_host(c, sg, 0, n, u, i, d, out, nullptr, nullptr, nullptr, nullptr, nullptr);
 }
 
 int tx_calc_rmse(tx_ctx* c, const tx_ratings* ds, double* out) {

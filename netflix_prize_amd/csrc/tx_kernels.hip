@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 
 #include "tx_launch.hpp"
 #include "tx_types.hpp"
@@ -1848,6 +1849,26 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st) {
     (void)hipGetLastError();
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 1) > 4096 ? 4096 : grid_for(n, 1)), dim3(256), 0, st, p, v, n);
     return hipGetLastError();
+}
+
+// tx_predict_noalloc's scratch: a file-scope device array, so that entry point
+// never allocates.  Every device that runs the library holds its own copy
+// until the process ends; all calls on one device share it.
+static __device__ __align__(256) char scratch[kNoAllocBytes];
+
+hipError_t noalloc_scratch(int device, void** out) {
+    static std::mutex mu;
+    static void* ptr[64] = {};
+    if (device < 0 || device >= 64) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        ptr[device] = p;
+    }
+    *out = ptr[device];
+    return hipSuccess;
 }
 
 hipError_t launch_fill_f64(double* p, double v, int64_t n, hipStream_t st) {

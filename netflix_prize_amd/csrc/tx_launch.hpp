@@ -31,4 +31,13 @@ hipError_t launch_fill_f32(float* p, float v, int64_t n, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st);
 hipError_t launch_fill_f64(double* p, double v, int64_t n, hipStream_t st);
 
+// tx_predict_noalloc's scratch array: staging for one batch of kNoAllocRows
+// host-buffer queries (user, item i32; day i16; three float outputs; is_test
+// i8; each part 256-byte aligned) followed by two FP64 sums.
+constexpr int64_t kNoAllocRows = int64_t{1} << 20;
+constexpr int64_t kNoAllocBytes = kNoAllocRows * (4 + 4 + 2 + 4 * 3 + 1) + 256;
+// Device address of that array on `device` (the current device), looked up
+// once per device and kept.
+hipError_t noalloc_scratch(int device, void** out);
+
 }  // namespace tx

@@ -37,6 +37,7 @@ PARAM_ID = {n: k for k, n in enumerate(PARAMS)}
 SCHED_SERIAL = 0
 SCHED_STRATA = 1
 SCHED_TSVDX4P = 2  # Tsvdx4pModel epochs (src/tsvdx4p.rs:798-805)
+NOALLOC_MAX_ROWS = 1 << 20  # TX_NOALLOC_MAX_ROWS
 
 
 class TxError(RuntimeError):
@@ -219,6 +220,7 @@ def lib() -> C.CDLL:
         L.tx_autotune.argtypes = [vp, i32, vp]
         L.tx_last_epoch_stats.argtypes = [vp, vp]
         L.tx_predict.argtypes = [vp, i64, vp, vp, vp, vp]
+        L.tx_predict_noalloc.argtypes = [vp, i64, vp, vp, vp, vp]
         L.tx_predict_device.argtypes = [vp, i64, vp, vp, vp, vp, vp]
         L.tx_predict_subscores.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp]
         L.tx_calc_rmse.argtypes = [vp, vp, C.POINTER(C.c_double)]
@@ -449,6 +451,17 @@ class TxModel:
         _check(lib().tx_predict(self._h, len(u), u.ctypes.data, i.ctypes.data, d.ctypes.data, out.ctypes.data),
                "tx_predict")
         return float(out[0]) if scalar else out
+
+    def predict_noalloc(self, u, i, day):
+        """predict through the library's fixed scratch array: no device memory
+        is allocated; at most NOALLOC_MAX_ROWS queries per call."""
+        u = np.ascontiguousarray(np.atleast_1d(u), np.int32)
+        i = np.ascontiguousarray(np.atleast_1d(i), np.int32)
+        d = np.ascontiguousarray(np.atleast_1d(day), np.int16)
+        out = np.empty(len(u), np.float32)
+        _check(lib().tx_predict_noalloc(self._h, len(u), u.ctypes.data, i.ctypes.data, d.ctypes.data,
+                                        out.ctypes.data), "tx_predict_noalloc")
+        return out
 
     def predict_device(self, n, d_users, d_items, d_days, d_out, stream=None) -> None:
         """Device-pointer predict (ints are raw device addresses)."""
