@@ -210,6 +210,15 @@ int tx_fit_users_epoch(tx_ctx* ctx, int32_t epoch);
 enum tx_precision { TX_PREC_F32 = 0, TX_PREC_BF16 = 1 };
 int tx_set_table_precision(tx_ctx* ctx, int32_t precision);
 
+/* Precision of the side a pass holds frozen (no reference counterpart; the
+ * reference computes in FP32 throughout): with TX_PREC_BF16 the user passes of
+ * TX_SCHED_TSVDX4P and tx_fit_users_epoch read q and q_f, and the item pass of
+ * TX_SCHED_TSVDX4P reads p, p2 and ycache, from BF16 copies (round to nearest
+ * even) made from the FP32 masters once before each pass; every update still
+ * goes to the FP32 masters, in FP32.  TX_SCHED_STRATA epochs, which update
+ * both sides per rating, are unaffected.  f <= 1024. */
+int tx_set_frozen_precision(tx_ctx* ctx, int32_t precision);
+
 /* rebuild_ycache (and rebuild_ycache_day when not low_memory) from the
  * current y tables (src/tx.rs:391-454): after tx_set_param of yfeat / ybias,
  * predictions use the new NSVD1 sums only once this has run. */

@@ -139,10 +139,15 @@ struct tx_ctx {
     bool bf16 = false, bf16_stale = true;
     uint16_t *b_ifeat = nullptr, *b_ifeat_freq = nullptr, *b_ufeat = nullptr, *b_ufeat2 = nullptr, *b_ycache = nullptr;
 
-    // tsvdx4p schedule: ratings by item, NSVD1 accumulators, per-item scales
+    // tsvdx4p schedule: ratings by item, per-user NSVD1 gradients, per-item scales
     bool iv_built = false;
     tx::ItemView iv{};
     float* d_scale = nullptr;
+    float *d_gacc = nullptr, *d_gacc_b = nullptr;  // [U * fs], [U]
+    // ycache equals the NSVD1 sums of the current y (rebuilt since y last changed)
+    bool ycache_current = false;
+    // passes that hold a side frozen read it from the BF16 copies
+    bool bf16_frozen = false;
 
     // default: one stratum, atomic item updates, lock-free NSVD1 row updates,
     // the reference's user order in length buckets
@@ -471,6 +476,23 @@ int default_blocks(tx_ctx* c) {
     return std::max(8, (blocks + 7) / 8 * 8);
 }
 
+// BF16 copies of q, q_f, p, p2 and ycache (row stride fs), allocated once.
+int alloc_bf16(tx_ctx* c) {
+    if (c->b_ifeat) return 0;
+    const size_t U = std::max<int64_t>(c->U, 1), I = std::max<int64_t>(c->I, 1), fs = c->fs, F = c->F;
+    RC(c->alloc(&c->b_ifeat, I * fs));
+    RC(c->alloc(&c->b_ifeat_freq, I * F * fs));
+    RC(c->alloc(&c->b_ufeat, U * fs));
+    RC(c->alloc(&c->b_ufeat2, U * fs));
+    RC(c->alloc(&c->b_ycache, U * fs));
+    c->dm.b_ifeat = c->b_ifeat;
+    c->dm.b_ifeat_freq = c->b_ifeat_freq;
+    c->dm.b_ufeat = c->b_ufeat;
+    c->dm.b_ufeat2 = c->b_ufeat2;
+    c->dm.b_ycache = c->b_ycache;
+    return 0;
+}
+
 // Refresh the BF16 copies when predictions read them and a parameter changed.
 // The copies are converted on the model's stream, which is then waited for, so
 // a prediction on any stream that starts after this returns reads them whole;
@@ -576,26 +598,29 @@ int build_item_view(tx_ctx* c) {
         const uint32_t nc = start[i + 1] - start[i];
         if (nc) scale[i] = ex == 0.0f ? 1.0f : std::pow(static_cast<float>(nc), ex);
     }
-    uint32_t *d_start = nullptr, *d_items = nullptr, *d_head = nullptr;
+    uint32_t *d_start = nullptr, *d_items = nullptr, *d_head = nullptr, *d_bu = nullptr;
     uint4* d_bi = nullptr;
     RC(c->upload(&d_start, start.data(), start.size()));
     RC(c->upload(&d_bi, rec.data(), rec.size()));
+    {
+        std::vector<uint32_t> bu(rec.size());
+        for (size_t k = 0; k < rec.size(); ++k) bu[k] = rec[k].x & 0xFFFFFu;
+        RC(c->upload(&d_bu, bu.data(), bu.size()));
+    }
     if (items.empty()) items.push_back(0);
     RC(c->upload(&d_items, items.data(), items.size()));
     RC(c->alloc(&d_head, 1));
     RC(c->upload(&c->d_scale, scale.data(), scale.size()));
-    float *dy = nullptr, *dyb = nullptr;
-    RC(c->alloc(&dy, static_cast<size_t>(std::max<int64_t>(I, 1)) * c->fs));
-    RC(c->alloc(&dyb, static_cast<size_t>(std::max<int64_t>(I, 1))));
-    HIPC(hipMemset(dy, 0, sizeof(float) * static_cast<size_t>(std::max<int64_t>(I, 1)) * c->fs));
-    HIPC(hipMemset(dyb, 0, sizeof(float) * static_cast<size_t>(std::max<int64_t>(I, 1))));
+    // each user's NSVD1 gradient over its norm, written by the user pass and
+    // gathered per item
+    RC(c->alloc(&c->d_gacc, static_cast<size_t>(std::max<int32_t>(c->U, 1)) * c->fs));
+    RC(c->alloc(&c->d_gacc_b, static_cast<size_t>(std::max<int32_t>(c->U, 1))));
     c->iv.q_items = d_items;
     c->iv.n_items = start[I] ? static_cast<uint32_t>(items.size()) : 0u;
     c->iv.q_head = d_head;
     c->iv.bi_start = d_start;
     c->iv.bi_rec = d_bi;
-    c->sv.d_y = dy;
-    c->sv.d_yb = dyb;
+    c->iv.bi_user = d_bu;
     c->iv_built = true;
     return 0;
 }
@@ -623,16 +648,34 @@ int tsvdx4p_epoch(tx_ctx* c, int32_t epoch, tx_epoch_stats& stats, hipStream_t s
     const int sgpr_blocks = c->sched.blocks > 0 ? c->sched.blocks : user_pass_blocks(c);
     HIPC(tx::launch_fill_u32(c->sv.q_head, 0u, c->sv.n_cells, st));
     HIPC(tx::launch_fill_u32(c->iv.q_head, 0u, 1, st));
+    tx::DevModel dm = c->dm;
+    tx::StrataView sv = c->sv;
+    sv.gacc = c->d_gacc;
+    sv.gacc_b = c->d_gacc_b;
+    sv.su_cached = (c->cfg.full_su && c->ycache_current) ? 1 : 0;
+    if (c->bf16_frozen) {
+        // the item side the user pass holds frozen, once per pass
+        dm.bf16_frozen = 1;
+        HIPC(tx::launch_to_bf16(c->dm.ifeat, c->b_ifeat, static_cast<int64_t>(c->I) * c->fs, st));
+        if (qf) HIPC(tx::launch_to_bf16(c->dm.ifeat_freq, c->b_ifeat_freq, static_cast<int64_t>(c->I) * c->F * c->fs, st));
+    }
     HIPC(hipEventRecord(c->ev[1], st));
     if (c->shape.waves > 1)
-        HIPC(tx::launch_sgd_par_mw(c->dm, c->sv, sgpr_blocks, c->shape.waves, qf, p2, false, false, st, 1));
+        HIPC(tx::launch_sgd_par_mw(dm, sv, sgpr_blocks, c->shape.waves, qf, p2, false, false, st, 1));
     else
-        HIPC(tx::launch_sgd_par(c->dm, c->sv, 0, c->shape.team, c->shape.vec, sgpr_blocks, qf, p2, false, 1, false, st));
-    HIPC(tx::launch_apply_nsvd(c->dm, c->sv.d_y, c->sv.d_yb, c->d_scale, st));
+        HIPC(tx::launch_sgd_par(dm, sv, 0, c->shape.team, c->shape.vec, sgpr_blocks, qf, p2, false, 1, false, st));
+    HIPC(tx::launch_nsvd_gather(c->dm, c->iv, c->d_gacc, c->d_gacc_b, c->d_scale, c->shape.vec, st));
     HIPC(hipEventRecord(c->ev[2], st));
     HIPC(tx::launch_rebuild(c->dm, c->n_ud, c->shape.team, c->shape.vec, st));
+    if (c->bf16_frozen) {
+        // the user side the item pass holds frozen
+        const int64_t un = static_cast<int64_t>(c->U) * c->fs;
+        HIPC(tx::launch_to_bf16(c->dm.ufeat, c->b_ufeat, un, st));
+        HIPC(tx::launch_to_bf16(c->dm.ycache, c->b_ycache, un, st));
+        if (p2) HIPC(tx::launch_to_bf16(c->dm.ufeat2, c->b_ufeat2, un, st));
+    }
     HIPC(hipEventRecord(c->ev[3], st));
-    HIPC(tx::launch_item_pass(c->dm, c->iv, c->shape.team, c->shape.vec, p2, qf, user_pass_blocks(c), st,
+    HIPC(tx::launch_item_pass(dm, c->iv, c->shape.team, c->shape.vec, p2, qf, user_pass_blocks(c), st,
                               c->shape.waves));
     HIPC(hipEventSynchronize(c->ev[3]));
     HIPC(hipStreamSynchronize(st));
@@ -1229,6 +1272,7 @@ int tx_fit_epoch(tx_ctx* c, int32_t epoch) {
         if (int rc = enforce_thresholds(c)) return rc;
     }
     c->last = stats;
+    c->ycache_current = true;  // every schedule's epoch ends with the rebuild
     return 0;
 }
 
@@ -1250,12 +1294,20 @@ int tx_fit_users_epoch(tx_ctx* c, int32_t epoch) {
     const bool qf = c->cfg.lr_iqf != 0.0f || c->cfg.sigma_iqf != 0.0f;
     const bool p2 = c->cfg.lr_u2 != 0.0f;
     HIPC(tx::launch_fill_u32(c->sv.q_head, 0u, c->sv.n_cells, st));
+    tx::DevModel dm = c->dm;
+    tx::StrataView sv = c->sv;
+    sv.su_cached = (c->cfg.full_su && c->ycache_current) ? 1 : 0;
+    if (c->bf16_frozen) {
+        dm.bf16_frozen = 1;
+        HIPC(tx::launch_to_bf16(c->dm.ifeat, c->b_ifeat, static_cast<int64_t>(c->I) * c->fs, st));
+        if (qf) HIPC(tx::launch_to_bf16(c->dm.ifeat_freq, c->b_ifeat_freq, static_cast<int64_t>(c->I) * c->F * c->fs, st));
+    }
     HIPC(hipEventRecord(c->ev[1], st));
     const int blocks = c->sched.blocks > 0 ? c->sched.blocks : user_pass_blocks(c);
     if (c->shape.waves > 1)
-        HIPC(tx::launch_sgd_par_mw(c->dm, c->sv, blocks, c->shape.waves, qf, p2, false, false, st, 2));
+        HIPC(tx::launch_sgd_par_mw(dm, sv, blocks, c->shape.waves, qf, p2, false, false, st, 2));
     else
-        HIPC(tx::launch_sgd_par(c->dm, c->sv, 0, c->shape.team, c->shape.vec, blocks, qf, p2, false, 2, false, st));
+        HIPC(tx::launch_sgd_par(dm, sv, 0, c->shape.team, c->shape.vec, blocks, qf, p2, false, 2, false, st));
     HIPC(hipEventRecord(c->ev[2], st));
     HIPC(tx::launch_rebuild(c->dm, c->n_ud, c->shape.team, c->shape.vec, st));
     HIPC(hipEventRecord(c->ev[3], st));
@@ -1270,12 +1322,14 @@ int tx_fit_users_epoch(tx_ctx* c, int32_t epoch) {
     stats.ms_other = t - a - b;
     stats.n_launches = 2;
     c->last = stats;
+    c->ycache_current = true;
     return 0;
 }
 
 int tx_autotune(tx_ctx* c, int32_t trial_epochs, tx_schedule* best_out) {
     g_err.clear();
     if (!c) return fail(TX_EINVAL, "tx_autotune: NULL ctx");
+    c->ycache_current = false;  // the trial epochs end with a restore of every table
     if (c->sched.mode != TX_SCHED_STRATA || c->sched.n_strata != 1)
         return fail(TX_EUNSUPPORTED, "tx_autotune tunes the one-stratum parallel schedule");
     if (trial_epochs < 1) trial_epochs = 1;
@@ -1359,22 +1413,25 @@ int tx_set_table_precision(tx_ctx* c, int32_t precision) {
     if (!c) return fail(TX_EINVAL, "tx_set_table_precision: NULL ctx");
     if (precision != TX_PREC_F32 && precision != TX_PREC_BF16)
         return fail(TX_EINVAL, "tx_set_table_precision: unknown precision %d", precision);
-    if (precision == TX_PREC_BF16 && !c->b_ifeat) {
-        const size_t U = std::max<int64_t>(c->U, 1), I = std::max<int64_t>(c->I, 1), fs = c->fs, F = c->F;
-        RC(c->alloc(&c->b_ifeat, I * fs));
-        RC(c->alloc(&c->b_ifeat_freq, I * F * fs));
-        RC(c->alloc(&c->b_ufeat, U * fs));
-        RC(c->alloc(&c->b_ufeat2, U * fs));
-        RC(c->alloc(&c->b_ycache, U * fs));
-        c->dm.b_ifeat = c->b_ifeat;
-        c->dm.b_ifeat_freq = c->b_ifeat_freq;
-        c->dm.b_ufeat = c->b_ufeat;
-        c->dm.b_ufeat2 = c->b_ufeat2;
-        c->dm.b_ycache = c->b_ycache;
-    }
+    if (precision == TX_PREC_BF16)
+        if (int rc = alloc_bf16(c)) return rc;
     c->bf16 = precision == TX_PREC_BF16;
     c->dm.bf16_pred = c->bf16 ? 1 : 0;
     c->bf16_stale = true;
+    return 0;
+}
+
+int tx_set_frozen_precision(tx_ctx* c, int32_t precision) {
+    g_err.clear();
+    if (!c) return fail(TX_EINVAL, "tx_set_frozen_precision: NULL ctx");
+    if (precision != TX_PREC_F32 && precision != TX_PREC_BF16)
+        return fail(TX_EINVAL, "tx_set_frozen_precision: unknown precision %d", precision);
+    if (precision == TX_PREC_BF16) {
+        if (c->shape.waves > 1)
+            return fail(TX_EUNSUPPORTED, "tx_set_frozen_precision: BF16 frozen sides need f <= 1024");
+        if (int rc = alloc_bf16(c)) return rc;
+    }
+    c->bf16_frozen = precision == TX_PREC_BF16;
     return 0;
 }
 
@@ -1384,6 +1441,7 @@ int tx_rebuild_ycache(tx_ctx* c) {
     c->bf16_stale = true;
     HIPC(tx::launch_rebuild(c->dm, c->n_ud, c->shape.team, c->shape.vec, c->st));
     HIPC(hipStreamSynchronize(c->st));
+    c->ycache_current = true;
     return 0;
 }
 
@@ -1639,6 +1697,7 @@ int tx_set_param(tx_ctx* c, int32_t id, const float* in, int64_t n) {
             if (in[k] != 0.0f) return false;
         return true;
     };
+    if (id == TX_P_YFEAT || id == TX_P_YBIAS || id == TX_P_YCACHE || id == TX_P_YCACHE_BIAS) c->ycache_current = false;
     if (id == TX_P_CUT) c->dm.skip_cut = all_zero() ? 1 : 0;
     if (id == TX_P_IBIAS_FREQ) c->dm.skip_ibf = (c->cfg.lr_ibf == 0.0f && all_zero()) ? 1 : 0;
     if (id == TX_P_BUT_BIN) c->dm.skip_but = (c->cfg.lr_tu == 0.0f && all_zero()) ? 1 : 0;

@@ -13,6 +13,7 @@
 //                    (src/tx.rs:303-339, 663-724; src/lib.rs:659-673).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <mutex>
 
@@ -584,8 +585,8 @@ struct Pending {
 
 // PASS 0: one SGD epoch of TxModel::fit_epoch (src/tx.rs:734-1115).
 // PASS 1: the user pass of Tsvdx4pModel (src/tsvdx4p.rs:390-639): item tables
-//         frozen, user parameters trained, each user's NSVD1 gradient added to
-//         per-item accumulators (sv.d_y, sv.d_yb) that a later kernel applies.
+//         frozen, user parameters trained, each user's NSVD1 gradient over its
+//         norm stored per user (sv.gacc, sv.gacc_b) for k_nsvd_gather.
 // PASS 2: the same user pass without the accumulators (fold-in of users
 //         against fixed item tables).
 // DAY:    the day terms of a model that is not low-memory (src/tx.rs:846-871,
@@ -649,6 +650,7 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_
     const bool first = (stratum == 0);
     const bool last = (stratum == S - 1);
     const int qm = sv.q_mode, ym = sv.y_mode;
+    const bool bfz = PASS != 0 && m.bf16_frozen != 0;  // q, q_f read from BF16 copies
 
     // Per-lane slot of the rating's scalars (sc): lane 0 b_i, 1 b_i,bin,
     // 2 b_u,day, 3 b_i,f(0); other lanes, and lane 3 when b_i,f is skipped,
@@ -713,10 +715,25 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_
                 n.r = __uint_as_float(rc.w);
                 n.has_bud = n.has_but = false;
                 n.bud = n.but = 0.0f;
-                load_str<VEC, TEAM>(q_base + static_cast<size_t>(n.i) * fs, n.q);
+                if (bfz) {
+                    // the user passes hold q and q_f frozen: their BF16 copies, the
+                    // raw 16 bits kept in the slot and widened where they are used
+                    // (a conversion here would wait for the load)
+                    const uint16_t* bq = m.b_ifeat + static_cast<size_t>(n.i) * fs + off;
+#pragma unroll
+                    for (int c = 0; c < VEC; ++c) n.q[c] = __uint_as_float(static_cast<uint32_t>(bq[c * TEAM]));
+                } else {
+                    load_str<VEC, TEAM>(q_base + static_cast<size_t>(n.i) * fs, n.q);
+                }
                 if constexpr (QF) {
                     const size_t qfr = static_cast<size_t>(n.i) * F + (DAY ? n.fb : 0u);
-                    load_str<VEC, TEAM>(m.ifeat_freq + qfr * fs + off, n.qf);
+                    if (bfz) {
+                        const uint16_t* bq = m.b_ifeat_freq + qfr * fs + off;
+#pragma unroll
+                        for (int c = 0; c < VEC; ++c) n.qf[c] = __uint_as_float(static_cast<uint32_t>(bq[c * TEAM]));
+                    } else {
+                        load_str<VEC, TEAM>(m.ifeat_freq + qfr * fs + off, n.qf);
+                    }
                 } else {
 #pragma unroll
                     for (int c = 0; c < VEC; ++c) n.qf[c] = 0.0f;
@@ -748,7 +765,14 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_
                 for (int c = 0; c < VEC; ++c) p2[c] = 0.0f;
             }
             float su_b;
-            if (first) {
+            if (PASS != 0 && first && sv.su_cached) {
+                // y is frozen for the whole user pass and ycache was rebuilt from
+                // it over the same rows: the NSVD1 sum without a second gather
+                load_str<VEC, TEAM>(m.ycache + prow, s);
+                su_b = m.ycache_bias[u];
+#pragma unroll
+                for (int c = 0; c < VEC; ++c) G[c] = 0.0f;
+            } else if (first) {
                 // NSVD1 user aggregation from the current y (src/tx.rs:801-822):
                 // rows summed in the contiguous layout, then transposed to the
                 // strided layout through the team's LDS slot
@@ -910,6 +934,13 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_
                 const float c_cut = m.skip_cut ? 0.0f : team_bcast_f<TEAM>(cur.sc2, 0);
                 const float c_but = cur.has_but ? cur.but : (m.skip_but ? 0.0f : team_bcast_f<TEAM>(cur.sc2, 1));
                 float bi_t = c_ib + c_bt + c_ibf;
+                // the rating's q and q_f rows (widened from BF16 in the user passes)
+                float qv[VEC], qfv[VEC];
+#pragma unroll
+                for (int c = 0; c < VEC; ++c) {
+                    qv[c] = bfz ? __uint_as_float(__float_as_uint(cur.q[c]) << 16) : cur.q[c];
+                    qfv[c] = (QF && bfz) ? __uint_as_float(__float_as_uint(cur.qf[c]) << 16) : cur.qf[c];
+                }
                 const float cu_t = cu + c_cut;
                 const float bu_t = ubias + c_but + alpha * dev + c_bud + su_b;
                 if constexpr (DAY) {
@@ -934,11 +965,11 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_
                         float pu = p[c] + s[c] + dev * p2[c];
                         pu += pud[c];
                         pu += sdr[c];
-                        part += (cur.q[c] + cur.qf[c]) * pu;
+                        part += (qv[c] + qfv[c]) * pu;
                     }
                 } else {
 #pragma unroll
-                    for (int c = 0; c < VEC; ++c) part += (cur.q[c] + cur.qf[c]) * (p[c] + s[c] + dev * p2[c]);
+                    for (int c = 0; c < VEC; ++c) part += (qv[c] + qfv[c]) * (p[c] + s[c] + dev * p2[c]);
                 }
                 float dot = team_sum_dpp<TEAM>(part);
                 if constexpr (MW) {
@@ -990,7 +1021,7 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_
                 float dq[VEC], dqf[VEC];
 #pragma unroll
                 for (int c = 0; c < VEC; ++c) {
-                    const float qe = cur.q[c] + cur.qf[c];
+                    const float qe = qv[c] + qfv[c];
                     G[c] += err_d * qe;
                     const float pe = DAY ? p[c] + dev * p2[c] + (has_ud ? pud[c] : 0.0f) + s[c] + sdr[c]
                                          : p[c] + dev * p2[c] + s[c];
@@ -1038,27 +1069,15 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_
                 m.cu[u] = cu;
             }
             if constexpr (PASS == 1) {
-                // the user's NSVD1 gradient into the per-item accumulators
-                // (NsvdDelta, src/tsvdx4p.rs:622-630): row j += G/norm, ybias
-                // j += sum_err/norm for every training item j of the user
+                // the user's NSVD1 gradient over its norm, once per user: the
+                // item-side gather (k_nsvd_gather) adds it into the step of each
+                // of the user's training items (NsvdDelta, src/tsvdx4p.rs:622-630)
+                // instead of a row of atomics per rating
                 float gq[VEC];
 #pragma unroll
                 for (int c = 0; c < VEC; ++c) gq[c] = G[c] / norm;
-                const uint32_t ntr = te - ts;
-                for (uint32_t b0 = 0; b0 < ntr; b0 += TEAM) {
-                    const uint32_t r = b0 + static_cast<uint32_t>(lane);
-                    const uint32_t jl = m.t_item[ts + (r < ntr ? r : ntr - 1)];
-                    if (r < ntr && lead)
-                        __hip_atomic_fetch_add(&sv.d_yb[jl], Gb / norm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    const uint32_t nb = ntr - b0 < static_cast<uint32_t>(TEAM) ? ntr - b0 : static_cast<uint32_t>(TEAM);
-                    for (uint32_t k = 0; k < nb; ++k) {
-                        const uint32_t j = team_bcast<TEAM>(jl, k);
-                        float* row = sv.d_y + static_cast<size_t>(j) * fs + off;
-#pragma unroll
-                        for (int c = 0; c < VEC; ++c)
-                            __hip_atomic_fetch_add(row + c * TEAM, gq[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    }
-                }
+                store_str<VEC, TEAM>(sv.gacc + prow, gq);
+                if (lane == 0) sv.gacc_b[u] = Gb / norm;
             } else if (PASS == 2) {
                 // fold-in: nothing beyond the user's own parameters
             } else if (last) {
@@ -1176,28 +1195,98 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_
 // Tsvdx4pModel: NSVD1 delta apply and the item pass
 // ---------------------------------------------------------------------------
 
-// apply_nsvd_deltas (src/tsvdx4p.rs:341-385): for every item with
-// contributors, y_j -= lr_y (D_j / s_j + reg_y y_j) and ybias_j -= lr_yb
-// D^b_j / s_j, with s_j = n_j^nsvd_norm_exp evaluated on the host (0 marks an
-// item without contributors); the accumulators are cleared for the next pass.
-__global__ __launch_bounds__(256) void k_apply_nsvd(DevModel m, float* d_y, float* d_yb, const float* scale) {
+// The NSVD1 step of the tsvdx4p user pass (apply_nsvd_deltas,
+// src/tsvdx4p.rs:341-385) as a gather: item j's summed gradient is the sum of
+// the stored gradients G_u / norm_u of the users who rated it, read in the
+// by-item order (the same order every epoch, so the step is deterministic and
+// needs no atomics), then y_j -= lr_y (D_j / s_j + reg_y y_j) and ybias_j -=
+// lr_yb Db_j / s_j with s_j = n_contrib^nsvd_norm_exp (0: no contributors,
+// no step).  One wave per item; rows wider than a wave (f > 1024) a chunk of
+// 64 * VEC elements at a time.
+template <int VEC>
+__global__ __launch_bounds__(256) void k_nsvd_gather(DevModel m, ItemView iv, const float* __restrict__ gacc,
+                                                     const float* __restrict__ gacc_b, const float* __restrict__ scale) {
 #pragma clang fp contract(off)
-    const int64_t n = static_cast<int64_t>(m.I) * m.fs;
-    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-    for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < n; e += stride) {
-        const int64_t j = e / m.fs;
-        const int k = static_cast<int>(e - j * m.fs);
+    constexpr int TEAM = 64;
+    const int lane = threadIdx.x % TEAM;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / TEAM;
+    const int nwaves = gridDim.x * blockDim.x / TEAM;
+    const int fs = m.fs;
+    for (int j = wave; j < m.I; j += nwaves) {
+        const uint32_t r0 = iv.bi_start[j], n = iv.bi_start[j + 1] - r0;
         const float sc = scale[j];
-        if (sc > 0.0f && k < m.f) {
-            const float y = m.yfeat[e];
-            m.yfeat[e] = y - m.h.lr_y * (d_y[e] / sc + m.h.reg_y * y);
-        }
-        d_y[e] = 0.0f;
-        if (k == 0) {
-            if (sc > 0.0f) m.ybias[j] = m.ybias[j] - m.h.lr_yb * (d_yb[j] / sc);
-            d_yb[j] = 0.0f;
+        if (n == 0 || !(sc > 0.0f)) continue;
+        for (int cb = 0; cb < fs; cb += TEAM * VEC) {
+            const bool first = cb == 0;
+            float acc[VEC];
+            float sb = 0.0f;
+#pragma unroll
+            for (int c = 0; c < VEC; ++c) acc[c] = 0.0f;
+            for_rows<TEAM, VEC, row_group<TEAM, VEC>()>(
+                n, lane, gacc + cb, fs, [&](uint32_t r) { return iv.bi_user + r0 + r; },
+                [&](uint32_t b0, uint32_t ib) {
+                    if (first && b0 + static_cast<uint32_t>(lane) < n) sb += gacc_b[ib];
+                },
+                [&](uint32_t, uint32_t, const float (&g)[VEC]) {
+#pragma unroll
+                    for (int c = 0; c < VEC; ++c) acc[c] += g[c];
+                });
+            float* y = m.yfeat + static_cast<size_t>(j) * fs + cb + lane * VEC;
+            float yv[VEC];
+            load_vec<VEC>(y, yv);
+#pragma unroll
+            for (int c = 0; c < VEC; ++c)
+                if (cb + lane * VEC + c < m.f) yv[c] = yv[c] - m.h.lr_y * (acc[c] / sc + m.h.reg_y * yv[c]);
+            store_vec<VEC>(y, yv);
+            if (first) {
+                const float db = team_sum_dpp<TEAM>(sb);
+                if (lane == 0) m.ybias[j] = m.ybias[j] - m.h.lr_yb * (db / sc);
+            }
         }
     }
+}
+
+// VEC contiguous FP32 words as raw bits.
+template <int VEC>
+__device__ __forceinline__ void load_raw(const float* __restrict__ p, uint32_t (&out)[VEC]) {
+    if constexpr (VEC % 4 == 0) {
+#pragma unroll
+        for (int c = 0; c < VEC; c += 4) {
+            const uint4 x = *reinterpret_cast<const uint4*>(p + c);
+            out[c] = x.x; out[c + 1] = x.y; out[c + 2] = x.z; out[c + 3] = x.w;
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < VEC; ++c) out[c] = __float_as_uint(p[c]);
+    }
+}
+
+// VEC contiguous BF16 values as raw words, two per word in out[0 .. VEC/2)
+// (one value in out[0] for VEC = 1).
+template <int VEC>
+__device__ __forceinline__ void load_bf16_raw(const uint16_t* __restrict__ p, uint32_t (&out)[VEC]) {
+    if constexpr (VEC == 1) {
+        out[0] = p[0];
+    } else if constexpr (VEC == 2) {
+        out[0] = *reinterpret_cast<const uint32_t*>(p);
+    } else if constexpr (VEC == 4) {
+        const uint2 x = *reinterpret_cast<const uint2*>(p);
+        out[0] = x.x; out[1] = x.y;
+    } else {
+#pragma unroll
+        for (int c = 0; c < VEC / 2; c += 4) {
+            const uint4 x = *reinterpret_cast<const uint4*>(p + 2 * c);
+            out[c] = x.x; out[c + 1] = x.y; out[c + 2] = x.z; out[c + 3] = x.w;
+        }
+    }
+}
+
+// Element c of a row loaded by load_raw (bf = false) or load_bf16_raw (bf = true).
+template <int VEC>
+__device__ __forceinline__ float widen(const uint32_t (&raw)[VEC], int c, bool bf) {
+    if (!bf) return __uint_as_float(raw[c]);
+    const uint32_t w = raw[c >> 1];
+    return __uint_as_float((c & 1) ? (w & 0xffff0000u) : (w << 16));
 }
 
 // One rating of the item pass and its frozen user-side inputs.
@@ -1207,7 +1296,11 @@ template <int VEC, bool P2, bool QF>
 struct ItemPending {
     uint32_t b, fb;
     float dev, r;
-    float pe[VEC];  // p + ycache + dev * p2, contiguous layout
+    // the rating's user rows as loaded, contiguous layout: p, ycache (and p2)
+    // as FP32 words or, from the BF16 copies, two values per word in the
+    // first VEC/2 words; widened where they are used, so no instruction here
+    // waits for their loads
+    uint32_t p[VEC], s[VEC], p2[P2 ? VEC : 1];
     float sc, sc2;
     // q_f row of the rating's freq bin as loaded, and the row written by an
     // earlier rating of the item after that load (has_qf), forwarded here so
@@ -1246,6 +1339,7 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256) void k_item_pass(DevModel m, I
     float* bq = bt + B;
     const int coff = wv * TEAM * VEC + lane * VEC;
     const Hyper h = m.h;
+    const bool bfz = m.bf16_frozen != 0;  // p, p2, ycache read from BF16 copies
     // lane-indexed user scalar slots: base + u*umul + (b & bm) + (ud & um)
     const float* sc_base = lane == 1 ? m.but_bin : lane == 2 ? m.alpha_u : lane == 3 ? m.ubias_day : m.ubias;
     const uint32_t sc_umul = lane == 1 ? static_cast<uint32_t>(B) : lane == 3 ? 0u : 1u;
@@ -1279,8 +1373,7 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256) void k_item_pass(DevModel m, I
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
-        auto fetch = [&](uint32_t t, ItemPending<VEC, P2, QF>& n) {
-            const uint4 rc = iv.bi_rec[t];  // same address in every lane of the team
+        auto fetch = [&](uint4 rc, ItemPending<VEC, P2, QF>& n) {
             const uint32_t u = rc.x & 0xFFFFFu;
             n.b = (rc.x >> 20) & 0xFFu;
             n.fb = rc.x >> 28;
@@ -1290,26 +1383,62 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256) void k_item_pass(DevModel m, I
             }
             n.dev = __uint_as_float(rc.z);
             n.r = __uint_as_float(rc.w);
-            float p[VEC], s[VEC], p2[VEC];
-            load_vec<VEC>(m.ufeat + static_cast<size_t>(u) * fs + coff, p);
-            load_vec<VEC>(m.ycache + static_cast<size_t>(u) * fs + coff, s);
-            if constexpr (P2) load_vec<VEC>(m.ufeat2 + static_cast<size_t>(u) * fs + coff, p2);
-#pragma unroll
-            for (int c = 0; c < VEC; ++c) n.pe[c] = P2 ? p[c] + s[c] + n.dev * p2[c] : p[c] + s[c];
+            const size_t urow = static_cast<size_t>(u) * fs + coff;
+            if (bfz) {
+                load_bf16_raw<VEC>(m.b_ufeat + urow, n.p);
+                load_bf16_raw<VEC>(m.b_ycache + urow, n.s);
+                if constexpr (P2) load_bf16_raw<VEC>(m.b_ufeat2 + urow, n.p2);
+            } else {
+                load_raw<VEC>(m.ufeat + urow, n.p);
+                load_raw<VEC>(m.ycache + urow, n.s);
+                if constexpr (P2) load_raw<VEC>(m.ufeat2 + urow, n.p2);
+            }
             n.sc = sc_base[static_cast<size_t>(u) * sc_umul + (n.b & sc_bm) + (rc.y & sc_um)];
             n.sc2 = sc2_base[static_cast<size_t>(u) * sc2_umul + (rc.y & sc2_um)];
         };
         if (r0 < r1) {
+            // the item's rating records TEAM at a time (lane l holds record
+            // tb + l), two batches ahead: a fetch takes its record from a
+            // register loaded a batch or more ago instead of loading it and
+            // waiting for it (which would drain the whole prefetch ring)
+            auto load_recs = [&](uint32_t tb) {
+                const uint32_t t = tb + static_cast<uint32_t>(lane);
+                return iv.bi_rec[t < r1 ? t : r1 - 1];
+            };
+            uint4 cb = load_recs(r0), nb = load_recs(r0 + TEAM), fb = load_recs(r0 + 2 * TEAM);
+            uint32_t tb = r0;
+            auto rec_at = [&](uint32_t t) {  // record t of the current batch or the next
+                const uint32_t j = t - tb;
+                uint4 r;
+                if (j < TEAM) {
+                    r = make_uint4(team_bcast<TEAM>(cb.x, j), team_bcast<TEAM>(cb.y, j), team_bcast<TEAM>(cb.z, j),
+                                   team_bcast<TEAM>(cb.w, j));
+                } else {
+                    asm volatile("");
+                    const uint32_t l = j - TEAM;
+                    r = make_uint4(team_bcast<TEAM>(nb.x, l), team_bcast<TEAM>(nb.y, l), team_bcast<TEAM>(nb.z, l),
+                                   team_bcast<TEAM>(nb.w, l));
+                }
+                return r;
+            };
             ItemPending<VEC, P2, QF> ring[RING];
 #pragma unroll
-            for (int k = 0; k < DEPTH; ++k) fetch(r0 + k < r1 ? r0 + k : r1 - 1, ring[k]);
-            for (uint32_t t0 = r0; t0 < r1; t0 += RING) {
+            for (int k = 0; k < DEPTH; ++k) fetch(rec_at(r0 + k < r1 ? r0 + k : r1 - 1), ring[k]);
+            for (; tb < r1; tb += TEAM) {
+            for (uint32_t t0 = tb; t0 < tb + TEAM && t0 < r1; t0 += RING) {
 #pragma unroll
                 for (int k = 0; k < RING; ++k) {
                     const uint32_t t = t0 + k;
                     if (t >= r1) break;
                     const ItemPending<VEC, P2, QF>& cur = ring[k];
-                    fetch(t + DEPTH < r1 ? t + DEPTH : r1 - 1, ring[(k + DEPTH) % RING]);
+                    fetch(rec_at(t + DEPTH < r1 ? t + DEPTH : r1 - 1), ring[(k + DEPTH) % RING]);
+                    float pe[VEC];  // p + ycache + dev * p2
+#pragma unroll
+                    for (int c = 0; c < VEC; ++c) {
+                        const float pv = widen<VEC>(cur.p, c, bfz), sv = widen<VEC>(cur.s, c, bfz);
+                        if constexpr (P2) pe[c] = pv + sv + cur.dev * widen<VEC>(cur.p2, c, bfz);
+                        else pe[c] = pv + sv;
+                    }
                     const float c_ub = team_bcast_f<TEAM>(cur.sc, 0), c_but = team_bcast_f<TEAM>(cur.sc, 1);
                     const float c_al = team_bcast_f<TEAM>(cur.sc, 2), c_bud = team_bcast_f<TEAM>(cur.sc, 3);
                     const float c_ycb = team_bcast_f<TEAM>(cur.sc2, 0), c_cu = team_bcast_f<TEAM>(cur.sc2, 1);
@@ -1328,7 +1457,7 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256) void k_item_pass(DevModel m, I
                     }
                     float part = 0.0f;
 #pragma unroll
-                    for (int c = 0; c < VEC; ++c) part += cur.pe[c] * (q[c] + qf[c]);
+                    for (int c = 0; c < VEC; ++c) part += pe[c] * (q[c] + qf[c]);
                     float dot = team_sum_dpp<TEAM>(part);
                     if constexpr (MW) {
                         if (lane == 0) dred[dpar][wv] = dot;
@@ -1344,11 +1473,11 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256) void k_item_pass(DevModel m, I
                     bt[cur.b] = c_bt - h.lr_ti * gi;   // every lane writes the same value
                     bq[cur.fb] = c_bq - h.lr_ibf * gi;
 #pragma unroll
-                    for (int c = 0; c < VEC; ++c) q[c] -= h.lr_i * (err * cur.pe[c] + h.reg_i * q[c]);
+                    for (int c = 0; c < VEC; ++c) q[c] -= h.lr_i * (err * pe[c] + h.reg_i * q[c]);
                     if constexpr (QF) {
                         float nq[VEC];
 #pragma unroll
-                        for (int c = 0; c < VEC; ++c) nq[c] = qf[c] - h.lr_iqf * (err * cur.pe[c] + h.reg_iqf * qf[c]);
+                        for (int c = 0; c < VEC; ++c) nq[c] = qf[c] - h.lr_iqf * (err * pe[c] + h.reg_iqf * qf[c]);
                         store_vec<VEC>(m.ifeat_freq + (static_cast<size_t>(i) * F + cur.fb) * fs + coff, nq);
                         // ratings t+1 .. t+DEPTH were loaded before this store
 #pragma unroll
@@ -1362,6 +1491,10 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256) void k_item_pass(DevModel m, I
                         }
                     }
                 }
+            }
+            cb = nb;
+            nb = fb;
+            fb = load_recs(tb + 3 * TEAM);
             }
         }
         store_vec<VEC>(m.ifeat + static_cast<size_t>(i) * fs + coff, q);
@@ -1732,13 +1865,19 @@ hipError_t launch_sgd_par_mw(const DevModel& m, const StrataView& sv, int teams,
                : sgd_par_mw<false, 8>(m, sv, teams, waves, qf, p2, ord, st);
 }
 
-hipError_t launch_apply_nsvd(const DevModel& m, float* d_y, float* d_yb, const float* scale, hipStream_t st) {
-    const int64_t n = static_cast<int64_t>(m.I) * m.fs;
-    if (n <= 0) return hipSuccess;
-    int64_t blocks = (n + 255) / 256;
-    if (blocks > 65536) blocks = 65536;
+hipError_t launch_nsvd_gather(const DevModel& m, const ItemView& iv, const float* gacc, const float* gacc_b,
+                              const float* scale, int vec, hipStream_t st) {
+    if (m.I <= 0) return hipSuccess;
+    const int blocks = std::min(2048, (m.I + 3) / 4);
     (void)hipGetLastError();
-    hipLaunchKernelGGL(k_apply_nsvd, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, m, d_y, d_yb, scale);
+    switch (vec) {
+        case 1: hipLaunchKernelGGL(k_nsvd_gather<1>, dim3(blocks), dim3(256), 0, st, m, iv, gacc, gacc_b, scale); break;
+        case 2: hipLaunchKernelGGL(k_nsvd_gather<2>, dim3(blocks), dim3(256), 0, st, m, iv, gacc, gacc_b, scale); break;
+        case 4: hipLaunchKernelGGL(k_nsvd_gather<4>, dim3(blocks), dim3(256), 0, st, m, iv, gacc, gacc_b, scale); break;
+        case 8: hipLaunchKernelGGL(k_nsvd_gather<8>, dim3(blocks), dim3(256), 0, st, m, iv, gacc, gacc_b, scale); break;
+        case 16: hipLaunchKernelGGL(k_nsvd_gather<16>, dim3(blocks), dim3(256), 0, st, m, iv, gacc, gacc_b, scale); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

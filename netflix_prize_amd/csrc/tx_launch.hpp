@@ -19,7 +19,10 @@ hipError_t launch_sgd_par(const DevModel& m, const StrataView& sv, int stratum, 
 hipError_t launch_sgd_par_mw(const DevModel& m, const StrataView& sv, int teams, int waves, bool qf, bool p2, bool ord,
                              bool day, hipStream_t st, int pass = 0);
 hipError_t launch_to_bf16(const float* x, uint16_t* y, int64_t n, hipStream_t st);
-hipError_t launch_apply_nsvd(const DevModel& m, float* d_y, float* d_yb, const float* scale, hipStream_t st);
+// the NSVD1 step of the tsvdx4p user pass from the stored per-user gradients
+// (gacc [U * fs], gacc_b [U]) gathered per item; vec = the shape's VEC
+hipError_t launch_nsvd_gather(const DevModel& m, const ItemView& iv, const float* gacc, const float* gacc_b,
+                              const float* scale, int vec, hipStream_t st);
 // waves > 1: multi-wave teams (f > 1024), one item per block of 64*waves threads
 hipError_t launch_item_pass(const DevModel& m, const ItemView& iv, int team, int vec, bool p2, bool qf,
                             int blocks, hipStream_t st, int waves = 1);

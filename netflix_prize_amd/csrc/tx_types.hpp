@@ -73,8 +73,10 @@ struct DevModel {
     const float* norm_rebuild;  // [U] sqrt(train_cnt + heldout_cnt)
 
     // BF16 read copies of the factor tables (row stride fs; FP32 tables stay
-    // the masters); predictions read them when bf16_pred is set
-    int32_t bf16_pred;
+    // the masters); predictions read them when bf16_pred is set, and the
+    // passes that hold a side frozen (the user passes: q, q_f; the item pass:
+    // p, p2, ycache) read that side from them when bf16_frozen is set
+    int32_t bf16_pred, bf16_frozen;
     const uint16_t *b_ifeat, *b_ifeat_freq, *b_ufeat, *b_ufeat2, *b_ycache;
 };
 
@@ -90,13 +92,14 @@ struct StrataView {
     const uint32_t* q_users;    // per (stratum, cell) queues of users, longest segment first
     const uint32_t* q_start;    // [n_strata * n_cells + 1]
     uint32_t* q_head;           // [n_strata * n_cells] work counters (reset per epoch)
-    float* gacc;                // [U * fs] per-user sum_err_q across strata
-    float* gacc_b;              // [U]      per-user sum_err across strata
+    float* gacc;                // [U * fs] per-user sum_err_q across strata; user pass: sum_err_q / norm
+    float* gacc_b;              // [U]      per-user sum_err across strata; user pass: sum_err / norm
     int32_t q_mode;             // item-row update: 0 device atomic, 1 XCD-local atomic, 2 plain
     int32_t y_mode;             // NSVD1-row update, same codes
-    float* d_y;                 // [I * fs] NSVD1 gradient accumulators of the user pass (tsvdx4p)
-    float* d_yb;                // [I]
     float* day_acc;             // [D * fs] day terms: a day's y_day sum (user start), then its gradient sum
+    // user passes: the NSVD1 sum of a user is ycache[u] (current, and built
+    // over the same rows: full_su), not gathered again from y
+    int32_t su_cached;
     int32_t block_threads;      // SGD workgroup size of the one-wave teams: 64 x teams per block (0 = 256)
 };
 
@@ -109,6 +112,7 @@ struct ItemView {
     // per rating, training-file order within an item (ByItemIndex, src/tsvdx4p.rs:29-57):
     //   x = user | bin << 20 | freq_bin << 28, y = SparseUD index, z = dev, w = target
     const uint4* bi_rec;
+    const uint32_t* bi_user;   // [N] the user of each rating, same order
 };
 
 }  // namespace tx

@@ -221,6 +221,7 @@ def lib() -> C.CDLL:
         L.tx_last_epoch_stats.argtypes = [vp, vp]
         L.tx_predict.argtypes = [vp, i64, vp, vp, vp, vp]
         L.tx_predict_noalloc.argtypes = [vp, i64, vp, vp, vp, vp]
+        L.tx_set_frozen_precision.argtypes = [vp, i32]
         L.tx_predict_device.argtypes = [vp, i64, vp, vp, vp, vp, vp]
         L.tx_predict_subscores.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp]
         L.tx_calc_rmse.argtypes = [vp, vp, C.POINTER(C.c_double)]
@@ -431,6 +432,13 @@ class TxModel:
         (BF16 copies of the FP32 masters, refreshed after every change)."""
         code = {"f32": 0, "bf16": 1}[precision]
         _check(lib().tx_set_table_precision(self._h, code), "tx_set_table_precision")
+
+    def set_frozen_precision(self, precision: str) -> None:
+        """"f32" (default) or "bf16": the side a user or item pass holds
+        frozen (q, q_f in the user passes; p, p2, ycache in the item pass) is
+        read from BF16 copies of the FP32 masters."""
+        code = {"f32": 0, "bf16": 1}[precision]
+        _check(lib().tx_set_frozen_precision(self._h, code), "tx_set_frozen_precision")
 
     def rebuild_ycache(self) -> None:
         """rebuild_ycache (src/tx.rs:391-418) from the current y tables."""

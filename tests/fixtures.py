@@ -68,3 +68,15 @@ def lm_config(**kw):
     per-test overrides, as a TxConfig."""
     from netflix_prize_amd.tx import TxConfig
     return TxConfig(**kw)
+
+
+_FULL = {}
+
+
+def full_data(seed=1):
+    """The full-shape synthetic set (480,189 x 17,770, 100,480,507 ratings),
+    generated once per process and shared by every test that needs it."""
+    if seed not in _FULL:
+        from netflix_prize_amd import synth
+        _FULL[seed] = synth.generate(synth.FULL, seed)
+    return _FULL[seed]

@@ -104,15 +104,9 @@ def test_parallel_schedule_holdout_rmse(gpu, n_strata):
     assert abs(r_g - r_o) <= 1e-3
 
 
-_FULL = {}
-
-
 def _full_data(seed):
-    """The full-shape synthetic set, generated once per process."""
-    if seed not in _FULL:
-        from netflix_prize_amd import synth
-        _FULL[seed] = synth.generate(synth.FULL, seed)
-    return _FULL[seed]
+    from tests.fixtures import full_data
+    return full_data(seed)
 
 
 @pytest.mark.parametrize("f", [64, 256])
