@@ -61,6 +61,28 @@ def profiled_traffic(f: int, args, kind: str = "") -> float | None:
     return float(json.loads(block)["traffic_bytes_per_launch"])
 
 
+def atomic_ceiling() -> dict | None:
+    """The device's rate for the SGD kernel's item-row update pattern
+    (netflix_prize_amd/atomic-ceiling: 1 KB rows of a 17,770-row table updated
+    by 64-lane FP32 atomic adds, by 64-bit integer atomics, and by plain
+    load + store), as the best rate over its wave counts per form."""
+    import subprocess
+    exe = os.path.join(ROOT, "netflix_prize_amd", "atomic-ceiling")
+    if not os.path.exists(exe):
+        return None
+    out = subprocess.run([exe, "1000"], capture_output=True, text=True, timeout=120)
+    if out.returncode != 0:
+        log(f"bench: atomic-ceiling failed: {out.stderr[-500:]}")
+        return None
+    best = {}
+    for line in out.stdout.splitlines():
+        r = json.loads(line)
+        if r["lanes_per_s"] > best.get(r["form"], {}).get("lanes_per_s", 0.0):
+            best[r["form"]] = r
+    return {k: {"rows_per_s": v["rows_per_s"], "lanes_per_s": v["lanes_per_s"], "waves": v["waves"]}
+            for k, v in best.items()}
+
+
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
@@ -142,6 +164,8 @@ def main(argv=None) -> int:
                     help="time KnnfModel (similarity matrix + held-out predictions) over the model's item factors")
     ap.add_argument("--knnf-f", type=int, default=0,
                     help="use random item factors of this width instead of the trained model's (0 = trained)")
+    ap.add_argument("--frozen-epochs", type=int, default=4,
+                    help="epochs of the tsvdx4p schedule with FP32 and with BF16 frozen sides (0 = skip)")
     ap.add_argument("--day-epochs", type=int, default=3,
                     help="epochs of the tsvdx4-60 job's model (day terms, f=60) in the parallel schedule (0 = skip)")
     args = ap.parse_args(argv)
@@ -163,8 +187,9 @@ def main(argv=None) -> int:
                                     reset_u_epoch=10_000))
     t0 = time.time()
     model = tx.TxModel(tr, pr, cfg)
+    t_create = time.time() - t0
     model.set_schedule(mode=tx.SCHED_STRATA, n_strata=args.strata, use_graph=bool(args.graph), blocks=args.blocks)
-    log(f"bench: model created ({time.time() - t0:.1f}s)")
+    log(f"bench: model created ({t_create:.1f}s)")
 
     have_torch_gpu = torch.cuda.is_available()  # synchronisation of the device around the timed region
 
@@ -300,6 +325,29 @@ def main(argv=None) -> int:
                         "predict_s": round(t_pred, 4), "queries_per_s": round(pr.n_ratings / t_pred, 1)}
         log(f"bench: knnf {knnf}")
 
+    # the tsvdx4p schedule (Tsvdx4pModel: user pass, NSVD1 gather, item pass)
+    # with the frozen side of each pass read in FP32 and from BF16 copies
+    frozen = None
+    if args.frozen_epochs > 0:
+        pcfg = tx.TxConfig(n_feat=cfg.n_feat, n_epochs=20, n_time_bins=30, nsvd_norm_exp=1.0, lr_yb=2e-4,
+                           reset_u_epoch=10_000)
+        frozen = {"workload": f"tsvdx4p schedule, lm set, f={cfg.n_feat}, nsvd_norm_exp 1, "
+                              f"{args.frozen_epochs} epochs (the first builds the by-item index)"}
+        for prec in ("f32", "bf16"):
+            pm = tx.TxModel(tr, pr, pcfg)
+            pm.set_schedule(mode=tx.SCHED_TSVDX4P)
+            pm.set_frozen_precision(prec)
+            st = []
+            for e in range(1, args.frozen_epochs + 1):
+                pm.fit_epoch(epoch=e)
+                st.append(pm.last_epoch_stats())
+            later = st[1:] or st
+            frozen[prec] = {"ms_per_epoch": round(statistics.median(x["ms_total"] for x in later), 2),
+                            "ms_user_pass_and_nsvd1": round(statistics.median(x["ms_sgd"] for x in later), 2),
+                            "holdout_rmse": round(pm.calc_rmse(pr), 6)}
+            pm.close()
+            log(f"bench: tsvdx4p {prec} frozen sides: {frozen[prec]}")
+
     # the tsvdx4-60 job's model (lm set with day terms, low_memory false, f=60,
     # 29 time bins, seed 42) in the default parallel schedule: epoch times and
     # the holdout RMSE against the serial oracle's after the same epochs
@@ -363,12 +411,30 @@ def main(argv=None) -> int:
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(cfg.n_feat, args),
                 "kernel": "k_sgd_par", "bytes_per_rating": round(ab["sgd"], 1),
                 "avg_launch_ms": round(avg_launch_ms, 3), "launches_per_epoch": launches}
+    # the item-row updates are FP32 atomic adds: the lanes of one launch against
+    # the device's measured atomic rate for that pattern
+    ceil = atomic_ceiling()
+    if ceil and "f32-atomic" in ceil:
+        ibias_lanes = 2 if (cfg.lr_ibf == 0.0) else 3          # b_i, b_i,bin (+ b_i,f)
+        lanes = cfg.n_feat * (2 if qf else 1) + ibias_lanes
+        rate = ceil["f32-atomic"]["lanes_per_s"]
+        got = lanes * n_train / launches / (avg_launch_ms / 1000.0)
+        roofline["atomic"] = {"unit": "FP32 atomic lanes/s", "achieved": round(got, 1), "ceiling": round(rate, 1),
+                              "frac": round(got / rate, 4), "lanes_per_rating": lanes,
+                              "floor_ms": round(lanes * n_train / launches / rate * 1000.0, 3),
+                              "measured_forms": ceil}
+        log(f"bench: atomic roofline {roofline['atomic']}")
 
     cpu = None
+    # host cores: the process's affinity; worker pools stay within the 16-core
+    # share the GPU box allots one GPU
+    n_aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     if not args.no_cpu:
-        workers = args.cpu_workers or min(16, os.cpu_count() or 1)
+        workers = args.cpu_workers or min(16, n_aff)
         log(f"bench: CPU baseline (~{args.cpu_seconds:.0f}s serial + parallel variant on {workers} threads)")
         cpu = cpu_baseline(tr, pr, cfg, args.cpu_seconds, workers)
+        cpu["host_cpu_count"] = os.cpu_count()
+        cpu["host_affinity_cores"] = n_aff
 
     line = {
         "metric": "timeSVD++ SGD ratings/s/epoch (f=256); time-to-holdout-RMSE vs plain GPU form",
@@ -388,11 +454,12 @@ def main(argv=None) -> int:
                                f"{args.strata} strat{'um' if args.strata == 1 else 'a'}, "
                                f"{'hipGraph epoch' if args.graph else 'stream launches'}",
                    "n_train": n_train, "n_heldout": pr.n_ratings, "holdout_rmse": round(rmse, 5),
-                   "epochs_trained": epoch},
+                   "epochs_trained": epoch, "tx_create_s": round(t_create, 2)},
         "roofline": roofline,
         "time_to_rmse": ttr,
         "fold_in": fold,
         "day_terms": day,
+        "tsvdx4p_frozen": frozen,
         "knnf": knnf,
         "cpu_baseline": cpu,
     }

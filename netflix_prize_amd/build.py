@@ -20,6 +20,7 @@ INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(PKG, "libtsvd_tx.so")
 ENC_LIB = os.path.join(PKG, "libtsvd_enc.so")
 DRIVER = os.path.join(PKG, "tsvdx4-new")
+PROBE = os.path.join(PKG, "atomic-ceiling")
 OBJ = os.path.join(PKG, "_obj")
 
 ARCH = os.environ.get("TSVD_OFFLOAD_ARCH", "gfx950")
@@ -88,6 +89,12 @@ def build(verbose: bool = False) -> str:
     dt = _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{INCLUDE}", f"-I{CSRC}", os.path.join(CSRC, "tsvdx4_new.cpp"),
                "-o", tmp, f"-L{PKG}", "-ltsvd_tx", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, DRIVER)
+    # the device's ceiling for the SGD kernel's item-row update pattern (bench.py runs it)
+    tmp = PROBE + ".tmp"
+    dt = _run([hipcc, *HIP_FLAGS, *inc, os.path.join(CSRC, "atomic_ceiling.hip"), "-o", tmp])
+    os.replace(tmp, PROBE)
+    if verbose:
+        print(f"atomic_ceiling.hip {dt:.1f}s")
     # the tsvdx4p-new jobs run from the same binary under the reference's name
     link = os.path.join(PKG, "tsvdx4p-new")
     if os.path.lexists(link):
