@@ -188,7 +188,8 @@ def main(argv=None) -> int:
     t0 = time.time()
     model = tx.TxModel(tr, pr, cfg)
     t_create = time.time() - t0
-    model.set_schedule(mode=tx.SCHED_STRATA, n_strata=args.strata, use_graph=bool(args.graph), blocks=args.blocks)
+    model.set_schedule(mode=tx.SCHED_STRATA, n_strata=args.strata, use_graph=bool(args.graph), blocks=args.blocks,
+                       experimental=args.strata > 1)
     log(f"bench: model created ({t_create:.1f}s)")
 
     have_torch_gpu = torch.cuda.is_available()  # synchronisation of the device around the timed region
@@ -219,10 +220,12 @@ def main(argv=None) -> int:
     log(f"bench: holdout RMSE after {epoch} epochs: {rmse:.5f}")
 
     # time-to-holdout-RMSE: fresh models trained for the full epoch budget in
-    # the batched form (length buckets, autotuned teams, hipGraph epoch) and in
-    # the plain GPU form (reference user order, 1,024 teams, stream launches);
-    # GPU epoch time until the holdout RMSE is within 1e-3 of the serial
-    # trainer's after the same epochs (or of the form's own final RMSE)
+    # the batched form (length buckets, autotuned teams, hipGraph epoch), in the
+    # plain GPU form (TX_SCHED_PLAIN: one wave per user over the epoch's user
+    # order, no prefetch, queue, buckets or graph, 1,024 waves) and in the
+    # tsvdx4p schedule with BF16 frozen sides; GPU epoch time until the holdout
+    # RMSE is within 1e-3 of the serial trainer's after the same epochs (or of
+    # the form's own final RMSE); tx_create time reported beside it
     ttr = None
     if args.ttr_epochs > 0:
         ref = None
@@ -233,33 +236,40 @@ def main(argv=None) -> int:
                 ref = g["epochs"][args.ttr_epochs - 1]["holdout_rmse"]
 
         def curve_of(form):
-            m2 = tx.TxModel(tr, pr, cfg)
+            t0 = time.time()
+            fcfg = cfg.replace(nsvd_norm_exp=1.0) if form == "tsvdx4p_bf16" else cfg
+            m2 = tx.TxModel(tr, pr, fcfg)
+            created = time.time() - t0
             tuned = None
             if form == "batched":
                 m2.set_schedule(mode=tx.SCHED_STRATA, n_strata=args.strata, use_graph=bool(args.graph),
-                                blocks=args.blocks)
+                                blocks=args.blocks, experimental=args.strata > 1)
                 if args.autotune:
                     tuned = m2.autotune(1)
+            elif form == "plain":
+                m2.set_schedule(mode=tx.SCHED_PLAIN, blocks=1024)
             else:
-                # 1,024 one-wave teams, four per 256-thread block
-                m2.set_schedule(mode=tx.SCHED_STRATA, user_order=0, blocks=1024 // 4)
+                m2.set_schedule(mode=tx.SCHED_TSVDX4P)
+                m2.set_frozen_precision("bf16")
             curve, secs = [], []
             for e in range(1, args.ttr_epochs + 1):
                 m2.fit_epoch(epoch=e)
                 secs.append(m2.last_epoch_stats()["ms_total"] / 1000.0)
                 curve.append(m2.calc_rmse(pr))
             m2.close()
-            return curve, secs, tuned
+            return curve, secs, tuned, created
 
         ttr = {"epochs": args.ttr_epochs, "serial_reference_rmse": None if ref is None else round(ref, 5)}
-        for form in ("batched", "plain"):
-            curve, secs, tuned = curve_of(form)
+        for form in ("batched", "plain", "tsvdx4p_bf16"):
+            curve, secs, tuned, created = curve_of(form)
             target = (ref if ref is not None else curve[-1]) + 1e-3
             hit = next((k for k, r in enumerate(curve) if r <= target), None)
             ttr[form] = {"final_holdout_rmse": round(curve[-1], 5), "target_rmse": round(target, 5),
                          "epochs_to_target": None if hit is None else hit + 1,
                          "seconds_to_target": None if hit is None else round(sum(secs[:hit + 1]), 3),
-                         "seconds_total": round(sum(secs), 3), "curve": [round(r, 5) for r in curve]}
+                         "seconds_total": round(sum(secs), 3), "tx_create_s": round(created, 2),
+                         "ms_per_epoch": round(statistics.median(secs[1:] or secs) * 1000.0, 2),
+                         "curve": [round(r, 5) for r in curve]}
             if tuned is not None:
                 ttr[form]["autotuned_schedule"] = {k: tuned[k] for k in ("blocks", "user_order", "heavy_min")}
             log(f"bench: time-to-RMSE {form}: {ttr[form]}")
@@ -399,7 +409,8 @@ def main(argv=None) -> int:
     if graph:
         # in graph mode ms_sgd spans the whole captured epoch; take out the rebuild
         # share measured in a non-graph epoch
-        model.set_schedule(mode=tx.SCHED_STRATA, n_strata=args.strata, use_graph=False, blocks=args.blocks)
+        model.set_schedule(mode=tx.SCHED_STRATA, n_strata=args.strata, use_graph=False, blocks=args.blocks,
+                           experimental=args.strata > 1)
         epoch += 1
         model.fit_epoch(epoch=epoch)
         s = model.last_epoch_stats()

@@ -94,7 +94,14 @@ typedef struct tx_ratings {
  *     epoch instead.  Low-memory models without ordinal head; the q_f stream
  *     reads bin 0 in the user pass and the day's real freq bin in the item
  *     pass, as the reference does. */
-enum tx_sched_mode { TX_SCHED_SERIAL = 0, TX_SCHED_STRATA = 1, TX_SCHED_TSVDX4P = 2 };
+/*   TX_SCHED_PLAIN: the plain per-user wavefront kernel: `blocks` one-wave
+ *     blocks (0 = 1,024) take the epoch's users in the reference's order, one
+ *     after another per wave; each loads and waits for what a rating needs,
+ *     atomic item updates, the y step at the user's end.  No prefetch, queue,
+ *     length buckets or graph: the reference point the batched
+ *     TX_SCHED_STRATA kernel is measured against.  Low-memory models without
+ *     ordinal head, f <= 1024. */
+enum tx_sched_mode { TX_SCHED_SERIAL = 0, TX_SCHED_STRATA = 1, TX_SCHED_TSVDX4P = 2, TX_SCHED_PLAIN = 3 };
 
 typedef struct tx_schedule {
     int32_t mode;
@@ -119,10 +126,16 @@ typedef struct tx_schedule {
  *   1 = XCD-local atomic add (coherent only when one XCD owns the rows,
  *       i.e. item rows with n_strata >= 8),
  *   2 = plain read-modify-write (lock-free "Hogwild" updates).
- * Default (tx_create): atomic item rows, plain NSVD1 rows. */
+ * Default (tx_create): atomic item rows, plain NSVD1 rows.
+ * Bit 4 (TX_UPD_STRATA_EXPERIMENTAL) is required for n_strata > 1: that
+ * schedule applies every user's NSVD1 step at the last stratum, and at the
+ * full Netflix shape (f = 256) its holdout RMSE diverges (NaN with atomic y
+ * rows, 1.136 after 6 epochs with plain y rows); without the bit
+ * tx_set_schedule returns TX_EUNSUPPORTED. */
 #define TX_UPD_ATOMIC 0
 #define TX_UPD_XCD_ATOMIC 1
 #define TX_UPD_PLAIN 2
+#define TX_UPD_STRATA_EXPERIMENTAL 16
 
 /* Per-epoch device timing (hipEvents on the library's stream). */
 typedef struct tx_epoch_stats {

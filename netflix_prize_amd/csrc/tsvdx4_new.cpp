@@ -353,6 +353,7 @@ struct Model {
         if (mode == "serial") s.mode = TX_SCHED_SERIAL;
         else if (mode == "strata") s.mode = TX_SCHED_STRATA;
         s.n_strata = std::atoi(env("TSVD_STRATA", "1").c_str());
+        if (s.n_strata > 1) s.update_flags |= TX_UPD_STRATA_EXPERIMENTAL;  // asked for by name
         int rc = tx_set_schedule(ctx, &s);
         if (rc == TX_EUNSUPPORTED && mode.empty() && !cfg.low_memory) {
             s.mode = TX_SCHED_SERIAL;  // day terms on ratings not sorted by date

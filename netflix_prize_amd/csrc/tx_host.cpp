@@ -1115,8 +1115,14 @@ int tx_destroy(tx_ctx* ctx) {
 int tx_set_schedule(tx_ctx* c, const tx_schedule* s) {
     g_err.clear();
     if (!c || !s) return fail(TX_EINVAL, "tx_set_schedule: NULL argument");
-    if (s->mode != TX_SCHED_SERIAL && s->mode != TX_SCHED_STRATA && s->mode != TX_SCHED_TSVDX4P)
+    if (s->mode != TX_SCHED_SERIAL && s->mode != TX_SCHED_STRATA && s->mode != TX_SCHED_TSVDX4P &&
+        s->mode != TX_SCHED_PLAIN)
         return fail(TX_EINVAL, "unknown schedule mode %d", s->mode);
+    if (s->mode == TX_SCHED_PLAIN) {
+        if (!c->cfg.low_memory || c->cfg.has_ordinal || c->shape.waves > 1)
+            return fail(TX_EUNSUPPORTED, "the plain schedule trains low_memory models without ordinal head, f <= 1024");
+        if (s->blocks < 0) return fail(TX_EINVAL, "blocks must be 0 (1024) or positive");
+    }
     if (s->mode == TX_SCHED_TSVDX4P) {
         if (!c->cfg.low_memory || c->cfg.has_ordinal)
             return fail(TX_EUNSUPPORTED, "the tsvdx4p schedule trains low_memory models without ordinal head");
@@ -1143,9 +1149,13 @@ int tx_set_schedule(tx_ctx* c, const tx_schedule* s) {
         return fail(TX_EINVAL, "teams_per_block must be 0 (4) or 1..4, got %d", s->teams_per_block);
     if (s->heavy_min < 0) return fail(TX_EINVAL, "heavy_min must be >= 0");
     const int qm = s->update_flags & 3, ym = (s->update_flags >> 2) & 3;
-    if (qm > 2 || ym > 2 || (s->update_flags >> 4) != 0) return fail(TX_EINVAL, "bad update_flags %d", s->update_flags);
+    if (qm > 2 || ym > 2 || (s->update_flags >> 5) != 0) return fail(TX_EINVAL, "bad update_flags %d", s->update_flags);
+    if (s->mode == TX_SCHED_STRATA && s->n_strata > 1 && !(s->update_flags & TX_UPD_STRATA_EXPERIMENTAL))
+        return fail(TX_EUNSUPPORTED,
+                    "n_strata > 1 applies every NSVD1 step at the last stratum and diverges at the full Netflix "
+                    "shape; set TX_UPD_STRATA_EXPERIMENTAL in update_flags to run it anyway");
     const int want_S = s->mode == TX_SCHED_TSVDX4P ? 1 : s->n_strata;
-    const bool relayout = s->mode != TX_SCHED_SERIAL && want_S != c->built_S;
+    const bool relayout = s->mode != TX_SCHED_SERIAL && s->mode != TX_SCHED_PLAIN && want_S != c->built_S;
     c->sched = *s;
     if (s->mode == TX_SCHED_TSVDX4P) c->sched.n_strata = 1;
     if (relayout) {
@@ -1185,6 +1195,26 @@ int tx_fit_epoch(tx_ctx* c, int32_t epoch) {
         HIPC(hipEventElapsedTime(&t, c->ev[0], c->ev[3]));
         stats.ms_total = t;
         stats.ms_other = t - stats.ms_sgd - stats.ms_rebuild;
+    } else if (c->sched.mode == TX_SCHED_PLAIN) {
+        std::vector<int64_t> order(c->U);
+        tx::user_order(c->U, c->cfg.shuffle_users != 0, c->cfg.seed, epoch, order.data());
+        HIPC(hipMemcpyAsync(c->d_order, order.data(), sizeof(int64_t) * c->U, hipMemcpyHostToDevice, st));
+        const bool qf = c->cfg.lr_iqf != 0.0f || c->cfg.sigma_iqf != 0.0f;
+        HIPC(hipEventRecord(c->ev[1], st));
+        HIPC(tx::launch_sgd_plain(c->dm, c->d_order, c->shape.vec, c->sched.blocks > 0 ? c->sched.blocks : 1024, qf, st));
+        HIPC(hipEventRecord(c->ev[2], st));
+        HIPC(tx::launch_rebuild(c->dm, c->n_ud, c->shape.team, c->shape.vec, st));
+        HIPC(hipEventRecord(c->ev[3], st));
+        HIPC(hipEventSynchronize(c->ev[3]));
+        float a = 0, b = 0, t = 0;
+        HIPC(hipEventElapsedTime(&a, c->ev[1], c->ev[2]));
+        HIPC(hipEventElapsedTime(&b, c->ev[2], c->ev[3]));
+        HIPC(hipEventElapsedTime(&t, c->ev[0], c->ev[3]));
+        stats.ms_sgd = a;
+        stats.ms_rebuild = b;
+        stats.ms_total = t;
+        stats.ms_other = t - a - b;
+        stats.n_launches = 2;
     } else if (c->sched.mode == TX_SCHED_SERIAL) {
         std::vector<int64_t> order(c->U);
         tx::user_order(c->U, c->cfg.shuffle_users != 0, c->cfg.seed, epoch, order.data());

@@ -363,6 +363,121 @@ __global__ __launch_bounds__(64) void k_sgd_serial(DevModel m, const int64_t* __
 }
 
 // ---------------------------------------------------------------------------
+// Plain schedule: the per-user wavefront kernel with nothing else.  Wave w of
+// the grid takes users order[w], order[w + waves], ... (the epoch's user order,
+// no queue); per user it sums the NSVD1 rows of N(u), then walks the ratings
+// in file order: loads the q row and the scalars the rating needs and waits for
+// them, a wave-reduced dot product, bias steps (user-owned in registers, item
+// biases by atomic add), p in registers, q by atomic add of its SGD delta; and
+// at the user's end the y step over R(u) (src/tx.rs:734-1115).  No prefetch,
+// no length buckets, no graph: the reference point of the batched kernel.
+// Low-memory models without ordinal head; lane l holds elements l, l + 64, ...
+// ---------------------------------------------------------------------------
+
+template <int VEC>
+__global__ __launch_bounds__(64) void k_sgd_plain(DevModel m, const int64_t* __restrict__ order, int qf_on) {
+    const int lane = threadIdx.x;
+    const int fs = m.fs, F = m.F, B = m.B;
+    const Hyper h = m.h;
+    // qf_on: the q_f rows (bin 0 in low-memory training) are not all zero
+    for (int64_t w = blockIdx.x; w < m.U; w += gridDim.x) {
+        const int u = static_cast<int>(order[w]);
+        const uint32_t ts = m.t_off[u], te = m.t_off[u + 1];
+        if (ts == te) continue;
+        const float norm = m.norm_train[u];
+        const size_t prow = static_cast<size_t>(u) * fs + lane;
+        float p[VEC], p2[VEC], s[VEC], G[VEC];
+        float sb = 0.0f;
+#pragma unroll
+        for (int c = 0; c < VEC; ++c) {
+            p[c] = m.ufeat[prow + c * 64];
+            p2[c] = m.ufeat2[prow + c * 64];
+            s[c] = 0.0f;
+            G[c] = 0.0f;
+        }
+        // NSVD1 sum over the training items and (full_su) the held-out items
+        const uint32_t hs = m.full_su ? m.h_start[u] : 0u, he = m.full_su ? m.h_start[u + 1] : 0u;
+        for (uint32_t r = ts; r < te + (he - hs); ++r) {
+            const uint32_t j = r < te ? m.t_item[r] : m.h_item[hs + (r - te)];
+            const float* y = m.yfeat + static_cast<size_t>(j) * fs + lane;
+#pragma unroll
+            for (int c = 0; c < VEC; ++c) s[c] += y[c * 64];
+            sb += m.ybias[j];
+        }
+        const float inv = 1.0f / norm;
+#pragma unroll
+        for (int c = 0; c < VEC; ++c) s[c] *= inv;
+        const float su_b = sb * inv;
+        float ubias = m.ubias[u], alpha = m.alpha_u[u], cu = m.cu[u], Gb = 0.0f;
+        for (uint32_t t = ts; t < te; ++t) {
+            const uint32_t i = m.t_item[t], ud = m.t_ud[t];
+            const int b = m.t_bin[t];
+            const float dev = m.t_dev[t], r = m.t_r[t];
+            const float* qrow = m.ifeat + static_cast<size_t>(i) * fs + lane;
+            const float* frow = m.ifeat_freq + static_cast<size_t>(i) * F * fs + lane;
+            float q[VEC], qf[VEC];
+            float part = 0.0f;
+#pragma unroll
+            for (int c = 0; c < VEC; ++c) {
+                q[c] = qrow[c * 64];
+                qf[c] = qf_on ? frow[c * 64] : 0.0f;
+                part += (q[c] + qf[c]) * (p[c] + s[c] + dev * p2[c]);
+            }
+            const float dot = team_sum<64>(part);
+            const float c_bud = m.ubias_day[ud];
+            const float c_but = m.but_bin[static_cast<size_t>(u) * B + b];
+            const float bi_t = m.ibias[i] + m.bit_bin[static_cast<size_t>(i) * B + b] + m.ibias_freq[static_cast<size_t>(i) * F];
+            const float cu_t = cu + m.cut[ud];
+            const float bu_t = ubias + c_but + alpha * dev + c_bud + su_b;
+            const float err = m.gbias + bu_t + bi_t * cu_t + dot - r;
+            const float err_b = err * h.w_bias, err_d = err * h.w_factor;
+            ubias -= h.lr_ub * err_b;
+            alpha -= h.lr_ta * err_b * dev;
+            if (lane == 0) {
+                m.ubias_day[ud] = c_bud - h.lr_ubd * err_b;
+                m.but_bin[static_cast<size_t>(u) * B + b] = c_but - h.lr_tu * err_b;
+                atomic_add_dev(&m.ibias[i], -h.lr_ib * (err_b * cu_t));
+                atomic_add_dev(&m.bit_bin[static_cast<size_t>(i) * B + b], -h.lr_ti * (err_b * cu_t));
+                atomic_add_dev(&m.ibias_freq[static_cast<size_t>(i) * F], -h.lr_ibf * (err_b * cu_t));
+            }
+            cu -= h.lr_cu * (err_b * bi_t + h.reg_cu * (cu - 1.0f));
+            Gb += m.sum_err_bug ? err_b * static_cast<float>(m.f) : err_b;
+#pragma unroll
+            for (int c = 0; c < VEC; ++c) {
+                const float qe = q[c] + qf[c];
+                G[c] += err_d * qe;
+                const float pe = p[c] + dev * p2[c] + s[c];
+                p[c] -= h.lr_u * (err_d * qe + h.reg_u * p[c]);
+                p2[c] -= h.lr_u2 * (err_d * qe * dev + h.reg_u2 * p2[c]);
+                atomic_add_dev(m.ifeat + static_cast<size_t>(i) * fs + lane + c * 64, -h.lr_i * (err_d * pe + h.reg_i * q[c]));
+                if (qf_on)
+                    atomic_add_dev(m.ifeat_freq + static_cast<size_t>(i) * F * fs + lane + c * 64,
+                                   -h.lr_iqf * (err_d * pe + h.reg_iqf * qf[c]));
+            }
+        }
+        // end of user: the y step over R(u) (src/tx.rs:1099-1106)
+        const float gb = h.lr_yb * (Gb * inv);
+        for (uint32_t t = ts; t < te; ++t) {
+            const uint32_t j = m.t_item[t];
+            float* y = m.yfeat + static_cast<size_t>(j) * fs + lane;
+#pragma unroll
+            for (int c = 0; c < VEC; ++c) y[c * 64] = y[c * 64] - h.lr_y * (G[c] * inv + h.reg_y * y[c * 64]);
+            if (lane == 0 && gb != 0.0f) m.ybias[j] = m.ybias[j] - gb;
+        }
+#pragma unroll
+        for (int c = 0; c < VEC; ++c) {
+            m.ufeat[prow + c * 64] = p[c];
+            m.ufeat2[prow + c * 64] = p2[c];
+        }
+        if (lane == 0) {
+            m.ubias[u] = ubias;
+            m.alpha_u[u] = alpha;
+            m.cu[u] = cu;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Parallel schedule.  One team of TEAM lanes owns one user segment; lane l
 // holds factor elements [l*VEC, l*VEC + VEC).  Users of a cell are pulled from
 // a per-(stratum, cell) queue in chunks; cell = block index mod n_cells, and
@@ -1811,6 +1926,19 @@ static hipError_t sgd_par_tv(const DevModel& m, const StrataView& sv, int stratu
     if (pass == 1) return sgd_par_tvp<TEAM, VEC, 1, false>(m, sv, stratum, blocks, qf, p2, ord, st);
     if (pass == 2) return sgd_par_tvp<TEAM, VEC, 2, false>(m, sv, stratum, blocks, qf, p2, ord, st);
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_sgd_plain(const DevModel& m, const int64_t* order, int vec, int blocks, bool qf, hipStream_t st) {
+    (void)hipGetLastError();
+    switch (vec) {
+        case 1: hipLaunchKernelGGL(k_sgd_plain<1>, dim3(blocks), dim3(64), 0, st, m, order, qf ? 1 : 0); break;
+        case 2: hipLaunchKernelGGL(k_sgd_plain<2>, dim3(blocks), dim3(64), 0, st, m, order, qf ? 1 : 0); break;
+        case 4: hipLaunchKernelGGL(k_sgd_plain<4>, dim3(blocks), dim3(64), 0, st, m, order, qf ? 1 : 0); break;
+        case 8: hipLaunchKernelGGL(k_sgd_plain<8>, dim3(blocks), dim3(64), 0, st, m, order, qf ? 1 : 0); break;
+        case 16: hipLaunchKernelGGL(k_sgd_plain<16>, dim3(blocks), dim3(64), 0, st, m, order, qf ? 1 : 0); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_sgd_par(const DevModel& m, const StrataView& sv, int stratum, int team, int vec, int blocks,

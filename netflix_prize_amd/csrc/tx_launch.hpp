@@ -10,6 +10,8 @@
 namespace tx {
 
 hipError_t launch_sgd_serial(const DevModel& m, const int64_t* order, float* scratch, hipStream_t st);
+// the plain per-user wavefront kernel: `blocks` one-wave blocks over the epoch's user order
+hipError_t launch_sgd_plain(const DevModel& m, const int64_t* order, int vec, int blocks, bool qf, hipStream_t st);
 // pass: 0 SGD epoch, 1 tsvdx4p user pass (NSVD1 accumulators), 2 user-only fold-in pass;
 // day: day terms (pass 0, one stratum, sv.day_acc allocated)
 hipError_t launch_sgd_par(const DevModel& m, const StrataView& sv, int stratum, int team, int vec, int blocks,

@@ -37,6 +37,7 @@ PARAM_ID = {n: k for k, n in enumerate(PARAMS)}
 SCHED_SERIAL = 0
 SCHED_STRATA = 1
 SCHED_TSVDX4P = 2  # Tsvdx4pModel epochs (src/tsvdx4p.rs:798-805)
+SCHED_PLAIN = 3    # the plain per-user wavefront kernel
 NOALLOC_MAX_ROWS = 1 << 20  # TX_NOALLOC_MAX_ROWS
 
 
@@ -396,11 +397,13 @@ class TxModel:
 
     def set_schedule(self, mode: int = SCHED_STRATA, n_strata: int = 1, blocks: int = 0, use_graph: bool = False,
                      teams_per_block: int = 0, q_update: int = 0, y_update: int = 2,
-                     user_order: int = 2, heavy_min: int = 1024) -> None:
+                     user_order: int = 2, heavy_min: int = 1024, experimental: bool = False) -> None:
+        """tx_set_schedule; experimental=True is needed for n_strata > 1
+        (TX_UPD_STRATA_EXPERIMENTAL: that schedule diverges at full shape)."""
         s = CSchedule()
         s.user_order = int(user_order)
         s.heavy_min = int(heavy_min)
-        s.update_flags = int(q_update) | (int(y_update) << 2)
+        s.update_flags = int(q_update) | (int(y_update) << 2) | (16 if experimental else 0)
         s.mode = int(mode)
         s.n_strata = int(n_strata)
         s.blocks = int(blocks)

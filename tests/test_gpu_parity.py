@@ -95,7 +95,7 @@ def test_parallel_schedule_holdout_rmse(gpu, n_strata):
     (within 0.001) on the 10k x 2k x 1M shape."""
     tr, pr = synthetic("small")
     cfg = lm_config(n_feat=32, n_epochs=5, n_time_bins=30)
-    m_o, m_g = _oracle_and_gpu(cfg, tr, pr, mode=tx.SCHED_STRATA, n_strata=n_strata)
+    m_o, m_g = _oracle_and_gpu(cfg, tr, pr, mode=tx.SCHED_STRATA, n_strata=n_strata, experimental=n_strata > 1)
     for e in range(1, cfg.n_epochs + 1):
         m_o.fit_epoch(e)
         m_g.fit_epoch(epoch=e)
@@ -244,7 +244,7 @@ def test_day_terms_schedule_rules(gpu):
     cfg = lm_config(n_feat=24, n_epochs=1, n_time_bins=30, low_memory=False)
     with tx.TxModel(tr, pr, cfg) as m:
         with pytest.raises(tx.TxError):
-            m.set_schedule(mode=tx.SCHED_STRATA, n_strata=8)
+            m.set_schedule(mode=tx.SCHED_STRATA, n_strata=8, experimental=True)
         m.set_schedule(mode=tx.SCHED_STRATA, n_strata=1)
     # one user's ratings out of date order: the default is the serial schedule,
     # and the parallel one is refused
@@ -318,3 +318,44 @@ def test_single_team_matches_serial_oracle(gpu, low_memory):
         err = float(np.max(np.abs(g.astype(np.float64) - o))) if o.size else 0.0
         print(f"{n}: movement {move:.3e} gpu-oracle {err:.3e}")
         assert err <= 1e-3 * move + 1e-6, n
+
+
+def test_plain_schedule_matches_serial_oracle(gpu):
+    """TX_SCHED_PLAIN, the plain per-user wavefront kernel: with one wave
+    (blocks = 1) it walks the users in the reference's order one after another,
+    so every table after two epochs (lm set with q_f and p2) is the serial
+    oracle's up to float association (within 1e-3 of each table's movement);
+    with 1,024 waves the 10k x 2k x 1M set trains to within 1e-3 of the
+    oracle's holdout RMSE."""
+    tr, pr = synthetic("tiny_synth")
+    cfg = lm_config(n_feat=24, n_epochs=2, n_time_bins=30, lr_iqf=0.005, lr_ibf=0.005, reset_u_epoch=100)
+    m_o = O.OracleModel(cfg, tr, pr)
+    init = {n: m_o.get(n).copy() for n in ALL_PARAMS}
+    with tx.TxModel(tr, pr, cfg) as m:
+        m.set_schedule(mode=tx.SCHED_PLAIN, blocks=1)
+        for e in (1, 2):
+            m.fit_epoch(epoch=e)
+            m_o.fit_epoch(e)
+        for n in ALL_PARAMS:
+            o, g = m_o.get(n), m.get(n)
+            move = float(np.max(np.abs(o.astype(np.float64) - init[n]))) if o.size else 0.0
+            err = float(np.max(np.abs(g.astype(np.float64) - o))) if o.size else 0.0
+            assert err <= 1e-3 * move + 1e-6, (n, err, move)
+    tr, pr = synthetic("small")
+    cfg = lm_config(n_feat=32, n_epochs=5, n_time_bins=30)
+    m_o, m_g = _oracle_and_gpu(cfg, tr, pr, mode=tx.SCHED_PLAIN, blocks=1024)
+    for e in range(1, cfg.n_epochs + 1):
+        m_o.fit_epoch(e)
+        m_g.fit_epoch(epoch=e)
+    r_o, r_g = m_o.calc_rmse(), m_g.calc_rmse(pr)
+    m_g.close()
+    print(f"plain 1024 waves: oracle {r_o:.6f} gpu {r_g:.6f}")
+    assert abs(r_g - r_o) <= 1e-3
+
+
+def test_multi_strata_needs_the_experimental_flag(gpu):
+    tr, pr = synthetic("tiny_synth")
+    with tx.TxModel(tr, pr, lm_config(n_feat=16, n_epochs=1)) as m:
+        with pytest.raises(tx.TxError, match="TX_UPD_STRATA_EXPERIMENTAL"):
+            m.set_schedule(mode=tx.SCHED_STRATA, n_strata=8)
+        m.set_schedule(mode=tx.SCHED_STRATA, n_strata=8, experimental=True)
