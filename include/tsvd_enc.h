@@ -59,13 +59,49 @@ int enc_adam(int dtype, float* w, const float* g, float* m, float* v, void* wcop
 /* y = x in dtype (round to nearest even for BF16). */
 int enc_cast(int dtype, const float* x, void* y, int64_t n, void* stream);
 
-/* Set pooling per user u < U over its rated items items[off[u] .. off[u+1])
- * (up to max_items of them, taken at evenly spaced positions):
- *   out[u*ldo + k]      = mean_j q[j*fs + k]
- *   out[u*ldo + fs + k] = mean_j w_j q[j*fs + k]       (k < fs)
- * with w_j the weight of that rating; users without items get zeros. */
-int enc_pool(const float* q, int fs, const uint32_t* off, const uint32_t* items, const float* w, int U, int max_items,
-             float* out, int ldo, void* stream);
+/* Columns of the per-user scalar block of enc_user_features. */
+#define ENC_SCALAR_COLS 64
+
+/* Per-item day statistics of n ratings (items[k] < n_items, days[k]):
+ * day_mean[j] and day_inv_std[j] = 1 / max(std, 1) of item j's rating days
+ * (population std, from exact 64-bit integer sums); both 0 for an item
+ * without ratings.  work: 3 * n_items uint64 of scratch. */
+int enc_item_day_stats(const uint32_t* items, const int16_t* days, int64_t n, int n_items, uint64_t* work,
+                       float* day_mean, float* day_inv_std, void* stream);
+
+/* Token features and set pooling per user u < U.  User u's training ratings
+ * are k = off[u] .. off[u+1]-1 (item items[k], rating class ratings[k] in
+ * 1..5, training target targets[k], day days[k]).  Each rating is a token of
+ *   e_k = q[items[k]] (BF16 bits, row pitch ldq; elements >= f zero),
+ *   o_k = one-hot of the rating class (all zero outside 1..5),
+ *   d_k = (days[k] - day_mean[j]) * day_inv_std[j]   (per-item normalised day),
+ *   w_k = targets[k] - mu - ibias[j]                   (residual vs the biases).
+ * Over S = up to max_items ratings at positions off[u] + floor(t cnt / |S|),
+ * t < |S| (cnt the user's count), row u of out (pitch ldo >= 7 f + 64) gets
+ *   [b f, b f + f)  mean_S o_k[b] e_k   for b = 0..4 (rating classes 1..5)
+ *   [5 f, 6 f)      mean_S d_k e_k
+ *   [6 f, 7 f)      mean_S w_k e_k
+ * and, over all cnt ratings, the scalars at 7 f + c:
+ *   c = 0 log1p(cnt) / 8;  1..5 share of rating class c;  6, 7 mean and std
+ *   of w;  8, 9 mean and std of d;  10 mean day / 2243;  11 (max - min day) /
+ *   2243;  12 min day / 2243;  13 max day / 2243;  14 the constant 1;
+ *   15..63 zero.
+ * A user without ratings gets zeros and the constant.  f <= 1024; ldq a
+ * multiple of V and >= 64 V, V the smallest power of two with 64 V >= f. */
+int enc_user_features(const uint16_t* q, int f, int ldq, const float* ibias, float mu, const float* day_mean,
+                      const float* day_inv_std, const uint32_t* off, const uint32_t* items, const int8_t* ratings,
+                      const float* targets, const int16_t* days, int U, int max_items, float* out, int64_t ldo,
+                      void* stream);
+
+/* out[r * ldo + c] = (x[rows[r] * ldx + c] - shift[c]) * scale[c] in dtype for
+ * r < n, c < d (rows NULL: rows[r] = r; shift / scale NULL: 0 / 1), and 0 in
+ * the padding n <= r < n_out or d <= c < ldo. */
+int enc_affine_rows(int dtype, const float* x, int64_t ldx, const int64_t* rows, int n, int d, const float* shift,
+                    const float* scale, void* out, int64_t ldo, int n_out, void* stream);
+
+/* Column mean and 1 / std (population std; 1 where std <= 1e-6) of x (M x N,
+ * pitch ld), with FP64 sums.  work: 512 * N doubles of scratch. */
+int enc_colstats(const float* x, int M, int N, int64_t ld, double* work, float* mean, float* inv_std, void* stream);
 
 const char* enc_last_error(void);
 

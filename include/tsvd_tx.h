@@ -279,6 +279,16 @@ int tx_param_len(tx_ctx* ctx, int32_t param_id, int64_t* n);
 int tx_get_param(tx_ctx* ctx, int32_t param_id, float* out, int64_t n);
 int tx_set_param(tx_ctx* ctx, int32_t param_id, const float* in, int64_t n);
 
+/* tx_set_param from GPU memory: `rows` rows of the table's column count,
+ * read from `in` with a row pitch of `ld` floats, after the work queued on
+ * `stream` (a hipStream_t, NULL: the default stream) has finished; returns
+ * when the copy is done.  No counterpart in the reference: it is for callers
+ * whose values are already on the device (the fold-in encoder's warm start of
+ * p_u and b_u).  gbias and the tables whose all-zero state lets the trainer
+ * skip work (cut, ibias_freq, but_bin, ybias) are refused (TX_EINVAL): set
+ * them with tx_set_param. */
+int tx_set_param_device(tx_ctx* ctx, int32_t param_id, const float* in, int64_t rows, int64_t ld, void* stream);
+
 /* get_users (src/lib.rs:486-494): the per-epoch user order. Host only. */
 int tx_get_users(int64_t n, int32_t shuffle, uint64_t seed, int64_t epoch, int64_t* out);
 

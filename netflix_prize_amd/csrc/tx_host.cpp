@@ -1741,4 +1741,32 @@ int tx_set_param(tx_ctx* c, int32_t id, const float* in, int64_t n) {
     return 0;
 }
 
+int tx_set_param_device(tx_ctx* c, int32_t id, const float* in, int64_t rows, int64_t ld, void* stream) {
+    g_err.clear();
+    if (!c || (rows > 0 && !in)) return fail(TX_EINVAL, "tx_set_param_device: NULL argument");
+    // tables whose all-zero state lets the trainer skip work are set from host memory only
+    if (id == TX_P_GBIAS || id == TX_P_CUT || id == TX_P_IBIAS_FREQ || id == TX_P_BUT_BIN || id == TX_P_YBIAS)
+        return fail(TX_EINVAL, "tx_set_param_device: parameter %d is set with tx_set_param", id);
+    const ParamDesc p = param_desc(c, id);
+    if (p.rows < 0) return fail(TX_EINVAL, "unknown parameter id %d", id);
+    if (rows != p.rows) return fail(TX_EINVAL, "parameter %d has %lld rows, caller passed %lld", id,
+                                    (long long)p.rows, (long long)rows);
+    if (ld < p.cols) return fail(TX_EINVAL, "tx_set_param_device: row pitch %lld < %lld columns", (long long)ld,
+                                 (long long)p.cols);
+    if (rows == 0) return 0;
+    c->bf16_stale = true;
+    if (id == TX_P_YFEAT || id == TX_P_YCACHE || id == TX_P_YCACHE_BIAS) c->ycache_current = false;
+    // the caller's producer work first, then every ctx stream's work, then the copy
+    HIPC(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+    HIPC(hipStreamSynchronize(c->st));
+    const size_t dpitch = (p.padded ? c->fs : p.cols) * sizeof(float);
+    HIPC(hipMemcpy2D(p.p, dpitch, in, ld * sizeof(float), p.cols * sizeof(float), p.rows, hipMemcpyDeviceToDevice));
+    if (c->graph) {
+        (void)hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+        c->graph_S = 0;
+    }
+    return 0;
+}
+
 }  // extern "C"

@@ -4,18 +4,25 @@ New users enter a trained tsvdx4 model through user-only SGD epochs with
 every item table frozen (``TxModel.fit_users_epoch``; the reference's closest
 analogue is the user pass of Tsvdx4pModel, src/tsvdx4p.rs:390-639).  The
 reference starts such users from random factors (src/tx.rs:460-649).  This
-encoder predicts a user's p_u (f values) and b_u from the items the user
-rated instead:
+encoder predicts a user's p_u (f values) and b_u from the ratings the user
+made instead.  Each rating k of item j is a token
 
-    x_u = [ mean_j q_j , mean_j w_uj q_j , time features ]      (2f + 64)
-    y_u = W2 gelu(W1 x_u + b1) + b2                             (p_u, b_u)
+    (q_j in BF16, one-hot of the rating class, d_k, w_k)
 
-where q_j are the model's item factors over up to 512 of the user's training
-ratings and w_uj = r_uj - mu - b_j the rating's residual against the model's
-biases.  The network is trained on the p_u, b_u that plain SGD produced for
-the training users.  All arithmetic runs in the kernels of libtsvd_enc.so
-(BF16 MFMA GEMMs with FP32 accumulation and FP32 master weights, or the FP32
-debug build); torch only provides device memory and the stream.
+with d_k the rating day normalised by item j's day statistics and
+w_k = r_k - mu - b_j the residual against the model's biases.  Mean set
+pooling over up to 512 sampled tokens gives
+
+    x_u = [ mean o_k[c] q_j (c = 1..5), mean d_k q_j, mean w_k q_j, scalars ]   (7f + 64)
+    y_u = W2 gelu(W1 x_u + b1) + b2                                            (p_u, b_u)
+
+(the scalars: count, class shares, mean/std of w and d, day statistics).
+The network is trained on the p_u, b_u that plain SGD produced for the
+training users.  Features, normalisation and the network run in the kernels
+of libtsvd_enc.so (BF16 MFMA GEMMs with FP32 accumulation and FP32 master
+weights, or the FP32 debug build) and stay on the device; the warm start
+writes p_u and b_u into the model from device memory.  torch only provides
+device memory and the stream.
 """
 from __future__ import annotations
 
@@ -36,9 +43,9 @@ ENC_LIB_PATH = os.path.join(PKG, "libtsvd_enc.so")
 
 ENC_BF16, ENC_F32 = 0, 1
 EPI_STORE, EPI_BIAS, EPI_BIAS_GELU, EPI_GELU_BWD = 0, 1, 2, 3
-N_TIME = 64          # time-feature columns (8 used, zero padded to a multiple of 64)
+N_SCALAR = 64        # per-user scalar columns (15 used, zero padded to a multiple of 64)
 MAX_ITEMS = 512      # rated items pooled per user
-DAYS = 2243.0        # day scale of the Netflix date range
+STAT_GROUPS = 256    # row groups of enc_colstats (its work buffer holds 2 x 256 x N doubles)
 
 
 class EncoderError(RuntimeError):
@@ -60,9 +67,13 @@ def lib() -> C.CDLL:
         L.enc_colsum.argtypes = [vp, i32, i32, i32, vp, vp]
         L.enc_adam.argtypes = [i32, vp, vp, vp, vp, vp, i64, f32, f32, f32, f32, C.c_int32, vp]
         L.enc_cast.argtypes = [i32, vp, vp, i64, vp]
-        L.enc_pool.argtypes = [vp, i32, vp, vp, vp, i32, i32, vp, i32, vp]
+        L.enc_item_day_stats.argtypes = [vp, vp, i64, i32, vp, vp, vp, vp]
+        L.enc_user_features.argtypes = [vp, i32, i32, vp, f32, vp, vp, vp, vp, vp, vp, vp, i32, i32, vp, i64, vp]
+        L.enc_affine_rows.argtypes = [i32, vp, i64, vp, i32, i32, vp, vp, vp, i64, i32, vp]
+        L.enc_colstats.argtypes = [vp, i32, i32, i64, vp, vp, vp, vp]
         L.enc_last_error.restype = C.c_char_p
-        for n in ("enc_gemm", "enc_loss_grad", "enc_colsum", "enc_adam", "enc_cast", "enc_pool"):
+        for n in ("enc_gemm", "enc_loss_grad", "enc_colsum", "enc_adam", "enc_cast", "enc_item_day_stats",
+                  "enc_user_features", "enc_affine_rows", "enc_colstats"):
             getattr(L, n).restype = C.c_int
         _lib = L
     return _lib
@@ -102,56 +113,69 @@ def user_offsets(ds) -> np.ndarray:
     return off
 
 
-def time_features(ds, w: np.ndarray) -> np.ndarray:
-    """Per-user count, residual mean/spread and date statistics (users sorted)."""
-    U = ds.n_users
-    off = user_offsets(ds)
-    cnt = np.diff(off).astype(np.float64)
-    has = cnt > 0
-    starts = np.minimum(off[:-1], max(len(w) - 1, 0))
-    days = ds.dates.astype(np.float64)
-
-    def seg_sum(x):
-        s = np.add.reduceat(x, starts) if len(x) else np.zeros(U)
-        return np.where(has, s, 0.0)
-
-    n = np.maximum(cnt, 1.0)
-    mw = seg_sum(w.astype(np.float64)) / n
-    sw = np.sqrt(np.maximum(seg_sum(w.astype(np.float64) ** 2) / n - mw * mw, 0.0))
-    md = seg_sum(days) / n
-    dmax = np.where(has, np.maximum.reduceat(days, starts) if len(days) else 0, 0.0)
-    dmin = np.where(has, np.minimum.reduceat(days, starts) if len(days) else 0, 0.0)
-    out = np.zeros((U, N_TIME), np.float32)
-    out[:, 0] = np.log1p(cnt) / 8.0
-    out[:, 1] = mw
-    out[:, 2] = sw
-    out[:, 3] = md / DAYS
-    out[:, 4] = (dmax - dmin) / DAYS
-    out[:, 5] = dmin / DAYS
-    out[:, 6] = dmax / DAYS
-    out[:, 7] = 1.0
-    return out
+def n_features(f: int) -> int:
+    """Columns of x_u for n_feat = f."""
+    return 7 * f + N_SCALAR
 
 
-def user_features(model: "tx.TxModel", ds, max_items: int = MAX_ITEMS) -> np.ndarray:
-    """x_u for every user of ds (training ratings) against model's item tables."""
+def _up(a, dtype=None):
+    """Host array -> device tensor (the features' inputs)."""
     torch = _torch()
-    f = model.cfg.n_feat
-    q = np.ascontiguousarray(model.get("ifeat").reshape(-1, f), np.float32)
-    ib = model.get("ibias")
-    mu = float(model.get("gbias")[0])
-    w = (ds.residuals - mu - ib[ds.item_idxs]).astype(np.float32)
+    a = np.ascontiguousarray(a if dtype is None else np.asarray(a, dtype))
+    return torch.from_numpy(a).to(torch.device("cuda:0"))
+
+
+class ItemSide:
+    """The item side the features read, on the device: the model's q as BF16
+    (rows padded to the kernel's pitch), item biases, mu, and each item's day
+    mean and 1 / max(std, 1) over the model's training ratings."""
+
+    def __init__(self, model: "tx.TxModel", train_ds):
+        torch = _torch()
+        f = model.cfg.n_feat
+        vec = 1
+        while 64 * vec < f:
+            vec *= 2
+        self.f, self.ldq, self.n_items = f, 64 * vec, train_ds.n_items
+        dev = torch.device("cuda:0")
+        qp = np.zeros((self.n_items, self.ldq), np.float32)
+        qp[:, :f] = model.get("ifeat").reshape(-1, f)
+        qf = _up(qp)
+        self.q = torch.empty((self.n_items, self.ldq), dtype=torch.int16, device=dev)
+        _check(lib().enc_cast(ENC_BF16, _p(qf), _p(self.q), qf.numel(), _stream()), "enc_cast q")
+        self.ibias = _up(model.get("ibias"), np.float32)
+        self.mu = float(model.get("gbias")[0])
+        items, days = _up(train_ds.item_idxs, np.int32), _up(train_ds.dates, np.int16)
+        if train_ds.n_ratings and int(train_ds.item_idxs.max()) >= self.n_items:
+            raise EncoderError("item index out of range")
+        work = torch.empty(3 * self.n_items, dtype=torch.int64, device=dev)
+        self.day_mean = torch.empty(self.n_items, dtype=torch.float32, device=dev)
+        self.day_inv = torch.empty(self.n_items, dtype=torch.float32, device=dev)
+        _check(lib().enc_item_day_stats(_p(items), _p(days), train_ds.n_ratings, self.n_items, _p(work),
+                                        _p(self.day_mean), _p(self.day_inv), _stream()), "enc_item_day_stats")
+        _torch().cuda.current_stream().synchronize()
+
+
+def user_features(side: ItemSide, ds, max_items: int = MAX_ITEMS):
+    """x_u for every user of ds (its training ratings) as a device FP32
+    tensor of n_features(f) columns."""
+    torch = _torch()
     off = user_offsets(ds)
-    dev = torch.device("cuda:0")
-    d_q = torch.from_numpy(q).to(dev)
-    d_off = torch.from_numpy(off.astype(np.uint32).view(np.int32)).to(dev)
-    d_items = torch.from_numpy(np.ascontiguousarray(ds.item_idxs, np.int32)).to(dev)
-    d_w = torch.from_numpy(w).to(dev)
-    out = torch.empty((ds.n_users, 2 * f), dtype=torch.float32, device=dev)
-    _check(lib().enc_pool(_p(d_q), f, _p(d_off), _p(d_items), _p(d_w), ds.n_users, max_items, _p(out), 2 * f,
-                          _stream()), "enc_pool")
-    pooled = out.cpu().numpy()
-    return np.concatenate([pooled, time_features(ds, w)], axis=1)
+    if off[-1] >= 2 ** 32:
+        raise EncoderError("more than 2^32 ratings")
+    d_off = _up(off.astype(np.uint32).view(np.int32))
+    d_items = _up(ds.item_idxs, np.int32)
+    d_r = _up(ds.raw_ratings, np.int8)
+    d_t = _up(ds.residuals, np.float32)
+    d_days = _up(ds.dates, np.int16)
+    if ds.n_ratings and int(ds.item_idxs.max()) >= side.n_items:
+        raise EncoderError("item index out of range")
+    n = n_features(side.f)
+    out = torch.empty((ds.n_users, n), dtype=torch.float32, device=torch.device("cuda:0"))
+    _check(lib().enc_user_features(_p(side.q), side.f, side.ldq, _p(side.ibias), side.mu, _p(side.day_mean),
+                                   _p(side.day_inv), _p(d_off), _p(d_items), _p(d_r), _p(d_t), _p(d_days),
+                                   ds.n_users, max_items, _p(out), n, _stream()), "enc_user_features")
+    return out
 
 
 # --------------------------------------------------------------------------
@@ -192,7 +216,8 @@ class FoldInEncoder:
         z = lambda a: torch.zeros_like(a)
         self.opt = [(z(a), z(a)) for a in self.masters]
         self.step_t = 0
-        self.x_mean = self.x_std = self.t_mean = self.t_std = None
+        self.x_mean_d = self.x_inv_d = self.y_shift_d = self.y_scale_d = self.x_cols = None
+        self.t_mean = self.t_std = None
         self._bufs = {}
 
     # weights -----------------------------------------------------------------
@@ -288,23 +313,60 @@ class FoldInEncoder:
                               self.step_t, st), "enc_adam")
 
     # training / inference on host arrays -------------------------------------
-    def fit(self, X: np.ndarray, Tg: np.ndarray, log=None) -> list:
-        """Train on rows X (n x d) -> targets Tg (n x n_out); returns the mean
-        loss per epoch (normalised-target units)."""
+    def _dev_f32(self, A):
+        """A host array or device tensor -> contiguous device FP32 matrix."""
+        torch = _torch()
+        if isinstance(A, torch.Tensor):
+            if A.device.type != "cuda" or A.dtype != torch.float32 or A.dim() != 2:
+                raise EncoderError("expected a 2-D FP32 device tensor")
+            return A.contiguous()
+        return torch.from_numpy(np.ascontiguousarray(A, np.float32)).to(self.dev)
+
+    def _colstats(self, A):
+        """Column mean and 1 / std (1 where std <= 1e-6) of a device matrix."""
+        torch = _torch()
+        M, N = A.shape
+        work = torch.empty(2 * STAT_GROUPS * N, dtype=torch.float64, device=self.dev)
+        mean = torch.empty(N, dtype=torch.float32, device=self.dev)
+        inv = torch.empty(N, dtype=torch.float32, device=self.dev)
+        _check(lib().enc_colstats(_p(A), M, N, N, _p(work), _p(mean), _p(inv), _stream()), "enc_colstats")
+        return mean, inv
+
+    def _affine(self, dt, x, ldx, rows, n, d, shift, scale, out, ldo, n_out) -> None:
+        _check(lib().enc_affine_rows(dt, x, ldx, None if rows is None else _p(rows), n, d,
+                                     None if shift is None else _p(shift), None if scale is None else _p(scale),
+                                     out, ldo, n_out, _stream()), "enc_affine_rows")
+
+    # training / inference ----------------------------------------------------
+    def fit(self, X, Tg, log=None) -> list:
+        """Train on rows X (n x d) -> targets Tg (n x n_out), host arrays or
+        device FP32 tensors; returns the mean loss per epoch
+        (normalised-target units).  Inputs and targets are normalised per
+        column on the device."""
         torch = _torch()
         c = self.cfg
-        self.x_mean = X.mean(0)
-        self.x_std = np.where(X.std(0) > 1e-6, X.std(0), 1.0)
-        self.t_mean = Tg.mean(0)
-        self.t_std = np.where(Tg.std(0) > 1e-6, Tg.std(0), 1.0)
+        Xd, Td = self._dev_f32(X), self._dev_f32(Tg)
+        n, dx = Xd.shape
+        if dx > self.d_in or Td.shape[1] != self.n_out or Td.shape[0] != n:
+            raise EncoderError("fit: shapes do not match the network")
+        self.x_mean_d, self.x_inv_d = self._colstats(Xd)
+        t_mean_d, t_inv_d = self._colstats(Td)
+        self.x_cols = dx
+        # de-normalisation y * std + mean as (y - shift) * scale
+        t_mean, t_inv = t_mean_d.cpu().numpy(), t_inv_d.cpu().numpy()
+        t_std = (1.0 / t_inv).astype(np.float32)
+        self.y_shift_d = torch.from_numpy((-t_mean * t_inv).astype(np.float32)).to(self.dev)
+        self.y_scale_d = torch.from_numpy(t_std).to(self.dev)
+        self.t_mean, self.t_std = t_mean, t_std
         rng = np.random.default_rng(c.seed + 1)
-        perm = rng.permutation(X.shape[0])
-        nb = X.shape[0] // c.batch
-        perm = perm[:nb * c.batch]
-        Xd = self.to_device_x(((X[perm] - self.x_mean) / self.x_std).astype(np.float32))
-        Tp = np.zeros((len(perm), self.d_out), np.float32)
-        Tp[:, :self.n_out] = (Tg[perm] - self.t_mean) / self.t_std
-        Td = torch.from_numpy(Tp).to(self.dev)
+        nb = n // c.batch
+        perm = torch.from_numpy(rng.permutation(n)[:nb * c.batch].astype(np.int64)).to(self.dev)
+        rows = nb * c.batch
+        Xn = torch.empty((rows, self.d_in), dtype=self.tdt, device=self.dev)
+        self._affine(self.dt, _p(Xd), dx, perm, rows, dx, self.x_mean_d, self.x_inv_d, _p(Xn), self.d_in, rows)
+        Tn = torch.empty((rows, self.d_out), dtype=torch.float32, device=self.dev)
+        self._affine(ENC_F32, _p(Td), self.n_out, perm, rows, self.n_out, t_mean_d, t_inv_d, _p(Tn), self.d_out,
+                     rows)
         colw = np.zeros(self.d_out, np.float32)
         colw[:self.n_out] = 1.0
         colw[self.n_out - 1] = c.bias_weight
@@ -314,25 +376,33 @@ class FoldInEncoder:
             tot = torch.zeros(1, dtype=torch.float64, device=self.dev)
             for b in rng.permutation(nb):
                 sl = slice(int(b) * c.batch, (int(b) + 1) * c.batch)
-                self.adam_step(self.gradients(Xd[sl], Td[sl], colw_d, tot))
+                self.adam_step(self.gradients(Xn[sl], Tn[sl], colw_d, tot))
             hist.append(float(tot.item()) / max(nb, 1) / self.n_out)
             if log:
                 log(f"encoder epoch {ep + 1}: loss {hist[-1]:.4f}")
         return hist
 
-    def predict(self, X: np.ndarray) -> np.ndarray:
-        """Outputs (n x n_out) in target units for host rows X."""
-        n = X.shape[0]
-        out = np.zeros((n, self.n_out), np.float32)
-        bs = self.cfg.batch
-        for s in range(0, n, bs):
-            rows = X[s:s + bs]
-            m = rows.shape[0]
-            pad = np.zeros((round64(m), X.shape[1]), np.float32)
-            pad[:m] = (rows - self.x_mean) / self.x_std
-            _, _, Y = self.forward(self.to_device_x(pad))
-            out[s:s + m] = Y[:m, :self.n_out].cpu().numpy() * self.t_std + self.t_mean
+    def predict_device(self, X, batch: int = 65536):
+        """Outputs (n x n_out, target units) for rows X as a device FP32 tensor."""
+        torch = _torch()
+        Xd = self._dev_f32(X)
+        n, dx = Xd.shape
+        if dx != self.x_cols:
+            raise EncoderError(f"predict: {dx} input columns, the network was fitted on {self.x_cols}")
+        out = torch.empty((n, self.n_out), dtype=torch.float32, device=self.dev)
+        for s in range(0, n, batch):
+            m = min(batch, n - s)
+            xb = self._buf("px", (round64(m), self.d_in), self.tdt)
+            self._affine(self.dt, _p(Xd) + 4 * s * dx, dx, None, m, dx, self.x_mean_d, self.x_inv_d, _p(xb),
+                         self.d_in, round64(m))
+            _, _, Y = self.forward(xb)
+            self._affine(ENC_F32, _p(Y), self.d_out, None, m, self.n_out, self.y_shift_d, self.y_scale_d,
+                         _p(out) + 4 * s * self.n_out, self.n_out, m)
         return out
+
+    def predict(self, X) -> np.ndarray:
+        """Outputs (n x n_out) in target units, as a host array."""
+        return self.predict_device(X).cpu().numpy()
 
 
 # --------------------------------------------------------------------------
@@ -382,14 +452,15 @@ def sgd_targets(model: "tx.TxModel") -> np.ndarray:
     return np.concatenate([model.get("ufeat").reshape(-1, f), model.get("ubias")[:, None]], axis=1)
 
 
-def warm_start(m: "tx.TxModel", enc: FoldInEncoder, X: np.ndarray) -> float:
-    """Write the encoder's p_u, b_u into m; returns the seconds it took."""
+def warm_start(m: "tx.TxModel", enc: FoldInEncoder, X) -> float:
+    """Write the encoder's p_u, b_u for rows X (device features) into m from
+    device memory; returns the seconds it took."""
     t0 = time.perf_counter()
-    Y = enc.predict(X)
-    f = Y.shape[1] - 1
-    m.set("ufeat", Y[:, :f])
-    m.set("ubias", Y[:, f])
-    m.rebuild_ycache()
+    Y = enc.predict_device(X)
+    n, f = Y.shape[0], enc.n_out - 1
+    st = _stream()
+    m.set_device("ufeat", _p(Y), n, enc.n_out, st)
+    m.set_device("ubias", _p(Y) + 4 * f, n, enc.n_out, st)
     return time.perf_counter() - t0
 
 
@@ -410,7 +481,8 @@ def evaluate_new_users(tr, pr, cfg, n_train: int, epochs: int, fold_epochs: int 
         base.fit_epoch(epoch=e)
         gpu_s += base.last_epoch_stats()["ms_total"] / 1000.0
     log(f"base model: {epochs} epochs on {n_train} users, {gpu_s:.2f} s GPU, holdout {base.calc_rmse(pr_a):.5f}")
-    Xa = user_features(base, tr_a)
+    side = ItemSide(base, tr_a)
+    Xa = user_features(side, tr_a)
     Ta = sgd_targets(base)
     enc = FoldInEncoder(Xa.shape[1], Ta.shape[1], enc_cfg)
     t1 = time.perf_counter()
@@ -422,7 +494,8 @@ def evaluate_new_users(tr, pr, cfg, n_train: int, epochs: int, fold_epochs: int 
     rand.close()
     warm = fold_in_model(base, tr_b, pr_b, cfg)
     t2 = time.perf_counter()
-    Xb = user_features(base, tr_b)
+    Xb = user_features(side, tr_b)
+    _torch().cuda.current_stream().synchronize()
     feat_s = time.perf_counter() - t2
     ws_s = warm_start(warm, enc, Xb)
     r_warm, s_warm = fold_in_curve(warm, pr_b, fold_epochs)
@@ -441,6 +514,7 @@ def evaluate_new_users(tr, pr, cfg, n_train: int, epochs: int, fold_epochs: int 
         "holdout_rmse_encoder_start": [round(x, 5) for x in r_warm],
         "fold_epoch_s": [round(x, 4) for x in s_rand[1:]],
         "encoder_start_s": round(feat_s + ws_s, 3),
+        "encoder_start_parts_s": {"features": round(feat_s, 4), "network_and_copy": round(ws_s, 4)},
         "random_start_epochs_to_encoder_rmse": k_eq,
         "random_start_s_to_encoder_rmse": None if k_eq is None else round(sum(s_rand[1:k_eq + 1]), 4),
         "wall_s": round(time.perf_counter() - t0, 1),

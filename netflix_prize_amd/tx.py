@@ -229,13 +229,14 @@ def lib() -> C.CDLL:
         L.tx_param_len.argtypes = [vp, i32, C.POINTER(i64)]
         L.tx_get_param.argtypes = [vp, i32, vp, i64]
         L.tx_set_param.argtypes = [vp, i32, vp, i64]
+        L.tx_set_param_device.argtypes = [vp, i32, vp, i64, i64, vp]
         L.tx_get_users.argtypes = [i64, i32, C.c_uint64, i64, vp]
         L.tx_rand_normal.argtypes = [C.c_uint64, C.c_float, i64, vp]
         for name in ["tx_create", "tx_destroy", "tx_set_schedule", "tx_fit_epoch", "tx_fit_users_epoch",
                      "tx_rebuild_ycache", "tx_set_table_precision", "tx_autotune",
                      "tx_last_epoch_stats",
                      "tx_predict", "tx_predict_device", "tx_predict_subscores", "tx_calc_rmse", "tx_param_len",
-                     "tx_get_param", "tx_set_param", "tx_get_users", "tx_rand_normal"]:
+                     "tx_get_param", "tx_set_param", "tx_set_param_device", "tx_get_users", "tx_rand_normal"]:
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -508,6 +509,12 @@ class TxModel:
     def set(self, name: str, value) -> None:
         v = np.ascontiguousarray(value, np.float32).ravel()
         _check(lib().tx_set_param(self._h, PARAM_ID[name], v.ctypes.data, len(v)), f"tx_set_param({name})")
+
+    def set_device(self, name: str, ptr: int, rows: int, ld: int, stream=None) -> None:
+        """tx_set_param_device: `rows` rows read from device address `ptr`
+        with a pitch of `ld` floats, after the work queued on `stream`."""
+        _check(lib().tx_set_param_device(self._h, PARAM_ID[name], ptr, rows, ld, stream),
+               f"tx_set_param_device({name})")
 
     def save_artifacts(self, model_name: str, tr_set: str, preds_dir: str) -> Optional[str]:
         """save_artifacts (src/tx.rs:726-732): the item factors as .ifeat."""

@@ -1,6 +1,7 @@
 """The fold-in encoder's FP64 oracle (oracle/encoder_ref.py), pinned on the
 CPU: gelu against closed-form values, backward() against central finite
-differences of loss()."""
+differences of loss(), the token features against hand-made users whose
+pooled blocks and scalars are written out by hand."""
 import numpy as np
 
 from oracle import encoder_ref as R
@@ -50,3 +51,74 @@ def test_loss_gradient_closed_form():
     val, dY = R.loss_and_grad_y(Y, T, colw)
     assert val == np.sum(colw * (Y - T) ** 2) / len(X)
     np.testing.assert_allclose(R.backward(X, params[2], Z, H, dY)[3], 2.0 / len(X) * (colw * (Y - T)).sum(0))
+
+
+# ---------------------------------------------------------------------------
+# features: hand-made users with the pooled blocks written out by hand
+# ---------------------------------------------------------------------------
+
+Q = np.array([[1.0, 0.0, 0.0], [0.0, 2.0, 0.0], [0.0, 0.0, 4.0], [1.0, 1.0, 1.0]])
+IBIAS = np.array([0.1, 0.2, 0.3, 0.4])
+MU = 3.0
+
+
+def _day_stats():
+    # item 0 on days 10, 20; item 1 on day 100; item 2 on days 0, 4, 8; item 3 never
+    return R.item_day_stats([0, 0, 1, 2, 2, 2], [10, 20, 100, 0, 4, 8], 4)
+
+
+def test_item_day_stats_closed_form():
+    mean, inv = _day_stats()
+    np.testing.assert_allclose(mean, [15.0, 100.0, 4.0, 0.0])
+    np.testing.assert_allclose(inv, [1 / 5.0, 1.0, 1 / np.sqrt(32 / 3), 0.0])  # std < 1 -> 1; no ratings -> 0
+
+
+def _users():
+    # user 0: (item 0, class 5, day 20), (item 1, class 1, day 100), (item 2, class 5, day 8)
+    # user 1: no ratings
+    # user 2: five ratings of classes 2, 3, 4, 2, 3 on items 3, 0, 1, 2, 3
+    off = np.array([0, 3, 3, 8])
+    items = np.array([0, 1, 2, 3, 0, 1, 2, 3])
+    ratings = np.array([5, 1, 5, 2, 3, 4, 2, 3])
+    targets = ratings.astype(np.float64)
+    days = np.array([20, 100, 8, 50, 60, 70, 80, 90])
+    return off, items, ratings, targets, days
+
+
+def test_user_features_closed_form():
+    mean, inv = _day_stats()
+    off, items, ratings, targets, days = _users()
+    X = R.user_features(Q, IBIAS, MU, mean, inv, off, items, ratings, targets, days, max_items=512)
+    f = 3
+    blk = lambda u, b: X[u, b * f:(b + 1) * f]
+    d = np.array([(20 - 15) / 5.0, 0.0, (8 - 4) / np.sqrt(32 / 3)])   # per-item normalised days
+    w = np.array([5 - 3 - 0.1, 1 - 3 - 0.2, 5 - 3 - 0.3])             # residuals against mu + b_i
+    np.testing.assert_allclose(blk(0, 0), [0, 2 / 3, 0])                 # class 1: q_1 / 3
+    for b in (1, 2, 3):
+        np.testing.assert_allclose(blk(0, b), 0.0)
+    np.testing.assert_allclose(blk(0, 4), [1 / 3, 0, 4 / 3])             # class 5: (q_0 + q_2) / 3
+    np.testing.assert_allclose(blk(0, 5), [d[0] / 3, 0, 4 * d[2] / 3])
+    np.testing.assert_allclose(blk(0, 6), [w[0] / 3, 2 * w[1] / 3, 4 * w[2] / 3])
+    sc = X[0, 7 * f:]
+    np.testing.assert_allclose(sc[:15], [np.log(4) / 8, 1 / 3, 0, 0, 0, 2 / 3, w.mean(), w.std(), d.mean(), d.std(),
+                                         128 / 3 / 2243, 92 / 2243, 8 / 2243, 100 / 2243, 1.0])
+    np.testing.assert_allclose(sc[15:], 0.0)
+    # no ratings: zeros and the constant column
+    expect = np.zeros(7 * f + 64)
+    expect[7 * f + 14] = 1.0
+    np.testing.assert_array_equal(X[1], expect)
+
+
+def test_user_features_sample_positions():
+    """Two of user 2's five ratings are pooled: positions floor(t * 5 / 2) = 0, 2
+    (item 3 in class 2, item 1 in class 4); the scalars still see all five."""
+    mean, inv = _day_stats()
+    off, items, ratings, targets, days = _users()
+    X = R.user_features(Q, IBIAS, MU, mean, inv, off, items, ratings, targets, days, max_items=2)
+    f = 3
+    np.testing.assert_allclose(X[2, 1 * f:2 * f], Q[3] / 2)
+    np.testing.assert_allclose(X[2, 2 * f:3 * f], 0.0)               # class 3 rated twice, never sampled
+    np.testing.assert_allclose(X[2, 3 * f:4 * f], Q[1] / 2)
+    np.testing.assert_allclose(X[2, 7 * f + 1:7 * f + 6], [0, 2 / 5, 2 / 5, 1 / 5, 0])
+    # item 3 has no day statistics (inv 0): its normalised day is 0
+    np.testing.assert_allclose(X[2, 5 * f:6 * f], ((70 - 100) * 1.0 * Q[1]) / 2)

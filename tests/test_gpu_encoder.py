@@ -129,15 +129,73 @@ def test_warm_start_beats_random_start(gpu):
     base = tx.TxModel(tr_a, pr_a, cfg)
     for e in range(1, 9):
         base.fit_epoch(epoch=e)
-    Xa = E.user_features(base, tr_a)
+    side = E.ItemSide(base, tr_a)
+    Xa = E.user_features(side, tr_a)
     Ta = E.sgd_targets(base)
     enc = E.FoldInEncoder(Xa.shape[1], Ta.shape[1], E.EncoderConfig(batch=512, epochs=30))
     enc.fit(Xa, Ta)
     rand = E.fold_in_model(base, tr_b, pr_b, cfg)
     r_rand, _ = E.fold_in_curve(rand, pr_b, 2)
     warm = E.fold_in_model(base, tr_b, pr_b, cfg)
-    E.warm_start(warm, enc, E.user_features(base, tr_b))
+    E.warm_start(warm, enc, E.user_features(side, tr_b))
     r_warm, _ = E.fold_in_curve(warm, pr_b, 2)
     print("random", [round(x, 5) for x in r_rand], "encoder", [round(x, 5) for x in r_warm])
     assert r_warm[0] < r_rand[0] - 0.002
     assert r_warm[1] < r_rand[1]
+
+
+def test_item_day_stats_match_oracle(gpu):
+    tr, _ = synthetic("tiny_synth")
+    cfg = lm_config(n_feat=8, n_time_bins=30)
+    m = tx.TxModel(tr, synthetic("tiny_synth")[1], cfg)
+    side = E.ItemSide(m, tr)
+    mean, inv = R.item_day_stats(tr.item_idxs, tr.dates, tr.n_items)
+    np.testing.assert_allclose(side.day_mean.cpu().numpy(), mean, rtol=1e-6)
+    np.testing.assert_allclose(side.day_inv.cpu().numpy(), inv, rtol=1e-6)
+    m.close()
+
+
+@pytest.mark.parametrize("f", [32, 60, 256])
+@pytest.mark.parametrize("max_items", [16, 512])
+def test_user_features_match_oracle(gpu, f, max_items):
+    """enc_user_features against the FP64 oracle on the 600 x 150 x 30k set
+    (users up to 150 ratings, so max_items = 16 samples most of them), with
+    the model's q rounded to BF16 as the kernel reads it and the device's
+    own per-item day statistics."""
+    tr, pr = synthetic("tiny_synth")
+    cfg = lm_config(n_feat=f, n_time_bins=30)
+    m = tx.TxModel(tr, pr, cfg)
+    m.fit_epoch(epoch=1)
+    side = E.ItemSide(m, tr)
+    X = E.user_features(side, tr, max_items).cpu().numpy()
+    q = bf16_round(m.get("ifeat").reshape(-1, f))
+    ref = R.user_features(q, m.get("ibias"), float(m.get("gbias")[0]), side.day_mean.cpu().numpy(),
+                          side.day_inv.cpu().numpy(), E.user_offsets(tr), tr.item_idxs, tr.raw_ratings, tr.residuals,
+                          tr.dates, max_items)
+    assert X.shape == ref.shape == (tr.n_users, E.n_features(f))
+    err = np.abs(X - ref)
+    print(f"f={f} max_items={max_items}: max |err| {err.max():.2e}")
+    np.testing.assert_allclose(X, ref, rtol=1e-4, atol=2e-5)
+    m.close()
+
+
+def test_set_param_device_round_trip(gpu):
+    """tx_set_param_device copies rows of a pitched device matrix into p_u and
+    b_u exactly; tables with a skip flag are refused."""
+    import torch
+    tr, pr = synthetic("tiny_synth")
+    f = 24
+    m = tx.TxModel(tr, pr, lm_config(n_feat=f, n_time_bins=30))
+    rng = np.random.default_rng(5)
+    ld = f + 7
+    A = rng.normal(size=(tr.n_users, ld)).astype(np.float32)
+    d = torch.from_numpy(A).cuda()
+    m.set_device("ufeat", d.data_ptr(), tr.n_users, ld)
+    m.set_device("ubias", d.data_ptr() + 4 * f, tr.n_users, ld)
+    np.testing.assert_array_equal(m.get("ufeat").reshape(-1, f), A[:, :f])
+    np.testing.assert_array_equal(m.get("ubias"), A[:, f])
+    with pytest.raises(tx.TxError):
+        m.set_device("ybias", d.data_ptr(), tr.n_items, ld)
+    with pytest.raises(tx.TxError):
+        m.set_device("ufeat", d.data_ptr(), tr.n_users - 1, ld)
+    m.close()
