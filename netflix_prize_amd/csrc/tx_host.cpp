@@ -276,8 +276,8 @@ int zeros(tx_ctx* c, float** dst, int64_t n) {
 
 struct ParamDesc {
     float* p;
-    int64_t rows, cols;  // cols == f means a padded factor table
-    bool padded;
+    int64_t rows, cols;
+    int64_t pitch;  // floats between device rows (cols, fs for factor tables, ibs for the item-bias rows)
 };
 
 ParamDesc param_desc(tx_ctx* c, int id) {
@@ -285,30 +285,30 @@ ParamDesc param_desc(tx_ctx* c, int id) {
     const int64_t U = c->U, I = c->I, D = c->n_ud, f = c->f;
     const bool lm = c->cfg.low_memory != 0;
     switch (id) {
-    case TX_P_UBIAS: return {m.ubias, U, 1, false};
-    case TX_P_IBIAS: return {m.ibias, I, 1, false};
-    case TX_P_UFEAT: return {m.ufeat, U, f, true};
-    case TX_P_UFEAT2: return {m.ufeat2, U, f, true};
-    case TX_P_IFEAT: return {m.ifeat, I, f, true};
-    case TX_P_YFEAT: return {m.yfeat, I, f, true};
-    case TX_P_YCACHE: return {m.ycache, U, f, true};
-    case TX_P_YFEAT_DAY: return {m.yfeat_day, lm ? 0 : I, f, true};
-    case TX_P_TU_MEAN: return {m.tu_mean, U, 1, false};
-    case TX_P_ALPHA_U: return {m.alpha_u, U, 1, false};
-    case TX_P_BUT_BIN: return {m.but_bin, U * c->B, 1, false};
-    case TX_P_BIT_BIN: return {m.bit_bin, I * c->B, 1, false};
-    case TX_P_YBIAS: return {m.ybias, I, 1, false};
-    case TX_P_YCACHE_BIAS: return {m.ycache_bias, U, 1, false};
-    case TX_P_UBIAS_DAY: return {m.ubias_day, D, 1, false};
-    case TX_P_UFEAT_DAY: return {m.ufeat_day, lm ? 0 : D, f, true};
-    case TX_P_YCACHE_DAY: return {m.ycache_day, lm ? 0 : D, f, true};
-    case TX_P_IBIAS_FREQ: return {m.ibias_freq, I * c->F, 1, false};
-    case TX_P_IFEAT_FREQ: return {m.ifeat_freq, I * c->F, f, true};
-    case TX_P_CU: return {m.cu, U, 1, false};
-    case TX_P_CUT: return {m.cut, D, 1, false};
-    case TX_P_GBIAS: return {nullptr, 1, 1, false};
-    case TX_P_THRESHOLDS: return {m.thresholds, 4, 1, false};
-    default: return {nullptr, -1, 0, false};
+    case TX_P_UBIAS: return {m.ubias, U, 1, 1};
+    case TX_P_IBIAS: return {m.ibias, I, 1, m.ibs};
+    case TX_P_UFEAT: return {m.ufeat, U, f, c->fs};
+    case TX_P_UFEAT2: return {m.ufeat2, U, f, c->fs};
+    case TX_P_IFEAT: return {m.ifeat, I, f, c->fs};
+    case TX_P_YFEAT: return {m.yfeat, I, f, c->fs};
+    case TX_P_YCACHE: return {m.ycache, U, f, c->fs};
+    case TX_P_YFEAT_DAY: return {m.yfeat_day, lm ? 0 : I, f, c->fs};
+    case TX_P_TU_MEAN: return {m.tu_mean, U, 1, 1};
+    case TX_P_ALPHA_U: return {m.alpha_u, U, 1, 1};
+    case TX_P_BUT_BIN: return {m.but_bin, U * c->B, 1, 1};
+    case TX_P_BIT_BIN: return {m.bit_bin, I, c->B, m.ibs};
+    case TX_P_YBIAS: return {m.ybias, I, 1, 1};
+    case TX_P_YCACHE_BIAS: return {m.ycache_bias, U, 1, 1};
+    case TX_P_UBIAS_DAY: return {m.ubias_day, D, 1, 1};
+    case TX_P_UFEAT_DAY: return {m.ufeat_day, lm ? 0 : D, f, c->fs};
+    case TX_P_YCACHE_DAY: return {m.ycache_day, lm ? 0 : D, f, c->fs};
+    case TX_P_IBIAS_FREQ: return {m.ibias_freq, I * c->F, 1, 1};
+    case TX_P_IFEAT_FREQ: return {m.ifeat_freq, I * c->F, f, c->fs};
+    case TX_P_CU: return {m.cu, U, 1, 1};
+    case TX_P_CUT: return {m.cut, D, 1, 1};
+    case TX_P_GBIAS: return {nullptr, 1, 1, 1};
+    case TX_P_THRESHOLDS: return {m.thresholds, 4, 1, 1};
+    default: return {nullptr, -1, 0, 0};
     }
 }
 
@@ -1001,7 +1001,10 @@ int tx_create(const tx_config* cfg, const tx_ratings* tr, const tx_ratings* pr, 
     m.gbias = gbias;
     m.h = make_hyper(*cfg);
     RC(zeros(c, &m.ubias, U));
-    RC(zeros(c, &m.ibias, I));
+    // b_i and b_i,bin(t) in one row per item (128-byte aligned rows up to 31 bins)
+    m.ibs = static_cast<int32_t>((1 + B + 31) / 32 * 32);
+    RC(zeros(c, &m.ibias, I * m.ibs));
+    m.bit_bin = m.ibias + 1;
     RC(upload_padded(c, &m.ufeat, h_ufeat.data(), U));
     RC(upload_padded(c, &m.ufeat2, nullptr, U));
     RC(upload_padded(c, &m.ifeat, h_ifeat.data(), I));
@@ -1011,7 +1014,6 @@ int tx_create(const tx_config* cfg, const tx_ratings* tr, const tx_ratings* pr, 
     RC(c->upload(&m.tu_mean, tu_mean.data(), U));
     RC(zeros(c, &m.alpha_u, U));
     RC(zeros(c, &m.but_bin, U * B));
-    RC(zeros(c, &m.bit_bin, I * B));
     RC(zeros(c, &m.ybias, I));
     RC(zeros(c, &m.ycache_bias, U));
     RC(zeros(c, &m.ubias_day, D));
@@ -1374,7 +1376,8 @@ int tx_autotune(tx_ctx* c, int32_t trial_epochs, tx_schedule* best_out) {
     for (int id = 0; id < TX_P_COUNT; ++id) {
         const ParamDesc pd = param_desc(c, id);
         if (!pd.p || pd.rows <= 0) continue;
-        const size_t n = static_cast<size_t>(pd.rows) * (pd.padded ? c->fs : pd.cols);
+        // from the first element to the last (the item-bias rows hold two tables)
+        const size_t n = static_cast<size_t>(pd.rows - 1) * pd.pitch + pd.cols;
         float* cp = nullptr;
         if (hipMalloc(&cp, n * sizeof(float)) != hipSuccess) {
             release();
@@ -1694,8 +1697,8 @@ int tx_get_param(tx_ctx* c, int32_t id, float* out, int64_t n) {
     }
     if (n == 0) return 0;
     HIPC(hipStreamSynchronize(c->st));
-    if (p.padded)
-        HIPC(hipMemcpy2D(out, p.cols * sizeof(float), p.p, c->fs * sizeof(float), p.cols * sizeof(float), p.rows,
+    if (p.pitch != p.cols)
+        HIPC(hipMemcpy2D(out, p.cols * sizeof(float), p.p, p.pitch * sizeof(float), p.cols * sizeof(float), p.rows,
                          hipMemcpyDeviceToHost));
     else
         HIPC(hipMemcpy(out, p.p, n * sizeof(float), hipMemcpyDeviceToHost));
@@ -1716,8 +1719,8 @@ int tx_set_param(tx_ctx* c, int32_t id, const float* in, int64_t n) {
     }
     if (n == 0) return 0;
     HIPC(hipStreamSynchronize(c->st));
-    if (p.padded)
-        HIPC(hipMemcpy2D(p.p, c->fs * sizeof(float), in, p.cols * sizeof(float), p.cols * sizeof(float), p.rows,
+    if (p.pitch != p.cols)
+        HIPC(hipMemcpy2D(p.p, p.pitch * sizeof(float), in, p.cols * sizeof(float), p.cols * sizeof(float), p.rows,
                          hipMemcpyHostToDevice));
     else
         HIPC(hipMemcpy(p.p, in, n * sizeof(float), hipMemcpyHostToDevice));
@@ -1759,7 +1762,7 @@ int tx_set_param_device(tx_ctx* c, int32_t id, const float* in, int64_t rows, in
     // the caller's producer work first, then every ctx stream's work, then the copy
     HIPC(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
     HIPC(hipStreamSynchronize(c->st));
-    const size_t dpitch = (p.padded ? c->fs : p.cols) * sizeof(float);
+    const size_t dpitch = p.pitch * sizeof(float);
     HIPC(hipMemcpy2D(p.p, dpitch, in, ld * sizeof(float), p.cols * sizeof(float), p.rows, hipMemcpyDeviceToDevice));
     if (c->graph) {
         (void)hipGraphExecDestroy(c->graph);

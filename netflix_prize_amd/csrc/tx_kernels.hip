@@ -256,7 +256,7 @@ __global__ __launch_bounds__(64) void k_sgd_serial(DevModel m, const int64_t* __
             const float bu_day = m.ubias_day[ud];
             const float bu_t = m.ubias[u] + m.but_bin[static_cast<size_t>(u) * B + b] + m.alpha_u[u] * dev + bu_day
                                + m.ycache_bias[u];
-            const float bi_t = m.ibias[i] + m.bit_bin[static_cast<size_t>(i) * B + b]
+            const float bi_t = m.ibias[static_cast<size_t>(i) * m.ibs] + m.bit_bin[static_cast<size_t>(i) * m.ibs + b]
                                + m.ibias_freq[static_cast<size_t>(i) * F + fb];
             const float cu_t = m.cu[u] + m.cut[ud];
             const size_t qrow = static_cast<size_t>(i) * fs;
@@ -303,8 +303,8 @@ __global__ __launch_bounds__(64) void k_sgd_serial(DevModel m, const int64_t* __
                 *bb = *bb - h.lr_tu * err_b;
                 m.alpha_u[u] = m.alpha_u[u] - h.lr_ta * err_b * dev;
                 m.ubias_day[ud] = m.ubias_day[ud] - h.lr_ubd * err_b;
-                m.ibias[i] = m.ibias[i] - h.lr_ib * (err_b * cu_t);
-                float* tb = &m.bit_bin[static_cast<size_t>(i) * B + b];
+                m.ibias[static_cast<size_t>(i) * m.ibs] = m.ibias[static_cast<size_t>(i) * m.ibs] - h.lr_ib * (err_b * cu_t);
+                float* tb = &m.bit_bin[static_cast<size_t>(i) * m.ibs + b];
                 *tb = *tb - h.lr_ti * (err_b * cu_t);
                 float* fbp = &m.ibias_freq[static_cast<size_t>(i) * F + fb];
                 *fbp = *fbp - h.lr_ibf * (err_b * cu_t);
@@ -426,7 +426,7 @@ __global__ __launch_bounds__(64) void k_sgd_plain(DevModel m, const int64_t* __r
             const float dot = team_sum<64>(part);
             const float c_bud = m.ubias_day[ud];
             const float c_but = m.but_bin[static_cast<size_t>(u) * B + b];
-            const float bi_t = m.ibias[i] + m.bit_bin[static_cast<size_t>(i) * B + b] + m.ibias_freq[static_cast<size_t>(i) * F];
+            const float bi_t = m.ibias[static_cast<size_t>(i) * m.ibs] + m.bit_bin[static_cast<size_t>(i) * m.ibs + b] + m.ibias_freq[static_cast<size_t>(i) * F];
             const float cu_t = cu + m.cut[ud];
             const float bu_t = ubias + c_but + alpha * dev + c_bud + su_b;
             const float err = m.gbias + bu_t + bi_t * cu_t + dot - r;
@@ -436,8 +436,8 @@ __global__ __launch_bounds__(64) void k_sgd_plain(DevModel m, const int64_t* __r
             if (lane == 0) {
                 m.ubias_day[ud] = c_bud - h.lr_ubd * err_b;
                 m.but_bin[static_cast<size_t>(u) * B + b] = c_but - h.lr_tu * err_b;
-                atomic_add_dev(&m.ibias[i], -h.lr_ib * (err_b * cu_t));
-                atomic_add_dev(&m.bit_bin[static_cast<size_t>(i) * B + b], -h.lr_ti * (err_b * cu_t));
+                atomic_add_dev(&m.ibias[static_cast<size_t>(i) * m.ibs], -h.lr_ib * (err_b * cu_t));
+                atomic_add_dev(&m.bit_bin[static_cast<size_t>(i) * m.ibs + b], -h.lr_ti * (err_b * cu_t));
                 atomic_add_dev(&m.ibias_freq[static_cast<size_t>(i) * F], -h.lr_ibf * (err_b * cu_t));
             }
             cu -= h.lr_cu * (err_b * bi_t + h.reg_cu * (cu - 1.0f));
@@ -774,7 +774,8 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_
     const bool l3 = lane == 3 && !m.skip_ibf;
     const bool sc_item = lane == 0 || lane == 1 || l3;
     float* const sc_base = lane == 1 ? m.bit_bin : lane == 2 ? m.ubias_day : l3 ? m.ibias_freq : m.ibias;
-    const uint32_t sc_imul = lane == 1 ? static_cast<uint32_t>(B) : lane == 2 ? 0u : l3 ? static_cast<uint32_t>(F) : 1u;
+    const uint32_t ibs = static_cast<uint32_t>(m.ibs);  // b_i and b_i,bin share one row per item
+    const uint32_t sc_imul = lane == 1 ? ibs : lane == 2 ? 0u : l3 ? static_cast<uint32_t>(F) : ibs;
     const uint32_t sc_bm = lane == 1 ? 0xffffffffu : 0u;
     const uint32_t sc_um = lane == 2 ? 0xffffffffu : 0u;
     const uint32_t sc_fm = (DAY && l3) ? 0xffffffffu : 0u;  // b_i,f(t): the day's freq bin
@@ -1481,8 +1482,8 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256) void k_item_pass(DevModel m, I
         const uint32_t r0 = iv.bi_start[i], r1 = iv.bi_start[i + 1];
         float q[VEC];
         load_vec<VEC>(m.ifeat + static_cast<size_t>(i) * fs + coff, q);
-        float ib = m.ibias[i];
-        for (int l = lane; l < B; l += TEAM) bt[l] = m.bit_bin[static_cast<size_t>(i) * B + l];
+        float ib = m.ibias[static_cast<size_t>(i) * m.ibs];
+        for (int l = lane; l < B; l += TEAM) bt[l] = m.bit_bin[static_cast<size_t>(i) * m.ibs + l];
         for (int l = lane; l < F; l += TEAM) bq[l] = m.ibias_freq[static_cast<size_t>(i) * F + l];
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -1613,12 +1614,12 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256) void k_item_pass(DevModel m, I
             }
         }
         store_vec<VEC>(m.ifeat + static_cast<size_t>(i) * fs + coff, q);
-        if (lane == 0 && lead) m.ibias[i] = ib;
+        if (lane == 0 && lead) m.ibias[static_cast<size_t>(i) * m.ibs] = ib;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         if (lead) {
-            for (int l = lane; l < B; l += TEAM) m.bit_bin[static_cast<size_t>(i) * B + l] = bt[l];
+            for (int l = lane; l < B; l += TEAM) m.bit_bin[static_cast<size_t>(i) * m.ibs + l] = bt[l];
             for (int l = lane; l < F; l += TEAM) m.ibias_freq[static_cast<size_t>(i) * F + l] = bq[l];
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1739,7 +1740,7 @@ __global__ __launch_bounds__(256) void k_predict(DevModel m, int64_t n, const in
         const int64_t ud = sud_find(m, u, static_cast<int16_t>(day));
         const int fb = ud >= 0 ? m.sud_fbin[ud] : 0;
         const float bu_day = ud >= 0 ? m.ubias_day[ud] : 0.0f;
-        const float bi_t = m.ibias[i] + m.bit_bin[static_cast<size_t>(i) * B + b] + m.ibias_freq[static_cast<size_t>(i) * F + fb];
+        const float bi_t = m.ibias[static_cast<size_t>(i) * m.ibs] + m.bit_bin[static_cast<size_t>(i) * m.ibs + b] + m.ibias_freq[static_cast<size_t>(i) * F + fb];
         const float cu_t = m.cu[u] + (ud >= 0 ? m.cut[ud] : 0.0f);
         const bool day_terms = !m.low_mem && ud >= 0;
         // partial sums over the row, a TEAM*VEC chunk at a time (one chunk up to f=1024)
@@ -1784,9 +1785,9 @@ __global__ __launch_bounds__(256) void k_predict(DevModel m, int64_t n, const in
             mf = team_sum<TEAM>(mf);
             ns = team_sum<TEAM>(ns);
             if (lane == 0) {
-                const float bias = m.gbias + m.ubias[u] + m.ibias[i] + m.ibias_freq[static_cast<size_t>(i) * F + fb]
+                const float bias = m.gbias + m.ubias[u] + m.ibias[static_cast<size_t>(i) * m.ibs] + m.ibias_freq[static_cast<size_t>(i) * F + fb]
                                    + m.but_bin[static_cast<size_t>(u) * B + b] + m.alpha_u[u] * dev + bu_day
-                                   + m.bit_bin[static_cast<size_t>(i) * B + b] + bi_t * (cu_t - 1.0f) + m.ycache_bias[u];
+                                   + m.bit_bin[static_cast<size_t>(i) * m.ibs + b] + bi_t * (cu_t - 1.0f) + m.ycache_bias[u];
                 out0[qn] = bias;
                 out1[qn] = mf;
                 out2[qn] = ns;

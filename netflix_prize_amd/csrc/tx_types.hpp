@@ -27,9 +27,13 @@ struct DevModel {
     float gbias;
     Hyper h;
 
-    // parameter tables (row stride fs for the factor tables)
+    // parameter tables (row stride fs for the factor tables).  b_i and its
+    // time bins share one row of ibs floats per item (ibias[i * ibs],
+    // bit_bin[i * ibs + b], bit_bin = ibias + 1; ibs a multiple of 32), so the
+    // two item-bias updates of a rating touch one 128-byte line.
     float *ubias, *ibias, *ufeat, *ufeat2, *ifeat, *yfeat, *ycache, *yfeat_day;
     float *tu_mean, *alpha_u, *but_bin, *bit_bin, *ybias, *ycache_bias;
+    int32_t ibs;
     float *ubias_day, *ufeat_day, *ycache_day, *ibias_freq, *ifeat_freq, *cu, *cut;
     float* thresholds;  // [4]
 
