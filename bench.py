@@ -294,7 +294,7 @@ def main(argv=None) -> int:
             F = model.get("ifeat").reshape(tr.n_items, cfg.n_feat)
         t0 = time.perf_counter()
         km = knn.KnnfModel(tr, F)
-        t_create = time.perf_counter() - t0
+        t_knn = time.perf_counter() - t0
         km.predict(pr.user_idxs, pr.item_idxs, pr.dates)
         t0 = time.perf_counter()
         km.predict(pr.user_idxs, pr.item_idxs, pr.dates)
@@ -303,27 +303,27 @@ def main(argv=None) -> int:
         ip, fp = (tr.n_items + 63) // 64 * 64, (F.shape[1] + 31) // 32 * 32
         knnf = {"workload": f"KnnfModel k=25 over {tr.n_items} x {F.shape[1]} item factors "
                             f"({'random' if args.knnf_f else 'trained'}), {pr.n_ratings} held-out queries",
-                "create_s": round(t_create, 4), "cosine_gflop": round(2.0 * ip * ip * fp / 1e9, 1),
+                "create_s": round(t_knn, 4), "cosine_gflop": round(2.0 * ip * ip * fp / 1e9, 1),
                 "predict_s": round(t_pred, 4), "queries_per_s": round(pr.n_ratings / t_pred, 1)}
         # KnnsModel (the *__knns jobs): support of every user's training and
         # held-out item pairs, then the same held-out queries
         t0 = time.perf_counter()
         km = knn.KnnsModel(tr, pr, knn.KnnsConfig())
-        t_create = time.perf_counter() - t0
+        t_knn = time.perf_counter() - t0
         km.predict(pr.user_idxs, pr.item_idxs, pr.dates)
         t0 = time.perf_counter()
         km.predict(pr.user_idxs, pr.item_idxs, pr.dates)
         t_pred = time.perf_counter() - t0
         km.close()
         nu = np.bincount(tr.user_idxs, minlength=tr.n_users) + np.bincount(pr.user_idxs, minlength=tr.n_users)
-        knnf["knns"] = {"create_s": round(t_create, 4), "support_pairs": int(np.sum(nu.astype(np.int64) ** 2)),
+        knnf["knns"] = {"create_s": round(t_knn, 4), "support_pairs": int(np.sum(nu.astype(np.int64) ** 2)),
                         "predict_s": round(t_pred, 4), "queries_per_s": round(pr.n_ratings / t_pred, 1)}
         # Knn3Model (the *__knn3 jobs, Knn3Config::default) on the training
         # set's residual targets: co-rating counts, candidates, the FP64 normal
         # equations and NNLS of every item, then the held-out queries
         t0 = time.perf_counter()
         km = knn.Knn3Model(tr, knn.Knn3Config())
-        t_create = time.perf_counter() - t0
+        t_knn = time.perf_counter() - t0
         km.predict(pr.user_idxs, pr.item_idxs)
         t0 = time.perf_counter()
         km.predict(pr.user_idxs, pr.item_idxs)
@@ -331,7 +331,7 @@ def main(argv=None) -> int:
         n_w = int(sum(len(x) for x in km.neighbors()))
         km.close()
         nt = np.bincount(tr.user_idxs, minlength=tr.n_users).astype(np.int64)
-        knnf["knn3"] = {"create_s": round(t_create, 4), "pair_lookups": int(np.sum(nt ** 2)), "weights": n_w,
+        knnf["knn3"] = {"create_s": round(t_knn, 4), "pair_lookups": int(np.sum(nt ** 2)), "weights": n_w,
                         "predict_s": round(t_pred, 4), "queries_per_s": round(pr.n_ratings / t_pred, 1)}
         log(f"bench: knnf {knnf}")
 

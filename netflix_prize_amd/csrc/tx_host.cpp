@@ -10,9 +10,11 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -78,6 +80,18 @@ void parallel_for(int64_t n, Fn fn) {
     }
     for (auto& t : th) t.join();
 }
+
+// Phase times of tx_create on stderr when TX_TIMING is set in the environment.
+struct PhaseClock {
+    bool on = std::getenv("TX_TIMING") != nullptr;
+    std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+    void mark(const char* what) {
+        if (!on) return;
+        const auto n = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "tx_create: %-30s %7.3f s\n", what, std::chrono::duration<double>(n - t).count());
+        t = n;
+    }
+};
 
 // freq_bin (src/tx.rs:87-90), evaluated with the host libm like the reference.
 int freq_bin_host(uint32_t cnt, int nfb) {
@@ -257,14 +271,52 @@ tx::Hyper make_hyper(const tx_config& c) {
     return h;
 }
 
+// Rows of a dense host table [rows, cols] to a device table whose rows are
+// `pitch` floats apart, and back.  A pitched host<->device copy goes row by
+// row (seconds for 480k rows), so the rows are re-pitched in a host buffer
+// and moved in one contiguous copy; columns beyond `cols` of each device row
+// (zero padding, or the other table sharing the item-bias rows) are read
+// first and written back unchanged, unless `fresh` says the device rows are
+// all zero.
+int rows_to_device(float* dst, int64_t pitch, const float* src, int64_t cols, int64_t rows, bool fresh = false) {
+    if (rows <= 0 || cols <= 0) return 0;
+    if (pitch == cols) {
+        HIPC(hipMemcpy(dst, src, static_cast<size_t>(rows * cols) * sizeof(float), hipMemcpyHostToDevice));
+        return 0;
+    }
+    const size_t span = static_cast<size_t>((rows - 1) * pitch + cols);
+    std::vector<float> st(span, 0.0f);
+    if (!fresh) HIPC(hipMemcpy(st.data(), dst, span * sizeof(float), hipMemcpyDeviceToHost));
+    parallel_for(rows, [&](int64_t lo, int64_t hi) {
+        for (int64_t r = lo; r < hi; ++r)
+            std::memcpy(st.data() + r * pitch, src + r * cols, static_cast<size_t>(cols) * sizeof(float));
+    });
+    HIPC(hipMemcpy(dst, st.data(), span * sizeof(float), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int rows_to_host(float* dst, const float* src, int64_t pitch, int64_t cols, int64_t rows) {
+    if (rows <= 0 || cols <= 0) return 0;
+    if (pitch == cols) {
+        HIPC(hipMemcpy(dst, src, static_cast<size_t>(rows * cols) * sizeof(float), hipMemcpyDeviceToHost));
+        return 0;
+    }
+    const size_t span = static_cast<size_t>((rows - 1) * pitch + cols);
+    std::vector<float> st(span);
+    HIPC(hipMemcpy(st.data(), src, span * sizeof(float), hipMemcpyDeviceToHost));
+    parallel_for(rows, [&](int64_t lo, int64_t hi) {
+        for (int64_t r = lo; r < hi; ++r)
+            std::memcpy(dst + r * cols, st.data() + r * pitch, static_cast<size_t>(cols) * sizeof(float));
+    });
+    return 0;
+}
+
 // Upload a dense [rows, f] host table into a zero-padded [rows, fs] device table.
 int upload_padded(tx_ctx* c, float** dst, const float* src, int64_t rows) {
     const int64_t alloc_rows = std::max<int64_t>(rows, 1);
     if (int rc = c->alloc(dst, static_cast<size_t>(alloc_rows) * c->fs)) return rc;
     HIPC(hipMemset(*dst, 0, alloc_rows * c->fs * sizeof(float)));
-    if (rows > 0 && c->f > 0 && src)
-        HIPC(hipMemcpy2D(*dst, c->fs * sizeof(float), src, c->f * sizeof(float), c->f * sizeof(float), rows,
-                         hipMemcpyHostToDevice));
+    if (rows > 0 && c->f > 0 && src) return rows_to_device(*dst, c->fs, src, c->f, rows, true);
     return 0;
 }
 
@@ -760,6 +812,7 @@ int tx_create(const tx_config* cfg, const tx_ratings* tr, const tx_ratings* pr, 
         return fail(TX_EINVAL, "tx_create: the ordinal head needs train raw_ratings");
     if (tr->n_ratings + pr->n_ratings >= (int64_t{1} << 32))
         return fail(TX_EUNSUPPORTED, "tx_create: more than 2^32 ratings");
+    PhaseClock clk;
     Shape shape;
     if (!pick_shape(cfg->n_feat, &shape)) return fail(TX_EUNSUPPORTED, "tx_create: n_feat > 8192");
 
@@ -795,6 +848,7 @@ int tx_create(const tx_config* cfg, const tx_ratings* tr, const tx_ratings* pr, 
         }
     }
 
+    clk.mark("checks and user offsets");
     // tu_mean and day_range (src/tx.rs:465-475): f32 sums in file order
     std::vector<float> tu_mean(U, 0.0f);
     int32_t day_range = 0;
@@ -806,6 +860,7 @@ int tx_create(const tx_config* cfg, const tx_ratings* tr, const tx_ratings* pr, 
         if (tr->user_cnts[u] > 0) tu_mean[u] /= static_cast<float>(tr->user_cnts[u]);
     if (day_range <= 0) day_range = 1;
 
+    clk.mark("tu_mean, day_range");
     // held-out rows grouped by user, in held-out file order
     c->h_start.assign(U + 1, 0);
     for (int64_t t = 0; t < NH; ++t) c->h_start[pr->user_idxs[t] + 1]++;
@@ -816,6 +871,7 @@ int tx_create(const tx_config* cfg, const tx_ratings* tr, const tx_ratings* pr, 
         for (int64_t t = 0; t < NH; ++t) h_item[pos[pr->user_idxs[t]]++] = static_cast<uint32_t>(pr->item_idxs[t]);
     }
 
+    clk.mark("held-out grouping");
     // SparseUD (src/tx.rs:31-69): per user the sorted distinct days of train
     // and held-out ratings, with their counts
     std::vector<uint32_t> sud_start(U + 1, 0);
@@ -864,6 +920,7 @@ int tx_create(const tx_config* cfg, const tx_ratings* tr, const tx_ratings* pr, 
         return (it != e && *it == day) ? (it - sud_date.begin()) : -1;
     };
 
+    clk.mark("SparseUD");
     // per training rating: item, SparseUD index, time bin, dev, target
     c->t_item.resize(N);
     c->t_ud.resize(N);
@@ -886,6 +943,7 @@ int tx_create(const tx_config* cfg, const tx_ratings* tr, const tx_ratings* pr, 
         }
     });
 
+    clk.mark("per-rating item/ud/bin/dev");
     std::vector<uint8_t>& sud_fbin = c->sud_fbin;
     sud_fbin.assign(D, 0);
     for (int64_t d = 0; d < D; ++d) sud_fbin[d] = static_cast<uint8_t>(freq_bin_host(sud_cnt[d], F));
@@ -899,6 +957,7 @@ int tx_create(const tx_config* cfg, const tx_ratings* tr, const tx_ratings* pr, 
     // day terms train in the parallel schedule when the ratings are date-ordered
     if (!cfg->low_memory && !c->days_sorted) c->sched.mode = TX_SCHED_SERIAL;
 
+    clk.mark("freq bins, date order");
     // norms: training su (src/tx.rs:808-819) and epoch-end rebuild (src/tx.rs:410-416)
     std::vector<float> norm_train(U), norm_rebuild(U);
     for (int64_t u = 0; u < U; ++u) {
@@ -914,6 +973,7 @@ int tx_create(const tx_config* cfg, const tx_ratings* tr, const tx_ratings* pr, 
     for (int64_t t = 0; t < N; ++t) rsum += static_cast<double>(tr->residuals[t]);
     const float gbias = N > 0 ? static_cast<float>(rsum / static_cast<double>(N)) : 0.0f;
 
+    clk.mark("norms, gbias");
     // day-term structures (non-low-memory): held-out / train items per
     // (user, day), same-day runs with their norms
     const bool low_mem = cfg->low_memory != 0;
@@ -970,6 +1030,7 @@ int tx_create(const tx_config* cfg, const tx_ratings* tr, const tx_ratings* pr, 
         }
     }
 
+    clk.mark("day-term structures");
     // initial factors in the reference's draw order (src/tx.rs:606-630):
     // ufeat, ifeat, yfeat, [yfeat_day], ifeat_freq
     tx::ChaChaRng rng(cfg->seed);
@@ -984,6 +1045,7 @@ int tx_create(const tx_config* cfg, const tx_ratings* tr, const tx_ratings* pr, 
     }
     tx::fill_normal(rng, h_iqf.data(), static_cast<int64_t>(h_iqf.size()), cfg->sigma_iqf);
 
+    clk.mark("initial draws");
     // ---- device ----
     HIPC(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
     for (auto& e : c->ev) HIPC(hipEventCreate(&e));
@@ -1028,6 +1090,7 @@ int tx_create(const tx_config* cfg, const tx_ratings* tr, const tx_ratings* pr, 
     RC(zeros(c, &m.cut, D));
     RC(c->upload(&m.thresholds, cfg->th_init, 4));
 
+    clk.mark("parameter tables to device");
     uint32_t* p32;
     RC(c->upload(&p32, sud_start.data(), sud_start.size())); m.sud_start = p32;
     int16_t* p16;
@@ -1052,6 +1115,7 @@ int tx_create(const tx_config* cfg, const tx_ratings* tr, const tx_ratings* pr, 
         });
         RC(c->upload(&c->d_rec, rec.data(), rec.size()));
     }
+    clk.mark("ratings and records to device");
     RC(c->upload(&p32, run_stop.data(), run_stop.size())); m.t_run_stop = p32;
     RC(c->upload(&pf, run_norm.data(), run_norm.size())); m.t_run_norm = pf;
     RC(c->upload(&p32, c->h_start.data(), c->h_start.size())); m.h_start = p32;
@@ -1104,6 +1168,7 @@ int tx_create(const tx_config* cfg, const tx_ratings* tr, const tx_ratings* pr, 
     HIPC(hipGetDevice(&c->device));
     HIPC(tx::noalloc_scratch(c->device, &c->d_noalloc));
     HIPC(hipDeviceSynchronize());
+    clk.mark("remaining uploads");
     *out = ctx.release();
     return 0;
 }
@@ -1697,11 +1762,7 @@ int tx_get_param(tx_ctx* c, int32_t id, float* out, int64_t n) {
     }
     if (n == 0) return 0;
     HIPC(hipStreamSynchronize(c->st));
-    if (p.pitch != p.cols)
-        HIPC(hipMemcpy2D(out, p.cols * sizeof(float), p.p, p.pitch * sizeof(float), p.cols * sizeof(float), p.rows,
-                         hipMemcpyDeviceToHost));
-    else
-        HIPC(hipMemcpy(out, p.p, n * sizeof(float), hipMemcpyDeviceToHost));
+    if (int rc = rows_to_host(out, p.p, p.pitch, p.cols, p.rows)) return rc;
     return 0;
 }
 
@@ -1719,11 +1780,7 @@ int tx_set_param(tx_ctx* c, int32_t id, const float* in, int64_t n) {
     }
     if (n == 0) return 0;
     HIPC(hipStreamSynchronize(c->st));
-    if (p.pitch != p.cols)
-        HIPC(hipMemcpy2D(p.p, p.pitch * sizeof(float), in, p.cols * sizeof(float), p.cols * sizeof(float), p.rows,
-                         hipMemcpyHostToDevice));
-    else
-        HIPC(hipMemcpy(p.p, in, n * sizeof(float), hipMemcpyHostToDevice));
+    if (int rc = rows_to_device(p.p, p.pitch, in, p.cols, p.rows)) return rc;
     // tables the parallel kernel may skip while they are all zero
     auto all_zero = [&]() {
         for (int64_t k = 0; k < n; ++k)

@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 namespace tx {
@@ -32,13 +33,27 @@ public:
         }
     }
 
+    ChaChaRng(const ChaChaRng&) = delete;
+    ChaChaRng& operator=(const ChaChaRng&) = delete;
+
     uint32_t next_u32() {
-        if (pos_ == kWords) refill();
-        return buf_[pos_++];
+        if (pos_ == len_) refill();
+        return cur_[pos_++];
     }
+
+    // Bulk mode: the keystream is produced kBulkBlocks blocks at a time by
+    // several threads (a block depends only on its counter) and served in
+    // stream order, so the words are those of the four-block buffer, only
+    // computed ahead.  Words already produced are served first either way.
+    void set_bulk(bool on) { bulk_ = on; }
 
     // BlockRng::next_u64: two consecutive words, low first.
     uint64_t next_u64() {
+        if (pos_ + 2 <= len_) {
+            const uint64_t v = cur_[pos_] | (static_cast<uint64_t>(cur_[pos_ + 1]) << 32);
+            pos_ += 2;
+            return v;
+        }
         const uint64_t lo = next_u32();
         const uint64_t hi = next_u32();
         return lo | (hi << 32);
@@ -49,9 +64,30 @@ private:
 
     static uint32_t rotl(uint32_t v, int n) { return (v << n) | (v >> (32 - n)); }
 
+    static constexpr int64_t kBulkBlocks = int64_t{1} << 18;  // 16 MB of keystream
+
     void refill() {
-        for (int b = 0; b < 4; ++b) block(ctr_ + static_cast<uint64_t>(b), buf_ + 16 * b);
-        ctr_ += 4;
+        if (bulk_) {
+            big_.resize(static_cast<size_t>(kBulkBlocks) * 16);
+            const uint64_t c0 = ctr_;
+            unsigned nt = std::thread::hardware_concurrency();
+            nt = nt < 1 ? 1 : (nt > 16 ? 16 : nt);
+            std::vector<std::thread> th;
+            for (unsigned t = 0; t < nt; ++t)
+                th.emplace_back([this, t, nt, c0]() {  // a contiguous range of blocks per thread
+                    const int64_t lo = kBulkBlocks * t / nt, hi = kBulkBlocks * (t + 1) / nt;
+                    for (int64_t b = lo; b < hi; ++b) block(c0 + static_cast<uint64_t>(b), big_.data() + 16 * b);
+                });
+            for (auto& x : th) x.join();
+            ctr_ += static_cast<uint64_t>(kBulkBlocks);
+            cur_ = big_.data();
+            len_ = static_cast<int64_t>(big_.size());
+        } else {
+            for (int b = 0; b < 4; ++b) block(ctr_ + static_cast<uint64_t>(b), buf_ + 16 * b);
+            ctr_ += 4;
+            cur_ = buf_;
+            len_ = kWords;
+        }
         pos_ = 0;
     }
 
@@ -76,8 +112,11 @@ private:
 
     uint32_t key_[8];
     uint32_t buf_[kWords];
+    std::vector<uint32_t> big_;
     uint64_t ctr_ = 0;
-    int pos_ = kWords;
+    const uint32_t* cur_ = buf_;  // buf_ or big_ (the object is not copyable)
+    int64_t pos_ = 0, len_ = 0;
+    bool bulk_ = false;
     int rounds_;
 };
 
@@ -146,10 +185,12 @@ inline double standard_normal(ChaChaRng& rng) {
 
 // rand_array2: Normal<f32>(0, sigma) = 0 + sigma * (f32)z, one draw per element.
 inline void fill_normal(ChaChaRng& rng, float* out, int64_t n, float sigma) {
+    rng.set_bulk(n >= (int64_t{1} << 20));
     for (int64_t k = 0; k < n; ++k) {
         const float z = static_cast<float>(standard_normal(rng));
         out[k] = 0.0f + sigma * z;
     }
+    rng.set_bulk(false);
 }
 
 // UniformInt<u32> single sample in [0, range) (rand 0.9's widening-multiply
