@@ -117,7 +117,7 @@ def lib():
         L.or_fit_users_epoch.argtypes = [vp, vp, vp, i64]
         L.or_rebuild_ycache.argtypes = [vp, vp, vp]
         L.or_predict.argtypes = [vp, i64, vp, vp, vp, vp]
-        L.or_predict_subscores.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp]
+        L.or_predict_subscores.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp]
         L.or_predict_f64.argtypes = [vp, i64, vp, vp, vp, vp]
         L.or_calc_rmse.restype = C.c_double
         L.or_calc_rmse.argtypes = [vp, vp]
@@ -239,10 +239,15 @@ class OracleModel:
     def predict_subscores(self, u, i, d):
         u, i, d = (np.ascontiguousarray(u, np.int32), np.ascontiguousarray(i, np.int32),
                    np.ascontiguousarray(d, np.int16))
-        outs = [np.empty(len(u), np.float32) for _ in range(3)]
+        outs = [np.empty(len(u), np.float32) for _ in range(4)]
         lib().or_predict_subscores(self._m, len(u), u.ctypes.data, i.ctypes.data, d.ctypes.data,
                                    *[o.ctypes.data for o in outs])
-        return outs
+        return outs if self.has_nbr else outs[:3]
+
+    @property
+    def has_nbr(self) -> bool:
+        """The neighbourhood terms are built (lr_w > 0 or lr_c > 0)."""
+        return (float(_get(self.cfg, "lr_w", 0.0) or 0.0) > 0.0) or (float(_get(self.cfg, "lr_c", 0.0) or 0.0) > 0.0)
 
     def calc_rmse(self, ds=None) -> float:
         d = self.pr if ds is None else (ds if isinstance(ds, OracleData) else OracleData(ds))

@@ -1,8 +1,9 @@
 /*
  * ORACLE — test infrastructure only.
  *
- * A plain-C, serial restatement of the reference's tsvdx4 timeSVD++ path
- * (TxModel in tx4 mode and the rayon variant Tsvdx4pModel), used by the
+ * A plain-C, serial restatement of the reference's timeSVD++ path (TxModel
+ * in tx4 mode, its tx5/tx6 neighbourhood terms, and the rayon variant
+ * Tsvdx4pModel), used by the
  * test-suite, smoke() and bench.py's cpu_baseline leg as the CHECKER of the
  * HIP implementation.  Nothing in the product (netflix_prize_amd/) links,
  * loads or calls this file.
@@ -20,6 +21,10 @@
  *   - tsvdx4p tiny-fixture probe RMSE == 1.02037806773939 exactly
  *     (tests/regressors_test.rs:157-175), which pins the user/item passes and
  *     apply_nsvd_deltas;
+ *   - tsvdx5 and tsvdx6 tiny-fixture probe RMSEs == 1.02023124170746 and
+ *     1.0199379853756045 exactly (tests/regressors_test.rs:95-155), which pin
+ *     the neighbourhood lists, the per-epoch neighbour sampling stream, the
+ *     global and same-day forward terms and steps, and nbr_score;
  *   - ChaCha20 all-zero key keystream (published test vector) pins the block
  *     function; the ziggurat table is regenerated from the published
  *     construction and checked against the published table prefix.
@@ -332,6 +337,20 @@ typedef struct {
     /* tsvdx4p ByItemIndex (src/tsvdx4p.rs:29-57) */
     int64_t* bi_starts;
     uint32_t *bi_user, *bi_rating;
+    /* neighbourhood terms (tx5/tx6 modes, src/tx.rs:255-262, 483-585):
+     * R(u) = the user's training ratings in file order with r - b~ per
+     * rating (rmb), N(u)\R(u) = held-out items not rated in training (CSR,
+     * sorted by item[, day], deduplicated), the two norms, and the dense
+     * item x item tables (NULL when not built) */
+    int has_nbr, has_wd, has_cd;
+    float* rmb;
+    uint16_t* r_item;
+    int16_t* r_day;
+    int64_t* imp_start;
+    uint16_t* imp_item;
+    int16_t* imp_day;
+    float *norm_nu, *norm_ru;
+    float *w, *c, *w_day, *c_day;
     char err[256];
 } or_model;
 
@@ -456,14 +475,177 @@ static float* zalloc(int64_t n) { return calloc((size_t)(n > 0 ? n : 1), sizeof(
 static int g_plant = 0;
 OR_API void or_set_plant(int k) { g_plant = k; }
 
+
+/* ------------------------------------------------------------------------ */
+/* Neighbourhood structures (src/tx.rs:483-585)                             */
+/* ------------------------------------------------------------------------ */
+
+typedef struct { uint16_t item; int16_t day; } or_imp;
+
+static int cmp_imp_item_day(const void* a, const void* b) {
+    const or_imp *x = a, *y = b;
+    if (x->item != y->item) return (x->item > y->item) - (x->item < y->item);
+    return (x->day > y->day) - (x->day < y->day);
+}
+
+static int cmp_imp_item(const void* a, const void* b) {
+    const or_imp *x = a, *y = b;
+    return (x->item > y->item) - (x->item < y->item);
+}
+
+static int cmp_u16(const void* a, const void* b) {
+    uint16_t x = *(const uint16_t*)a, y = *(const uint16_t*)b;
+    return (x > y) - (x < y);
+}
+
+static void nbr_build(or_model* m, const or_dataset* tr, const or_dataset* pr) {
+    const or_config* cfg = &m->cfg;
+    const int build_nbr = cfg->lr_w > 0.0f || cfg->lr_c > 0.0f;
+    const int build_nbr_day = cfg->lr_w_day > 0.0f || cfg->lr_c_day > 0.0f;
+    const int32_t U = m->n_users, I = m->n_items;
+    m->has_nbr = build_nbr;
+    m->has_wd = build_nbr_day;  /* w_day and c_day are built together */
+    m->has_cd = build_nbr_day;
+    m->norm_nu = zalloc(U);
+    m->norm_ru = zalloc(U);
+    m->imp_start = calloc((size_t)U + 1, sizeof(int64_t));
+    m->rmb = zalloc(build_nbr ? tr->n_ratings : 0);
+    m->r_item = malloc(sizeof(uint16_t) * (size_t)(build_nbr && tr->n_ratings ? tr->n_ratings : 1));
+    m->r_day = malloc(sizeof(int16_t) * (size_t)(build_nbr && tr->n_ratings ? tr->n_ratings : 1));
+    if (build_nbr) {
+        for (int64_t t = 0; t < tr->n_ratings; t++) {
+            m->r_item[t] = (uint16_t)tr->item_idxs[t];
+            m->r_day[t] = tr->dates[t];
+        }
+        /* b~_i, b~_u: FP64 sums over the training rows, raw ratings */
+        const double mu = (double)m->gbias;
+        double* isum = calloc((size_t)I, sizeof(double));
+        double* icnt = calloc((size_t)I, sizeof(double));
+        for (int64_t t = 0; t < tr->n_ratings; t++) {
+            isum[tr->item_idxs[t]] += (double)tr->raw_ratings[t] - mu;
+            icnt[tr->item_idxs[t]] += 1.0;
+        }
+        float* bti = zalloc(I);
+        for (int32_t i = 0; i < I; i++) bti[i] = (float)(isum[i] / (icnt[i] + (double)cfg->lambda1));
+        double* usum = calloc((size_t)U, sizeof(double));
+        double* ucnt = calloc((size_t)U, sizeof(double));
+        for (int64_t t = 0; t < tr->n_ratings; t++) {
+            usum[tr->user_idxs[t]] += (double)tr->raw_ratings[t] - mu - (double)bti[tr->item_idxs[t]];
+            ucnt[tr->user_idxs[t]] += 1.0;
+        }
+        float* btu = zalloc(U);
+        for (int32_t u = 0; u < U; u++) btu[u] = (float)(usum[u] / (ucnt[u] + (double)cfg->lambda2));
+        for (int64_t t = 0; t < tr->n_ratings; t++) {
+            const float btilde = m->gbias + bti[tr->item_idxs[t]] + btu[tr->user_idxs[t]];
+            m->rmb[t] = (float)tr->raw_ratings[t] - btilde;
+        }
+        /* N(u)\R(u): held-out items the user has not rated in training */
+        int64_t maxc = 1;
+        for (int32_t u = 0; u < U; u++)
+            if (m->user_offsets[u + 1] - m->user_offsets[u] > maxc) maxc = m->user_offsets[u + 1] - m->user_offsets[u];
+        uint16_t* rated = malloc(sizeof(uint16_t) * (size_t)maxc);
+        int64_t* hcnt = calloc((size_t)U + 1, sizeof(int64_t));
+        for (int64_t t = 0; t < pr->n_ratings; t++) hcnt[pr->user_idxs[t] + 1]++;
+        for (int32_t u = 0; u < U; u++) hcnt[u + 1] += hcnt[u];
+        or_imp* hb = malloc(sizeof(or_imp) * (size_t)(pr->n_ratings ? pr->n_ratings : 1));
+        int64_t* hp = malloc(sizeof(int64_t) * ((size_t)U + 1));
+        memcpy(hp, hcnt, sizeof(int64_t) * ((size_t)U + 1));
+        for (int64_t t = 0; t < pr->n_ratings; t++) {
+            or_imp e = {(uint16_t)pr->item_idxs[t], pr->dates[t]};
+            hb[hp[pr->user_idxs[t]]++] = e;
+        }
+        int64_t nimp = 0;
+        for (int32_t u = 0; u < U; u++) {
+            const int64_t a = m->user_offsets[u], n = m->user_offsets[u + 1] - a;
+            for (int64_t k = 0; k < n; k++) rated[k] = (uint16_t)tr->item_idxs[a + k];
+            qsort(rated, (size_t)n, sizeof(uint16_t), cmp_u16);
+            int64_t w = hcnt[u];
+            for (int64_t k = hcnt[u]; k < hcnt[u + 1]; k++)
+                if (!bsearch(&hb[k].item, rated, (size_t)n, sizeof(uint16_t), cmp_u16)) hb[w++] = hb[k];
+            const int64_t len = w - hcnt[u];
+            qsort(hb + hcnt[u], (size_t)len, sizeof(or_imp), build_nbr_day ? cmp_imp_item_day : cmp_imp_item);
+            int64_t d = 0;
+            for (int64_t k = 0; k < len; k++) {
+                const or_imp e = hb[hcnt[u] + k];
+                if (d > 0) {
+                    const or_imp p = hb[hcnt[u] + d - 1];
+                    if (p.item == e.item && (!build_nbr_day || p.day == e.day)) continue;
+                }
+                hb[hcnt[u] + d++] = e;
+            }
+            m->imp_start[u + 1] = d;   /* count for now */
+            nimp += d;
+        }
+        for (int32_t u = 0; u < U; u++) m->imp_start[u + 1] += m->imp_start[u];
+        m->imp_item = malloc(sizeof(uint16_t) * (size_t)(nimp ? nimp : 1));
+        m->imp_day = malloc(sizeof(int16_t) * (size_t)(nimp ? nimp : 1));
+        for (int32_t u = 0; u < U; u++)
+            for (int64_t k = 0; k < m->imp_start[u + 1] - m->imp_start[u]; k++) {
+                m->imp_item[m->imp_start[u] + k] = hb[hcnt[u] + k].item;
+                m->imp_day[m->imp_start[u] + k] = hb[hcnt[u] + k].day;
+            }
+        for (int32_t u = 0; u < U; u++) {
+            const int64_t n_r = m->user_offsets[u + 1] - m->user_offsets[u];
+            const int64_t n_n = n_r + (m->imp_start[u + 1] - m->imp_start[u]);
+            m->norm_ru[u] = n_r > 0 ? powf((float)n_r, -0.5f) : 0.0f;
+            m->norm_nu[u] = n_n > 0 ? powf((float)n_n, -0.5f) : 0.0f;
+        }
+        free(isum); free(icnt); free(bti); free(usum); free(ucnt); free(btu);
+        free(rated); free(hcnt); free(hb); free(hp);
+    } else {
+        m->imp_item = malloc(sizeof(uint16_t));
+        m->imp_day = malloc(sizeof(int16_t));
+    }
+    const int64_t n2 = (int64_t)I * I;
+    m->w = build_nbr ? zalloc(n2) : NULL;
+    m->c = build_nbr ? zalloc(n2) : NULL;
+    m->w_day = build_nbr_day ? zalloc(n2) : NULL;
+    m->c_day = build_nbr_day ? zalloc(n2) : NULL;
+}
+
+/* nbr_score (src/tx.rs:341-389): every neighbour, unscaled. */
+static float nbr_score(const or_model* m, int32_t u, int32_t i, int16_t day) {
+    const int64_t ni = m->n_items;
+    const float nn = m->norm_nu[u], nr = m->norm_ru[u];
+    float c_sum = 0.0f, w_sum = 0.0f;
+    for (int64_t t = m->user_offsets[u]; t < m->user_offsets[u + 1]; t++) {
+        const int64_t j = m->r_item[t];
+        if (j == i) continue;
+        c_sum += m->c[i * ni + j];
+        w_sum += m->rmb[t] * m->w[i * ni + j];
+    }
+    for (int64_t k = m->imp_start[u]; k < m->imp_start[u + 1]; k++) {
+        const int64_t j = m->imp_item[k];
+        if (j == i) continue;
+        c_sum += m->c[i * ni + j];
+    }
+    float c_day_sum = 0.0f, w_day_sum = 0.0f;
+    uint32_t n_same_rated = 0, n_same_total = 0;
+    if (m->has_wd || m->has_cd) {
+        for (int64_t t = m->user_offsets[u]; t < m->user_offsets[u + 1]; t++) {
+            const int64_t j = m->r_item[t];
+            if (j == i || m->r_day[t] != day) continue;
+            n_same_rated++;
+            n_same_total++;
+            if (m->has_cd) c_day_sum += m->c_day[i * ni + j];
+            if (m->has_wd) w_day_sum += m->rmb[t] * m->w_day[i * ni + j];
+        }
+        for (int64_t k = m->imp_start[u]; k < m->imp_start[u + 1]; k++) {
+            const int64_t j = m->imp_item[k];
+            if (j == i || m->imp_day[k] != day) continue;
+            n_same_total++;
+            if (m->has_cd) c_day_sum += m->c_day[i * ni + j];
+        }
+    }
+    const float nn_day = n_same_total > 0 ? powf((float)n_same_total, -0.5f) : 0.0f;
+    const float nr_day = n_same_rated > 0 ? powf((float)n_same_rated, -0.5f) : 0.0f;
+    return nn * c_sum + nr * w_sum + nn_day * c_day_sum + nr_day * w_day_sum;
+}
+
 OR_API or_model* or_new(const or_config* cfg, const or_dataset* tr, const or_dataset* pr) {
     zig_init();
     or_model* m = calloc(1, sizeof(or_model));
     m->cfg = *cfg;
-    if (cfg->lr_w > 0.0f || cfg->lr_c > 0.0f || cfg->lr_w_day > 0.0f || cfg->lr_c_day > 0.0f) {
-        snprintf(m->err, sizeof m->err, "neighbourhood terms are not part of tx4 mode");
-        return m;
-    }
     const int32_t U = tr->n_users, I = tr->n_items, f = cfg->n_feat, F = cfg->n_freq_bins;
     m->n_users = U; m->n_items = I; m->f = f; m->F = F; m->B = cfg->n_time_bins;
     or_rng rng;
@@ -563,6 +745,7 @@ OR_API or_model* or_new(const or_config* cfg, const or_dataset* tr, const or_dat
     for (int32_t u = 0; u < U; u++) m->cu[u] = 1.0f;
     m->cut = zalloc(n_ud);
     for (int k = 0; k < 4; k++) m->thresholds[k] = cfg->th_init[k];
+    nbr_build(m, tr, pr);
     return m;
 }
 
@@ -575,7 +758,8 @@ OR_API void or_free(or_model* m) {
                   m->ycache_bias, m->ud.starts, m->ud.dates, m->ud.day_cnts, m->ubias_day,
                   m->ufeat_day, m->ycache_day, m->ibias_freq, m->ifeat_freq, m->cu, m->cut,
                   m->user_offsets, m->pbu_start, m->pbd_start, m->pbu_items, m->pbd_items,
-                  m->bi_starts, m->bi_user, m->bi_rating};
+                  m->bi_starts, m->bi_user, m->bi_rating, m->rmb, m->r_item, m->r_day, m->imp_start,
+                  m->imp_item, m->imp_day, m->norm_nu, m->norm_ru, m->w, m->c, m->w_day, m->c_day};
     for (size_t k = 0; k < sizeof ps / sizeof ps[0]; k++) free(ps[k]);
     free(m);
 }
@@ -661,6 +845,11 @@ static int64_t day_ranges(const int16_t* dates, int64_t start, int64_t end, int6
 /* TxModel::fit_epoch (src/tx.rs:734-1115), tx4 mode (no neighbourhood)     */
 /* ------------------------------------------------------------------------ */
 
+static int cmp_i64(const void* a, const void* b) {
+    int64_t x = *(const int64_t*)a, y = *(const int64_t*)b;
+    return (x > y) - (x < y);
+}
+
 static void fit_epoch_users(or_model* m, const or_dataset* tr, const or_dataset* pr, int64_t epoch,
                             const int64_t* users) {
     const or_config c = m->cfg;
@@ -684,12 +873,54 @@ static void fit_epoch_users(or_model* m, const or_dataset* tr, const or_dataset*
     }
     int64_t* rs = malloc(sizeof(int64_t) * (size_t)maxcnt);
     int64_t* re = malloc(sizeof(int64_t) * (size_t)maxcnt);
+    /* neighbour sampling (src/tx.rs:750-793): one stream per epoch */
+    const int has_nbr = m->has_nbr, has_wd = m->has_wd, has_cd = m->has_cd;
+    const int64_t ni = m->n_items;
+    or_rng srng;
+    rng_seed_u64(&srng, c.seed + (uint64_t)epoch * 1000003ull, 12);
+    int64_t maximp = 1;
+    for (int32_t u = 0; u < m->n_users; u++)
+        if (m->imp_start[u + 1] - m->imp_start[u] > maximp) maximp = m->imp_start[u + 1] - m->imp_start[u];
+    int64_t* smp_r = malloc(sizeof(int64_t) * (size_t)maxcnt);
+    int64_t* smp_i = malloc(sizeof(int64_t) * (size_t)maximp);
 
     for (int32_t ui = 0; ui < m->n_users; ui++) {
         const int32_t u = (int32_t)users[ui];
         const int64_t start = m->user_offsets[u], end = m->user_offsets[u + 1];
         const int64_t cnt = end - start;
         if (cnt == 0) continue;
+
+        int64_t iter_rated = 0, iter_impl = 0;
+        float rated_scale = 1.0f, impl_scale = 1.0f;
+        int use_sampling = 0;
+        const int64_t i0 = m->imp_start[u];
+        if (has_nbr) {
+            const int64_t nr_full = cnt, ni_full = m->imp_start[u + 1] - i0, n_full = nr_full + ni_full;
+            const int64_t max_nbr = c.max_neighbors;
+            if (max_nbr > 0 && n_full > max_nbr) {
+                int64_t rb = (int64_t)round((double)max_nbr * (double)nr_full / (double)n_full);
+                if (rb < 1) rb = 1;
+                if (rb > nr_full) rb = nr_full;
+                int64_t ib = max_nbr - rb;
+                if (ib > ni_full) ib = ni_full;
+                for (int64_t k = 0; k < nr_full; k++) smp_r[k] = k;
+                rng_shuffle_i64(&srng, smp_r, nr_full);
+                qsort(smp_r, (size_t)rb, sizeof(int64_t), cmp_i64);
+                for (int64_t k = 0; k < ni_full; k++) smp_i[k] = k;
+                rng_shuffle_i64(&srng, smp_i, ni_full);
+                qsort(smp_i, (size_t)ib, sizeof(int64_t), cmp_i64);
+                rated_scale = (float)nr_full / (float)rb;
+                impl_scale = ib > 0 ? (float)ni_full / (float)ib : 1.0f;
+                iter_rated = rb;
+                iter_impl = ib;
+                use_sampling = 1;
+            } else {
+                iter_rated = nr_full;
+                iter_impl = ni_full;
+            }
+        }
+        const float nr = has_nbr ? m->norm_ru[u] : 0.0f;
+        const float nn = has_nbr ? m->norm_nu[u] : 0.0f;
         if (!low_mem) day_ranges(tr->dates, start, end, rs, re);
 
         /* NSVD1 user-level aggregation (src/tx.rs:801-822) */
@@ -781,7 +1012,46 @@ static void fit_epoch_users(or_model* m, const or_dataset* tr, const or_dataset*
                 dot += pu_eff * qi_eff;
             }
             const float bias_part = m->gbias + bu_t + bi_t * cu_t;
-            const float score = bias_part + dot + 0.0f;
+
+            /* same-day neighbourhood, every neighbour (src/tx.rs:903-929) */
+            float c_day_sum = 0.0f, w_day_sum = 0.0f, nn_day = 0.0f, nr_day = 0.0f;
+            if (has_wd || has_cd) {
+                uint32_t n_same_rated = 0, n_same_total = 0;
+                for (int64_t t2 = start; t2 < end; t2++) {
+                    const int64_t j = m->r_item[t2];
+                    if (j == i || m->r_day[t2] != day16) continue;
+                    n_same_rated++;
+                    n_same_total++;
+                    if (has_cd) c_day_sum += m->c_day[i * ni + j];
+                    if (has_wd) w_day_sum += m->rmb[t2] * m->w_day[i * ni + j];
+                }
+                for (int64_t k = i0; k < m->imp_start[u + 1]; k++) {
+                    const int64_t j = m->imp_item[k];
+                    if (j == i || m->imp_day[k] != day16) continue;
+                    n_same_total++;
+                    if (has_cd) c_day_sum += m->c_day[i * ni + j];
+                }
+                nn_day = n_same_total > 0 ? powf((float)n_same_total, -0.5f) : 0.0f;
+                nr_day = n_same_rated > 0 ? powf((float)n_same_rated, -0.5f) : 0.0f;
+            }
+            /* global neighbourhood over the (sampled) neighbours (src/tx.rs:931-956) */
+            float c_sum = 0.0f, w_sum = 0.0f;
+            if (has_nbr) {
+                for (int64_t k = 0; k < iter_rated; k++) {
+                    const int64_t t2 = start + (use_sampling ? smp_r[k] : k);
+                    const int64_t j = m->r_item[t2];
+                    if (j == i) continue;
+                    c_sum += m->c[i * ni + j] * rated_scale;
+                    w_sum += m->rmb[t2] * m->w[i * ni + j] * rated_scale;
+                }
+                for (int64_t k = 0; k < iter_impl; k++) {
+                    const int64_t j = m->imp_item[i0 + (use_sampling ? smp_i[k] : k)];
+                    if (j == i) continue;
+                    c_sum += m->c[i * ni + j] * impl_scale;
+                }
+            }
+            const float nbr_part = nn * c_sum + nr * w_sum + nn_day * c_day_sum + nr_day * w_day_sum;
+            const float score = bias_part + dot + nbr_part;
 
             float err;
             if (has_ordinal) {
@@ -796,6 +1066,7 @@ static void fit_epoch_users(or_model* m, const or_dataset* tr, const or_dataset*
             }
             const float err_b = err * c.w_bias;
             const float err_d = err * c.w_factor;
+            const float err_n = err * c.w_nbr;
 
             /* bias steps (src/tx.rs:981-990) */
             m->ubias[u] -= c.lr_ub * err_b;
@@ -847,6 +1118,53 @@ static void fit_epoch_users(or_model* m, const or_dataset* tr, const or_dataset*
                 }
                 dr_idx += 1;
             }
+
+            /* global neighbourhood steps over the sampled neighbours (src/tx.rs:1038-1069) */
+            if (has_nbr) {
+                for (int64_t k = 0; k < iter_rated; k++) {
+                    const int64_t t2 = start + (use_sampling ? smp_r[k] : k);
+                    const int64_t j = m->r_item[t2];
+                    if (j == i) continue;
+                    const float rmb = m->rmb[t2];
+                    const int64_t idx = i * ni + j;
+                    const float w_ij = m->w[idx];
+                    m->w[idx] -= c.lr_w * (err_n * nr * rmb * rated_scale + c.reg_w * w_ij);
+                    const float c_ij = m->c[idx];
+                    m->c[idx] -= c.lr_c * (err_n * nn * rated_scale + c.reg_c * c_ij);
+                }
+                for (int64_t k = 0; k < iter_impl; k++) {
+                    const int64_t j = m->imp_item[i0 + (use_sampling ? smp_i[k] : k)];
+                    if (j == i) continue;
+                    const int64_t idx = i * ni + j;
+                    const float c_ij = m->c[idx];
+                    m->c[idx] -= c.lr_c * (err_n * nn * impl_scale + c.reg_c * c_ij);
+                }
+            }
+            /* same-day neighbourhood steps, every neighbour (src/tx.rs:1071-1095) */
+            if (has_wd || has_cd) {
+                for (int64_t t2 = start; t2 < end; t2++) {
+                    const int64_t j = m->r_item[t2];
+                    if (j == i || m->r_day[t2] != day16) continue;
+                    const int64_t idx = i * ni + j;
+                    if (has_wd) {
+                        const float wd = m->w_day[idx];
+                        m->w_day[idx] -= c.lr_w_day * (err_n * nr_day * m->rmb[t2] + c.reg_w_day * wd);
+                    }
+                    if (has_cd) {
+                        const float cd = m->c_day[idx];
+                        m->c_day[idx] -= c.lr_c_day * (err_n * nn_day + c.reg_c_day * cd);
+                    }
+                }
+                for (int64_t k = i0; k < m->imp_start[u + 1]; k++) {
+                    const int64_t j = m->imp_item[k];
+                    if (j == i || m->imp_day[k] != day16) continue;
+                    if (has_cd) {
+                        const int64_t idx = i * ni + j;
+                        const float cd = m->c_day[idx];
+                        m->c_day[idx] -= c.lr_c_day * (err_n * nn_day + c.reg_c_day * cd);
+                    }
+                }
+            }
         }
 
         /* end of user (src/tx.rs:1099-1106) */
@@ -872,6 +1190,7 @@ static void fit_epoch_users(or_model* m, const or_dataset* tr, const or_dataset*
         }
     }
     free(su); free(sum_err_q); free(sum_err_q_day); free(su_day); free(su_freq); free(rs); free(re);
+    free(smp_r); free(smp_i);
 }
 
 OR_API void or_fit_epoch(or_model* m, const or_dataset* tr, const or_dataset* pr, int64_t epoch) {
@@ -917,7 +1236,8 @@ static float predict_one(const or_model* m, int32_t u, int32_t i, int32_t day) {
         dot += pu_eff * qi_eff;
     }
     const float bias_part = m->gbias + bu_t + bi_t * cu_t;
-    const float score = bias_part + dot + 0.0f;
+    const float nbr_part = m->has_nbr ? nbr_score(m, u, i, (int16_t)day) : 0.0f;
+    const float score = bias_part + dot + nbr_part;
     if (m->cfg.has_ordinal) {
         float p[5];
         ord_probs(m->thresholds, score, p);
@@ -932,7 +1252,7 @@ OR_API void or_predict(const or_model* m, int64_t n, const int32_t* u, const int
 }
 
 OR_API void or_predict_subscores(const or_model* m, int64_t n, const int32_t* us, const int32_t* is,
-                                 const int16_t* ds, float* bias_o, float* mf_o, float* nsvd_o) {
+                                 const int16_t* ds, float* bias_o, float* mf_o, float* nsvd_o, float* nbr_o) {
     const int f = m->f, F = m->F;
     for (int64_t q = 0; q < n; q++) {
         const int32_t u = us[q], i = is[q], day = ds[q];
@@ -962,7 +1282,42 @@ OR_API void or_predict_subscores(const or_model* m, int64_t n, const int32_t* us
             nsvd1 += su_eff * qi_eff;
         }
         bias_o[q] = bias; mf_o[q] = mf; nsvd_o[q] = nsvd1;
+        if (nbr_o) nbr_o[q] = m->has_nbr ? nbr_score(m, u, i, (int16_t)day) : 0.0f;
     }
+}
+
+/* nbr_score with FP64 sums and norms. */
+static double nbr_score_f64(const or_model* m, int32_t u, int32_t i, int16_t day) {
+    const int64_t ni = m->n_items;
+    double c_sum = 0.0, w_sum = 0.0, c_day_sum = 0.0, w_day_sum = 0.0;
+    int64_t n_same_rated = 0, n_same_total = 0;
+    for (int64_t t = m->user_offsets[u]; t < m->user_offsets[u + 1]; t++) {
+        const int64_t j = m->r_item[t];
+        if (j == i) continue;
+        c_sum += m->c[i * ni + j];
+        w_sum += (double)m->rmb[t] * m->w[i * ni + j];
+        if ((m->has_wd || m->has_cd) && m->r_day[t] == day) {
+            n_same_rated++;
+            n_same_total++;
+            if (m->has_cd) c_day_sum += m->c_day[i * ni + j];
+            if (m->has_wd) w_day_sum += (double)m->rmb[t] * m->w_day[i * ni + j];
+        }
+    }
+    for (int64_t k = m->imp_start[u]; k < m->imp_start[u + 1]; k++) {
+        const int64_t j = m->imp_item[k];
+        if (j == i) continue;
+        c_sum += m->c[i * ni + j];
+        if ((m->has_wd || m->has_cd) && m->imp_day[k] == day) {
+            n_same_total++;
+            if (m->has_cd) c_day_sum += m->c_day[i * ni + j];
+        }
+    }
+    const int64_t n_r = m->user_offsets[u + 1] - m->user_offsets[u];
+    const int64_t n_n = n_r + (m->imp_start[u + 1] - m->imp_start[u]);
+    const double nr = n_r > 0 ? 1.0 / sqrt((double)n_r) : 0.0, nn = n_n > 0 ? 1.0 / sqrt((double)n_n) : 0.0;
+    const double nr_day = n_same_rated > 0 ? 1.0 / sqrt((double)n_same_rated) : 0.0;
+    const double nn_day = n_same_total > 0 ? 1.0 / sqrt((double)n_same_total) : 0.0;
+    return nn * c_sum + nr * w_sum + nn_day * c_day_sum + nr_day * w_day_sum;
 }
 
 /* FP64 predict oracle: the predict formula of src/tx.rs:303-339 evaluated in
@@ -989,6 +1344,7 @@ OR_API void or_predict_f64(const or_model* m, int64_t n, const int32_t* us, cons
             dot += pu_eff * qi_eff;
         }
         double score = (double)m->gbias + bu_t + bi_t * cu_t + dot;
+        if (m->has_nbr) score += nbr_score_f64(m, u, i, (int16_t)day);
         if (m->cfg.has_ordinal) {
             double fc[6] = {0, 0, 0, 0, 0, 1};
             for (int k = 1; k < 5; k++) fc[k] = 1.0 / (1.0 + exp(-((double)m->thresholds[k - 1] - score)));

@@ -65,6 +65,46 @@ def test_tsvdx4p_tiny_probe_rmse_kat():
     assert m.calc_rmse() == kat["probe_rmse"]
 
 
+@pytest.mark.parametrize("which", ["tsvdx5", "tsvdx6"])
+def test_neighbourhood_tiny_probe_rmse_kat(which):
+    """tests/regressors_test.rs:95-155: TxModel with the global (tsvdx5) and
+    the global + same-day (tsvdx6) neighbourhood terms, 5 sampled neighbours."""
+    fx, tr, pr = tiny()
+    kat = fx["kat"][which]
+    m = O.OracleModel(kat["config"], tr, pr)
+    for e in range(1, kat["config"]["n_epochs"] + 1):
+        m.fit_epoch(e)
+    assert m.calc_rmse() == kat["probe_rmse"]
+
+
+@pytest.mark.parametrize("plant", [dict(w_nbr=1.0), dict(max_neighbors=4), dict(lambda1=20.0)])
+@pytest.mark.parametrize("which", ["tsvdx5", "tsvdx6"])
+def test_neighbourhood_kat_breaks_under_planted_mistake(which, plant):
+    """The neighbourhood steps' weight, the sample size and the baseline's
+    shrinkage each move the KAT."""
+    fx, tr, pr = tiny()
+    cfg = dict(fx["kat"][which]["config"], **plant)
+    m = O.OracleModel(cfg, tr, pr)
+    for e in range(1, cfg["n_epochs"] + 1):
+        m.fit_epoch(e)
+    assert m.calc_rmse() != fx["kat"][which]["probe_rmse"]
+
+
+def test_neighbourhood_subscores_sum_to_predict():
+    """predict = bias + mf + nsvd1 + nbr (src/tx.rs:663-724), and the nbr part
+    is nonzero once the w and c tables have trained."""
+    fx, tr, pr = tiny()
+    cfg = fx["kat"]["tsvdx6"]["config"]
+    m = O.OracleModel(cfg, tr, pr)
+    for e in range(1, cfg["n_epochs"] + 1):
+        m.fit_epoch(e)
+    b, mf, ns, nb = m.predict_subscores(pr.user_idxs, pr.item_idxs, pr.dates)
+    assert np.any(nb != 0)
+    p = m.predict(pr.user_idxs, pr.item_idxs, pr.dates)
+    np.testing.assert_allclose(b + mf + ns + nb, p, atol=1e-5)
+    np.testing.assert_allclose(m.predict_f64(pr.user_idxs, pr.item_idxs, pr.dates), p, atol=1e-5)
+
+
 @pytest.mark.parametrize("which", ["tsvdx4", "tsvdx4p"])
 def test_kat_breaks_under_planted_mistake(which):
     """A dropped term must move the KAT: zeroing lr_ub changes the result."""
