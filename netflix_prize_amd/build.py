@@ -67,7 +67,9 @@ def build(verbose: bool = False) -> str:
                              os.path.join(CSRC, src), "-o", os.path.join(OBJ, obj)]
     dev = lambda src, obj: [hipcc, *HIP_FLAGS, *inc, "-c", os.path.join(CSRC, src), "-o", os.path.join(OBJ, obj)]
     jobs = [host("tx_host.cpp", "tx_host.o"), host("knn_host.cpp", "knn_host.o"), host("knn3_host.cpp", "knn3_host.o"),
-            dev("tx_kernels.hip", "tx_kernels.o"), dev("knn_kernels.hip", "knn_kernels.o"),
+            dev("tx_kernels.hip", "tx_kernels.o"), dev("tx_sgd_lm.hip", "tx_sgd_lm.o"),
+            dev("tx_sgd_day.hip", "tx_sgd_day.o"), dev("tx_sgd_user.hip", "tx_sgd_user.o"),
+            dev("tx_sgd_mw.hip", "tx_sgd_mw.o"), dev("knn_kernels.hip", "knn_kernels.o"),
             dev("knn3_kernels.hip", "knn3_kernels.o")]
     with ThreadPoolExecutor(max_workers=len(jobs)) as ex:
         times = list(ex.map(_run, jobs))

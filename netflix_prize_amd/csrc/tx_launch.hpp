@@ -1,4 +1,4 @@
-// Kernel launchers (defined in tx_kernels.hip).
+// Kernel launchers (defined in tx_kernels.hip and tx_sgd_*.hip).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -20,6 +20,13 @@ hipError_t launch_sgd_par(const DevModel& m, const StrataView& sv, int stratum, 
 // each, fs = 1024*waves, one stratum; pass as for launch_sgd_par
 hipError_t launch_sgd_par_mw(const DevModel& m, const StrataView& sv, int teams, int waves, bool qf, bool p2, bool ord,
                              bool day, hipStream_t st, int pass = 0);
+// the variant families behind launch_sgd_par (one translation unit each)
+hipError_t launch_sgd_par_lm(const DevModel& m, const StrataView& sv, int stratum, int team, int vec, int blocks,
+                             bool qf, bool p2, bool ord, hipStream_t st);
+hipError_t launch_sgd_par_day(const DevModel& m, const StrataView& sv, int stratum, int team, int vec, int blocks,
+                              bool qf, bool p2, bool ord, hipStream_t st);
+hipError_t launch_sgd_par_user(const DevModel& m, const StrataView& sv, int stratum, int team, int vec, int blocks,
+                               bool qf, bool p2, int pass, hipStream_t st);
 hipError_t launch_to_bf16(const float* x, uint16_t* y, int64_t n, hipStream_t st);
 // the NSVD1 step of the tsvdx4p user pass from the stored per-user gradients
 // (gacc [U * fs], gacc_b [U]) gathered per item; vec = the shape's VEC
