@@ -41,8 +41,10 @@ enum tx_status {
 
 /* Mirror of TxConfig (src/tx.rs:128-204) plus OrdinalHeadConfig
  * (src/lib.rs:502-507).  Booleans are int32 0/1.  The neighbourhood knobs
- * (lr_w .. reg_c_day, max_neighbors, lambda1/2) belong to the tsvdx5/6 modes
- * and must be 0 here; tx_create rejects anything else. */
+ * (lr_w .. reg_c_day, w_nbr, max_neighbors, lambda1/2) switch on the tsvdx5
+ * (global w/c) and tsvdx6 (plus same-day w_day/c_day) terms as the reference
+ * does: w and c exist when lr_w > 0 or lr_c > 0, w_day and c_day when
+ * lr_w_day > 0 or lr_c_day > 0 (src/tx.rs:483-585). */
 typedef struct tx_config {
     uint64_t seed;
     int32_t n_feat, n_epochs, n_time_bins, n_freq_bins;
@@ -175,6 +177,10 @@ enum tx_param {
     TX_P_CUT,            /* [D]          */
     TX_P_GBIAS,          /* [1]          */
     TX_P_THRESHOLDS,     /* [4]          ordinal head thresholds */
+    TX_P_NB_W,           /* [I, I]       w     (0 elements unless lr_w > 0 or lr_c > 0) */
+    TX_P_NB_C,           /* [I, I]       c     (same) */
+    TX_P_NB_W_DAY,       /* [I, I]       w_day (0 elements unless lr_w_day > 0 or lr_c_day > 0) */
+    TX_P_NB_C_DAY,       /* [I, I]       c_day (same) */
     TX_P_COUNT
 };
 
@@ -253,7 +259,7 @@ int tx_predict_device(tx_ctx* ctx, int64_t n, const int32_t* d_users, const int3
 /* No counterpart in the reference.  tx_predict for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving loops,
  * harnesses that forbid hipMalloc): the same predictions, staged through a
- * fixed scratch array of the library (23 MiB per device, held until the
+ * fixed scratch array of the library (27 MiB per device, held until the
  * process ends) instead of memory allocated on first use.  At most
  * TX_NOALLOC_MAX_ROWS queries per call (more: TX_EINVAL).  Every call on one
  * device shares that array, whatever its ctx: calls that overlap on one
@@ -262,9 +268,15 @@ int tx_predict_device(tx_ctx* ctx, int64_t n, const int32_t* d_users, const int3
 int tx_predict_noalloc(tx_ctx* ctx, int64_t n, const int32_t* users, const int32_t* items, const int16_t* days,
                        float* out);
 
-/* Regressor::predict_subscores (src/tx.rs:663-711): bias, mf, nsvd1. */
+/* Regressor::predict_subscores (src/tx.rs:663-711): bias, mf, nsvd1, and nbr
+ * (nbr_score, src/tx.rs:341-389) when the model has the neighbourhood terms
+ * (tx_n_subscores = 4); nbr may be NULL otherwise. */
 int tx_predict_subscores(tx_ctx* ctx, int64_t n, const int32_t* users, const int32_t* items,
-                         const int16_t* days, float* bias, float* mf, float* nsvd1);
+                         const int16_t* days, float* bias, float* mf, float* nsvd1, float* nbr);
+
+/* Regressor::n_subscores (src/tx.rs:653): 4 with the neighbourhood terms
+ * ("bias", "mf", "nsvd1", "nbr"), else 3. */
+int tx_n_subscores(tx_ctx* ctx, int32_t* out);
 
 /* calc_rmse (src/lib.rs:659-673): FP64 RMSE of predict against residuals
  * over rows with is_test == 0. */

@@ -56,8 +56,21 @@ def _run(cmd):
     return time.time() - t0
 
 
-def build(verbose: bool = False) -> str:
-    """Compile every HIP/C++ source and link libtsvd_tx.so; returns its path."""
+def _stale(cmd) -> bool:
+    """An object is stale when it is missing or older than its source or any
+    shared header (csrc/*.hpp, include/*.h)."""
+    obj, src = cmd[-1], cmd[-3]
+    if not os.path.exists(obj):
+        return True
+    deps = [src] + [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hpp")] + \
+        [os.path.join(INCLUDE, f) for f in os.listdir(INCLUDE) if f.endswith(".h")]
+    t = os.path.getmtime(obj)
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def build(verbose: bool = False, incremental: bool = False) -> str:
+    """Compile every HIP/C++ source and link libtsvd_tx.so; returns its path.
+    incremental=True recompiles only the stale objects (see _stale)."""
     hipcc = _hipcc()
     os.makedirs(OBJ, exist_ok=True)
     rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
@@ -69,10 +82,13 @@ def build(verbose: bool = False) -> str:
     jobs = [host("tx_host.cpp", "tx_host.o"), host("knn_host.cpp", "knn_host.o"), host("knn3_host.cpp", "knn3_host.o"),
             dev("tx_kernels.hip", "tx_kernels.o"), dev("tx_sgd_lm.hip", "tx_sgd_lm.o"),
             dev("tx_sgd_day.hip", "tx_sgd_day.o"), dev("tx_sgd_user.hip", "tx_sgd_user.o"),
-            dev("tx_sgd_mw.hip", "tx_sgd_mw.o"), dev("knn_kernels.hip", "knn_kernels.o"),
+            dev("tx_sgd_mw.hip", "tx_sgd_mw.o"), dev("tx_sgd_nb1.hip", "tx_sgd_nb1.o"),
+            dev("tx_sgd_nb2.hip", "tx_sgd_nb2.o"), dev("knn_kernels.hip", "knn_kernels.o"),
             dev("knn3_kernels.hip", "knn3_kernels.o")]
-    with ThreadPoolExecutor(max_workers=len(jobs)) as ex:
-        times = list(ex.map(_run, jobs))
+    todo = [j for j in jobs if _stale(j)] if incremental else jobs
+    with ThreadPoolExecutor(max_workers=max(len(todo), 1)) as ex:
+        done = dict(zip(map(id, todo), ex.map(_run, todo)))
+    times = [done.get(id(j), 0.0) for j in jobs]
     if verbose:
         for cmd, dt in zip(jobs, times):
             print(f"{os.path.basename(cmd[-3])} {dt:.1f}s")
@@ -97,11 +113,13 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, PROBE)
     if verbose:
         print(f"atomic_ceiling.hip {dt:.1f}s")
-    # the tsvdx4p-new jobs run from the same binary under the reference's name
-    link = os.path.join(PKG, "tsvdx4p-new")
-    if os.path.lexists(link):
-        os.remove(link)
-    os.symlink(os.path.basename(DRIVER), link)
+    # the tsvdx4p-new, tsvdx5-new and tsvdx6 jobs run from the same binary
+    # under the reference's binary names
+    for name in ("tsvdx4p-new", "tsvdx5-new", "tsvdx6-300-new", "tsvdx6-300-v2-new"):
+        link = os.path.join(PKG, name)
+        if os.path.lexists(link):
+            os.remove(link)
+        os.symlink(os.path.basename(DRIVER), link)
     if verbose:
         print(f"tsvdx4_new.cpp {dt:.1f}s")
     return LIB

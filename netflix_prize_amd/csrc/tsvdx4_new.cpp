@@ -419,13 +419,17 @@ void write_preds(const std::string& path, const std::vector<float>& p) {
 void save_subscores(Model& m, const Dataset& ds, const std::string& preds_dir, const std::string& job,
                     const std::string& set) {
     const int64_t n = ds.n_ratings();
-    std::vector<float> b(n), mf(n), ns(n);
+    int32_t k = 0;
+    check(tx_n_subscores(m.ctx, &k), "n_subscores");
+    std::vector<float> b(n), mf(n), ns(n), nb(k > 3 ? n : 0);
     check(tx_predict_subscores(m.ctx, n, ds.user_idxs.data(), ds.item_idxs.data(), ds.dates.data(), b.data(), mf.data(),
-                               ns.data()),
+                               ns.data(), k > 3 ? nb.data() : nullptr),
           "predict_subscores");
+    // subscore_names (src/tx.rs:655-661): bias, mf, nsvd1[, nbr]
     write_preds(preds_dir + "/" + job + "-bias." + set + ".npy", b);
     write_preds(preds_dir + "/" + job + "-mf." + set + ".npy", mf);
     write_preds(preds_dir + "/" + job + "-nsvd1." + set + ".npy", ns);
+    if (k > 3) write_preds(preds_dir + "/" + job + "-nbr." + set + ".npy", nb);
 }
 
 // save_artifacts (src/tx.rs:726-732)
@@ -875,6 +879,69 @@ bool find_job(const std::string& job, Job* out) {
     return false;
 }
 
+// tsvdx5-new base jobs (src/bin/tsvdx5-new.rs:21-33, run_base :130-183 and
+// run_ord :185-225) and the two tsvdx6 binaries (src/bin/tsvdx6-300-new.rs,
+// src/bin/tsvdx6-300-v2-new.rs): the lm set with global neighbourhood terms
+// (and same-day terms for tsvdx6), sub-model weights w_bias = w_factor = w_nbr.
+bool find_job_nbr(const std::string& job, Job* out) {
+    struct B { const char* name; int f, epochs, max_nbr; float lr_wc; bool save_ifeat; int reset; bool lm, sub, each; };
+    static const B kBase[] = {
+        {"tsvdx5-20", 20, 20, 25, 0.002f, false, 1024, false, true, false},
+        {"tsvdx5-40", 40, 25, 25, 0.002f, false, 1024, false, true, false},
+        {"tsvdx5-60", 60, 23, 50, 0.002f, false, 1024, false, false, false},
+        {"tsvdx5-120", 120, 23, 100, 0.0015f, true, 1024, false, false, false},
+        {"tsvdx5-150", 150, 23, 100, 0.0015f, false, 1024, false, true, false},
+        {"tsvdx5-300", 300, 23, 100, 0.0015f, false, 10, false, false, true},
+        {"tsvdx5-400", 400, 26, 100, 0.0015f, false, 10, false, false, false},
+        {"tsvdx5-1000", 1000, 30, 100, 0.0015f, false, 10, false, false, false},
+        {"tsvdx5-1200lm", 1200, 23, 100, 0.0015f, false, 1024, true, true, true},
+    };
+    auto nbr = [](tx_config& c, int max_nbr, float lr_wc, float w) {
+        c.max_neighbors = max_nbr;
+        c.lr_w = c.lr_c = lr_wc;
+        c.reg_w = c.reg_c = 0.002f;
+        c.w_bias = c.w_factor = c.w_nbr = w;
+        c.lambda1 = 25.0f;
+        c.lambda2 = 10.0f;
+    };
+    for (const B& b : kBase) {
+        if (job != b.name) continue;
+        tx_config c = lm(b.f, b.epochs, 42, 32, b.save_ifeat, b.lm);
+        c.reset_u_epoch = b.reset;
+        nbr(c, b.max_nbr, b.lr_wc, 0.8f);
+        Opts o;
+        o.save_train = true;
+        o.save_subscores = b.sub;
+        o.save_probe_each_epoch = b.each;
+        *out = {c, o};
+        return true;
+    }
+    if (job == "tsvdx5-120o" || job == "tsvdx5-150o") {
+        const bool f120 = job == "tsvdx5-120o";
+        tx_config c = small_o(f120 ? 120 : 150);
+        c.n_epochs = f120 ? 28 : 25;
+        nbr(c, f120 ? 100 : 120, 0.0015f, f120 ? 0.8f : 1.2f);
+        Opts o;
+        o.save_train = o.save_subscores = o.save_probe_each_epoch = true;
+        *out = {c, o};
+        return true;
+    }
+    if (job == "tsvdx6-300" || job == "tsvdx6-300-v2") {
+        const bool v2 = job == "tsvdx6-300-v2";
+        tx_config c = lm(300, 23, 42, 32, false, false);
+        c.reset_u_epoch = 10;
+        nbr(c, 100, 0.0015f, 0.8f);
+        c.lr_w_day = c.lr_c_day = v2 ? 0.0008f : 0.001f;
+        c.reg_w_day = c.reg_c_day = 0.002f;
+        if (v2) c.w_nbr = 0.7f;
+        Opts o;
+        o.save_train = o.save_probe_each_epoch = !v2;
+        *out = {c, o};
+        return true;
+    }
+    return false;
+}
+
 // tsvdx4p-new job table (src/bin/tsvdx4p-new.rs:12-26, run_lm :30-85): the lm
 // set with lr_yb 2e-4, reset at epoch 1024, n^1.0 NSVD1 normalisation, 16
 // threads, low-memory; save_train jobs write train predictions, the others
@@ -921,6 +988,17 @@ bool find_eblend(const std::string& job, std::vector<double>* w, double* off) {
         {"tsvdx4-f1__epochs", {{0.127902, 0.128805, 0.129345, 0.129645, 0.129651}, 0.972784}},
         {"tsvdx4-q16tb__epochs",
          {{0.091397, 0.092004, 0.092560, 0.093097, 0.093596, 0.094026, 0.094293, 0.094378}, 0.702795}},
+        // tsvdx5-new run_eblend jobs (src/bin/tsvdx5-new.rs:104-128)
+        {"tsvdx5-120o__epochs",
+         {{0.027859, 0.028633, 0.029052, 0.029406, 0.029674, 0.030008, 0.030212, 0.030423, 0.030624, 0.030802,
+           0.030952, 0.031114, 0.031269, 0.031435, 0.031551, 0.031699, 0.031830, 0.031948, 0.032081, 0.032203,
+           0.032305, 0.032421, 0.032529, 0.032643, 0.032748, 0.032844, 0.032927, 0.033024},
+          0.381012}},
+        {"tsvdx5-300__epochs",
+         {{0.034800, 0.035376, 0.035884, 0.036167, 0.036416, 0.036645, 0.036883, 0.037075, 0.037255, 0.036797,
+           0.037004, 0.037221, 0.037410, 0.037604, 0.037767, 0.037958, 0.038098, 0.038248, 0.038387, 0.038531,
+           0.038679, 0.038773, 0.038903},
+          0.440369}},
         {"tsvdx4-400lm__epochs",
          {{0.039362, 0.039928, 0.040560, 0.040957, 0.041354, 0.041709, 0.042063, 0.042378, 0.042626, 0.042909,
            0.043160, 0.043400, 0.041114, 0.041567, 0.041924, 0.042314, 0.042636, 0.042944, 0.043212, 0.043466},
@@ -936,15 +1014,24 @@ bool find_eblend(const std::string& job, std::vector<double>* w, double* off) {
 }  // namespace
 
 int main(int argc, char** argv) {
-    if (argc < 2) {
+    const std::string self = std::string(argv[0]).substr(std::string(argv[0]).find_last_of('/') + 1);
+    if (argc < 2 && self != "tsvdx6-300-new" && self != "tsvdx6-300-v2-new") {
         std::fprintf(stderr, "usage: %s <job_name> [--print-config]\n", argv[0]);
         return 2;
     }
-    const std::string job = argv[1];
-    const bool print_only = argc > 2 && std::string(argv[2]) == "--print-config";
+    // the tsvdx6 binaries take no job argument (src/bin/tsvdx6-300-new.rs:3)
+    std::string prog = argv[0];
+    prog = prog.substr(prog.find_last_of('/') + 1);
+    std::string job = argc > 1 ? argv[1] : "";
+    int arg0 = 1;
+    if (prog == "tsvdx6-300-new" || prog == "tsvdx6-300-v2-new") {
+        job = prog == "tsvdx6-300-new" ? "tsvdx6-300" : "tsvdx6-300-v2";
+        arg0 = 0;
+    }
+    const bool print_only = argc > arg0 + 1 && std::string(argv[arg0 + 1]) == "--print-config";
     Job j;
     g_tsvdx4p = find_job_p(job, &j);
-    if (g_tsvdx4p || find_job(job, &j)) {
+    if (g_tsvdx4p || find_job(job, &j) || find_job_nbr(job, &j)) {
         if (print_only) {
             std::printf("%s\n", (g_tsvdx4p ? debug_config_p(j.cfg) : debug_config(j.cfg)).c_str());
             return 0;
@@ -985,7 +1072,7 @@ int main(int argc, char** argv) {
         return 0;
     }
     static const char* kKnn3[] = {"tsvdx4-d0tb__knn3", "tsvdx4-h8nb__knn3", "tsvdx4-n8nbo__knn3",
-                                  "tsvdx4-p0tbo__knn3"};
+                                  "tsvdx4-p0tbo__knn3", "tsvdx5-120o__knn3"};
     for (const char* name : kKnn3) {
         if (job != name) continue;
         if (print_only) return 0;

@@ -354,12 +354,15 @@ constexpr uint32_t kDayHead = 0x80000000u;
 // is forwarded in bud/but (has_bud/has_but set) rather than written into
 // sc/sc2: those registers may still be waiting for their load, and writing
 // them would stall the wave until every younger memory operation returned.
-template <int VEC, bool DAY>
+template <int VEC, bool DAY, int NB = 0>
 struct Pending {
     uint32_t i, ud, fb;
     float dev, r;
     int b, raw;
     float q[VEC], qf[VEC];
+    // neighbourhood terms: c(i, j) and w(i, j) of this lane's two global
+    // neighbour slots
+    float cv[NB ? 2 : 1], wv[NB ? 2 : 1];
     // day terms: p_u,day, the day's NSVD1 sum over y_day (strided) and the run norm
     float pd[DAY ? VEC : 1], sd[DAY ? VEC : 1];
     float nd;
@@ -403,8 +406,9 @@ struct Pending {
 //         Every wave computes the same scalars; the item-bias atomics and
 //         the shared NSVD1-bias writes are issued by wave 0 alone, user-owned
 //         slots by every wave (each wave's own later loads then see them).
-template <int TEAM, int VEC, bool QF, bool P2, bool ORD, int PASS, bool DAY, int MW>
+template <int TEAM, int VEC, bool QF, bool P2, bool ORD, int PASS, bool DAY, int MW, int NB = 0>
 __global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_sgd_par(DevModel m, StrataView sv, int stratum) {
+    static_assert(NB == 0 || (PASS == 0 && TEAM == 64), "neighbourhood terms: SGD epoch, one-wave lanes");
     static_assert(PASS == 0 || !ORD, "the user passes have no ordinal head");
     static_assert(PASS == 0 || !DAY, "the user passes train low-memory models");
     static_assert(!MW || TEAM == 64, "multi-wave teams: 64-lane waves");
@@ -497,8 +501,46 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_
             const float norm = m.norm_train[u];
             const size_t ubrow = static_cast<size_t>(u) * B;
 
+            // the user's global neighbours (src/tx.rs:762-796), this epoch's
+            // sample or every one: lane l holds neighbours l and l + 64 (at
+            // most 128): item, c scale (rated_scale or impl_scale), and for a
+            // rated one (r - b~) * rated_scale.  Multi-wave teams: wave 0 alone.
+            uint32_t nj[2] = {0u, 0u};
+            float ncs[2] = {0.0f, 0.0f}, nwc[2] = {0.0f, 0.0f};
+            bool nok[2] = {false, false}, nrt[2] = {false, false};
+            float nb_nn = 0.0f, nb_nr = 0.0f;
+            if constexpr (NB > 0) {
+                if (lead) {
+                    const uint32_t i0 = m.imp_start[u];
+                    const uint32_t nr_full = te - ts, ni_full = m.imp_start[u + 1] - i0;
+                    const uint32_t cw = m.smp_cnt[u];
+                    const bool smp = cw != 0xffffffffu;
+                    const uint32_t rb = smp ? (cw & 0xffffu) : nr_full, ib = smp ? (cw >> 16) : ni_full;
+                    const float rs = smp ? static_cast<float>(nr_full) / static_cast<float>(rb) : 1.0f;
+                    const float is = (smp && ib > 0) ? static_cast<float>(ni_full) / static_cast<float>(ib) : 1.0f;
+                    const uint16_t* pos = m.smp + static_cast<size_t>(u) * m.nb_max;
+#pragma unroll
+                    for (int q = 0; q < 2; ++q) {
+                        const uint32_t k = static_cast<uint32_t>(lane) + 64u * q;
+                        if (k < rb) {
+                            const uint32_t t2 = ts + (smp ? pos[k] : k);
+                            nj[q] = m.t_item[t2];
+                            nwc[q] = m.t_rmb[t2] * rs;
+                            ncs[q] = rs;
+                            nok[q] = nrt[q] = true;
+                        } else if (k < rb + ib) {
+                            nj[q] = m.imp_item[i0 + (smp ? pos[k] : k - rb)];
+                            ncs[q] = is;
+                            nok[q] = true;
+                        }
+                    }
+                    nb_nn = m.norm_nu[u];
+                    nb_nr = m.norm_ru[u];
+                }
+            }
+
             // issue the loads one rating needs, from its packed record
-            auto fetch = [&](uint4 rc, Pending<VEC, DAY>& n) {
+            auto fetch = [&](uint4 rc, Pending<VEC, DAY, NB>& n) {
                 n.i = rc.x & 0xFFFFFu;
                 n.raw = static_cast<int>((rc.x >> 20) & 15u);
                 n.b = static_cast<int>(rc.x >> 24);
@@ -536,6 +578,16 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_
                 if constexpr (DAY) {
                     load_str<VEC, TEAM>(m.ufeat_day + static_cast<size_t>(n.ud) * fs + off, n.pd);
                     load_str<VEC, TEAM>(sv.day_acc + static_cast<size_t>(n.ud) * fs + off, n.sd);
+                }
+                if constexpr (NB > 0) {
+                    // c(i, j) and w(i, j) of the lane's neighbour slots (empty
+                    // slots read column 0, never used)
+                    const size_t nrow = static_cast<size_t>(n.i) * static_cast<uint32_t>(m.I);
+#pragma unroll
+                    for (int q = 0; q < 2; ++q) {
+                        n.cv[q] = m.nc[nrow + nj[q]];
+                        n.wv[q] = m.nw[nrow + nj[q]];
+                    }
                 }
                 n.sc_addr = sc_base + static_cast<size_t>(n.i) * sc_imul + (static_cast<uint32_t>(n.b) & sc_bm) +
                             (n.ud & sc_um) + (n.fb & sc_fm);
@@ -644,6 +696,7 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_
             float Gb = first ? 0.0f : sv.gacc_b[u];
             float ubias = m.ubias[u], alpha = m.alpha_u[u], cu = m.cu[u];
 
+
             // records arrive TEAM at a time (lane l holds record tb + l); the two
             // batches after the current one are in flight, so the copies at a
             // batch boundary only touch registers loaded a batch or more ago
@@ -683,7 +736,7 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_
             // load lands in the slot consumed one step earlier, whose value is
             // dead: no register copy (and no wait for the load) at the back-edge.
             constexpr int RING = DEPTH + 1;
-            Pending<VEC, DAY> ring[RING];
+            Pending<VEC, DAY, NB> ring[RING];
             // day terms of the current same-day run (DAY): p_u,day, the day's
             // NSVD1 sum and the run's gradient sum, and the run's day
             float pud[VEC], sdr[VEC], Gd[VEC];
@@ -718,7 +771,7 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_
                 for (int k = 0; k < RING; ++k) {
                 const uint32_t t = t0 + k;
                 if (t >= s1) break;
-                const Pending<VEC, DAY>& cur = ring[k];
+                const Pending<VEC, DAY, NB>& cur = ring[k];
                 fetch(rec_at(t + DEPTH < s1 ? t + DEPTH : s1 - 1), ring[(k + DEPTH) % RING]);
                 if constexpr (DAY) ring[(k + DEPTH) % RING].nd = m.t_run_norm[t + DEPTH < s1 ? t + DEPTH : s1 - 1];
                 const uint32_t i = cur.i;
@@ -766,6 +819,42 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_
 #pragma unroll
                     for (int c = 0; c < VEC; ++c) part += (qv[c] + qfv[c]) * (p[c] + s[c] + dev * p2[c]);
                 }
+                // neighbourhood terms (src/tx.rs:903-956) join the dot product's
+                // lane partials: nn c(i,j) scale + nr (r - b~) w(i,j) scale per
+                // global neighbour j != i, then the same-day neighbours of the
+                // rating's day (NB 2: read now, counted for their norms)
+                float sd_nn = 0.0f, sd_nr = 0.0f;
+                uint32_t sd_a = 0, sd_b = 0;
+                if constexpr (NB > 0) {
+                    if (lead) {
+#pragma unroll
+                        for (int q = 0; q < 2; ++q)
+                            if (nok[q] && nj[q] != i) part += nb_nn * (ncs[q] * cur.cv[q]) + nb_nr * (nwc[q] * cur.wv[q]);
+                    }
+                }
+                if constexpr (NB == 2) {
+                    if (lead) {
+                        sd_a = m.sd_start[cur.ud];
+                        sd_b = m.sd_start[cur.ud + 1];
+                        const size_t irow = static_cast<size_t>(i) * static_cast<uint32_t>(m.I);
+                        float cds = 0.0f, wds = 0.0f, nt = 0.0f, nr = 0.0f;
+                        for (uint32_t e = sd_a + static_cast<uint32_t>(lane); e < sd_b; e += 64u) {
+                            const uint32_t it = m.sd_item[e], j = it & 0x7fffffffu;
+                            if (j == i) continue;
+                            nt += 1.0f;
+                            cds += m.nc_day[irow + j];
+                            if (it >> 31) {
+                                nr += 1.0f;
+                                wds += m.sd_rmb[e] * m.nw_day[irow + j];
+                            }
+                        }
+                        const int n_tot = static_cast<int>(team_sum_dpp<TEAM>(nt));
+                        const int n_rat = static_cast<int>(team_sum_dpp<TEAM>(nr));
+                        sd_nn = m.inv_sqrt_n[n_tot < m.n_isq ? n_tot : m.n_isq - 1];
+                        sd_nr = m.inv_sqrt_n[n_rat < m.n_isq ? n_rat : m.n_isq - 1];
+                        part += sd_nn * cds + sd_nr * wds;
+                    }
+                }
                 float dot = team_sum_dpp<TEAM>(part);
                 if constexpr (MW) {
                     // the waves' chunk sums, added in wave order by every wave, and
@@ -791,6 +880,38 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_
                 // the user passes weight bias and factor errors alike (src/tsvdx4p.rs:560-600)
                 const float err_b = PASS == 0 ? err * h.w_bias : err;
                 const float err_d = PASS == 0 ? err * h.w_factor : err;
+                if constexpr (NB > 0) {
+                    // neighbourhood steps (src/tx.rs:1038-1095) as deltas of the
+                    // values read for the forward pass
+                    if (lead) {
+                        const float err_n = err * h.w_nbr;
+                        const size_t irow = static_cast<size_t>(i) * static_cast<uint32_t>(m.I);
+#pragma unroll
+                        for (int q = 0; q < 2; ++q) {
+                            if (nok[q] && nj[q] != i) {
+                                if (nrt[q])
+                                    apply_delta(m.nw + irow + nj[q], cur.wv[q],
+                                                -h.lr_w * (err_n * nb_nr * nwc[q] + h.reg_w * cur.wv[q]), qm);
+                                apply_delta(m.nc + irow + nj[q], cur.cv[q],
+                                            -h.lr_c * (err_n * nb_nn * ncs[q] + h.reg_c * cur.cv[q]), qm);
+                            }
+                        }
+                        if constexpr (NB == 2) {
+                            for (uint32_t e = sd_a + static_cast<uint32_t>(lane); e < sd_b; e += 64u) {
+                                const uint32_t it = m.sd_item[e], j = it & 0x7fffffffu;
+                                if (j == i) continue;
+                                const size_t idx = irow + j;
+                                if (it >> 31) {
+                                    const float wd = m.nw_day[idx];
+                                    apply_delta(m.nw_day + idx, wd,
+                                                -h.lr_w_day * (err_n * sd_nr * m.sd_rmb[e] + h.reg_w_day * wd), qm);
+                                }
+                                const float cd = m.nc_day[idx];
+                                apply_delta(m.nc_day + idx, cd, -h.lr_c_day * (err_n * sd_nn + h.reg_c_day * cd), qm);
+                            }
+                        }
+                    }
+                }
 
                 // user-owned biases: registers, or written back now and forwarded
                 // to every prefetched rating that reads the same slot
@@ -800,7 +921,7 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_
                 const float bud_new = c_bud - h.lr_ubd * err_b;
 #pragma unroll
                 for (int jj = 1; jj <= DEPTH; ++jj) {
-                    Pending<VEC, DAY>& nx = ring[(k + jj) % RING];  // ratings t+1 .. t+DEPTH
+                    Pending<VEC, DAY, NB>& nx = ring[(k + jj) % RING];  // ratings t+1 .. t+DEPTH
                     if (nx.b == cur.b) { nx.has_but = true; nx.but = but_new; }
                     if (nx.ud == cur.ud) { nx.has_bud = true; nx.bud = bud_new; }
                 }
@@ -1036,6 +1157,36 @@ inline hipError_t sgd_par_tvp(const DevModel& m, const StrataView& sv, int strat
     if (qf) { if (p2) TX_PAR(true, true, false); else TX_PAR(true, false, false); }
     else { if (p2) TX_PAR(false, true, false); else TX_PAR(false, false, false); }
 #undef TX_PAR
+    return hipGetLastError();
+}
+
+// neighbourhood models (NB 1: global w/c, NB 2: plus same-day w_day/c_day):
+// q_f and p2 always on (all-zero tables leave every value as without them),
+// ORD chosen at run time
+template <int TEAM, int VEC, bool DAY, int NB>
+inline hipError_t sgd_par_nb_tvp(const DevModel& m, const StrataView& sv, int stratum, int blocks, bool ord,
+                                 hipStream_t st) {
+    (void)hipGetLastError();
+    const dim3 bt(sv.block_threads > 0 ? sv.block_threads : 256);
+    if (ord)
+        hipLaunchKernelGGL((k_sgd_par<TEAM, VEC, true, true, true, 0, DAY, 0, NB>), dim3(blocks), bt, 0, st, m, sv,
+                           stratum);
+    else
+        hipLaunchKernelGGL((k_sgd_par<TEAM, VEC, true, true, false, 0, DAY, 0, NB>), dim3(blocks), bt, 0, st, m, sv,
+                           stratum);
+    return hipGetLastError();
+}
+
+// the same with multi-wave teams (f > 1024, no ordinal head)
+template <bool DAY, int NB>
+inline hipError_t sgd_par_nb_mw(const DevModel& m, const StrataView& sv, int teams, int waves, hipStream_t st) {
+    (void)hipGetLastError();
+    if (waves <= 4)
+        hipLaunchKernelGGL((k_sgd_par<64, 16, true, true, false, 0, DAY, 4, NB>), dim3(teams), dim3(64 * waves), 0, st,
+                           m, sv, 0);
+    else
+        hipLaunchKernelGGL((k_sgd_par<64, 16, true, true, false, 0, DAY, 8, NB>), dim3(teams), dim3(64 * waves), 0, st,
+                           m, sv, 0);
     return hipGetLastError();
 }
 

@@ -172,6 +172,20 @@ struct tx_ctx {
     hipGraphExec_t graph = nullptr;
     int graph_S = 0;
 
+    // neighbourhood terms (tx5/tx6): |N(u)\R(u)| per user, and the epoch's
+    // global neighbour samples drawn on the host (pinned) for the device
+    // (two pinned buffers: the next epoch's samples are drawn while the GPU
+    // runs the current epoch)
+    bool nb = false, nbd = false;
+    std::vector<uint32_t> imp_cnt;
+    uint32_t* h_smp_cnt[2] = {nullptr, nullptr};
+    uint16_t* h_smp[2] = {nullptr, nullptr};
+    int32_t smp_buf_epoch[2] = {-1, -1};  // epoch drawn into each host buffer
+    int smp_cur = 0;                       // host buffer last uploaded
+    uint32_t* d_smp_cnt = nullptr;
+    uint16_t* d_smp = nullptr;
+    int32_t smp_epoch = -1;  // epoch whose samples are on the device
+
     std::vector<uint32_t> h_queue;  // pinned order of the current epoch's queue
     // the next epoch's queue, built on the host while the GPU runs this one
     std::vector<uint32_t> h_queue_next;
@@ -233,6 +247,10 @@ struct tx_ctx {
             if (g->st) (void)hipStreamDestroy(g->st);
         }
         for (void* p : allocs) (void)hipFree(p);
+        for (int b = 0; b < 2; ++b) {
+            if (h_smp[b]) (void)hipHostFree(h_smp[b]);
+            if (h_smp_cnt[b]) (void)hipHostFree(h_smp_cnt[b]);
+        }
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
         if (st) (void)hipStreamDestroy(st);
@@ -268,6 +286,9 @@ tx::Hyper make_hyper(const tx_config& c) {
     h.w_bias = c.w_bias; h.w_factor = c.w_factor;
     h.beta = c.beta;
     h.lr_t = c.lr_t; h.reg_t = c.reg_t;
+    h.lr_w = c.lr_w; h.lr_c = c.lr_c; h.reg_w = c.reg_w; h.reg_c = c.reg_c;
+    h.lr_w_day = c.lr_w_day; h.lr_c_day = c.lr_c_day; h.reg_w_day = c.reg_w_day; h.reg_c_day = c.reg_c_day;
+    h.w_nbr = c.w_nbr;
     return h;
 }
 
@@ -360,6 +381,10 @@ ParamDesc param_desc(tx_ctx* c, int id) {
     case TX_P_CUT: return {m.cut, D, 1, 1};
     case TX_P_GBIAS: return {nullptr, 1, 1, 1};
     case TX_P_THRESHOLDS: return {m.thresholds, 4, 1, 1};
+    case TX_P_NB_W: return {m.nw, c->nb ? I : 0, I, I};
+    case TX_P_NB_C: return {m.nc, c->nb ? I : 0, I, I};
+    case TX_P_NB_W_DAY: return {m.nw_day, c->nbd ? I : 0, I, I};
+    case TX_P_NB_C_DAY: return {m.nc_day, c->nbd ? I : 0, I, I};
     default: return {nullptr, -1, 0, 0};
     }
 }
@@ -618,6 +643,92 @@ int prepare_user_queue(tx_ctx* c, int32_t epoch, hipStream_t st) {
 // (ByItemIndex, src/tsvdx4p.rs:29-57) with the user, bins, dev and target of
 // each; items queued most ratings first; NSVD1 accumulators; per-item scale
 // n_contrib^nsvd_norm_exp (src/tsvdx4p.rs:356-365) with the host libm powf.
+// The epoch's global neighbour samples (src/tx.rs:750-793): one
+// StdRng::seed_from_u64(seed + epoch * 1000003) stream, consumed user by user
+// in the epoch's order (get_users); a user whose |R(u)| + |N(u)\R(u)|
+// exceeds max_neighbors keeps round(max * |R| / n) (at least 1, at most |R|)
+// of its R(u) positions and the rest of the budget of its N(u)\R(u)
+// positions, each set the prefix of a full shuffle, sorted.  The stream is
+// sequential, so its draws are taken in one pass over the users in order
+// (no divisions: shuffle_draws) and turned into the shuffles in parallel over
+// users (shuffle_apply), into host buffer `buf`.
+int draw_nbr_samples(tx_ctx* c, int32_t epoch, int buf) {
+    const int64_t U = c->U;
+    const int64_t mx = c->cfg.max_neighbors;
+    uint32_t* cnt = c->h_smp_cnt[buf];
+    for (int64_t u = 0; u < U; ++u) cnt[u] = 0xffffffffu;
+    if (mx > 0) {
+        std::vector<int64_t> order(U);
+        tx::user_order(U, c->cfg.shuffle_users != 0, c->cfg.seed, epoch, order.data());
+        tx::ChaChaRng rng(c->cfg.seed + static_cast<uint64_t>(epoch) * 1000003ull);
+        rng.set_bulk(true);
+        std::vector<uint32_t> pk;
+        std::vector<int64_t> su, pk_at;  // sampled users and the offset of their draws
+        pk.reserve(static_cast<size_t>(c->N / 2 + 1024));
+        for (int64_t k = 0; k < U; ++k) {
+            const int64_t u = order[k];
+            const int64_t nr = c->t_off[u + 1] - c->t_off[u], ni = c->imp_cnt[u];
+            if (nr == 0 || nr + ni <= mx) continue;
+            su.push_back(u);
+            pk_at.push_back(static_cast<int64_t>(pk.size()));
+            tx::shuffle_draws(rng, nr, pk);
+            tx::shuffle_draws(rng, ni, pk);
+        }
+        parallel_for(static_cast<int64_t>(su.size()), [&](int64_t lo, int64_t hi) {
+            std::vector<int64_t> ir, ii;
+            for (int64_t k = lo; k < hi; ++k) {
+                const int64_t u = su[k];
+                const int64_t nr = c->t_off[u + 1] - c->t_off[u], ni = c->imp_cnt[u], n = nr + ni;
+                int64_t rb = static_cast<int64_t>(std::round(static_cast<double>(mx) * static_cast<double>(nr) /
+                                                             static_cast<double>(n)));
+                rb = std::min<int64_t>(std::max<int64_t>(rb, 1), nr);
+                const int64_t ib = std::min<int64_t>(mx - rb, ni);
+                const uint32_t* p = pk.data() + pk_at[k];
+                int64_t ng = 0;  // groups of the first shuffle
+                tx::shuffle_groups(nr, [&](int64_t, uint32_t, int64_t, uint32_t) { ++ng; });
+                ir.resize(nr);
+                std::iota(ir.begin(), ir.end(), int64_t{0});
+                tx::shuffle_apply(p, ir.data(), nr);
+                std::sort(ir.begin(), ir.begin() + rb);
+                ii.resize(ni);
+                std::iota(ii.begin(), ii.end(), int64_t{0});
+                tx::shuffle_apply(p + ng, ii.data(), ni);
+                std::sort(ii.begin(), ii.begin() + ib);
+                uint16_t* out = c->h_smp[buf] + u * mx;
+                for (int64_t q = 0; q < rb; ++q) out[q] = static_cast<uint16_t>(ir[q]);
+                for (int64_t q = 0; q < ib; ++q) out[rb + q] = static_cast<uint16_t>(ii[q]);
+                cnt[u] = static_cast<uint32_t>(rb) | (static_cast<uint32_t>(ib) << 16);
+            }
+        });
+    }
+    c->smp_buf_epoch[buf] = epoch;
+    return 0;
+}
+
+// The epoch's samples on the device (drawn now unless a host buffer holds
+// them already), copied on the ctx's stream ahead of the epoch's kernels.
+int upload_nbr_samples(tx_ctx* c, int32_t epoch, hipStream_t st) {
+    int b = c->smp_buf_epoch[0] == epoch ? 0 : (c->smp_buf_epoch[1] == epoch ? 1 : -1);
+    if (b < 0) {
+        b = 1 - c->smp_cur;
+        if (int rc = draw_nbr_samples(c, epoch, b)) return rc;
+    }
+    const int64_t mx = c->cfg.max_neighbors;
+    if (mx > 0)
+        HIPC(hipMemcpyAsync(c->d_smp, c->h_smp[b], sizeof(uint16_t) * c->U * mx, hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(c->d_smp_cnt, c->h_smp_cnt[b], sizeof(uint32_t) * c->U, hipMemcpyHostToDevice, st));
+    c->smp_cur = b;
+    c->smp_epoch = epoch;
+    return 0;
+}
+
+// While the GPU runs epoch `epoch`, draw the next epoch's samples into the
+// host buffer not being copied.
+int draw_next_nbr_samples(tx_ctx* c, int32_t epoch) {
+    if (!c->nb || c->smp_buf_epoch[1 - c->smp_cur] == epoch + 1) return 0;
+    return draw_nbr_samples(c, epoch + 1, 1 - c->smp_cur);
+}
+
 int build_item_view(tx_ctx* c) {
     if (c->iv_built) return 0;
     const int64_t I = c->I, N = c->N;
@@ -800,8 +911,14 @@ int tx_create(const tx_config* cfg, const tx_ratings* tr, const tx_ratings* pr, 
     g_err.clear();
     if (!cfg || !out) return fail(TX_EINVAL, "tx_create: NULL argument");
     *out = nullptr;
-    if (cfg->lr_w != 0.0f || cfg->lr_c != 0.0f || cfg->lr_w_day != 0.0f || cfg->lr_c_day != 0.0f)
-        return fail(TX_EUNSUPPORTED, "tx_create: neighbourhood terms (tsvdx5/6) are not part of the tsvdx4 path");
+    const bool build_nbr = cfg->lr_w > 0.0f || cfg->lr_c > 0.0f;          // src/tx.rs:483-484
+    const bool build_nbr_day = cfg->lr_w_day > 0.0f || cfg->lr_c_day > 0.0f;
+    if (build_nbr && tr->n_ratings > 0 && !tr->raw_ratings)
+        return fail(TX_EINVAL, "tx_create: the neighbourhood terms need train raw_ratings (r - b~)");
+    if ((build_nbr || build_nbr_day) && cfg->max_neighbors < 0)
+        return fail(TX_EINVAL, "tx_create: max_neighbors >= 0 required");
+    if ((build_nbr || build_nbr_day) && static_cast<int64_t>(tr->n_items) * tr->n_items > (int64_t{1} << 32))
+        return fail(TX_EUNSUPPORTED, "tx_create: the neighbourhood tables need n_items^2 <= 2^32");
     if (cfg->n_feat < 0 || cfg->n_time_bins < 1 || cfg->n_freq_bins < 1 || cfg->n_epochs < 0)
         return fail(TX_EINVAL, "tx_create: n_feat >= 0, n_time_bins >= 1, n_freq_bins >= 1 required");
     if (int rc = check_ratings(tr, "train", true)) return rc;
@@ -956,6 +1073,9 @@ int tx_create(const tx_config* cfg, const tx_ratings* tr, const tx_ratings* pr, 
             }
     // day terms train in the parallel schedule when the ratings are date-ordered
     if (!cfg->low_memory && !c->days_sorted) c->sched.mode = TX_SCHED_SERIAL;
+    // neighbourhood terms: in parallel with 1..128 global neighbours
+    if (build_nbr && (cfg->max_neighbors < 1 || cfg->max_neighbors > 128 || (shape.waves > 1 && cfg->has_ordinal)))
+        c->sched.mode = TX_SCHED_SERIAL;
 
     clk.mark("freq bins, date order");
     // norms: training su (src/tx.rs:808-819) and epoch-end rebuild (src/tx.rs:410-416)
@@ -1031,6 +1151,125 @@ int tx_create(const tx_config* cfg, const tx_ratings* tr, const tx_ratings* pr, 
     }
 
     clk.mark("day-term structures");
+    // neighbourhood structures (src/tx.rs:483-585): b~_i and b~_u by FP64 sums
+    // over the training rows, r - b~ per training rating, N(u)\R(u) per user
+    // (held-out items the user has not rated in training, sorted by item, or
+    // by (item, day) with same-day terms, and deduplicated on that key), the
+    // two norms, and for the same-day terms a list per SparseUD day: the
+    // user's training items of that day, then its N(u)\R(u) items of that day
+    c->nb = build_nbr;
+    c->nbd = build_nbr_day;
+    std::vector<float> t_rmb, norm_nu, norm_ru, inv_sqrt_n, sd_rmb;
+    std::vector<uint32_t> imp_start, imp_item, sd_start, sd_item;
+    std::vector<int16_t> imp_day;
+    if (build_nbr) {
+        const double mu = static_cast<double>(gbias);
+        std::vector<double> isum(I, 0.0), icnt(I, 0.0), usum(U, 0.0), ucnt(U, 0.0);
+        for (int64_t t = 0; t < N; ++t) {
+            isum[tr->item_idxs[t]] += static_cast<double>(tr->raw_ratings[t]) - mu;
+            icnt[tr->item_idxs[t]] += 1.0;
+        }
+        std::vector<float> bti(I), btu(U);
+        for (int64_t i = 0; i < I; ++i) bti[i] = static_cast<float>(isum[i] / (icnt[i] + static_cast<double>(cfg->lambda1)));
+        for (int64_t t = 0; t < N; ++t) {
+            usum[tr->user_idxs[t]] += static_cast<double>(tr->raw_ratings[t]) - mu - static_cast<double>(bti[tr->item_idxs[t]]);
+            ucnt[tr->user_idxs[t]] += 1.0;
+        }
+        for (int64_t u = 0; u < U; ++u) btu[u] = static_cast<float>(usum[u] / (ucnt[u] + static_cast<double>(cfg->lambda2)));
+        t_rmb.resize(N);
+        parallel_for(N, [&](int64_t lo, int64_t hi) {
+            for (int64_t t = lo; t < hi; ++t) {
+                const float btilde = gbias + bti[tr->item_idxs[t]] + btu[tr->user_idxs[t]];
+                t_rmb[t] = static_cast<float>(tr->raw_ratings[t]) - btilde;
+            }
+        });
+        // held-out rows by user with their days (file order), then the filter
+        std::vector<int16_t> h_day(NH);
+        {
+            std::vector<uint32_t> pos(c->h_start.begin(), c->h_start.end() - 1);
+            for (int64_t t = 0; t < NH; ++t) h_day[pos[pr->user_idxs[t]]++] = pr->dates[t];
+        }
+        std::vector<uint32_t> keep(NH);  // item | day bits of the kept rows, per user from h_start[u]
+        c->imp_cnt.assign(U, 0);
+        parallel_for(U, [&](int64_t lo, int64_t hi) {
+            std::vector<uint32_t> rated;
+            std::vector<std::pair<uint32_t, int16_t>> v;
+            for (int64_t u = lo; u < hi; ++u) {
+                rated.assign(c->t_item.begin() + c->t_off[u], c->t_item.begin() + c->t_off[u + 1]);
+                std::sort(rated.begin(), rated.end());
+                v.clear();
+                for (uint32_t k = c->h_start[u]; k < c->h_start[u + 1]; ++k)
+                    if (!std::binary_search(rated.begin(), rated.end(), h_item[k])) v.emplace_back(h_item[k], h_day[k]);
+                if (build_nbr_day) {
+                    std::sort(v.begin(), v.end());
+                    v.erase(std::unique(v.begin(), v.end()), v.end());
+                } else {
+                    // key = item only: which day a duplicate keeps is irrelevant
+                    // (days are read by the same-day terms only)
+                    std::sort(v.begin(), v.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+                    v.erase(std::unique(v.begin(), v.end(), [](const auto& a, const auto& b) { return a.first == b.first; }),
+                            v.end());
+                }
+                uint32_t w = c->h_start[u];
+                for (const auto& e : v) {
+                    keep[w] = e.first;
+                    h_day[w] = e.second;
+                    ++w;
+                }
+                c->imp_cnt[u] = static_cast<uint32_t>(v.size());
+            }
+        });
+        imp_start.assign(U + 1, 0);
+        for (int64_t u = 0; u < U; ++u) imp_start[u + 1] = imp_start[u] + c->imp_cnt[u];
+        imp_item.resize(imp_start[U]);
+        imp_day.resize(imp_start[U]);
+        for (int64_t u = 0; u < U; ++u)
+            for (uint32_t k = 0; k < c->imp_cnt[u]; ++k) {
+                imp_item[imp_start[u] + k] = keep[c->h_start[u] + k];
+                imp_day[imp_start[u] + k] = h_day[c->h_start[u] + k];
+            }
+        // norms with the host powf, as the reference (src/tx.rs:577-583)
+        norm_nu.resize(U);
+        norm_ru.resize(U);
+        uint32_t maxn = 0;
+        for (int64_t u = 0; u < U; ++u) {
+            const uint32_t n_r = c->t_off[u + 1] - c->t_off[u], n_n = n_r + c->imp_cnt[u];
+            norm_ru[u] = n_r > 0 ? powf(static_cast<float>(n_r), -0.5f) : 0.0f;
+            norm_nu[u] = n_n > 0 ? powf(static_cast<float>(n_n), -0.5f) : 0.0f;
+            maxn = std::max(maxn, n_n);
+            if (n_r > 0xffffu || c->imp_cnt[u] > 0xffffu)
+                return fail(TX_EUNSUPPORTED, "tx_create: a user with 65536 or more training or N(u)\\R(u) items");
+        }
+        // same-day counts k -> k^-0.5 (src/tx.rs:925-926)
+        inv_sqrt_n.resize(static_cast<size_t>(maxn) + 2);
+        for (size_t k = 0; k < inv_sqrt_n.size(); ++k) inv_sqrt_n[k] = k > 0 ? powf(static_cast<float>(k), -0.5f) : 0.0f;
+        if (build_nbr_day) {
+            sd_start.assign(D + 1, 0);
+            for (int64_t t = 0; t < N; ++t) sd_start[c->t_ud[t] + 1]++;
+            std::vector<uint32_t> imp_ud(imp_item.size());
+            for (int64_t u = 0; u < U; ++u)
+                for (uint32_t k = imp_start[u]; k < imp_start[u + 1]; ++k) {
+                    const int64_t d = sud_index(u, imp_day[k]);
+                    imp_ud[k] = static_cast<uint32_t>(d);
+                    sd_start[d + 1]++;
+                }
+            for (int64_t d = 0; d < D; ++d) sd_start[d + 1] += sd_start[d];
+            sd_item.resize(sd_start[D]);
+            sd_rmb.resize(sd_start[D]);
+            std::vector<uint32_t> pos(sd_start.begin(), sd_start.end() - 1);
+            for (int64_t t = 0; t < N; ++t) {
+                const uint32_t w = pos[c->t_ud[t]]++;
+                sd_item[w] = c->t_item[t] | 0x80000000u;
+                sd_rmb[w] = t_rmb[t];
+            }
+            for (size_t k = 0; k < imp_item.size(); ++k) {
+                const uint32_t w = pos[imp_ud[k]]++;
+                sd_item[w] = imp_item[k];
+                sd_rmb[w] = 0.0f;
+            }
+        }
+    }
+    clk.mark("neighbourhood structures");
     // initial factors in the reference's draw order (src/tx.rs:606-630):
     // ufeat, ifeat, yfeat, [yfeat_day], ifeat_freq
     tx::ChaChaRng rng(cfg->seed);
@@ -1159,6 +1398,39 @@ int tx_create(const tx_config* cfg, const tx_ratings* tr, const tx_ratings* pr, 
         RC(c->upload(&p32, dy_off.data(), dy_off.size())); m.dy_off = p32;
         RC(c->upload(&p32, dy.data(), dy.size())); m.dy_item = p32;
     }
+    // neighbourhood terms: lists, norms and the dense I x I tables (zero)
+    m.nb_on = build_nbr ? 1 : 0;
+    m.nbd_on = (build_nbr && build_nbr_day) ? 1 : 0;
+    m.nb_max = cfg->max_neighbors;
+    if (build_nbr) {
+        RC(c->upload(&pf, t_rmb.data(), t_rmb.size())); m.t_rmb = pf;
+        RC(c->upload(&p32, imp_start.data(), imp_start.size())); m.imp_start = p32;
+        RC(c->upload(&p32, imp_item.data(), imp_item.size())); m.imp_item = p32;
+        RC(c->upload(&p16, imp_day.data(), imp_day.size())); m.imp_day = p16;
+        RC(c->upload(&pf, norm_nu.data(), norm_nu.size())); m.norm_nu = pf;
+        RC(c->upload(&pf, norm_ru.data(), norm_ru.size())); m.norm_ru = pf;
+        RC(c->upload(&pf, inv_sqrt_n.data(), inv_sqrt_n.size())); m.inv_sqrt_n = pf;
+        m.n_isq = static_cast<int32_t>(inv_sqrt_n.size());
+        RC(c->upload(&p32, sd_start.data(), sd_start.size())); m.sd_start = p32;
+        RC(c->upload(&p32, sd_item.data(), sd_item.size())); m.sd_item = p32;
+        RC(c->upload(&pf, sd_rmb.data(), sd_rmb.size())); m.sd_rmb = pf;
+        RC(zeros(c, &m.nw, I * I));
+        RC(zeros(c, &m.nc, I * I));
+        // the epoch's samples: U words of counts and U x max_neighbors positions
+        const int64_t slots = U * std::max<int64_t>(cfg->max_neighbors, 1);
+        for (int b = 0; b < 2; ++b) {
+            HIPC(hipHostMalloc(reinterpret_cast<void**>(&c->h_smp_cnt[b]), sizeof(uint32_t) * std::max<int64_t>(U, 1)));
+            HIPC(hipHostMalloc(reinterpret_cast<void**>(&c->h_smp[b]), sizeof(uint16_t) * std::max<int64_t>(slots, 1)));
+        }
+        RC(c->alloc(&c->d_smp_cnt, static_cast<size_t>(U)));
+        RC(c->alloc(&c->d_smp, static_cast<size_t>(slots)));
+        m.smp_cnt = c->d_smp_cnt;
+        m.smp = c->d_smp;
+    }
+    if (build_nbr_day) {
+        RC(zeros(c, &m.nw_day, I * I));
+        RC(zeros(c, &m.nc_day, I * I));
+    }
     RC(c->upload(&pf, norm_train.data(), norm_train.size())); m.norm_train = pf;
     RC(c->upload(&pf, norm_rebuild.data(), norm_rebuild.size())); m.norm_rebuild = pf;
     RC(c->alloc(&c->d_order, static_cast<size_t>(U)));
@@ -1211,6 +1483,16 @@ int tx_set_schedule(tx_ctx* c, const tx_schedule* s) {
         if (s->blocks < 0 || (s->blocks > 0 && s->blocks < s->n_strata))
             return fail(TX_EINVAL, "blocks must be 0 (auto) or at least n_strata");
     }
+    if (c->nb && s->mode != TX_SCHED_SERIAL) {
+        // the neighbourhood terms train in the serial schedule or the
+        // one-stratum parallel one (two global neighbours per lane)
+        if (s->mode != TX_SCHED_STRATA || s->n_strata != 1)
+            return fail(TX_EUNSUPPORTED, "the neighbourhood terms train in TX_SCHED_SERIAL or one-stratum TX_SCHED_STRATA");
+        if (c->cfg.max_neighbors < 1 || c->cfg.max_neighbors > 128)
+            return fail(TX_EUNSUPPORTED, "the parallel schedule holds at most 128 global neighbours (max_neighbors 1..128)");
+        if (c->shape.waves > 1 && c->cfg.has_ordinal)
+            return fail(TX_EUNSUPPORTED, "neighbourhood terms with an ordinal head train in parallel up to n_feat 1024");
+    }
     if (s->user_order < 0 || s->user_order > 2) return fail(TX_EINVAL, "user_order must be 0, 1 or 2");
     if (s->teams_per_block < 0 || s->teams_per_block > 4)
         return fail(TX_EINVAL, "teams_per_block must be 0 (4) or 1..4, got %d", s->teams_per_block);
@@ -1252,6 +1534,8 @@ int tx_fit_epoch(tx_ctx* c, int32_t epoch) {
         HIPC(tx::launch_fill_f32(c->dm.ufeat, 0.0f, static_cast<int64_t>(c->U) * c->fs, st));
         HIPC(tx::launch_fill_f32(c->dm.ufeat2, 0.0f, static_cast<int64_t>(c->U) * c->fs, st));
     }
+    if (c->nb && c->smp_epoch != epoch)
+        if (int rc = upload_nbr_samples(c, epoch, st)) return rc;
     bool seq_epoch = false;  // Tsvdx4pConfig::seq_epochs run the TxModel epoch (src/tsvdx4p.rs:799-800)
     for (int k = 0; k < 5; ++k) seq_epoch = seq_epoch || c->cfg.seq_epochs[k] == epoch;
     if (c->sched.mode == TX_SCHED_TSVDX4P && !seq_epoch) {
@@ -1291,6 +1575,7 @@ int tx_fit_epoch(tx_ctx* c, int32_t epoch) {
         HIPC(hipEventRecord(c->ev[2], st));
         HIPC(tx::launch_rebuild(c->dm, c->n_ud, c->shape.team, c->shape.vec, st));
         HIPC(hipEventRecord(c->ev[3], st));
+        if (int rc = draw_next_nbr_samples(c, epoch)) return rc;
         HIPC(hipEventSynchronize(c->ev[3]));
         float a = 0, b = 0, t = 0;
         HIPC(hipEventElapsedTime(&a, c->ev[1], c->ev[2]));
@@ -1326,6 +1611,7 @@ int tx_fit_epoch(tx_ctx* c, int32_t epoch) {
             HIPC(hipGraphLaunch(c->graph, st));
             HIPC(hipEventRecord(c->ev[3], st));
             if (c->built_S == 1 && c->sched.user_order != 1) build_next_queue(c, epoch + 1);
+            if (int rc = draw_next_nbr_samples(c, epoch)) return rc;
             HIPC(hipEventSynchronize(c->ev[3]));
             float a = 0, t = 0;
             HIPC(hipEventElapsedTime(&a, c->ev[1], c->ev[3]));
@@ -1353,6 +1639,7 @@ int tx_fit_epoch(tx_ctx* c, int32_t epoch) {
             HIPC(tx::launch_rebuild(c->dm, c->n_ud, c->shape.team, c->shape.vec, st));
             HIPC(hipEventRecord(c->ev[3], st));
             if (c->built_S == 1 && c->sched.user_order != 1) build_next_queue(c, epoch + 1);
+            if (int rc = draw_next_nbr_samples(c, epoch)) return rc;
             HIPC(hipEventSynchronize(c->ev[3]));
             float a = 0, b = 0, t = 0;
             HIPC(hipEventElapsedTime(&a, c->ev[1], c->ev[2]));
@@ -1378,8 +1665,8 @@ int tx_fit_users_epoch(tx_ctx* c, int32_t epoch) {
     if (!c) return fail(TX_EINVAL, "tx_fit_users_epoch: NULL ctx");
     c->bf16_stale = true;
     if (epoch < 1) return fail(TX_EINVAL, "tx_fit_users_epoch: epochs are 1-based");
-    if (!c->cfg.low_memory || c->cfg.has_ordinal)
-        return fail(TX_EUNSUPPORTED, "tx_fit_users_epoch: low-memory models without ordinal head only");
+    if (!c->cfg.low_memory || c->cfg.has_ordinal || c->nb)
+        return fail(TX_EUNSUPPORTED, "tx_fit_users_epoch: low-memory models without ordinal head or neighbourhood terms");
     if (c->built_S > 1) return fail(TX_EUNSUPPORTED, "tx_fit_users_epoch: the model uses a stratified layout");
     if (c->built_S == 0)
         if (int rc = build_strata(c, 1)) return rc;
@@ -1556,7 +1843,7 @@ int tx_predict_device(tx_ctx* c, int64_t n, const int32_t* d_u, const int32_t* d
     if (!c || n < 0 || (n > 0 && (!d_u || !d_i || !d_d || !d_out))) return fail(TX_EINVAL, "tx_predict_device: bad arguments");
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->st;
     if (int rc = ensure_bf16(c)) return rc;
-    HIPC(tx::launch_predict(c->dm, 0, n, d_u, d_i, d_d, d_out, nullptr, nullptr, nullptr, nullptr, nullptr,
+    HIPC(tx::launch_predict(c->dm, 0, n, d_u, d_i, d_d, d_out, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
                             c->shape.team, c->shape.vec, st));
     return 0;
 }
@@ -1582,16 +1869,16 @@ struct Staging {
 
 size_t round256(size_t bytes) { return (bytes + 255) / 256 * 256; }
 
-// Bytes of staging for `rows` queries: user, item (i32), day (i16), three
+// Bytes of staging for `rows` queries: user, item (i32), day (i16), four
 // float outputs (the first doubles as calc_rmse's target), is_test (i8).
 size_t staging_bytes(int64_t rows) {
     const size_t r = static_cast<size_t>(rows);
-    return round256(r * 4) * 2 + round256(r * 2) + round256(r * 4) * 3 + round256(r);
+    return round256(r * 4) * 2 + round256(r * 2) + round256(r * 4) * 4 + round256(r);
 }
 
 // Host-buffer predict / subscores / rmse in chunks through `sg`.
 int run_host(tx_ctx* c, const Staging& sg, int mode, int64_t n, const int32_t* u, const int32_t* i, const int16_t* d,
-             float* o0, float* o1, float* o2, const float* target, const int8_t* is_test, double* acc) {
+             float* o0, float* o1, float* o2, float* o3, const float* target, const int8_t* is_test, double* acc) {
     const int64_t chunk = std::min<int64_t>(std::max<int64_t>(n, 1), sg.rows);
     char* base = sg.base;
     auto carve = [&](size_t bytes) {
@@ -1605,6 +1892,7 @@ int run_host(tx_ctx* c, const Staging& sg, int mode, int64_t n, const int32_t* u
     float* d0 = reinterpret_cast<float*>(carve(chunk * 4));
     float* d1 = reinterpret_cast<float*>(carve(chunk * 4));
     float* d2 = reinterpret_cast<float*>(carve(chunk * 4));
+    float* d3 = reinterpret_cast<float*>(carve(chunk * 4));
     int8_t* dt = reinterpret_cast<int8_t*>(carve(chunk));
     hipStream_t st = sg.st;
     if (mode == 2) HIPC(tx::launch_fill_f64(sg.acc, 0.0, 2, st));
@@ -1617,13 +1905,14 @@ int run_host(tx_ctx* c, const Staging& sg, int mode, int64_t n, const int32_t* u
             HIPC(hipMemcpyAsync(d0, target + lo, m * 4, hipMemcpyHostToDevice, st));
             if (is_test) HIPC(hipMemcpyAsync(dt, is_test + lo, m, hipMemcpyHostToDevice, st));
         }
-        HIPC(tx::launch_predict(c->dm, mode, m, du, di, dd, d0, d1, d2, mode == 2 ? d0 : nullptr,
+        HIPC(tx::launch_predict(c->dm, mode, m, du, di, dd, d0, d1, d2, d3, mode == 2 ? d0 : nullptr,
                                 (mode == 2 && is_test) ? dt : nullptr, sg.acc, c->shape.team, c->shape.vec, st));
         if (mode == 0) HIPC(hipMemcpyAsync(o0 + lo, d0, m * 4, hipMemcpyDeviceToHost, st));
         if (mode == 1) {
             HIPC(hipMemcpyAsync(o0 + lo, d0, m * 4, hipMemcpyDeviceToHost, st));
             HIPC(hipMemcpyAsync(o1 + lo, d1, m * 4, hipMemcpyDeviceToHost, st));
             HIPC(hipMemcpyAsync(o2 + lo, d2, m * 4, hipMemcpyDeviceToHost, st));
+            if (o3) HIPC(hipMemcpyAsync(o3 + lo, d3, m * 4, hipMemcpyDeviceToHost, st));
         }
         HIPC(hipStreamSynchronize(st));
     }
@@ -1680,14 +1969,14 @@ int checkout(tx_ctx* c, int64_t n, tx_ctx::Stager** out) {
 
 // run_host through a stager of the pool.
 int run_staged(tx_ctx* c, int mode, int64_t n, const int32_t* u, const int32_t* i, const int16_t* d, float* o0,
-               float* o1, float* o2, const float* target, const int8_t* is_test, double* acc) {
+               float* o1, float* o2, float* o3, const float* target, const int8_t* is_test, double* acc) {
     if (int rc = check_queries(c, n, u, i)) return rc;
     if (int rc = ensure_bf16(c)) return rc;
     tx_ctx::Stager* g = nullptr;
     if (int rc = checkout(c, n, &g)) return rc;
     const int64_t rows = std::min<int64_t>(std::max<int64_t>(n, 1), int64_t{1} << 22);
     const Staging sg{g->st, static_cast<char*>(g->buf), rows, g->acc};
-    const int rc = run_host(c, sg, mode, n, u, i, d, o0, o1, o2, target, is_test, acc);
+    const int rc = run_host(c, sg, mode, n, u, i, d, o0, o1, o2, o3, target, is_test, acc);
     if (rc) (void)hipStreamSynchronize(g->st);
     std::lock_guard<std::mutex> lock(c->stage_mu);
     c->free_stagers.push_back(g);
@@ -1699,15 +1988,22 @@ int run_staged(tx_ctx* c, int mode, int64_t n, const int32_t* u, const int32_t* 
 int tx_predict(tx_ctx* c, int64_t n, const int32_t* u, const int32_t* i, const int16_t* d, float* out) {
     g_err.clear();
     if (!c || n < 0 || (n > 0 && (!u || !i || !d || !out))) return fail(TX_EINVAL, "tx_predict: bad arguments");
-    return run_staged(c, 0, n, u, i, d, out, nullptr, nullptr, nullptr, nullptr, nullptr);
+    return run_staged(c, 0, n, u, i, d, out, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
 }
 
 int tx_predict_subscores(tx_ctx* c, int64_t n, const int32_t* u, const int32_t* i, const int16_t* d, float* bias,
-                         float* mf, float* nsvd1) {
+                         float* mf, float* nsvd1, float* nbr) {
     g_err.clear();
-    if (!c || n < 0 || (n > 0 && (!u || !i || !d || !bias || !mf || !nsvd1)))
-        return fail(TX_EINVAL, "tx_predict_subscores: bad arguments");
-    return run_staged(c, 1, n, u, i, d, bias, mf, nsvd1, nullptr, nullptr, nullptr);
+    if (!c || n < 0 || (n > 0 && (!u || !i || !d || !bias || !mf || !nsvd1 || (c->nb && !nbr))))
+        return fail(TX_EINVAL, "tx_predict_subscores: bad arguments (nbr is required with the neighbourhood terms)");
+    return run_staged(c, 1, n, u, i, d, bias, mf, nsvd1, c->nb ? nbr : nullptr, nullptr, nullptr, nullptr);
+}
+
+int tx_n_subscores(tx_ctx* c, int32_t* out) {
+    g_err.clear();
+    if (!c || !out) return fail(TX_EINVAL, "tx_n_subscores: NULL argument");
+    *out = c->nb ? 4 : 3;
+    return 0;
 }
 
 int tx_predict_noalloc(tx_ctx* c, int64_t n, const int32_t* u, const int32_t* i, const int16_t* d, float* out) {
@@ -1724,7 +2020,7 @@ int tx_predict_noalloc(tx_ctx* c, int64_t n, const int32_t* u, const int32_t* i,
     static_assert(tx::kNoAllocRows == TX_NOALLOC_MAX_ROWS, "scratch sized for the header's limit");
     const Staging sg{c->st, base, TX_NOALLOC_MAX_ROWS,
                      reinterpret_cast<double*>(base + staging_bytes(TX_NOALLOC_MAX_ROWS))};
-    return run_host(c, sg, 0, n, u, i, d, out, nullptr, nullptr, nullptr, nullptr, nullptr);
+    return run_host(c, sg, 0, n, u, i, d, out, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
 }
 
 int tx_calc_rmse(tx_ctx* c, const tx_ratings* ds, double* out) {
@@ -1734,7 +2030,7 @@ int tx_calc_rmse(tx_ctx* c, const tx_ratings* ds, double* out) {
         return fail(TX_EINVAL, "tx_calc_rmse: rating set lacks indices or residuals");
     double acc[2] = {0.0, 0.0};
     if (int rc = run_staged(c, 2, ds->n_ratings, ds->user_idxs, ds->item_idxs, ds->dates, nullptr, nullptr, nullptr,
-                          ds->residuals, ds->is_test, acc))
+                          nullptr, ds->residuals, ds->is_test, acc))
         return rc;
     *out = std::sqrt(acc[0] / acc[1]);
     return 0;

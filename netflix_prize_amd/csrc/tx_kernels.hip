@@ -61,6 +61,38 @@ __global__ __launch_bounds__(64) void k_sgd_serial(DevModel m, const int64_t* __
         const uint32_t start = m.t_off[u], end = m.t_off[u + 1];
         if (start == end) continue;
         const float norm = m.norm_train[u];
+        // the epoch's global neighbours of u (src/tx.rs:762-796): rb sampled
+        // R(u) positions then ib sampled N(u)\R(u) positions, or every one
+        uint32_t nb_rb = 0, nb_ib = 0;
+        float nb_rs = 1.0f, nb_is = 1.0f, nb_nr = 0.0f, nb_nn = 0.0f;
+        bool nb_smp = false;
+        const uint32_t imp0 = m.nb_on ? m.imp_start[u] : 0u;
+        if (m.nb_on) {
+            const uint32_t nr_full = end - start, ni_full = m.imp_start[u + 1] - imp0;
+            const uint32_t cw = m.smp_cnt[u];
+            nb_smp = cw != 0xffffffffu;
+            nb_rb = nb_smp ? (cw & 0xffffu) : nr_full;
+            nb_ib = nb_smp ? (cw >> 16) : ni_full;
+            if (nb_smp) {
+                nb_rs = static_cast<float>(nr_full) / static_cast<float>(nb_rb);
+                nb_is = nb_ib > 0 ? static_cast<float>(ni_full) / static_cast<float>(nb_ib) : 1.0f;
+            }
+            nb_nr = m.norm_ru[u];
+            nb_nn = m.norm_nu[u];
+        }
+        const uint16_t* nb_pos = m.nb_on && nb_smp ? m.smp + static_cast<size_t>(u) * m.nb_max : nullptr;
+        // k-th global neighbour: item, r - b~ (rated) and whether it is rated
+        auto nb_at = [&](uint32_t k, uint32_t& j, float& rmb) {
+            if (k < nb_rb) {
+                const uint32_t t2 = start + (nb_smp ? nb_pos[k] : k);
+                j = m.t_item[t2];
+                rmb = m.t_rmb[t2];
+                return true;
+            }
+            j = m.imp_item[imp0 + (nb_smp ? nb_pos[k] : k - nb_rb)];
+            rmb = 0.0f;
+            return false;
+        };
 
         // NSVD1 user aggregation (src/tx.rs:801-822)
         for (int k = lane; k < f; k += 64) {
@@ -143,7 +175,42 @@ __global__ __launch_bounds__(64) void k_sgd_serial(DevModel m, const int64_t* __
             const float dot = prod[fs];
             __syncthreads();
             const float bias_part = m.gbias + bu_t + bi_t * cu_t;
-            const float score = bias_part + dot + 0.0f;
+            // neighbourhood terms (src/tx.rs:903-956), each sum in list order;
+            // every lane evaluates them from the same loads
+            float nbr_part = 0.0f, nn_day = 0.0f, nr_day = 0.0f;
+            const size_t irow = static_cast<size_t>(i) * m.I;
+            if (m.nb_on) {
+                float c_day_sum = 0.0f, w_day_sum = 0.0f;
+                if (m.nbd_on) {
+                    uint32_t n_same_rated = 0, n_same_total = 0;
+                    for (uint32_t e = m.sd_start[ud]; e < m.sd_start[ud + 1]; ++e) {
+                        const uint32_t j = m.sd_item[e] & 0x7fffffffu;
+                        const bool rated = (m.sd_item[e] >> 31) != 0u;
+                        if (j == i) continue;
+                        n_same_rated += rated ? 1u : 0u;
+                        n_same_total += 1u;
+                        c_day_sum += m.nc_day[irow + j];
+                        if (rated) w_day_sum += m.sd_rmb[e] * m.nw_day[irow + j];
+                    }
+                    nn_day = m.inv_sqrt_n[n_same_total];
+                    nr_day = m.inv_sqrt_n[n_same_rated];
+                }
+                float c_sum = 0.0f, w_sum = 0.0f;
+                for (uint32_t k = 0; k < nb_rb + nb_ib; ++k) {
+                    uint32_t j;
+                    float rmb;
+                    const bool rated = nb_at(k, j, rmb);
+                    if (j == i) continue;
+                    if (rated) {
+                        c_sum += m.nc[irow + j] * nb_rs;
+                        w_sum += rmb * m.nw[irow + j] * nb_rs;
+                    } else {
+                        c_sum += m.nc[irow + j] * nb_is;
+                    }
+                }
+                nbr_part = nb_nn * c_sum + nb_nr * w_sum + nn_day * c_day_sum + nr_day * w_day_sum;
+            }
+            const float score = bias_part + dot + nbr_part;
 
             float err;
             if (m.has_ord) {
@@ -157,6 +224,7 @@ __global__ __launch_bounds__(64) void k_sgd_serial(DevModel m, const int64_t* __
             }
             const float err_b = err * h.w_bias;
             const float err_d = err * h.w_factor;
+            const float err_n = err * h.w_nbr;
 
             // bias steps (src/tx.rs:981-990)
             if (lane == 0) {
@@ -206,6 +274,39 @@ __global__ __launch_bounds__(64) void k_sgd_serial(DevModel m, const int64_t* __
                     for (int k = lane; k < f; k += 64) {
                         const float yj = m.yfeat_day[yr + k];
                         m.yfeat_day[yr + k] = yj - h.lr_yd * (Gd[k] / norm_day + h.reg_yd * yj);
+                    }
+                }
+            }
+
+            // neighbourhood steps in list order (src/tx.rs:1038-1095), lane 0
+            if (m.nb_on && lane == 0) {
+                for (uint32_t k = 0; k < nb_rb + nb_ib; ++k) {
+                    uint32_t j;
+                    float rmb;
+                    const bool rated = nb_at(k, j, rmb);
+                    if (j == i) continue;
+                    const size_t idx = irow + j;
+                    if (rated) {
+                        const float w_ij = m.nw[idx];
+                        m.nw[idx] = w_ij - h.lr_w * (err_n * nb_nr * rmb * nb_rs + h.reg_w * w_ij);
+                        const float c_ij = m.nc[idx];
+                        m.nc[idx] = c_ij - h.lr_c * (err_n * nb_nn * nb_rs + h.reg_c * c_ij);
+                    } else {
+                        const float c_ij = m.nc[idx];
+                        m.nc[idx] = c_ij - h.lr_c * (err_n * nb_nn * nb_is + h.reg_c * c_ij);
+                    }
+                }
+                if (m.nbd_on) {
+                    for (uint32_t e = m.sd_start[ud]; e < m.sd_start[ud + 1]; ++e) {
+                        const uint32_t j = m.sd_item[e] & 0x7fffffffu;
+                        if (j == i) continue;
+                        const size_t idx = irow + j;
+                        if ((m.sd_item[e] >> 31) != 0u) {
+                            const float wd = m.nw_day[idx];
+                            m.nw_day[idx] = wd - h.lr_w_day * (err_n * nr_day * m.sd_rmb[e] + h.reg_w_day * wd);
+                        }
+                        const float cd = m.nc_day[idx];
+                        m.nc_day[idx] = cd - h.lr_c_day * (err_n * nn_day + h.reg_c_day * cd);
                     }
                 }
             }
@@ -753,11 +854,54 @@ __global__ __launch_bounds__(256) void k_rebuild_ycache_day(DevModel m, int64_t 
 //   MODE 2: squared error vs. target, FP64 sum into out_d[0], count in out_d[1]
 // ---------------------------------------------------------------------------
 
+// nbr_score (src/tx.rs:341-389) of one query, the team's lanes striding over
+// the user's lists: every R(u) and N(u)\R(u) neighbour unscaled (no sampling
+// at prediction time), and the same-day neighbours from the list of the
+// query's SparseUD day (ud < 0: the user has no rating that day, no terms).
+// Every lane returns the team's sum.
+template <int TEAM>
+__device__ float nbr_score_team(const DevModel& m, int u, int i, int64_t ud, int lane) {
+    const size_t irow = static_cast<size_t>(i) * m.I;
+    const uint32_t ui = static_cast<uint32_t>(i);
+    float cs = 0.0f, ws = 0.0f, cd = 0.0f, wd = 0.0f, nsr = 0.0f, nst = 0.0f;
+    for (uint32_t t = m.t_off[u] + lane; t < m.t_off[u + 1]; t += TEAM) {
+        const uint32_t j = m.t_item[t];
+        if (j == ui) continue;
+        cs += m.nc[irow + j];
+        ws += m.t_rmb[t] * m.nw[irow + j];
+    }
+    for (uint32_t k = m.imp_start[u] + lane; k < m.imp_start[u + 1]; k += TEAM) {
+        const uint32_t j = m.imp_item[k];
+        if (j != ui) cs += m.nc[irow + j];
+    }
+    if (m.nbd_on && ud >= 0) {
+        for (uint32_t e = m.sd_start[ud] + lane; e < m.sd_start[ud + 1]; e += TEAM) {
+            const uint32_t it = m.sd_item[e], j = it & 0x7fffffffu;
+            if (j == ui) continue;
+            nst += 1.0f;
+            cd += m.nc_day[irow + j];
+            if (it >> 31) {
+                nsr += 1.0f;
+                wd += m.sd_rmb[e] * m.nw_day[irow + j];
+            }
+        }
+    }
+    cs = team_sum<TEAM>(cs);
+    ws = team_sum<TEAM>(ws);
+    cd = team_sum<TEAM>(cd);
+    wd = team_sum<TEAM>(wd);
+    const int n_tot = static_cast<int>(team_sum<TEAM>(nst)), n_rat = static_cast<int>(team_sum<TEAM>(nsr));
+    const float nn_day = m.inv_sqrt_n[n_tot < m.n_isq ? n_tot : m.n_isq - 1];
+    const float nr_day = m.inv_sqrt_n[n_rat < m.n_isq ? n_rat : m.n_isq - 1];
+    return m.norm_nu[u] * cs + m.norm_ru[u] * ws + nn_day * cd + nr_day * wd;
+}
+
 template <int TEAM, int VEC, int MODE>
 __global__ __launch_bounds__(256) void k_predict(DevModel m, int64_t n, const int32_t* __restrict__ us,
                                                  const int32_t* __restrict__ is, const int16_t* __restrict__ ds,
                                                  float* __restrict__ out0, float* __restrict__ out1,
-                                                 float* __restrict__ out2, const float* __restrict__ target,
+                                                 float* __restrict__ out2, float* __restrict__ out3,
+                                                 const float* __restrict__ target,
                                                  const int8_t* __restrict__ is_test, double* __restrict__ out_d) {
     const int64_t team = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / TEAM;
     const int lane = threadIdx.x % TEAM;
@@ -814,9 +958,11 @@ __global__ __launch_bounds__(256) void k_predict(DevModel m, int64_t n, const in
             for (int c = 0; c < VEC; ++c) part += (q[c] + qf[c]) * (p[c] + s[c] + dev * p2[c] + pd[c] + sd[c]);
         }
         }
+        const float nbr = m.nb_on ? nbr_score_team<TEAM>(m, u, i, ud, lane) : 0.0f;
         if (MODE == 1) {
             mf = team_sum<TEAM>(mf);
             ns = team_sum<TEAM>(ns);
+            if (lane == 0 && out3) out3[qn] = nbr;
             if (lane == 0) {
                 const float bias = m.gbias + m.ubias[u] + m.ibias[static_cast<size_t>(i) * m.ibs] + m.ibias_freq[static_cast<size_t>(i) * F + fb]
                                    + m.but_bin[static_cast<size_t>(u) * B + b] + m.alpha_u[u] * dev + bu_day
@@ -831,6 +977,7 @@ __global__ __launch_bounds__(256) void k_predict(DevModel m, int64_t n, const in
         const float bu_t = m.ubias[u] + m.but_bin[static_cast<size_t>(u) * B + b] + m.alpha_u[u] * dev + bu_day
                            + m.ycache_bias[u];
         float score = m.gbias + bu_t + bi_t * cu_t + dot;
+        score = score + nbr;
         if (m.has_ord) {
             const float th[4] = {m.thresholds[0], m.thresholds[1], m.thresholds[2], m.thresholds[3]};
             score = ordinal_expect(th, score);
@@ -919,6 +1066,11 @@ hipError_t launch_sgd_plain(const DevModel& m, const int64_t* order, int vec, in
 
 hipError_t launch_sgd_par(const DevModel& m, const StrataView& sv, int stratum, int team, int vec, int blocks,
                           bool qf, bool p2, bool ord, int pass, bool day, hipStream_t st) {
+    if (m.nb_on && pass == 0) {
+        if (sv.n_strata != 1 || (day && !sv.day_acc) || m.nb_max < 1 || m.nb_max > 128) return hipErrorInvalidValue;
+        return m.nbd_on ? launch_sgd_par_nb2(m, sv, stratum, team, vec, blocks, ord, day, st)
+                        : launch_sgd_par_nb1(m, sv, stratum, team, vec, blocks, ord, day, st);
+    }
     if (day) {
         if (pass != 0 || sv.n_strata != 1 || !sv.day_acc) return hipErrorInvalidValue;
         return launch_sgd_par_day(m, sv, stratum, team, vec, blocks, qf, p2, ord, st);
@@ -1010,7 +1162,7 @@ hipError_t launch_rebuild(const DevModel& m, int64_t n_ud, int team, int vec, hi
 }
 
 hipError_t launch_predict(const DevModel& m, int mode, int64_t n, const int32_t* us, const int32_t* is,
-                          const int16_t* ds, float* o0, float* o1, float* o2, const float* target,
+                          const int16_t* ds, float* o0, float* o1, float* o2, float* o3, const float* target,
                           const int8_t* is_test, double* out_d, int team, int vec, hipStream_t st) {
     if (n <= 0) return hipSuccess;
 #define X(T, V)                                                                                            \
@@ -1019,13 +1171,13 @@ hipError_t launch_predict(const DevModel& m, int mode, int64_t n, const int32_t*
         (void)hipGetLastError();                                                                            \
         if (mode == 0)                                                                                      \
             hipLaunchKernelGGL((k_predict<T, V, 0>), dim3(g), dim3(256), 0, st, m, n, us, is, ds, o0, o1, o2, \
-                               target, is_test, out_d);                                                     \
+                               o3, target, is_test, out_d);                                                     \
         else if (mode == 1)                                                                                 \
             hipLaunchKernelGGL((k_predict<T, V, 1>), dim3(g), dim3(256), 0, st, m, n, us, is, ds, o0, o1, o2, \
-                               target, is_test, out_d);                                                     \
+                               o3, target, is_test, out_d);                                                     \
         else                                                                                                \
             hipLaunchKernelGGL((k_predict<T, V, 2>), dim3(g), dim3(256), 0, st, m, n, us, is, ds, o0, o1, o2, \
-                               target, is_test, out_d);                                                     \
+                               o3, target, is_test, out_d);                                                     \
         return hipGetLastError();                                                                           \
     }
     TX_SHAPES(X)

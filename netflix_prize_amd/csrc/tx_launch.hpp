@@ -27,6 +27,16 @@ hipError_t launch_sgd_par_day(const DevModel& m, const StrataView& sv, int strat
                               bool qf, bool p2, bool ord, hipStream_t st);
 hipError_t launch_sgd_par_user(const DevModel& m, const StrataView& sv, int stratum, int team, int vec, int blocks,
                                bool qf, bool p2, int pass, hipStream_t st);
+// neighbourhood models (m.nb_on; launch_sgd_par and launch_sgd_par_mw route
+// PASS 0 here): 1 = global terms, 2 = global and same-day terms
+hipError_t launch_sgd_par_nb1(const DevModel& m, const StrataView& sv, int stratum, int team, int vec, int blocks,
+                              bool ord, bool day, hipStream_t st);
+hipError_t launch_sgd_par_nb2(const DevModel& m, const StrataView& sv, int stratum, int team, int vec, int blocks,
+                              bool ord, bool day, hipStream_t st);
+hipError_t launch_sgd_par_nb1_mw(const DevModel& m, const StrataView& sv, int teams, int waves, bool day,
+                                 hipStream_t st);
+hipError_t launch_sgd_par_nb2_mw(const DevModel& m, const StrataView& sv, int teams, int waves, bool day,
+                                 hipStream_t st);
 hipError_t launch_to_bf16(const float* x, uint16_t* y, int64_t n, hipStream_t st);
 // the NSVD1 step of the tsvdx4p user pass from the stored per-user gradients
 // (gacc [U * fs], gacc_b [U]) gathered per item; vec = the shape's VEC
@@ -36,18 +46,19 @@ hipError_t launch_nsvd_gather(const DevModel& m, const ItemView& iv, const float
 hipError_t launch_item_pass(const DevModel& m, const ItemView& iv, int team, int vec, bool p2, bool qf,
                             int blocks, hipStream_t st, int waves = 1);
 hipError_t launch_rebuild(const DevModel& m, int64_t n_ud, int team, int vec, hipStream_t st);
+// mode 0 predict (o0), 1 subscores (o0..o2, and o3 = nbr when m.nb_on and o3 != NULL), 2 FP64 SSE
 hipError_t launch_predict(const DevModel& m, int mode, int64_t n, const int32_t* us, const int32_t* is,
-                          const int16_t* ds, float* o0, float* o1, float* o2, const float* target,
+                          const int16_t* ds, float* o0, float* o1, float* o2, float* o3, const float* target,
                           const int8_t* is_test, double* out_d, int team, int vec, hipStream_t st);
 hipError_t launch_fill_f32(float* p, float v, int64_t n, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st);
 hipError_t launch_fill_f64(double* p, double v, int64_t n, hipStream_t st);
 
 // tx_predict_noalloc's scratch array: staging for one batch of kNoAllocRows
-// host-buffer queries (user, item i32; day i16; three float outputs; is_test
+// host-buffer queries (user, item i32; day i16; four float outputs; is_test
 // i8; each part 256-byte aligned) followed by two FP64 sums.
 constexpr int64_t kNoAllocRows = int64_t{1} << 20;
-constexpr int64_t kNoAllocBytes = kNoAllocRows * (4 + 4 + 2 + 4 * 3 + 1) + 256;
+constexpr int64_t kNoAllocBytes = kNoAllocRows * (4 + 4 + 2 + 4 * 4 + 1) + 256;
 // Device address of that array on `device` (the current device), looked up
 // once per device and kept.
 hipError_t noalloc_scratch(int device, void** out);

@@ -241,6 +241,85 @@ void shuffle(ChaChaRng& rng, T* v, int64_t n) {
     }
 }
 
+// The same shuffle in two passes, for many shuffles drawn from one stream:
+// shuffle_draws consumes the stream exactly as shuffle does and records the
+// packed value of each group of indices (a multiply and a compare per group,
+// no division); shuffle_apply turns recorded values into the swaps (the
+// divisions), in any order or thread once drawn.  Both walk the groups as
+// shuffle does: index 0 is free, then each group packs consecutive ranges
+// while their product fits in u32.
+// The groups do not depend on n (an n-element shuffle uses the groups that
+// start below n), so the chain is computed once for ranges up to kChain.
+struct ShuffleChain {
+    static constexpr uint32_t kChain = 1u << 17;
+    std::vector<uint32_t> start, prod;  // group g covers ranges [start[g], start[g+1])
+    ShuffleChain() {
+        uint32_t range = 2;
+        while (range < kChain) {
+            uint64_t p = range;
+            uint32_t next = range + 1;
+            while (p * next <= 0xffffffffull) { p *= next; ++next; }
+            start.push_back(range);
+            prod.push_back(static_cast<uint32_t>(p));
+            range = next;
+        }
+        start.push_back(range);
+    }
+    static const ShuffleChain& get() {
+        static const ShuffleChain c;
+        return c;
+    }
+};
+
+template <typename Group>
+inline void shuffle_groups(int64_t n, Group group) {
+    if (n <= 1) return;
+    uint32_t range = 2;  // ranges 1 (index 0) done; groups start at range 2
+    int64_t i = 1;
+    const ShuffleChain& ch = ShuffleChain::get();
+    for (size_t g = 0; g + 1 < ch.start.size() && i < n; ++g) {
+        const int64_t cnt = static_cast<int64_t>(ch.start[g + 1] - ch.start[g]);
+        group(i, ch.start[g], cnt, ch.prod[g]);
+        i += cnt;
+        range = ch.start[g + 1];
+    }
+    while (i < n) {
+        uint64_t prod = range;
+        uint32_t next = range + 1;
+        while (prod * next <= 0xffffffffull) { prod *= next; ++next; }
+        const int64_t cnt = static_cast<int64_t>(next - range);  // indices in the group
+        group(i, range, cnt, static_cast<uint32_t>(prod));
+        i += cnt;
+        range = next;
+    }
+}
+
+// Appends one packed value per group of an n-element shuffle to `out`.
+inline void shuffle_draws(ChaChaRng& rng, int64_t n, std::vector<uint32_t>& out) {
+    shuffle_groups(n, [&](int64_t, uint32_t, int64_t, uint32_t prod) { out.push_back(uniform_below(rng, prod)); });
+}
+
+// Applies the packed values `pk` (as shuffle_draws recorded them) to v[0..n).
+template <typename T>
+inline void shuffle_apply(const uint32_t* pk, T* v, int64_t n) {
+    shuffle_groups(n, [&](int64_t i0, uint32_t range, int64_t cnt, uint32_t) {
+        uint32_t packed = *pk++;
+        for (int64_t q = 0; q < cnt && i0 + q < n; ++q) {
+            const uint32_t r = range + static_cast<uint32_t>(q);
+            uint32_t j;
+            if (q == cnt - 1) {
+                j = packed;
+            } else {
+                j = packed % r;
+                packed /= r;
+            }
+            T tmp = v[i0 + q];
+            v[i0 + q] = v[j];
+            v[j] = tmp;
+        }
+    });
+}
+
 // get_users: identity, or shuffled with seed ^ epoch * golden-ratio constant.
 inline void user_order(int64_t n, bool shuffle_users, uint64_t seed, int64_t epoch, int64_t* out) {
     for (int64_t k = 0; k < n; ++k) out[k] = k;

@@ -31,6 +31,11 @@ hipError_t launch_sgd_par_mw(const DevModel& m, const StrataView& sv, int teams,
                              bool day, hipStream_t st, int pass) {
     if (waves < 2 || waves > 8 || m.fs != 1024 * waves || sv.n_strata != 1 || (day && !sv.day_acc))
         return hipErrorInvalidValue;
+    if (m.nb_on && pass == 0) {
+        if (ord || m.nb_max < 1 || m.nb_max > 128) return hipErrorInvalidValue;
+        return m.nbd_on ? launch_sgd_par_nb2_mw(m, sv, teams, waves, day, st)
+                        : launch_sgd_par_nb1_mw(m, sv, teams, waves, day, st);
+    }
     if (pass == 1 || pass == 2) {
         if (day || ord) return hipErrorInvalidValue;
         if (waves <= 4)

@@ -15,6 +15,8 @@ struct Hyper {
     float w_bias, w_factor;
     float beta;
     float lr_t, reg_t;
+    // neighbourhood terms (tx5/tx6 modes)
+    float lr_w, lr_c, reg_w, reg_c, lr_w_day, lr_c_day, reg_w_day, reg_c_day, w_nbr;
 };
 
 struct DevModel {
@@ -75,6 +77,36 @@ struct DevModel {
 
     const float* norm_train;    // [U] sqrt(|R(u)| + |P(u)| if full_su)
     const float* norm_rebuild;  // [U] sqrt(train_cnt + heldout_cnt)
+
+    // Neighbourhood terms (tx5 / tx6 modes, src/tx.rs:253-262, 483-585).
+    // nb_on: w and c built (lr_w > 0 or lr_c > 0), with the per-user lists
+    // R(u) (the training ratings, file order, r - b~ in t_rmb) and N(u)\R(u)
+    // (imp_*: held-out items not rated in training, sorted by item [, day],
+    // deduplicated); nbd_on: w_day and c_day built (their terms need the
+    // lists, so they act only when nb_on is set too).  The dense tables are
+    // I x I row-major, element (i, j) at i * I + j.
+    int32_t nb_on, nbd_on, nb_max;
+    float *nw, *nc, *nw_day, *nc_day;
+    const float* t_rmb;          // [N]
+    const uint32_t* imp_start;   // [U+1]
+    const uint32_t* imp_item;    // [n_imp]
+    const int16_t* imp_day;      // [n_imp]
+    const float* norm_nu;        // [U] |N(u)|^-0.5 (0 for an empty set), host powf
+    const float* norm_ru;        // [U] |R(u)|^-0.5
+    const float* inv_sqrt_n;     // [n_isq] k^-0.5 by host powf (k = 0: 0), for same-day counts
+    int32_t n_isq;
+    // same-day lists per SparseUD day (nb_on && nbd_on): the user's training
+    // items of that day in file order (bit 31 set), then its N(u)\R(u) items
+    // of that day in list order
+    const uint32_t* sd_start;    // [D+1]
+    const uint32_t* sd_item;
+    const float* sd_rmb;         // r - b~ of the rated entries, 0 for the others
+    // the epoch's global neighbour samples (src/tx.rs:750-793), drawn on the
+    // host from the reference's stream: smp_cnt[u] = rb | ib << 16 with the
+    // rb sorted R(u) positions then the ib sorted N(u)\R(u) positions in
+    // smp[u * nb_max ..]; 0xffffffff = no sampling (every neighbour, scale 1)
+    const uint32_t* smp_cnt;     // [U]
+    const uint16_t* smp;         // [U * nb_max]
 
     // BF16 read copies of the factor tables (row stride fs; FP32 tables stay
     // the masters); predictions read them when bf16_pred is set, and the

@@ -31,7 +31,9 @@ LIB_PATH = os.path.join(_PKG, "libtsvd_tx.so")
 PARAMS = ["ubias", "ibias", "ufeat", "ufeat2", "ifeat", "yfeat", "ycache", "yfeat_day",
           "tu_mean", "alpha_u", "but_bin", "bit_bin", "ybias", "ycache_bias", "ubias_day",
           "ufeat_day", "ycache_day", "ibias_freq", "ifeat_freq", "cu", "cut", "gbias",
-          "thresholds"]
+          "thresholds", "w", "c", "w_day", "c_day"]
+# the dense item x item neighbourhood tables (empty unless the model has them)
+NBR_PARAMS = ("w", "c", "w_day", "c_day")
 PARAM_ID = {n: k for k, n in enumerate(PARAMS)}
 
 SCHED_SERIAL = 0
@@ -224,7 +226,8 @@ def lib() -> C.CDLL:
         L.tx_predict_noalloc.argtypes = [vp, i64, vp, vp, vp, vp]
         L.tx_set_frozen_precision.argtypes = [vp, i32]
         L.tx_predict_device.argtypes = [vp, i64, vp, vp, vp, vp, vp]
-        L.tx_predict_subscores.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp]
+        L.tx_predict_subscores.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp]
+        L.tx_n_subscores.argtypes = [vp, C.POINTER(i32)]
         L.tx_calc_rmse.argtypes = [vp, vp, C.POINTER(C.c_double)]
         L.tx_param_len.argtypes = [vp, i32, C.POINTER(i64)]
         L.tx_get_param.argtypes = [vp, i32, vp, i64]
@@ -235,7 +238,8 @@ def lib() -> C.CDLL:
         for name in ["tx_create", "tx_destroy", "tx_set_schedule", "tx_fit_epoch", "tx_fit_users_epoch",
                      "tx_rebuild_ycache", "tx_set_table_precision", "tx_autotune",
                      "tx_last_epoch_stats",
-                     "tx_predict", "tx_predict_device", "tx_predict_subscores", "tx_calc_rmse", "tx_param_len",
+                     "tx_predict", "tx_predict_device", "tx_predict_subscores", "tx_n_subscores", "tx_calc_rmse",
+                     "tx_param_len",
                      "tx_get_param", "tx_set_param", "tx_set_param_device", "tx_get_users", "tx_rand_normal"]:
             getattr(L, name).restype = C.c_int
         _lib = L
@@ -391,10 +395,13 @@ class TxModel:
         return int(self.cfg.n_epochs)
 
     def n_subscores(self) -> int:
-        return 3
+        """4 with the neighbourhood terms (src/tx.rs:653), else 3."""
+        n = C.c_int32()
+        _check(lib().tx_n_subscores(self._h, C.byref(n)), "tx_n_subscores")
+        return int(n.value)
 
     def subscore_names(self):
-        return ["bias", "mf", "nsvd1"]
+        return ["bias", "mf", "nsvd1", "nbr"][: self.n_subscores()]
 
     def set_schedule(self, mode: int = SCHED_STRATA, n_strata: int = 1, blocks: int = 0, use_graph: bool = False,
                      teams_per_block: int = 0, q_update: int = 0, y_update: int = 2,
@@ -484,9 +491,11 @@ class TxModel:
         u = np.ascontiguousarray(np.atleast_1d(u), np.int32)
         i = np.ascontiguousarray(np.atleast_1d(i), np.int32)
         d = np.ascontiguousarray(np.atleast_1d(day), np.int16)
-        outs = [np.empty(len(u), np.float32) for _ in range(3)]
-        _check(lib().tx_predict_subscores(self._h, len(u), u.ctypes.data, i.ctypes.data, d.ctypes.data,
-                                          *[o.ctypes.data for o in outs]), "tx_predict_subscores")
+        k = self.n_subscores()
+        outs = [np.empty(len(u), np.float32) for _ in range(k)]
+        ptrs = [o.ctypes.data for o in outs] + [None] * (4 - k)
+        _check(lib().tx_predict_subscores(self._h, len(u), u.ctypes.data, i.ctypes.data, d.ctypes.data, *ptrs),
+               "tx_predict_subscores")
         return outs
 
     def calc_rmse(self, ds) -> float:

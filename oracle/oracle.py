@@ -20,7 +20,7 @@ _LIB_PATH = os.path.join(_HERE, "libtxoracle.so")
 PARAMS = ["ubias", "ibias", "ufeat", "ufeat2", "ifeat", "yfeat", "ycache", "yfeat_day",
           "tu_mean", "alpha_u", "but_bin", "bit_bin", "ybias", "ycache_bias", "ubias_day",
           "ufeat_day", "ycache_day", "ibias_freq", "ifeat_freq", "cu", "cut", "gbias",
-          "thresholds"]
+          "thresholds", "w", "c", "w_day", "c_day"]
 PARAM_ID = {n: k for k, n in enumerate(PARAMS)}
 
 

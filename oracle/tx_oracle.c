@@ -1015,7 +1015,7 @@ static void fit_epoch_users(or_model* m, const or_dataset* tr, const or_dataset*
 
             /* same-day neighbourhood, every neighbour (src/tx.rs:903-929) */
             float c_day_sum = 0.0f, w_day_sum = 0.0f, nn_day = 0.0f, nr_day = 0.0f;
-            if (has_wd || has_cd) {
+            if ((has_wd || has_cd) && has_nbr) {
                 uint32_t n_same_rated = 0, n_same_total = 0;
                 for (int64_t t2 = start; t2 < end; t2++) {
                     const int64_t j = m->r_item[t2];
@@ -1141,7 +1141,7 @@ static void fit_epoch_users(or_model* m, const or_dataset* tr, const or_dataset*
                 }
             }
             /* same-day neighbourhood steps, every neighbour (src/tx.rs:1071-1095) */
-            if (has_wd || has_cd) {
+            if ((has_wd || has_cd) && has_nbr) {
                 for (int64_t t2 = start; t2 < end; t2++) {
                     const int64_t j = m->r_item[t2];
                     if (j == i || m->r_day[t2] != day16) continue;
@@ -1726,6 +1726,7 @@ enum {
     P_UBIAS = 0, P_IBIAS, P_UFEAT, P_UFEAT2, P_IFEAT, P_YFEAT, P_YCACHE, P_YFEAT_DAY,
     P_TU_MEAN, P_ALPHA_U, P_BUT_BIN, P_BIT_BIN, P_YBIAS, P_YCACHE_BIAS, P_UBIAS_DAY,
     P_UFEAT_DAY, P_YCACHE_DAY, P_IBIAS_FREQ, P_IFEAT_FREQ, P_CU, P_CUT, P_GBIAS, P_THRESHOLDS,
+    P_NB_W, P_NB_C, P_NB_W_DAY, P_NB_C_DAY,
     P_COUNT
 };
 
@@ -1756,6 +1757,11 @@ static float* param_ptr(or_model* m, int id, int64_t* n) {
     case P_CUT: *n = nud; return m->cut;
     case P_GBIAS: *n = 1; return &m->gbias;
     case P_THRESHOLDS: *n = 4; return m->thresholds;
+    /* the dense neighbourhood tables, I x I row-major (0 elements when not built) */
+    case P_NB_W: *n = m->w ? I * I : 0; return m->w;
+    case P_NB_C: *n = m->c ? I * I : 0; return m->c;
+    case P_NB_W_DAY: *n = m->w_day ? I * I : 0; return m->w_day;
+    case P_NB_C_DAY: *n = m->c_day ? I * I : 0; return m->c_day;
     default: *n = -1; return NULL;
     }
 }
@@ -1764,6 +1770,7 @@ OR_API int64_t or_param_len(or_model* m, int id) { int64_t n; param_ptr(m, id, &
 
 OR_API int or_get_param(or_model* m, int id, float* out, int64_t n) {
     int64_t len; float* p = param_ptr(m, id, &len);
+    if (len >= 0 && n == len && n == 0) return 0;  /* a table that is not built */
     if (!p || n != len) return -1;
     memcpy(out, p, sizeof(float) * (size_t)n);
     return 0;
@@ -1771,6 +1778,7 @@ OR_API int or_get_param(or_model* m, int id, float* out, int64_t n) {
 
 OR_API int or_set_param(or_model* m, int id, const float* in, int64_t n) {
     int64_t len; float* p = param_ptr(m, id, &len);
+    if (len >= 0 && n == len && n == 0) return 0;
     if (!p || n != len) return -1;
     memcpy(p, in, sizeof(float) * (size_t)n);
     return 0;

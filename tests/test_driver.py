@@ -64,15 +64,49 @@ REFERENCE_JOBS = (
     ["tsvdx4p-1000lm", "tsvdx4p-2000lm", "tsvdx4p-4000lm", "tsvdx4p-6000lm", "tsvdx4p-8000lm",
      "tsvdx4p-16000lm", "tsvdx4p-1000lm__knnf", "tsvdx4p-2000lm__knnf", "tsvdx4p-6000lm__knnf",
      "tsvdx4p-8000lm__knnf"],
+    # src/bin/tsvdx5-new.rs:23-35 (TxModel base and ordinal jobs), :84-89 (knn3 on
+    # the ordinal base), :104-128 (epoch blends); the NLPP, paropt, knnf and
+    # knn-w chains of that binary belong to other model families
+    ["tsvdx5-20", "tsvdx5-40", "tsvdx5-60", "tsvdx5-120", "tsvdx5-150", "tsvdx5-300", "tsvdx5-400", "tsvdx5-1000",
+     "tsvdx5-1200lm", "tsvdx5-120o", "tsvdx5-150o", "tsvdx5-120o__knn3", "tsvdx5-120o__epochs",
+     "tsvdx5-300__epochs"],
 )
 
 
 @pytest.mark.parametrize("binary,job", [("tsvdx4-new", j) for j in REFERENCE_JOBS[0]] +
-                         [("tsvdx4p-new", j) for j in REFERENCE_JOBS[1]])
+                         [("tsvdx4p-new", j) for j in REFERENCE_JOBS[1]] +
+                         [("tsvdx5-new", j) for j in REFERENCE_JOBS[2]])
 def test_every_reference_job_name_is_in_the_table(binary, job):
     out = subprocess.run([os.path.join(ROOT, "netflix_prize_amd", binary), job, "--print-config"],
                          capture_output=True, text=True, timeout=60)
     assert out.returncode == 0, (job, out.stderr)
+
+
+def test_print_config_tsvdx5_and_tsvdx6_jobs():
+    """run_base(300, 23, 100, 0.0015, false, 10, false, false, true)
+    (src/bin/tsvdx5-new.rs:28, 130-161), run_ord(150, 25, 120, 1.2)
+    (:33, 185-217), and the two tsvdx6 binaries, which take no job argument
+    (src/bin/tsvdx6-300-new.rs:3-61, src/bin/tsvdx6-300-v2-new.rs:45-48)."""
+    c = _config("tsvdx5-300")
+    expect = {"n_feat": "300", "n_epochs": "23", "seed": "42", "n_time_bins": "32", "reset_u_epoch": "10",
+              "max_neighbors": "100", "lr_w": "0.0015", "lr_c": "0.0015", "reg_w": "0.002", "reg_c": "0.002",
+              "lr_w_day": "0.0", "w_bias": "0.8", "w_factor": "0.8", "w_nbr": "0.8", "lambda1": "25.0",
+              "lambda2": "10.0", "low_memory": "false", "save_ifeat": "false", "lr_tu": "0.0", "sigma_u": "0.0015"}
+    for k, v in expect.items():
+        assert c[k].strip() == v, (k, c[k])
+    c = _config("tsvdx5-150o")
+    expect = {"n_feat": "150", "n_epochs": "25", "seed": "64", "n_time_bins": "20", "max_neighbors": "120",
+              "w_bias": "1.2", "w_factor": "1.2", "w_nbr": "1.2", "lr_i": "0.0072", "sigma_u": "0.0",
+              "ordinal_head": "Some(OrdinalHeadConfig"}
+    for k, v in expect.items():
+        assert c[k].strip().startswith(v), (k, c[k])
+    for binary, w_day, w_nbr in (("tsvdx6-300-new", "0.001", "0.8"), ("tsvdx6-300-v2-new", "0.0008", "0.7")):
+        out = subprocess.run([os.path.join(ROOT, "netflix_prize_amd", binary), "--print-config"],
+                             capture_output=True, text=True, timeout=60)
+        assert out.returncode == 0, out.stderr
+        c = dict(re.findall(r"(\w+): ([^,{}]+)", out.stdout))
+        assert (c["n_feat"], c["lr_w_day"], c["lr_c_day"], c["w_nbr"], c["max_neighbors"]) == ("300", w_day, w_day,
+                                                                                                 w_nbr, "100")
 
 
 def test_unknown_job_is_an_error():

@@ -1,0 +1,142 @@
+"""The neighbourhood terms of TxModel (tsvdx5: global w/c; tsvdx6: plus the
+same-day w_day/c_day) on the GPU, against the oracle (needs an MI355X)."""
+import numpy as np
+import pytest
+
+from netflix_prize_amd import tx
+from oracle import oracle as O
+from tests.fixtures import lm_config, synthetic, tiny
+
+pytestmark = pytest.mark.gpu
+
+ALL_PARAMS = [p for p in tx.PARAMS if p != "thresholds"]
+
+
+def _nbr_config(day_terms: bool, **kw):
+    """The tsvdx5 run_base knobs (src/bin/tsvdx5-new.rs:130-160) on the lm
+    set; day_terms adds tsvdx6's same-day ones (src/bin/tsvdx6-300-new.rs:41-46)."""
+    d = dict(n_feat=24, n_epochs=2, n_time_bins=32, reset_u_epoch=1024, max_neighbors=40, lr_w=0.0015, lr_c=0.0015,
+             reg_w=0.002, reg_c=0.002, w_bias=0.8, w_factor=0.8, w_nbr=0.8, lambda1=25.0, lambda2=10.0,
+             low_memory=False)
+    if day_terms:
+        d.update(lr_w_day=0.001, lr_c_day=0.001, reg_w_day=0.002, reg_c_day=0.002)
+    d.update(kw)
+    return lm_config(**d)
+
+
+@pytest.mark.parametrize("which", ["tsvdx5", "tsvdx6"])
+def test_nbr_kat_on_gpu_serial(gpu, which):
+    """The reference's tsvdx5 / tsvdx6 KATs (tests/regressors_test.rs:95-155,
+    max_neighbors 5: the per-epoch neighbour sampling is exercised), trained on
+    the GPU in the serial schedule; every table after every epoch equals the
+    oracle's bit for bit, the dense w, c (and w_day, c_day) included."""
+    fx, tr, pr = tiny()
+    kat = fx["kat"][which]
+    cfg = tx.TxConfig.from_dict(kat["config"])
+    m_o = O.OracleModel(kat["config"], tr, pr)
+    with tx.TxModel(tr, pr, cfg) as m:
+        m.set_schedule(mode=tx.SCHED_SERIAL)
+        assert m.n_subscores() == 4
+        for e in range(1, cfg.n_epochs + 1):
+            m.fit_epoch(tr, pr, e)
+            m_o.fit_epoch(e)
+            for n in ALL_PARAMS:
+                np.testing.assert_array_equal(m.get(n), m_o.get(n), err_msg=f"{n} epoch {e}")
+        assert m.get("w").size == fx["n_items"] ** 2 and np.any(m.get("c") != 0)
+        assert (m.get("w_day").size > 0) == (which == "tsvdx6")
+        rmse = m.calc_rmse(pr)
+    print(f"{which}: GPU {rmse!r} reference {kat['probe_rmse']!r}")
+    assert abs(rmse - kat["probe_rmse"]) < 1e-7
+
+
+@pytest.mark.parametrize("day_terms", [False, True])
+def test_nbr_predict_and_subscores_match_oracle(gpu, day_terms):
+    """Identical parameters (trained by the oracle on the 600 x 150 x 30k set)
+    -> GPU predict and the four subscores (bias, mf, nsvd1, nbr = nbr_score
+    over every neighbour, src/tx.rs:341-389) within 1e-5 of the FP32 oracle,
+    predictions within 1e-5 of the FP64 one; the subscores add up to the
+    prediction."""
+    tr, pr = synthetic("tiny_synth")
+    cfg = _nbr_config(day_terms)
+    m_o = O.OracleModel(cfg, tr, pr)
+    for e in (1, 2):
+        m_o.fit_epoch(e)
+    with tx.TxModel(tr, pr, cfg) as m:
+        for n in ALL_PARAMS:
+            m.set(n, m_o.get(n))
+        u, i, d = pr.user_idxs, pr.item_idxs, pr.dates
+        g, o = m.predict(u, i, d), m_o.predict(u, i, d)
+        np.testing.assert_allclose(g, o, rtol=0, atol=1e-5)
+        np.testing.assert_allclose(g, m_o.predict_f64(u, i, d), rtol=0, atol=1e-5)
+        sg, so = m.predict_subscores(u, i, d), m_o.predict_subscores(u, i, d)
+        assert len(sg) == 4 and np.any(so[3] != 0)
+        for a, b, name in zip(sg, so, ["bias", "mf", "nsvd1", "nbr"]):
+            np.testing.assert_allclose(a, b, rtol=0, atol=1e-5, err_msg=name)
+        np.testing.assert_allclose(sum(sg), g, rtol=0, atol=1e-5)
+        assert m.calc_rmse(pr) == pytest.approx(m_o.calc_rmse(), abs=1e-6)
+
+
+@pytest.mark.parametrize("low_memory,day_terms,max_nbr", [(False, False, 40), (False, True, 100), (True, True, 40)])
+def test_nbr_single_team_matches_serial_oracle(gpu, low_memory, day_terms, max_nbr):
+    """The parallel kernel with the neighbourhood terms in serial-debug order
+    (one team: blocks = 1, teams_per_block = 1, the reference's user order)
+    against the serial oracle on the 600 x 150 x 30k set: every table, the
+    dense neighbourhood tables included, within 1e-3 of its movement over two
+    epochs.  max_neighbors 40 samples most users; 100 fills a lane's second
+    neighbour slot."""
+    tr, pr = synthetic("tiny_synth")
+    cfg = _nbr_config(day_terms, low_memory=low_memory, max_neighbors=max_nbr, lr_iqf=0.005, lr_ibf=0.005)
+    m_o = O.OracleModel(cfg, tr, pr)
+    init = {n: m_o.get(n).copy() for n in ALL_PARAMS}
+    with tx.TxModel(tr, pr, cfg) as m:
+        m.set_schedule(mode=tx.SCHED_STRATA, blocks=1, teams_per_block=1, user_order=0)
+        for e in (1, 2):
+            m.fit_epoch(epoch=e)
+            m_o.fit_epoch(e)
+        for n in ALL_PARAMS:
+            o, g = m_o.get(n), m.get(n)
+            assert o.shape == g.shape, n
+            move = float(np.max(np.abs(o.astype(np.float64) - init[n]))) if o.size else 0.0
+            err = float(np.max(np.abs(g.astype(np.float64) - o))) if o.size else 0.0
+            print(f"{n}: movement {move:.3e} gpu-oracle {err:.3e}")
+            assert err <= 1e-3 * move + 1e-6, n
+
+
+@pytest.mark.parametrize("day_terms", [False, True])
+def test_nbr_parallel_holdout_rmse(gpu, day_terms):
+    """The default parallel schedule (teams in flight, length buckets) on the
+    10k x 2k x 1M set, f=32, 5 epochs: holdout RMSE within 2e-3 of the serial
+    oracle's, and the neighbourhood terms help (below the same model without
+    them)."""
+    tr, pr = synthetic("small")
+    cfg = _nbr_config(day_terms, n_feat=32, n_epochs=5, max_neighbors=100)
+    m_o = O.OracleModel(cfg, tr, pr)
+    with tx.TxModel(tr, pr, cfg) as m:
+        for e in range(1, cfg.n_epochs + 1):
+            m.fit_epoch(epoch=e)
+            m_o.fit_epoch(e)
+        r_g, r_o = m.calc_rmse(pr), m_o.calc_rmse()
+        ms = m.last_epoch_stats()["ms_total"]
+    base = cfg.replace(lr_w=0.0, lr_c=0.0, lr_w_day=0.0, lr_c_day=0.0)
+    with tx.TxModel(tr, pr, base) as m:
+        for e in range(1, cfg.n_epochs + 1):
+            m.fit_epoch(epoch=e)
+        r_b = m.calc_rmse(pr)
+    print(f"parallel {r_g:.6f} serial oracle {r_o:.6f} without neighbourhood {r_b:.6f}; {ms:.1f} ms/epoch")
+    assert abs(r_g - r_o) <= 2e-3
+    assert r_o < r_b
+
+
+def test_nbr_schedule_limits(gpu):
+    """More than 128 global neighbours (or none, max_neighbors 0 = every one)
+    train in the serial schedule only; the fold-in pass has no neighbourhood
+    terms."""
+    tr, pr = synthetic("tiny_synth")
+    for k in (0, 200):
+        with tx.TxModel(tr, pr, _nbr_config(False, max_neighbors=k)) as m:
+            with pytest.raises(tx.TxError, match="max_neighbors"):
+                m.set_schedule(mode=tx.SCHED_STRATA)
+            m.set_schedule(mode=tx.SCHED_SERIAL)
+    with tx.TxModel(tr, pr, _nbr_config(False, low_memory=True)) as m:
+        with pytest.raises(tx.TxError):
+            m.fit_users_epoch(1)
