@@ -133,7 +133,10 @@ typedef struct tx_schedule {
  * schedule applies every user's NSVD1 step at the last stratum, and at the
  * full Netflix shape (f = 256) its holdout RMSE diverges (NaN with atomic y
  * rows, 1.136 after 6 epochs with plain y rows); without the bit
- * tx_set_schedule returns TX_EUNSUPPORTED. */
+ * tx_set_schedule returns TX_EUNSUPPORTED.
+ * Bits 5-6: the neighbourhood tables (w, c, w_day, c_day), 0 (default) or 2;
+ * set as (mode << TX_UPD_NBR_SHIFT). */
+#define TX_UPD_NBR_SHIFT 5
 #define TX_UPD_ATOMIC 0
 #define TX_UPD_XCD_ATOMIC 1
 #define TX_UPD_PLAIN 2

@@ -447,7 +447,7 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_
     const int off = cbase + lane;  // strided: element cbase + c*TEAM + lane
     const bool first = (stratum == 0);
     const bool last = (stratum == S - 1);
-    const int qm = sv.q_mode, ym = sv.y_mode;
+    const int qm = sv.q_mode, ym = sv.y_mode, nm = sv.nb_mode;
     const bool bfz = PASS != 0 && m.bf16_frozen != 0;  // q, q_f read from BF16 copies
 
     // Per-lane slot of the rating's scalars (sc): lane 0 b_i, 1 b_i,bin,
@@ -889,11 +889,12 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_
 #pragma unroll
                         for (int q = 0; q < 2; ++q) {
                             if (nok[q] && nj[q] != i) {
+                                const size_t idx = irow + nj[q];
                                 if (nrt[q])
-                                    apply_delta(m.nw + irow + nj[q], cur.wv[q],
-                                                -h.lr_w * (err_n * nb_nr * nwc[q] + h.reg_w * cur.wv[q]), qm);
-                                apply_delta(m.nc + irow + nj[q], cur.cv[q],
-                                            -h.lr_c * (err_n * nb_nn * ncs[q] + h.reg_c * cur.cv[q]), qm);
+                                    apply_delta(m.nw + idx, cur.wv[q],
+                                                -h.lr_w * (err_n * nb_nr * nwc[q] + h.reg_w * cur.wv[q]), nm);
+                                apply_delta(m.nc + idx, cur.cv[q],
+                                            -h.lr_c * (err_n * nb_nn * ncs[q] + h.reg_c * cur.cv[q]), nm);
                             }
                         }
                         if constexpr (NB == 2) {
@@ -904,10 +905,10 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_
                                 if (it >> 31) {
                                     const float wd = m.nw_day[idx];
                                     apply_delta(m.nw_day + idx, wd,
-                                                -h.lr_w_day * (err_n * sd_nr * m.sd_rmb[e] + h.reg_w_day * wd), qm);
+                                                -h.lr_w_day * (err_n * sd_nr * m.sd_rmb[e] + h.reg_w_day * wd), nm);
                                 }
                                 const float cd = m.nc_day[idx];
-                                apply_delta(m.nc_day + idx, cd, -h.lr_c_day * (err_n * sd_nn + h.reg_c_day * cd), qm);
+                                apply_delta(m.nc_day + idx, cd, -h.lr_c_day * (err_n * sd_nn + h.reg_c_day * cd), nm);
                             }
                         }
                     }
@@ -996,6 +997,18 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_
                 if (lane == 0) sv.gacc_b[u] = Gb / norm;
             } else if (PASS == 2) {
                 // fold-in: nothing beyond the user's own parameters
+            } else if (last && S > 1) {
+                // stratified epoch: the user's whole-epoch NSVD1 gradient over its
+                // norm, for the item-side gather after the last stratum (the
+                // y rows stay fixed through the epoch's strata; k_nsvd_gather
+                // applies every user's step at once, divided by n_contrib^exp as
+                // in apply_nsvd_deltas, src/tsvdx4p.rs:341-385)
+                const float inv = 1.0f / norm;
+                float gq[VEC];
+#pragma unroll
+                for (int c = 0; c < VEC; ++c) gq[c] = G[c] * inv;
+                store_str<VEC, TEAM>(sv.gacc + prow, gq);
+                if (lane == 0) sv.gacc_b[u] = Gb * inv;
             } else if (last) {
                 // end-of-user NSVD1 step over R(u) (src/tx.rs:1099-1106)
                 const float inv = 1.0f / norm;

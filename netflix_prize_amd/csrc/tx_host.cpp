@@ -157,6 +157,9 @@ struct tx_ctx {
     bool iv_built = false;
     tx::ItemView iv{};
     float* d_scale = nullptr;
+    // the stratified epoch's NSVD1 apply: n_contrib^e per item, e = nsvd_norm_exp
+    // or 1 when that is 0
+    float* d_scale_strata = nullptr;
     float *d_gacc = nullptr, *d_gacc_b = nullptr;  // [U * fs], [U]
     // ycache equals the NSVD1 sums of the current y (rebuilt since y last changed)
     bool ycache_current = false;
@@ -406,6 +409,8 @@ inline uint4 pack_rec(const tx_ctx* c, int64_t t) {
 // count (largest first to the lightest block), segments of each user's
 // ratings per item block in the rotation order of its block, and per
 // (stratum, cell) user queues sorted by segment length (longest first).
+int build_item_view(tx_ctx* c);
+
 int build_strata(tx_ctx* c, int S) {
     c->free_strata();
     const int64_t U = c->U, I = c->I, N = c->N;
@@ -488,6 +493,7 @@ int build_strata(tx_ctx* c, int S) {
     sv = tx::StrataView{};
     sv.q_mode = c->sched.update_flags & 3;
     sv.y_mode = (c->sched.update_flags >> 2) & 3;
+    sv.nb_mode = (c->sched.update_flags >> 5) & 3;
     sv.block_threads = (c->sched.teams_per_block > 0 ? c->sched.teams_per_block : 4) * 64;
     sv.n_strata = S;
     sv.n_cells = C;
@@ -522,6 +528,8 @@ int build_strata(tx_ctx* c, int S) {
         sv.p_rec = pr;
         sv.gacc = ga;
         sv.gacc_b = gb;
+        // the epoch's NSVD1 step is gathered per item after the last stratum
+        if (int rc = build_item_view(c)) return rc;
     }
     c->built_S = S;
     return 0;
@@ -774,6 +782,12 @@ int build_item_view(tx_ctx* c) {
     RC(c->upload(&d_items, items.data(), items.size()));
     RC(c->alloc(&d_head, 1));
     RC(c->upload(&c->d_scale, scale.data(), scale.size()));
+    const float exs = ex > 0.0f ? ex : 1.0f;
+    for (int64_t i = 0; i < I; ++i) {
+        const uint32_t nc = start[i + 1] - start[i];
+        scale[i] = nc ? std::pow(static_cast<float>(nc), exs) : 0.0f;
+    }
+    RC(c->upload(&c->d_scale_strata, scale.data(), scale.size()));
     // each user's NSVD1 gradient over its norm, written by the user pass and
     // gathered per item
     RC(c->alloc(&c->d_gacc, static_cast<size_t>(std::max<int32_t>(c->U, 1)) * c->fs));
@@ -865,6 +879,8 @@ int enqueue_strata_epoch(tx_ctx* c, hipStream_t st) {
     else
         for (int s = 0; s < S; ++s)
             HIPC(tx::launch_sgd_par(c->dm, c->sv, s, c->shape.team, c->shape.vec, blocks, qf, p2, ord, 0, day, st));
+    if (S > 1)
+        HIPC(tx::launch_nsvd_gather(c->dm, c->iv, c->sv.gacc, c->sv.gacc_b, c->d_scale_strata, c->shape.vec, st));
     HIPC(tx::launch_rebuild(c->dm, c->n_ud, c->shape.team, c->shape.vec, st));
     return 0;
 }
@@ -1497,12 +1513,9 @@ int tx_set_schedule(tx_ctx* c, const tx_schedule* s) {
     if (s->teams_per_block < 0 || s->teams_per_block > 4)
         return fail(TX_EINVAL, "teams_per_block must be 0 (4) or 1..4, got %d", s->teams_per_block);
     if (s->heavy_min < 0) return fail(TX_EINVAL, "heavy_min must be >= 0");
-    const int qm = s->update_flags & 3, ym = (s->update_flags >> 2) & 3;
-    if (qm > 2 || ym > 2 || (s->update_flags >> 5) != 0) return fail(TX_EINVAL, "bad update_flags %d", s->update_flags);
-    if (s->mode == TX_SCHED_STRATA && s->n_strata > 1 && !(s->update_flags & TX_UPD_STRATA_EXPERIMENTAL))
-        return fail(TX_EUNSUPPORTED,
-                    "n_strata > 1 applies every NSVD1 step at the last stratum and diverges at the full Netflix "
-                    "shape; set TX_UPD_STRATA_EXPERIMENTAL in update_flags to run it anyway");
+    const int qm = s->update_flags & 3, ym = (s->update_flags >> 2) & 3, nm = (s->update_flags >> 5) & 3;
+    if (qm > 2 || ym > 2 || nm == 1 || nm > 2 || (s->update_flags >> 7) != 0)
+        return fail(TX_EINVAL, "bad update_flags %d", s->update_flags);
     const int want_S = s->mode == TX_SCHED_TSVDX4P ? 1 : s->n_strata;
     const bool relayout = s->mode != TX_SCHED_SERIAL && s->mode != TX_SCHED_PLAIN && want_S != c->built_S;
     c->sched = *s;
@@ -1512,6 +1525,7 @@ int tx_set_schedule(tx_ctx* c, const tx_schedule* s) {
     }
     c->sv.q_mode = qm;
     c->sv.y_mode = ym;
+    c->sv.nb_mode = nm;
     c->sv.block_threads = (s->teams_per_block > 0 ? s->teams_per_block : 4) * 64;
     if (c->graph) {
         (void)hipGraphExecDestroy(c->graph);
@@ -1634,6 +1648,9 @@ int tx_fit_epoch(tx_ctx* c, int32_t epoch) {
             else
                 for (int s = 0; s < S; ++s)
                     HIPC(tx::launch_sgd_par(c->dm, c->sv, s, c->shape.team, c->shape.vec, blocks, qf, p2, ord, 0, day,
+                                            st));
+            if (S > 1)
+                HIPC(tx::launch_nsvd_gather(c->dm, c->iv, c->sv.gacc, c->sv.gacc_b, c->d_scale_strata, c->shape.vec,
                                             st));
             HIPC(hipEventRecord(c->ev[2], st));
             HIPC(tx::launch_rebuild(c->dm, c->n_ud, c->shape.team, c->shape.vec, st));

@@ -136,6 +136,7 @@ struct StrataView {
     // user passes: the NSVD1 sum of a user is ycache[u] (current, and built
     // over the same rows: full_su), not gathered again from y
     int32_t su_cached;
+    int32_t nb_mode;            // neighbourhood-table update: 0 device atomic, 2 plain
     int32_t block_threads;      // SGD workgroup size of the one-wave teams: 64 x teams per block (0 = 256)
 };
 

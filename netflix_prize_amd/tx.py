@@ -405,13 +405,14 @@ class TxModel:
 
     def set_schedule(self, mode: int = SCHED_STRATA, n_strata: int = 1, blocks: int = 0, use_graph: bool = False,
                      teams_per_block: int = 0, q_update: int = 0, y_update: int = 2,
-                     user_order: int = 2, heavy_min: int = 1024, experimental: bool = False) -> None:
+                     user_order: int = 2, heavy_min: int = 1024, experimental: bool = False,
+                     nbr_update: int = 0) -> None:
         """tx_set_schedule; experimental=True is needed for n_strata > 1
         (TX_UPD_STRATA_EXPERIMENTAL: that schedule diverges at full shape)."""
         s = CSchedule()
         s.user_order = int(user_order)
         s.heavy_min = int(heavy_min)
-        s.update_flags = int(q_update) | (int(y_update) << 2) | (16 if experimental else 0)
+        s.update_flags = int(q_update) | (int(y_update) << 2) | (16 if experimental else 0) | (int(nbr_update) << 5)
         s.mode = int(mode)
         s.n_strata = int(n_strata)
         s.blocks = int(blocks)

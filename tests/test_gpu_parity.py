@@ -95,7 +95,7 @@ def test_parallel_schedule_holdout_rmse(gpu, n_strata):
     (within 0.001) on the 10k x 2k x 1M shape."""
     tr, pr = synthetic("small")
     cfg = lm_config(n_feat=32, n_epochs=5, n_time_bins=30)
-    m_o, m_g = _oracle_and_gpu(cfg, tr, pr, mode=tx.SCHED_STRATA, n_strata=n_strata, experimental=n_strata > 1)
+    m_o, m_g = _oracle_and_gpu(cfg, tr, pr, mode=tx.SCHED_STRATA, n_strata=n_strata)
     for e in range(1, cfg.n_epochs + 1):
         m_o.fit_epoch(e)
         m_g.fit_epoch(epoch=e)
@@ -244,7 +244,7 @@ def test_day_terms_schedule_rules(gpu):
     cfg = lm_config(n_feat=24, n_epochs=1, n_time_bins=30, low_memory=False)
     with tx.TxModel(tr, pr, cfg) as m:
         with pytest.raises(tx.TxError):
-            m.set_schedule(mode=tx.SCHED_STRATA, n_strata=8, experimental=True)
+            m.set_schedule(mode=tx.SCHED_STRATA, n_strata=8)
         m.set_schedule(mode=tx.SCHED_STRATA, n_strata=1)
     # one user's ratings out of date order: the default is the serial schedule,
     # and the parallel one is refused
@@ -353,9 +353,41 @@ def test_plain_schedule_matches_serial_oracle(gpu):
     assert abs(r_g - r_o) <= 1e-3
 
 
-def test_multi_strata_needs_the_experimental_flag(gpu):
+def test_strata_counts(gpu):
+    """n_strata is 1 or a multiple of 8 up to 64 (8 cells: one XCD each); the
+    old experimental bit is still accepted."""
     tr, pr = synthetic("tiny_synth")
     with tx.TxModel(tr, pr, lm_config(n_feat=16, n_epochs=1)) as m:
-        with pytest.raises(tx.TxError, match="TX_UPD_STRATA_EXPERIMENTAL"):
-            m.set_schedule(mode=tx.SCHED_STRATA, n_strata=8)
-        m.set_schedule(mode=tx.SCHED_STRATA, n_strata=8, experimental=True)
+        for bad in (2, 12, 72):
+            with pytest.raises(tx.TxError, match="n_strata"):
+                m.set_schedule(mode=tx.SCHED_STRATA, n_strata=bad)
+        m.set_schedule(mode=tx.SCHED_STRATA, n_strata=8)
+        m.set_schedule(mode=tx.SCHED_STRATA, n_strata=64, experimental=True)
+        m.fit_epoch(epoch=1)
+
+
+@pytest.mark.parametrize("n_strata", [8, 64])
+def test_full_shape_stratified_matches_serial_reference(gpu, n_strata):
+    """The block-stratified schedule at the full Netflix shape, plain form,
+    f=256, 20 epochs: every user's ratings split over n_strata item blocks, cell
+    c of stratum s = user block c x item block (c + s) mod n_strata on XCD c mod
+    8, the NSVD1 step gathered per item after the last stratum (n_contrib^1
+    normalisation); holdout RMSE within 1e-3 of the serial oracle's after the
+    last epoch (tests/golden/oracle_curve_full_f256.json)."""
+    import json
+    import os
+    golden = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "oracle_curve_full_f256.json")))
+    tr, pr = _full_data(golden["seed"])
+    cfg = tx.plain_form(lm_config(n_feat=256, n_epochs=20, n_time_bins=30, reset_u_epoch=10_000))
+    with tx.TxModel(tr, pr, cfg) as m:
+        m.set_schedule(mode=tx.SCHED_STRATA, n_strata=n_strata)
+        curve, ms = [], []
+        for e in range(1, 21):
+            m.fit_epoch(epoch=e)
+            ms.append(m.last_epoch_stats()["ms_total"])
+            curve.append(m.calc_rmse(pr))
+    ref = [x["holdout_rmse"] for x in golden["epochs"]]
+    print(f"{n_strata} strata: {np.median(ms):.1f} ms/epoch")
+    print("gpu", [round(x, 5) for x in curve])
+    print("ref", [round(x, 5) for x in ref])
+    assert abs(curve[-1] - ref[-1]) <= 1e-3
