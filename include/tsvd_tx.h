@@ -87,7 +87,11 @@ typedef struct tx_ratings {
  *     identical to the serial reference trainer (src/tx.rs:734-1115).
  *   TX_SCHED_STRATA: all users in parallel, one team of lanes per user, items
  *     split into n_strata blocks processed as a rotating user-block x
- *     item-block schedule (n_strata = 1: every item in one stratum).
+ *     item-block schedule (n_strata = 1: every item in one stratum, each
+ *     user's NSVD1 step at its end; n_strata > 1: the y rows stay fixed
+ *     through the epoch's strata and the NSVD1 steps are applied per item after
+ *     the last one, divided by n_contrib^nsvd_norm_exp, exponent 1 when
+ *     nsvd_norm_exp is 0, as apply_nsvd_deltas, src/tsvdx4p.rs:341-385).
  *   TX_SCHED_TSVDX4P: the epoch of Tsvdx4pModel::fit_epoch
  *     (src/tsvdx4p.rs:798-805): a user pass with item tables frozen and the
  *     NSVD1 gradients accumulated per item, the delta apply divided by
@@ -129,11 +133,9 @@ typedef struct tx_schedule {
  *       i.e. item rows with n_strata >= 8),
  *   2 = plain read-modify-write (lock-free "Hogwild" updates).
  * Default (tx_create): atomic item rows, plain NSVD1 rows.
- * Bit 4 (TX_UPD_STRATA_EXPERIMENTAL) is required for n_strata > 1: that
- * schedule applies every user's NSVD1 step at the last stratum, and at the
- * full Netflix shape (f = 256) its holdout RMSE diverges (NaN with atomic y
- * rows, 1.136 after 6 epochs with plain y rows); without the bit
- * tx_set_schedule returns TX_EUNSUPPORTED.
+ * Bit 4 (TX_UPD_STRATA_EXPERIMENTAL) is accepted and ignored (it was required
+ * for n_strata > 1 while that schedule applied the NSVD1 steps inside the last
+ * stratum; it now gathers them per item after it).
  * Bits 5-6: the neighbourhood tables (w, c, w_day, c_day), 0 (default) or 2;
  * set as (mode << TX_UPD_NBR_SHIFT). */
 #define TX_UPD_NBR_SHIFT 5

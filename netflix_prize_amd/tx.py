@@ -407,8 +407,9 @@ class TxModel:
                      teams_per_block: int = 0, q_update: int = 0, y_update: int = 2,
                      user_order: int = 2, heavy_min: int = 1024, experimental: bool = False,
                      nbr_update: int = 0) -> None:
-        """tx_set_schedule; experimental=True is needed for n_strata > 1
-        (TX_UPD_STRATA_EXPERIMENTAL: that schedule diverges at full shape)."""
+        """tx_set_schedule.  q_update / y_update / nbr_update: 0 atomic, 2 plain
+        updates of the item rows, the NSVD1 rows and the neighbourhood tables;
+        experimental is accepted and ignored (n_strata > 1 no longer needs it)."""
         s = CSchedule()
         s.user_order = int(user_order)
         s.heavy_min = int(heavy_min)
