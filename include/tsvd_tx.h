@@ -122,9 +122,22 @@ typedef struct tx_schedule {
                                 order (get_users), 1 = longest user first, 2 = length
                                 buckets: the reference's order with the users of at
                                 least heavy_min ratings spread evenly over the first
-                                half of the epoch (so no long user starts near its end) */
+                                half of the epoch (so no long user starts near its end),
+                                3 = length buckets launched separately (below) */
     int32_t heavy_min;       /* user_order 2: bound of the long-user bucket, 0 = 1024 */
 } tx_schedule;
+/* user_order 3 (one-stratum TX_SCHED_STRATA, f <= 1024): the users in length
+ * buckets by training-rating count, [1, 16], [17, 256], [257, 4096] and
+ * [4097, ...), each bucket its own queue (the reference's per-epoch order
+ * within it) and its own launch, all launched together: per bucket a team
+ * shape (lanes per user: 16-lane teams, four users per wave, or one 64-lane
+ * wave; waves per user: 1 or 2, each wave half of every row) and a number of
+ * teams in flight.  Defaults: 16 lanes for the first bucket (padded row <=
+ * 256), two waves for the last (128 <= padded row <= 1024), for low-memory
+ * models without ordinal head or neighbourhood terms; the fidelity bound's
+ * teams split by each bucket's ratings.  tx_autotune chooses them per bucket
+ * for this order; tx_get_buckets reports them. */
+#define TX_N_BUCKETS 4
 
 /* update_flags: bits 0-1 item rows/biases (q, qf, b_i, b_i,bin, b_i,f),
  * bits 2-3 NSVD1 rows (y, y-bias):
@@ -214,6 +227,27 @@ int tx_set_schedule(tx_ctx* ctx, const tx_schedule* sched);
  * are unchanged; *best (may be NULL) receives the chosen schedule.  Needs
  * device memory for one copy of the parameter tables. */
 int tx_autotune(tx_ctx* ctx, int32_t trial_epochs, tx_schedule* best);
+
+/* One length bucket of user_order 3 (no reference counterpart). */
+typedef struct tx_bucket_info {
+    int32_t lo, hi;            /* bounds of the bucket's training-rating counts (inclusive) */
+    int64_t users, ratings;    /* users of the bucket and their training ratings */
+    int32_t lanes_per_user;    /* 16 (four users per wave) or 64 */
+    int32_t waves_per_user;    /* 1 or 2 */
+    int32_t teams;             /* teams in flight */
+    int32_t reserved;
+    double ms;                 /* the bucket's launch in the last epoch run with stream launches, else 0 */
+} tx_bucket_info;
+
+/* The buckets of user_order 3 (out[TX_N_BUCKETS]); valid once a schedule
+ * with user_order 3 has been set. */
+int tx_get_buckets(tx_ctx* ctx, tx_bucket_info* out);
+
+/* Set one bucket's team shape and teams in flight (user_order 3; the shapes
+ * other than 64 lanes x 1 wave need a low-memory model without ordinal head
+ * or neighbourhood terms: 16 lanes with a padded row <= 256, 2 waves with a
+ * padded row of 128 to 1024, not both). */
+int tx_set_bucket_shape(tx_ctx* ctx, int32_t bucket, int32_t lanes_per_user, int32_t waves_per_user, int32_t teams);
 
 /* Regressor::fit_epoch (src/lib.rs:630, src/tx.rs:734-1115); epoch is
  * 1-based.  Includes the epoch-end ycache rebuild (src/tx.rs:1109-1110). */

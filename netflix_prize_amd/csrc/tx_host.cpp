@@ -58,8 +58,11 @@ namespace {
         if (e_ != hipSuccess) return fail(TX_EDEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
     } while (0)
 
-#define RC(x) \
-    if (int rc_ = (x)) return rc_
+// a statement like HIPC (safe inside if / else without braces)
+#define RC(x)                          \
+    do {                               \
+        if (int rc_ = (x)) return rc_; \
+    } while (0)
 
 // Run fn(lo, hi) over [0, n) on up to hardware_concurrency threads.
 template <typename Fn>
@@ -189,6 +192,23 @@ struct tx_ctx {
     uint16_t* d_smp = nullptr;
     int32_t smp_epoch = -1;  // epoch whose samples are on the device
 
+    // length-bucketed epoch (user_order 3): users by training-rating count in
+    // [1,16], [17,256], [257,4096], [4097,...), each bucket its own queue and
+    // its own concurrent launch with its own team shape and teams in flight
+    struct Bucket {
+        int32_t lo, hi;
+        int64_t users = 0, ratings = 0;
+        int lanes = 64, waves = 1, teams = 0;
+        double ms = 0.0;
+    };
+    Bucket bk[TX_N_BUCKETS] = {{1, 16}, {17, 256}, {257, 4096}, {4097, 0x7fffffff}};
+    bool bk_ready = false;
+    uint32_t* d_bq_start = nullptr;  // [5] queue offsets of the buckets
+    uint32_t* d_bq_head = nullptr;   // [4] their work counters
+    hipStream_t bst[TX_N_BUCKETS - 1] = {};
+    hipEvent_t bev_fork = nullptr, bev_join[TX_N_BUCKETS - 1] = {};
+    hipEvent_t bev_t[2 * TX_N_BUCKETS] = {};
+
     std::vector<uint32_t> h_queue;  // pinned order of the current epoch's queue
     // the next epoch's queue, built on the host while the GPU runs this one
     std::vector<uint32_t> h_queue_next;
@@ -250,6 +270,13 @@ struct tx_ctx {
             if (g->st) (void)hipStreamDestroy(g->st);
         }
         for (void* p : allocs) (void)hipFree(p);
+        for (auto& x : bst)
+            if (x) (void)hipStreamDestroy(x);
+        for (auto& e : bev_join)
+            if (e) (void)hipEventDestroy(e);
+        for (auto& e : bev_t)
+            if (e) (void)hipEventDestroy(e);
+        if (bev_fork) (void)hipEventDestroy(bev_fork);
         for (int b = 0; b < 2; ++b) {
             if (h_smp[b]) (void)hipHostFree(h_smp[b]);
             if (h_smp_cnt[b]) (void)hipHostFree(h_smp_cnt[b]);
@@ -608,7 +635,14 @@ void build_queue(const tx_ctx* c, int32_t epoch, std::vector<uint32_t>& q) {
     std::vector<int64_t> order(c->U);
     tx::user_order(c->U, c->cfg.shuffle_users != 0, c->cfg.seed, epoch, order.data());
     q.clear();
-    if (c->sched.user_order == 2) {
+    if (c->sched.user_order == 3) {
+        // the buckets one after another, each in the epoch's order
+        for (int b = 0; b < TX_N_BUCKETS; ++b)
+            for (int64_t u : order) {
+                const int64_t cnt = c->t_off[u + 1] - c->t_off[u];
+                if (cnt >= c->bk[b].lo && cnt <= c->bk[b].hi) q.push_back(static_cast<uint32_t>(u));
+            }
+    } else if (c->sched.user_order == 2) {
         const uint32_t hmin = c->sched.heavy_min > 0 ? static_cast<uint32_t>(c->sched.heavy_min) : 1024u;
         std::vector<uint32_t> heavy, light;
         for (int64_t u : order) {
@@ -647,10 +681,6 @@ int prepare_user_queue(tx_ctx* c, int32_t epoch, hipStream_t st) {
     return 0;
 }
 
-// The tsvdx4p layout: ratings grouped by item in training-file order
-// (ByItemIndex, src/tsvdx4p.rs:29-57) with the user, bins, dev and target of
-// each; items queued most ratings first; NSVD1 accumulators; per-item scale
-// n_contrib^nsvd_norm_exp (src/tsvdx4p.rs:356-365) with the host libm powf.
 // The epoch's global neighbour samples (src/tx.rs:750-793): one
 // StdRng::seed_from_u64(seed + epoch * 1000003) stream, consumed user by user
 // in the epoch's order (get_users); a user whose |R(u)| + |N(u)\R(u)|
@@ -737,6 +767,10 @@ int draw_next_nbr_samples(tx_ctx* c, int32_t epoch) {
     return draw_nbr_samples(c, epoch + 1, 1 - c->smp_cur);
 }
 
+// The tsvdx4p layout: ratings grouped by item in training-file order
+// (ByItemIndex, src/tsvdx4p.rs:29-57) with the user, bins, dev and target of
+// each; items queued most ratings first; NSVD1 accumulators; per-item scale
+// n_contrib^nsvd_norm_exp (src/tsvdx4p.rs:356-365) with the host libm powf.
 int build_item_view(tx_ctx* c) {
     if (c->iv_built) return 0;
     const int64_t I = c->I, N = c->N;
@@ -865,6 +899,102 @@ int tsvdx4p_epoch(tx_ctx* c, int32_t epoch, tx_epoch_stats& stats, hipStream_t s
     return 0;
 }
 
+// Whether the bucket shapes other than one 64-lane wave per user apply:
+// low-memory models without ordinal head or neighbourhood terms.
+bool bucket_shapes_ok(const tx_ctx* c) {
+    return c->cfg.low_memory && !c->cfg.has_ordinal && !c->nb && c->shape.waves == 1;
+}
+
+// Bucket sizes, default shapes and teams in flight (once per ctx): 16-lane
+// teams for the shortest users (f <= 256), two-wave teams for the longest
+// (128 <= padded row <= 1,024), one wave elsewhere; the teams of the
+// fidelity bound split in proportion to each bucket's ratings plus a
+// 16-rating per-user overhead, at least 8 (at most the bucket's users).
+int init_buckets(tx_ctx* c) {
+    if (c->bk_ready) return 0;
+    for (auto& b : c->bk) b.users = b.ratings = 0;
+    for (int64_t u = 0; u < c->U; ++u) {
+        const int64_t cnt = c->t_off[u + 1] - c->t_off[u];
+        for (auto& b : c->bk)
+            if (cnt >= b.lo && cnt <= b.hi) {
+                b.users++;
+                b.ratings += cnt;
+            }
+    }
+    const bool shapes = bucket_shapes_ok(c);
+    double work[TX_N_BUCKETS], tot = 0.0;
+    for (int b = 0; b < TX_N_BUCKETS; ++b) {
+        work[b] = static_cast<double>(c->bk[b].ratings) + 16.0 * static_cast<double>(c->bk[b].users);
+        tot += work[b];
+    }
+    const int T = target_teams(c->cfg.n_feat);
+    for (int b = 0; b < TX_N_BUCKETS; ++b) {
+        auto& k = c->bk[b];
+        k.lanes = (b == 0 && shapes && c->fs <= 256) ? 16 : 64;
+        k.waves = (b == TX_N_BUCKETS - 1 && shapes && c->fs >= 128 && c->fs <= 1024) ? 2 : 1;
+        const int64_t want = tot > 0.0 ? static_cast<int64_t>(T * work[b] / tot + 0.5) : 8;
+        k.teams = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(std::max<int64_t>(want, 8), k.users)));
+        k.ms = 0.0;
+    }
+    uint32_t off[TX_N_BUCKETS + 1] = {0};
+    for (int b = 0; b < TX_N_BUCKETS; ++b) off[b + 1] = off[b] + static_cast<uint32_t>(c->bk[b].users);
+    RC(c->upload(&c->d_bq_start, off, TX_N_BUCKETS + 1));
+    RC(c->alloc(&c->d_bq_head, TX_N_BUCKETS));
+    for (auto& x : c->bst) HIPC(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
+    HIPC(hipEventCreateWithFlags(&c->bev_fork, hipEventDisableTiming));
+    for (auto& e : c->bev_join) HIPC(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    for (auto& e : c->bev_t) HIPC(hipEventCreate(&e));
+    c->bk_ready = true;
+    return 0;
+}
+
+// One launch per bucket, concurrently: bucket 0 on the ctx's stream, the
+// others on their own streams forked from it and joined back.  `timed`
+// records each launch between events (stream launches only, not under graph
+// capture).
+int enqueue_buckets(tx_ctx* c, hipStream_t st, bool timed) {
+    const bool qf = c->cfg.lr_iqf != 0.0f || c->cfg.sigma_iqf != 0.0f;
+    const bool p2 = c->cfg.lr_u2 != 0.0f;
+    const bool ord = c->cfg.has_ordinal != 0;
+    const bool day = c->cfg.low_memory == 0;
+    HIPC(tx::launch_fill_u32(c->d_bq_head, 0u, TX_N_BUCKETS, st));
+    HIPC(hipEventRecord(c->bev_fork, st));
+    for (int b = 1; b < TX_N_BUCKETS; ++b) HIPC(hipStreamWaitEvent(c->bst[b - 1], c->bev_fork, 0));
+    for (int b = 0; b < TX_N_BUCKETS; ++b) {
+        const auto& k = c->bk[b];
+        if (k.users == 0) continue;
+        hipStream_t s = b == 0 ? st : c->bst[b - 1];
+        tx::StrataView sv = c->sv;
+        sv.q_start = c->d_bq_start + b;
+        sv.q_head = c->d_bq_head + b;
+        if (timed) HIPC(hipEventRecord(c->bev_t[2 * b], s));
+        if (k.lanes == 16) {
+            HIPC(tx::launch_sgd_par_t16(c->dm, sv, (k.teams + 15) / 16, qf, p2, s));
+        } else if (k.waves == 2) {
+            HIPC(tx::launch_sgd_par_w2(c->dm, sv, k.teams, qf, p2, s));
+        } else {
+            sv.block_threads = 64;  // one team per workgroup: teams = blocks
+            HIPC(tx::launch_sgd_par(c->dm, sv, 0, c->shape.team, c->shape.vec, k.teams, qf, p2, ord, 0, day, s));
+        }
+        if (timed) HIPC(hipEventRecord(c->bev_t[2 * b + 1], s));
+    }
+    for (int b = 1; b < TX_N_BUCKETS; ++b) {
+        HIPC(hipEventRecord(c->bev_join[b - 1], c->bst[b - 1]));
+        HIPC(hipStreamWaitEvent(st, c->bev_join[b - 1], 0));
+    }
+    return 0;
+}
+
+// Per-bucket launch times of the last timed epoch.
+int read_bucket_times(tx_ctx* c) {
+    for (int b = 0; b < TX_N_BUCKETS; ++b) {
+        float ms = 0.0f;
+        if (c->bk[b].users > 0) HIPC(hipEventElapsedTime(&ms, c->bev_t[2 * b], c->bev_t[2 * b + 1]));
+        c->bk[b].ms = ms;
+    }
+    return 0;
+}
+
 // Enqueue one parallel epoch (queue reset, S strata, ycache rebuild).
 int enqueue_strata_epoch(tx_ctx* c, hipStream_t st) {
     const int S = c->built_S;
@@ -874,7 +1004,9 @@ int enqueue_strata_epoch(tx_ctx* c, hipStream_t st) {
     const bool day = c->cfg.low_memory == 0;
     const int blocks = c->sched.blocks > 0 ? c->sched.blocks : default_blocks(c);
     HIPC(tx::launch_fill_u32(c->sv.q_head, 0u, static_cast<int64_t>(S) * c->sv.n_cells, st));
-    if (c->shape.waves > 1)
+    if (c->sched.user_order == 3 && S == 1)
+        RC(enqueue_buckets(c, st, false));
+    else if (c->shape.waves > 1)
         HIPC(tx::launch_sgd_par_mw(c->dm, c->sv, blocks, c->shape.waves, qf, p2, ord, day, st));
     else
         for (int s = 0; s < S; ++s)
@@ -1509,7 +1641,9 @@ int tx_set_schedule(tx_ctx* c, const tx_schedule* s) {
         if (c->shape.waves > 1 && c->cfg.has_ordinal)
             return fail(TX_EUNSUPPORTED, "neighbourhood terms with an ordinal head train in parallel up to n_feat 1024");
     }
-    if (s->user_order < 0 || s->user_order > 2) return fail(TX_EINVAL, "user_order must be 0, 1 or 2");
+    if (s->user_order < 0 || s->user_order > 3) return fail(TX_EINVAL, "user_order must be 0, 1, 2 or 3");
+    if (s->user_order == 3 && (s->mode != TX_SCHED_STRATA || s->n_strata != 1 || c->shape.waves > 1))
+        return fail(TX_EUNSUPPORTED, "length-bucketed launches (user_order 3): one-stratum TX_SCHED_STRATA, f <= 1024");
     if (s->teams_per_block < 0 || s->teams_per_block > 4)
         return fail(TX_EINVAL, "teams_per_block must be 0 (4) or 1..4, got %d", s->teams_per_block);
     if (s->heavy_min < 0) return fail(TX_EINVAL, "heavy_min must be >= 0");
@@ -1526,6 +1660,8 @@ int tx_set_schedule(tx_ctx* c, const tx_schedule* s) {
     c->sv.q_mode = qm;
     c->sv.y_mode = ym;
     c->sv.nb_mode = nm;
+    if (s->user_order == 3)
+        if (int rc = init_buckets(c)) return rc;
     c->sv.block_threads = (s->teams_per_block > 0 ? s->teams_per_block : 4) * 64;
     if (c->graph) {
         (void)hipGraphExecDestroy(c->graph);
@@ -1643,7 +1779,10 @@ int tx_fit_epoch(tx_ctx* c, int32_t epoch) {
             const int blocks = c->sched.blocks > 0 ? c->sched.blocks : default_blocks(c);
             HIPC(tx::launch_fill_u32(c->sv.q_head, 0u, static_cast<int64_t>(S) * c->sv.n_cells, st));
             HIPC(hipEventRecord(c->ev[1], st));
-            if (c->shape.waves > 1)
+            const bool bucketed = c->sched.user_order == 3 && S == 1;
+            if (bucketed)
+                RC(enqueue_buckets(c, st, true));
+            else if (c->shape.waves > 1)
                 HIPC(tx::launch_sgd_par_mw(c->dm, c->sv, blocks, c->shape.waves, qf, p2, ord, day, st));
             else
                 for (int s = 0; s < S; ++s)
@@ -1667,6 +1806,10 @@ int tx_fit_epoch(tx_ctx* c, int32_t epoch) {
             stats.ms_total = t;
             stats.ms_other = t - a - b;
             stats.n_launches = S + 2;
+            if (bucketed) {
+                RC(read_bucket_times(c));
+                stats.n_launches = TX_N_BUCKETS + 2;
+            }
         }
         // OrdinalHead::enforce_sorted_with_gap at epoch end (src/tx.rs:1113):
         // the parallel schedule keeps lr_t = 0, so this only acts on th_init
@@ -1766,6 +1909,68 @@ int tx_autotune(tx_ctx* c, int32_t trial_epochs, tx_schedule* best_out) {
     };
     const tx_schedule orig = c->sched;
     const int tpb = teams_per_block(c), cap = target_teams(c->cfg.n_feat);
+    if (orig.user_order == 3) {
+        // length-bucketed launches: per bucket (most ratings first) its team
+        // shape and teams in flight, one at a time with the others held, each
+        // trial an epoch from the snapshot; teams in all stay within the
+        // fidelity bound
+        if (int rc = init_buckets(c)) { release(); return rc; }
+        tx_schedule s1 = orig;
+        s1.use_graph = 0;
+        if (int rc = tx_set_schedule(c, &s1)) { release(); return rc; }
+        auto trial = [&](double* ms) -> int {
+            if (int rc = restore()) return rc;
+            if (int rc = tx_fit_epoch(c, 1)) return rc;
+            *ms = c->last.ms_total;
+            return 0;
+        };
+        int order[TX_N_BUCKETS];
+        std::iota(order, order + TX_N_BUCKETS, 0);
+        std::sort(order, order + TX_N_BUCKETS, [&](int a, int b) { return c->bk[a].ratings > c->bk[b].ratings; });
+        double best_ms = 0.0;
+        int rc = trial(&best_ms);
+        for (int b : order) {
+            if (rc) break;
+            auto& k = c->bk[b];
+            if (k.users == 0) continue;
+            const tx_ctx::Bucket keep = k;
+            tx_ctx::Bucket best = k;
+            int others = 0;
+            for (int o = 0; o < TX_N_BUCKETS; ++o)
+                if (o != b) others += c->bk[o].teams;
+            std::vector<std::pair<int, int>> shapes = {{keep.lanes, keep.waves}};
+            if (bucket_shapes_ok(c)) {
+                if (c->fs <= 256 && keep.lanes != 16) shapes.push_back({16, 1});
+                if (keep.lanes != 64 || keep.waves != 1) shapes.push_back({64, 1});
+                if (c->fs >= 128 && c->fs <= 1024 && keep.waves != 2) shapes.push_back({64, 2});
+            }
+            for (const auto& sh : shapes)
+                for (double mult : {0.5, 1.0, 1.5}) {
+                    if (rc) break;
+                    const int t = std::max(1, static_cast<int>(std::min<int64_t>(keep.teams * mult + 0.5, k.users)));
+                    if (sh.first == keep.lanes && sh.second == keep.waves && t == keep.teams) continue;
+                    if (others + t > cap) continue;
+                    k.lanes = sh.first;
+                    k.waves = sh.second;
+                    k.teams = t;
+                    double ms = 0.0;
+                    if ((rc = trial(&ms))) break;
+                    if (ms < best_ms) {
+                        best_ms = ms;
+                        best = k;
+                    }
+                }
+            k = best;
+        }
+        const int rc2 = restore();
+        release();
+        if (rc) return rc;
+        if (rc2) return rc2;
+        if (int r = tx_set_schedule(c, &orig)) return r;  // back to the caller's graph setting
+        c->bf16_stale = true;
+        if (best_out) *best_out = orig;
+        return 0;
+    }
     std::vector<tx_schedule> cands;
     for (int quarters : {2, 3, 4}) {
         int blocks = (cap * quarters / 4 + tpb - 1) / tpb;
@@ -1844,6 +2049,42 @@ int tx_rebuild_ycache(tx_ctx* c) {
     HIPC(tx::launch_rebuild(c->dm, c->n_ud, c->shape.team, c->shape.vec, c->st));
     HIPC(hipStreamSynchronize(c->st));
     c->ycache_current = true;
+    return 0;
+}
+
+int tx_get_buckets(tx_ctx* c, tx_bucket_info* out) {
+    g_err.clear();
+    if (!c || !out) return fail(TX_EINVAL, "tx_get_buckets: NULL argument");
+    if (!c->bk_ready) return fail(TX_EINVAL, "tx_get_buckets: no schedule with user_order 3 has been set");
+    for (int b = 0; b < TX_N_BUCKETS; ++b) {
+        const auto& k = c->bk[b];
+        out[b] = tx_bucket_info{k.lo, k.hi, k.users, k.ratings, k.lanes, k.waves, k.teams, 0, k.ms};
+    }
+    return 0;
+}
+
+int tx_set_bucket_shape(tx_ctx* c, int32_t bucket, int32_t lanes, int32_t waves, int32_t teams) {
+    g_err.clear();
+    if (!c) return fail(TX_EINVAL, "tx_set_bucket_shape: NULL ctx");
+    if (!c->bk_ready) return fail(TX_EINVAL, "tx_set_bucket_shape: set a schedule with user_order 3 first");
+    if (bucket < 0 || bucket >= TX_N_BUCKETS) return fail(TX_EINVAL, "bucket must be 0..%d", TX_N_BUCKETS - 1);
+    if ((lanes != 16 && lanes != 64) || (waves != 1 && waves != 2) || (lanes == 16 && waves == 2) || teams < 1)
+        return fail(TX_EINVAL, "lanes_per_user 16 or 64, waves_per_user 1 or 2 (not with 16 lanes), teams >= 1");
+    if ((lanes == 16 || waves == 2) && !bucket_shapes_ok(c))
+        return fail(TX_EUNSUPPORTED, "bucket shapes other than one wave: low-memory models without ordinal head or "
+                                     "neighbourhood terms");
+    if (lanes == 16 && c->fs > 256) return fail(TX_EUNSUPPORTED, "16-lane teams need a padded row of at most 256");
+    if (waves == 2 && (c->fs < 128 || c->fs > 1024))
+        return fail(TX_EUNSUPPORTED, "two-wave teams need a padded row of 128 to 1024");
+    auto& k = c->bk[bucket];
+    k.lanes = lanes;
+    k.waves = waves;
+    k.teams = teams;
+    if (c->graph) {
+        (void)hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+        c->graph_S = 0;
+    }
     return 0;
 }
 

@@ -37,6 +37,11 @@ hipError_t launch_sgd_par_nb1_mw(const DevModel& m, const StrataView& sv, int te
                                  hipStream_t st);
 hipError_t launch_sgd_par_nb2_mw(const DevModel& m, const StrataView& sv, int teams, int waves, bool day,
                                  hipStream_t st);
+// length-bucket shapes (low-memory models, no ordinal head, one stratum):
+// 16-lane teams, 16 per 256-thread block (fs 64..256); two-wave teams, one
+// per 128-thread block (fs 128..1024)
+hipError_t launch_sgd_par_t16(const DevModel& m, const StrataView& sv, int blocks, bool qf, bool p2, hipStream_t st);
+hipError_t launch_sgd_par_w2(const DevModel& m, const StrataView& sv, int teams, bool qf, bool p2, hipStream_t st);
 hipError_t launch_to_bf16(const float* x, uint16_t* y, int64_t n, hipStream_t st);
 // the NSVD1 step of the tsvdx4p user pass from the stored per-user gradients
 // (gacc [U * fs], gacc_b [U]) gathered per item; vec = the shape's VEC

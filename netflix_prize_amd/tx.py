@@ -177,6 +177,16 @@ class CEpochStats(C.Structure):
                 ("graph", C.c_int32)]
 
 
+N_BUCKETS = 4  # TX_N_BUCKETS
+USER_ORDER_BUCKETS = 3  # length buckets launched separately
+
+
+class CBucketInfo(C.Structure):
+    _fields_ = [("lo", C.c_int32), ("hi", C.c_int32), ("users", C.c_int64), ("ratings", C.c_int64),
+                ("lanes_per_user", C.c_int32), ("waves_per_user", C.c_int32), ("teams", C.c_int32),
+                ("reserved", C.c_int32), ("ms", C.c_double)]
+
+
 def config_struct(cfg: TxConfig) -> CConfig:
     c = CConfig()
     c.seed = int(cfg.seed)
@@ -228,6 +238,8 @@ def lib() -> C.CDLL:
         L.tx_predict_device.argtypes = [vp, i64, vp, vp, vp, vp, vp]
         L.tx_predict_subscores.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp]
         L.tx_n_subscores.argtypes = [vp, C.POINTER(i32)]
+        L.tx_get_buckets.argtypes = [vp, vp]
+        L.tx_set_bucket_shape.argtypes = [vp, i32, i32, i32, i32]
         L.tx_calc_rmse.argtypes = [vp, vp, C.POINTER(C.c_double)]
         L.tx_param_len.argtypes = [vp, i32, C.POINTER(i64)]
         L.tx_get_param.argtypes = [vp, i32, vp, i64]
@@ -239,6 +251,7 @@ def lib() -> C.CDLL:
                      "tx_rebuild_ycache", "tx_set_table_precision", "tx_autotune",
                      "tx_last_epoch_stats",
                      "tx_predict", "tx_predict_device", "tx_predict_subscores", "tx_n_subscores", "tx_calc_rmse",
+                     "tx_get_buckets", "tx_set_bucket_shape",
                      "tx_param_len",
                      "tx_get_param", "tx_set_param", "tx_set_param_device", "tx_get_users", "tx_rand_normal"]:
             getattr(L, name).restype = C.c_int
@@ -439,6 +452,17 @@ class TxModel:
         s = CSchedule()
         _check(lib().tx_autotune(self._h, int(trial_epochs), C.byref(s)), "tx_autotune")
         return {k: getattr(s, k) for k, _ in CSchedule._fields_}
+
+    def buckets(self) -> list:
+        """The length buckets of user_order 3 (tx_get_buckets): bounds, users,
+        ratings, lanes and waves per user, teams in flight, last launch ms."""
+        arr = (CBucketInfo * N_BUCKETS)()
+        _check(lib().tx_get_buckets(self._h, arr), "tx_get_buckets")
+        return [{k: getattr(b, k) for k, _ in CBucketInfo._fields_ if k != "reserved"} for b in arr]
+
+    def set_bucket_shape(self, bucket: int, lanes_per_user: int, waves_per_user: int, teams: int) -> None:
+        _check(lib().tx_set_bucket_shape(self._h, int(bucket), int(lanes_per_user), int(waves_per_user), int(teams)),
+               "tx_set_bucket_shape")
 
     def set_table_precision(self, precision: str) -> None:
         """"f32" (default) or "bf16": the factor tables predictions read

@@ -1,0 +1,122 @@
+"""Length-bucketed launches (user_order 3): each bucket of users by training
+count its own queue, team shape and teams in flight, launched together
+(needs an MI355X)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from netflix_prize_amd import tx
+from oracle import oracle as O
+from tests.fixtures import lm_config, synthetic
+
+pytestmark = pytest.mark.gpu
+
+ALL_PARAMS = [p for p in tx.PARAMS if p != "thresholds"]
+
+
+def _disjoint_lengths(counts, n_hold=4, seed=5):
+    """One user per entry of `counts` with that many training ratings (plus
+    n_hold held-out ones), no item shared between users: every schedule of
+    users gives the serial result up to float association.  Rows sorted by
+    (user, date)."""
+    rng = np.random.default_rng(seed)
+    rows = {"tr": [], "pr": []}
+    base = 0
+    for u, n in enumerate(counts):
+        items = base + rng.permutation(n + n_hold)
+        base += n + n_hold
+        days = np.sort(rng.integers(0, 2243, n + n_hold))
+        for part, its, ds in (("tr", items[:n], days[:n]), ("pr", items[n:], days[n:])):
+            for k in range(len(its)):
+                rows[part].append((u, its[k], int(rng.integers(1, 6)), ds[k]))
+    out = []
+    for part in ("tr", "pr"):
+        a = np.array(rows[part], np.int64)
+        out.append(tx.Dataset.from_arrays(a[:, 0], a[:, 1], a[:, 2], a[:, 3], len(counts), base))
+    return out
+
+
+@pytest.mark.parametrize("f", [64, 256])
+def test_bucket_shapes_match_serial_on_disjoint_users(gpu, f):
+    """Users in all four buckets (3 to 5,000 training ratings), no shared
+    item: the bucketed launches with their default shapes (16-lane teams for
+    the 1-16 bucket; two-wave teams for the 4,097+ bucket at f=256) against
+    the serial oracle, every table within 1e-3 of its movement over two
+    epochs."""
+    counts = [3, 9, 16, 17, 40, 120, 256, 257, 700, 4096, 4097, 5000] * 2
+    tr, pr = _disjoint_lengths(counts)
+    cfg = lm_config(n_feat=f, n_epochs=2, n_time_bins=30, lr_iqf=0.005, lr_ibf=0.005, lr_y=0.005, reset_u_epoch=100)
+    m_o = O.OracleModel(cfg, tr, pr)
+    init = {n: m_o.get(n).copy() for n in ALL_PARAMS}
+    with tx.TxModel(tr, pr, cfg) as m:
+        m.set_schedule(mode=tx.SCHED_STRATA, user_order=tx.USER_ORDER_BUCKETS)
+        b = m.buckets()
+        assert [x["users"] for x in b] == [6, 8, 6, 4]
+        assert b[0]["lanes_per_user"] == 16
+        assert b[3]["waves_per_user"] == (2 if f == 256 else 1)
+        for e in (1, 2):
+            m.fit_epoch(epoch=e)
+            m_o.fit_epoch(e)
+        print([(x["lanes_per_user"], x["waves_per_user"], x["teams"], round(x["ms"], 3)) for x in m.buckets()])
+        for n in ALL_PARAMS:
+            o, g = m_o.get(n), m.get(n)
+            move = float(np.max(np.abs(o.astype(np.float64) - init[n]))) if o.size else 0.0
+            err = float(np.max(np.abs(g.astype(np.float64) - o))) if o.size else 0.0
+            print(f"{n}: movement {move:.3e} gpu-oracle {err:.3e}")
+            assert err <= 1e-3 * move + 1e-6, n
+
+
+def test_bucketed_schedule_holdout_rmse_and_autotune(gpu):
+    """The 10k x 2k x 1M set, f=32: the bucketed launches (autotuned per
+    bucket, from a snapshot that is restored) train to within 1e-3 of the
+    serial oracle's holdout RMSE after 5 epochs; a hipGraph-captured epoch
+    gives the same kind of result."""
+    tr, pr = synthetic("small")
+    cfg = lm_config(n_feat=32, n_epochs=5, n_time_bins=30)
+    m_o = O.OracleModel(cfg, tr, pr)
+    with tx.TxModel(tr, pr, cfg) as m, tx.TxModel(tr, pr, cfg) as mg:
+        m.set_schedule(mode=tx.SCHED_STRATA, user_order=tx.USER_ORDER_BUCKETS)
+        mg.set_schedule(mode=tx.SCHED_STRATA, user_order=tx.USER_ORDER_BUCKETS, use_graph=True)
+        before = {n: m.get(n) for n in ALL_PARAMS}
+        m.autotune()
+        for n, v in before.items():
+            np.testing.assert_array_equal(m.get(n), v, err_msg=n)
+        b = m.buckets()
+        assert sum(x["teams"] for x in b) <= 3072
+        for e in range(1, cfg.n_epochs + 1):
+            m.fit_epoch(epoch=e)
+            mg.fit_epoch(epoch=e)
+            m_o.fit_epoch(e)
+        r_o, r_g, r_gg = m_o.calc_rmse(), m.calc_rmse(pr), mg.calc_rmse(pr)
+        print("buckets", m.buckets())
+    print(f"oracle {r_o:.6f} bucketed {r_g:.6f} graph {r_gg:.6f}")
+    assert abs(r_g - r_o) <= 1e-3 and abs(r_gg - r_o) <= 1e-3
+
+
+def test_full_shape_bucketed_matches_serial_reference(gpu):
+    """Full Netflix shape, plain form, f=256, 20 epochs with the bucketed
+    launches (autotuned per bucket after the first epoch): holdout RMSE within
+    1e-3 of the serial oracle's (tests/golden/oracle_curve_full_f256.json);
+    the per-bucket shapes, teams and launch times are printed."""
+    from tests.fixtures import full_data
+    golden = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "oracle_curve_full_f256.json")))
+    tr, pr = full_data(golden["seed"])
+    cfg = tx.plain_form(lm_config(n_feat=256, n_epochs=20, n_time_bins=30, reset_u_epoch=10_000))
+    with tx.TxModel(tr, pr, cfg) as m:
+        m.set_schedule(mode=tx.SCHED_STRATA, user_order=tx.USER_ORDER_BUCKETS)
+        curve, ms = [], []
+        for e in range(1, 21):
+            m.fit_epoch(epoch=e)
+            if e == 1:
+                m.autotune()
+            ms.append(m.last_epoch_stats()["ms_total"])
+            curve.append(m.calc_rmse(pr))
+        for x in m.buckets():
+            print(x)
+    ref = [x["holdout_rmse"] for x in golden["epochs"]]
+    print(f"bucketed: {np.median(ms):.1f} ms/epoch")
+    print("gpu", [round(x, 5) for x in curve])
+    print("ref", [round(x, 5) for x in ref])
+    assert abs(curve[-1] - ref[-1]) <= 1e-3

@@ -89,6 +89,33 @@ def test_predict_matches_oracle_and_fp64(gpu):
     assert m_g.calc_rmse(pr) == pytest.approx(m_o.calc_rmse(), abs=1e-6)
 
 
+def test_full_shape_predict_fp64_oracle_on_1m_subset(gpu):
+    """Parity (a) at the full Netflix shape (the f=64 configuration): after
+    two epochs of the default parallel schedule, the GPU predictions for the
+    first 1M held-out ratings against the FP64 predict oracle evaluated on
+    the GPU model's own parameters: within 1e-5 per rating and 1e-6 RMS; the
+    FP32 oracle predictor within 1e-5."""
+    from tests.fixtures import full_data
+    tr, pr = full_data(1)
+    cfg = lm_config(n_feat=64, n_epochs=2, n_time_bins=30)
+    n = 1_000_000
+    u, i, d = pr.user_idxs[:n], pr.item_idxs[:n], pr.dates[:n]
+    with tx.TxModel(tr, pr, cfg) as m:
+        for e in (1, 2):
+            m.fit_epoch(epoch=e)
+        g = m.predict(u, i, d).astype(np.float64)
+        o = O.OracleModel(cfg, tr, pr)
+        for name in ALL_PARAMS:
+            o.set(name, m.get(name))
+    ref = o.predict_f64(u, i, d)
+    f32 = o.predict(u, i, d).astype(np.float64)
+    err = g - ref
+    print(f"vs FP64 oracle on {n} ratings: max {np.abs(err).max():.2e} rms {np.sqrt(np.mean(err ** 2)):.2e}; "
+          f"vs FP32 oracle max {np.abs(g - f32).max():.2e}")
+    assert np.abs(err).max() <= 1e-5 and np.sqrt(np.mean(err ** 2)) <= 1e-6
+    assert np.abs(g - f32).max() <= 1e-5
+
+
 @pytest.mark.parametrize("n_strata", [1, 8])
 def test_parallel_schedule_holdout_rmse(gpu, n_strata):
     """Parallel schedules train to the serial reference's holdout RMSE
