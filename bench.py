@@ -97,8 +97,10 @@ def profiled_traffic(f: int, args, kind: str = "") -> float | None:
 def atomic_ceiling() -> dict | None:
     """The device's rate for the SGD kernel's item-row update pattern
     (netflix_prize_amd/atomic-ceiling: 1 KB rows of a 17,770-row table updated
-    by 64-lane FP32 atomic adds, by 64-bit integer atomics, and by plain
-    load + store), as the best rate over its wave counts per form."""
+    by 64-lane FP32 atomic adds, by 64-bit integer atomics and by plain
+    load + store; a 2-lane FP32 atomic add on a 128-byte bias row; and the
+    kernel's whole per-rating item update, row + bias row), as the best rate
+    over its wave counts per form."""
     import subprocess
     exe = os.path.join(ROOT, "netflix_prize_amd", "atomic-ceiling")
     if not os.path.exists(exe):
@@ -112,8 +114,8 @@ def atomic_ceiling() -> dict | None:
         r = json.loads(line)
         if r["lanes_per_s"] > best.get(r["form"], {}).get("lanes_per_s", 0.0):
             best[r["form"]] = r
-    return {k: {"rows_per_s": v["rows_per_s"], "lanes_per_s": v["lanes_per_s"], "waves": v["waves"]}
-            for k, v in best.items()}
+    return {k: {"rows_per_s": v["rows_per_s"], "lanes_per_s": v["lanes_per_s"],
+                "atomic_instr_per_s": v.get("atomic_instr_per_s"), "waves": v["waves"]} for k, v in best.items()}
 
 
 def log(msg: str) -> None:
@@ -199,6 +201,8 @@ def main(argv=None) -> int:
                     help="use random item factors of this width instead of the trained model's (0 = trained)")
     ap.add_argument("--frozen-epochs", type=int, default=4,
                     help="epochs of the tsvdx4p schedule with FP32 and with BF16 frozen sides (0 = skip)")
+    ap.add_argument("--bucket-epochs", type=int, default=20,
+                    help="epochs of the length-bucketed launches (user_order 3) run (0 = skip)")
     ap.add_argument("--nbr-epochs", type=int, default=3,
                     help="epochs of each tsvdx5/6 neighbourhood model (0 = skip)")
     ap.add_argument("--day-epochs", type=int, default=3,
@@ -308,6 +312,29 @@ def main(argv=None) -> int:
             if tuned is not None:
                 ttr[form]["autotuned_schedule"] = {k: tuned[k] for k in ("blocks", "user_order", "heavy_min")}
             log(f"bench: time-to-RMSE {form}: {ttr[form]}")
+
+    # length-bucketed launches (user_order 3): the users of 1-16, 17-256,
+    # 257-4096 and 4097+ training ratings in four queues and launches, each with
+    # its own team shape and teams in flight, autotuned per bucket after the
+    # first epoch; epoch time, holdout RMSE after the epoch budget and the
+    # per-bucket choice against the one-queue default
+    buckets = None
+    if args.bucket_epochs > 0:
+        bm = tx.TxModel(tr, pr, cfg)
+        bm.set_schedule(mode=tx.SCHED_STRATA, user_order=tx.USER_ORDER_BUCKETS)
+        bms = []
+        for e in range(1, args.bucket_epochs + 1):
+            bm.fit_epoch(epoch=e)
+            if e == 1:
+                bm.autotune(1)
+            bms.append(bm.last_epoch_stats()["ms_total"])
+        r_b = bm.calc_rmse(pr)
+        buckets = {"epochs": args.bucket_epochs, "ms_per_epoch": round(statistics.median(bms[1:] or bms), 2),
+                   "final_holdout_rmse": round(r_b, 5),
+                   "buckets": [{k: (round(v, 3) if isinstance(v, float) else v) for k, v in b.items()}
+                               for b in bm.buckets()]}
+        bm.close()
+        log(f"bench: length-bucketed launches: {buckets}")
 
     # new users folded into a model trained on the other users: the encoder's
     # warm start against the reference's random start
@@ -507,17 +534,20 @@ def main(argv=None) -> int:
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(cfg.n_feat, args),
                 "kernel": "k_sgd_par", "bytes_per_rating": round(ab["sgd"], 1),
                 "avg_launch_ms": round(avg_launch_ms, 3), "launches_per_epoch": launches}
-    # the item-row updates are FP32 atomic adds: the lanes of one launch against
-    # the device's measured atomic rate for that pattern
+    # the item-row updates are FP32 atomic adds, which the device performs at a
+    # rate set by the 128-byte lines they touch, not by their lanes (a 2-lane
+    # add costs about as much as a 64-lane add on one line): the lines one
+    # launch updates against the device's measured line rate for the kernel's
+    # own per-rating pattern (q row + item-bias row, row+bias-atomic)
     ceil = atomic_ceiling()
-    if ceil and "f32-atomic" in ceil:
-        ibias_lanes = 2 if (cfg.lr_ibf == 0.0) else 3          # b_i, b_i,bin (+ b_i,f)
-        lanes = cfg.n_feat * (2 if qf else 1) + ibias_lanes
-        rate = ceil["f32-atomic"]["lanes_per_s"]
-        got = lanes * n_train / launches / (avg_launch_ms / 1000.0)
-        roofline["atomic"] = {"unit": "FP32 atomic lanes/s", "achieved": round(got, 1), "ceiling": round(rate, 1),
-                              "frac": round(got / rate, 4), "lanes_per_rating": lanes,
-                              "floor_ms": round(lanes * n_train / launches / rate * 1000.0, 3),
+    if ceil and "row+bias-atomic" in ceil:
+        row_lines = -(-cfg.n_feat * 4 // 128)                   # q row (and q_f row) in 128-byte lines
+        lines = row_lines * (2 if qf else 1) + 1                 # + the item-bias row (b_i, b_i,bin, b_i,f)
+        rate = ceil["row+bias-atomic"]["rows_per_s"] * (256 * 4 // 128 + 1)
+        got = lines * n_train / launches / (avg_launch_ms / 1000.0)
+        roofline["atomic"] = {"unit": "atomic 128-byte lines/s", "achieved": round(got, 1), "ceiling": round(rate, 1),
+                              "frac": round(got / rate, 4), "lines_per_rating": lines,
+                              "floor_ms": round(lines * n_train / launches / rate * 1000.0, 3),
                               "measured_forms": ceil}
         log(f"bench: atomic roofline {roofline['atomic']}")
 
@@ -556,6 +586,7 @@ def main(argv=None) -> int:
         "fold_in": fold,
         "day_terms": day,
         "neighbourhood": nbr,
+        "length_buckets": buckets,
         "tsvdx4p_frozen": frozen,
         "knnf": knnf,
         "cpu_baseline": cpu,
