@@ -129,14 +129,17 @@ typedef struct tx_schedule {
 /* user_order 3 (one-stratum TX_SCHED_STRATA, f <= 1024): the users in length
  * buckets by training-rating count, [1, 16], [17, 256], [257, 4096] and
  * [4097, ...), each bucket its own queue (the reference's per-epoch order
- * within it) and its own launch, all launched together: per bucket a team
- * shape (lanes per user: 16-lane teams, four users per wave, or one 64-lane
- * wave; waves per user: 1 or 2, each wave half of every row) and a number of
- * teams in flight.  Defaults: 16 lanes for the first bucket (padded row <=
- * 256), two waves for the last (128 <= padded row <= 1024), for low-memory
- * models without ordinal head or neighbourhood terms; the fidelity bound's
- * teams split by each bucket's ratings.  tx_autotune chooses them per bucket
- * for this order; tx_get_buckets reports them. */
+ * within it), team shape (lanes per user: 16-lane teams, four users per
+ * wave, or one 64-lane wave; waves per user: 1 or 2, each wave half of every
+ * row) and number of teams in flight.  With one-wave teams in every bucket
+ * (the default) one launch serves all buckets, its workgroups split between
+ * the bucket queues; any other shape (16 lanes: padded row <= 256; two
+ * waves: 128 <= padded row <= 1024; low-memory models without ordinal head
+ * or neighbourhood terms only) gives each bucket a launch of its own on a
+ * stream of its own, and kernels on streams that share a hardware queue run
+ * one after the other.  Default teams: the fidelity bound's, split by each
+ * bucket's ratings.  tx_autotune chooses shape and teams per bucket for this
+ * order; tx_get_buckets reports them. */
 #define TX_N_BUCKETS 4
 
 /* update_flags: bits 0-1 item rows/biases (q, qf, b_i, b_i,bin, b_i,f),
@@ -235,8 +238,9 @@ typedef struct tx_bucket_info {
     int32_t lanes_per_user;    /* 16 (four users per wave) or 64 */
     int32_t waves_per_user;    /* 1 or 2 */
     int32_t teams;             /* teams in flight */
-    int32_t reserved;
-    double ms;                 /* the bucket's launch in the last epoch run with stream launches, else 0 */
+    int32_t one_launch;        /* 1: the last bucketed epoch ran every bucket in one launch */
+    double ms;                 /* the bucket's launch in the last epoch run with stream launches
+                                  (with one launch: that launch), else 0 */
 } tx_bucket_info;
 
 /* The buckets of user_order 3 (out[TX_N_BUCKETS]); valid once a schedule

@@ -427,7 +427,11 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_
     uint32_t dpar = 0;
     const int S = sv.n_strata;
     const int cell = sv.n_cells > 1 ? static_cast<int>(blockIdx.x % sv.n_cells) : 0;
-    const int qid = stratum * sv.n_cells + cell;
+    // with length buckets in one launch the workgroup's range names its queue
+    // (bq_first nondecreasing: the count of bucket starts at or below it)
+    const int32_t bx = static_cast<int32_t>(blockIdx.x);
+    const int qid = sv.n_bq > 0 ? (bx >= sv.bq_first[1]) + (bx >= sv.bq_first[2]) + (bx >= sv.bq_first[3])
+                                : stratum * sv.n_cells + cell;
     const uint32_t qbeg = sv.q_start[qid];
     const uint32_t qlen = sv.q_start[qid + 1] - qbeg;
     const int fs = m.fs, F = m.F, B = m.B;

@@ -193,8 +193,9 @@ struct tx_ctx {
     int32_t smp_epoch = -1;  // epoch whose samples are on the device
 
     // length-bucketed epoch (user_order 3): users by training-rating count in
-    // [1,16], [17,256], [257,4096], [4097,...), each bucket its own queue and
-    // its own concurrent launch with its own team shape and teams in flight
+    // [1,16], [17,256], [257,4096], [4097,...), each bucket its own queue,
+    // team shape and teams in flight; one launch for all of them when every
+    // bucket runs one-wave teams, else one concurrent launch per bucket
     struct Bucket {
         int32_t lo, hi;
         int64_t users = 0, ratings = 0;
@@ -203,6 +204,7 @@ struct tx_ctx {
     };
     Bucket bk[TX_N_BUCKETS] = {{1, 16}, {17, 256}, {257, 4096}, {4097, 0x7fffffff}};
     bool bk_ready = false;
+    bool bk_one_launch = false;      // the last bucketed epoch ran as one launch
     uint32_t* d_bq_start = nullptr;  // [5] queue offsets of the buckets
     uint32_t* d_bq_head = nullptr;   // [4] their work counters
     hipStream_t bst[TX_N_BUCKETS - 1] = {};
@@ -905,11 +907,10 @@ bool bucket_shapes_ok(const tx_ctx* c) {
     return c->cfg.low_memory && !c->cfg.has_ordinal && !c->nb && c->shape.waves == 1;
 }
 
-// Bucket sizes, default shapes and teams in flight (once per ctx): 16-lane
-// teams for the shortest users (f <= 256), two-wave teams for the longest
-// (128 <= padded row <= 1,024), one wave elsewhere; the teams of the
-// fidelity bound split in proportion to each bucket's ratings plus a
-// 16-rating per-user overhead, at least 8 (at most the bucket's users).
+// Bucket sizes, default shapes and teams in flight (once per ctx): one-wave
+// teams in every bucket (one launch for all), the teams of the fidelity
+// bound split in proportion to each bucket's ratings plus a 16-rating
+// per-user overhead, at least 8 (at most the bucket's users).
 int init_buckets(tx_ctx* c) {
     if (c->bk_ready) return 0;
     for (auto& b : c->bk) b.users = b.ratings = 0;
@@ -921,7 +922,6 @@ int init_buckets(tx_ctx* c) {
                 b.ratings += cnt;
             }
     }
-    const bool shapes = bucket_shapes_ok(c);
     double work[TX_N_BUCKETS], tot = 0.0;
     for (int b = 0; b < TX_N_BUCKETS; ++b) {
         work[b] = static_cast<double>(c->bk[b].ratings) + 16.0 * static_cast<double>(c->bk[b].users);
@@ -930,8 +930,8 @@ int init_buckets(tx_ctx* c) {
     const int T = target_teams(c->cfg.n_feat);
     for (int b = 0; b < TX_N_BUCKETS; ++b) {
         auto& k = c->bk[b];
-        k.lanes = (b == 0 && shapes && c->fs <= 256) ? 16 : 64;
-        k.waves = (b == TX_N_BUCKETS - 1 && shapes && c->fs >= 128 && c->fs <= 1024) ? 2 : 1;
+        k.lanes = 64;
+        k.waves = 1;
         const int64_t want = tot > 0.0 ? static_cast<int64_t>(T * work[b] / tot + 0.5) : 8;
         k.teams = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(std::max<int64_t>(want, 8), k.users)));
         k.ms = 0.0;
@@ -948,16 +948,43 @@ int init_buckets(tx_ctx* c) {
     return 0;
 }
 
-// One launch per bucket, concurrently: bucket 0 on the ctx's stream, the
-// others on their own streams forked from it and joined back.  `timed`
-// records each launch between events (stream launches only, not under graph
-// capture).
+// The bucketed epoch.  Every bucket with one-wave teams: one launch on the
+// ctx's stream whose workgroup ranges serve the bucket queues, so the buckets
+// run together whatever hardware queues the process's streams share.
+// Otherwise one launch per bucket, concurrently: bucket 0 on the ctx's
+// stream, the others on their own streams forked from it and joined back
+// (kernels on streams that share a hardware queue run one after the other).
+// `timed` records the launches between events (stream launches only, not
+// under graph capture).
 int enqueue_buckets(tx_ctx* c, hipStream_t st, bool timed) {
     const bool qf = c->cfg.lr_iqf != 0.0f || c->cfg.sigma_iqf != 0.0f;
     const bool p2 = c->cfg.lr_u2 != 0.0f;
     const bool ord = c->cfg.has_ordinal != 0;
     const bool day = c->cfg.low_memory == 0;
     HIPC(tx::launch_fill_u32(c->d_bq_head, 0u, TX_N_BUCKETS, st));
+    bool one = true;
+    for (const auto& k : c->bk)
+        if (k.users > 0 && (k.lanes != 64 || k.waves != 1)) one = false;
+    c->bk_one_launch = one;
+    if (one) {
+        static_assert(TX_N_BUCKETS == 4, "the kernel names a workgroup's queue from four bucket ranges");
+        tx::StrataView sv = c->sv;
+        sv.q_start = c->d_bq_start;
+        sv.q_head = c->d_bq_head;
+        sv.block_threads = 64;  // one team per workgroup
+        sv.n_bq = TX_N_BUCKETS;
+        int total = 0;
+        sv.bq_first[0] = 0;
+        for (int b = 0; b < TX_N_BUCKETS; ++b) {
+            total += c->bk[b].users > 0 ? c->bk[b].teams : 0;
+            sv.bq_first[b + 1] = total;
+        }
+        if (total < 1) return 0;
+        if (timed) HIPC(hipEventRecord(c->bev_t[0], st));
+        HIPC(tx::launch_sgd_par(c->dm, sv, 0, c->shape.team, c->shape.vec, total, qf, p2, ord, 0, day, st));
+        if (timed) HIPC(hipEventRecord(c->bev_t[1], st));
+        return 0;
+    }
     HIPC(hipEventRecord(c->bev_fork, st));
     for (int b = 1; b < TX_N_BUCKETS; ++b) HIPC(hipStreamWaitEvent(c->bst[b - 1], c->bev_fork, 0));
     for (int b = 0; b < TX_N_BUCKETS; ++b) {
@@ -985,11 +1012,17 @@ int enqueue_buckets(tx_ctx* c, hipStream_t st, bool timed) {
     return 0;
 }
 
-// Per-bucket launch times of the last timed epoch.
+// Per-bucket launch times of the last timed epoch (with one launch for all
+// buckets, that launch's time for each).
 int read_bucket_times(tx_ctx* c) {
+    float one_ms = 0.0f;
+    if (c->bk_one_launch) HIPC(hipEventElapsedTime(&one_ms, c->bev_t[0], c->bev_t[1]));
     for (int b = 0; b < TX_N_BUCKETS; ++b) {
         float ms = 0.0f;
-        if (c->bk[b].users > 0) HIPC(hipEventElapsedTime(&ms, c->bev_t[2 * b], c->bev_t[2 * b + 1]));
+        if (c->bk[b].users > 0) {
+            if (c->bk_one_launch) ms = one_ms;
+            else HIPC(hipEventElapsedTime(&ms, c->bev_t[2 * b], c->bev_t[2 * b + 1]));
+        }
         c->bk[b].ms = ms;
     }
     return 0;
@@ -2058,7 +2091,7 @@ int tx_get_buckets(tx_ctx* c, tx_bucket_info* out) {
     if (!c->bk_ready) return fail(TX_EINVAL, "tx_get_buckets: no schedule with user_order 3 has been set");
     for (int b = 0; b < TX_N_BUCKETS; ++b) {
         const auto& k = c->bk[b];
-        out[b] = tx_bucket_info{k.lo, k.hi, k.users, k.ratings, k.lanes, k.waves, k.teams, 0, k.ms};
+        out[b] = tx_bucket_info{k.lo, k.hi, k.users, k.ratings, k.lanes, k.waves, k.teams, c->bk_one_launch ? 1 : 0, k.ms};
     }
     return 0;
 }

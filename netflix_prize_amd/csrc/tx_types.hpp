@@ -138,6 +138,13 @@ struct StrataView {
     int32_t su_cached;
     int32_t nb_mode;            // neighbourhood-table update: 0 device atomic, 2 plain
     int32_t block_threads;      // SGD workgroup size of the one-wave teams: 64 x teams per block (0 = 256)
+    // length buckets in one launch (one stratum, one-wave teams, one team per
+    // workgroup): workgroups [bq_first[k], bq_first[k + 1]) serve queue k of
+    // q_start / q_head, k < n_bq = 4; bq_first is nondecreasing (an empty
+    // bucket has an empty range).  n_bq = 0: every workgroup serves the
+    // (stratum, cell) queue.
+    int32_t n_bq;
+    int32_t bq_first[5];
 };
 
 // Item pass of the tsvdx4p schedule: ratings grouped by item.

@@ -184,7 +184,7 @@ USER_ORDER_BUCKETS = 3  # length buckets launched separately
 class CBucketInfo(C.Structure):
     _fields_ = [("lo", C.c_int32), ("hi", C.c_int32), ("users", C.c_int64), ("ratings", C.c_int64),
                 ("lanes_per_user", C.c_int32), ("waves_per_user", C.c_int32), ("teams", C.c_int32),
-                ("reserved", C.c_int32), ("ms", C.c_double)]
+                ("one_launch", C.c_int32), ("ms", C.c_double)]
 
 
 def config_struct(cfg: TxConfig) -> CConfig:
@@ -455,10 +455,11 @@ class TxModel:
 
     def buckets(self) -> list:
         """The length buckets of user_order 3 (tx_get_buckets): bounds, users,
-        ratings, lanes and waves per user, teams in flight, last launch ms."""
+        ratings, lanes and waves per user, teams in flight, whether the last
+        epoch ran every bucket in one launch, last launch ms."""
         arr = (CBucketInfo * N_BUCKETS)()
         _check(lib().tx_get_buckets(self._h, arr), "tx_get_buckets")
-        return [{k: getattr(b, k) for k, _ in CBucketInfo._fields_ if k != "reserved"} for b in arr]
+        return [{k: getattr(b, k) for k, _ in CBucketInfo._fields_} for b in arr]
 
     def set_bucket_shape(self, bucket: int, lanes_per_user: int, waves_per_user: int, teams: int) -> None:
         _check(lib().tx_set_bucket_shape(self._h, int(bucket), int(lanes_per_user), int(waves_per_user), int(teams)),
