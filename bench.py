@@ -94,6 +94,9 @@ def profiled_traffic(f: int, args, kind: str = "") -> float | None:
     return float(json.loads(block)["traffic_bytes_per_launch"])
 
 
+_ATOMIC_CEILING = None
+
+
 def atomic_ceiling() -> dict | None:
     """The device's rate for the SGD kernel's item-row update pattern
     (netflix_prize_amd/atomic-ceiling: 1 KB rows of a 17,770-row table updated
@@ -102,6 +105,9 @@ def atomic_ceiling() -> dict | None:
     kernel's whole per-rating item update, row + bias row), as the best rate
     over its wave counts per form."""
     import subprocess
+    global _ATOMIC_CEILING
+    if _ATOMIC_CEILING is not None:
+        return _ATOMIC_CEILING
     exe = os.path.join(ROOT, "netflix_prize_amd", "atomic-ceiling")
     if not os.path.exists(exe):
         return None
@@ -114,8 +120,56 @@ def atomic_ceiling() -> dict | None:
         r = json.loads(line)
         if r["lanes_per_s"] > best.get(r["form"], {}).get("lanes_per_s", 0.0):
             best[r["form"]] = r
-    return {k: {"rows_per_s": v["rows_per_s"], "lanes_per_s": v["lanes_per_s"],
-                "atomic_instr_per_s": v.get("atomic_instr_per_s"), "waves": v["waves"]} for k, v in best.items()}
+    _ATOMIC_CEILING = {k: {"rows_per_s": v["rows_per_s"], "lanes_per_s": v["lanes_per_s"],
+                           "atomic_instr_per_s": v.get("atomic_instr_per_s"), "waves": v["waves"]}
+                       for k, v in best.items()}
+    return _ATOMIC_CEILING
+
+
+def atomic_cost_model(ceil: dict) -> dict | None:
+    """Time per FP32 atomic instruction as a x instructions + b x 128-byte
+    lines + c x dwords, solved from three probe forms (scatter-atomic: 1,
+    64, 64; f32-atomic row: 4, 8, 256; bias-atomic: 1, 1, 2) and checked on a
+    fourth (row+bias-atomic: 5, 9, 258)."""
+    need = ("scatter-atomic", "f32-atomic", "bias-atomic", "row+bias-atomic")
+    if not ceil or any(k not in ceil for k in need):
+        return None
+    A = np.array([[1.0, 64.0, 64.0], [4.0, 8.0, 256.0], [1.0, 1.0, 2.0]])
+    t = np.array([1.0 / ceil[k]["rows_per_s"] for k in need[:3]])
+    a, b, c = np.linalg.solve(A, t)
+    pred = 5 * a + 9 * b + 258 * c
+    meas = 1.0 / ceil["row+bias-atomic"]["rows_per_s"]
+    return {"s_per_instr": a, "s_per_line": b, "s_per_dword": c, "check_rel_err": (pred - meas) / meas}
+
+
+def nbr_atomic_work(tr, pr, max_nbr: int, seed: int = 0) -> dict:
+    """Per training rating, the atomic updates of the global neighbourhood
+    terms (src/tx.rs:1041-1070): w(i, j) for the sampled rated neighbours,
+    c(i, j) for every sampled neighbour, all in row i of their table.  The
+    sample follows the reference's budget (src/tx.rs:763-781), drawn here
+    uniformly with a fixed seed; lines = distinct 32-element (128-byte)
+    column blocks of the row; instructions = two (w, c) per 64 neighbours."""
+    U = tr.n_users
+    toff = np.searchsorted(tr.user_idxs, np.arange(U + 1))
+    poff = np.searchsorted(pr.user_idxs, np.arange(U + 1))
+    nr = np.diff(toff).astype(np.int64)
+    rng = np.random.default_rng(seed)
+    lines = dwords = instr = 0.0
+    for u in np.nonzero(nr)[0]:
+        R = tr.item_idxs[toff[u]:toff[u + 1]]
+        P = np.setdiff1d(pr.item_idxs[poff[u]:poff[u + 1]], R)
+        n = len(R) + len(P)
+        if n > max_nbr:
+            rb = min(max(int(round(max_nbr * len(R) / n)), 1), len(R))
+            ib = min(max_nbr - rb, len(P))
+            R = rng.choice(R, rb, replace=False)
+            P = rng.choice(P, ib, replace=False) if ib else P[:0]
+        k = nr[u]
+        lines += k * (len(np.unique(R >> 5)) + len(np.unique(np.concatenate([R, P]) >> 5)))
+        dwords += k * (2 * len(R) + len(P))
+        instr += k * 2 * (1 + (len(R) + len(P) > 64))
+    N = float(nr.sum())
+    return {"lines": lines / N, "dwords": dwords / N, "instructions": instr / N}
 
 
 def log(msg: str) -> None:
@@ -507,6 +561,22 @@ def main(argv=None) -> int:
             res["roofline"] = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                "frac": round(ach / HBM_PEAK_GBS, 4), "kernel": "k_sgd_par (neighbourhood)",
                                "bytes_per_rating": round(bpr, 1), "avg_launch_ms": sgd}
+            # the neighbourhood terms' increment of the SGD launch against the
+            # atomic floor of their element updates (the probe's cost model)
+            cost = atomic_cost_model(atomic_ceiling())
+            if cost is not None:
+                t0 = time.time()
+                wk = nbr_atomic_work(tr, pr, 100)
+                per = (cost["s_per_instr"] * wk["instructions"] + cost["s_per_line"] * wk["lines"]
+                       + cost["s_per_dword"] * wk["dwords"])
+                inc = res["nbr"]["ms_sgd"] - res["without_nbr"]["ms_sgd"]
+                floor = per * n_train * 1000.0
+                res["atomic"] = {"per_rating": {k: round(float(v), 2) for k, v in wk.items()},
+                                 "ns_per_rating": round(float(per) * 1e9, 3), "floor_ms": round(float(floor), 1),
+                                 "increment_ms": round(inc, 1),
+                                 "frac": round(float(floor / inc), 4) if inc > 0 else None,
+                                 "model": {k: float(f"{v:.4g}") for k, v in cost.items()}}
+                log(f"bench: {name} neighbourhood atomics ({time.time() - t0:.1f}s): {res['atomic']}")
             res["workload"] = (f"{name} model: lm set, day terms, f={f}, 32 time bins, max_neighbors 100"
                                + (", same-day terms" if same_day else "") + ", 1 stratum")
             nbr[name] = res

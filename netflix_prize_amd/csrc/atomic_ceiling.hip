@@ -13,6 +13,9 @@
 //                    table: the SGD kernel's item-bias update)
 //   row+bias-atomic  f32-atomic and bias-atomic on the same row: the SGD
 //                    kernel's whole per-rating item update at f=256
+//   scatter-atomic   one 64-lane FP32 atomic add per "row", each lane on a
+//                    pseudo-random element of a 17,770 x 17,770 table (1.26 GB,
+//                    the neighbourhood tables' shape): 64 lines per instruction
 // and prints rows/s, lanes/s and atomic instructions/s per form and wave
 // count, one line each.
 // Usage: atomic-ceiling [iterations per wave]
@@ -47,7 +50,7 @@ __device__ __forceinline__ uint32_t mix(uint32_t x) {
 
 // one wave per block; row r of the table is q[r * kRowFloats, (r + 1) * kRowFloats)
 template <int FORM>
-__global__ __launch_bounds__(64) void k_rows(float* q, float* bias, int iters) {
+__global__ __launch_bounds__(64) void k_rows(float* q, float* bias, float* big, int iters) {
     const int lane = threadIdx.x;
     float acc = 0.0f;
     for (int it = 0; it < iters; ++it) {
@@ -65,6 +68,12 @@ __global__ __launch_bounds__(64) void k_rows(float* q, float* bias, int iters) {
                     __hip_atomic_fetch_add(bias + static_cast<size_t>(r) * 32 + (lane == 0 ? 0 : 1 + it % 30), 1e-7f,
                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
+        } else if constexpr (FORM == 6) {
+            // (i, j) from the lane's own hash: both < kRows
+            const uint32_t h = mix(r * 64u + static_cast<uint32_t>(lane) + 0x9e3779b9u * static_cast<uint32_t>(it));
+            const uint32_t j = h % kRows;
+            __hip_atomic_fetch_add(big + static_cast<size_t>(r) * kRows + j, 1e-7f, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
         } else if constexpr (FORM == 1) {
             unsigned long long* p8 = reinterpret_cast<unsigned long long*>(q + static_cast<size_t>(r) * kRowFloats) + lane;
 #pragma unroll
@@ -99,25 +108,31 @@ int main(int argc, char** argv) {
     CK(hipMemset(q, 0, sizeof(float) * kRows * kRowFloats));
     CK(hipMalloc(&bias, sizeof(float) * kRows * 32));
     CK(hipMemset(bias, 0, sizeof(float) * kRows * 32));
+    float* big = nullptr;
+    const size_t n_big = static_cast<size_t>(kRows) * kRows;
+    CK(hipMalloc(&big, sizeof(float) * n_big));
+    CK(hipMemset(big, 0, sizeof(float) * n_big));
     hipEvent_t a, b;
     CK(hipEventCreate(&a));
     CK(hipEventCreate(&b));
-    const char* names[6] = {"f32-atomic", "u64-atomic", "plain-rmw", "loads", "bias-atomic", "row+bias-atomic"};
-    const int lanes_per_row[6] = {256, 128, 256, 256, 2, 258};
-    const int atomics_per_row[6] = {4, 2, 0, 0, 1, 5};  // atomic instructions per row and wave
+    const char* names[7] = {"f32-atomic",  "u64-atomic",      "plain-rmw",     "loads",
+                            "bias-atomic", "row+bias-atomic", "scatter-atomic"};
+    const int lanes_per_row[7] = {256, 128, 256, 256, 2, 258, 64};
+    const int atomics_per_row[7] = {4, 2, 0, 0, 1, 5, 1};  // atomic instructions per row and wave
     const int waves_list[3] = {1536, 4096, 16384};
-    for (int form = 0; form < 6; ++form) {
+    for (int form = 0; form < 7; ++form) {
         for (int w : waves_list) {
             float ms = 0.0f;
             for (int rep = 0; rep < 2; ++rep) {  // the first launch warms up
                 CK(hipEventRecord(a));
                 switch (form) {
-                    case 0: hipLaunchKernelGGL(k_rows<0>, dim3(w), dim3(64), 0, 0, q, bias, iters); break;
-                    case 1: hipLaunchKernelGGL(k_rows<1>, dim3(w), dim3(64), 0, 0, q, bias, iters); break;
-                    case 2: hipLaunchKernelGGL(k_rows<2>, dim3(w), dim3(64), 0, 0, q, bias, iters); break;
-                    case 3: hipLaunchKernelGGL(k_rows<3>, dim3(w), dim3(64), 0, 0, q, bias, iters); break;
-                    case 4: hipLaunchKernelGGL(k_rows<4>, dim3(w), dim3(64), 0, 0, q, bias, iters); break;
-                    default: hipLaunchKernelGGL(k_rows<5>, dim3(w), dim3(64), 0, 0, q, bias, iters); break;
+                    case 0: hipLaunchKernelGGL(k_rows<0>, dim3(w), dim3(64), 0, 0, q, bias, big, iters); break;
+                    case 1: hipLaunchKernelGGL(k_rows<1>, dim3(w), dim3(64), 0, 0, q, bias, big, iters); break;
+                    case 2: hipLaunchKernelGGL(k_rows<2>, dim3(w), dim3(64), 0, 0, q, bias, big, iters); break;
+                    case 3: hipLaunchKernelGGL(k_rows<3>, dim3(w), dim3(64), 0, 0, q, bias, big, iters); break;
+                    case 4: hipLaunchKernelGGL(k_rows<4>, dim3(w), dim3(64), 0, 0, q, bias, big, iters); break;
+                    case 5: hipLaunchKernelGGL(k_rows<5>, dim3(w), dim3(64), 0, 0, q, bias, big, iters); break;
+                    default: hipLaunchKernelGGL(k_rows<6>, dim3(w), dim3(64), 0, 0, q, bias, big, iters); break;
                 }
                 CK(hipGetLastError());
                 CK(hipEventRecord(b));
@@ -134,5 +149,6 @@ int main(int argc, char** argv) {
     }
     CK(hipFree(q));
     CK(hipFree(bias));
+    CK(hipFree(big));
     return 0;
 }
