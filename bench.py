@@ -398,6 +398,8 @@ def main(argv=None) -> int:
         n_new = args.fold_in_users if args.fold_in_users < spec.n_users else spec.n_users // 6
         log(f"bench: fold-in of {n_new} new users (encoder vs random start)")
         fold = E.evaluate_new_users(tr, pr, cfg, spec.n_users - n_new, args.ttr_epochs or 20, 5, 20, log=log)
+        fold["gemm"] = E.gemm_benchmark(E.n_features(cfg.n_feat), cfg.n_feat + 1)
+        log(f"bench: encoder step products: {fold['gemm']}")
 
     # KnnfModel (the *__knnf jobs) over the trained model's item factors: the
     # cosine similarity matrix by FP32 MFMA at create, then every held-out query

@@ -37,9 +37,26 @@ enum enc_epi { ENC_EPI_STORE = 0, ENC_EPI_BIAS = 1, ENC_EPI_BIAS_GELU = 2, ENC_E
 /* C (M x N) = op(A) (M x K) * op(B) (K x N) with FP32 accumulation, where
  * op(A) = A (A stored M x K, leading dimension lda) or, with ta, A^T (A stored
  * K x M); likewise op(B) = B (K x N, ldb) or, with tb, B^T (B stored N x K).
- * M and N multiples of 64, K a multiple of 32. */
+ * M and N multiples of 64, K a multiple of 32.  BF16 products with K a
+ * multiple of 64, 16-byte aligned operands and leading dimensions a multiple
+ * of 8 run on the 16 x 16 x 32 MFMA kernel with the tile enc_gemm_tile
+ * reports; the others on the generic 64 x 64 kernel.  An ENC_EPI_STORE
+ * product on the MFMA kernel with fewer than 256 tiles splits K over up to 8
+ * workgroups per tile: partial products go to a 16 MiB workspace of the
+ * library (one per device) and are summed in split order, so the result does
+ * not depend on scheduling, but two such calls on one device must not run at
+ * the same time (different streams). */
 int enc_gemm(int dtype, int ta, int tb, int epi, const void* A, const void* B, int M, int N, int K, int lda, int ldb,
              float* cf, void* ct, const float* bias, const float* zin, int ldc, void* stream);
+
+/* The C tile (*bm x *bn rows x columns) enc_gemm's MFMA kernel takes for an
+ * M x N x K product of dtype with epilogue epi (aligned operands): the largest
+ * of 128 x 128, 128 x 64 and 64 x 128 that divides M and N in at least 256
+ * tiles, else 64 x 64; 0 x 0 when the generic kernel runs (FP32, or K not a
+ * multiple of 64).  *splits: the K splits (1, or for ENC_EPI_STORE under 256
+ * tiles the largest power of two <= 8 keeping >= 4 stages of 64 per split, at
+ * most 1,024 workgroups and the slabs within the workspace).  No device work. */
+int enc_gemm_tile(int dtype, int epi, int M, int N, int K, int* bm, int* bn, int* splits);
 
 /* Weighted squared error of Y (M x N, ld) against T: loss += sum_{r,c}
  * colw[c] (Y - T)^2 / M (FP64 accumulation into *loss); gradient dY = 2 colw[c]
@@ -55,6 +72,33 @@ int enc_colsum(const float* X, int M, int N, int ld, float* out, void* stream);
  * wcopy in dtype (the copy the GEMMs read). */
 int enc_adam(int dtype, float* w, const float* g, float* m, float* v, void* wcopy, int64_t n, float lr, float beta1,
              float beta2, float eps, int32_t t, void* stream);
+
+/* The 2-layer network y = W2 gelu(W1 x + b1) + b2 (widths multiples of 64)
+ * with its FP32 masters, the dtype copies the GEMMs read and Adam moments. */
+typedef struct enc_mlp {
+    int32_t dtype, d_in, hid, d_out;
+    float *W1, *b1, *W2, *b2;          /* masters: d_in x hid, hid, hid x d_out, d_out */
+    void *W1c, *W2c;                   /* dtype copies of W1 and W2 */
+    float *m[4], *v[4];                /* Adam moments of W1, b1, W2, b2 */
+} enc_mlp;
+
+/* Per-step buffers for a batch of B rows: Z, dyf, dzf, Y FP32 (B x hid, B x
+ * d_out, B x hid, B x d_out); H, dyt, dzt in dtype; gradients FP32 shaped
+ * like the masters. */
+typedef struct enc_mlp_work {
+    float *Z, *Y, *dyf, *dzf;
+    void *H, *dyt, *dzt;
+    float *gW1, *gb1, *gW2, *gb2;
+} enc_mlp_work;
+
+/* One training step on B rows (X: B x d_in in dtype, T: B x d_out FP32
+ * targets, colw: d_out column weights): forward, loss and gradient
+ * (enc_loss_grad: loss += the batch's weighted mean squared error), the four
+ * gradients (enc_gemm, enc_colsum) and the Adam step t (enc_adam) of every
+ * parameter, in that order on one stream: the launches of the separate entry
+ * points in one call. */
+int enc_train_step(const enc_mlp* net, const enc_mlp_work* w, const void* X, const float* T, const float* colw, int B,
+                   float lr, float beta1, float beta2, float eps, int32_t t, double* loss, void* stream);
 
 /* y = x in dtype (round to nearest even for BF16). */
 int enc_cast(int dtype, const float* x, void* y, int64_t n, void* stream);

@@ -7,7 +7,9 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdint>
+#include <algorithm>
 #include <cstdio>
+#include <mutex>
 #include <string>
 
 #include "tsvd_enc.h"
@@ -182,6 +184,215 @@ __global__ __launch_bounds__(256) void k_gemm(const T* __restrict__ A, const T* 
             }
 }
 
+// ---------------------------------------------------------------------------
+// BF16 GEMM on gfx950's 16 x 16 x 32 MFMA (K a multiple of 64, operands and
+// leading dimensions 16-byte aligned).  256 threads = 4 waves in 2 x 2 per
+// BM x BN tile of C (BM, BN in {64, 128}); a wave owns a (BM/2) x (BN/2)
+// quarter as 16 x 16 blocks.  K advances 64 per stage through two LDS
+// buffers, both operands stored [row][k] (row = m for A, n for B; rows padded
+// to 72 elements = 144 bytes): a lane's 8 consecutive k of one row are one
+// 16-byte LDS read.  Global loads are 16 bytes (8 elements) along whichever
+// dimension is contiguous in memory; a k-contiguous chunk is one LDS write,
+// an m/n-contiguous chunk is scattered over 8 rows.  The next stage's loads
+// are issued before the current stage's MFMAs, written to the other buffer
+// after them, and one barrier per stage separates the two.
+//
+// Operands of v_mfma_f32_16x16x32_bf16 (wave64, lane l, element e < 8):
+//   A[l%16][8*(l/16)+e], B[8*(l/16)+e][l%16]; result D[4*(l/16)+e][l%16] (e < 4).
+// A and B take the same lane/element -> k assignment, so the product is the
+// sum over k whatever order the unit adds the 32 products in.
+// ---------------------------------------------------------------------------
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+// staging registers: a native vector type (HIP's uint4 is a union wrapper that
+// the compiler keeps in memory when arrays of it are passed around)
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// Workgroup barrier that waits for this wave's LDS operations only.
+// __syncthreads() is a workgroup-scope release/acquire around s_barrier, and
+// the release also waits for every global load in flight, which would drain
+// the next stages' prefetch at each barrier; the stages share nothing but LDS.
+__device__ __forceinline__ void lds_barrier() {
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+constexpr int GK = 64;       // K per stage
+constexpr int GKP = GK + 8;  // LDS row pitch in elements
+
+template <int R, bool KCONT>
+__device__ __forceinline__ void stage_load(const bf16_t* __restrict__ X, int ld, int r0, int k0, int tid,
+                                           u32x4 (&v)[R * GK / 8 / 256]) {
+#pragma unroll
+    for (int q = 0; q < R * GK / 8 / 256; ++q) {
+        const int c = tid + 256 * q;
+        if constexpr (KCONT) {  // X stored [row][k]
+            const int row = c >> 3, kc = c & 7;
+            v[q] = *reinterpret_cast<const u32x4*>(X + static_cast<size_t>(r0 + row) * ld + k0 + kc * 8);
+        } else {  // X stored [k][row]: rows g*8 .. g*8+7 of k rows kq*Q .. kq*Q+Q-1
+            constexpr int Q = R * GK / 8 / 256;
+            const int g = tid % (R / 8), kq = tid / (R / 8);
+            v[q] = *reinterpret_cast<const u32x4*>(X + static_cast<size_t>(k0 + kq * Q + q) * ld + r0 + g * 8);
+        }
+    }
+}
+
+template <int R, bool KCONT>
+__device__ __forceinline__ void stage_store(bf16_t* __restrict__ S, int tid, const u32x4 (&v)[R * GK / 8 / 256]) {
+#pragma unroll
+    for (int q = 0; q < R * GK / 8 / 256; ++q) {
+        const int c = tid + 256 * q;
+        if constexpr (KCONT) {
+            const int row = c >> 3, kc = c & 7;
+            *reinterpret_cast<u32x4*>(S + row * GKP + kc * 8) = v[q];
+        }
+    }
+    if constexpr (!KCONT) {
+        // the thread's Q chunks are Q consecutive k of the same 8 rows: row
+        // g*8+e takes element e of every chunk, Q consecutive k in one write
+        constexpr int Q = R * GK / 8 / 256;
+        static_assert(Q == 2 || Q == 4, "two or four k per row and thread");
+        const int g = tid % (R / 8), kq = tid / (R / 8);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int w = e >> 1, sh = 16 * (e & 1);
+            uint32_t h[Q];
+#pragma unroll
+            for (int q = 0; q < Q; ++q) h[q] = (v[q][w] >> sh) & 0xffffu;
+            bf16_t* dst = S + (g * 8 + e) * GKP + kq * Q;
+            if constexpr (Q == 2) {
+                *reinterpret_cast<uint32_t*>(dst) = h[0] | (h[1] << 16);
+            } else {
+                typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+                *reinterpret_cast<u32x2*>(dst) = u32x2{h[0] | (h[1] << 16), h[2] | (h[3] << 16)};
+            }
+        }
+    }
+}
+
+template <int BM, int BN, bool TA, bool TB, int EPI>
+__global__ __launch_bounds__(256) void k_gemm_mfma(const bf16_t* __restrict__ A, const bf16_t* __restrict__ B, int M,
+                                                   int N, int K, int lda, int ldb, EpiArgs<bf16_t> ep, int kper,
+                                                   float* __restrict__ part) {
+    constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
+    constexpr int QA = BM * GK / 8 / 256, QB = BN * GK / 8 / 256;
+    __shared__ __attribute__((aligned(16))) bf16_t As[2][BM * GKP];
+    __shared__ __attribute__((aligned(16))) bf16_t Bs[2][BN * GKP];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    const int wm = (wave >> 1) * WM, wn = (wave & 1) * WN;
+    // split-K: workgroup z takes K range [z kper, (z + 1) kper)
+    const int kb = static_cast<int>(blockIdx.z) * kper;
+    const int nst = kper / GK;
+    v4f acc[TM][TN];
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+#pragma unroll
+        for (int b = 0; b < TN; ++b) acc[a][b] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    // two register sets: stage s + 2 loads while stage s computes and stage
+    // s + 1 (loaded one stage earlier) goes to LDS
+    u32x4 ra0[QA], rb0[QB], ra1[QA], rb1[QB];
+    const int arow = lane & 15, koff = 8 * (lane >> 4);
+    auto compute = [&](int buf) {
+        const bf16_t* as = As[buf];
+        const bf16_t* bs = Bs[buf];
+#pragma unroll
+        for (int ks = 0; ks < GK; ks += 32) {
+            v8bf a[TM], b[TN];
+#pragma unroll
+            for (int t = 0; t < TM; ++t)
+                a[t] = *reinterpret_cast<const v8bf*>(as + (wm + t * 16 + arow) * GKP + ks + koff);
+#pragma unroll
+            for (int t = 0; t < TN; ++t)
+                b[t] = *reinterpret_cast<const v8bf*>(bs + (wn + t * 16 + arow) * GKP + ks + koff);
+#pragma unroll
+            for (int ti = 0; ti < TM; ++ti)
+#pragma unroll
+                for (int tj = 0; tj < TN; ++tj)
+                    acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[ti], b[tj], acc[ti][tj], 0, 0, 0);
+        }
+    };
+
+    stage_load<BM, !TA>(A, lda, m0, kb, tid, ra0);
+    stage_load<BN, TB>(B, ldb, n0, kb, tid, rb0);
+    if (nst > 1) {
+        stage_load<BM, !TA>(A, lda, m0, kb + GK, tid, ra1);
+        stage_load<BN, TB>(B, ldb, n0, kb + GK, tid, rb1);
+    }
+    stage_store<BM, !TA>(As[0], tid, ra0);
+    stage_store<BN, TB>(Bs[0], tid, rb0);
+    lds_barrier();
+    for (int s = 0; s < nst; s += 2) {
+        // even stage s: buffer 0 computes, set 1 (stage s + 1) goes to buffer 1
+        if (s + 2 < nst) {
+            stage_load<BM, !TA>(A, lda, m0, kb + (s + 2) * GK, tid, ra0);
+            stage_load<BN, TB>(B, ldb, n0, kb + (s + 2) * GK, tid, rb0);
+        }
+        compute(0);
+        if (s + 1 < nst) {
+            stage_store<BM, !TA>(As[1], tid, ra1);
+            stage_store<BN, TB>(Bs[1], tid, rb1);
+        }
+        lds_barrier();
+        if (s + 1 >= nst) break;
+        // odd stage s + 1: buffer 1 computes, set 0 (stage s + 2) goes to buffer 0
+        if (s + 3 < nst) {
+            stage_load<BM, !TA>(A, lda, m0, kb + (s + 3) * GK, tid, ra1);
+            stage_load<BN, TB>(B, ldb, n0, kb + (s + 3) * GK, tid, rb1);
+        }
+        compute(1);
+        if (s + 2 < nst) {
+            stage_store<BM, !TA>(As[0], tid, ra0);
+            stage_store<BN, TB>(Bs[0], tid, rb0);
+        }
+        lds_barrier();
+    }
+
+#pragma unroll
+    for (int ti = 0; ti < TM; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < TN; ++tj)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int r = m0 + wm + ti * 16 + 4 * (lane >> 4) + e;
+                const int c = n0 + wn + tj * 16 + (lane & 15);
+                const float v = acc[ti][tj][e];
+                if (part) {  // split-K partial (EPI_STORE only): dense M x N slab z
+                    part[(static_cast<size_t>(blockIdx.z) * M + r) * N + c] = v;
+                    continue;
+                }
+                const size_t o = static_cast<size_t>(r) * ep.ldc + c;
+                if constexpr (EPI == ENC_EPI_STORE) {
+                    ep.cf[o] = v;
+                } else if constexpr (EPI == ENC_EPI_BIAS) {
+                    ep.cf[o] = v + ep.bias[c];
+                } else if constexpr (EPI == ENC_EPI_BIAS_GELU) {
+                    const float z = v + ep.bias[c];
+                    ep.cf[o] = z;
+                    ep.ct[o] = bf16_rne(gelu(z));
+                } else {
+                    const float g = v * gelu_grad(ep.zin[o]);
+                    ep.cf[o] = g;
+                    ep.ct[o] = bf16_rne(g);
+                }
+            }
+}
+
+// Split-K partial slabs of the MFMA GEMM (at most kSplitWs floats; calls that
+// split K share it, so such calls must not run concurrently).
+constexpr size_t kSplitWs = size_t(4) << 20;
+__device__ __align__(256) float g_split_ws[kSplitWs];
+
+// cf[r*ldc + c] = sum over z (in order) of slab z's (r, c).
+__global__ __launch_bounds__(256) void k_splitk_reduce(const float* __restrict__ part, int S, int M, int N,
+                                                       float* __restrict__ cf, int ldc) {
+    const int64_t n = static_cast<int64_t>(M) * N;
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < n;
+         e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        float t = 0.0f;
+        for (int z = 0; z < S; ++z) t += part[z * n + e];
+        const int r = static_cast<int>(e / N), c = static_cast<int>(e % N);
+        cf[static_cast<size_t>(r) * ldc + c] = t;
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_loss_grad(const float* Y, const float* Tg, const float* colw, float* dyf,
                                                    T* dyt, int M, int N, int ld, double* loss) {
@@ -207,12 +418,35 @@ __global__ __launch_bounds__(256) void k_loss_grad(const float* Y, const float* 
     if (threadIdx.x == 0) atomicAdd(loss, part[0] / M);
 }
 
-__global__ __launch_bounds__(256) void k_colsum(const float* X, int M, int N, int ld, float* out) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= N) return;
+// Column sums: one 1,024-thread block per 64 columns; thread (row phase p,
+// column c) sums rows p, p + 16, ... in order, then phase 0 adds the 16
+// partial sums in phase order (a fixed order: the result does not depend on
+// scheduling).
+__global__ __launch_bounds__(1024) void k_colsum(const float* X, int M, int N, int ld, float* out) {
+    __shared__ float part[16][64];
+    const int cx = threadIdx.x & 63, ph = threadIdx.x >> 6;
+    const int c = blockIdx.x * 64 + cx;
     float s = 0.0f;
-    for (int r = 0; r < M; ++r) s += X[static_cast<size_t>(r) * ld + c];
-    out[c] = s;
+    if (c < N) {
+        // 8 rows per round in flight, added in row order
+        int r = ph;
+        for (; r + 7 * 16 < M; r += 8 * 16) {
+            float v[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] = X[static_cast<size_t>(r + 16 * j) * ld + c];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) s += v[j];
+        }
+        for (; r < M; r += 16) s += X[static_cast<size_t>(r) * ld + c];
+    }
+    part[ph][cx] = s;
+    __syncthreads();
+    if (ph == 0 && c < N) {
+        float t = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) t += part[q][cx];
+        out[c] = t;
+    }
 }
 
 template <typename T>
@@ -517,9 +751,96 @@ unsigned grid_for(int64_t n) {
     return static_cast<unsigned>(b);
 }
 
+// Tile of the MFMA GEMM for an M x N x K product: the largest of 128 x 128,
+// 128 x 64, 64 x 128 that divides M and N and still gives at least 256 tiles
+// (one per CU), else 64 x 64; {0, 0} where K is not a multiple of 64 (the
+// generic kernel then runs).
+void mfma_tile(int M, int N, int K, int* bm, int* bn) {
+    *bm = *bn = 0;
+    if (K % GK) return;
+    const int cand[3][2] = {{128, 128}, {128, 64}, {64, 128}};
+    for (const auto& t : cand)
+        if (M % t[0] == 0 && N % t[1] == 0 && static_cast<int64_t>(M / t[0]) * (N / t[1]) >= 256) {
+            *bm = t[0];
+            *bn = t[1];
+            return;
+        }
+    *bm = *bn = 64;
+}
+
+// Device address of g_split_ws, looked up once per device.
+float* split_ws() {
+    static std::mutex mu;
+    static float* addr[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(g_split_ws)) != hipSuccess) return nullptr;
+        addr[dev] = static_cast<float*>(p);
+    }
+    return addr[dev];
+}
+
+// K splits of a plain-store product whose tiles do not fill the CUs: the
+// largest power of two <= 8 that keeps at least 4 stages per split, at most
+// 1,024 workgroups and the slabs within the workspace.
+int split_k(int epi, int tiles, int M, int N, int K) {
+    if (epi != ENC_EPI_STORE || tiles >= 256) return 1;
+    int s = 1;
+    while (s < 8 && K % (GK * 2 * s) == 0 && K / (GK * 2 * s) >= 4 && tiles * 2 * s <= 1024 &&
+           static_cast<size_t>(2 * s) * M * N <= kSplitWs)
+        s *= 2;
+    return s;
+}
+
+template <int BM, int BN, bool TA, bool TB>
+int gemm_mfma(int epi, const bf16_t* a, const bf16_t* b, int M, int N, int K, int lda, int ldb, EpiArgs<bf16_t> ep,
+              hipStream_t st) {
+    const int tiles = (N / BN) * (M / BM);
+    const int S = split_k(epi, tiles, M, N, K);
+    float* part = nullptr;
+    if (S > 1 && !(part = split_ws())) return fail(-3, "enc_gemm: split-K workspace address lookup failed");
+    dim3 grid(N / BN, M / BM, S);
+    const int kper = K / S;
+    (void)hipGetLastError();
+    switch (epi) {
+    case ENC_EPI_STORE: hipLaunchKernelGGL((k_gemm_mfma<BM, BN, TA, TB, ENC_EPI_STORE>), grid, dim3(256), 0, st, a, b, M, N, K, lda, ldb, ep, kper, part); break;
+    case ENC_EPI_BIAS: hipLaunchKernelGGL((k_gemm_mfma<BM, BN, TA, TB, ENC_EPI_BIAS>), grid, dim3(256), 0, st, a, b, M, N, K, lda, ldb, ep, kper, part); break;
+    case ENC_EPI_BIAS_GELU: hipLaunchKernelGGL((k_gemm_mfma<BM, BN, TA, TB, ENC_EPI_BIAS_GELU>), grid, dim3(256), 0, st, a, b, M, N, K, lda, ldb, ep, kper, part); break;
+    case ENC_EPI_GELU_BWD: hipLaunchKernelGGL((k_gemm_mfma<BM, BN, TA, TB, ENC_EPI_GELU_BWD>), grid, dim3(256), 0, st, a, b, M, N, K, lda, ldb, ep, kper, part); break;
+    default: return fail(-1, "enc_gemm: unknown epilogue %d", epi);
+    }
+    if (S > 1) {
+        const int64_t n = static_cast<int64_t>(M) * N;
+        const unsigned g = static_cast<unsigned>(std::min<int64_t>((n + 255) / 256, 2048));
+        hipLaunchKernelGGL(k_splitk_reduce, dim3(g), dim3(256), 0, st, part, S, M, N, ep.cf, ep.ldc);
+    }
+    return launched("enc_gemm");
+}
+
+template <bool TA, bool TB>
+int gemm_mfma_tile(int bm, int bn, int epi, const bf16_t* a, const bf16_t* b, int M, int N, int K, int lda, int ldb,
+                   EpiArgs<bf16_t> ep, hipStream_t st) {
+    if (bm == 128 && bn == 128) return gemm_mfma<128, 128, TA, TB>(epi, a, b, M, N, K, lda, ldb, ep, st);
+    if (bm == 128) return gemm_mfma<128, 64, TA, TB>(epi, a, b, M, N, K, lda, ldb, ep, st);
+    if (bn == 128) return gemm_mfma<64, 128, TA, TB>(epi, a, b, M, N, K, lda, ldb, ep, st);
+    return gemm_mfma<64, 64, TA, TB>(epi, a, b, M, N, K, lda, ldb, ep, st);
+}
+
 template <typename T, bool TA, bool TB>
 int gemm_t(int epi, const void* A, const void* B, int M, int N, int K, int lda, int ldb, EpiArgs<T> ep,
            hipStream_t st) {
+    if constexpr (sizeof(T) == 2) {
+        int bm, bn;
+        mfma_tile(M, N, K, &bm, &bn);
+        const bool aligned = (reinterpret_cast<uintptr_t>(A) % 16 == 0) && (reinterpret_cast<uintptr_t>(B) % 16 == 0) &&
+                             lda % 8 == 0 && ldb % 8 == 0;
+        if (bm && aligned)
+            return gemm_mfma_tile<TA, TB>(bm, bn, epi, static_cast<const bf16_t*>(A), static_cast<const bf16_t*>(B), M,
+                                          N, K, lda, ldb, ep, st);
+    }
     dim3 grid(N / BN, M / BM);
     const T* a = static_cast<const T*>(A);
     const T* b = static_cast<const T*>(B);
@@ -564,6 +885,18 @@ int enc_gemm(int dtype, int ta, int tb, int epi, const void* A, const void* B, i
     return fail(-1, "enc_gemm: unknown dtype %d", dtype);
 }
 
+int enc_gemm_tile(int dtype, int epi, int M, int N, int K, int* bm, int* bn, int* splits) {
+    g_err.clear();
+    if (!bm || !bn || !splits) return fail(-1, "enc_gemm_tile: NULL output");
+    *bm = *bn = 0;
+    *splits = 1;
+    if (M <= 0 || N <= 0 || K <= 0 || M % BM || N % BN || K % BK)
+        return fail(-1, "enc_gemm_tile: M, N must be positive multiples of 64 and K of 32 (got %d, %d, %d)", M, N, K);
+    if (dtype == ENC_BF16) mfma_tile(M, N, K, bm, bn);
+    if (*bm) *splits = split_k(epi, (M / *bm) * (N / *bn), M, N, K);
+    return 0;
+}
+
 int enc_loss_grad(int dtype, const float* Y, const float* T, const float* colw, float* dyf, void* dyt, int M, int N,
                   int ld, double* loss, void* stream) {
     g_err.clear();
@@ -586,7 +919,7 @@ int enc_colsum(const float* X, int M, int N, int ld, float* out, void* stream) {
     g_err.clear();
     if (M <= 0 || N <= 0 || ld < N) return fail(-1, "enc_colsum: bad shape");
     (void)hipGetLastError();
-    hipLaunchKernelGGL(k_colsum, dim3((N + 255) / 256), dim3(256), 0, static_cast<hipStream_t>(stream), X, M, N, ld,
+    hipLaunchKernelGGL(k_colsum, dim3((N + 63) / 64), dim3(1024), 0, static_cast<hipStream_t>(stream), X, M, N, ld,
                        out);
     return launched("enc_colsum");
 }
@@ -608,6 +941,43 @@ int enc_adam(int dtype, float* w, const float* g, float* m, float* v, void* wcop
     else
         return fail(-1, "enc_adam: unknown dtype %d", dtype);
     return launched("enc_adam");
+}
+
+int enc_train_step(const enc_mlp* n, const enc_mlp_work* w, const void* X, const float* T, const float* colw, int B,
+                   float lr, float beta1, float beta2, float eps, int32_t t, double* loss, void* stream) {
+    g_err.clear();
+    if (!n || !w || !X || !T || !colw || !loss) return fail(-1, "enc_train_step: NULL argument");
+    const int dt = n->dtype, di = n->d_in, hd = n->hid, dout = n->d_out;
+    int rc;
+    // forward: Z = X W1 + b1, H = gelu(Z); Y = H W2 + b2
+    if ((rc = enc_gemm(dt, 0, 0, ENC_EPI_BIAS_GELU, X, n->W1c, B, hd, di, di, hd, w->Z, w->H, n->b1, nullptr, hd,
+                       stream)))
+        return rc;
+    if ((rc = enc_gemm(dt, 0, 0, ENC_EPI_BIAS, w->H, n->W2c, B, dout, hd, hd, dout, w->Y, nullptr, n->b2, nullptr,
+                       dout, stream)))
+        return rc;
+    if ((rc = enc_loss_grad(dt, w->Y, T, colw, w->dyf, w->dyt, B, dout, dout, loss, stream))) return rc;
+    // dW2 = H^T dY, db2 = colsum dY; dZ = (dY W2^T) gelu'(Z); dW1 = X^T dZ, db1 = colsum dZ
+    if ((rc = enc_gemm(dt, 1, 0, ENC_EPI_STORE, w->H, w->dyt, hd, dout, B, hd, dout, w->gW2, nullptr, nullptr, nullptr,
+                       dout, stream)))
+        return rc;
+    if ((rc = enc_colsum(w->dyf, B, dout, dout, w->gb2, stream))) return rc;
+    if ((rc = enc_gemm(dt, 0, 1, ENC_EPI_GELU_BWD, w->dyt, n->W2c, B, hd, dout, dout, dout, w->dzf, w->dzt, nullptr,
+                       w->Z, hd, stream)))
+        return rc;
+    if ((rc = enc_gemm(dt, 1, 0, ENC_EPI_STORE, X, w->dzt, di, hd, B, di, hd, w->gW1, nullptr, nullptr, nullptr, hd,
+                       stream)))
+        return rc;
+    if ((rc = enc_colsum(w->dzf, B, hd, hd, w->gb1, stream))) return rc;
+    float* ws[4] = {n->W1, n->b1, n->W2, n->b2};
+    float* gs[4] = {w->gW1, w->gb1, w->gW2, w->gb2};
+    void* cs[4] = {n->W1c, n->b1, n->W2c, n->b2};
+    const int dts[4] = {dt, ENC_F32, dt, ENC_F32};
+    const int64_t sz[4] = {static_cast<int64_t>(di) * hd, hd, static_cast<int64_t>(hd) * dout, dout};
+    for (int k = 0; k < 4; ++k)
+        if ((rc = enc_adam(dts[k], ws[k], gs[k], n->m[k], n->v[k], cs[k], sz[k], lr, beta1, beta2, eps, t, stream)))
+            return rc;
+    return 0;
 }
 
 int enc_cast(int dtype, const float* x, void* y, int64_t n, void* stream) {

@@ -64,6 +64,7 @@ def lib() -> C.CDLL:
         vp, i32, i64, f32 = C.c_void_p, C.c_int, C.c_int64, C.c_float
         L.enc_gemm.argtypes = [i32, i32, i32, i32, vp, vp, i32, i32, i32, i32, i32, vp, vp, vp, vp, i32, vp]
         L.enc_loss_grad.argtypes = [i32, vp, vp, vp, vp, vp, i32, i32, i32, vp, vp]
+        L.enc_gemm_tile.argtypes = [i32, i32, i32, i32, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]
         L.enc_colsum.argtypes = [vp, i32, i32, i32, vp, vp]
         L.enc_adam.argtypes = [i32, vp, vp, vp, vp, vp, i64, f32, f32, f32, f32, C.c_int32, vp]
         L.enc_cast.argtypes = [i32, vp, vp, i64, vp]
@@ -71,12 +72,38 @@ def lib() -> C.CDLL:
         L.enc_user_features.argtypes = [vp, i32, i32, vp, f32, vp, vp, vp, vp, vp, vp, vp, i32, i32, vp, i64, vp]
         L.enc_affine_rows.argtypes = [i32, vp, i64, vp, i32, i32, vp, vp, vp, i64, i32, vp]
         L.enc_colstats.argtypes = [vp, i32, i32, i64, vp, vp, vp, vp]
+        L.enc_train_step.argtypes = [C.POINTER(_Mlp), C.POINTER(_MlpWork), vp, vp, vp, i32, f32, f32, f32, f32,
+                                     C.c_int32, vp, vp]
         L.enc_last_error.restype = C.c_char_p
-        for n in ("enc_gemm", "enc_loss_grad", "enc_colsum", "enc_adam", "enc_cast", "enc_item_day_stats",
+        for n in ("enc_gemm", "enc_gemm_tile", "enc_train_step", "enc_loss_grad", "enc_colsum", "enc_adam", "enc_cast", "enc_item_day_stats",
                   "enc_user_features", "enc_affine_rows", "enc_colstats"):
             getattr(L, n).restype = C.c_int
         _lib = L
     return _lib
+
+
+def gemm_plan(M: int, N: int, K: int, epi: int = EPI_STORE, dtype: int = ENC_BF16) -> tuple[int, int, int]:
+    """(tile rows, tile columns, K splits) enc_gemm takes for an M x N x K
+    product ((0, 0, 1): the generic kernel)."""
+    bm, bn, sp = C.c_int(0), C.c_int(0), C.c_int(0)
+    _check(lib().enc_gemm_tile(dtype, epi, M, N, K, C.byref(bm), C.byref(bn), C.byref(sp)), "enc_gemm_tile")
+    return bm.value, bn.value, sp.value
+
+
+def gemm_tile(M: int, N: int, K: int, dtype: int = ENC_BF16) -> tuple[int, int]:
+    """The C tile enc_gemm's MFMA kernel takes for an M x N x K product
+    ((0, 0): the generic kernel)."""
+    return gemm_plan(M, N, K, EPI_STORE, dtype)[:2]
+
+
+class _Mlp(C.Structure):
+    _fields_ = [("dtype", C.c_int32), ("d_in", C.c_int32), ("hid", C.c_int32), ("d_out", C.c_int32),
+                ("W1", C.c_void_p), ("b1", C.c_void_p), ("W2", C.c_void_p), ("b2", C.c_void_p),
+                ("W1c", C.c_void_p), ("W2c", C.c_void_p), ("m", C.c_void_p * 4), ("v", C.c_void_p * 4)]
+
+
+class _MlpWork(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("Z", "Y", "dyf", "dzf", "H", "dyt", "dzt", "gW1", "gb1", "gW2", "gb2")]
 
 
 def _check(rc: int, what: str) -> None:
@@ -302,6 +329,34 @@ class FoldInEncoder:
         _check(L.enc_colsum(_p(dzf), B, self.hid, self.hid, _p(gb1), st), "enc_colsum db1")
         return gW1, gb1, gW2, gb2
 
+    def train_step(self, X, T, colw, loss) -> None:
+        """gradients() and adam_step() in one library call (enc_train_step):
+        the same launches in the same order on the same stream."""
+        torch = _torch()
+        c = self.cfg
+        B = X.shape[0]
+        # the tables' addresses are part of the key: set_weights() replaces them
+        ptrs = tuple(_p(a) for a in (self.W1, self.b1, self.W2, self.b2, self.W1c, self.W2c)) + \
+            tuple(_p(a) for mv in self.opt for a in mv)
+        key = ("step", B, ptrs)
+        if self._bufs.get(key) is None:
+            for k in [k for k in self._bufs if isinstance(k, tuple) and k[0] == "step"]:
+                del self._bufs[k]
+            net = _Mlp(self.dt, self.d_in, self.hid, self.d_out, *ptrs[:6],
+                       (C.c_void_p * 4)(*ptrs[6::2]), (C.c_void_p * 4)(*ptrs[7::2]))
+            f32, dt = torch.float32, self.tdt
+            bufs = {"Z": ((B, self.hid), f32), "Y": ((B, self.d_out), f32), "dyf": ((B, self.d_out), f32),
+                    "dzf": ((B, self.hid), f32), "H": ((B, self.hid), dt), "dyt": ((B, self.d_out), dt),
+                    "dzt": ((B, self.hid), dt), "gW1": ((self.d_in, self.hid), f32), "gb1": ((self.hid,), f32),
+                    "gW2": ((self.hid, self.d_out), f32), "gb2": ((self.d_out,), f32)}
+            # buffers of their own ("ts_" names: forward() may resize its "Z" etc.)
+            work = _MlpWork(**{n: _p(self._buf("ts_" + n, sh, d)) for n, (sh, d) in bufs.items()})
+            self._bufs[key] = (net, work)
+        net, work = self._bufs[key]
+        self.step_t += 1
+        _check(lib().enc_train_step(C.byref(net), C.byref(work), _p(X), _p(T), _p(colw), B, c.lr, c.beta1, c.beta2,
+                                    c.eps, self.step_t, _p(loss), _stream()), "enc_train_step")
+
     def adam_step(self, grads) -> None:
         c = self.cfg
         self.step_t += 1
@@ -376,7 +431,7 @@ class FoldInEncoder:
             tot = torch.zeros(1, dtype=torch.float64, device=self.dev)
             for b in rng.permutation(nb):
                 sl = slice(int(b) * c.batch, (int(b) + 1) * c.batch)
-                self.adam_step(self.gradients(Xn[sl], Tn[sl], colw_d, tot))
+                self.train_step(Xn[sl], Tn[sl], colw_d, tot)
             hist.append(float(tot.item()) / max(nb, 1) / self.n_out)
             if log:
                 log(f"encoder epoch {ep + 1}: loss {hist[-1]:.4f}")
@@ -403,6 +458,51 @@ class FoldInEncoder:
     def predict(self, X) -> np.ndarray:
         """Outputs (n x n_out) in target units, as a host array."""
         return self.predict_device(X).cpu().numpy()
+
+
+BF16_DENSE_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
+
+
+def gemm_benchmark(d_in: int, n_out: int, hidden: int = 512, batch: int = 4096, reps: int = 50) -> list:
+    """Times the five products of one training step of the d_in -> hidden ->
+    n_out network (BF16, batch rows) on the current stream with events:
+    forward 1 and 2, dW2, dZ, dW1, each with its epilogue (split-K reduce
+    included).  Returns per product the shape, tile, splits, microseconds
+    and TFLOP/s against the dense BF16 MFMA peak."""
+    torch = _torch()
+    L, st = lib(), _stream()
+    di, hd, do = round64(d_in), round64(hidden), round64(n_out)
+    g = torch.Generator(device="cuda").manual_seed(0)
+    mk = lambda r, c: torch.randint(-8000, 8000, (r, c), dtype=torch.int16, device="cuda", generator=g)
+    prods = [("forward 1 (X W1, bias + GELU)", batch, hd, di, 0, 0, EPI_BIAS_GELU),
+             ("forward 2 (H W2, bias)", batch, do, hd, 0, 0, EPI_BIAS),
+             ("dW2 = H^T dY", hd, do, batch, 1, 0, EPI_STORE),
+             ("dZ = (dY W2^T) gelu'(Z)", batch, hd, do, 0, 1, EPI_GELU_BWD),
+             ("dW1 = X^T dZ", di, hd, batch, 1, 0, EPI_STORE)]
+    out = []
+    for name, M, N, K, ta, tb, epi in prods:
+        A = mk(K, M) if ta else mk(M, K)
+        B = mk(N, K) if tb else mk(K, N)
+        cf = torch.empty((M, N), dtype=torch.float32, device="cuda")
+        ct = torch.empty((M, N), dtype=torch.int16, device="cuda")
+        bias = torch.zeros(N, dtype=torch.float32, device="cuda")
+        z = torch.zeros((M, N), dtype=torch.float32, device="cuda")
+        run = lambda: _check(L.enc_gemm(ENC_BF16, ta, tb, epi, _p(A), _p(B), M, N, K, A.shape[1], B.shape[1], _p(cf),
+                                        _p(ct), _p(bias), _p(z), N, st), name)
+        for _ in range(3):
+            run()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            run()
+        e1.record()
+        e1.synchronize()
+        us = e0.elapsed_time(e1) / reps * 1e3
+        tf = 2.0 * M * N * K / us / 1e6
+        bm, bn, sp = gemm_plan(M, N, K, epi)
+        out.append({"product": name, "M": M, "N": N, "K": K, "tile": [bm, bn], "k_splits": sp, "us": round(us, 2),
+                    "tflops": round(tf, 1), "frac_of_bf16_peak": round(tf / BF16_DENSE_PEAK_TFLOPS, 4)})
+    return out
 
 
 # --------------------------------------------------------------------------

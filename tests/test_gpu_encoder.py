@@ -52,6 +52,71 @@ def test_gemm_layouts(gpu, dtype, ta, tb):
     assert _rel(C.cpu().numpy(), ref) < (1e-6 if dtype == E.ENC_F32 else 1e-5)
 
 
+# (M, N, K) per MFMA tile: each shape takes the tile named (enc_gemm_tile's rule)
+MFMA_SHAPES = {(128, 128): (4096, 1024, 128), (128, 64): (4096, 512, 1856), (64, 128): (2112, 1024, 64),
+               (64, 64): (512, 320, 4096)}
+
+
+@pytest.mark.parametrize("tile", list(MFMA_SHAPES))
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_mfma_gemm_tiles_and_layouts(gpu, tile, ta, tb):
+    """The 16x16x32 MFMA kernel in each tile and operand layout against an FP64
+    product of the same BF16 inputs (the fold-in network's shapes among them)."""
+    import torch
+    M, N, K = MFMA_SHAPES[tile]
+    assert E.gemm_tile(M, N, K) == tile
+    rng = np.random.default_rng(3)
+    A = bf16_round(rng.normal(size=(K, M) if ta else (M, K)).astype(np.float32))
+    B = bf16_round(rng.normal(size=(N, K) if tb else (K, N)).astype(np.float32))
+    ref = (A.T if ta else A).astype(np.float64) @ (B.T if tb else B).astype(np.float64)
+    C = torch.empty((M, N), dtype=torch.float32, device="cuda")
+    dA, dB = _dev(A, E.ENC_BF16), _dev(B, E.ENC_BF16)
+    E._check(E.lib().enc_gemm(E.ENC_BF16, ta, tb, E.EPI_STORE, dA.data_ptr(), dB.data_ptr(), M, N, K,
+                              A.shape[1], B.shape[1], C.data_ptr(), None, None, None, N, E._stream()), "gemm")
+    assert _rel(C.cpu().numpy(), ref) < 1e-5
+
+
+def test_mfma_gemm_epilogues(gpu):
+    import torch
+    rng = np.random.default_rng(4)
+    M, N, K = 4096, 512, 128
+    assert E.gemm_tile(M, N, K) == (128, 64)
+    A = bf16_round(rng.normal(size=(M, K)).astype(np.float32))
+    B = bf16_round(rng.normal(size=(K, N)).astype(np.float32) * 0.1)
+    bias = rng.normal(size=N).astype(np.float32)
+    Zin = rng.normal(size=(M, N)).astype(np.float32)
+    acc = A.astype(np.float64) @ B.astype(np.float64)
+    cf = torch.empty((M, N), dtype=torch.float32, device="cuda")
+    ct = torch.empty((M, N), dtype=torch.int16, device="cuda")
+    db, dz = torch.from_numpy(bias).cuda(), torch.from_numpy(Zin).cuda()
+    dA, dB = _dev(A, E.ENC_BF16), _dev(B, E.ENC_BF16)
+    L, st = E.lib(), E._stream()
+    bf = lambda t: (t.cpu().numpy().astype(np.uint16).astype(np.uint32) << 16).view(np.float32)
+    E._check(L.enc_gemm(E.ENC_BF16, 0, 0, E.EPI_BIAS, dA.data_ptr(), dB.data_ptr(), M, N, K, K, N, cf.data_ptr(),
+                        None, db.data_ptr(), None, N, st), "gemm")
+    np.testing.assert_allclose(cf.cpu().numpy(), acc + bias, atol=2e-5)
+    E._check(L.enc_gemm(E.ENC_BF16, 0, 0, E.EPI_BIAS_GELU, dA.data_ptr(), dB.data_ptr(), M, N, K, K, N,
+                        cf.data_ptr(), ct.data_ptr(), db.data_ptr(), None, N, st), "gemm")
+    np.testing.assert_allclose(cf.cpu().numpy(), acc + bias, atol=2e-5)
+    np.testing.assert_allclose(bf(ct), R.gelu(acc + bias), rtol=8e-3, atol=1e-6)
+    E._check(L.enc_gemm(E.ENC_BF16, 0, 0, E.EPI_GELU_BWD, dA.data_ptr(), dB.data_ptr(), M, N, K, K, N,
+                        cf.data_ptr(), ct.data_ptr(), None, dz.data_ptr(), N, st), "gemm")
+    g = acc * R.gelu_grad(Zin)
+    np.testing.assert_allclose(cf.cpu().numpy(), g, atol=2e-5)
+    np.testing.assert_allclose(bf(ct), g, rtol=8e-3, atol=1e-6)
+
+
+def test_colsum_matches_fp64(gpu):
+    import torch
+    rng = np.random.default_rng(5)
+    for M, N in ((4096, 320), (4096, 512), (100, 70)):
+        X = rng.normal(size=(M, N)).astype(np.float32)
+        out = torch.empty(N, dtype=torch.float32, device="cuda")
+        dx = torch.from_numpy(X).cuda()
+        E._check(E.lib().enc_colsum(dx.data_ptr(), M, N, N, out.data_ptr(), E._stream()), "colsum")
+        np.testing.assert_allclose(out.cpu().numpy(), X.astype(np.float64).sum(0), rtol=0, atol=1e-3)
+
+
 @pytest.mark.parametrize("dtype", [E.ENC_F32, E.ENC_BF16])
 def test_gemm_epilogues(gpu, dtype):
     import torch
@@ -116,6 +181,37 @@ def test_training_fits_a_teacher(gpu):
     assert hist[-1] < 0.1 * hist[0], hist
     P = enc.predict(X[:1000])
     assert np.mean((P - T[:1000]) ** 2) < 0.1 * np.var(T)
+
+
+@pytest.mark.parametrize("dtype", ["bf16", "f32"])
+def test_train_step_equals_separate_launches(gpu, dtype):
+    """enc_train_step (one call per batch, used by fit) leaves every master,
+    moment and copy bit-identical to gradients() + adam_step(), three steps in
+    a row at the fold-in network's widths (the MFMA GEMMs and split-K among
+    them)."""
+    import torch
+    rng = np.random.default_rng(6)
+    d_in, n_out, B = 1856, 257, 1024
+    X = rng.normal(size=(B, d_in)).astype(np.float32)
+    T = rng.normal(size=(B, n_out)).astype(np.float32)
+    nets = [E.FoldInEncoder(d_in, n_out, E.EncoderConfig(dtype=dtype)) for _ in range(2)]
+    a, b = nets
+    Xd = a.to_device_x(X)
+    Tp = np.zeros((B, a.d_out), np.float32)
+    Tp[:, :n_out] = T
+    Td = torch.from_numpy(Tp).cuda()
+    colw = torch.ones(a.d_out, dtype=torch.float32, device="cuda")
+    la = torch.zeros(1, dtype=torch.float64, device="cuda")
+    lb = torch.zeros(1, dtype=torch.float64, device="cuda")
+    for _ in range(3):
+        a.adam_step(a.gradients(Xd, Td, colw, la))
+        b.train_step(Xd, Td, colw, lb)
+    torch.cuda.synchronize()
+    # the loss sums block partials with FP64 atomics (order free): equal to rounding
+    assert la.item() == pytest.approx(lb.item(), rel=1e-12)
+    for ta, tb in zip(a.masters + [a.W1c, a.W2c] + [t for mv in a.opt for t in mv],
+                      b.masters + [b.W1c, b.W2c] + [t for mv in b.opt for t in mv]):
+        assert torch.equal(ta, tb)
 
 
 def test_warm_start_beats_random_start(gpu):
