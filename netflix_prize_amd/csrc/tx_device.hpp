@@ -228,6 +228,17 @@ __device__ __forceinline__ float in_vgpr(float x) {
     return r;
 }
 
+// A wave-uniform pointer the compiler must keep in VGPRs: loads through it
+// land in VGPRs (and free the scalar registers the value would hold).
+template <class T>
+__device__ __forceinline__ T* in_vgpr_ptr(T* p) {
+    const uint64_t a = reinterpret_cast<uint64_t>(p);
+    uint32_t lo, hi;
+    asm("v_mov_b32 %0, %1" : "=v"(lo) : "s"(static_cast<uint32_t>(a)));
+    asm("v_mov_b32 %0, %1" : "=v"(hi) : "s"(static_cast<uint32_t>(a >> 32)));
+    return reinterpret_cast<T*>((static_cast<uint64_t>(hi) << 32) | lo);
+}
+
 // Walks rows 0..n-1 of a user's item list in order.  Row r's item index is
 // *idx_ptr(r): the indices arrive TEAM at a time (lane l loads index b0 + l),
 // a batch ahead.  The rows are read contiguously (lane l: elements l*VEC ..),
@@ -372,6 +383,10 @@ struct Pending {
     bool has_bud, has_but;
 };
 
+#ifndef TX_DAY_GB
+#define TX_DAY_GB 16
+#endif
+
 #ifndef TX_PREFETCH
 #define TX_PREFETCH 3
 #endif
@@ -413,6 +428,10 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_
     static_assert(PASS == 0 || !DAY, "the user passes train low-memory models");
     static_assert(!MW || TEAM == 64, "multi-wave teams: 64-lane waves");
     constexpr int CHUNK = 4;
+    // rows per group of the row streams: day-term models stream one-element
+    // rows (VEC 1) in groups of 16, which keeps the kernel within the
+    // registers of three waves per SIMD
+    constexpr int GBS = (DAY && VEC == 1) ? TX_DAY_GB : row_group<TEAM, VEC>();
     // up to 4 waves share a SIMD's register file with nothing else; with 8,
     // two waves share each SIMD and one rating ahead keeps them closer to it
     constexpr int DEPTH = MW == 8 ? 1 : TX_PREFETCH;
@@ -468,6 +487,8 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_
     const uint32_t sc_fm = (DAY && l3) ? 0xffffffffu : 0u;  // b_i,f(t): the day's freq bin
     const float sc_lr = lane == 0 ? h.lr_ib : lane == 1 ? h.lr_ti : h.lr_ibf;
     float* const q_base = m.ifeat + off;
+    // the run norms load into VGPRs (day-term kernels)
+    const float* const run_norm = DAY ? in_vgpr_ptr(m.t_run_norm) : m.t_run_norm;
 
     float th[4] = {0.f, 0.f, 0.f, 0.f};
     if constexpr (ORD) {
@@ -552,8 +573,10 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_
                 // only (low-memory records carry bin 0, so y is the index)
                 n.ud = DAY ? (rc.y & 0x0FFFFFFFu) : rc.y;
                 n.fb = DAY ? (rc.y >> 28) : 0u;
-                n.dev = __uint_as_float(rc.z);
-                n.r = __uint_as_float(rc.w);
+                // day-term kernels hold a slot's dev and target in VGPRs: their
+                // rating loop has more live scalars than the scalar file holds
+                n.dev = DAY ? in_vgpr(__uint_as_float(rc.z)) : __uint_as_float(rc.z);
+                n.r = DAY ? in_vgpr(__uint_as_float(rc.w)) : __uint_as_float(rc.w);
                 n.has_bud = n.has_but = false;
                 n.bud = n.but = 0.0f;
                 if (bfz) {
@@ -638,7 +661,7 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_
                 const uint32_t* hitems = m.h_item + m.h_start[u] - ntr;  // row r >= ntr is hitems[r]
                 const int coff = lane * VEC;
                 const bool yb_on = !m.skip_yb;
-                for_rows<TEAM, VEC, row_group<TEAM, VEC>()>(
+                for_rows<TEAM, VEC, GBS>(
                     ntot, lane, m.yfeat + cbase, fs,
                     [&](uint32_t r) { return r < ntr ? titems + r : hitems + r; },
                     [&](uint32_t b0, uint32_t ib) {
@@ -669,7 +692,7 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_
                     float dacc[VEC];
 #pragma unroll
                     for (int c = 0; c < VEC; ++c) dacc[c] = 0.0f;
-                    for_rows_at<TEAM, VEC, row_group<TEAM, VEC>()>(
+                    for_rows_at<TEAM, VEC, GBS>(
                         m.dg_off[u + 1] - g0, lane, m.dg_item + g0,
                         [&](uint32_t j) {
                             return (j & kDayHead) ? sv.day_acc + static_cast<size_t>(j & ~kDayHead) * fs + cbase
@@ -755,7 +778,7 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_
 #pragma unroll
             for (int k = 0; k < DEPTH; ++k) {
                 fetch(rec_at(s0 + k < s1 ? s0 + k : s1 - 1), ring[k]);
-                if constexpr (DAY) ring[k].nd = m.t_run_norm[s0 + k < s1 ? s0 + k : s1 - 1];
+                if constexpr (DAY) ring[k].nd = run_norm[s0 + k < s1 ? s0 + k : s1 - 1];
             }
 #ifdef TX_PROLOGUE_PAD
             {
@@ -777,7 +800,7 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_
                 if (t >= s1) break;
                 const Pending<VEC, DAY, NB>& cur = ring[k];
                 fetch(rec_at(t + DEPTH < s1 ? t + DEPTH : s1 - 1), ring[(k + DEPTH) % RING]);
-                if constexpr (DAY) ring[(k + DEPTH) % RING].nd = m.t_run_norm[t + DEPTH < s1 ? t + DEPTH : s1 - 1];
+                if constexpr (DAY) ring[(k + DEPTH) % RING].nd = run_norm[t + DEPTH < s1 ? t + DEPTH : s1 - 1];
                 const uint32_t i = cur.i;
                 const float dev = cur.dev;
                 const float c_ib = team_bcast_f<TEAM>(cur.sc, 0), c_bt = team_bcast_f<TEAM>(cur.sc, 1);
@@ -1037,7 +1060,7 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_
                     __builtin_amdgcn_wave_barrier();
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                     const uint32_t* titems = m.t_item + ts;
-                    for_rows<TEAM, VEC, row_group<TEAM, VEC>()>(
+                    for_rows<TEAM, VEC, GBS>(
                         te - ts, lane, m.yfeat + cbase, fs, [&](uint32_t r) { return titems + r; },
                         [&](uint32_t b0, uint32_t ib) {
                             if (gb != 0.0f && lead && b0 + static_cast<uint32_t>(lane) < te - ts)
@@ -1095,7 +1118,7 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_
                 float g[VEC];
 #pragma unroll
                 for (int c = 0; c < VEC; ++c) g[c] = 0.0f;
-                for_rows_at<TEAM, VEC, row_group<TEAM, VEC>()>(
+                for_rows_at<TEAM, VEC, GBS>(
                     m.dy_off[u + 1] - y0, lane, m.dy_item + y0,
                     [&](uint32_t j) {
                         return (j & kDayHead) ? sv.day_acc + static_cast<size_t>(j & ~kDayHead) * fs + cbase

@@ -575,6 +575,16 @@ int build_strata(tx_ctx* c, int S) {
 // trainer (within 1e-4 measured); 4096 at f=64 and 3072 at f=256 do not.
 int target_teams(int f) { return std::min(3072, std::max(768, 393216 / std::max(f, 1))); }
 
+// Teams in flight for a model: target_teams(f), and at most 2,048 for a
+// day-term model without neighbourhood terms.  Its y_day steps land once per
+// same-day run and are the most staleness-sensitive update: full shape, job
+// tsvdx4-60 (f=60), 20 epochs, holdout RMSE 0.90867 with 2,048 teams resident
+// against the serial 0.90851, and 0.91548 with 3,072.
+int target_teams(const tx_ctx* c) {
+    const int t = target_teams(c->cfg.n_feat);
+    return (c->cfg.low_memory == 0 && !c->nb) ? std::min(t, 2048) : t;
+}
+
 // One-wave teams per SGD workgroup: the schedule's teams_per_block (1-4), 4 by
 // default; a multi-wave team is a workgroup of its own.
 int teams_per_block(const tx_ctx* c) {
@@ -584,7 +594,7 @@ int teams_per_block(const tx_ctx* c) {
 
 int default_blocks(tx_ctx* c) {
     const int teams_per_block = ::teams_per_block(c);
-    const int want = target_teams(c->cfg.n_feat);
+    const int want = target_teams(c);
     int blocks = (want + teams_per_block - 1) / teams_per_block;
     // a multiple of 8 keeps the cell -> XCD mapping of the round-robin dispatch
     return std::max(8, (blocks + 7) / 8 * 8);
@@ -927,7 +937,7 @@ int init_buckets(tx_ctx* c) {
         work[b] = static_cast<double>(c->bk[b].ratings) + 16.0 * static_cast<double>(c->bk[b].users);
         tot += work[b];
     }
-    const int T = target_teams(c->cfg.n_feat);
+    const int T = target_teams(c);
     for (int b = 0; b < TX_N_BUCKETS; ++b) {
         auto& k = c->bk[b];
         k.lanes = 64;
@@ -1941,7 +1951,7 @@ int tx_autotune(tx_ctx* c, int32_t trial_epochs, tx_schedule* best_out) {
         return 0;
     };
     const tx_schedule orig = c->sched;
-    const int tpb = teams_per_block(c), cap = target_teams(c->cfg.n_feat);
+    const int tpb = teams_per_block(c), cap = target_teams(c);
     if (orig.user_order == 3) {
         // length-bucketed launches: per bucket (most ratings first) its team
         // shape and teams in flight, one at a time with the others held, each
