@@ -339,6 +339,9 @@ def main(argv=None) -> int:
                                 blocks=args.blocks, experimental=args.strata > 1)
                 if args.autotune:
                     tuned = m2.autotune(1)
+                    tuned["trials"] = [{k: (round(v, 2) if k == "ms" else v) for k, v in t.items()
+                                        if k not in ("bucket", "lanes_per_user", "waves_per_user")}
+                                       for t in m2.autotune_trials()]
             elif form == "plain":
                 m2.set_schedule(mode=tx.SCHED_PLAIN, blocks=1024)
             else:
@@ -364,7 +367,8 @@ def main(argv=None) -> int:
                          "ms_per_epoch": round(statistics.median(secs[1:] or secs) * 1000.0, 2),
                          "curve": [round(r, 5) for r in curve]}
             if tuned is not None:
-                ttr[form]["autotuned_schedule"] = {k: tuned[k] for k in ("blocks", "user_order", "heavy_min")}
+                ttr[form]["autotuned_schedule"] = {k: tuned[k] for k in ("blocks", "user_order", "heavy_min",
+                                                                          "n_strata", "trials")}
             log(f"bench: time-to-RMSE {form}: {ttr[form]}")
 
     # length-bucketed launches (user_order 3): the users of 1-16, 17-256,
@@ -381,12 +385,16 @@ def main(argv=None) -> int:
             bm.fit_epoch(epoch=e)
             if e == 1:
                 bm.autotune(1)
+                bucket_trials = [{k: (round(v, 2) if k == "ms" else v) for k, v in t.items()
+                                  if k not in ("n_strata", "blocks", "user_order", "heavy_min")}
+                                 for t in bm.autotune_trials()]
             bms.append(bm.last_epoch_stats()["ms_total"])
         r_b = bm.calc_rmse(pr)
         buckets = {"epochs": args.bucket_epochs, "ms_per_epoch": round(statistics.median(bms[1:] or bms), 2),
                    "final_holdout_rmse": round(r_b, 5),
                    "buckets": [{k: (round(v, 3) if isinstance(v, float) else v) for k, v in b.items()}
-                               for b in bm.buckets()]}
+                               for b in bm.buckets()],
+                   "autotune_trials": bucket_trials}
         bm.close()
         log(f"bench: length-bucketed launches: {buckets}")
 

@@ -221,15 +221,35 @@ int tx_destroy(tx_ctx* ctx);
  * TX_EUNSUPPORTED. */
 int tx_set_schedule(tx_ctx* ctx, const tx_schedule* sched);
 
-/* Autotuner of the one-stratum parallel schedule (no reference counterpart):
- * times trial_epochs epochs for each candidate of teams in flight (1/2, 3/4
- * and all of the measured fidelity bound for this f) x user order (the
- * reference's, or length buckets with heavy_min 256 / 1024 / 4096), each
- * from the same snapshot of every parameter table, then restores the tables
- * and applies the fastest candidate (use_graph kept).  The model's parameters
- * are unchanged; *best (may be NULL) receives the chosen schedule.  Needs
- * device memory for one copy of the parameter tables. */
+/* Autotuner of the parallel schedule (no reference counterpart): times
+ * trial_epochs epochs for each candidate of teams in flight (1/2, 3/4 and all
+ * of the measured fidelity bound for this f) x user order (the reference's,
+ * or length buckets with heavy_min 256 / 1024 / 4096), then the fastest of
+ * those over the stratum grids 8 x 8 and 64 x 64 (low-memory models without
+ * ordinal head or neighbourhood terms), each trial from the same snapshot of
+ * every parameter table, then restores the tables and applies the fastest
+ * candidate (use_graph kept).  With user_order 3 it tunes each bucket's team
+ * shape and teams instead (tx_get_buckets).  The model's parameters are
+ * unchanged; *best (may be NULL) receives the chosen schedule.  Needs device
+ * memory for one copy of the parameter tables.  Called on a one-stratum
+ * TX_SCHED_STRATA schedule. */
 int tx_autotune(tx_ctx* ctx, int32_t trial_epochs, tx_schedule* best);
+
+/* One trial of the last tx_autotune (no reference counterpart). */
+typedef struct tx_autotune_trial {
+    int32_t bucket;          /* -1: a whole-schedule trial; 0..3: a length bucket's shape (user_order 3) */
+    int32_t n_strata;        /* stratum grid (n_strata x n_strata user x item blocks) */
+    int32_t blocks;          /* persistent grid of the trial */
+    int32_t user_order, heavy_min;
+    int32_t lanes_per_user, waves_per_user;  /* bucket trials: the bucket's team shape */
+    int32_t teams;           /* teams in flight (of the bucket, for bucket trials) */
+    int32_t chosen;          /* 1: the candidate tx_autotune applied */
+    double ms;               /* epoch time (median of the trial's epochs) */
+} tx_autotune_trial;
+
+/* The trials of the last tx_autotune: *n = their count, the first max of them
+ * copied to out. */
+int tx_get_autotune_trials(tx_ctx* ctx, tx_autotune_trial* out, int32_t max, int32_t* n);
 
 /* One length bucket of user_order 3 (no reference counterpart). */
 typedef struct tx_bucket_info {

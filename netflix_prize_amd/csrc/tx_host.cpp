@@ -135,6 +135,7 @@ bool pick_shape(int f, Shape* s) {
 
 struct tx_ctx {
     tx_config cfg{};
+    std::vector<tx_autotune_trial> trials;  // the last tx_autotune's trials
     int32_t U = 0, I = 0, f = 0, fs = 0, F = 0, B = 0;
     int64_t N = 0, NH = 0, n_ud = 0;
     Shape shape{4, 4};
@@ -1961,17 +1962,34 @@ int tx_autotune(tx_ctx* c, int32_t trial_epochs, tx_schedule* best_out) {
         tx_schedule s1 = orig;
         s1.use_graph = 0;
         if (int rc = tx_set_schedule(c, &s1)) { release(); return rc; }
+        c->trials.clear();
         auto trial = [&](double* ms) -> int {
             if (int rc = restore()) return rc;
             if (int rc = tx_fit_epoch(c, 1)) return rc;
             *ms = c->last.ms_total;
             return 0;
         };
+        auto record = [&](int b, double ms) {
+            tx_autotune_trial t{};
+            t.bucket = b;
+            t.n_strata = 1;
+            t.blocks = c->sched.blocks;
+            t.user_order = 3;
+            t.heavy_min = c->sched.heavy_min;
+            if (b >= 0) {
+                t.lanes_per_user = c->bk[b].lanes;
+                t.waves_per_user = c->bk[b].waves;
+                t.teams = c->bk[b].teams;
+            }
+            t.ms = ms;
+            c->trials.push_back(t);
+        };
         int order[TX_N_BUCKETS];
         std::iota(order, order + TX_N_BUCKETS, 0);
         std::sort(order, order + TX_N_BUCKETS, [&](int a, int b) { return c->bk[a].ratings > c->bk[b].ratings; });
         double best_ms = 0.0;
         int rc = trial(&best_ms);
+        if (!rc) record(-1, best_ms);
         for (int b : order) {
             if (rc) break;
             auto& k = c->bk[b];
@@ -1998,6 +2016,7 @@ int tx_autotune(tx_ctx* c, int32_t trial_epochs, tx_schedule* best_out) {
                     k.teams = t;
                     double ms = 0.0;
                     if ((rc = trial(&ms))) break;
+                    record(b, ms);
                     if (ms < best_ms) {
                         best_ms = ms;
                         best = k;
@@ -2005,6 +2024,10 @@ int tx_autotune(tx_ctx* c, int32_t trial_epochs, tx_schedule* best_out) {
                 }
             k = best;
         }
+        // the chosen shape of each bucket
+        for (auto& t : c->trials)
+            t.chosen = t.bucket >= 0 && t.lanes_per_user == c->bk[t.bucket].lanes &&
+                       t.waves_per_user == c->bk[t.bucket].waves && t.teams == c->bk[t.bucket].teams;
         const int rc2 = restore();
         release();
         if (rc) return rc;
@@ -2031,22 +2054,54 @@ int tx_autotune(tx_ctx* c, int32_t trial_epochs, tx_schedule* best_out) {
     double best_ms = 1e30;
     tx_schedule best = orig;
     int rc = 0;
-    for (const auto& s : cands) {
-        if ((rc = tx_set_schedule(c, &s))) break;
+    c->trials.clear();
+    auto run = [&](const tx_schedule& s) -> int {
+        if (int r = tx_set_schedule(c, &s)) return r;
         std::vector<double> ms;
-        for (int e = 1; e <= trial_epochs && !rc; ++e) {
-            if ((rc = restore())) break;
-            if ((rc = tx_fit_epoch(c, e))) break;
+        for (int e = 1; e <= trial_epochs; ++e) {
+            if (int r = restore()) return r;
+            if (int r = tx_fit_epoch(c, e)) return r;
             ms.push_back(c->last.ms_total);
         }
-        if (rc) break;
         std::sort(ms.begin(), ms.end());
         const double med = ms[ms.size() / 2];
+        tx_autotune_trial t{};
+        t.bucket = -1;
+        t.n_strata = s.n_strata;
+        t.blocks = c->sched.blocks;
+        t.user_order = s.user_order;
+        t.heavy_min = s.heavy_min;
+        t.teams = c->sched.blocks * teams_per_block(c);
+        t.ms = med;
+        c->trials.push_back(t);
         if (med < best_ms) {
             best_ms = med;
             best = s;
         }
+        return 0;
+    };
+    for (const auto& s : cands)
+        if ((rc = run(s))) break;
+    // the stratum grid: the fastest one-stratum candidate over 8 and 64
+    // user x item blocks (count-balanced, one launch per stratum, NSVD1 steps
+    // gathered per item after the last stratum) where the model supports it
+    if (!rc && best.n_strata == 1 && c->cfg.low_memory && !c->nb && !c->cfg.has_ordinal) {
+        const tx_schedule one = best;
+        for (int S : {8, 64}) {
+            tx_schedule s = one;
+            s.n_strata = S;
+            s.blocks = 0;
+            if ((rc = run(s)) == TX_EUNSUPPORTED) {  // a grid this model cannot take
+                rc = 0;
+                g_err.clear();
+                continue;
+            }
+            if (rc) break;
+        }
     }
+    for (auto& t : c->trials)
+        t.chosen = t.n_strata == best.n_strata && t.user_order == best.user_order && t.heavy_min == best.heavy_min &&
+                   (best.n_strata > 1 || t.blocks == best.blocks);
     const int rc2 = restore();
     release();
     if (rc) return rc;
@@ -2055,6 +2110,14 @@ int tx_autotune(tx_ctx* c, int32_t trial_epochs, tx_schedule* best_out) {
     if (int r = tx_set_schedule(c, &best)) return r;
     c->bf16_stale = true;
     if (best_out) *best_out = best;
+    return 0;
+}
+
+int tx_get_autotune_trials(tx_ctx* c, tx_autotune_trial* out, int32_t max, int32_t* n) {
+    g_err.clear();
+    if (!c || !n || (max > 0 && !out)) return fail(TX_EINVAL, "tx_get_autotune_trials: NULL argument");
+    *n = static_cast<int32_t>(c->trials.size());
+    for (int32_t k = 0; k < max && k < *n; ++k) out[k] = c->trials[static_cast<size_t>(k)];
     return 0;
 }
 

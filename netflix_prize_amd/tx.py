@@ -187,6 +187,12 @@ class CBucketInfo(C.Structure):
                 ("one_launch", C.c_int32), ("ms", C.c_double)]
 
 
+class CAutotuneTrial(C.Structure):
+    _fields_ = [("bucket", C.c_int32), ("n_strata", C.c_int32), ("blocks", C.c_int32), ("user_order", C.c_int32),
+                ("heavy_min", C.c_int32), ("lanes_per_user", C.c_int32), ("waves_per_user", C.c_int32),
+                ("teams", C.c_int32), ("chosen", C.c_int32), ("ms", C.c_double)]
+
+
 def config_struct(cfg: TxConfig) -> CConfig:
     c = CConfig()
     c.seed = int(cfg.seed)
@@ -239,6 +245,7 @@ def lib() -> C.CDLL:
         L.tx_predict_subscores.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp]
         L.tx_n_subscores.argtypes = [vp, C.POINTER(i32)]
         L.tx_get_buckets.argtypes = [vp, vp]
+        L.tx_get_autotune_trials.argtypes = [vp, vp, i32, vp]
         L.tx_set_bucket_shape.argtypes = [vp, i32, i32, i32, i32]
         L.tx_calc_rmse.argtypes = [vp, vp, C.POINTER(C.c_double)]
         L.tx_param_len.argtypes = [vp, i32, C.POINTER(i64)]
@@ -248,7 +255,7 @@ def lib() -> C.CDLL:
         L.tx_get_users.argtypes = [i64, i32, C.c_uint64, i64, vp]
         L.tx_rand_normal.argtypes = [C.c_uint64, C.c_float, i64, vp]
         for name in ["tx_create", "tx_destroy", "tx_set_schedule", "tx_fit_epoch", "tx_fit_users_epoch",
-                     "tx_rebuild_ycache", "tx_set_table_precision", "tx_autotune",
+                     "tx_rebuild_ycache", "tx_set_table_precision", "tx_autotune", "tx_get_autotune_trials",
                      "tx_last_epoch_stats",
                      "tx_predict", "tx_predict_device", "tx_predict_subscores", "tx_n_subscores", "tx_calc_rmse",
                      "tx_get_buckets", "tx_set_bucket_shape",
@@ -452,6 +459,17 @@ class TxModel:
         s = CSchedule()
         _check(lib().tx_autotune(self._h, int(trial_epochs), C.byref(s)), "tx_autotune")
         return {k: getattr(s, k) for k, _ in CSchedule._fields_}
+
+    def autotune_trials(self) -> list:
+        """The last autotune's trials (tx_get_autotune_trials): per trial the
+        bucket (-1: whole schedule), stratum grid, grid size, user order,
+        heavy_min, lanes and waves per user, teams, whether it was chosen,
+        and its epoch time in ms."""
+        n = C.c_int32(0)
+        _check(lib().tx_get_autotune_trials(self._h, None, 0, C.byref(n)), "tx_get_autotune_trials")
+        arr = (CAutotuneTrial * max(n.value, 1))()
+        _check(lib().tx_get_autotune_trials(self._h, arr, n.value, C.byref(n)), "tx_get_autotune_trials")
+        return [{k: getattr(t, k) for k, _ in CAutotuneTrial._fields_} for t in arr[:n.value]]
 
     def buckets(self) -> list:
         """The length buckets of user_order 3 (tx_get_buckets): bounds, users,

@@ -92,6 +92,17 @@ def test_bucketed_schedule_holdout_rmse_and_autotune(gpu):
             np.testing.assert_array_equal(m.get(n), v, err_msg=n)
         b = m.buckets()
         assert sum(x["teams"] for x in b) <= 3072
+        # the trials: the starting shapes, then each populated bucket's
+        # candidates, at most one of them kept (none when the start won)
+        trials = m.autotune_trials()
+        assert trials[0]["bucket"] == -1
+        for k, x in enumerate(b):
+            tk = [t for t in trials if t["bucket"] == k]
+            assert bool(tk) == (x["users"] > 0) and sum(t["chosen"] for t in tk) <= 1
+            for t in tk:
+                if t["chosen"]:
+                    assert (t["lanes_per_user"], t["waves_per_user"], t["teams"]) == \
+                        (x["lanes_per_user"], x["waves_per_user"], x["teams"])
         for e in range(1, cfg.n_epochs + 1):
             m.fit_epoch(epoch=e)
             mg.fit_epoch(epoch=e)
