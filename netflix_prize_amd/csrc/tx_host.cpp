@@ -2068,10 +2068,10 @@ int tx_autotune(tx_ctx* c, int32_t trial_epochs, tx_schedule* best_out) {
         tx_autotune_trial t{};
         t.bucket = -1;
         t.n_strata = s.n_strata;
-        t.blocks = c->sched.blocks;
+        t.blocks = c->sched.blocks > 0 ? c->sched.blocks : default_blocks(c);
         t.user_order = s.user_order;
         t.heavy_min = s.heavy_min;
-        t.teams = c->sched.blocks * teams_per_block(c);
+        t.teams = t.blocks * teams_per_block(c);
         t.ms = med;
         c->trials.push_back(t);
         if (med < best_ms) {
