@@ -41,7 +41,7 @@ enum enc_epi { ENC_EPI_STORE = 0, ENC_EPI_BIAS = 1, ENC_EPI_BIAS_GELU = 2, ENC_E
  * multiple of 64, 16-byte aligned operands and leading dimensions a multiple
  * of 8 run on the 16 x 16 x 32 MFMA kernel with the tile enc_gemm_tile
  * reports; the others on the generic 64 x 64 kernel.  An ENC_EPI_STORE
- * product on the MFMA kernel with fewer than 256 tiles splits K over up to 8
+ * product on the MFMA kernel with fewer than 512 tiles splits K over up to 8
  * workgroups per tile: partial products go to a 16 MiB workspace of the
  * library (one per device) and are summed in split order, so the result does
  * not depend on scheduling, but two such calls on one device must not run at
@@ -51,9 +51,9 @@ int enc_gemm(int dtype, int ta, int tb, int epi, const void* A, const void* B, i
 
 /* The C tile (*bm x *bn rows x columns) enc_gemm's MFMA kernel takes for an
  * M x N x K product of dtype with epilogue epi (aligned operands): the largest
- * of 128 x 128, 128 x 64 and 64 x 128 that divides M and N in at least 256
+ * of 128 x 128, 128 x 64 and 64 x 128 that divides M and N in at least 512
  * tiles, else 64 x 64; 0 x 0 when the generic kernel runs (FP32, or K not a
- * multiple of 64).  *splits: the K splits (1, or for ENC_EPI_STORE under 256
+ * multiple of 64).  *splits: the K splits (1, or for ENC_EPI_STORE under 512
  * tiles the largest power of two <= 8 keeping >= 4 stages of 64 per split, at
  * most 1,024 workgroups and the slabs within the workspace).  No device work. */
 int enc_gemm_tile(int dtype, int epi, int M, int N, int K, int* bm, int* bn, int* splits);

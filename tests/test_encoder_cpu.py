@@ -126,21 +126,22 @@ def test_user_features_sample_positions():
 
 def test_mfma_gemm_tile_rule():
     """enc_gemm_tile (host-only, no device work): the largest tile that divides
-    M and N in at least 256 tiles, 64 x 64 otherwise, the generic kernel for
+    M and N in at least 512 tiles, 64 x 64 otherwise, the generic kernel for
     FP32 and for K not a multiple of 64; the fold-in network's five products."""
     from netflix_prize_amd import encoder as E
-    assert E.gemm_tile(4096, 1024, 128) == (128, 128)
-    assert E.gemm_tile(4096, 512, 1856) == (128, 64)      # forward, layer 1
+    assert E.gemm_tile(8192, 1024, 128) == (128, 128)
+    assert E.gemm_tile(4096, 1024, 128) == (128, 64)
+    assert E.gemm_tile(2112, 2048, 64) == (64, 128)
+    assert E.gemm_tile(4096, 512, 1856) == (64, 64)       # forward, layer 1: 512 tiles
     assert E.gemm_tile(4096, 320, 512) == (64, 64)        # forward, layer 2
     assert E.gemm_tile(512, 320, 4096) == (64, 64)        # dW2
-    assert E.gemm_tile(4096, 512, 320) == (128, 64)       # dZ
+    assert E.gemm_tile(4096, 512, 320) == (64, 64)        # dZ
     assert E.gemm_tile(1856, 512, 4096) == (64, 64)       # dW1
-    assert E.gemm_tile(2112, 1024, 64) == (64, 128)
     assert E.gemm_tile(4096, 1024, 96) == (0, 0)
     assert E.gemm_tile(4096, 1024, 128, E.ENC_F32) == (0, 0)
-    # K splits: plain-store products under 256 tiles only
+    # K splits: plain-store products under 512 tiles only
     assert E.gemm_plan(1856, 512, 4096) == (64, 64, 4)       # dW1: 232 tiles
     assert E.gemm_plan(512, 320, 4096) == (64, 64, 8)        # dW2: 40 tiles
     assert E.gemm_plan(512, 320, 4096, E.EPI_BIAS) == (64, 64, 1)
-    assert E.gemm_plan(4096, 512, 320) == (128, 64, 1)       # 256 tiles
+    assert E.gemm_plan(4096, 512, 320) == (64, 64, 1)        # 512 tiles
     assert E.gemm_plan(512, 320, 512) == (64, 64, 2)         # 4 stages per split at most

@@ -52,19 +52,21 @@ def test_gemm_layouts(gpu, dtype, ta, tb):
     assert _rel(C.cpu().numpy(), ref) < (1e-6 if dtype == E.ENC_F32 else 1e-5)
 
 
-# (M, N, K) per MFMA tile: each shape takes the tile named (enc_gemm_tile's rule)
-MFMA_SHAPES = {(128, 128): (4096, 1024, 128), (128, 64): (4096, 512, 1856), (64, 128): (2112, 1024, 64),
-               (64, 64): (512, 320, 4096)}
+# (M, N, K) -> (tile rows, tile columns, K splits) of a plain-store product:
+# every tile, and K split over 8 and 4 workgroups (the fold-in network's dW2
+# and dW1 shapes)
+MFMA_SHAPES = {(8192, 1024, 128): (128, 128, 1), (4096, 1024, 192): (128, 64, 1), (2112, 2048, 64): (64, 128, 1),
+               (4096, 512, 1856): (64, 64, 1), (512, 320, 4096): (64, 64, 8), (1856, 512, 4096): (64, 64, 4)}
 
 
-@pytest.mark.parametrize("tile", list(MFMA_SHAPES))
+@pytest.mark.parametrize("shape", list(MFMA_SHAPES))
 @pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
-def test_mfma_gemm_tiles_and_layouts(gpu, tile, ta, tb):
-    """The 16x16x32 MFMA kernel in each tile and operand layout against an FP64
-    product of the same BF16 inputs (the fold-in network's shapes among them)."""
+def test_mfma_gemm_tiles_and_layouts(gpu, shape, ta, tb):
+    """The 16x16x32 MFMA kernel in each tile, K split and operand layout against
+    an FP64 product of the same BF16 inputs."""
     import torch
-    M, N, K = MFMA_SHAPES[tile]
-    assert E.gemm_tile(M, N, K) == tile
+    M, N, K = shape
+    assert E.gemm_plan(M, N, K) == MFMA_SHAPES[shape]
     rng = np.random.default_rng(3)
     A = bf16_round(rng.normal(size=(K, M) if ta else (M, K)).astype(np.float32))
     B = bf16_round(rng.normal(size=(N, K) if tb else (K, N)).astype(np.float32))
@@ -79,7 +81,7 @@ def test_mfma_gemm_tiles_and_layouts(gpu, tile, ta, tb):
 def test_mfma_gemm_epilogues(gpu):
     import torch
     rng = np.random.default_rng(4)
-    M, N, K = 4096, 512, 128
+    M, N, K = 4096, 1024, 128
     assert E.gemm_tile(M, N, K) == (128, 64)
     A = bf16_round(rng.normal(size=(M, K)).astype(np.float32))
     B = bf16_round(rng.normal(size=(K, N)).astype(np.float32) * 0.1)
