@@ -283,7 +283,8 @@ def main(argv=None) -> int:
     t_create = time.time() - t0
     model.set_schedule(mode=tx.SCHED_STRATA, n_strata=args.strata, use_graph=bool(args.graph), blocks=args.blocks,
                        experimental=args.strata > 1)
-    log(f"bench: model created ({t_create:.1f}s)")
+    team_split = bool(model.schedule()["split_waves"])
+    log(f"bench: model created ({t_create:.1f}s); schedule {model.schedule()}")
 
     have_torch_gpu = torch.cuda.is_available()  # synchronisation of the device around the timed region
 
@@ -368,7 +369,7 @@ def main(argv=None) -> int:
                          "curve": [round(r, 5) for r in curve]}
             if tuned is not None:
                 ttr[form]["autotuned_schedule"] = {k: tuned[k] for k in ("blocks", "user_order", "heavy_min",
-                                                                          "n_strata", "trials")}
+                                                                          "n_strata", "split_waves", "trials")}
             log(f"bench: time-to-RMSE {form}: {ttr[form]}")
 
     # length-bucketed launches (user_order 3): the users of 1-16, 17-256,
@@ -658,6 +659,7 @@ def main(argv=None) -> int:
         "config": {"workload": f"plain timeSVD++ {spec.n_users}x{spec.n_items}, {spec.n_ratings} ratings, "
                                f"f={cfg.n_feat}, FP32, 30 time bins, low-memory, "
                                f"{args.strata} strat{'um' if args.strata == 1 else 'a'}, "
+                               f"{'split teams (rating wave + streaming wave)' if team_split else 'one-wave teams'}, "
                                f"{'hipGraph epoch' if args.graph else 'stream launches'}",
                    "n_train": n_train, "n_heldout": pr.n_ratings, "holdout_rmse": round(rmse, 5),
                    "epochs_trained": epoch, "tx_create_s": round(t_create, 2)},

@@ -125,6 +125,13 @@ typedef struct tx_schedule {
                                 half of the epoch (so no long user starts near its end),
                                 3 = length buckets launched separately (below) */
     int32_t heavy_min;       /* user_order 2: bound of the long-user bucket, 0 = 1024 */
+    int32_t split_waves;     /* STRATA, one stratum, user_order 0-2, plain NSVD1 rows, low-memory
+                                model without ordinal head or neighbourhood terms, 64-lane teams:
+                                1 = two waves per team (128-thread workgroups, `blocks` = teams):
+                                a rating wave and a streaming wave that takes the team's users
+                                from the queue, gathers each one's NSVD1 sum before its rating
+                                loop and applies its y step during the next user's loop; the
+                                two meet once per user.  0 = one wave per team. */
 } tx_schedule;
 /* user_order 3 (one-stratum TX_SCHED_STRATA, f <= 1024): the users in length
  * buckets by training-rating count, [1, 16], [17, 256], [257, 4096] and
@@ -221,10 +228,14 @@ int tx_destroy(tx_ctx* ctx);
  * TX_EUNSUPPORTED. */
 int tx_set_schedule(tx_ctx* ctx, const tx_schedule* sched);
 
+/* The model's current schedule (after tx_create: its default schedule). */
+int tx_get_schedule(tx_ctx* ctx, tx_schedule* out);
+
 /* Autotuner of the parallel schedule (no reference counterpart): times
  * trial_epochs epochs for each candidate of teams in flight (1/2, 3/4 and all
  * of the measured fidelity bound for this f) x user order (the reference's,
- * or length buckets with heavy_min 256 / 1024 / 4096), then the fastest of
+ * or length buckets with heavy_min 256 / 1024 / 4096) x team form (one wave
+ * per team, and split teams where the model takes them), then the fastest of
  * those over the stratum grids 8 x 8 and 64 x 64 (low-memory models without
  * ordinal head or neighbourhood terms), each trial from the same snapshot of
  * every parameter table, then restores the tables and applies the fastest
@@ -244,6 +255,7 @@ typedef struct tx_autotune_trial {
     int32_t lanes_per_user, waves_per_user;  /* bucket trials: the bucket's team shape */
     int32_t teams;           /* teams in flight (of the bucket, for bucket trials) */
     int32_t chosen;          /* 1: the candidate tx_autotune applied */
+    int32_t split_waves;     /* the trial's team form (tx_schedule.split_waves) */
     double ms;               /* epoch time (median of the trial's epochs) */
 } tx_autotune_trial;
 

@@ -84,6 +84,7 @@ def build(verbose: bool = False, incremental: bool = False) -> str:
             dev("tx_sgd_day.hip", "tx_sgd_day.o"), dev("tx_sgd_user.hip", "tx_sgd_user.o"),
             dev("tx_sgd_mw.hip", "tx_sgd_mw.o"), dev("tx_sgd_nb1.hip", "tx_sgd_nb1.o"),
             dev("tx_sgd_nb2.hip", "tx_sgd_nb2.o"), dev("tx_sgd_bucket.hip", "tx_sgd_bucket.o"),
+            dev("tx_sgd_split.hip", "tx_sgd_split.o"),
             dev("knn_kernels.hip", "knn_kernels.o"),
             dev("knn3_kernels.hip", "knn3_kernels.o")]
     todo = [j for j in jobs if _stale(j)] if incremental else jobs

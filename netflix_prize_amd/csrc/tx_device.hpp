@@ -421,9 +421,21 @@ struct Pending {
 //         Every wave computes the same scalars; the item-bias atomics and
 //         the shared NSVD1-bias writes are issued by wave 0 alone, user-owned
 //         slots by every wave (each wave's own later loads then see them).
-template <int TEAM, int VEC, bool QF, bool P2, bool ORD, int PASS, bool DAY, int MW, int NB = 0>
+// SPLIT:  a team of two waves per user stream (128-thread workgroups, one
+//         stratum, plain y rows, low-memory models without ordinal head or
+//         neighbourhood terms).  Wave 0 runs the rating loops; wave 1 takes
+//         the users from the queue, gathers each user's NSVD1 sum before its
+//         rating loop and applies its y step during the next user's loop.
+//         The sums and gradients pass through LDS slots (by user parity) and
+//         the two waves meet at one workgroup barrier per user, so the rating
+//         wave never waits for the row streams.  Against one wave per user,
+//         a user's y step lands one user later and its NSVD1 sum is taken one
+//         user earlier; the arithmetic is the same.
+template <int TEAM, int VEC, bool QF, bool P2, bool ORD, int PASS, bool DAY, int MW, int NB = 0, bool SPLIT = false>
 __global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_sgd_par(DevModel m, StrataView sv, int stratum) {
     static_assert(NB == 0 || (PASS == 0 && TEAM == 64), "neighbourhood terms: SGD epoch, one-wave lanes");
+    static_assert(!SPLIT || (TEAM == 64 && PASS == 0 && !DAY && !MW && NB == 0 && !ORD),
+                  "split teams: the one-wave SGD epoch of low-memory models");
     static_assert(PASS == 0 || !ORD, "the user passes have no ordinal head");
     static_assert(PASS == 0 || !DAY, "the user passes train low-memory models");
     static_assert(!MW || TEAM == 64, "multi-wave teams: 64-lane waves");
@@ -438,6 +450,13 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_
     __shared__ float xbuf[(MW == 8 ? 512 : 256) * VEC];  // per-wave transpose slot, TEAM*VEC floats each
     __shared__ float dred[2][MW ? 9 : 1];            // per-wave dot products + wave 0's b_i(t) (MW), double-buffered
     __shared__ uint32_t qgrab[1];                    // the team's queue grab (MW)
+    // SPLIT: per user parity the NSVD1 sum (contiguous row order) and its bias
+    // part, the gradient over the norm (row order) and its bias part, the user
+    constexpr uint32_t kNoUser = 0xffffffffu;
+    __shared__ float sp_s[SPLIT ? 2 : 1][SPLIT ? TEAM * VEC : 1];
+    __shared__ float sp_g[SPLIT ? 2 : 1][SPLIT ? TEAM * VEC : 1];
+    __shared__ float sp_sb[2], sp_gb[2];
+    __shared__ uint32_t sp_user[2];
     const int lane = threadIdx.x % TEAM;
     const int wv = MW ? static_cast<int>(threadIdx.x / 64) : 0;  // wave of the team
     const int nwv = MW ? static_cast<int>(blockDim.x / 64) : 1;
@@ -496,9 +515,91 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_
         for (int k = 0; k < 4; ++k) th[k] = m.thresholds[k];
     }
 
+    if constexpr (SPLIT) {
+        if (threadIdx.x >= TEAM) {
+            // the streaming wave: queue, NSVD1 gathers, y steps
+            const int coff = lane * VEC;
+            auto grab = [&]() -> uint32_t {
+                uint32_t w = 0;
+                if (lane == 0) w = __hip_atomic_fetch_add(&sv.q_head[qid], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                w = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(w)));
+                return w < qlen ? sv.q_users[qbeg + w] : kNoUser;
+            };
+            auto gather = [&](uint32_t u, int slot) {  // src/tx.rs:801-822
+                float acc[VEC];
+                float sb = 0.0f;
+#pragma unroll
+                for (int c = 0; c < VEC; ++c) acc[c] = 0.0f;
+                const uint32_t ts = m.t_off[u], te = m.t_off[u + 1];
+                const uint32_t nh = m.full_su ? m.h_start[u + 1] - m.h_start[u] : 0;
+                const uint32_t ntr = te - ts, ntot = ntr + nh;
+                const uint32_t* titems = m.t_item + ts;
+                const uint32_t* hitems = m.h_item + m.h_start[u] - ntr;  // row r >= ntr is hitems[r]
+                const bool yb_on = !m.skip_yb;
+                for_rows<TEAM, VEC, GBS>(
+                    ntot, lane, m.yfeat + cbase, fs, [&](uint32_t r) { return r < ntr ? titems + r : hitems + r; },
+                    [&](uint32_t b0, uint32_t ib) {
+                        if (yb_on && b0 + static_cast<uint32_t>(lane) < ntot) sb += m.ybias[ib];
+                    },
+                    [&](uint32_t, uint32_t, const float (&y)[VEC]) {
+#pragma unroll
+                        for (int c = 0; c < VEC; ++c) acc[c] += y[c];
+                    });
+                const float inv = 1.0f / m.norm_train[u];
+#pragma unroll
+                for (int c = 0; c < VEC; ++c) sp_s[slot][coff + c] = acc[c] * inv;
+                const float sbv = m.skip_yb ? 0.0f : team_sum_dpp<TEAM>(sb) * inv;
+                if (lane == 0) sp_sb[slot] = sbv;
+            };
+            auto ystep = [&](uint32_t u, int slot) {  // src/tx.rs:1099-1106, plain rows
+                const uint32_t ts = m.t_off[u], te = m.t_off[u + 1];
+                float gc[VEC];
+#pragma unroll
+                for (int c = 0; c < VEC; ++c) gc[c] = sp_g[slot][coff + c];
+                const float gb = h.lr_yb * sp_gb[slot];
+                const uint32_t* titems = m.t_item + ts;
+                for_rows<TEAM, VEC, GBS>(
+                    te - ts, lane, m.yfeat + cbase, fs, [&](uint32_t r) { return titems + r; },
+                    [&](uint32_t b0, uint32_t ib) {
+                        if (gb != 0.0f && b0 + static_cast<uint32_t>(lane) < te - ts) m.ybias[ib] = m.ybias[ib] - gb;
+                    },
+                    [&](uint32_t, uint32_t j, const float (&y)[VEC]) {
+                        float ny[VEC];
+#pragma unroll
+                        for (int c = 0; c < VEC; ++c) ny[c] = y[c] - h.lr_y * (gc[c] + h.reg_y * y[c]);
+                        store_vec<VEC>(m.yfeat + static_cast<size_t>(j) * fs + cbase + coff, ny);
+                    });
+            };
+            // user k's sum is in slot k & 1 before barrier k - 1; its gradient
+            // is in slot k & 1 after barrier k; one barrier per user, and one
+            // more after the last (the runner's count is the same)
+            const uint32_t u0 = grab();
+            if (lane == 0) sp_user[0] = u0;
+            if (u0 != kNoUser) gather(u0, 0);
+            wg_barrier();
+            uint32_t uprev = kNoUser;
+            for (uint32_t k = 0;; ++k) {
+                const uint32_t uk =
+                    static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(sp_user[k & 1])));
+                if (uprev != kNoUser) ystep(uprev, static_cast<int>((k - 1) & 1));
+                const uint32_t nx = uk != kNoUser ? grab() : kNoUser;
+                if (lane == 0) sp_user[(k + 1) & 1] = nx;
+                if (nx != kNoUser) gather(nx, static_cast<int>((k + 1) & 1));
+                uprev = uk;
+                wg_barrier();
+                if (uk == kNoUser) break;
+            }
+            return;
+        }
+        wg_barrier();  // the first user's sum is ready
+    }
+    uint32_t split_k = 0;  // SPLIT: index of the runner's current user
     for (;;) {
         uint32_t w0 = 0;
-        if constexpr (MW) {
+        uint32_t split_u = 0;
+        if constexpr (SPLIT) {
+            split_u = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(sp_user[split_k & 1])));
+        } else if constexpr (MW) {
             // one grab for the team, passed to its waves through LDS
             wg_barrier();
             if (lead && lane == 0)
@@ -515,10 +616,16 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_
         // from it) as wave-uniform: scalar loop control, no exec-mask loops
         if constexpr (TEAM == 64) w0 = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(w0)));
         else w0 = __shfl(w0, 0, TEAM);
-        if (w0 >= qlen) break;
-        const uint32_t w1 = w0 + CHUNK < qlen ? w0 + CHUNK : qlen;
+        uint32_t w1;
+        if constexpr (SPLIT) {
+            w0 = 0;
+            w1 = split_u != kNoUser ? 1u : 0u;
+        } else {
+            if (w0 >= qlen) break;
+            w1 = w0 + CHUNK < qlen ? w0 + CHUNK : qlen;
+        }
         for (uint32_t w = w0; w < w1; ++w) {
-            const uint32_t u = sv.q_users[qbeg + w];
+            const uint32_t u = SPLIT ? split_u : sv.q_users[qbeg + w];
             const uint32_t s0 = sv.seg_off[static_cast<size_t>(u) * (S + 1) + stratum];
             const uint32_t s1 = sv.seg_off[static_cast<size_t>(u) * (S + 1) + stratum + 1];
             const size_t prow = static_cast<size_t>(u) * fs + off;
@@ -639,7 +746,15 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_
                 for (int c = 0; c < VEC; ++c) p2[c] = 0.0f;
             }
             float su_b;
-            if (PASS != 0 && first && sv.su_cached) {
+            if constexpr (SPLIT) {
+                // the NSVD1 sum the streaming wave gathered (row order -> strided)
+#pragma unroll
+                for (int c = 0; c < VEC; ++c) {
+                    s[c] = sp_s[split_k & 1][c * TEAM + lane];
+                    G[c] = 0.0f;
+                }
+                su_b = sp_sb[split_k & 1];
+            } else if (PASS != 0 && first && sv.su_cached) {
                 // y is frozen for the whole user pass and ycache was rebuilt from
                 // it over the same rows: the NSVD1 sum without a second gather
                 load_str<VEC, TEAM>(m.ycache + prow, s);
@@ -1024,6 +1139,12 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_
                 if (lane == 0) sv.gacc_b[u] = Gb / norm;
             } else if (PASS == 2) {
                 // fold-in: nothing beyond the user's own parameters
+            } else if (SPLIT) {
+                // the gradient over the norm, for the streaming wave's y step
+                const float inv = 1.0f / norm;
+#pragma unroll
+                for (int c = 0; c < VEC; ++c) sp_g[split_k & 1][c * TEAM + lane] = G[c] * inv;
+                if (lane == 0) sp_gb[split_k & 1] = Gb * inv;
             } else if (last && S > 1) {
                 // stratified epoch: the user's whole-epoch NSVD1 gradient over its
                 // norm, for the item-side gather after the last stratum (the
@@ -1143,6 +1264,11 @@ __global__ __launch_bounds__(MW == 8 ? 512 : 256, MW ? 1 : TX_SGD_WAVES) void k_
                         }
                     });
             }
+        }
+        if constexpr (SPLIT) {
+            wg_barrier();  // the gradient is in its slot; the next sum is in the other
+            if (split_u == kNoUser) break;
+            ++split_k;
         }
     }
 }

@@ -42,6 +42,11 @@ hipError_t launch_sgd_par_nb2_mw(const DevModel& m, const StrataView& sv, int te
 // per 128-thread block (fs 128..1024)
 hipError_t launch_sgd_par_t16(const DevModel& m, const StrataView& sv, int blocks, bool qf, bool p2, hipStream_t st);
 hipError_t launch_sgd_par_w2(const DevModel& m, const StrataView& sv, int teams, bool qf, bool p2, hipStream_t st);
+// split teams (SPLIT in k_sgd_par): `teams` workgroups of 128 threads, a
+// rating wave and a streaming wave per user stream (one stratum, plain y rows,
+// low-memory models without ordinal head or neighbourhood terms, 64-lane teams)
+hipError_t launch_sgd_par_split(const DevModel& m, const StrataView& sv, int vec, int teams, bool qf, bool p2,
+                                hipStream_t st);
 hipError_t launch_to_bf16(const float* x, uint16_t* y, int64_t n, hipStream_t st);
 // the NSVD1 step of the tsvdx4p user pass from the stored per-user gradients
 // (gacc [U * fs], gacc_b [U]) gathered per item; vec = the shape's VEC

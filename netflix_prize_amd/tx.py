@@ -168,7 +168,7 @@ class CRatings(C.Structure):
 class CSchedule(C.Structure):
     _fields_ = [("mode", C.c_int32), ("n_strata", C.c_int32), ("teams_per_block", C.c_int32),
                 ("blocks", C.c_int32), ("use_graph", C.c_int32), ("update_flags", C.c_int32),
-                ("user_order", C.c_int32), ("heavy_min", C.c_int32)]
+                ("user_order", C.c_int32), ("heavy_min", C.c_int32), ("split_waves", C.c_int32)]
 
 
 class CEpochStats(C.Structure):
@@ -190,7 +190,7 @@ class CBucketInfo(C.Structure):
 class CAutotuneTrial(C.Structure):
     _fields_ = [("bucket", C.c_int32), ("n_strata", C.c_int32), ("blocks", C.c_int32), ("user_order", C.c_int32),
                 ("heavy_min", C.c_int32), ("lanes_per_user", C.c_int32), ("waves_per_user", C.c_int32),
-                ("teams", C.c_int32), ("chosen", C.c_int32), ("ms", C.c_double)]
+                ("teams", C.c_int32), ("chosen", C.c_int32), ("split_waves", C.c_int32), ("ms", C.c_double)]
 
 
 def config_struct(cfg: TxConfig) -> CConfig:
@@ -246,6 +246,7 @@ def lib() -> C.CDLL:
         L.tx_n_subscores.argtypes = [vp, C.POINTER(i32)]
         L.tx_get_buckets.argtypes = [vp, vp]
         L.tx_get_autotune_trials.argtypes = [vp, vp, i32, vp]
+        L.tx_get_schedule.argtypes = [vp, vp]
         L.tx_set_bucket_shape.argtypes = [vp, i32, i32, i32, i32]
         L.tx_calc_rmse.argtypes = [vp, vp, C.POINTER(C.c_double)]
         L.tx_param_len.argtypes = [vp, i32, C.POINTER(i64)]
@@ -256,6 +257,7 @@ def lib() -> C.CDLL:
         L.tx_rand_normal.argtypes = [C.c_uint64, C.c_float, i64, vp]
         for name in ["tx_create", "tx_destroy", "tx_set_schedule", "tx_fit_epoch", "tx_fit_users_epoch",
                      "tx_rebuild_ycache", "tx_set_table_precision", "tx_autotune", "tx_get_autotune_trials",
+                     "tx_get_schedule",
                      "tx_last_epoch_stats",
                      "tx_predict", "tx_predict_device", "tx_predict_subscores", "tx_n_subscores", "tx_calc_rmse",
                      "tx_get_buckets", "tx_set_bucket_shape",
@@ -390,6 +392,7 @@ class TxModel:
         h = C.c_void_p()
         _check(lib().tx_create(C.byref(self._cc), C.byref(tref.s), C.byref(pref.s), C.byref(h)), "tx_create")
         self._h = h
+        self._default_split = bool(self.schedule()["split_waves"])
         if schedule is not None:
             self.set_schedule(**schedule)
 
@@ -426,11 +429,18 @@ class TxModel:
     def set_schedule(self, mode: int = SCHED_STRATA, n_strata: int = 1, blocks: int = 0, use_graph: bool = False,
                      teams_per_block: int = 0, q_update: int = 0, y_update: int = 2,
                      user_order: int = 2, heavy_min: int = 1024, experimental: bool = False,
-                     nbr_update: int = 0) -> None:
+                     nbr_update: int = 0, split_waves: bool | None = None) -> None:
         """tx_set_schedule.  q_update / y_update / nbr_update: 0 atomic, 2 plain
         updates of the item rows, the NSVD1 rows and the neighbourhood tables;
-        experimental is accepted and ignored (n_strata > 1 no longer needs it)."""
+        experimental is accepted and ignored (n_strata > 1 no longer needs it);
+        split_waves: a rating wave and a streaming wave per team (blocks =
+        teams); None keeps the model's default where the schedule takes them
+        (one stratum, user order 0-2, plain NSVD1 rows)."""
         s = CSchedule()
+        if split_waves is None:
+            split_waves = (self._default_split and mode == SCHED_STRATA and n_strata == 1
+                           and user_order != USER_ORDER_BUCKETS and y_update == 2)
+        s.split_waves = int(bool(split_waves))
         s.user_order = int(user_order)
         s.heavy_min = int(heavy_min)
         s.update_flags = int(q_update) | (int(y_update) << 2) | (16 if experimental else 0) | (int(nbr_update) << 5)
@@ -458,6 +468,12 @@ class TxModel:
         chosen schedule is applied and returned."""
         s = CSchedule()
         _check(lib().tx_autotune(self._h, int(trial_epochs), C.byref(s)), "tx_autotune")
+        return {k: getattr(s, k) for k, _ in CSchedule._fields_}
+
+    def schedule(self) -> dict:
+        """The current schedule (tx_get_schedule)."""
+        s = CSchedule()
+        _check(lib().tx_get_schedule(self._h, C.byref(s)), "tx_get_schedule")
         return {k: getattr(s, k) for k, _ in CSchedule._fields_}
 
     def autotune_trials(self) -> list:

@@ -173,15 +173,18 @@ def test_autotune_leaves_parameters_and_picks_bounded_schedule(gpu):
     best = m.autotune(trial_epochs=1)
     for n, v in before.items():
         np.testing.assert_array_equal(m.get(n), v, err_msg=n)
-    teams = best["blocks"] * 4                   # one-wave teams, four per block
+    teams = best["blocks"] * (1 if best["split_waves"] else 4)   # split teams: one per block
     assert best["user_order"] in (0, 2) and 0 < teams <= 3072 + 32, best
     # every candidate is reported, the stratum grids among them, and the
     # applied schedule is the one trial marked chosen (the fastest)
     trials = m.autotune_trials()
-    assert len(trials) == 14 and {t["n_strata"] for t in trials} == {1, 8, 64}
+    # four orders x three team counts x two team forms, then two grids
+    assert len(trials) == 26 and {t["n_strata"] for t in trials} == {1, 8, 64}
+    assert {t["split_waves"] for t in trials if t["n_strata"] == 1} == {0, 1}
     chosen = [t for t in trials if t["chosen"]]
     assert len(chosen) == 1 and chosen[0]["ms"] == min(t["ms"] for t in trials)
-    assert (chosen[0]["n_strata"], chosen[0]["user_order"]) == (best["n_strata"], best["user_order"])
+    assert (chosen[0]["n_strata"], chosen[0]["user_order"], chosen[0]["split_waves"]) == \
+        (best["n_strata"], best["user_order"], best["split_waves"])
     m.fit_epoch(epoch=2)
     assert m.calc_rmse(pr) < 1.1
 
