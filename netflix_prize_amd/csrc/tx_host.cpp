@@ -1143,6 +1143,35 @@ int tx_create(const tx_config* cfg, const tx_ratings* tr, const tx_ratings* pr, 
     const int64_t U = c->U, I = c->I, N = c->N, NH = c->NH;
     const int f = c->f, F = c->F, B = c->B;
 
+    // initial factors in the reference's draw order (src/tx.rs:606-630):
+    // ufeat, ifeat, yfeat, [yfeat_day], ifeat_freq.  They depend on the sizes
+    // and the seed only, so they are drawn on a thread of their own while the
+    // rating structures below are built (joined before the tables go to the
+    // device; `draws` is declared after the tables, so an early return joins
+    // the thread before the tables are freed).
+    std::vector<float> h_ufeat, h_ifeat, h_yfeat, h_yday, h_iqf;
+    struct Joiner {
+        std::thread t;
+        ~Joiner() {
+            if (t.joinable()) t.join();
+        }
+    } draws;
+    draws.t = std::thread([&h_ufeat, &h_ifeat, &h_yfeat, &h_yday, &h_iqf, U, I, f, F, cc = c->cfg]() {
+        tx::ChaChaRng rng(cc.seed);
+        h_ufeat.resize(static_cast<size_t>(U) * f);
+        h_ifeat.resize(static_cast<size_t>(I) * f);
+        h_yfeat.resize(static_cast<size_t>(I) * f);
+        h_iqf.resize(static_cast<size_t>(I) * F * f);
+        tx::fill_normal(rng, h_ufeat.data(), static_cast<int64_t>(h_ufeat.size()), cc.sigma_u);
+        tx::fill_normal(rng, h_ifeat.data(), static_cast<int64_t>(h_ifeat.size()), cc.sigma_i);
+        tx::fill_normal(rng, h_yfeat.data(), static_cast<int64_t>(h_yfeat.size()), cc.sigma_y);
+        if (!cc.low_memory) {
+            h_yday.resize(static_cast<size_t>(I) * f);
+            tx::fill_normal(rng, h_yday.data(), static_cast<int64_t>(h_yday.size()), cc.sigma_yd);
+        }
+        tx::fill_normal(rng, h_iqf.data(), static_cast<int64_t>(h_iqf.size()), cc.sigma_iqf);
+    });
+
     // calc_user_offsets (src/lib.rs:469-483): user-sorted input, spans = counts
     for (int64_t t = 1; t < N; ++t)
         if (tr->user_idxs[t] < tr->user_idxs[t - 1])
@@ -1161,15 +1190,25 @@ int tx_create(const tx_config* cfg, const tx_ratings* tr, const tx_ratings* pr, 
     }
 
     clk.mark("checks and user offsets");
-    // tu_mean and day_range (src/tx.rs:465-475): f32 sums in file order
+    // tu_mean and day_range (src/tx.rs:465-475): f32 sums in file order; the
+    // rows are user-sorted, so each user's sum runs over its own span, in
+    // parallel over users
     std::vector<float> tu_mean(U, 0.0f);
+    std::vector<int32_t> dmax(U, 0);
+    parallel_for(U, [&](int64_t lo, int64_t hi) {
+        for (int64_t u = lo; u < hi; ++u) {
+            float sum = 0.0f;
+            int32_t mx = 0;
+            for (int64_t t = c->t_off[u]; t < static_cast<int64_t>(c->t_off[u + 1]); ++t) {
+                sum += static_cast<float>(tr->dates[t]);
+                mx = std::max<int32_t>(mx, static_cast<int32_t>(tr->dates[t]) + 1);
+            }
+            tu_mean[u] = tr->user_cnts[u] > 0 ? sum / static_cast<float>(tr->user_cnts[u]) : sum;
+            dmax[u] = mx;
+        }
+    });
     int32_t day_range = 0;
-    for (int64_t t = 0; t < N; ++t) {
-        tu_mean[tr->user_idxs[t]] += static_cast<float>(tr->dates[t]);
-        day_range = std::max<int32_t>(day_range, static_cast<int32_t>(tr->dates[t]) + 1);
-    }
-    for (int64_t u = 0; u < U; ++u)
-        if (tr->user_cnts[u] > 0) tu_mean[u] /= static_cast<float>(tr->user_cnts[u]);
+    for (int64_t u = 0; u < U; ++u) day_range = std::max(day_range, dmax[u]);
     if (day_range <= 0) day_range = 1;
 
     clk.mark("tu_mean, day_range");
@@ -1465,21 +1504,8 @@ int tx_create(const tx_config* cfg, const tx_ratings* tr, const tx_ratings* pr, 
         }
     }
     clk.mark("neighbourhood structures");
-    // initial factors in the reference's draw order (src/tx.rs:606-630):
-    // ufeat, ifeat, yfeat, [yfeat_day], ifeat_freq
-    tx::ChaChaRng rng(cfg->seed);
-    std::vector<float> h_ufeat(static_cast<size_t>(U) * f), h_ifeat(static_cast<size_t>(I) * f),
-        h_yfeat(static_cast<size_t>(I) * f), h_yday, h_iqf(static_cast<size_t>(I) * F * f);
-    tx::fill_normal(rng, h_ufeat.data(), static_cast<int64_t>(h_ufeat.size()), cfg->sigma_u);
-    tx::fill_normal(rng, h_ifeat.data(), static_cast<int64_t>(h_ifeat.size()), cfg->sigma_i);
-    tx::fill_normal(rng, h_yfeat.data(), static_cast<int64_t>(h_yfeat.size()), cfg->sigma_y);
-    if (!low_mem) {
-        h_yday.resize(static_cast<size_t>(I) * f);
-        tx::fill_normal(rng, h_yday.data(), static_cast<int64_t>(h_yday.size()), cfg->sigma_yd);
-    }
-    tx::fill_normal(rng, h_iqf.data(), static_cast<int64_t>(h_iqf.size()), cfg->sigma_iqf);
-
-    clk.mark("initial draws");
+    draws.t.join();
+    clk.mark("initial draws (waited for)");
     // ---- device ----
     HIPC(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
     for (auto& e : c->ev) HIPC(hipEventCreate(&e));
