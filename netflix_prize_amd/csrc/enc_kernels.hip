@@ -349,6 +349,26 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(const bf16_t* __restrict__ A,
         lds_barrier();
     }
 
+    // the epilogue's operands (bias, Z) are loaded before its first store: the
+    // compiler cannot rule out that a store aliases them, and would otherwise
+    // issue each element's load only after the previous element's store
+    float bv[TN], zv[TM][TN][4];
+    if (!part) {
+#pragma unroll
+        for (int tj = 0; tj < TN; ++tj) {
+            const int c = n0 + wn + tj * 16 + (lane & 15);
+            if constexpr (EPI == ENC_EPI_BIAS || EPI == ENC_EPI_BIAS_GELU) bv[tj] = ep.bias[c];
+#pragma unroll
+            for (int ti = 0; ti < TM; ++ti)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int r = m0 + wm + ti * 16 + 4 * (lane >> 4) + e;
+                    if constexpr (EPI != ENC_EPI_STORE && EPI != ENC_EPI_BIAS && EPI != ENC_EPI_BIAS_GELU)
+                        zv[ti][tj][e] = ep.zin[static_cast<size_t>(r) * ep.ldc + c];
+                }
+        }
+    }
+
 #pragma unroll
     for (int ti = 0; ti < TM; ++ti)
 #pragma unroll
@@ -366,13 +386,13 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(const bf16_t* __restrict__ A,
                 if constexpr (EPI == ENC_EPI_STORE) {
                     ep.cf[o] = v;
                 } else if constexpr (EPI == ENC_EPI_BIAS) {
-                    ep.cf[o] = v + ep.bias[c];
+                    ep.cf[o] = v + bv[tj];
                 } else if constexpr (EPI == ENC_EPI_BIAS_GELU) {
-                    const float z = v + ep.bias[c];
+                    const float z = v + bv[tj];
                     ep.cf[o] = z;
                     ep.ct[o] = bf16_rne(gelu(z));
                 } else {
-                    const float g = v * gelu_grad(ep.zin[o]);
+                    const float g = v * gelu_grad(zv[ti][tj][e]);
                     ep.cf[o] = g;
                     ep.ct[o] = bf16_rne(g);
                 }
